@@ -1,0 +1,272 @@
+"""cubismup_2d_amd -- MI355X (gfx950) implementation of CubismUP_2D's per-step hot path.
+
+The product is ``libcup2d_hip.so`` (HIP kernels behind the C-ABI declared in
+``include/cup2d.h``).  This module is a thin ctypes binding of that C-ABI; it
+mirrors the reference's operator sequence (Simulation::reset /
+calcMaxTimestep / advance and the operators of the PressureSingle pipeline)
+under the same names.  There is no CPU fallback: if the shared library is
+missing or no HIP device is present the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcup2d_hip.so")
+BS = 32
+
+VEL, CHI, PRES, TMP, TMPV, UDEF, INVRHO = range(7)
+NCOMP = [2, 1, 1, 1, 2, 2, 1]
+LAYOUT_BLOCKS, LAYOUT_DENSE = 0, 1
+POISSON = {"periodic": 0, "freespace": 1}
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_BODY_OUT, ERR_CAPACITY, ERR_NONFINITE = -1, -2, -3, -4, -5, -6
+
+
+class Cup2dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cup2d error {code}: {msg}")
+        self.code = code
+
+
+class Cfg(C.Structure):
+    _fields_ = [("bpdx", C.c_int), ("bpdy", C.c_int), ("extent", C.c_double),
+                ("nu", C.c_double), ("CFL", C.c_double), ("lam", C.c_double),
+                ("fadeLen", C.c_double), ("poisson", C.c_int), ("implicit_penal", C.c_int),
+                ("stream", C.c_void_p)]
+
+
+class ShapeDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("radius", C.c_double), ("xpos", C.c_double),
+                ("ypos", C.c_double), ("angle_deg", C.c_double), ("rhoS", C.c_double),
+                ("bFixedx", C.c_int), ("bFixedy", C.c_int), ("bForcedx", C.c_int),
+                ("bForcedy", C.c_int), ("bBlockAng", C.c_int), ("xvel", C.c_double),
+                ("yvel", C.c_double), ("angvel", C.c_double), ("timeForced", C.c_double),
+                ("tAccel", C.c_double)]
+
+
+SHAPE_FIELDS = ["center", "centerOfMass", "labCenterOfMass", "d_gm", "orientation",
+                "M", "J", "u", "v", "omega",
+                "fluidAngMom", "fluidMomX", "fluidMomY", "penalDX", "penalDY", "penalM", "penalJ",
+                "perimeter", "forcex", "forcey", "forcex_P", "forcey_P", "forcex_V", "forcey_V",
+                "torque", "torque_P", "torque_V", "drag", "thrust", "circulation",
+                "Pout", "PoutBnd", "defPower", "defPowerBnd", "Pthrust", "Pdrag", "EffPDef",
+                "EffPDefBnd", "n_obstacle_blocks", "n_surface_points"]
+_VEC2 = ("center", "centerOfMass", "labCenterOfMass", "d_gm")
+
+
+class ShapeState(C.Structure):
+    _fields_ = [(n, C.c_double * 2) if n in _VEC2 else (n, C.c_double) for n in SHAPE_FIELDS]
+
+    def as_dict(self):
+        return {n: (tuple(getattr(self, n)) if n in _VEC2 else getattr(self, n))
+                for n in SHAPE_FIELDS}
+
+
+EXPORTS = [
+    "cup2d_create", "cup2d_destroy", "cup2d_last_error", "cup2d_add_shape",
+    "cup2d_upload_field", "cup2d_download_field", "cup2d_field_device_ptr",
+    "cup2d_get_scalars", "cup2d_set_scalars", "cup2d_get_shape", "cup2d_reset",
+    "cup2d_find_max_u", "cup2d_calc_max_timestep", "cup2d_put_objects", "cup2d_advdiff",
+    "cup2d_pressure_rhs", "cup2d_poisson_solve", "cup2d_pressure_correction",
+    "cup2d_body_update", "cup2d_penalize", "cup2d_pressure_single", "cup2d_compute_forces",
+    "cup2d_advance", "cup2d_simulate", "cup2d_synchronize", "cup2d_poisson_dense",
+    "cup2d_grid_size",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libcup2d_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first "
+                          "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P, D, I = C.c_void_p, C.c_double, C.c_int
+    dp = C.POINTER(C.c_double)
+    L.cup2d_create.argtypes = [C.POINTER(Cfg), C.POINTER(C.c_void_p)]
+    L.cup2d_destroy.argtypes = [P]
+    L.cup2d_last_error.argtypes = [P]
+    L.cup2d_last_error.restype = C.c_char_p
+    L.cup2d_add_shape.argtypes = [P, C.POINTER(ShapeDesc), C.POINTER(I)]
+    L.cup2d_upload_field.argtypes = [P, I, I, dp]
+    L.cup2d_download_field.argtypes = [P, I, I, dp]
+    L.cup2d_field_device_ptr.argtypes = [P, I, I, C.POINTER(C.c_void_p)]
+    L.cup2d_get_scalars.argtypes = [P, dp, dp, C.POINTER(I), dp, dp]
+    L.cup2d_set_scalars.argtypes = [P, D, I, D, D]
+    L.cup2d_get_shape.argtypes = [P, I, C.POINTER(ShapeState)]
+    for n in ["cup2d_reset", "cup2d_poisson_solve", "cup2d_compute_forces",
+              "cup2d_synchronize"]:
+        getattr(L, n).argtypes = [P]
+    for n in ["cup2d_put_objects", "cup2d_advdiff", "cup2d_pressure_rhs",
+              "cup2d_pressure_correction", "cup2d_body_update", "cup2d_penalize",
+              "cup2d_pressure_single", "cup2d_advance"]:
+        getattr(L, n).argtypes = [P, D]
+    L.cup2d_find_max_u.argtypes = [P, dp]
+    L.cup2d_calc_max_timestep.argtypes = [P, dp]
+    L.cup2d_simulate.argtypes = [P, I]
+    L.cup2d_poisson_dense.argtypes = [P, dp, dp]
+    L.cup2d_grid_size.argtypes = [P, C.POINTER(I), C.POINTER(I)]
+    for n in EXPORTS:
+        if n != "cup2d_last_error":
+            getattr(L, n).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def disk(radius, xpos, ypos, xvel=0.0, yvel=0.0, angvel=0.0, rhoS=1.0, angle=0.0,
+         bFixed=False, bForced=False, bFixedx=None, bFixedy=None, bForcedx=None,
+         bForcedy=None, bBlockAng=None, timeForced=1.7976931348623157e308, tAccel=-1.0):
+    """Disk descriptor with the reference's flag defaults (Shape.cpp:331-345)."""
+    bFixedx = bFixed if bFixedx is None else bFixedx
+    bFixedy = bFixed if bFixedy is None else bFixedy
+    bForcedx = bForced if bForcedx is None else bForcedx
+    bForcedy = bForced if bForcedy is None else bForcedy
+    bBlockAng = (bForcedx or bForcedy) if bBlockAng is None else bBlockAng
+    return dict(kind=0, radius=radius, xpos=xpos, ypos=ypos, angle_deg=angle, rhoS=rhoS,
+                bFixedx=int(bFixedx), bFixedy=int(bFixedy), bForcedx=int(bForcedx),
+                bForcedy=int(bForcedy), bBlockAng=int(bBlockAng), xvel=xvel, yvel=yvel,
+                angvel=angvel, timeForced=timeForced, tAccel=tAccel)
+
+
+class Simulation:
+    """One simulation on the GPU; methods mirror the reference's operators."""
+
+    def __init__(self, bpdx, bpdy, extent=1.0, nu=1e-2, CFL=0.1, lam=None, fadeLen=0.01,
+                 poisson="periodic", implicit_penal=False, shapes=(), stream=None):
+        self.L = lib()
+        lam = 1e3 / CFL if lam is None else lam
+        cfg = Cfg(bpdx, bpdy, extent, nu, CFL, lam, fadeLen, POISSON[poisson],
+                  int(implicit_penal), stream)
+        h = C.c_void_p()
+        rc = self.L.cup2d_create(C.byref(cfg), C.byref(h))
+        if rc != OK:
+            raise Cup2dError(rc, self.L.cup2d_last_error(None).decode())
+        self.h = h
+        self.bpdx, self.bpdy = bpdx, bpdy
+        self.NX, self.NY = bpdx * BS, bpdy * BS
+        for sd in shapes:
+            self.add_shape(sd)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cup2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise Cup2dError(rc, self.L.cup2d_last_error(self.h).decode())
+        return rc
+
+    def add_shape(self, d):
+        i = C.c_int()
+        self._chk(self.L.cup2d_add_shape(self.h, C.byref(ShapeDesc(**d)), C.byref(i)))
+        return i.value
+
+    # -- data
+    def set_field(self, f, arr, dense=True):
+        nc = NCOMP[f]
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        assert a.size == nc * self.NX * self.NY
+        self._chk(self.L.cup2d_upload_field(self.h, f, LAYOUT_DENSE if dense else LAYOUT_BLOCKS,
+                                            _dp(a)))
+
+    def get_field(self, f, dense=True):
+        nc = NCOMP[f]
+        a = np.empty(nc * self.NX * self.NY)
+        self._chk(self.L.cup2d_download_field(self.h, f,
+                                              LAYOUT_DENSE if dense else LAYOUT_BLOCKS, _dp(a)))
+        return a.reshape(nc, self.NY, self.NX) if dense else a
+
+    def device_ptr(self, f, comp=0):
+        p = C.c_void_p()
+        self._chk(self.L.cup2d_field_device_ptr(self.h, f, comp, C.byref(p)))
+        return p.value
+
+    def scalars(self):
+        t, dt, hh = C.c_double(), C.c_double(), C.c_double()
+        st = C.c_int()
+        u = (C.c_double * 2)()
+        self._chk(self.L.cup2d_get_scalars(self.h, C.byref(t), C.byref(dt), C.byref(st), u,
+                                           C.byref(hh)))
+        return dict(time=t.value, dt=dt.value, step=st.value, uinf=(u[0], u[1]), h=hh.value)
+
+    def set_scalars(self, time=0.0, step=0, uinf=(0.0, 0.0)):
+        self._chk(self.L.cup2d_set_scalars(self.h, time, step, uinf[0], uinf[1]))
+
+    def shape(self, i=0):
+        s = ShapeState()
+        self._chk(self.L.cup2d_get_shape(self.h, i, C.byref(s)))
+        return s.as_dict()
+
+    # -- operators (same names as the oracle wrapper)
+    def reset(self):
+        self._chk(self.L.cup2d_reset(self.h))
+
+    def find_max_u(self):
+        m = C.c_double()
+        self._chk(self.L.cup2d_find_max_u(self.h, C.byref(m)))
+        return m.value
+
+    def calc_max_timestep(self):
+        m = C.c_double()
+        self._chk(self.L.cup2d_calc_max_timestep(self.h, C.byref(m)))
+        return m.value
+
+    def put_objects(self, dt):
+        return self._chk(self.L.cup2d_put_objects(self.h, dt))
+
+    def advdiff(self, dt):
+        self._chk(self.L.cup2d_advdiff(self.h, dt))
+
+    def pressure_rhs(self, dt):
+        self._chk(self.L.cup2d_pressure_rhs(self.h, dt))
+
+    def poisson_solve(self):
+        self._chk(self.L.cup2d_poisson_solve(self.h))
+
+    def pressure_correction(self, dt):
+        self._chk(self.L.cup2d_pressure_correction(self.h, dt))
+
+    def body_update(self, dt):
+        self._chk(self.L.cup2d_body_update(self.h, dt))
+
+    def penalize(self, dt):
+        self._chk(self.L.cup2d_penalize(self.h, dt))
+
+    def pressure_single(self, dt):
+        self._chk(self.L.cup2d_pressure_single(self.h, dt))
+
+    def compute_forces(self):
+        self._chk(self.L.cup2d_compute_forces(self.h))
+
+    def advance(self, dt):
+        return self._chk(self.L.cup2d_advance(self.h, dt))
+
+    def simulate(self, nsteps, sync=True):
+        self._chk(self.L.cup2d_simulate(self.h, nsteps))
+        if sync:
+            self.synchronize()
+        return 0
+
+    def synchronize(self):
+        self._chk(self.L.cup2d_synchronize(self.h))
+
+    def poisson_dense(self, rhs):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        out = np.empty_like(rhs)
+        self._chk(self.L.cup2d_poisson_dense(self.h, _dp(rhs), _dp(out)))
+        return out

@@ -1,0 +1,484 @@
+// api.hip -- the extern "C" boundary of libcup2d_hip.so (declared in
+// include/cup2d.h).  Host-side bookkeeping only: allocation, validation,
+// layout conversion through a device staging buffer, and the operator
+// sequence of Simulation::advance (Simulation.cpp:279-305).
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace cup2d;
+
+namespace {
+thread_local std::string g_create_error;
+
+int fail(cup2d_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define CHECK_CTX(c)                                        \
+  do {                                                      \
+    if (!(c)) return CUP2D_ERR_ARG;                         \
+    (void)hipSetDevice((c)->device);                        \
+  } while (0)
+
+#define RC(x)                 \
+  do {                        \
+    const int rc_ = (x);      \
+    if (rc_) return rc_;      \
+  } while (0)
+
+int nfield_comps(int f) {
+  switch (f) {
+    case CUP2D_VEL: case CUP2D_TMPV: case CUP2D_UDEF: return 2;
+    case CUP2D_CHI: case CUP2D_PRES: case CUP2D_TMP: case CUP2D_INVRHO: return 1;
+    default: return 0;
+  }
+}
+
+void field_planes(cup2d_ctx* c, int f, double** a, double** b) {
+  *b = nullptr;
+  switch (f) {
+    case CUP2D_VEL: *a = c->u; *b = c->v; break;
+    case CUP2D_TMPV: *a = c->tu; *b = c->tv; break;
+    case CUP2D_UDEF: *a = c->udx; *b = c->udy; break;
+    case CUP2D_CHI: *a = c->chi; break;
+    case CUP2D_PRES: *a = c->pres; break;
+    case CUP2D_TMP: *a = c->rhs; break;
+    case CUP2D_INVRHO: *a = c->invrho; break;
+    default: *a = nullptr;
+  }
+}
+
+// PoissonSolver::cub2rhs `_is_touching` (PoissonSolver.cpp:71-79).
+std::vector<int> band_blocks(const Grid& g) {
+  std::vector<int> out;
+  for (int by = 0; by < g.bpdy; ++by)
+    for (int bx = 0; bx < g.bpdx; ++bx) {
+      const double mn0 = (double)bx * g.hb + g.h * (0 + 0.5);
+      const double mn1 = (double)by * g.hb + g.h * (0 + 0.5);
+      const double mx0 = (double)bx * g.hb + g.h * (BS - 1 + 0.5);
+      const double mx1 = (double)by * g.hb + g.h * (BS - 1 + 0.5);
+      const bool touchW = g.fadeLenX >= mn0, touchE = g.fadeLenX >= g.extent0 - mx0;
+      const bool touchS = g.fadeLenY >= mn1, touchN = g.fadeLenY >= g.extent1 - mx1;
+      if (touchN || touchE || touchS || touchW) out.push_back(bx + g.bpdx * by);
+    }
+  return out;
+}
+
+int ensure_frozen(cup2d_ctx* c) {
+  if (c->shapes_frozen) return CUP2D_OK;
+  RC(obstacles_alloc(c));
+  c->shapes_frozen = true;
+  return CUP2D_OK;
+}
+
+int sync_and_check(cup2d_ctx* c) {
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  int err = 0;
+  CUP2D_HIP_CHECK(c, hipMemcpy(&err, &c->st->error, sizeof(int), hipMemcpyDeviceToHost));
+  if (err == CUP2D_ERR_BODY_OUT) return fail(c, err, "a body left the domain");
+  if (err == CUP2D_ERR_CAPACITY) return fail(c, err, "obstacle-block pool exhausted");
+  if (err) return fail(c, err, "device error");
+  return CUP2D_OK;
+}
+
+int need_fft(cup2d_ctx* c) {
+  if (c->fft) return CUP2D_OK;
+  return fail(c, CUP2D_ERR_ARG, c->fft_err);
+}
+
+int pressure_single(cup2d_ctx* c) {
+  RC(need_fft(c));
+  RC(enqueue_pressure_rhs(c));
+  RC(enqueue_rhs_obstacles(c));
+  RC(enqueue_fade(c));
+  RC(enqueue_poisson_fft(c, c->rhs, c->pres));
+  RC(enqueue_pressure_correction(c));
+  RC(enqueue_body_update(c));
+  RC(enqueue_penalize(c));
+  return CUP2D_OK;
+}
+
+int advance_enqueued(cup2d_ctx* c) {
+  RC(enqueue_put_objects(c));
+  RC(enqueue_advdiff(c));
+  RC(pressure_single(c));
+  RC(enqueue_forces(c));
+  RC(enqueue_step_end(c));
+  return CUP2D_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* cup2d_last_error(const cup2d_ctx* c) {
+  return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
+  if (!cfg || !out) { g_create_error = "null argument"; return CUP2D_ERR_ARG; }
+  *out = nullptr;
+  if (cfg->bpdx <= 0 || cfg->bpdy <= 0 || !(cfg->extent > 0) || !(cfg->nu > 0) ||
+      !(cfg->CFL > 0) || (cfg->poisson != 0 && cfg->poisson != 1)) {
+    g_create_error = "invalid configuration (bpdx, bpdy, extent, nu, CFL > 0; poisson 0|1)";
+    return CUP2D_ERR_ARG;
+  }
+  cup2d_ctx* c = new cup2d_ctx();
+  c->cfg = *cfg;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    g_create_error = "no HIP device";
+    delete c;
+    return CUP2D_ERR_HIP;
+  }
+  Grid& g = c->g;
+  g.bpdx = cfg->bpdx; g.bpdy = cfg->bpdy;
+  g.NX = BS * g.bpdx; g.NY = BS * g.bpdy;
+  g.hb = cfg->extent / (double)(g.bpdx > g.bpdy ? g.bpdx : g.bpdy);
+  g.h = g.hb / BS;
+  g.extent0 = (double)g.bpdx * g.h * BS;  // SimulationData.cpp:34-35
+  g.extent1 = (double)g.bpdy * g.h * BS;
+  g.nu = cfg->nu; g.CFL = cfg->CFL; g.lambda = cfg->lambda;
+  g.fadeLenX = cfg->fadeLen * cfg->extent;  // Simulation.cpp:97-98
+  g.fadeLenY = cfg->fadeLen * cfg->extent;
+  g.implicit_penal = cfg->implicit_penal;
+  g.nshapes = 0;
+  int rc = CUP2D_OK;
+  auto bail = [&](int code) {
+    g_create_error = c->err;
+    cup2d_destroy(c);
+    return code;
+  };
+  if (cfg->stream) {
+    c->stream = (hipStream_t)cfg->stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      c->err = "hipStreamCreate failed";
+      return bail(CUP2D_ERR_HIP);
+    }
+    c->own_stream = true;
+  }
+  const size_t n = (size_t)g.NX * g.NY;
+  double** planes[] = {&c->u, &c->v, &c->tu, &c->tv, &c->pres, &c->rhs,
+                       &c->sdf, &c->chi, &c->udx, &c->udy, &c->invrho};
+  for (double** p : planes)
+    if (hipMalloc(p, sizeof(double) * n) != hipSuccess) { c->err = "out of device memory"; return bail(CUP2D_ERR_HIP); }
+  if (hipMalloc(&c->staging, sizeof(double) * 2 * n) != hipSuccess) { c->err = "out of device memory"; return bail(CUP2D_ERR_HIP); }
+  c->red_n = 1 << 16;
+  if (hipMalloc(&c->red, sizeof(double) * c->red_n) != hipSuccess) { c->err = "out of device memory"; return bail(CUP2D_ERR_HIP); }
+  if (hipMalloc(&c->st, sizeof(StepState)) != hipSuccess ||
+      hipMalloc(&c->sp, sizeof(ShapeParams) * MAX_SHAPES) != hipSuccess ||
+      hipMalloc(&c->ss, sizeof(cup2d_shape_state) * MAX_SHAPES) != hipSuccess ||
+      hipMalloc(&c->ramp, sizeof(double) * 100) != hipSuccess) {
+    c->err = "out of device memory";
+    return bail(CUP2D_ERR_HIP);
+  }
+  (void)hipMemset(c->ss, 0, sizeof(cup2d_shape_state) * MAX_SHAPES);
+  (void)hipMemset(c->sp, 0, sizeof(ShapeParams) * MAX_SHAPES);
+  // Simulation.cpp:264-269: rampCFL for the first 100 steps
+  double ramp[100];
+  for (int s = 0; s < 100; ++s) {
+    const double x = (s + 1.0) / 100;
+    ramp[s] = std::exp(std::log(1e-3) * (1 - x) + std::log(cfg->CFL) * x);
+  }
+  if (hipMemcpy(c->ramp, ramp, sizeof(ramp), hipMemcpyHostToDevice) != hipSuccess) { c->err = "copy failed"; return bail(CUP2D_ERR_HIP); }
+  const std::vector<int> band = band_blocks(g);
+  c->band_nblocks = (int)band.size();
+  if (c->band_nblocks > 0) {
+    if (hipMalloc(&c->band_blocks, sizeof(int) * band.size()) != hipSuccess ||
+        hipMemcpy(c->band_blocks, band.data(), sizeof(int) * band.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      c->err = "out of device memory";
+      return bail(CUP2D_ERR_HIP);
+    }
+  }
+  if ((size_t)3 * c->band_nblocks > c->red_n) { c->err = "grid too large for fade partials"; return bail(CUP2D_ERR_ARG); }
+  rc = fft_create(c);
+  if (rc == CUP2D_ERR_ARG) {
+    // unsupported FFT size: stencil operators still work, Poisson calls fail
+    c->fft_err = c->err;
+    fft_destroy(c);
+  } else if (rc) {
+    return bail(rc);
+  }
+  rc = enqueue_ic(c);
+  if (rc) return bail(rc);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) { c->err = "initialisation failed"; return bail(CUP2D_ERR_HIP); }
+  *out = c;
+  return CUP2D_OK;
+}
+
+int cup2d_destroy(cup2d_ctx* c) {
+  if (!c) return CUP2D_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  fft_destroy(c);
+  obstacles_free(c);
+  double* ptrs[] = {c->u, c->v, c->tu, c->tv, c->pres, c->rhs, c->sdf, c->chi,
+                    c->udx, c->udy, c->invrho, c->staging, c->red, c->ramp};
+  for (double* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->st) (void)hipFree(c->st);
+  if (c->sp) (void)hipFree(c->sp);
+  if (c->ss) (void)hipFree(c->ss);
+  if (c->band_blocks) (void)hipFree(c->band_blocks);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CUP2D_OK;
+}
+
+int cup2d_add_shape(cup2d_ctx* c, const cup2d_shape_desc* d, int* shape_id) {
+  CHECK_CTX(c);
+  if (!d) return fail(c, CUP2D_ERR_ARG, "null shape");
+  if (c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "shapes must be added before reset");
+  if (d->kind != 0) return fail(c, CUP2D_ERR_ARG, "only disk shapes (kind 0) are supported");
+  if (!(d->radius > 0)) return fail(c, CUP2D_ERR_ARG, "radius must be > 0");
+  if (c->g.nshapes >= MAX_SHAPES) return fail(c, CUP2D_ERR_ARG, "too many shapes");
+  ShapeParams p{};
+  p.kind = d->kind;
+  p.bFixedx = d->bFixedx; p.bFixedy = d->bFixedy;
+  p.bForcedx = d->bForcedx; p.bForcedy = d->bForcedy; p.bBlockAng = d->bBlockAng;
+  p.radius = d->radius; p.rhoS = d->rhoS;
+  // Shape::Shape (Shape.cpp:331-345)
+  p.forcedu = -d->xvel; p.forcedv = -d->yvel; p.forcedomega = -d->angvel;
+  p.timeForced = d->timeForced; p.tAccel = d->tAccel;
+  p.origC[0] = d->xpos; p.origC[1] = d->ypos;
+  p.origAng = d->angle_deg * M_PI / 180;
+  const int id = c->g.nshapes;
+  CUP2D_HIP_CHECK(c, hipMemcpy(c->sp + id, &p, sizeof(p), hipMemcpyHostToDevice));
+  c->hsp.push_back(p);
+  c->g.nshapes++;
+  if (shape_id) *shape_id = id;
+  return CUP2D_OK;
+}
+
+int cup2d_grid_size(const cup2d_ctx* c, int* NX, int* NY) {
+  if (!c || !NX || !NY) return CUP2D_ERR_ARG;
+  *NX = c->g.NX;
+  *NY = c->g.NY;
+  return CUP2D_OK;
+}
+
+int cup2d_upload_field(cup2d_ctx* c, int f, int layout, const double* host) {
+  CHECK_CTX(c);
+  const int nc = nfield_comps(f);
+  if (!nc || !host || (layout != CUP2D_LAYOUT_BLOCKS && layout != CUP2D_LAYOUT_DENSE))
+    return fail(c, CUP2D_ERR_ARG, "upload_field: bad field, layout or pointer");
+  double *a, *b;
+  field_planes(c, f, &a, &b);
+  const size_t n = (size_t)c->g.NX * c->g.NY;
+  if (layout == CUP2D_LAYOUT_DENSE) {
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(a, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    if (nc > 1)
+      CUP2D_HIP_CHECK(c, hipMemcpyAsync(b, host + n, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  } else {
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(c->staging, host, sizeof(double) * n * nc, hipMemcpyHostToDevice, c->stream));
+    RC(enqueue_blocks_to_dense(c, c->staging, a, b, nc));
+  }
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return CUP2D_OK;
+}
+
+int cup2d_download_field(cup2d_ctx* c, int f, int layout, double* host) {
+  CHECK_CTX(c);
+  const int nc = nfield_comps(f);
+  if (!nc || !host || (layout != CUP2D_LAYOUT_BLOCKS && layout != CUP2D_LAYOUT_DENSE))
+    return fail(c, CUP2D_ERR_ARG, "download_field: bad field, layout or pointer");
+  double *a, *b;
+  field_planes(c, f, &a, &b);
+  const size_t n = (size_t)c->g.NX * c->g.NY;
+  if (layout == CUP2D_LAYOUT_DENSE) {
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(host, a, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (nc > 1)
+      CUP2D_HIP_CHECK(c, hipMemcpyAsync(host + n, b, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    RC(enqueue_dense_to_blocks(c, a, b, c->staging, nc));
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(host, c->staging, sizeof(double) * n * nc, hipMemcpyDeviceToHost, c->stream));
+  }
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return CUP2D_OK;
+}
+
+int cup2d_field_device_ptr(cup2d_ctx* c, int f, int comp, void** dptr) {
+  CHECK_CTX(c);
+  const int nc = nfield_comps(f);
+  if (!nc || comp < 0 || comp >= nc || !dptr) return fail(c, CUP2D_ERR_ARG, "bad field/comp");
+  double *a, *b;
+  field_planes(c, f, &a, &b);
+  *dptr = comp == 0 ? a : b;
+  return CUP2D_OK;
+}
+
+int cup2d_get_scalars(cup2d_ctx* c, double* time, double* dt, int* step, double uinf[2],
+                      double* h) {
+  CHECK_CTX(c);
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  StepState s;
+  CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  if (time) *time = s.time;
+  if (dt) *dt = s.dt;
+  if (step) *step = s.step;
+  if (uinf) { uinf[0] = s.uinf[0]; uinf[1] = s.uinf[1]; }
+  if (h) *h = c->g.h;
+  return CUP2D_OK;
+}
+
+int cup2d_set_scalars(cup2d_ctx* c, double time, int step, double uinfx, double uinfy) {
+  CHECK_CTX(c);
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  StepState s;
+  CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  s.time = time; s.step = step; s.uinf[0] = uinfx; s.uinf[1] = uinfy;
+  CUP2D_HIP_CHECK(c, hipMemcpy(c->st, &s, sizeof(s), hipMemcpyHostToDevice));
+  return CUP2D_OK;
+}
+
+int cup2d_get_shape(cup2d_ctx* c, int i, cup2d_shape_state* out) {
+  CHECK_CTX(c);
+  if (i < 0 || i >= c->g.nshapes || !out) return fail(c, CUP2D_ERR_ARG, "bad shape id");
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  CUP2D_HIP_CHECK(c, hipMemcpy(out, c->ss + i, sizeof(*out), hipMemcpyDeviceToHost));
+  return CUP2D_OK;
+}
+
+int cup2d_reset(cup2d_ctx* c) {
+  CHECK_CTX(c);
+  RC(ensure_frozen(c));
+  RC(enqueue_shapes_reset(c));
+  RC(enqueue_ic(c));
+  RC(enqueue_put_objects(c));
+  RC(enqueue_apply_obj_vel(c));
+  return sync_and_check(c);
+}
+
+int cup2d_find_max_u(cup2d_ctx* c, double* maxU) {
+  CHECK_CTX(c);
+  RC(enqueue_find_max_u(c));
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  StepState s;
+  CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  if (maxU) *maxU = s.maxU;
+  return CUP2D_OK;
+}
+
+int cup2d_calc_max_timestep(cup2d_ctx* c, double* dt) {
+  CHECK_CTX(c);
+  RC(enqueue_dt(c));
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  StepState s;
+  CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  if (dt) *dt = s.dt;
+  return CUP2D_OK;
+}
+
+int cup2d_put_objects(cup2d_ctx* c, double dt) {
+  CHECK_CTX(c);
+  RC(ensure_frozen(c));
+  RC(enqueue_set_dt(c, dt));
+  RC(enqueue_put_objects(c));
+  return sync_and_check(c);
+}
+
+int cup2d_advdiff(cup2d_ctx* c, double dt) {
+  CHECK_CTX(c);
+  RC(enqueue_set_dt(c, dt));
+  RC(enqueue_advdiff(c));
+  return sync_and_check(c);
+}
+
+int cup2d_pressure_rhs(cup2d_ctx* c, double dt) {
+  CHECK_CTX(c);
+  RC(enqueue_set_dt(c, dt));
+  RC(enqueue_pressure_rhs(c));
+  if (c->shapes_frozen) RC(enqueue_rhs_obstacles(c));
+  return sync_and_check(c);
+}
+
+int cup2d_poisson_solve(cup2d_ctx* c) {
+  CHECK_CTX(c);
+  RC(need_fft(c));
+  RC(enqueue_fade(c));
+  RC(enqueue_poisson_fft(c, c->rhs, c->pres));
+  return sync_and_check(c);
+}
+
+int cup2d_pressure_correction(cup2d_ctx* c, double dt) {
+  CHECK_CTX(c);
+  RC(enqueue_set_dt(c, dt));
+  RC(enqueue_pressure_correction(c));
+  return sync_and_check(c);
+}
+
+int cup2d_body_update(cup2d_ctx* c, double dt) {
+  CHECK_CTX(c);
+  if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
+  RC(enqueue_set_dt(c, dt));
+  RC(enqueue_body_update(c));
+  return sync_and_check(c);
+}
+
+int cup2d_penalize(cup2d_ctx* c, double dt) {
+  CHECK_CTX(c);
+  if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
+  RC(enqueue_set_dt(c, dt));
+  RC(enqueue_penalize(c));
+  return sync_and_check(c);
+}
+
+int cup2d_pressure_single(cup2d_ctx* c, double dt) {
+  CHECK_CTX(c);
+  if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
+  RC(enqueue_set_dt(c, dt));
+  RC(pressure_single(c));
+  return sync_and_check(c);
+}
+
+int cup2d_compute_forces(cup2d_ctx* c) {
+  CHECK_CTX(c);
+  if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
+  RC(enqueue_forces(c));
+  return sync_and_check(c);
+}
+
+int cup2d_advance(cup2d_ctx* c, double dt) {
+  CHECK_CTX(c);
+  if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
+  RC(enqueue_set_dt(c, dt));
+  RC(advance_enqueued(c));
+  return sync_and_check(c);
+}
+
+int cup2d_simulate(cup2d_ctx* c, int nsteps) {
+  CHECK_CTX(c);
+  if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
+  if (nsteps < 0) return fail(c, CUP2D_ERR_ARG, "nsteps < 0");
+  for (int s = 0; s < nsteps; ++s) {
+    RC(enqueue_dt(c));
+    RC(advance_enqueued(c));
+  }
+  return CUP2D_OK;
+}
+
+int cup2d_synchronize(cup2d_ctx* c) {
+  CHECK_CTX(c);
+  return sync_and_check(c);
+}
+
+int cup2d_poisson_dense(cup2d_ctx* c, const double* rhs, double* sol) {
+  CHECK_CTX(c);
+  if (!rhs || !sol) return fail(c, CUP2D_ERR_ARG, "null pointer");
+  RC(need_fft(c));
+  const size_t n = (size_t)c->g.NX * c->g.NY;
+  double* din = c->staging;
+  double* dout = c->staging + n;
+  CUP2D_HIP_CHECK(c, hipMemcpyAsync(din, rhs, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  RC(enqueue_poisson_fft(c, din, dout));
+  CUP2D_HIP_CHECK(c, hipMemcpyAsync(sol, dout, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return CUP2D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,604 @@
+// fft.hip -- FFT Poisson solvers (periodic and zero-padded free-space) on gfx950.
+//
+// Replaces FFTW_periodic::solve (FFTW_periodic.h:135-166), FFTW_freespace::solve
+// (FFTW_freespace.cpp:14-56) and the reference's cuFFT path (CUDA_all.cu:41-128).
+// Three kernels per solve, each reading and writing HBM exactly once:
+//   1. rows forward: two real rows are packed as one complex row (a + i b),
+//      one complex Stockham FFT in LDS, then split into both half spectra.
+//      DC and Nyquist (both real) share one complex slot.  Output goes to a
+//      column-group-major spectrum spec[cg][y][G] (G = 4 complex = 64 B), so
+//      the two rows of a workgroup fill whole 128 B lines.
+//   2. columns: one workgroup owns a column group (contiguous in memory),
+//      forward FFT, spectral multiply (-1/k^2 or the Green's function
+//      transform), inverse FFT, write back in place.  Free-space columns are
+//      zero-pruned: only the NY data rows are read and written.
+//   3. rows inverse: unpack two half-spectrum rows, inverse complex FFT, real
+//      part -> row y, imaginary part -> row y+1.
+// The FFT core is a radix-16 Stockham autosort (radix 2/4/8 for the last
+// pass), 16 complex values per thread in registers, LDS padded by one slot
+// every 16 entries so stride-16 stores are bank-conflict free.  Twiddles come
+// from per-length tables exp(-2 pi i k/L) built in long double on the host.
+#include <cmath>
+#include <vector>
+
+#include "device_util.hpp"
+
+namespace cup2d {
+
+constexpr int G = 4;  // complex columns per group (64 B in FP64)
+
+struct FFTPlan {
+  int Lr = 0, Lc = 0;     // row / column FFT lengths (complex)
+  int NYs = 0;            // rows stored in the spectrum
+  int ncg = 0;            // column groups
+  int fs = 0;             // free-space
+  double2* spec = nullptr;
+  double2* tw_r = nullptr;
+  double2* tw_c = nullptr;
+  double* ghat = nullptr;   // free-space: [ncg][NY+1][G]
+  double* ghatN = nullptr;  // free-space Nyquist column [NY+1]
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// multiply by -i (forward) or +i (inverse)
+template <bool FWD>
+__device__ __forceinline__ double2 rot(double2 a) {
+  return FWD ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+}
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+
+template <bool FWD>
+__device__ __forceinline__ void dft2(double2& a, double2& b) {
+  const double2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <bool FWD>
+__device__ __forceinline__ void dft4(double2& x0, double2& x1, double2& x2, double2& x3) {
+  const double2 a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3);
+  const double2 a3 = rot<FWD>(csub(x1, x3));
+  x0 = cadd(a0, a2);
+  x1 = cadd(a1, a3);
+  x2 = csub(a0, a2);
+  x3 = csub(a1, a3);
+}
+
+// W_N^m = exp(-2 pi i m/N) (forward) constants
+template <bool FWD>
+__device__ __forceinline__ double2 w16(int m) {
+  constexpr double c1 = 0.92387953251128673848, s1 = 0.38268343236508978178;
+  constexpr double c2 = 0.70710678118654752440;
+  double2 w;
+  switch (m) {
+    case 0: w = make_double2(1, 0); break;
+    case 1: w = make_double2(c1, -s1); break;
+    case 2: w = make_double2(c2, -c2); break;
+    case 3: w = make_double2(s1, -c1); break;
+    case 4: w = make_double2(0, -1); break;
+    case 6: w = make_double2(-c2, -c2); break;
+    case 9: w = make_double2(-c1, s1); break;
+    default: w = make_double2(1, 0); break;
+  }
+  if (!FWD) w.y = -w.y;
+  return w;
+}
+
+template <int R, bool FWD>
+__device__ __forceinline__ void dft(double2* v) {
+  if constexpr (R == 2) {
+    dft2<FWD>(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    dft4<FWD>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 8) {
+    double2 y[2][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 2; ++n2) {
+      y[n2][0] = v[n2]; y[n2][1] = v[n2 + 2]; y[n2][2] = v[n2 + 4]; y[n2][3] = v[n2 + 6];
+      dft4<FWD>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+    }
+    // W8^k1 = W16^(2 k1)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) y[1][k1] = cmul(y[1][k1], w16<FWD>(2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      v[k1] = cadd(y[0][k1], y[1][k1]);
+      v[k1 + 4] = csub(y[0][k1], y[1][k1]);
+    }
+  } else {  // 16
+    double2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      y[n2][0] = v[n2]; y[n2][1] = v[n2 + 4]; y[n2][2] = v[n2 + 8]; y[n2][3] = v[n2 + 12];
+      dft4<FWD>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+      for (int k1 = 1; k1 < 4; ++k1) y[n2][k1] = cmul(y[n2][k1], w16<FWD>(n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      dft4<FWD>(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
+      v[k1] = y[0][k1]; v[k1 + 4] = y[1][k1]; v[k1 + 8] = y[2][k1]; v[k1 + 12] = y[3][k1];
+    }
+  }
+}
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+// One Stockham pass of radix R at span NS over an L-point sequence held in
+// LDS at lds[base + pad(i)*stride]; NT = L/16 threads, 16/R butterflies each.
+template <int L, int R, int NS, bool FWD>
+__device__ __forceinline__ void stockham_pass(double2* lds, int base, int stride, int t,
+                                              const double2* __restrict__ tw) {
+  constexpr int NT = L / 16, NBF = L / R, PER = 16 / R;
+  double2 v[PER][R];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int j = t + p * NT;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[p][r] = lds[base + pad(j + r * NBF) * stride];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int j = t + p * NT;
+    const int k = j % NS;
+    if constexpr (NS > 1) {
+      constexpr int TSTEP = L / (NS * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        double2 w = tw[k * TSTEP * r];
+        if (!FWD) w.y = -w.y;
+        v[p][r] = cmul(v[p][r], w);
+      }
+    }
+    dft<R, FWD>(v[p]);
+    const int idxD = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[base + pad(idxD + r * NS) * stride] = v[p][r];
+  }
+  __syncthreads();
+}
+
+template <int L, int NS, bool FWD>
+__device__ __forceinline__ void fft_passes(double2* lds, int base, int stride, int t,
+                                           const double2* __restrict__ tw) {
+  if constexpr (NS < L) {
+    constexpr int REM = L / NS;
+    constexpr int R = REM >= 16 ? 16 : REM;
+    stockham_pass<L, R, NS, FWD>(lds, base, stride, t, tw);
+    fft_passes<L, NS * R, FWD>(lds, base, stride, t, tw);
+  }
+}
+
+// ---------------------------------------------------------- rows forward
+// in: nrows rows of ncols valid reals (row pitch in_pitch), zero beyond;
+// out: spec[cg][y][G], y < spec_rows.
+template <int L, int NP>
+__global__ __launch_bounds__(NP * L / 16) void k_rows_fwd(const double* __restrict__ in,
+                                                          int in_pitch, int ncols, int nrows,
+                                                          double2* __restrict__ spec,
+                                                          int spec_rows,
+                                                          const double2* __restrict__ tw) {
+  constexpr int NT = L / 16, PADL = L + L / 16;
+  extern __shared__ double2 lds[];
+  const int f = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int y0 = 2 * (blockIdx.x * NP + f);
+  const bool valid = y0 < nrows;
+  const int base = f * PADL;
+  const double* r0 = in + (size_t)y0 * in_pitch;
+  const double* r1 = r0 + in_pitch;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = t + r * NT;
+    double a = 0, b = 0;
+    if (valid && i < ncols) { a = r0[i]; b = r1[i]; }
+    lds[base + pad(i)] = make_double2(a, b);
+  }
+  __syncthreads();
+  fft_passes<L, 1, true>(lds, base, 1, t, tw);
+  if (!valid) return;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int k = t + r * NT;  // 0 .. L/2-1
+    const double2 Z = lds[base + pad(k)];
+    double2 A, B;
+    if (k == 0) {
+      const double2 ZN = lds[base + pad(L / 2)];
+      A = make_double2(Z.x, ZN.x);
+      B = make_double2(Z.y, ZN.y);
+    } else {
+      const double2 Zm = lds[base + pad(L - k)];
+      const double2 s = make_double2(Z.x + Zm.x, Z.y - Zm.y);  // Z + conj(Zm)
+      const double2 d = make_double2(Z.x - Zm.x, Z.y + Zm.y);  // Z - conj(Zm)
+      A = make_double2(0.5 * s.x, 0.5 * s.y);
+      B = make_double2(0.5 * d.y, -0.5 * d.x);
+    }
+    const int cg = k / G, c = k % G;
+    double2* o = spec + ((size_t)cg * spec_rows + y0) * G + c;
+    o[0] = A;
+    o[G] = B;
+  }
+}
+
+// ---------------------------------------------------------- rows inverse
+template <int L, int NP>
+__global__ __launch_bounds__(NP * L / 16) void k_rows_inv(const double2* __restrict__ spec,
+                                                          int spec_rows, double* __restrict__ out,
+                                                          int out_pitch, int ncols, int nrows,
+                                                          const double2* __restrict__ tw) {
+  constexpr int NT = L / 16, PADL = L + L / 16;
+  extern __shared__ double2 lds[];
+  const int f = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int y0 = 2 * (blockIdx.x * NP + f);
+  const bool valid = y0 < nrows;
+  const int base = f * PADL;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int k = t + r * NT;
+    double2 A = make_double2(0, 0), B = make_double2(0, 0);
+    if (valid) {
+      const double2* s = spec + ((size_t)(k / G) * spec_rows + y0) * G + (k % G);
+      A = s[0];
+      B = s[G];
+    }
+    if (k == 0) {
+      lds[base + pad(0)] = make_double2(A.x, B.x);
+      lds[base + pad(L / 2)] = make_double2(A.y, B.y);
+    } else {
+      lds[base + pad(k)] = make_double2(A.x - B.y, A.y + B.x);
+      lds[base + pad(L - k)] = make_double2(A.x + B.y, -A.y + B.x);
+    }
+  }
+  __syncthreads();
+  fft_passes<L, 1, false>(lds, base, 1, t, tw);
+  if (!valid) return;
+  double* o0 = out + (size_t)y0 * out_pitch;
+  double* o1 = o0 + out_pitch;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = t + r * NT;
+    if (i < ncols) {
+      const double2 z = lds[base + pad(i)];
+      o0[i] = z.x;
+      o1[i] = z.y;
+    }
+  }
+}
+
+// --------------------------------------------------------------- columns
+// MODE 0: periodic solve, 1: free-space solve (zero-pruned), 2: forward only.
+struct ColParams {
+  int NX, NY;            // physical grid
+  int rows;              // data rows stored per column (NY, or Lc for MODE 2)
+  double waveFactX, waveFactY, norm;
+  const double* ghat;    // MODE 1
+  const double* ghatN;
+};
+
+__device__ __forceinline__ double periodic_factor(int kx, int ky, const ColParams& P) {
+  // FFTW_periodic.h:82-91: kinv*norm_factor, DC set to zero
+  const int kyy = ky <= P.NY / 2 ? ky : P.NY - ky;
+  if (kx == 0 && kyy == 0) return 0.0;
+  const double rkx = kx * P.waveFactX, rky = kyy * P.waveFactY;
+  const double kinv = -1 / (rkx * rkx + rky * rky);
+  return kinv * P.norm;
+}
+
+template <int LC, int MODE>
+__global__ __launch_bounds__(G * LC / 16) void k_cols(double2* __restrict__ spec, ColParams P,
+                                                      const double2* __restrict__ tw) {
+  constexpr int NT = LC / 16;
+  extern __shared__ double2 lds[];
+  const int c = threadIdx.x % G, t = threadIdx.x / G;
+  const int cg = blockIdx.x;
+  double2* col = spec + (size_t)cg * P.rows * G;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = t + r * NT;
+    double2 z = make_double2(0, 0);
+    if (MODE != 1 || i < P.rows) z = col[(size_t)i * G + c];
+    lds[pad(i) * G + c] = z;
+  }
+  __syncthreads();
+  fft_passes<LC, 1, true>(lds, c, G, t, tw);
+  if constexpr (MODE == 2) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = t + r * NT;
+      col[(size_t)i * G + c] = lds[pad(i) * G + c];
+    }
+    return;
+  } else {
+    const int kx = cg * G + c;
+    if (cg == 0) {
+      // column 0 packs the DC (kx=0) and Nyquist (kx=NX/2) columns as
+      // c = dc + i nq; with real, even symbols s0, sN:
+      // C'(k) = C(k) (s0+sN)/2 + conj(C(-k)) (s0-sN)/2
+      double2 o[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = t + r * NT;
+        const double2 z = lds[pad(i) * G + c];
+        double s0, sN = 0.0;
+        if constexpr (MODE == 0) {
+          s0 = periodic_factor(kx, i, P);
+          if (kx == 0) sN = periodic_factor(P.NX / 2, i, P);
+        } else {
+          const int kyy = i <= P.NY ? i : LC - i;
+          s0 = P.ghat[(size_t)kyy * G + c];
+          if (kx == 0) sN = P.ghatN[kyy];
+        }
+        if (kx == 0) {
+          const double2 m = lds[pad((LC - i) % LC) * G + c];
+          const double a = 0.5 * (s0 + sN), b = 0.5 * (s0 - sN);
+          o[r] = make_double2(z.x * a + m.x * b, z.y * a - m.y * b);
+        } else {
+          o[r] = make_double2(z.x * s0, z.y * s0);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 16; ++r) lds[pad(t + r * NT) * G + c] = o[r];
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = t + r * NT;
+        double s;
+        if constexpr (MODE == 0) {
+          s = periodic_factor(kx, i, P);
+        } else {
+          const int kyy = i <= P.NY ? i : LC - i;
+          s = P.ghat[((size_t)cg * (P.NY + 1) + kyy) * G + c];
+        }
+        const double2 z = lds[pad(i) * G + c];
+        lds[pad(i) * G + c] = make_double2(z.x * s, z.y * s);
+      }
+      __syncthreads();
+    }
+    fft_passes<LC, 1, false>(lds, c, G, t, tw);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = t + r * NT;
+      if (MODE != 1 || i < P.rows) col[(size_t)i * G + c] = lds[pad(i) * G + c];
+    }
+  }
+}
+
+// Green's function spectrum: Re(FFT G) * norm, even-folded to ky <= NY.
+// spec holds the full forward transform of the (2NX)x(2NY) kernel.
+__global__ void k_ghat_extract(const double2* __restrict__ spec, int Lc, int ncg, int NY,
+                               double norm, double* __restrict__ ghat,
+                               double* __restrict__ ghatN) {
+  const int n = ncg * G * (NY + 1);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int c = q % G, ky = (q / G) % (NY + 1), cg = q / (G * (NY + 1));
+    const double2 C = spec[((size_t)cg * Lc + ky) * G + c];
+    if (cg == 0 && c == 0) {
+      const double2 Cm = spec[((size_t)0 * Lc + (Lc - ky) % Lc) * G + 0];
+      // DC = (C + conj Cm)/2, Nyq = (C - conj Cm)/(2i): real parts
+      ghat[q] = 0.5 * (C.x + Cm.x) * norm;
+      ghatN[ky] = 0.5 * (C.y + Cm.y) * norm;
+    } else {
+      ghat[q] = C.x * norm;
+    }
+  }
+}
+
+// ----------------------------------------------------------- dispatch
+template <int L>
+static int rows_np() { return (L / 16) >= 256 ? 1 : 256 / (L / 16); }
+
+typedef void (*launch_fn)(cup2d_ctx*, const void*, void*, int, int, int, int, int);
+
+template <int L>
+static void launch_rows_fwd(cup2d_ctx* c, const double* in, int pitch, int ncols, int nrows,
+                            double2* spec, int spec_rows, const double2* tw) {
+  constexpr int NT = L / 16;
+  constexpr int NP = NT >= 256 ? 1 : 256 / NT;
+  const size_t shm = (size_t)NP * (L + L / 16) * sizeof(double2);
+  const int pairs = nrows / 2;
+  hipLaunchKernelGGL((k_rows_fwd<L, NP>), dim3((pairs + NP - 1) / NP), dim3(NP * NT), shm,
+                     c->stream, in, pitch, ncols, nrows, spec, spec_rows, tw);
+}
+template <int L>
+static void launch_rows_inv(cup2d_ctx* c, const double2* spec, int spec_rows, double* out,
+                            int pitch, int ncols, int nrows, const double2* tw) {
+  constexpr int NT = L / 16;
+  constexpr int NP = NT >= 256 ? 1 : 256 / NT;
+  const size_t shm = (size_t)NP * (L + L / 16) * sizeof(double2);
+  const int pairs = nrows / 2;
+  hipLaunchKernelGGL((k_rows_inv<L, NP>), dim3((pairs + NP - 1) / NP), dim3(NP * NT), shm,
+                     c->stream, spec, spec_rows, out, pitch, ncols, nrows, tw);
+}
+template <int LC, int MODE>
+static void launch_cols(cup2d_ctx* c, double2* spec, int ncg, const ColParams& P,
+                        const double2* tw) {
+  const size_t shm = (size_t)(LC + LC / 16) * G * sizeof(double2);
+  hipLaunchKernelGGL((k_cols<LC, MODE>), dim3(ncg), dim3(G * LC / 16), shm, c->stream, spec, P,
+                     tw);
+}
+
+#define CUP2D_POW2_CASES(M, ...)                                                 \
+  switch (L) {                                                                   \
+    case 32: M<32>(__VA_ARGS__); break;                                          \
+    case 64: M<64>(__VA_ARGS__); break;                                          \
+    case 128: M<128>(__VA_ARGS__); break;                                        \
+    case 256: M<256>(__VA_ARGS__); break;                                        \
+    case 512: M<512>(__VA_ARGS__); break;                                        \
+    case 1024: M<1024>(__VA_ARGS__); break;                                      \
+    case 2048: M<2048>(__VA_ARGS__); break;                                      \
+    case 4096: M<4096>(__VA_ARGS__); break;                                      \
+    case 8192: M<8192>(__VA_ARGS__); break;                                      \
+    default: return CUP2D_ERR_ARG;                                               \
+  }
+
+static int rows_fwd(cup2d_ctx* c, int L, const double* in, int pitch, int ncols, int nrows,
+                    double2* spec, int spec_rows, const double2* tw) {
+  CUP2D_POW2_CASES(launch_rows_fwd, c, in, pitch, ncols, nrows, spec, spec_rows, tw);
+  return CUP2D_OK;
+}
+static int rows_inv(cup2d_ctx* c, int L, const double2* spec, int spec_rows, double* out,
+                    int pitch, int ncols, int nrows, const double2* tw) {
+  CUP2D_POW2_CASES(launch_rows_inv, c, spec, spec_rows, out, pitch, ncols, nrows, tw);
+  return CUP2D_OK;
+}
+template <int MODE>
+static int cols(cup2d_ctx* c, int LC, double2* spec, int ncg, const ColParams& P,
+                const double2* tw) {
+  switch (LC) {
+    case 32: launch_cols<32, MODE>(c, spec, ncg, P, tw); break;
+    case 64: launch_cols<64, MODE>(c, spec, ncg, P, tw); break;
+    case 128: launch_cols<128, MODE>(c, spec, ncg, P, tw); break;
+    case 256: launch_cols<256, MODE>(c, spec, ncg, P, tw); break;
+    case 512: launch_cols<512, MODE>(c, spec, ncg, P, tw); break;
+    case 1024: launch_cols<1024, MODE>(c, spec, ncg, P, tw); break;
+    case 2048: launch_cols<2048, MODE>(c, spec, ncg, P, tw); break;
+    default: return CUP2D_ERR_ARG;
+  }
+  return CUP2D_OK;
+}
+
+template <int L, int NP>
+static void set_rows_attr() {
+  const int shm = NP * (L + L / 16) * (int)sizeof(double2);
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+}
+template <int L>
+static void set_rows_attr_L() {
+  constexpr int NT = L / 16;
+  constexpr int NP = NT >= 256 ? 1 : 256 / NT;
+  set_rows_attr<L, NP>();
+}
+template <int LC>
+static void set_cols_attr() {
+  const int shm = (LC + LC / 16) * G * (int)sizeof(double2);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+}
+
+static std::vector<double2> twiddles(int L) {
+  std::vector<double2> w(L);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < L; ++k) {
+    const long double a = -2.0L * pi * (long double)k / (long double)L;
+    w[k] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+  return w;
+}
+
+static bool pow2(int n) { return n >= 32 && (n & (n - 1)) == 0; }
+
+int fft_create(cup2d_ctx* c) {
+  const int NX = c->g.NX, NY = c->g.NY;
+  const bool fs = c->cfg.poisson == CUP2D_POISSON_FREESPACE;
+  FFTPlan* P = new FFTPlan();
+  c->fft = P;
+  P->fs = fs;
+  P->Lr = fs ? 2 * NX : NX;
+  P->Lc = fs ? 2 * NY : NY;
+  P->NYs = NY;
+  P->ncg = (P->Lr / 2) / G;
+  if (!pow2(NX) || !pow2(NY) || P->Lr > 8192 || P->Lc > 2048) {
+    c->err = "FFT Poisson: NX and NY must be powers of two, row length <= 8192 and column "
+             "length <= 2048 (periodic NY <= 2048, free-space NY <= 1024)";
+    return CUP2D_ERR_ARG;
+  }
+  set_rows_attr_L<32>(); set_rows_attr_L<64>(); set_rows_attr_L<128>(); set_rows_attr_L<256>();
+  set_rows_attr_L<512>(); set_rows_attr_L<1024>(); set_rows_attr_L<2048>();
+  set_rows_attr_L<4096>(); set_rows_attr_L<8192>();
+  set_cols_attr<32>(); set_cols_attr<64>(); set_cols_attr<128>(); set_cols_attr<256>();
+  set_cols_attr<512>(); set_cols_attr<1024>(); set_cols_attr<2048>();
+
+  CUP2D_HIP_CHECK(c, hipMalloc(&P->spec, sizeof(double2) * (size_t)P->ncg * G * P->NYs));
+  const std::vector<double2> twr = twiddles(P->Lr), twc = twiddles(P->Lc);
+  CUP2D_HIP_CHECK(c, hipMalloc(&P->tw_r, sizeof(double2) * twr.size()));
+  CUP2D_HIP_CHECK(c, hipMalloc(&P->tw_c, sizeof(double2) * twc.size()));
+  CUP2D_HIP_CHECK(c, hipMemcpy(P->tw_r, twr.data(), sizeof(double2) * twr.size(), hipMemcpyHostToDevice));
+  CUP2D_HIP_CHECK(c, hipMemcpy(P->tw_c, twc.data(), sizeof(double2) * twc.size(), hipMemcpyHostToDevice));
+  if (!fs) return CUP2D_OK;
+
+  // Free-space Green's function on the 2NX x 2NY cyclic grid
+  // (FFTW_freespace.cpp:321-358; with 2N padding the first NX x NY outputs of
+  // the cyclic convolution equal the reference's (2N-1) result).
+  const int MX = 2 * NX, MY = 2 * NY;
+  std::vector<double> Gk((size_t)MX * MY);
+  const double h = c->g.h, fac = 1 / (2 * M_PI);
+  for (int j = 0; j < MY; ++j)
+    for (int i = 0; i < MX; ++i) {
+      const double yi = j >= NY ? MY - j : j;
+      const double xi = i >= NX ? MX - i : i;
+      const double r = std::sqrt(xi * xi + yi * yi);
+      Gk[(size_t)j * MX + i] = fac * std::log(h * r);
+    }
+  Gk[0] = fac / 2 * (2 * std::log(h / std::sqrt(M_PI)) - 1);
+  double* dG = nullptr;
+  double2* tspec = nullptr;
+  CUP2D_HIP_CHECK(c, hipMalloc(&dG, sizeof(double) * Gk.size()));
+  CUP2D_HIP_CHECK(c, hipMalloc(&tspec, sizeof(double2) * (size_t)P->ncg * G * MY));
+  CUP2D_HIP_CHECK(c, hipMemcpy(dG, Gk.data(), sizeof(double) * Gk.size(), hipMemcpyHostToDevice));
+  CUP2D_HIP_CHECK(c, hipMalloc(&P->ghat, sizeof(double) * (size_t)P->ncg * G * (NY + 1)));
+  CUP2D_HIP_CHECK(c, hipMalloc(&P->ghatN, sizeof(double) * (size_t)(NY + 1)));
+  int rc = rows_fwd(c, P->Lr, dG, MX, MX, MY, tspec, MY, P->tw_r);
+  if (rc) return rc;
+  ColParams cp{};
+  cp.NX = NX; cp.NY = NY; cp.rows = MY;
+  rc = cols<2>(c, P->Lc, tspec, P->ncg, cp, P->tw_c);
+  if (rc) return rc;
+  const double norm = 1.0 / ((double)MY * MX);
+  hipLaunchKernelGGL(k_ghat_extract, dim3(256), dim3(256), 0, c->stream, tspec, MY, P->ncg, NY,
+                     norm, P->ghat, P->ghatN);
+  rc = launch_check(c, "green kernel");
+  if (rc) return rc;
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(dG);
+  (void)hipFree(tspec);
+  return CUP2D_OK;
+}
+
+void fft_destroy(cup2d_ctx* c) {
+  FFTPlan* P = c->fft;
+  if (!P) return;
+  (void)hipFree(P->spec);
+  (void)hipFree(P->tw_r);
+  (void)hipFree(P->tw_c);
+  if (P->ghat) (void)hipFree(P->ghat);
+  if (P->ghatN) (void)hipFree(P->ghatN);
+  delete P;
+  c->fft = nullptr;
+}
+
+int enqueue_poisson_fft(cup2d_ctx* c, const double* rhs, double* sol) {
+  FFTPlan* P = c->fft;
+  const int NX = c->g.NX, NY = c->g.NY;
+  int rc = rows_fwd(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r);
+  if (rc) return rc;
+  ColParams cp{};
+  cp.NX = NX; cp.NY = NY; cp.rows = NY;
+  if (!P->fs) {
+    cp.waveFactX = 2 * M_PI / NX;
+    cp.waveFactY = 2 * M_PI / NY;
+    cp.norm = 1.0 / ((double)NX * NY);
+    rc = cols<0>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
+  } else {
+    cp.ghat = P->ghat;
+    cp.ghatN = P->ghatN;
+    rc = cols<1>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
+  }
+  if (rc) return rc;
+  rc = rows_inv(c, P->Lr, P->spec, P->NYs, sol, NX, NX, NY, P->tw_r);
+  if (rc) return rc;
+  return launch_check(c, "poisson_fft");
+}
+
+}  // namespace cup2d

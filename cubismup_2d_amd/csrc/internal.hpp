@@ -1,0 +1,161 @@
+// internal.hpp -- device data structures of libcup2d_hip (gfx950).
+//
+// Device layout: every field is a dense row-major plane per component,
+// index y*NX + x, with NX = 32*bpdx, NY = 32*bpdy.  The reference's block
+// grid (32x32 blocks in blockID order, AoS vector elements) is only a host
+// interchange format; ghosts are the neighbouring row/column of the dense
+// plane, domain faces are filled at load time with BlockLabOpen's
+// zero-gradient rule (Definitions.h:125-157).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cup2d.h"
+
+namespace cup2d {
+
+constexpr int BS = CUP2D_BS;
+constexpr int MAX_SHAPES = 64;
+constexpr int NPART = 24;  // doubles of partial sums per obstacle slot
+
+// Constant per-shape parameters (Shape.h:22-41, ShapesSimple.h:16-17).
+struct ShapeParams {
+  int kind;
+  int bFixedx, bFixedy, bForcedx, bForcedy, bBlockAng;
+  double radius, rhoS;
+  double forcedu, forcedv, forcedomega, timeForced, tAccel;
+  double origC[2], origAng;
+  double appliedForceX, appliedForceY, appliedTorque;
+};
+
+// Step scalars held on the device (SimulationData.h:346-368).
+struct StepState {
+  double time, dt;
+  double uinf[2];
+  int step;
+  int error;     // sticky: CUP2D_ERR_BODY_OUT / CUP2D_ERR_CAPACITY
+  int nob;       // obstacle slots (shape, block) this step
+  int nub;       // distinct obstacle blocks this step
+  int nprev;     // blocks touched by the previous step (cleared next step)
+  int pad_;
+  double maxU;   // last findMaxU result
+  double fade_corr;  // cub2rhs correction factor of the last solve
+  double flux_corr;  // advDiff boundary-flux correction of the last step
+};
+
+// Constant run parameters passed by value to kernels.
+struct Grid {
+  int NX, NY, bpdx, bpdy;
+  double h, hb, extent0, extent1;
+  double nu, CFL, lambda, fadeLenX, fadeLenY;
+  int implicit_penal, nshapes;
+};
+
+// Obstacle pool (ObstacleBlock.h:266-378, one slot per (shape, block)).
+struct Obstacles {
+  int cap;            // slot capacity
+  int* slot_shape;    // [cap]
+  int* slot_block;    // [cap] blockID
+  int* ub_block;      // [cap] distinct block ids (ascending)
+  int* ub_start;      // [cap]
+  int* ub_count;      // [cap]
+  int* ub_slots;      // [cap] slots grouped per distinct block, shape order
+  int* prev_blocks;   // [cap] blocks to clear next step
+  int* slot_of_shape_start;  // [MAX_SHAPES+1] slot range per shape
+  double* chi;        // [cap][BS*BS]
+  double* dist;
+  double* rho;
+  double* udef;       // [cap][2][BS*BS]
+  double* sdelta;     // [cap][BS*BS] surface delta D (>0 marks a surface point)
+  double* sdx;        // [cap][BS*BS] dchidx
+  double* sdy;        // [cap][BS*BS] dchidy
+  double* part;       // [cap][NPART]
+};
+
+struct FFTPlan;  // fft.hip
+
+}  // namespace cup2d
+
+struct cup2d_ctx {
+  cup2d_cfg cfg;
+  cup2d::Grid g;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int device = 0;
+  std::string err;
+
+  // fields (dense planes)
+  double* u = nullptr;   // velocity x
+  double* v = nullptr;   // velocity y
+  double* tu = nullptr;  // tmpV x (advDiff output buffer, swapped with u)
+  double* tv = nullptr;
+  double* pres = nullptr;
+  double* rhs = nullptr;   // sim.tmp during PressureSingle (the Poisson RHS)
+  double* sdf = nullptr;   // sim.tmp during PutObjectsOnGrid (signed distance)
+  double* chi = nullptr;
+  double* udx = nullptr;   // sim.uDef
+  double* udy = nullptr;
+  double* invrho = nullptr;
+  double* staging = nullptr;  // host-layout staging (2 comps)
+  double* red = nullptr;      // reduction partials
+  size_t red_n = 0;
+
+  cup2d::StepState* st = nullptr;          // device
+  cup2d::ShapeParams* sp = nullptr;        // device [MAX_SHAPES]
+  cup2d_shape_state* ss = nullptr;         // device [MAX_SHAPES]
+  double* ramp = nullptr;                  // device [100] CFL ramp factors
+  std::vector<cup2d::ShapeParams> hsp;     // host copy of the parameters
+  cup2d::Obstacles ob{};
+  int band_nblocks = 0;
+  int* band_blocks = nullptr;              // blocks touched by the cub2rhs fade
+  cup2d::FFTPlan* fft = nullptr;
+  std::string fft_err;                     // why no FFT plan exists (size limits)
+  bool shapes_frozen = false;
+};
+
+#define CUP2D_HIP_CHECK(ctx, call)                                             \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+      return CUP2D_ERR_HIP;                                                    \
+    }                                                                          \
+  } while (0)
+
+// launch helpers / entry points implemented in the .hip files
+namespace cup2d {
+int launch_check(cup2d_ctx* c, const char* what);
+
+// stencils.hip
+int enqueue_find_max_u(cup2d_ctx* c);
+int enqueue_dt(cup2d_ctx* c);
+int enqueue_advdiff(cup2d_ctx* c);
+int enqueue_pressure_rhs(cup2d_ctx* c);
+int enqueue_fade(cup2d_ctx* c);
+int enqueue_pressure_correction(cup2d_ctx* c);
+int enqueue_set_dt(cup2d_ctx* c, double dt);
+int enqueue_ic(cup2d_ctx* c);
+int enqueue_apply_obj_vel(cup2d_ctx* c);
+int enqueue_step_end(cup2d_ctx* c);
+int enqueue_blocks_to_dense(cup2d_ctx* c, const double* stage, double* a, double* b, int nc);
+int enqueue_dense_to_blocks(cup2d_ctx* c, const double* a, const double* b, double* stage, int nc);
+
+// fft.hip
+int fft_create(cup2d_ctx* c);
+void fft_destroy(cup2d_ctx* c);
+int enqueue_poisson_fft(cup2d_ctx* c, const double* rhs, double* sol);
+
+// obstacles.hip
+int obstacles_alloc(cup2d_ctx* c);
+void obstacles_free(cup2d_ctx* c);
+int enqueue_put_objects(cup2d_ctx* c);
+int enqueue_rhs_obstacles(cup2d_ctx* c);
+int enqueue_body_update(cup2d_ctx* c);
+int enqueue_penalize(cup2d_ctx* c);
+int enqueue_forces(cup2d_ctx* c);
+int enqueue_shapes_reset(cup2d_ctx* c);
+}  // namespace cup2d

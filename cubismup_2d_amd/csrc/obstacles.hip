@@ -1,0 +1,860 @@
+// obstacles.hip -- obstacle-block work of the CubismUP_2D step on gfx950.
+//
+// The reference allocates an ObstacleBlock for every (shape, block) pair whose
+// block touches the shape's bounding box (ShapesSimple.cpp:14-35).  Here those
+// pairs are "slots" in a fixed-capacity device pool, rebuilt each step by one
+// thread from the bounding box (no host round trip).  Kernels run either one
+// workgroup per slot (work private to one shape) or one workgroup per distinct
+// obstacle block (work that combines shapes on shared cells, done in shape
+// order like the reference's `for blocks: for shapes` loops).
+//
+// Sums over a shape are per-slot partials (each a fixed-order tree inside the
+// workgroup) added in slot order, i.e. blockID order as in
+// Shape::computeForces (Shape.cpp:260-271).  Every workgroup that needs a
+// shape total recomputes it from the partials in the same order, so all of
+// them see bit-identical values.
+#include "device_util.hpp"
+
+namespace cup2d {
+
+constexpr double EPSD = 2.220446049250313e-16;  // std::numeric_limits<double>::epsilon()
+constexpr int NT = 256;
+
+// partial-sum offsets inside a slot's NPART doubles
+enum { P_COM = 0, P_MOM = 3, P_RHS = 10, P_PEN = 13, P_FRC = 20, P_NPART_USED = 38 };
+static_assert(P_NPART_USED <= NPART + 16, "partials");
+constexpr int SLOT_PART = 40;
+
+struct SlotView {
+  double *chi, *dist, *rho, *udx, *udy, *sd, *sx, *sy, *part;
+};
+__device__ inline SlotView slot(const Obstacles& ob, int s) {
+  SlotView v;
+  const size_t o = (size_t)s * BS * BS;
+  v.chi = ob.chi + o; v.dist = ob.dist + o; v.rho = ob.rho + o;
+  v.udx = ob.udef + 2 * o; v.udy = ob.udef + 2 * o + BS * BS;
+  v.sd = ob.sdelta + o; v.sx = ob.sdx + o; v.sy = ob.sdy + o;
+  v.part = ob.part + (size_t)s * SLOT_PART;
+  return v;
+}
+
+// value of a dense field at global cell (gx, gy) with BlockLabOpen ghosts
+// (zero-gradient at the four domain faces, Definitions.h:125-157)
+__device__ inline double lab_at(const double* f, const Grid& g, int gx, int gy) {
+  gx = gx < 0 ? 0 : (gx >= g.NX ? g.NX - 1 : gx);
+  gy = gy < 0 ? 0 : (gy >= g.NY ? g.NY - 1 : gy);
+  return f[(size_t)gy * g.NX + gx];
+}
+
+// ------------------------------------------------------------ begin step
+// PutObjectsOnGrid.cpp:200-220: U_inf from the fixed shapes
+// (Shape::updateLabVelocity, Shape.cpp:61-65), Shape::updatePosition
+// (Shape.cpp:67-88), domain check; then the obstacle-block lists
+// (Disk::create + FillBlocks_Cylinder::is_touching, ShapeLibrary.h:24-57).
+__device__ inline bool touching(const Grid& g, int bx, int by, const double bbox[2][2]) {
+  const double mn0 = cell_pos(bx, 0, g.hb, g.h), mn1 = cell_pos(by, 0, g.hb, g.h);
+  const double mx0 = cell_pos(bx, BS - 1, g.hb, g.h), mx1 = cell_pos(by, BS - 1, g.hb, g.h);
+  const double i00 = dmax(mn0, bbox[0][0]), i01 = dmin(mx0, bbox[0][1]);
+  const double i10 = dmax(mn1, bbox[1][0]), i11 = dmin(mx1, bbox[1][1]);
+  return i01 - i00 > 0 && i11 - i10 > 0;
+}
+
+__global__ void k_obst_begin(Grid g, const ShapeParams* __restrict__ sp,
+                             cup2d_shape_state* __restrict__ ss, StepState* __restrict__ st,
+                             Obstacles ob) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double dt = st->dt;
+  const int ns = g.nshapes;
+  int nSum[2] = {0, 0};
+  double uSum[2] = {0, 0};
+  for (int k = 0; k < ns; ++k) {
+    if (sp[k].bFixedx) { nSum[0]++; uSum[0] -= ss[k].u; }
+    if (sp[k].bFixedy) { nSum[1]++; uSum[1] -= ss[k].v; }
+  }
+  if (nSum[0] > 0) st->uinf[0] = uSum[0] / nSum[0];
+  if (nSum[1] > 0) st->uinf[1] = uSum[1] / nSum[1];
+  for (int k = 0; k < ns; ++k) {
+    cup2d_shape_state& t = ss[k];
+    t.centerOfMass[0] += dt * (t.u + st->uinf[0]);
+    t.centerOfMass[1] += dt * (t.v + st->uinf[1]);
+    t.labCenterOfMass[0] += dt * t.u;
+    t.labCenterOfMass[1] += dt * t.v;
+    t.orientation += dt * t.omega;
+    t.orientation = t.orientation > M_PI ? t.orientation - 2 * M_PI : t.orientation;
+    t.orientation = t.orientation < -M_PI ? t.orientation + 2 * M_PI : t.orientation;
+    const double cosang = cos(t.orientation), sinang = sin(t.orientation);
+    t.center[0] = t.centerOfMass[0] + cosang * t.d_gm[0] - sinang * t.d_gm[1];
+    t.center[1] = t.centerOfMass[1] + sinang * t.d_gm[0] + cosang * t.d_gm[1];
+    if (t.center[0] < 0 || t.center[0] > g.extent0 || t.center[1] < 0 ||
+        t.center[1] > g.extent1)
+      st->error = CUP2D_ERR_BODY_OUT;
+  }
+  int nob = 0;
+  for (int k = 0; k < ns; ++k) {
+    ob.slot_of_shape_start[k] = nob;
+    const double safety = 2 * g.h, R = sp[k].radius;
+    const double c0 = ss[k].center[0], c1 = ss[k].center[1];
+    const double bbox[2][2] = {{c0 - R - safety, c0 + R + safety},
+                               {c1 - R - safety, c1 + R + safety}};
+    int bx0 = (int)floor(bbox[0][0] / g.hb) - 1, bx1 = (int)floor(bbox[0][1] / g.hb) + 1;
+    int by0 = (int)floor(bbox[1][0] / g.hb) - 1, by1 = (int)floor(bbox[1][1] / g.hb) + 1;
+    bx0 = bx0 < 0 ? 0 : bx0; by0 = by0 < 0 ? 0 : by0;
+    bx1 = bx1 >= g.bpdx ? g.bpdx - 1 : bx1; by1 = by1 >= g.bpdy ? g.bpdy - 1 : by1;
+    for (int by = by0; by <= by1; ++by)
+      for (int bx = bx0; bx <= bx1; ++bx) {
+        if (!touching(g, bx, by, bbox)) continue;
+        if (nob >= ob.cap) { st->error = CUP2D_ERR_CAPACITY; continue; }
+        ob.slot_shape[nob] = k;
+        ob.slot_block[nob] = bx + g.bpdx * by;
+        ++nob;
+      }
+  }
+  ob.slot_of_shape_start[ns] = nob;
+  // distinct blocks, in first-appearance order; slots per block in shape order
+  int nub = 0;
+  for (int s = 0; s < nob; ++s) {
+    const int b = ob.slot_block[s];
+    bool found = false;
+    for (int u = 0; u < nub; ++u)
+      if (ob.ub_block[u] == b) { found = true; break; }
+    if (!found) ob.ub_block[nub++] = b;
+  }
+  int pos = 0;
+  for (int u = 0; u < nub; ++u) {
+    ob.ub_start[u] = pos;
+    int cnt = 0;
+    for (int s = 0; s < nob; ++s)
+      if (ob.slot_block[s] == ob.ub_block[u]) { ob.ub_slots[pos++] = s; ++cnt; }
+    ob.ub_count[u] = cnt;
+  }
+  st->nob = nob;
+  st->nub = nub;
+}
+
+// PutObjectsOnGrid.cpp:191-197 restricted to the blocks written last step
+// (every other cell already holds the cleared values).
+__global__ __launch_bounds__(NT) void k_obst_clear(Grid g, const StepState* __restrict__ st,
+                                                   Obstacles ob, double* sdf, double* chi,
+                                                   double* udx, double* udy, double* invrho) {
+  if ((int)blockIdx.x >= st->nub) return;
+  const int b = ob.ub_block[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  for (int k = threadIdx.x; k < BS * BS; k += NT) {
+    const size_t d = (size_t)(by * BS + k / BS) * g.NX + bx * BS + k % BS;
+    chi[d] = 0; sdf[d] = -1; udx[d] = 0; udy[d] = 0; invrho[d] = 1;
+  }
+}
+
+// Disk::create / FillBlocks_Cylinder::operator() (ShapeLibrary.cpp:21-34):
+// dist = r - |x - c|, tmp = max(tmp, dist), rho = rhoS; per distinct block,
+// shapes in order (Disk::create runs shape by shape, PutObjectsOnGrid.cpp:224).
+__global__ __launch_bounds__(NT) void k_obst_create(Grid g, const ShapeParams* __restrict__ sp,
+                                                    const cup2d_shape_state* __restrict__ ss,
+                                                    const StepState* __restrict__ st,
+                                                    Obstacles ob, double* __restrict__ sdf) {
+  if ((int)blockIdx.x >= st->nub) return;
+  const int b = ob.ub_block[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  for (int q = 0; q < cnt; ++q) {
+    const int s = ob.ub_slots[s0 + q];
+    const int k = ob.slot_shape[s];
+    SlotView o = slot(ob, s);
+    const double R = sp[k].radius, rhoS = sp[k].rhoS;
+    const double c0 = ss[k].center[0], c1 = ss[k].center[1];
+    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+      const int ix = i % BS, iy = i / BS;
+      const double p0 = cell_pos(bx, ix, g.hb, g.h) - c0, p1 = cell_pos(by, iy, g.hb, g.h) - c1;
+      const double d = R - sqrt(p0 * p0 + p1 * p1);
+      o.dist[i] = d;
+      o.rho[i] = rhoS;
+      o.chi[i] = 0; o.udx[i] = 0; o.udy[i] = 0;
+      o.sd[i] = 0; o.sx[i] = 0; o.sy[i] = 0;
+      double* t = sdf + (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+      *t = dmax(*t, d);
+    }
+  }
+}
+
+// PutObjectsOnGrid::putChiOnGrid (PutObjectsOnGrid.cpp:17-116): Towers chi,
+// surface delta + normals (ObstacleBlock::write, ObstacleBlock.h:336-346),
+// centre-of-mass partial sums.  One workgroup per slot.
+__global__ __launch_bounds__(NT) void k_obst_chi(Grid g, const cup2d_shape_state* __restrict__ ss,
+                                                 const StepState* __restrict__ st, Obstacles ob,
+                                                 const double* __restrict__ sdf) {
+  __shared__ double lds[8];
+  if ((int)blockIdx.x >= st->nob) return;
+  const int s = blockIdx.x;
+  const int k = ob.slot_shape[s], b = ob.slot_block[s];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  SlotView o = slot(ob, s);
+  const double h = g.h, i2h = 0.5 / h, fac = 0.5 * h;
+  const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
+  double _x = 0, _y = 0, _m = 0;
+  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    const int ix = i % BS, iy = i / BS;
+    const int gx = bx * BS + ix, gy = by * BS + iy;
+    const double sd = o.dist[i];
+    const double distPx = ix + 1 == BS ? lab_at(sdf, g, gx + 1, gy) : o.dist[i + 1];
+    const double distMx = ix == 0 ? lab_at(sdf, g, gx - 1, gy) : o.dist[i - 1];
+    const double distPy = iy + 1 == BS ? lab_at(sdf, g, gx, gy + 1) : o.dist[i + BS];
+    const double distMy = iy == 0 ? lab_at(sdf, g, gx, gy - 1) : o.dist[i - BS];
+    double X;
+    if (sd > +2 * h || sd < -2 * h) {
+      X = sd > 0 ? 1 : 0;
+    } else {
+      const double IplusX = distPx < 0 ? 0 : distPx, IminuX = distMx < 0 ? 0 : distMx;
+      const double IplusY = distPy < 0 ? 0 : distPy, IminuY = distMy < 0 ? 0 : distMy;
+      const double gradIX = i2h * (IplusX - IminuX), gradIY = i2h * (IplusY - IminuY);
+      const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
+      const double gradUSq = gradUX * gradUX + gradUY * gradUY;
+      const double denum = 1 / (gradUSq < EPSD ? EPSD : gradUSq);
+      X = (gradIX * gradUX + gradIY * gradUY) * denum;
+    }
+    o.chi[i] = X;
+    if (X > 0) {
+      const double p0 = cell_pos(bx, ix, g.hb, h), p1 = cell_pos(by, iy, g.hb, h);
+      const double rho = o.rho[i];
+      _x += rho * X * h * h * (p0 - C0);
+      _y += rho * X * h * h * (p1 - C1);
+      _m += rho * X * h * h;
+    }
+    if (sd > +2 * h || sd < -2 * h) continue;
+    const double HplusX = fabs(distPx) < EPSD ? 0.5 : (distPx < 0 ? 0 : 1);
+    const double HminuX = fabs(distMx) < EPSD ? 0.5 : (distMx < 0 ? 0 : 1);
+    const double HplusY = fabs(distPy) < EPSD ? 0.5 : (distPy < 0 ? 0 : 1);
+    const double HminuY = fabs(distMy) < EPSD ? 0.5 : (distMy < 0 ? 0 : 1);
+    const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
+    const double gradHX = (HplusX - HminuX), gradHY = (HplusY - HminuY);
+    const double gradUSq = gradUX * gradUX + gradUY * gradUY;
+    const double denum = 1 / (gradUSq < EPSD ? EPSD : gradUSq);
+    const double D = fac * (gradHX * gradUX + gradHY * gradUY) * denum;
+    if (D > 0) {
+      o.sd[i] = D;
+      o.sx[i] = -D * gradUX;
+      o.sy[i] = -D * gradUY;
+    }
+  }
+  _x = block_sum<NT>(_x, lds);
+  _y = block_sum<NT>(_y, lds);
+  _m = block_sum<NT>(_m, lds);
+  if (threadIdx.x == 0) {
+    o.part[P_COM] = _x; o.part[P_COM + 1] = _y; o.part[P_COM + 2] = _m;
+  }
+}
+
+// PutObjectsOnGrid.cpp:69-73 (grid CHI = max, invRho blend; per distinct
+// block, shapes in order) and :118-122 (centre-of-mass shift and mass).
+__global__ __launch_bounds__(NT) void k_obst_chigrid(Grid g, cup2d_shape_state* __restrict__ ss,
+                                                     const StepState* __restrict__ st,
+                                                     Obstacles ob, double* __restrict__ chi,
+                                                     double* __restrict__ invrho) {
+  if ((int)blockIdx.x >= st->nub) return;
+  const int b = ob.ub_block[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  for (int q = 0; q < cnt; ++q) {
+    SlotView o = slot(ob, ob.ub_slots[s0 + q]);
+    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+      const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+      const double X = o.chi[i];
+      if (X >= chi[d]) {
+        chi[d] = X;
+        invrho[d] = X / o.rho[i] + (1 - X) * invrho[d];
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < g.nshapes; ++k) {
+      double _x = 0, _y = 0, _m = 0;
+      for (int s = ob.slot_of_shape_start[k]; s < ob.slot_of_shape_start[k + 1]; ++s) {
+        const double* p = ob.part + (size_t)s * SLOT_PART;
+        _x += p[P_COM]; _y += p[P_COM + 1]; _m += p[P_COM + 2];
+      }
+      if (_m > EPSD) {
+        ss[k].centerOfMass[0] += _x / _m;
+        ss[k].centerOfMass[1] += _y / _m;
+        ss[k].M = _m;
+      }
+    }
+  }
+}
+
+// Shape::integrateObstBlock (Shape.cpp:117-155): per-slot partials.
+__global__ __launch_bounds__(NT) void k_obst_moments(Grid g,
+                                                     const cup2d_shape_state* __restrict__ ss,
+                                                     const StepState* __restrict__ st,
+                                                     Obstacles ob) {
+  __shared__ double lds[8];
+  if ((int)blockIdx.x >= st->nob) return;
+  const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  SlotView o = slot(ob, s);
+  const double hsq = g.h * g.h;
+  const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    if (o.chi[i] <= 0) continue;
+    const int ix = i % BS, iy = i / BS;
+    const double rhochi = o.chi[i] * o.rho[i] * hsq;
+    const double p0 = cell_pos(bx, ix, g.hb, g.h) - C0, p1 = cell_pos(by, iy, g.hb, g.h) - C1;
+    a[0] += rhochi * p0;
+    a[1] += rhochi * p1;
+    a[2] += rhochi;
+    a[3] += rhochi * (p0 * p0 + p1 * p1);
+    a[4] += rhochi * o.udx[i];
+    a[5] += rhochi * o.udy[i];
+    a[6] += rhochi * (p0 * o.udy[i] - p1 * o.udx[i]);
+  }
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const double t = block_sum<NT>(a[j], lds);
+    if (threadIdx.x == 0) o.part[P_MOM + j] = t;
+  }
+}
+
+// Shape::removeMoments (Shape.cpp:157-212) + putObjectVelOnGrid
+// (PutObjectsOnGrid.cpp:149-184), per distinct block, shapes in order.
+__global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __restrict__ ss,
+                                                  const StepState* __restrict__ st, Obstacles ob,
+                                                  const double* __restrict__ chi,
+                                                  double* __restrict__ udx,
+                                                  double* __restrict__ udy) {
+  if ((int)blockIdx.x >= st->nub) return;
+  const int b = ob.ub_block[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  for (int q = 0; q < cnt; ++q) {
+    const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
+    double I[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int t = ob.slot_of_shape_start[k]; t < ob.slot_of_shape_start[k + 1]; ++t)
+      for (int j = 0; j < 7; ++j) I[j] += ob.part[(size_t)t * SLOT_PART + P_MOM + j];
+    const double Iu = I[4] / I[2], Iv = I[5] / I[2], Ia = I[6] / I[3];
+    const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
+    SlotView o = slot(ob, s);
+    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+      const int ix = i % BS, iy = i / BS;
+      const double p0 = cell_pos(bx, ix, g.hb, g.h) - C0, p1 = cell_pos(by, iy, g.hb, g.h) - C1;
+      o.udx[i] -= Iu - Ia * p1;
+      o.udy[i] -= Iv + Ia * p0;
+      const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+      if (o.chi[i] < chi[d] || o.chi[i] <= 0) continue;
+      udx[d] += o.udx[i];
+      udy[d] += o.udy[i];
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < g.nshapes; ++k) {
+      double I[7] = {0, 0, 0, 0, 0, 0, 0};
+      for (int t = ob.slot_of_shape_start[k]; t < ob.slot_of_shape_start[k + 1]; ++t)
+        for (int j = 0; j < 7; ++j) I[j] += ob.part[(size_t)t * SLOT_PART + P_MOM + j];
+      cup2d_shape_state& T = ss[k];
+      T.M = I[2];
+      T.J = I[3];
+      const double dCx = T.center[0] - T.centerOfMass[0], dCy = T.center[1] - T.centerOfMass[1];
+      T.d_gm[0] = dCx * cos(T.orientation) + dCy * sin(T.orientation);
+      T.d_gm[1] = -dCx * sin(T.orientation) + dCy * cos(T.orientation);
+    }
+  }
+}
+
+static int grid_cap(cup2d_ctx* c) { return c->ob.cap > 0 ? c->ob.cap : 1; }
+
+int enqueue_put_objects(cup2d_ctx* c) {
+  const int G = grid_cap(c);
+  hipLaunchKernelGGL(k_obst_clear, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->sdf,
+                     c->chi, c->udx, c->udy, c->invrho);
+  hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(64), 0, c->stream, c->g, c->sp, c->ss, c->st,
+                     c->ob);
+  if (c->g.nshapes == 0) return launch_check(c, "put_objects");
+  hipLaunchKernelGGL(k_obst_create, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
+                     c->ob, c->sdf);
+  hipLaunchKernelGGL(k_obst_chi, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob,
+                     c->sdf);
+  hipLaunchKernelGGL(k_obst_chigrid, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob,
+                     c->chi, c->invrho);
+  hipLaunchKernelGGL(k_obst_moments, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob);
+  hipLaunchKernelGGL(k_obst_udef, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob,
+                     c->chi, c->udx, c->udy);
+  return launch_check(c, "put_objects");
+}
+
+// ------------------------------------------------ RHS obstacle correction
+// PressureSingle.cpp:130-168 (source term + perimeter sums) and :173-212
+// (sign-selective rescale), per distinct block, shapes in order.
+__global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __restrict__ st,
+                                                  Obstacles ob, const double* __restrict__ chi,
+                                                  double* __restrict__ r) {
+  __shared__ double lds[8];
+  if ((int)blockIdx.x >= st->nub) return;
+  const int b = ob.ub_block[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  for (int q = 0; q < cnt; ++q) {
+    const int s = ob.ub_slots[s0 + q];
+    SlotView o = slot(ob, s);
+    double sum = 0, pos = 0, neg = 0;
+    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+      const int ix = i % BS, iy = i / BS, gx = bx * BS + ix, gy = by * BS + iy;
+      const size_t d = (size_t)gy * g.NX + gx;
+      if (chi[d] > o.chi[i]) continue;
+      const double gradXx = lab_at(chi, g, gx + 1, gy) - lab_at(chi, g, gx - 1, gy);
+      const double gradXy = lab_at(chi, g, gx, gy + 1) - lab_at(chi, g, gx, gy - 1);
+      const double srcBulk = -o.chi[i] * r[d];
+      sum += srcBulk;
+      r[d] += srcBulk;
+      const double isPerim = (gradXx * gradXx + gradXy * gradXy > 0) ? 1.0 : 0.0;
+      const double T = r[d];
+      pos += isPerim * T * (double)(T > 0);
+      neg -= isPerim * T * (double)(T < 0);
+    }
+    sum = block_sum<NT>(sum, lds);
+    pos = block_sum<NT>(pos, lds);
+    neg = block_sum<NT>(neg, lds);
+    if (threadIdx.x == 0) {
+      o.part[P_RHS] = sum; o.part[P_RHS + 1] = pos; o.part[P_RHS + 2] = neg;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __restrict__ st,
+                                                  Obstacles ob, const double* __restrict__ chi,
+                                                  double* __restrict__ r) {
+  if ((int)blockIdx.x >= st->nub) return;
+  const int b = ob.ub_block[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  for (int q = 0; q < cnt; ++q) {
+    const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
+    double sum = 0, pos = 0, neg = 0;
+    for (int t = ob.slot_of_shape_start[k]; t < ob.slot_of_shape_start[k + 1]; ++t) {
+      const double* p = ob.part + (size_t)t * SLOT_PART + P_RHS;
+      sum += p[0]; pos += p[1]; neg += p[2];
+    }
+    const double corrDenom = sum > 0 ? pos : neg;
+    const double corr = sum / dmax(corrDenom, EPSD);
+    SlotView o = slot(ob, s);
+    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+      const int ix = i % BS, iy = i / BS, gx = bx * BS + ix, gy = by * BS + iy;
+      const size_t d = (size_t)gy * g.NX + gx;
+      if (chi[d] > o.chi[i]) continue;
+      const double gradXx = lab_at(chi, g, gx + 1, gy) - lab_at(chi, g, gx - 1, gy);
+      const double gradXy = lab_at(chi, g, gx, gy + 1) - lab_at(chi, g, gx, gy - 1);
+      const bool isPerim = gradXx * gradXx + gradXy * gradXy > 0;
+      const double T = r[d];
+      if (isPerim && T > 0 && corr > 0) r[d] = T - corr * T;
+      else if (isPerim && T < 0 && corr < 0) r[d] = T + corr * T;
+    }
+  }
+}
+
+int enqueue_rhs_obstacles(cup2d_ctx* c) {
+  if (c->g.nshapes == 0) return CUP2D_OK;
+  const int G = grid_cap(c);
+  hipLaunchKernelGGL(k_rhs_obst1, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
+                     c->rhs);
+  hipLaunchKernelGGL(k_rhs_obst2, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
+                     c->rhs);
+  return launch_check(c, "rhs_obstacles");
+}
+
+// ------------------------------------------------------------ body update
+// PressureSingle::integrateMomenta (PressureSingle.cpp:23-67), per slot.
+__global__ __launch_bounds__(NT) void k_momenta(Grid g, const cup2d_shape_state* __restrict__ ss,
+                                                const StepState* __restrict__ st, Obstacles ob,
+                                                const double* __restrict__ u,
+                                                const double* __restrict__ v) {
+  __shared__ double lds[8];
+  if ((int)blockIdx.x >= st->nob) return;
+  const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  SlotView o = slot(ob, s);
+  const double hsq = g.h * g.h;
+  const double lambdt = g.lambda * st->dt;
+  const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    if (o.chi[i] <= 0) continue;
+    const int ix = i % BS, iy = i / BS;
+    const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+    const double ud0 = u[d] - o.udx[i], ud1 = v[d] - o.udy[i];
+    double F;
+    if (!g.implicit_penal) {
+      F = hsq * o.rho[i] * o.chi[i];
+    } else {
+      const double Xlamdt = o.chi[i] * lambdt;
+      F = hsq * o.rho[i] * Xlamdt / (1 + Xlamdt);
+    }
+    const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
+    a[0] += F;
+    a[1] += F * (p0 * p0 + p1 * p1);
+    a[2] += F * p0;
+    a[3] += F * p1;
+    a[4] += F * ud0;
+    a[5] += F * ud1;
+    a[6] += F * (p0 * ud1 - p1 * ud0);
+  }
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const double t = block_sum<NT>(a[j], lds);
+    if (threadIdx.x == 0) o.part[P_PEN + j] = t;
+  }
+}
+
+// 3x3 LU with partial pivoting (restates gsl_linalg_LU_decomp/LU_solve as
+// called in Shape::updateVelocity, Shape.cpp:48-54).
+__device__ void lu3_solve(double A[3][3], const double b[3], double x[3]) {
+  int perm[3] = {0, 1, 2};
+  for (int j = 0; j < 2; ++j) {
+    double amax = fabs(A[j][j]);
+    int ip = j;
+    for (int i = j + 1; i < 3; ++i)
+      if (fabs(A[i][j]) > amax) { amax = fabs(A[i][j]); ip = i; }
+    if (ip != j) {
+      for (int k = 0; k < 3; ++k) { const double t = A[j][k]; A[j][k] = A[ip][k]; A[ip][k] = t; }
+      const int t = perm[j]; perm[j] = perm[ip]; perm[ip] = t;
+    }
+    const double ajj = A[j][j];
+    if (ajj != 0.0)
+      for (int i = j + 1; i < 3; ++i) {
+        const double aij = A[i][j] / ajj;
+        A[i][j] = aij;
+        for (int k = j + 1; k < 3; ++k) A[i][k] -= aij * A[j][k];
+      }
+  }
+  double y[3];
+  for (int i = 0; i < 3; ++i) {
+    double v = b[perm[i]];
+    for (int k = 0; k < i; ++k) v -= A[i][k] * y[k];
+    y[i] = v;
+  }
+  for (int i = 2; i >= 0; --i) {
+    double v = y[i];
+    for (int k = i + 1; k < 3; ++k) v -= A[i][k] * x[k];
+    x[i] = v / A[i][i];
+  }
+}
+
+// Shape::updateVelocity (Shape.cpp:25-59), Disk::updateVelocity
+// (ShapesSimple.cpp:83-87), preventCollidingObstacles (PressureSingle.cpp:243-344).
+__global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __restrict__ sp,
+                                                   cup2d_shape_state* __restrict__ ss,
+                                                   const StepState* __restrict__ st,
+                                                   Obstacles ob) {
+  __shared__ double lds[8];
+  __shared__ double coll[MAX_SHAPES][10];
+  const double dt = st->dt, time = st->time;
+  const int ns = g.nshapes;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < ns; ++k) {
+      double S[7] = {0, 0, 0, 0, 0, 0, 0};
+      for (int t = ob.slot_of_shape_start[k]; t < ob.slot_of_shape_start[k + 1]; ++t)
+        for (int j = 0; j < 7; ++j) S[j] += ob.part[(size_t)t * SLOT_PART + P_PEN + j];
+      cup2d_shape_state& T = ss[k];
+      const ShapeParams& P = sp[k];
+      T.penalM = S[0]; T.penalJ = S[1]; T.penalDX = S[2]; T.penalDY = S[3];
+      T.fluidMomX = S[4]; T.fluidMomY = S[5]; T.fluidAngMom = S[6];
+      double A[3][3] = {{T.penalM, 0, -T.penalDY}, {0, T.penalM, T.penalDX},
+                        {-T.penalDY, T.penalDX, T.penalJ}};
+      double bb[3] = {T.fluidMomX + dt * P.appliedForceX, T.fluidMomY + dt * P.appliedForceY,
+                      T.fluidAngMom + dt * P.appliedTorque};
+      const double tf = P.timeForced;
+      if (P.bForcedx && time < tf) { A[0][1] = 0; A[0][2] = 0; bb[0] = T.penalM * P.forcedu; }
+      if (P.bForcedy && time < tf) { A[1][0] = 0; A[1][2] = 0; bb[1] = T.penalM * P.forcedv; }
+      if (P.bBlockAng && time < tf) { A[2][0] = 0; A[2][1] = 0; bb[2] = T.penalJ * P.forcedomega; }
+      double x[3];
+      lu3_solve(A, bb, x);
+      if (!P.bForcedy || time > tf) T.u = x[0];  // Shape.cpp:56 tests bForcedy for u
+      if (!P.bForcedy || time > tf) T.v = x[1];
+      if (!P.bBlockAng || time > tf) T.omega = x[2];
+      if (P.tAccel > 0) {
+        if (P.bForcedx && time < P.tAccel) T.u = (time / P.tAccel) * P.forcedu;
+        if (P.bForcedy && time < P.tAccel) T.v = (time / P.tAccel) * P.forcedv;
+      }
+    }
+  }
+  if (ns < 2) return;
+  __syncthreads();
+  // collision sums: coll[i] accumulates over every j != i (PressureSingle.cpp:255-298)
+  for (int i = 0; i < ns; ++i) {
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < ns; ++j) {
+      if (i == j) continue;
+      const cup2d_shape_state &si = ss[i], &sj = ss[j];
+      for (int a = ob.slot_of_shape_start[i]; a < ob.slot_of_shape_start[i + 1]; ++a)
+        for (int bsl = ob.slot_of_shape_start[j]; bsl < ob.slot_of_shape_start[j + 1]; ++bsl) {
+          if (ob.slot_block[a] != ob.slot_block[bsl]) continue;
+          const int blk = ob.slot_block[a], bx = blk % g.bpdx, by = blk / g.bpdx;
+          SlotView oi = slot(ob, a), oj = slot(ob, bsl);
+          for (int c = threadIdx.x; c < BS * BS; c += NT) {
+            if (oi.chi[c] <= 0 || oj.chi[c] <= 0) continue;
+            const double p0 = cell_pos(bx, c % BS, g.hb, g.h), p1 = cell_pos(by, c / BS, g.hb, g.h);
+            const double iUr = -si.omega * (p1 - si.centerOfMass[1]), iVr = si.omega * (p0 - si.centerOfMass[0]);
+            const double jUr = -sj.omega * (p1 - sj.centerOfMass[1]), jVr = sj.omega * (p0 - sj.centerOfMass[0]);
+            const double X = oi.chi[c], Y = oj.chi[c];
+            acc[0] += X; acc[1] += X * p0; acc[2] += X * p1;
+            acc[3] += X * (si.u + iUr + oi.udx[c]);
+            acc[4] += X * (si.v + iVr + oi.udy[c]);
+            acc[5] += Y; acc[6] += Y * p0; acc[7] += Y * p1;
+            acc[8] += Y * (sj.u + jUr + oj.udx[c]);
+            acc[9] += Y * (sj.v + jVr + oj.udy[c]);
+          }
+        }
+    }
+    for (int q = 0; q < 10; ++q) {
+      const double t = block_sum<NT>(acc[q], lds);
+      if (threadIdx.x == 0) coll[i][q] = t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < ns; ++i)
+    for (int j = 0; j < ns; ++j) {
+      if (i == j) continue;
+      const double* c = coll[i];
+      if (c[0] < 1 || c[5] < 1) continue;
+      const double iPX = c[1] / c[0], iPY = c[2] / c[0], iUX = c[3] / c[0], iUY = c[4] / c[0];
+      const double jPX = c[6] / c[5], jPY = c[7] / c[5], jUX = c[8] / c[5], jUY = c[9] / c[5];
+      const double CX = (iPX + jPX) / 2, CY = (iPY + jPY) / 2;
+      const double dirX = iPX - jPX, dirY = iPY - jPY;
+      const double hitVelX = jUX - iUX, hitVelY = jUY - iUY;
+      const double normF = dmax(sqrt(dirX * dirX + dirY * dirY), EPSD);
+      const double NXn = dirX / normF, NYn = dirY / normF;
+      const double projVel = hitVelX * NXn + hitVelY * NYn;
+      if (projVel <= 0) continue;
+      const ShapeParams &Pi = sp[i], &Pj = sp[j];
+      const bool iForcedX = Pi.bForcedx && time < Pi.timeForced;
+      const bool iForcedY = Pi.bForcedy && time < Pi.timeForced;
+      const bool iForcedA = Pi.bBlockAng && time < Pi.timeForced;
+      const bool jForcedX = Pj.bForcedx && time < Pj.timeForced;
+      const bool jForcedY = Pj.bForcedy && time < Pj.timeForced;
+      const double iInvMassX = iForcedX ? 0 : 1 / ss[i].M, iInvMassY = iForcedY ? 0 : 1 / ss[i].M;
+      const double jInvMassX = jForcedX ? 0 : 1 / ss[j].M, jInvMassY = jForcedY ? 0 : 1 / ss[j].M;
+      const double iInvJ = iForcedA ? 0 : 1 / ss[i].J;
+      const double meanMassX = 2 / dmax(iInvMassX + jInvMassX, EPSD);
+      const double meanMassY = 2 / dmax(iInvMassY + jInvMassY, EPSD);
+      const double FXdt = meanMassX * projVel * NXn, FYdt = meanMassY * projVel * NYn;
+      ss[i].u += FXdt * iInvMassX;
+      ss[i].v += FYdt * iInvMassY;
+      const double iCx = ss[i].centerOfMass[0], iCy = ss[i].centerOfMass[1];
+      const double RcrossF = (CX - iCx) * FYdt - (CY - iCy) * FXdt;
+      ss[i].omega += iInvJ * RcrossF;
+    }
+}
+
+int enqueue_body_update(cup2d_ctx* c) {
+  if (c->g.nshapes == 0) return CUP2D_OK;
+  const int G = grid_cap(c);
+  hipLaunchKernelGGL(k_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob, c->u,
+                     c->v);
+  hipLaunchKernelGGL(k_body_solve, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
+                     c->ob);
+  return launch_check(c, "body_update");
+}
+
+// PressureSingle::penalize (PressureSingle.cpp:69-108), per distinct block,
+// shapes in order.
+__global__ __launch_bounds__(NT) void k_penalize(Grid g, const cup2d_shape_state* __restrict__ ss,
+                                                 const StepState* __restrict__ st, Obstacles ob,
+                                                 const double* __restrict__ chi,
+                                                 double* __restrict__ u, double* __restrict__ v) {
+  if ((int)blockIdx.x >= st->nub) return;
+  const int b = ob.ub_block[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  const double dt = st->dt;
+  for (int q = 0; q < cnt; ++q) {
+    const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
+    SlotView o = slot(ob, s);
+    const double u_s = ss[k].u, v_s = ss[k].v, omega_s = ss[k].omega;
+    const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
+    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+      const int ix = i % BS, iy = i / BS;
+      const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+      const double X = o.chi[i];
+      if (chi[d] > X) continue;
+      if (X <= 0) continue;
+      const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
+      const double alpha = g.implicit_penal ? 1 / (1 + g.lambda * dt * X) : 1 - X;
+      const double US = u_s - omega_s * p1 + o.udx[i];
+      const double VS = v_s + omega_s * p0 + o.udy[i];
+      u[d] = alpha * u[d] + (1 - alpha) * US;
+      v[d] = alpha * v[d] + (1 - alpha) * VS;
+    }
+  }
+}
+
+int enqueue_penalize(cup2d_ctx* c) {
+  if (c->g.nshapes == 0) return CUP2D_OK;
+  hipLaunchKernelGGL(k_penalize, dim3(grid_cap(c)), dim3(NT), 0, c->stream, c->g, c->ss, c->st,
+                     c->ob, c->chi, c->u, c->v);
+  return launch_check(c, "penalize");
+}
+
+// ----------------------------------------------------------- ComputeForces
+// ComputeForces.cpp:16-107, per slot; surface points are the cells with D>0.
+__global__ __launch_bounds__(NT) void k_forces(Grid g, const cup2d_shape_state* __restrict__ ss,
+                                               const StepState* __restrict__ st, Obstacles ob,
+                                               const double* __restrict__ u,
+                                               const double* __restrict__ v,
+                                               const double* __restrict__ p) {
+  __shared__ double lds[8];
+  if ((int)blockIdx.x >= st->nob) return;
+  const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  SlotView o = slot(ob, s);
+  const double NUoH = g.nu / g.h;
+  const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
+  const double su = ss[k].u, sv = ss[k].v;
+  const double vel_norm = sqrt(su * su + sv * sv);
+  const double vu0 = vel_norm > 0 ? su / vel_norm : 0, vu1 = vel_norm > 0 ? sv / vel_norm : 0;
+  double a[18];
+#pragma unroll
+  for (int j = 0; j < 18; ++j) a[j] = 0;
+  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    if (!(o.sd[i] > 0)) continue;
+    const int ix = i % BS, iy = i / BS, gx = bx * BS + ix, gy = by * BS + iy;
+    const double p0 = cell_pos(bx, ix, g.hb, g.h), p1 = cell_pos(by, iy, g.hb, g.h);
+    const double D11 = NUoH * (lab_at(u, g, gx + 1, gy) - lab_at(u, g, gx - 1, gy));
+    const double D22 = NUoH * (lab_at(v, g, gx, gy + 1) - lab_at(v, g, gx, gy - 1));
+    const double D12 = NUoH / 2 *
+                       (lab_at(u, g, gx, gy + 1) - lab_at(u, g, gx, gy - 1) +
+                        lab_at(v, g, gx + 1, gy) - lab_at(v, g, gx - 1, gy));
+    const double normX = o.sx[i], normY = o.sy[i];
+    const size_t d = (size_t)gy * g.NX + gx;
+    const double Pc = p[d];
+    const double fXV = D11 * normX + D12 * normY, fXP = -Pc * normX;
+    const double fYV = D12 * normX + D22 * normY, fYP = -Pc * normY;
+    const double fXT = fXV + fXP, fYT = fYV + fYP;
+    const double vx = u[d], vy = v[d], vxDef = o.udx[i], vyDef = o.udy[i];
+    a[0] += sqrt(normX * normX + normY * normY);
+    a[1] += normX * vy - normY * vx;
+    a[2] += fXT; a[3] += fYT;
+    a[4] += fXV; a[5] += fYV;
+    a[6] += fXP; a[7] += fYP;
+    a[8] += (p0 - Cx) * fYT - (p1 - Cy) * fXT;
+    a[9] += (p0 - Cx) * fYP - (p1 - Cy) * fXP;
+    a[10] += (p0 - Cx) * fYV - (p1 - Cy) * fXV;
+    const double forcePar = fXT * vu0 + fYT * vu1;
+    a[11] += .5 * (forcePar + fabs(forcePar));
+    a[12] -= .5 * (forcePar - fabs(forcePar));
+    const double powOut = fXT * vx + fYT * vy;
+    const double powDef = fXT * vxDef + fYT * vyDef;
+    a[13] += powOut;
+    a[14] += powDef;
+    a[15] += dmin(0.0, powOut);
+    a[16] += dmin(0.0, powDef);
+    a[17] += 1;
+  }
+#pragma unroll
+  for (int j = 0; j < 18; ++j) {
+    const double t = block_sum<NT>(a[j], lds);
+    if (threadIdx.x == 0) o.part[P_FRC + j] = t;
+  }
+}
+
+// Shape::computeForces (Shape.cpp:252-281): block partials in block order.
+__global__ void k_forces_fin(Grid g, cup2d_shape_state* __restrict__ ss, Obstacles ob) {
+  const int k = threadIdx.x;
+  if (k >= g.nshapes) return;
+  double a[18];
+  for (int j = 0; j < 18; ++j) a[j] = 0;
+  const int s0 = ob.slot_of_shape_start[k], s1 = ob.slot_of_shape_start[k + 1];
+  for (int s = s0; s < s1; ++s)
+    for (int j = 0; j < 18; ++j) a[j] += ob.part[(size_t)s * SLOT_PART + P_FRC + j];
+  cup2d_shape_state& T = ss[k];
+  T.perimeter = a[0]; T.circulation = a[1];
+  T.forcex = a[2]; T.forcey = a[3]; T.forcex_V = a[4]; T.forcey_V = a[5];
+  T.forcex_P = a[6]; T.forcey_P = a[7];
+  T.torque = a[8]; T.torque_P = a[9]; T.torque_V = a[10];
+  T.thrust = a[11]; T.drag = a[12];
+  T.Pout = a[13]; T.defPower = a[14]; T.PoutBnd = a[15]; T.defPowerBnd = a[16];
+  T.n_surface_points = a[17];
+  T.n_obstacle_blocks = s1 - s0;
+  T.Pthrust = T.thrust * sqrt(T.u * T.u + T.v * T.v);
+  T.Pdrag = T.drag * sqrt(T.u * T.u + T.v * T.v);
+  const double denUnb = T.Pthrust - dmin(T.defPower, 0.0);
+  const double demBnd = T.Pthrust - T.defPowerBnd;
+  T.EffPDef = T.Pthrust / dmax(denUnb, EPSD);
+  T.EffPDefBnd = T.Pthrust / dmax(demBnd, EPSD);
+}
+
+int enqueue_forces(cup2d_ctx* c) {
+  if (c->g.nshapes == 0) return CUP2D_OK;
+  hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, c->stream, c->g, c->ss, c->st,
+                     c->ob, c->u, c->v, c->pres);
+  hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(MAX_SHAPES), 0, c->stream, c->g, c->ss, c->ob);
+  return launch_check(c, "forces");
+}
+
+// Shape::resetAll (Shape.h:64-88).
+__global__ void k_shapes_reset(Grid g, const ShapeParams* __restrict__ sp,
+                               cup2d_shape_state* __restrict__ ss) {
+  const int k = threadIdx.x;
+  if (k >= g.nshapes) return;
+  cup2d_shape_state T;
+  double* z = (double*)&T;
+  for (size_t i = 0; i < sizeof(T) / sizeof(double); ++i) z[i] = 0;
+  T.center[0] = sp[k].origC[0]; T.center[1] = sp[k].origC[1];
+  T.centerOfMass[0] = sp[k].origC[0]; T.centerOfMass[1] = sp[k].origC[1];
+  T.orientation = sp[k].origAng;
+  T.u = sp[k].forcedu; T.v = sp[k].forcedv; T.omega = sp[k].forcedomega;
+  ss[k] = T;
+}
+
+int enqueue_shapes_reset(cup2d_ctx* c) {
+  if (c->g.nshapes == 0) return CUP2D_OK;
+  hipLaunchKernelGGL(k_shapes_reset, dim3(1), dim3(MAX_SHAPES), 0, c->stream, c->g, c->sp, c->ss);
+  return launch_check(c, "shapes_reset");
+}
+
+// ------------------------------------------------------------ allocation
+int obstacles_alloc(cup2d_ctx* c) {
+  Obstacles& o = c->ob;
+  // capacity: blocks a disk's bounding box can touch, per shape
+  int cap = 0;
+  for (const ShapeParams& p : c->hsp) {
+    const double w = 2 * p.radius + 4 * c->g.h;
+    int n1 = (int)ceil(w / c->g.hb) + 2;
+    cap += n1 * n1;
+  }
+  const int nblocks = c->g.bpdx * c->g.bpdy;
+  cap = cap > 0 ? cap : 1;
+  cap = cap > nblocks * (int)(c->hsp.size() ? c->hsp.size() : 1)
+            ? nblocks * (int)(c->hsp.size() ? c->hsp.size() : 1)
+            : cap;
+  o.cap = cap;
+  const size_t cells = (size_t)cap * BS * BS;
+  int* ibuf = nullptr;
+  CUP2D_HIP_CHECK(c, hipMalloc(&ibuf, sizeof(int) * ((size_t)7 * cap + MAX_SHAPES + 1)));
+  CUP2D_HIP_CHECK(c, hipMemset(ibuf, 0, sizeof(int) * ((size_t)7 * cap + MAX_SHAPES + 1)));
+  o.slot_shape = ibuf;
+  o.slot_block = ibuf + cap;
+  o.ub_block = ibuf + 2 * cap;
+  o.ub_start = ibuf + 3 * cap;
+  o.ub_count = ibuf + 4 * cap;
+  o.ub_slots = ibuf + 5 * cap;
+  o.prev_blocks = ibuf + 6 * cap;
+  o.slot_of_shape_start = ibuf + 7 * cap;
+  double* dbuf = nullptr;
+  const size_t nd = cells * 9 + (size_t)cap * SLOT_PART;
+  CUP2D_HIP_CHECK(c, hipMalloc(&dbuf, sizeof(double) * nd));
+  CUP2D_HIP_CHECK(c, hipMemset(dbuf, 0, sizeof(double) * nd));
+  o.chi = dbuf;
+  o.dist = dbuf + cells;
+  o.rho = dbuf + 2 * cells;
+  o.udef = dbuf + 3 * cells;
+  o.sdelta = dbuf + 5 * cells;
+  o.sdx = dbuf + 6 * cells;
+  o.sdy = dbuf + 7 * cells;
+  o.part = dbuf + 9 * cells;
+  return CUP2D_OK;
+}
+
+void obstacles_free(cup2d_ctx* c) {
+  if (c->ob.slot_shape) (void)hipFree(c->ob.slot_shape);
+  if (c->ob.chi) (void)hipFree(c->ob.chi);
+  c->ob = Obstacles{};
+}
+
+}  // namespace cup2d

@@ -1,0 +1,422 @@
+// stencils.hip -- full-grid stencil kernels of the CubismUP_2D step on gfx950.
+//
+// Grid layout: dense row-major planes (see internal.hpp).  Each stencil
+// kernel runs one workgroup over a strip of TX columns x RY rows; every
+// thread owns one column and marches up the rows with a three-row register
+// window, so each plane is read once from HBM (left/right neighbours come
+// from the same cache lines).  Domain faces use the zero-gradient ghost of
+// BlockLabOpen (Definitions.h:125-157), multiplied by advDiff's inflow fade
+// where the reference fades the lab (advDiff.cpp:50-55,68-71).
+//
+// All arithmetic follows the reference expression order and is compiled with
+// -ffp-contract=off, so every cell value is the reference's IEEE result.
+#include "device_util.hpp"
+
+namespace cup2d {
+
+constexpr int BC_KILL_FAC = 100;  // Definitions.h:38
+constexpr double EPSD = 2.220446049250313e-16;
+
+int launch_check(cup2d_ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return CUP2D_ERR_HIP;
+  }
+  return CUP2D_OK;
+}
+
+static inline int strip_width(int NX) {
+  for (int t = 256; t >= 32; t >>= 1)
+    if (NX % t == 0) return t;
+  return 32;
+}
+
+// ------------------------------------------------------------ findMaxU
+// Helpers.cpp:136-176: max over cells of |u+Uinf|, |v+Vinf|, |u|, |v|.
+__global__ __launch_bounds__(256) void k_find_max_u(const double* __restrict__ u,
+                                                    const double* __restrict__ v, size_t n,
+                                                    const StepState* __restrict__ st,
+                                                    double* __restrict__ part) {
+  __shared__ double lds[8];
+  const double UINF = st->uinf[0], VINF = st->uinf[1];
+  double m = 0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const double a = u[i], b = v[i];
+    m = dmax(m, fabs(a + UINF));
+    m = dmax(m, fabs(b + VINF));
+    m = dmax(m, fabs(a));
+    m = dmax(m, fabs(b));
+  }
+  m = block_max<256>(m, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// Simulation::calcMaxTimestep (Simulation.cpp:254-277).  rampCFL for
+// step<100 is tabulated on the host with the same libm expression.
+__global__ __launch_bounds__(256) void k_dt(const double* __restrict__ part, int nparts, Grid g,
+                                            const cup2d_shape_state* __restrict__ ss,
+                                            const double* __restrict__ ramp,
+                                            StepState* __restrict__ st, int set_dt) {
+  __shared__ double lds[8];
+  double m = 0;
+  for (int i = threadIdx.x; i < nparts; i += 256) m = dmax(m, part[i]);
+  const double maxU = block_max<256>(m, lds);
+  if (threadIdx.x == 0) {
+    st->maxU = maxU;
+    if (set_dt) {
+      const double h = g.h;
+      const double dtFourier = h * h / g.nu, dtCFL = maxU < 2.2e-16 ? 1 : h / maxU;
+      double maxUb = 0;
+      for (int k = 0; k < g.nshapes; ++k)
+        maxUb = dmax(maxUb, sqrt(ss[k].u * ss[k].u + ss[k].v * ss[k].v));
+      const double dtBody = maxUb < 2.2e-16 ? 1 : h / maxUb;
+      const double mn = dmin(dmin(dtCFL, dtFourier), dtBody);
+      double dt = g.CFL * mn;
+      if (st->step < 100) dt = ramp[st->step] * mn;
+      st->dt = dt;
+    }
+  }
+}
+
+int enqueue_find_max_u(cup2d_ctx* c) {
+  const int nb = 1024;
+  hipLaunchKernelGGL(k_find_max_u, dim3(nb), dim3(256), 0, c->stream, c->u, c->v,
+                     (size_t)c->g.NX * c->g.NY, c->st, c->red);
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->g, c->ss, c->ramp,
+                     c->st, 0);
+  return launch_check(c, "find_max_u");
+}
+
+int enqueue_dt(cup2d_ctx* c) {
+  const int nb = 1024;
+  hipLaunchKernelGGL(k_find_max_u, dim3(nb), dim3(256), 0, c->stream, c->u, c->v,
+                     (size_t)c->g.NX * c->g.NY, c->st, c->red);
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->g, c->ss, c->ramp,
+                     c->st, 1);
+  return launch_check(c, "calc_max_timestep");
+}
+
+__global__ void k_set_dt(StepState* st, double dt) { st->dt = dt; }
+int enqueue_set_dt(cup2d_ctx* c, double dt) {
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->stream, c->st, dt);
+  return launch_check(c, "set_dt");
+}
+
+__global__ void k_step_end(StepState* st) {
+  st->time += st->dt;
+  st->step += 1;
+}
+int enqueue_step_end(cup2d_ctx* c) {
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->st);
+  return launch_check(c, "step_end");
+}
+
+// ------------------------------------------------------------- advDiff
+// advDiff.cpp:14-34 (stencil), :47-55 (factors), :68-79 (faded lab).
+template <int RY>
+__global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
+                                                 const double* __restrict__ v,
+                                                 double* __restrict__ uo, double* __restrict__ vo,
+                                                 Grid g, const StepState* __restrict__ st) {
+  const int NX = g.NX, NY = g.NY;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y0 = blockIdx.y * RY;
+  const double dt = st->dt, h = g.h;
+  const double UINF0 = st->uinf[0], UINF1 = st->uinf[1];
+  const double dfac = (g.nu / h) * (dt / h), afac = -0.5 * dt / h;
+  const double norUinf = dmax(dmax(fabs(UINF0), fabs(UINF1)), EPSD);
+  const double tW = dmax(UINF0, 0.0) / norUinf, tS = dmax(UINF1, 0.0) / norUinf;
+  const double tE = dmin(UINF0, 0.0) / norUinf, tN = dmin(UINF1, 0.0) / norUinf;
+  const double fadeW = 1 - tW * tW / BC_KILL_FAC, fadeS = 1 - tS * tS / BC_KILL_FAC;
+  const double fadeE = 1 - tE * tE / BC_KILL_FAC, fadeN = 1 - tN * tN / BC_KILL_FAC;
+
+  size_t idx = (size_t)y0 * NX + x;
+  double uc = u[idx], vc = v[idx];
+  double us, vs;
+  if (y0 == 0) { us = uc * fadeS; vs = vc * fadeS; }
+  else { us = u[idx - NX]; vs = v[idx - NX]; }
+#pragma unroll 4
+  for (int iy = 0; iy < RY; ++iy, idx += NX) {
+    const int y = y0 + iy;
+    double un, vn;
+    if (y + 1 < NY) { un = u[idx + NX]; vn = v[idx + NX]; }
+    else { un = uc * fadeN; vn = vc * fadeN; }
+    double uw, vw, ue, ve;
+    if (x > 0) { uw = u[idx - 1]; vw = v[idx - 1]; }
+    else { uw = uc * fadeW; vw = vc * fadeW; }
+    if (x + 1 < NX) { ue = u[idx + 1]; ve = v[idx + 1]; }
+    else { ue = uc * fadeE; ve = vc * fadeE; }
+    const double dUadv = (uc + UINF0) * (ue - uw) + (vc + UINF1) * (un - us);
+    const double dUdif = ue + un + uw + us - 4 * uc;
+    const double dVadv = (uc + UINF0) * (ve - vw) + (vc + UINF1) * (vn - vs);
+    const double dVdif = ve + vn + vw + vs - 4 * vc;
+    uo[idx] = uc + (afac * dUadv + dfac * dUdif);
+    vo[idx] = vc + (afac * dVadv + dfac * dVdif);
+    us = uc; vs = vc;
+    uc = un; vc = vn;
+  }
+}
+
+// advDiff.cpp:89-116: IF = sum_E u - sum_W u + sum_N v - sum_S v over the
+// boundary cells; corr = IF / (2 NY + 2 NX); u_W += corr, u_E -= corr,
+// v_S += corr, v_N -= corr.
+__global__ __launch_bounds__(1024) void k_flux_fix(double* __restrict__ u,
+                                                   double* __restrict__ v, Grid g,
+                                                   StepState* __restrict__ st) {
+  __shared__ double lds[16];
+  const int NX = g.NX, NY = g.NY;
+  const int n = 2 * NY + 2 * NX;
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    if (i < NY) s -= u[(size_t)i * NX];
+    else if (i < 2 * NY) s += u[(size_t)(i - NY) * NX + NX - 1];
+    else if (i < 2 * NY + NX) s -= v[i - 2 * NY];
+    else s += v[(size_t)(NY - 1) * NX + (i - 2 * NY - NX)];
+  }
+  const double IF = block_sum<1024>(s, lds);
+  const double corr = IF / (2 * (BS * g.bpdy - 0) + 2 * (BS * g.bpdx - 0));
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    if (i < NY) u[(size_t)i * NX] += corr;
+    else if (i < 2 * NY) u[(size_t)(i - NY) * NX + NX - 1] -= corr;
+    else if (i < 2 * NY + NX) v[i - 2 * NY] += corr;
+    else v[(size_t)(NY - 1) * NX + (i - 2 * NY - NX)] -= corr;
+  }
+  if (threadIdx.x == 0) st->flux_corr = corr;
+}
+
+int enqueue_advdiff(cup2d_ctx* c) {
+  constexpr int RY = 16;
+  const int tx = strip_width(c->g.NX);
+  hipLaunchKernelGGL(k_advdiff<RY>, dim3(c->g.NX / tx, c->g.NY / RY), dim3(tx), 0, c->stream,
+                     c->u, c->v, c->tu, c->tv, c->g, c->st);
+  std::swap(c->u, c->tu);  // advDiff.cpp:81-87 copy tmpV -> vel
+  std::swap(c->v, c->tv);
+  hipLaunchKernelGGL(k_flux_fix, dim3(1), dim3(1024), 0, c->stream, c->u, c->v, c->g, c->st);
+  return launch_check(c, "advdiff");
+}
+
+// --------------------------------------------------------- pressure RHS
+// PressureSingle.cpp:110-128: r = (h/2dt) (u_E - u_W + v_N - v_S), zero-
+// gradient ghosts.
+template <int RY>
+__global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
+                                             const double* __restrict__ v,
+                                             double* __restrict__ r, Grid g,
+                                             const StepState* __restrict__ st) {
+  const int NX = g.NX, NY = g.NY;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y0 = blockIdx.y * RY;
+  const double facDiv = 0.5 * g.h / st->dt;
+  size_t idx = (size_t)y0 * NX + x;
+  double vc = v[idx];
+  double vs = (y0 == 0) ? vc : v[idx - NX];
+#pragma unroll 4
+  for (int iy = 0; iy < RY; ++iy, idx += NX) {
+    const int y = y0 + iy;
+    const double vn = (y + 1 < NY) ? v[idx + NX] : vc;
+    const double uc = u[idx];
+    const double uw = (x > 0) ? u[idx - 1] : uc;
+    const double ue = (x + 1 < NX) ? u[idx + 1] : uc;
+    const double divVx = ue - uw;
+    const double divVy = vn - vs;
+    r[idx] = facDiv * (divVx + divVy);
+    vs = vc;
+    vc = vn;
+  }
+}
+
+int enqueue_pressure_rhs(cup2d_ctx* c) {
+  constexpr int RY = 16;
+  const int tx = strip_width(c->g.NX);
+  hipLaunchKernelGGL(k_rhs<RY>, dim3(c->g.NX / tx, c->g.NY / RY), dim3(tx), 0, c->stream, c->u,
+                     c->v, c->rhs, c->g, c->st);
+  return launch_check(c, "pressure_rhs");
+}
+
+// ------------------------------------------------------- cub2rhs fade
+// PoissonSolver.cpp:61-122: fadeArg from the distance to the four faces,
+// sums of the faded RHS, correction preserving sum(r).
+__device__ inline double fade_arg(const Grid& g, double px, double py) {
+  const double invFadeX = 1 / dmax(g.fadeLenX, EPSD), invFadeY = 1 / dmax(g.fadeLenY, EPSD);
+  const double yt = invFadeY * dmax(0.0, g.fadeLenY - g.extent1 + py);
+  const double yb = invFadeY * dmax(0.0, g.fadeLenY - py);
+  const double xt = invFadeX * dmax(0.0, g.fadeLenX - g.extent0 + px);
+  const double xb = invFadeX * dmax(0.0, g.fadeLenX - px);
+  return dmin(dmax(dmax(dmax(yt, yb), xt), xb), 1.0);
+}
+
+__global__ __launch_bounds__(256) void k_fade_sums(const double* __restrict__ r,
+                                                   const int* __restrict__ blocks, Grid g,
+                                                   double* __restrict__ part) {
+  __shared__ double lds[8];
+  const int b = blocks[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  double sR = 0, sP = 0, sN = 0;
+  for (int k = threadIdx.x; k < BS * BS; k += 256) {
+    const int ix = k % BS, iy = k / BS;
+    const double px = cell_pos(bx, ix, g.hb, g.h), py = cell_pos(by, iy, g.hb, g.h);
+    const double fa = fade_arg(g, px, py);
+    const double f = r[(size_t)(by * BS + iy) * g.NX + bx * BS + ix] * (fa * fa);
+    if (f < 0) sN -= f;
+    if (f > 0) sP += f;
+    sR += f;
+  }
+  sR = block_sum<256>(sR, lds);
+  sP = block_sum<256>(sP, lds);
+  sN = block_sum<256>(sN, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = sR;
+    part[3 * blockIdx.x + 1] = sP;
+    part[3 * blockIdx.x + 2] = sN;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fade_apply(double* __restrict__ r,
+                                                    const int* __restrict__ blocks, int nblocks,
+                                                    Grid g, const double* __restrict__ part,
+                                                    StepState* __restrict__ st) {
+  __shared__ double lds[8];
+  double sumRHS = block_sum_array<256>(part, nblocks, 3, lds);
+  double sumPOS = block_sum_array<256>(part + 1, nblocks, 3, lds);
+  double sumNEG = block_sum_array<256>(part + 2, nblocks, 3, lds);
+  sumNEG = dmax(sumNEG, EPSD);
+  sumPOS = dmax(sumPOS, EPSD);
+  const double corr = sumRHS > 0 ? sumRHS / sumPOS : sumRHS / sumNEG;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->fade_corr = corr;
+  const int b = blocks[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  for (int k = threadIdx.x; k < BS * BS; k += 256) {
+    const int ix = k % BS, iy = k / BS;
+    const double px = cell_pos(bx, ix, g.hb, g.h), py = cell_pos(by, iy, g.hb, g.h);
+    const double fa = fade_arg(g, px, py);
+    double* p = r + (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+    const double fadeRHS = *p * (fa * fa);
+    const double fadeCorrection = corr * (corr > 0 ? dmax(fadeRHS, 0.0) : -dmin(fadeRHS, 0.0));
+    *p = *p * (1 - fa * fa) + fadeCorrection;
+  }
+}
+
+int enqueue_fade(cup2d_ctx* c) {
+  if (c->band_nblocks == 0) return CUP2D_OK;
+  hipLaunchKernelGGL(k_fade_sums, dim3(c->band_nblocks), dim3(256), 0, c->stream, c->rhs,
+                     c->band_blocks, c->g, c->red);
+  hipLaunchKernelGGL(k_fade_apply, dim3(c->band_nblocks), dim3(256), 0, c->stream, c->rhs,
+                     c->band_blocks, c->band_nblocks, c->g, c->red, c->st);
+  return launch_check(c, "cub2rhs fade");
+}
+
+// ------------------------------------------------- pressure correction
+// PressureSingle.cpp:216-241: u -= (dt/2h)(p_E - p_W), v -= (dt/2h)(p_N - p_S).
+template <int RY>
+__global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double* __restrict__ v,
+                                                 const double* __restrict__ p, Grid g,
+                                                 const StepState* __restrict__ st) {
+  const int NX = g.NX, NY = g.NY;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y0 = blockIdx.y * RY;
+  const double pFac = -0.5 * st->dt / g.h;
+  size_t idx = (size_t)y0 * NX + x;
+  double pc = p[idx];
+  double ps = (y0 == 0) ? pc : p[idx - NX];
+#pragma unroll 4
+  for (int iy = 0; iy < RY; ++iy, idx += NX) {
+    const int y = y0 + iy;
+    const double pn = (y + 1 < NY) ? p[idx + NX] : pc;
+    const double pw = (x > 0) ? p[idx - 1] : pc;
+    const double pe = (x + 1 < NX) ? p[idx + 1] : pc;
+    u[idx] = u[idx] + pFac * (pe - pw);
+    v[idx] = v[idx] + pFac * (pn - ps);
+    ps = pc;
+    pc = pn;
+  }
+}
+
+int enqueue_pressure_correction(cup2d_ctx* c) {
+  constexpr int RY = 16;
+  const int tx = strip_width(c->g.NX);
+  hipLaunchKernelGGL(k_project<RY>, dim3(c->g.NX / tx, c->g.NY / RY), dim3(tx), 0, c->stream,
+                     c->u, c->v, c->pres, c->g, c->st);
+  return launch_check(c, "pressure_correction");
+}
+
+// ------------------------------------------------------ IC / ApplyObjVel
+// IC::operator() (Helpers.cpp:38-82) plus the PutObjectsOnGrid clear values
+// (chi=0, sdf=-1, udef=0, invRho=1; PutObjectsOnGrid.cpp:191-197).
+__global__ void k_ic(double* u, double* v, double* tu, double* tv, double* pres, double* rhs,
+                     double* sdf, double* chi, double* udx, double* udy, double* invrho,
+                     size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    u[i] = 0; v[i] = 0; tu[i] = 0; tv[i] = 0; pres[i] = 0; rhs[i] = 0;
+    sdf[i] = -1; chi[i] = 0; udx[i] = 0; udy[i] = 0; invrho[i] = 1;
+  }
+}
+__global__ void k_ic_state(StepState* st) {
+  st->time = 0; st->dt = 0; st->step = 0; st->uinf[0] = 0; st->uinf[1] = 0;
+  st->error = 0; st->nob = 0; st->nub = 0; st->nprev = 0;
+  st->maxU = 0; st->fade_corr = 0; st->flux_corr = 0;
+}
+int enqueue_ic(cup2d_ctx* c) {
+  hipLaunchKernelGGL(k_ic, dim3(2048), dim3(256), 0, c->stream, c->u, c->v, c->tu, c->tv,
+                     c->pres, c->rhs, c->sdf, c->chi, c->udx, c->udy, c->invrho,
+                     (size_t)c->g.NX * c->g.NY);
+  hipLaunchKernelGGL(k_ic_state, dim3(1), dim3(1), 0, c->stream, c->st);
+  return launch_check(c, "ic");
+}
+
+// Helpers.cpp:290-307: u = u(1-chi) + udef*chi.
+__global__ void k_apply_obj_vel(double* u, double* v, const double* chi, const double* udx,
+                                const double* udy, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double X = chi[i];
+    u[i] = u[i] * (1 - X) + udx[i] * X;
+    v[i] = v[i] * (1 - X) + udy[i] * X;
+  }
+}
+int enqueue_apply_obj_vel(cup2d_ctx* c) {
+  hipLaunchKernelGGL(k_apply_obj_vel, dim3(2048), dim3(256), 0, c->stream, c->u, c->v, c->chi,
+                     c->udx, c->udy, (size_t)c->g.NX * c->g.NY);
+  return launch_check(c, "apply_obj_vel");
+}
+
+// ------------------------------------------------------ layout converters
+// Block layout: [(by*bpdx+bx)][iy][ix][comp]; dense: plane[(by*32+iy)*NX + bx*32+ix].
+__global__ void k_blocks_to_dense(const double* __restrict__ s, double* a, double* b, int nc,
+                                  Grid g, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t blk = i / (BS * BS);
+    const int k = (int)(i % (BS * BS)), ix = k % BS, iy = k / BS;
+    const int bx = (int)(blk % g.bpdx), by = (int)(blk / g.bpdx);
+    const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+    a[d] = s[i * nc];
+    if (nc > 1) b[d] = s[i * nc + 1];
+  }
+}
+__global__ void k_dense_to_blocks(const double* __restrict__ a, const double* __restrict__ b,
+                                  double* s, int nc, Grid g, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t blk = i / (BS * BS);
+    const int k = (int)(i % (BS * BS)), ix = k % BS, iy = k / BS;
+    const int bx = (int)(blk % g.bpdx), by = (int)(blk / g.bpdx);
+    const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+    s[i * nc] = a[d];
+    if (nc > 1) s[i * nc + 1] = b[d];
+  }
+}
+int enqueue_blocks_to_dense(cup2d_ctx* c, const double* stage, double* a, double* b, int nc) {
+  hipLaunchKernelGGL(k_blocks_to_dense, dim3(2048), dim3(256), 0, c->stream, stage, a, b, nc,
+                     c->g, (size_t)c->g.NX * c->g.NY);
+  return launch_check(c, "blocks_to_dense");
+}
+int enqueue_dense_to_blocks(cup2d_ctx* c, const double* a, const double* b, double* stage,
+                            int nc) {
+  hipLaunchKernelGGL(k_dense_to_blocks, dim3(2048), dim3(256), 0, c->stream, a, b, stage, nc,
+                     c->g, (size_t)c->g.NX * c->g.NY);
+  return launch_check(c, "dense_to_blocks");
+}
+
+}  // namespace cup2d

@@ -1,0 +1,158 @@
+/*
+ * cup2d.h -- C-ABI of libcup2d_hip.so, the MI355X (gfx950) implementation of
+ * CubismUP_2D's per-timestep hot path on the uniform block grid.
+ *
+ * Every entry point returns an int status (CUP2D_OK == 0, negative on error;
+ * the message is in cup2d_last_error()).  No torch types, plain pointers and
+ * sizes.  Each entry point names the reference interface it replaces.
+ *
+ * Data contract: the host exchanges fields either in the reference's block
+ * layout (Cubism GridBlock<Element>::data[iy][ix], blocks in blockID order
+ * blockID = bx + bpdx*by, vector components interleaved; Definitions.h:51-110)
+ * or as dense row-major (comp, NY, NX) arrays.  On the device every field is
+ * a dense row-major plane per component.
+ *
+ * Operators are enqueued on the context's stream; the device holds dt, time,
+ * step, U_inf and every body's state, so a whole step runs without a host
+ * round trip.  Entry points that return values to the host synchronize.
+ */
+#ifndef CUP2D_H
+#define CUP2D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUP2D_BS 32
+
+enum cup2d_status {
+  CUP2D_OK = 0,
+  CUP2D_ERR_ARG = -1,       /* invalid argument / size / layout            */
+  CUP2D_ERR_HIP = -2,       /* HIP runtime or launch failure              */
+  CUP2D_ERR_STATE = -3,     /* call not valid in the current state        */
+  CUP2D_ERR_BODY_OUT = -4,  /* a body left the domain (PutObjectsOnGrid.cpp:213-218) */
+  CUP2D_ERR_CAPACITY = -5,  /* obstacle-block pool exhausted              */
+  CUP2D_ERR_NONFINITE = -6  /* NaN/Inf detected (Helpers.cpp:178-288)    */
+};
+
+enum cup2d_poisson { CUP2D_POISSON_PERIODIC = 0, CUP2D_POISSON_FREESPACE = 1 };
+enum cup2d_field {
+  CUP2D_VEL = 0, CUP2D_CHI, CUP2D_PRES, CUP2D_TMP, CUP2D_TMPV, CUP2D_UDEF, CUP2D_INVRHO,
+  CUP2D_NFIELDS
+};
+enum cup2d_layout { CUP2D_LAYOUT_BLOCKS = 0, CUP2D_LAYOUT_DENSE = 1 };
+
+/* Run configuration: the SimulationData parameters parsed in
+ * Simulation::parseRuntime (Simulation.cpp:66-104). */
+typedef struct {
+  int bpdx, bpdy;        /* -bpdx -bpdy                                      */
+  double extent;         /* -extent                                          */
+  double nu;             /* -nu                                              */
+  double CFL;            /* -CFL                                             */
+  double lambda;         /* -lambda (implicit penalization only)             */
+  double fadeLen;        /* -fadeLen (fraction of extent)                    */
+  int poisson;           /* -poissonType periodic|freespace                  */
+  int implicit_penal;    /* 0: EXPL_INTEGRATE_MOM (PressureSingle.cpp:20)    */
+  void* stream;          /* hipStream_t to enqueue on; NULL = own stream     */
+} cup2d_cfg;
+
+/* Shape description: the keys parsed by Shape::Shape and Disk::Disk
+ * (Shape.cpp:331-345, ShapesSimple.h:19-21), defaults already resolved. */
+typedef struct {
+  int kind;              /* 0 = disk                                         */
+  double radius, xpos, ypos, angle_deg, rhoS;
+  int bFixedx, bFixedy, bForcedx, bForcedy, bBlockAng;
+  double xvel, yvel, angvel, timeForced, tAccel;
+} cup2d_shape_desc;
+
+/* Body state after a step (Shape.h:22-62). */
+typedef struct {
+  double center[2], centerOfMass[2], labCenterOfMass[2], d_gm[2], orientation;
+  double M, J, u, v, omega;
+  double fluidAngMom, fluidMomX, fluidMomY, penalDX, penalDY, penalM, penalJ;
+  double perimeter, forcex, forcey, forcex_P, forcey_P, forcex_V, forcey_V;
+  double torque, torque_P, torque_V, drag, thrust, circulation;
+  double Pout, PoutBnd, defPower, defPowerBnd, Pthrust, Pdrag, EffPDef, EffPDefBnd;
+  double n_obstacle_blocks, n_surface_points;
+} cup2d_shape_state;
+
+typedef struct cup2d_ctx cup2d_ctx;
+
+/* --- lifetime ------------------------------------------------------------
+ * Replaces SimulationData::allocateGrid (SimulationData.cpp:17-47) and the
+ * solver constructors FFTW_periodic::FFTW_periodic (FFTW_periodic.h:99-130),
+ * FFTW_freespace::FFTW_freespace (FFTW_freespace.cpp:299-359). */
+int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out);
+/* Replaces SimulationData::~SimulationData (SimulationData.cpp:150-170). */
+int cup2d_destroy(cup2d_ctx* ctx);
+/* Message of the last error on this context ("" if none). */
+const char* cup2d_last_error(const cup2d_ctx* ctx);
+/* Replaces Simulation::createShapes for "disk" (Simulation.cpp:106-173). */
+int cup2d_add_shape(cup2d_ctx* ctx, const cup2d_shape_desc* desc, int* shape_id);
+
+/* --- data exchange ---------------------------------------------------------
+ * Host <-> device copy of one field (all components).  BLOCKS layout is the
+ * reference's grid memory reached through BlockInfo::ptrBlock; DENSE is
+ * (comp, NY, NX).  These are the data movements of PoissonSolver::cub2rhs /
+ * sol2cub (PoissonSolver.cpp:125-168) when the caller keeps host blocks. */
+int cup2d_upload_field(cup2d_ctx* ctx, int field, int layout, const double* host);
+int cup2d_download_field(cup2d_ctx* ctx, int field, int layout, double* host);
+/* Device pointer of one component plane (dense, row-major, pitch NX). */
+int cup2d_field_device_ptr(cup2d_ctx* ctx, int field, int comp, void** dptr);
+/* time, dt, step, U_inf (SimulationData.h:346-350) and grid spacing h. */
+int cup2d_get_scalars(cup2d_ctx* ctx, double* time, double* dt, int* step, double uinf[2],
+                      double* h);
+int cup2d_set_scalars(cup2d_ctx* ctx, double time, int step, double uinfx, double uinfy);
+int cup2d_get_shape(cup2d_ctx* ctx, int shape_id, cup2d_shape_state* out);
+
+/* --- operators (Operator::operator()(dt), Operator.h:24-26) --------------- */
+/* Simulation::reset: IC + PutObjectsOnGrid(0) + ApplyObjVel (Simulation.cpp:232-241). */
+int cup2d_reset(cup2d_ctx* ctx);
+/* findMaxU::run (Helpers.cpp:136-176). */
+int cup2d_find_max_u(cup2d_ctx* ctx, double* maxU);
+/* Simulation::calcMaxTimestep (Simulation.cpp:254-277); dt also stays on device. */
+int cup2d_calc_max_timestep(cup2d_ctx* ctx, double* dt);
+/* PutObjectsOnGrid::operator() (PutObjectsOnGrid.cpp:186-241). */
+int cup2d_put_objects(cup2d_ctx* ctx, double dt);
+/* advDiff::operator() (advDiff.cpp:36-118). */
+int cup2d_advdiff(cup2d_ctx* ctx, double dt);
+/* PressureSingle::updatePressureRHS (PressureSingle.cpp:110-214). */
+int cup2d_pressure_rhs(cup2d_ctx* ctx, double dt);
+/* PoissonSolver::solve(BSRC=tmp, BDST=pres): cub2rhs fade + FFT solve +
+ * sol2cub (PoissonSolver.cpp:61-168, FFTW_periodic.h:135-166,
+ * FFTW_freespace.cpp:14-56, CUDA_all.cu:41-128). */
+int cup2d_poisson_solve(cup2d_ctx* ctx);
+/* PressureSingle::pressureCorrection (PressureSingle.cpp:216-241). */
+int cup2d_pressure_correction(cup2d_ctx* ctx, double dt);
+/* integrateMomenta + Shape::updateVelocity + preventCollidingObstacles
+ * (PressureSingle.cpp:23-67,243-344,362-367; Shape.cpp:25-59). */
+int cup2d_body_update(cup2d_ctx* ctx, double dt);
+/* PressureSingle::penalize (PressureSingle.cpp:69-108). */
+int cup2d_penalize(cup2d_ctx* ctx, double dt);
+/* PressureSingle::operator() (PressureSingle.cpp:346-373). */
+int cup2d_pressure_single(cup2d_ctx* ctx, double dt);
+/* ComputeForces::operator() + Shape::computeForces (ComputeForces.cpp:16-111,
+ * Shape.cpp:252-281). */
+int cup2d_compute_forces(cup2d_ctx* ctx);
+/* Simulation::advance (Simulation.cpp:279-305) with the given dt. */
+int cup2d_advance(cup2d_ctx* ctx, double dt);
+/* nsteps x (calcMaxTimestep + advance), dt computed on the device, no host
+ * synchronization inside (Simulation::simulate, Simulation.cpp:243-252). */
+int cup2d_simulate(cup2d_ctx* ctx, int nsteps);
+/* Wait for the context's stream; reports device-side errors (body out of
+ * domain) raised by enqueued steps. */
+int cup2d_synchronize(cup2d_ctx* ctx);
+
+/* Dense Poisson solve without the boundary fade: rhs (NY*NX, row-major,
+ * host) -> sol (host).  Same transform as cup2d_poisson_solve. */
+int cup2d_poisson_dense(cup2d_ctx* ctx, const double* rhs, double* sol);
+
+/* Cells per dimension of the grid and the number of built-in timers. */
+int cup2d_grid_size(const cup2d_ctx* ctx, int* NX, int* NY);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

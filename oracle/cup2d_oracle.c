@@ -1,0 +1,1303 @@
+/*
+ * cup2d_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 C/OpenMP restatement of the CubismUP_2D per-timestep hot path (the
+ * checker for the HIP library, and the timed CPU baseline).  Every function
+ * cites the reference file:line it follows.  Deliberate, documented choices
+ * where the reference is not deterministic or not available:
+ *   - OpenMP `reduction(+:...)` clauses (advDiff.cpp:94, PressureSingle.cpp:30,
+ *     136, PoissonSolver.cpp:81, PutObjectsOnGrid.cpp:23, Shape.cpp:121) are
+ *     replaced by per-block partial sums accumulated sequentially in the
+ *     reference's loop order inside the block, then summed in blockID order.
+ *   - std::pow(x,2) is written x*x; GSL LU (Shape.cpp:48-54) is restated as
+ *     Doolittle LU with partial pivoting; FFTW is the oracle's own FFT
+ *     (cup2d_fft.c).
+ *   - Cubism (absent) semantics: blockID = bx + bpdx*by, block spacing
+ *     hb = extent/max(bpdx,bpdy), h = hb/32, origin = (bx*hb, by*hb),
+ *     pos(ix,iy) = origin + h*(i+0.5); labs copy the block plus a one-cell
+ *     halo from neighbour blocks, domain faces filled by BlockLabOpen's
+ *     zero-gradient rule (Definitions.h:125-157).
+ */
+#include "cup2d_oracle.h"
+#include "cup2d_fft.h"
+#include <float.h>
+#include <math.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define BS ORC_BS
+#define BC_KILL_FAC 100 /* Definitions.h:38 */
+#define LW (BS + 2)
+
+static const double EPS = DBL_EPSILON;
+
+typedef struct {
+  double chi[BS][BS], dist[BS][BS], rho[BS][BS], udef[BS][BS][2];
+  int nsurf;
+  int sx[BS * BS], sy[BS * BS];
+  double dchidx[BS * BS], dchidy[BS * BS], delta[BS * BS];
+  /* additive per-block quantities (ObstacleBlock.h:285-288) */
+  double perimeter, forcex, forcey, forcex_P, forcey_P, forcex_V, forcey_V;
+  double torque, torque_P, torque_V, drag, thrust, Pout, PoutBnd, defPower,
+      defPowerBnd, circulation;
+} oblock;
+
+typedef struct {
+  orc_shape_desc d;
+  double origC[2], origAng, forcedu, forcedv, forcedomega;
+  double appliedForceX, appliedForceY, appliedTorque;
+  orc_shape_state st;
+  oblock** ob; /* [nblocks], NULL where the shape does not touch the block */
+} shape_t;
+
+struct orc_sim {
+  orc_cfg cfg;
+  int bpdx, bpdy, nb, NX, NY;
+  double hb, h, extents[2], fadeLenX, fadeLenY;
+  double* f[ORC_NFIELDS];
+  double* pRHS; /* scanned by the Checker only */
+  double time, dt;
+  int step;
+  double uinfx, uinfy;
+  shape_t* shapes;
+  int nshapes;
+  int* bnd;
+  int nbnd;
+  /* Poisson */
+  int MX, MY, MXh;
+  size_t stride;
+  double* buffer;
+  double* kernel;
+  fft2d fft;
+  double t_total, t_fft;
+};
+
+static const int NCOMP[ORC_NFIELDS] = {2, 1, 1, 1, 2, 2, 1};
+int orc_ncomp(int field) { return (field >= 0 && field < ORC_NFIELDS) ? NCOMP[field] : 0; }
+int orc_num_threads(void) { return omp_get_max_threads(); }
+
+static inline double sq(double x) { return x * x; }
+static inline double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
+static inline double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
+
+static inline int BX(const orc_sim* s, int b) { return b % s->bpdx; }
+static inline int BY(const orc_sim* s, int b) { return b / s->bpdx; }
+
+/* Cubism BlockInfo::pos: origin + h_gridpoint*(i+0.5) */
+static inline void bpos(const orc_sim* s, int b, int ix, int iy, double p[2])
+{
+  const double ox = (double)BX(s, b) * s->hb, oy = (double)BY(s, b) * s->hb;
+  p[0] = ox + s->h * (ix + 0.5);
+  p[1] = oy + s->h * (iy + 0.5);
+}
+
+static inline double* blk(const orc_sim* s, int field, int b)
+{
+  return s->f[field] + (size_t)b * BS * BS * NCOMP[field];
+}
+
+/* ------------------------------------------------------------------ labs */
+/* lab(ix,iy,c) for ix,iy in [-1,BS]; Cubism BlockLab::load + _apply_bc */
+#define LAB(L, nc, ix, iy, c) (L)[(((iy) + 1) * LW + ((ix) + 1)) * (nc) + (c)]
+
+static void lab_load(const orc_sim* s, const double* field, int nc, int b, double* L)
+{
+  const int bx = BX(s, b), by = BY(s, b);
+  const size_t bsz = (size_t)BS * BS * nc;
+  const double* B = field + (size_t)b * bsz;
+  for (int iy = 0; iy < BS; ++iy)
+    for (int ix = 0; ix < BS; ++ix)
+      for (int c = 0; c < nc; ++c) LAB(L, nc, ix, iy, c) = B[(iy * BS + ix) * nc + c];
+  /* neighbour faces and corners */
+  for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+      if (!dx && !dy) continue;
+      const int nx = bx + dx, ny = by + dy;
+      if (nx < 0 || ny < 0 || nx >= s->bpdx || ny >= s->bpdy) continue;
+      const double* N = field + (size_t)(nx + s->bpdx * ny) * bsz;
+      const int x0 = dx < 0 ? -1 : (dx > 0 ? BS : 0), x1 = dx < 0 ? -1 : (dx > 0 ? BS : BS - 1);
+      const int y0 = dy < 0 ? -1 : (dy > 0 ? BS : 0), y1 = dy < 0 ? -1 : (dy > 0 ? BS : BS - 1);
+      for (int iy = y0; iy <= y1; ++iy)
+        for (int ix = x0; ix <= x1; ++ix) {
+          const int sx = ix - dx * BS, sy = iy - dy * BS;
+          for (int c = 0; c < nc; ++c) LAB(L, nc, ix, iy, c) = N[(sy * BS + sx) * nc + c];
+        }
+    }
+  /* BlockLabOpen::_apply_bc, Definitions.h:151-157 (x faces, then y faces) */
+  if (bx == 0)
+    for (int iy = -1; iy <= BS; ++iy)
+      for (int c = 0; c < nc; ++c) LAB(L, nc, -1, iy, c) = LAB(L, nc, 0, iy, c);
+  if (bx == s->bpdx - 1)
+    for (int iy = -1; iy <= BS; ++iy)
+      for (int c = 0; c < nc; ++c) LAB(L, nc, BS, iy, c) = LAB(L, nc, BS - 1, iy, c);
+  if (by == 0)
+    for (int ix = -1; ix <= BS; ++ix)
+      for (int c = 0; c < nc; ++c) LAB(L, nc, ix, -1, c) = LAB(L, nc, ix, 0, c);
+  if (by == s->bpdy - 1)
+    for (int ix = -1; ix <= BS; ++ix)
+      for (int c = 0; c < nc; ++c) LAB(L, nc, ix, BS, c) = LAB(L, nc, ix, BS - 1, c);
+}
+
+/* -------------------------------------------------------------- Checker */
+/* Checker::run, Helpers.cpp:178-288 (NaN/Inf scan of 10 reals per cell) */
+static int checker(orc_sim* s)
+{
+  if (!s->cfg.checker) return 0;
+  const size_t ncell = (size_t)s->nb * BS * BS;
+  int bad = 0;
+  #pragma omp parallel for schedule(static) reduction(| : bad)
+  for (size_t i = 0; i < ncell; ++i) {
+    const double a[10] = {s->f[ORC_VEL][2 * i], s->f[ORC_VEL][2 * i + 1], s->f[ORC_UDEF][2 * i],
+                          s->f[ORC_UDEF][2 * i + 1], s->f[ORC_TMPV][2 * i], s->f[ORC_TMPV][2 * i + 1],
+                          s->f[ORC_CHI][i], s->f[ORC_PRES][i], s->f[ORC_TMP][i], s->pRHS[i]};
+    for (int k = 0; k < 10; ++k) bad |= (isnan(a[k]) || isinf(a[k]));
+  }
+  if (bad) fprintf(stderr, "oracle Checker: NaN/Inf found\n");
+  return bad;
+}
+
+/* ------------------------------------------------------------ creation */
+orc_sim* orc_create(const orc_cfg* cfg)
+{
+  if (cfg->bpdx <= 0 || cfg->bpdy <= 0) return NULL;
+  orc_sim* s = (orc_sim*)calloc(1, sizeof(orc_sim));
+  s->cfg = *cfg;
+  s->bpdx = cfg->bpdx; s->bpdy = cfg->bpdy; s->nb = s->bpdx * s->bpdy;
+  s->NX = BS * s->bpdx; s->NY = BS * s->bpdy;
+  /* Cubism Grid(bpdx,bpdy,1,extent): block spacing extent/max(blocks) */
+  s->hb = cfg->extent / (double)(s->bpdx > s->bpdy ? s->bpdx : s->bpdy);
+  s->h = s->hb / BS;
+  /* SimulationData.cpp:34-35 */
+  s->extents[0] = (double)s->bpdx * s->h * BS;
+  s->extents[1] = (double)s->bpdy * s->h * BS;
+  /* Simulation.cpp:97-98 */
+  s->fadeLenX = cfg->fadeLen * cfg->extent;
+  s->fadeLenY = cfg->fadeLen * cfg->extent;
+  const size_t ncell = (size_t)s->nb * BS * BS;
+  for (int f = 0; f < ORC_NFIELDS; ++f) s->f[f] = (double*)calloc(ncell * NCOMP[f], sizeof(double));
+  s->pRHS = (double*)calloc(ncell, sizeof(double));
+  /* SimulationData.cpp:38-46 */
+  s->bnd = (int*)malloc(sizeof(int) * (size_t)s->nb);
+  for (int b = 0; b < s->nb; ++b)
+    if (BX(s, b) == 0 || BX(s, b) == s->bpdx - 1 || BY(s, b) == 0 || BY(s, b) == s->bpdy - 1)
+      s->bnd[s->nbnd++] = b;
+
+  if (cfg->poisson == 0) { /* FFTW_periodic.h:25,96-97 */
+    s->MX = s->NX; s->MY = s->NY;
+  } else { /* FFTW_freespace.h:25-27 (2N-1), or 2N */
+    s->MX = cfg->fs_pad2 ? 2 * s->NX : 2 * s->NX - 1;
+    s->MY = cfg->fs_pad2 ? 2 * s->NY : 2 * s->NY - 1;
+  }
+  s->MXh = s->MX / 2 + 1;
+  s->stride = 2 * (size_t)s->MXh;
+  s->buffer = (double*)calloc(s->stride * (size_t)s->MY, sizeof(double));
+  fft2d_init(&s->fft, s->MX, s->MY, s->stride);
+  if (cfg->poisson != 0) {
+    /* FFTW_freespace.cpp:321-358 */
+    const double h = s->h, fac = 1 / (2 * M_PI);
+    const int totNx = s->NX, totNy = s->NY, MX = s->MX, MY = s->MY;
+    #pragma omp parallel for schedule(static)
+    for (int j = 0; j < MY; ++j)
+      for (int i = 0; i < MX; ++i) {
+        const double yi = j >= totNy ? MY - j : j;
+        const double xi = i >= totNx ? MX - i : i;
+        const double r = sqrt(xi * xi + yi * yi);
+        s->buffer[i + s->stride * j] = fac * log(h * r);
+      }
+    s->buffer[0] = fac / 2 * (2 * log(h / sqrt(M_PI)) - 1);
+    s->kernel = (double*)malloc(sizeof(double) * (size_t)s->MY * s->MXh);
+    fft2d_exec(&s->fft, s->buffer, -1, MY);
+    const double norm_factor = 1.0 / ((double)MY * MX);
+    for (int j = 0; j < MY; ++j)
+      for (int i = 0; i < s->MXh; ++i)
+        s->kernel[i + (size_t)s->MXh * j] = s->buffer[2 * (i + (size_t)s->MXh * j)] * norm_factor;
+    memset(s->buffer, 0, sizeof(double) * s->stride * (size_t)s->MY);
+  }
+  return s;
+}
+
+void orc_destroy(orc_sim* s)
+{
+  if (!s) return;
+  for (int f = 0; f < ORC_NFIELDS; ++f) free(s->f[f]);
+  free(s->pRHS);
+  for (int k = 0; k < s->nshapes; ++k) {
+    for (int b = 0; b < s->nb; ++b) free(s->shapes[k].ob[b]);
+    free(s->shapes[k].ob);
+  }
+  free(s->shapes); free(s->bnd); free(s->buffer); free(s->kernel);
+  fft2d_free(&s->fft);
+  free(s);
+}
+
+static void shape_reset(orc_sim* s, shape_t* sh)
+{ /* Shape::resetAll, Shape.h:64-88 */
+  orc_shape_state* t = &sh->st;
+  memset(t, 0, sizeof(*t));
+  t->center[0] = sh->origC[0]; t->center[1] = sh->origC[1];
+  t->centerOfMass[0] = sh->origC[0]; t->centerOfMass[1] = sh->origC[1];
+  t->orientation = sh->origAng;
+  t->u = sh->forcedu; t->v = sh->forcedv; t->omega = sh->forcedomega;
+  sh->appliedForceX = sh->appliedForceY = sh->appliedTorque = 0;
+  for (int b = 0; b < s->nb; ++b) { free(sh->ob[b]); sh->ob[b] = NULL; }
+}
+
+int orc_add_shape(orc_sim* s, const orc_shape_desc* d)
+{
+  if (d->kind != 0) return -1;
+  s->shapes = (shape_t*)realloc(s->shapes, sizeof(shape_t) * (size_t)(s->nshapes + 1));
+  shape_t* sh = &s->shapes[s->nshapes];
+  memset(sh, 0, sizeof(*sh));
+  sh->d = *d;
+  /* Shape::Shape, Shape.cpp:331-345 */
+  sh->origC[0] = d->xpos; sh->origC[1] = d->ypos;
+  sh->origAng = d->angle_deg * M_PI / 180;
+  sh->forcedu = -d->xvel; sh->forcedv = -d->yvel; sh->forcedomega = -d->angvel;
+  sh->ob = (oblock**)calloc((size_t)s->nb, sizeof(oblock*));
+  shape_reset(s, sh);
+  return s->nshapes++;
+}
+
+int orc_set_field(orc_sim* s, int field, const double* src)
+{
+  if (field < 0 || field >= ORC_NFIELDS) return -1;
+  memcpy(s->f[field], src, sizeof(double) * (size_t)s->nb * BS * BS * NCOMP[field]);
+  return 0;
+}
+int orc_get_field(const orc_sim* s, int field, double* dst)
+{
+  if (field < 0 || field >= ORC_NFIELDS) return -1;
+  memcpy(dst, s->f[field], sizeof(double) * (size_t)s->nb * BS * BS * NCOMP[field]);
+  return 0;
+}
+void orc_get_scalars(const orc_sim* s, double* time, double* dt, int* step, double uinf[2], double* h)
+{
+  *time = s->time; *dt = s->dt; *step = s->step; uinf[0] = s->uinfx; uinf[1] = s->uinfy; *h = s->h;
+}
+void orc_set_scalars(orc_sim* s, double time, int step, double uinfx, double uinfy)
+{
+  s->time = time; s->step = step; s->uinfx = uinfx; s->uinfy = uinfy;
+}
+int orc_get_shape(const orc_sim* s, int i, orc_shape_state* out)
+{
+  if (i < 0 || i >= s->nshapes) return -1;
+  *out = s->shapes[i].st;
+  int nob = 0, ns = 0;
+  for (int b = 0; b < s->nb; ++b)
+    if (s->shapes[i].ob[b]) { ++nob; ns += s->shapes[i].ob[b]->nsurf; }
+  out->n_obstacle_blocks = nob;
+  out->n_surface_points = ns;
+  return 0;
+}
+void orc_get_timing(const orc_sim* s, double* total, double* fft) { *total = s->t_total; *fft = s->t_fft; }
+
+/* --------------------------------------------------------------- helpers */
+/* IC::operator(), Helpers.cpp:38-82 */
+static void ic(orc_sim* s)
+{
+  const size_t ncell = (size_t)s->nb * BS * BS;
+  for (int f = 0; f < ORC_NFIELDS; ++f) memset(s->f[f], 0, sizeof(double) * ncell * NCOMP[f]);
+  memset(s->pRHS, 0, sizeof(double) * ncell);
+  for (size_t i = 0; i < ncell; ++i) s->f[ORC_INVRHO][i] = 1;
+}
+
+/* ApplyObjVel, Helpers.cpp:290-307 */
+void orc_apply_obj_vel(orc_sim* s)
+{
+  const size_t ncell = (size_t)s->nb * BS * BS;
+  double* U = s->f[ORC_VEL];
+  const double *US = s->f[ORC_UDEF], *X = s->f[ORC_CHI];
+  #pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < ncell; ++i) {
+    U[2 * i] = U[2 * i] * (1 - X[i]) + US[2 * i] * X[i];
+    U[2 * i + 1] = U[2 * i + 1] * (1 - X[i]) + US[2 * i + 1] * X[i];
+  }
+}
+
+/* findMaxU::run, Helpers.cpp:136-176 */
+double orc_find_max_u(orc_sim* s)
+{
+  checker(s);
+  const double UINF = s->uinfx, VINF = s->uinfy;
+  const size_t ncell = (size_t)s->nb * BS * BS;
+  const double* V = s->f[ORC_VEL];
+  double U = 0, Vm = 0, u = 0, v = 0;
+  #pragma omp parallel for schedule(static) reduction(max : U, Vm, u, v)
+  for (size_t i = 0; i < ncell; ++i) {
+    U = dmax(U, fabs(V[2 * i] + UINF));
+    Vm = dmax(Vm, fabs(V[2 * i + 1] + VINF));
+    u = dmax(u, fabs(V[2 * i]));
+    v = dmax(v, fabs(V[2 * i + 1]));
+  }
+  return dmax(dmax(U, Vm), dmax(u, v));
+}
+
+static double max_rel_speed(const orc_sim* s)
+{ /* SimulationData::maxRelSpeed + Shape::getMaxVel */
+  double m = 0;
+  for (int k = 0; k < s->nshapes; ++k) {
+    const orc_shape_state* t = &s->shapes[k].st;
+    m = dmax(m, sqrt(t->u * t->u + t->v * t->v));
+  }
+  return m;
+}
+
+/* Simulation::calcMaxTimestep, Simulation.cpp:254-277 */
+double orc_calc_max_timestep(orc_sim* s)
+{
+  const double maxU = orc_find_max_u(s);
+  const double h = s->h;
+  const double dtFourier = h * h / s->cfg.nu, dtCFL = maxU < 2.2e-16 ? 1 : h / maxU;
+  const double maxUb = max_rel_speed(s), dtBody = maxUb < 2.2e-16 ? 1 : h / maxUb;
+  const double mn = dmin(dmin(dtCFL, dtFourier), dtBody);
+  s->dt = s->cfg.CFL * mn;
+  if (s->step < 100) {
+    const double x = (s->step + 1.0) / 100;
+    const double rampCFL = exp(log(1e-3) * (1 - x) + log(s->cfg.CFL) * x);
+    s->dt = rampCFL * mn;
+  }
+  return s->dt;
+}
+
+/* ------------------------------------------------------- PutObjectsOnGrid */
+static int disk_touching(const orc_sim* s, int b, const double bbox[2][2])
+{ /* _is_touching, ShapeLibrary.h:24-36 */
+  double MINP[2], MAXP[2];
+  bpos(s, b, 0, 0, MINP);
+  bpos(s, b, BS - 1, BS - 1, MAXP);
+  const double i00 = dmax(MINP[0], bbox[0][0]), i01 = dmin(MAXP[0], bbox[0][1]);
+  const double i10 = dmax(MINP[1], bbox[1][0]), i11 = dmin(MAXP[1], bbox[1][1]);
+  return i01 - i00 > 0 && i11 - i10 > 0;
+}
+
+/* Disk::create + FillBlocks_Cylinder, ShapesSimple.cpp:14-35,
+ * ShapeLibrary.h:38-57, ShapeLibrary.cpp:21-34 */
+static void disk_create(orc_sim* s, shape_t* sh)
+{
+  const double h = s->h, radius = sh->d.radius, safety = 2 * h;
+  const double pos[2] = {sh->st.center[0], sh->st.center[1]};
+  const double bbox[2][2] = {{pos[0] - radius - safety, pos[0] + radius + safety},
+                             {pos[1] - radius - safety, pos[1] + radius + safety}};
+  for (int b = 0; b < s->nb; ++b) { free(sh->ob[b]); sh->ob[b] = NULL; }
+  double* TMP = s->f[ORC_TMP];
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int b = 0; b < s->nb; ++b) {
+    if (!disk_touching(s, b, bbox)) continue;
+    oblock* o = (oblock*)calloc(1, sizeof(oblock));
+    sh->ob[b] = o;
+    double* B = TMP + (size_t)b * BS * BS;
+    for (int iy = 0; iy < BS; iy++)
+      for (int ix = 0; ix < BS; ix++) {
+        double p[2];
+        bpos(s, b, ix, iy, p);
+        p[0] -= pos[0]; p[1] -= pos[1];
+        o->dist[iy][ix] = radius - sqrt(p[0] * p[0] + p[1] * p[1]);
+        B[iy * BS + ix] = dmax(B[iy * BS + ix], o->dist[iy][ix]);
+        o->rho[iy][ix] = sh->d.rhoS;
+      }
+  }
+}
+
+/* PutObjectsOnGrid::putChiOnGrid, PutObjectsOnGrid.cpp:17-147 */
+static void put_chi_on_grid(orc_sim* s, shape_t* sh)
+{
+  const double h = s->h, i2h = 0.5 / h, fac = 0.5 * h;
+  double* px = (double*)calloc((size_t)s->nb * 3, sizeof(double));
+  #pragma omp parallel
+  {
+    double* L = (double*)malloc(sizeof(double) * LW * LW);
+    #pragma omp for schedule(dynamic, 1)
+    for (int b = 0; b < s->nb; ++b) {
+      oblock* o = sh->ob[b];
+      if (!o) continue;
+      lab_load(s, s->f[ORC_TMP], 1, b, L);
+      double* CHI = blk(s, ORC_CHI, b);
+      double* IRHO = blk(s, ORC_INVRHO, b);
+      double _x = 0, _y = 0, _m = 0;
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          const double sdf = o->dist[iy][ix];
+          double X;
+          if (sdf > +2 * h || sdf < -2 * h) {
+            X = sdf > 0 ? 1 : 0;
+          } else {
+            const double distPx = ix + 1 == BS ? LAB(L, 1, ix + 1, iy, 0) : o->dist[iy][ix + 1];
+            const double distMx = ix == 0 ? LAB(L, 1, ix - 1, iy, 0) : o->dist[iy][ix - 1];
+            const double distPy = iy + 1 == BS ? LAB(L, 1, ix, iy + 1, 0) : o->dist[iy + 1][ix];
+            const double distMy = iy == 0 ? LAB(L, 1, ix, iy - 1, 0) : o->dist[iy - 1][ix];
+            const double IplusX = distPx < 0 ? 0 : distPx, IminuX = distMx < 0 ? 0 : distMx;
+            const double IplusY = distPy < 0 ? 0 : distPy, IminuY = distMy < 0 ? 0 : distMy;
+            const double gradIX = i2h * (IplusX - IminuX), gradIY = i2h * (IplusY - IminuY);
+            const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
+            const double gradUSq = gradUX * gradUX + gradUY * gradUY;
+            const double denum = 1 / (gradUSq < EPS ? EPS : gradUSq);
+            X = (gradIX * gradUX + gradIY * gradUY) * denum;
+          }
+          o->chi[iy][ix] = X;
+          if (X >= CHI[iy * BS + ix]) {
+            CHI[iy * BS + ix] = X;
+            IRHO[iy * BS + ix] = X / o->rho[iy][ix] + (1 - X) * IRHO[iy * BS + ix];
+          }
+          if (X > 0) {
+            double p[2];
+            bpos(s, b, ix, iy, p);
+            _x += o->rho[iy][ix] * X * h * h * (p[0] - sh->st.centerOfMass[0]);
+            _y += o->rho[iy][ix] * X * h * h * (p[1] - sh->st.centerOfMass[1]);
+            _m += o->rho[iy][ix] * X * h * h;
+          }
+          if (sdf > +2 * h || sdf < -2 * h) continue;
+          {
+            const double distPx = ix + 1 == BS ? LAB(L, 1, ix + 1, iy, 0) : o->dist[iy][ix + 1];
+            const double distMx = ix == 0 ? LAB(L, 1, ix - 1, iy, 0) : o->dist[iy][ix - 1];
+            const double distPy = iy + 1 == BS ? LAB(L, 1, ix, iy + 1, 0) : o->dist[iy + 1][ix];
+            const double distMy = iy == 0 ? LAB(L, 1, ix, iy - 1, 0) : o->dist[iy - 1][ix];
+            const double HplusX = fabs(distPx) < EPS ? 0.5 : (distPx < 0 ? 0 : 1);
+            const double HminuX = fabs(distMx) < EPS ? 0.5 : (distMx < 0 ? 0 : 1);
+            const double HplusY = fabs(distPy) < EPS ? 0.5 : (distPy < 0 ? 0 : 1);
+            const double HminuY = fabs(distMy) < EPS ? 0.5 : (distMy < 0 ? 0 : 1);
+            const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
+            const double gradHX = (HplusX - HminuX), gradHY = (HplusY - HminuY);
+            const double gradUSq = gradUX * gradUX + gradUY * gradUY;
+            const double denum = 1 / (gradUSq < EPS ? EPS : gradUSq);
+            const double D = fac * (gradHX * gradUX + gradHY * gradUY) * denum;
+            if (D > 0) { /* ObstacleBlock::write, ObstacleBlock.h:336-346 */
+              const int k = o->nsurf++;
+              o->sx[k] = ix; o->sy[k] = iy;
+              o->dchidx[k] = -D * gradUX; o->dchidy[k] = -D * gradUY; o->delta[k] = D;
+            }
+          }
+        }
+      px[3 * b] = _x; px[3 * b + 1] = _y; px[3 * b + 2] = _m;
+    }
+    free(L);
+  }
+  double _x = 0, _y = 0, _m = 0;
+  for (int b = 0; b < s->nb; ++b) { _x += px[3 * b]; _y += px[3 * b + 1]; _m += px[3 * b + 2]; }
+  free(px);
+  if (_m > EPS) {
+    sh->st.centerOfMass[0] += _x / _m;
+    sh->st.centerOfMass[1] += _y / _m;
+    sh->st.M = _m;
+  }
+}
+
+/* Shape::integrateObstBlock + removeMoments, Shape.cpp:117-212 */
+static void remove_moments(orc_sim* s, shape_t* sh)
+{
+  orc_shape_state* t = &sh->st;
+  const double hsq = s->h * s->h;
+  double* part = (double*)calloc((size_t)s->nb * 7, sizeof(double));
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int b = 0; b < s->nb; ++b) {
+    const oblock* o = sh->ob[b];
+    if (!o) continue;
+    double _x = 0, _y = 0, _m = 0, _j = 0, _u = 0, _v = 0, _a = 0;
+    for (int iy = 0; iy < BS; ++iy)
+      for (int ix = 0; ix < BS; ++ix) {
+        if (o->chi[iy][ix] <= 0) continue;
+        double p[2];
+        bpos(s, b, ix, iy, p);
+        const double rhochi = o->chi[iy][ix] * o->rho[iy][ix] * hsq;
+        p[0] -= t->centerOfMass[0];
+        p[1] -= t->centerOfMass[1];
+        _x += rhochi * p[0];
+        _y += rhochi * p[1];
+        _m += rhochi;
+        _j += rhochi * (p[0] * p[0] + p[1] * p[1]);
+        _u += rhochi * o->udef[iy][ix][0];
+        _v += rhochi * o->udef[iy][ix][1];
+        _a += rhochi * (p[0] * o->udef[iy][ix][1] - p[1] * o->udef[iy][ix][0]);
+      }
+    double* q = part + 7 * (size_t)b;
+    q[0] = _x; q[1] = _y; q[2] = _m; q[3] = _j; q[4] = _u; q[5] = _v; q[6] = _a;
+  }
+  double I[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < s->nb; ++b)
+    for (int k = 0; k < 7; ++k) I[k] += part[7 * (size_t)b + k];
+  free(part);
+  I[4] /= I[2]; I[5] /= I[2]; I[6] /= I[3];
+  t->M = I[2]; t->J = I[3];
+  const double dCx = t->center[0] - t->centerOfMass[0];
+  const double dCy = t->center[1] - t->centerOfMass[1];
+  t->d_gm[0] = dCx * cos(t->orientation) + dCy * sin(t->orientation);
+  t->d_gm[1] = -dCx * sin(t->orientation) + dCy * cos(t->orientation);
+  #pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < s->nb; ++b) {
+    oblock* o = sh->ob[b];
+    if (!o) continue;
+    for (int iy = 0; iy < BS; ++iy)
+      for (int ix = 0; ix < BS; ++ix) {
+        double p[2];
+        bpos(s, b, ix, iy, p);
+        p[0] -= t->centerOfMass[0];
+        p[1] -= t->centerOfMass[1];
+        o->udef[iy][ix][0] -= I[4] - I[6] * p[1];
+        o->udef[iy][ix][1] -= I[5] + I[6] * p[0];
+      }
+  }
+}
+
+/* PutObjectsOnGrid::putObjectVelOnGrid, PutObjectsOnGrid.cpp:149-184 */
+static void put_object_vel_on_grid(orc_sim* s, shape_t* sh)
+{
+  #pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < s->nb; ++b) {
+    const oblock* o = sh->ob[b];
+    if (!o) continue;
+    double* UDEF = blk(s, ORC_UDEF, b);
+    const double* CHI = blk(s, ORC_CHI, b);
+    for (int iy = 0; iy < BS; iy++)
+      for (int ix = 0; ix < BS; ix++) {
+        if (o->chi[iy][ix] < CHI[iy * BS + ix] || o->chi[iy][ix] <= 0) continue;
+        UDEF[2 * (iy * BS + ix)] += o->udef[iy][ix][0];
+        UDEF[2 * (iy * BS + ix) + 1] += o->udef[iy][ix][1];
+      }
+  }
+}
+
+/* Shape::updatePosition, Shape.cpp:67-88 */
+static void update_position(orc_sim* s, shape_t* sh, double dt)
+{
+  orc_shape_state* t = &sh->st;
+  t->centerOfMass[0] += dt * (t->u + s->uinfx);
+  t->centerOfMass[1] += dt * (t->v + s->uinfy);
+  t->labCenterOfMass[0] += dt * t->u;
+  t->labCenterOfMass[1] += dt * t->v;
+  t->orientation += dt * t->omega;
+  t->orientation = t->orientation > M_PI ? t->orientation - 2 * M_PI : t->orientation;
+  t->orientation = t->orientation < -M_PI ? t->orientation + 2 * M_PI : t->orientation;
+  const double cosang = cos(t->orientation), sinang = sin(t->orientation);
+  t->center[0] = t->centerOfMass[0] + cosang * t->d_gm[0] - sinang * t->d_gm[1];
+  t->center[1] = t->centerOfMass[1] + sinang * t->d_gm[0] + cosang * t->d_gm[1];
+}
+
+/* PutObjectsOnGrid::operator(), PutObjectsOnGrid.cpp:186-241 */
+int orc_put_objects(orc_sim* s, double dt)
+{
+  checker(s);
+  const size_t ncell = (size_t)s->nb * BS * BS;
+  #pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < ncell; ++i) {
+    s->f[ORC_CHI][i] = 0;
+    s->f[ORC_TMP][i] = -1;
+    s->f[ORC_UDEF][2 * i] = 0;
+    s->f[ORC_UDEF][2 * i + 1] = 0;
+    s->f[ORC_INVRHO][i] = 1;
+  }
+  checker(s);
+  int nSum[2] = {0, 0};
+  double uSum[2] = {0, 0};
+  for (int k = 0; k < s->nshapes; ++k) { /* Shape::updateLabVelocity, Shape.cpp:61-65 */
+    shape_t* sh = &s->shapes[k];
+    if (sh->d.bFixedx) { nSum[0]++; uSum[0] -= sh->st.u; }
+    if (sh->d.bFixedy) { nSum[1]++; uSum[1] -= sh->st.v; }
+  }
+  if (nSum[0] > 0) s->uinfx = uSum[0] / nSum[0];
+  if (nSum[1] > 0) s->uinfy = uSum[1] / nSum[1];
+  int err = 0;
+  for (int k = 0; k < s->nshapes; ++k) {
+    shape_t* sh = &s->shapes[k];
+    update_position(s, sh, dt);
+    const double* p = sh->st.center;
+    if (p[0] < 0 || p[0] > s->extents[0] || p[1] < 0 || p[1] > s->extents[1]) err = -2;
+  }
+  if (err) return err; /* reference: printf + exit(0), PutObjectsOnGrid.cpp:213-218 */
+  checker(s);
+  for (int k = 0; k < s->nshapes; ++k) disk_create(s, &s->shapes[k]);
+  checker(s);
+  for (int k = 0; k < s->nshapes; ++k) put_chi_on_grid(s, &s->shapes[k]);
+  checker(s);
+  for (int k = 0; k < s->nshapes; ++k) {
+    remove_moments(s, &s->shapes[k]);
+    put_object_vel_on_grid(s, &s->shapes[k]);
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- advDiff */
+static inline double dU_adv_dif(const double* V, const double uinf[2], double advF, double difF,
+                                int ix, int iy)
+{ /* advDiff.cpp:14-23 */
+  const double upx = LAB(V, 2, ix + 1, iy, 0), upy = LAB(V, 2, ix, iy + 1, 0);
+  const double ucc = LAB(V, 2, ix, iy, 0), vcc = LAB(V, 2, ix, iy, 1);
+  const double ulx = LAB(V, 2, ix - 1, iy, 0), uly = LAB(V, 2, ix, iy - 1, 0);
+  const double dUadv = (ucc + uinf[0]) * (upx - ulx) + (vcc + uinf[1]) * (upy - uly);
+  const double dUdif = upx + upy + ulx + uly - 4 * ucc;
+  return advF * dUadv + difF * dUdif;
+}
+static inline double dV_adv_dif(const double* V, const double uinf[2], double advF, double difF,
+                                int ix, int iy)
+{ /* advDiff.cpp:25-34 */
+  const double vpx = LAB(V, 2, ix + 1, iy, 1), vpy = LAB(V, 2, ix, iy + 1, 1);
+  const double ucc = LAB(V, 2, ix, iy, 0), vcc = LAB(V, 2, ix, iy, 1);
+  const double vlx = LAB(V, 2, ix - 1, iy, 1), vly = LAB(V, 2, ix, iy - 1, 1);
+  const double dVadv = (ucc + uinf[0]) * (vpx - vlx) + (vcc + uinf[1]) * (vpy - vly);
+  const double dVdif = vpx + vpy + vlx + vly - 4 * vcc;
+  return advF * dVadv + difF * dVdif;
+}
+
+/* advDiff::operator(), advDiff.cpp:36-118 */
+void orc_advdiff(orc_sim* s, double dt)
+{
+  checker(s);
+  const double UINF[2] = {s->uinfx, s->uinfy}, h = s->h;
+  const double dfac = (s->cfg.nu / h) * (dt / h), afac = -0.5 * dt / h;
+  const double norUinf = dmax(dmax(fabs(UINF[0]), fabs(UINF[1])), EPS);
+  const double fadeW = 1 - sq(dmax(UINF[0], 0) / norUinf) / BC_KILL_FAC;
+  const double fadeS = 1 - sq(dmax(UINF[1], 0) / norUinf) / BC_KILL_FAC;
+  const double fadeE = 1 - sq(dmin(UINF[0], 0) / norUinf) / BC_KILL_FAC;
+  const double fadeN = 1 - sq(dmin(UINF[1], 0) / norUinf) / BC_KILL_FAC;
+  #pragma omp parallel
+  {
+    double* V = (double*)malloc(sizeof(double) * LW * LW * 2);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; ++b) {
+      lab_load(s, s->f[ORC_VEL], 2, b, V);
+      double* TMP = blk(s, ORC_TMPV, b);
+      if (BX(s, b) == 0)
+        for (int iy = -1; iy <= BS; ++iy) { LAB(V, 2, -1, iy, 0) *= fadeW; LAB(V, 2, -1, iy, 1) *= fadeW; }
+      if (BY(s, b) == 0)
+        for (int ix = -1; ix <= BS; ++ix) { LAB(V, 2, ix, -1, 0) *= fadeS; LAB(V, 2, ix, -1, 1) *= fadeS; }
+      if (BX(s, b) == s->bpdx - 1)
+        for (int iy = -1; iy <= BS; ++iy) { LAB(V, 2, BS, iy, 0) *= fadeE; LAB(V, 2, BS, iy, 1) *= fadeE; }
+      if (BY(s, b) == s->bpdy - 1)
+        for (int ix = -1; ix <= BS; ++ix) { LAB(V, 2, ix, BS, 0) *= fadeN; LAB(V, 2, ix, BS, 1) *= fadeN; }
+      for (int iy = 0; iy < BS; ++iy)
+        for (int ix = 0; ix < BS; ++ix) {
+          TMP[2 * (iy * BS + ix)] = LAB(V, 2, ix, iy, 0) + dU_adv_dif(V, UINF, afac, dfac, ix, iy);
+          TMP[2 * (iy * BS + ix) + 1] = LAB(V, 2, ix, iy, 1) + dV_adv_dif(V, UINF, afac, dfac, ix, iy);
+        }
+    }
+    free(V);
+  }
+  memcpy(s->f[ORC_VEL], s->f[ORC_TMPV], sizeof(double) * (size_t)s->nb * BS * BS * 2);
+  /* boundary flux correction, advDiff.cpp:89-116 */
+  double* part = (double*)calloc((size_t)s->nbnd, sizeof(double));
+  #pragma omp parallel for schedule(static)
+  for (int k = 0; k < s->nbnd; ++k) {
+    const int b = s->bnd[k];
+    const double* V = blk(s, ORC_VEL, b);
+    double IF = 0;
+    if (BX(s, b) == 0) for (int iy = 0; iy < BS; ++iy) IF -= V[2 * (iy * BS + 0)];
+    if (BX(s, b) == s->bpdx - 1) for (int iy = 0; iy < BS; ++iy) IF += V[2 * (iy * BS + BS - 1)];
+    if (BY(s, b) == 0) for (int ix = 0; ix < BS; ++ix) IF -= V[2 * (0 * BS + ix) + 1];
+    if (BY(s, b) == s->bpdy - 1) for (int ix = 0; ix < BS; ++ix) IF += V[2 * ((BS - 1) * BS + ix) + 1];
+    part[k] = IF;
+  }
+  double IF = 0;
+  for (int k = 0; k < s->nbnd; ++k) IF += part[k];
+  free(part);
+  const double corr = IF / (2 * (BS * s->bpdy - 0) + 2 * (BS * s->bpdx - 0));
+  #pragma omp parallel for schedule(static)
+  for (int k = 0; k < s->nbnd; ++k) {
+    const int b = s->bnd[k];
+    double* V = blk(s, ORC_VEL, b);
+    if (BX(s, b) == 0) for (int iy = 0; iy < BS; ++iy) V[2 * (iy * BS + 0)] += corr;
+    if (BX(s, b) == s->bpdx - 1) for (int iy = 0; iy < BS; ++iy) V[2 * (iy * BS + BS - 1)] -= corr;
+    if (BY(s, b) == 0) for (int ix = 0; ix < BS; ++ix) V[2 * (0 * BS + ix) + 1] += corr;
+    if (BY(s, b) == s->bpdy - 1) for (int ix = 0; ix < BS; ++ix) V[2 * ((BS - 1) * BS + ix) + 1] -= corr;
+  }
+}
+
+/* --------------------------------------------------------- PressureSingle */
+/* PressureSingle::updatePressureRHS, PressureSingle.cpp:110-214 */
+void orc_pressure_rhs(orc_sim* s, double dt)
+{
+  const double h = s->h, facDiv = 0.5 * h / dt;
+  #pragma omp parallel
+  {
+    double* V = (double*)malloc(sizeof(double) * LW * LW * 2);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; ++b) {
+      lab_load(s, s->f[ORC_VEL], 2, b, V);
+      double* TMP = blk(s, ORC_TMP, b);
+      for (int iy = 0; iy < BS; ++iy)
+        for (int ix = 0; ix < BS; ++ix) {
+          const double divVx = LAB(V, 2, ix + 1, iy, 0) - LAB(V, 2, ix - 1, iy, 0);
+          const double divVy = LAB(V, 2, ix, iy + 1, 1) - LAB(V, 2, ix, iy - 1, 1);
+          TMP[iy * BS + ix] = facDiv * (divVx + divVy);
+        }
+    }
+    free(V);
+  }
+  const int ns = s->nshapes;
+  if (ns == 0) return;
+  double* part = (double*)calloc((size_t)s->nb * ns * 3, sizeof(double));
+  #pragma omp parallel
+  {
+    double* C = (double*)malloc(sizeof(double) * LW * LW);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; ++b)
+      for (int j = 0; j < ns; ++j) {
+        const oblock* o = s->shapes[j].ob[b];
+        if (!o) continue;
+        lab_load(s, s->f[ORC_CHI], 1, b, C);
+        double* TMP = blk(s, ORC_TMP, b);
+        double sum = 0, pos = 0, neg = 0;
+        for (int iy = 0; iy < BS; ++iy)
+          for (int ix = 0; ix < BS; ++ix) {
+            if (LAB(C, 1, ix, iy, 0) > o->chi[iy][ix]) continue;
+            const double gradXx = LAB(C, 1, ix + 1, iy, 0) - LAB(C, 1, ix - 1, iy, 0);
+            const double gradXy = LAB(C, 1, ix, iy + 1, 0) - LAB(C, 1, ix, iy - 1, 0);
+            const double srcBulk = -o->chi[iy][ix] * TMP[iy * BS + ix];
+            sum += srcBulk;
+            TMP[iy * BS + ix] += srcBulk;
+            const double isPerim = (gradXx * gradXx + gradXy * gradXy > 0) ? 1.0 : 0.0;
+            const double T = TMP[iy * BS + ix];
+            pos += isPerim * T * (double)(T > 0);
+            neg -= isPerim * T * (double)(T < 0);
+          }
+        double* q = part + 3 * ((size_t)b * ns + j);
+        q[0] = sum; q[1] = pos; q[2] = neg;
+      }
+    free(C);
+  }
+  double* corr = (double*)calloc((size_t)ns, sizeof(double));
+  for (int j = 0; j < ns; ++j) {
+    double sum = 0, pos = 0, neg = 0;
+    for (int b = 0; b < s->nb; ++b) {
+      const double* q = part + 3 * ((size_t)b * ns + j);
+      sum += q[0]; pos += q[1]; neg += q[2];
+    }
+    const double corrDenom = sum > 0 ? pos : neg;
+    corr[j] = sum / dmax(corrDenom, EPS);
+  }
+  free(part);
+  #pragma omp parallel
+  {
+    double* C = (double*)malloc(sizeof(double) * LW * LW);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; ++b)
+      for (int j = 0; j < ns; ++j) {
+        const oblock* o = s->shapes[j].ob[b];
+        if (!o) continue;
+        double* TMP = blk(s, ORC_TMP, b);
+        lab_load(s, s->f[ORC_CHI], 1, b, C);
+        for (int iy = 0; iy < BS; ++iy)
+          for (int ix = 0; ix < BS; ++ix) {
+            if (LAB(C, 1, ix, iy, 0) > o->chi[iy][ix]) continue;
+            const double gradXx = LAB(C, 1, ix + 1, iy, 0) - LAB(C, 1, ix - 1, iy, 0);
+            const double gradXy = LAB(C, 1, ix, iy + 1, 0) - LAB(C, 1, ix, iy - 1, 0);
+            const int isPerim = gradXx * gradXx + gradXy * gradXy > 0;
+            double* T = &TMP[iy * BS + ix];
+            if (isPerim && *T > 0 && corr[j] > 0) *T -= corr[j] * *T;
+            else if (isPerim && *T < 0 && corr[j] < 0) *T += corr[j] * *T;
+          }
+      }
+    free(C);
+  }
+  free(corr);
+}
+
+/* PoissonSolver::cub2rhs, PoissonSolver.cpp:61-141 */
+static inline double fade_arg(const orc_sim* s, const double p[2])
+{
+  const double invFadeX = 1 / dmax(s->fadeLenX, EPS), invFadeY = 1 / dmax(s->fadeLenY, EPS);
+  const double yt = invFadeY * dmax(0, s->fadeLenY - s->extents[1] + p[1]);
+  const double yb = invFadeY * dmax(0, s->fadeLenY - p[1]);
+  const double xt = invFadeX * dmax(0, s->fadeLenX - s->extents[0] + p[0]);
+  const double xb = invFadeX * dmax(0, s->fadeLenX - p[0]);
+  return dmin(dmax(dmax(dmax(yt, yb), xt), xb), 1);
+}
+static int fade_touching(const orc_sim* s, int b)
+{
+  double mn[2], mx[2];
+  bpos(s, b, 0, 0, mn);
+  bpos(s, b, BS - 1, BS - 1, mx);
+  return s->fadeLenX >= mn[0] || s->fadeLenX >= s->extents[0] - mx[0] ||
+         s->fadeLenY >= mn[1] || s->fadeLenY >= s->extents[1] - mx[1];
+}
+
+void orc_cub2rhs(orc_sim* s)
+{
+  double* part = (double*)calloc((size_t)s->nb * 3, sizeof(double));
+  #pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < s->nb; ++b) {
+    if (!fade_touching(s, b)) continue;
+    const double* B = blk(s, ORC_TMP, b);
+    double sumRHS = 0, sumPOS = 0, sumNEG = 0;
+    for (int iy = 0; iy < BS; ++iy)
+      for (int ix = 0; ix < BS; ++ix) {
+        double p[2];
+        bpos(s, b, ix, iy, p);
+        const double fa = fade_arg(s, p);
+        const double fadedRHS = B[iy * BS + ix] * (fa * fa);
+        if (fadedRHS < 0) sumNEG -= fadedRHS;
+        if (fadedRHS > 0) sumPOS += fadedRHS;
+        sumRHS += fadedRHS;
+      }
+    part[3 * b] = sumRHS; part[3 * b + 1] = sumPOS; part[3 * b + 2] = sumNEG;
+  }
+  double sumRHS = 0, sumPOS = 0, sumNEG = 0;
+  for (int b = 0; b < s->nb; ++b) { sumRHS += part[3 * b]; sumPOS += part[3 * b + 1]; sumNEG += part[3 * b + 2]; }
+  free(part);
+  sumNEG = dmax(sumNEG, EPS);
+  sumPOS = dmax(sumPOS, EPS);
+  const double corr = sumRHS > 0 ? sumRHS / sumPOS : sumRHS / sumNEG;
+  #pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < s->nb; ++b) {
+    if (!fade_touching(s, b)) continue;
+    double* B = blk(s, ORC_TMP, b);
+    for (int iy = 0; iy < BS; ++iy)
+      for (int ix = 0; ix < BS; ++ix) {
+        double p[2];
+        bpos(s, b, ix, iy, p);
+        const double fa = fade_arg(s, p);
+        const double fadeRHS = B[iy * BS + ix] * (fa * fa);
+        const double fadeCorrection = corr * (corr > 0 ? dmax(fadeRHS, 0) : -dmin(fadeRHS, 0));
+        B[iy * BS + ix] = B[iy * BS + ix] * (1 - fa * fa) + fadeCorrection;
+      }
+  }
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; ++b) {
+    const size_t blockStart = (size_t)BS * BX(s, b) + s->stride * (size_t)(BS * BY(s, b));
+    const double* B = blk(s, ORC_TMP, b);
+    for (int iy = 0; iy < BS; iy++)
+      for (int ix = 0; ix < BS; ix++) s->buffer[blockStart + ix + s->stride * iy] = B[iy * BS + ix];
+  }
+}
+
+/* PoissonSolver::sol2cub, PoissonSolver.cpp:143-168 */
+static void sol2cub(orc_sim* s)
+{
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; ++b) {
+    const size_t blockStart = (size_t)BS * BX(s, b) + s->stride * (size_t)(BS * BY(s, b));
+    double* B = blk(s, ORC_PRES, b);
+    for (int iy = 0; iy < BS; iy++)
+      for (int ix = 0; ix < BS; ix++) B[iy * BS + ix] = s->buffer[blockStart + ix + s->stride * iy];
+  }
+}
+
+/* spectral step: FFTW_periodic::_solve_spectral (FFTW_periodic.h:77-92) or
+ * the Green's function product (FFTW_freespace.cpp:252-263) */
+static void spectral(orc_sim* s)
+{
+  double* io = s->buffer;
+  const int MX = s->MX, MY = s->MY, MXh = s->MXh;
+  if (s->cfg.poisson == 0) {
+    const double waveFactX = 2 * M_PI / MX, waveFactY = 2 * M_PI / MY;
+    const double norm_factor = 1.0 / ((double)MX * MY);
+    #pragma omp parallel for schedule(static)
+    for (int j = 0; j < MY; ++j)
+      for (int i = 0; i < MXh; ++i) {
+        const int kx = i <= MX / 2 ? i : MX - i, ky = j <= MY / 2 ? j : MY - j;
+        const double rkx = kx * waveFactX, rky = ky * waveFactY;
+        const double kinv = -1 / (rkx * rkx + rky * rky);
+        io[2 * ((size_t)j * MXh + i)] *= kinv * norm_factor;
+        io[2 * ((size_t)j * MXh + i) + 1] *= kinv * norm_factor;
+      }
+    io[0] = 0; io[1] = 0;
+  } else {
+    #pragma omp parallel for schedule(static)
+    for (int j = 0; j < MY; ++j)
+      for (int i = 0; i < MXh; ++i) {
+        const size_t li = (size_t)j * MXh + i;
+        io[2 * li] *= s->kernel[li];
+        io[2 * li + 1] *= s->kernel[li];
+      }
+  }
+}
+
+static void fft_solve_buffer(orc_sim* s)
+{
+  const double t0 = omp_get_wtime();
+  fft2d_exec(&s->fft, s->buffer, -1, s->cfg.poisson == 0 ? s->MY : s->NY);
+  spectral(s);
+  fft2d_exec(&s->fft, s->buffer, +1, -1);
+  s->t_fft += omp_get_wtime() - t0;
+}
+
+/* PoissonSolver::solve (FFTW_periodic.h:135-166, FFTW_freespace.cpp:14-56) */
+void orc_poisson_solve(orc_sim* s)
+{
+  if (s->cfg.poisson != 0) memset(s->buffer, 0, sizeof(double) * s->stride * (size_t)s->MY);
+  checker(s);
+  orc_cub2rhs(s);
+  checker(s);
+  fft_solve_buffer(s);
+  checker(s); checker(s); checker(s);
+  sol2cub(s);
+}
+
+int orc_poisson_dense(orc_sim* s, const double* rhs, double* sol)
+{
+  memset(s->buffer, 0, sizeof(double) * s->stride * (size_t)s->MY);
+  for (int j = 0; j < s->NY; ++j)
+    memcpy(s->buffer + s->stride * (size_t)j, rhs + (size_t)j * s->NX, sizeof(double) * s->NX);
+  fft_solve_buffer(s);
+  for (int j = 0; j < s->NY; ++j)
+    memcpy(sol + (size_t)j * s->NX, s->buffer + s->stride * (size_t)j, sizeof(double) * s->NX);
+  return 0;
+}
+
+int orc_fft1d(double* data, int n, int sign)
+{
+  fft1d p;
+  fft1d_init(&p, n);
+  double* work = (double*)malloc(sizeof(double) * (fft1d_work_size(&p) + 2));
+  fft1d_exec(&p, data, sign, work);
+  free(work);
+  fft1d_free(&p);
+  return 0;
+}
+
+/* PressureSingle::pressureCorrection, PressureSingle.cpp:216-241 */
+void orc_pressure_correction(orc_sim* s, double dt)
+{
+  const double h = s->h, pFac = -0.5 * dt / h;
+  #pragma omp parallel
+  {
+    double* P = (double*)malloc(sizeof(double) * LW * LW);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; ++b) {
+      lab_load(s, s->f[ORC_PRES], 1, b, P);
+      double* V = blk(s, ORC_VEL, b);
+      for (int iy = 0; iy < BS; ++iy)
+        for (int ix = 0; ix < BS; ++ix) {
+          V[2 * (iy * BS + ix)] = V[2 * (iy * BS + ix)] + pFac * (LAB(P, 1, ix + 1, iy, 0) - LAB(P, 1, ix - 1, iy, 0));
+          V[2 * (iy * BS + ix) + 1] = V[2 * (iy * BS + ix) + 1] + pFac * (LAB(P, 1, ix, iy + 1, 0) - LAB(P, 1, ix, iy - 1, 0));
+        }
+    }
+    free(P);
+  }
+}
+
+/* 3x3 solve: Doolittle LU with partial pivoting (restates gsl_linalg_LU_decomp/solve) */
+static void lu3_solve(double A[3][3], const double b[3], double x[3])
+{
+  int perm[3] = {0, 1, 2};
+  for (int j = 0; j < 2; ++j) {
+    double amax = fabs(A[j][j]);
+    int ip = j;
+    for (int i = j + 1; i < 3; ++i)
+      if (fabs(A[i][j]) > amax) { amax = fabs(A[i][j]); ip = i; }
+    if (ip != j) {
+      for (int k = 0; k < 3; ++k) { const double t = A[j][k]; A[j][k] = A[ip][k]; A[ip][k] = t; }
+      const int t = perm[j]; perm[j] = perm[ip]; perm[ip] = t;
+    }
+    const double ajj = A[j][j];
+    if (ajj != 0.0)
+      for (int i = j + 1; i < 3; ++i) {
+        const double aij = A[i][j] / ajj;
+        A[i][j] = aij;
+        for (int k = j + 1; k < 3; ++k) A[i][k] -= aij * A[j][k];
+      }
+  }
+  double y[3];
+  for (int i = 0; i < 3; ++i) {
+    double v = b[perm[i]];
+    for (int k = 0; k < i; ++k) v -= A[i][k] * y[k];
+    y[i] = v;
+  }
+  for (int i = 2; i >= 0; --i) {
+    double v = y[i];
+    for (int k = i + 1; k < 3; ++k) v -= A[i][k] * x[k];
+    x[i] = v / A[i][i];
+  }
+}
+
+/* PressureSingle::integrateMomenta (PressureSingle.cpp:23-67), Shape::updateVelocity
+ * (Shape.cpp:25-59), Disk::updateVelocity (ShapesSimple.cpp:83-87),
+ * PressureSingle::preventCollidingObstacles (PressureSingle.cpp:243-344) */
+void orc_body_update(orc_sim* s, double dt)
+{
+  const double hsq = s->h * s->h;
+  const int impl = s->cfg.implicit_penal;
+  double* part = (double*)malloc(sizeof(double) * (size_t)s->nb * 7);
+  for (int k = 0; k < s->nshapes; ++k) {
+    shape_t* sh = &s->shapes[k];
+    orc_shape_state* t = &sh->st;
+    const double Cx = t->centerOfMass[0], Cy = t->centerOfMass[1];
+    memset(part, 0, sizeof(double) * (size_t)s->nb * 7);
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < s->nb; ++b) {
+      const oblock* o = sh->ob[b];
+      if (!o) continue;
+      const double* VEL = blk(s, ORC_VEL, b);
+      const double lambdt = s->cfg.lambda * dt;
+      double PM = 0, PJ = 0, PX = 0, PY = 0, UM = 0, VM = 0, AM = 0;
+      for (int iy = 0; iy < BS; ++iy)
+        for (int ix = 0; ix < BS; ++ix) {
+          if (o->chi[iy][ix] <= 0) continue;
+          const double udiff[2] = {VEL[2 * (iy * BS + ix)] - o->udef[iy][ix][0],
+                                   VEL[2 * (iy * BS + ix) + 1] - o->udef[iy][ix][1]};
+          double F;
+          if (!impl) F = hsq * o->rho[iy][ix] * o->chi[iy][ix];
+          else {
+            const double Xlamdt = o->chi[iy][ix] * lambdt;
+            F = hsq * o->rho[iy][ix] * Xlamdt / (1 + Xlamdt);
+          }
+          double p[2];
+          bpos(s, b, ix, iy, p);
+          p[0] -= Cx; p[1] -= Cy;
+          PM += F;
+          PJ += F * (p[0] * p[0] + p[1] * p[1]);
+          PX += F * p[0]; PY += F * p[1];
+          UM += F * udiff[0]; VM += F * udiff[1];
+          AM += F * (p[0] * udiff[1] - p[1] * udiff[0]);
+        }
+      double* q = part + 7 * (size_t)b;
+      q[0] = PM; q[1] = PJ; q[2] = PX; q[3] = PY; q[4] = UM; q[5] = VM; q[6] = AM;
+    }
+    double S[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < s->nb; ++b)
+      for (int i = 0; i < 7; ++i) S[i] += part[7 * (size_t)b + i];
+    t->penalM = S[0]; t->penalJ = S[1]; t->penalDX = S[2]; t->penalDY = S[3];
+    t->fluidMomX = S[4]; t->fluidMomY = S[5]; t->fluidAngMom = S[6];
+    /* Shape::updateVelocity */
+    double A[3][3] = {{t->penalM, 0, -t->penalDY}, {0, t->penalM, t->penalDX},
+                      {-t->penalDY, t->penalDX, t->penalJ}};
+    double bb[3] = {t->fluidMomX + dt * sh->appliedForceX, t->fluidMomY + dt * sh->appliedForceY,
+                    t->fluidAngMom + dt * sh->appliedTorque};
+    const double tf = sh->d.timeForced;
+    if (sh->d.bForcedx && s->time < tf) { A[0][1] = 0; A[0][2] = 0; bb[0] = t->penalM * sh->forcedu; }
+    if (sh->d.bForcedy && s->time < tf) { A[1][0] = 0; A[1][2] = 0; bb[1] = t->penalM * sh->forcedv; }
+    if (sh->d.bBlockAng && s->time < tf) { A[2][0] = 0; A[2][1] = 0; bb[2] = t->penalJ * sh->forcedomega; }
+    double x[3];
+    lu3_solve(A, bb, x);
+    if (!sh->d.bForcedy || s->time > tf) t->u = x[0]; /* quirk: Shape.cpp:56 */
+    if (!sh->d.bForcedy || s->time > tf) t->v = x[1];
+    if (!sh->d.bBlockAng || s->time > tf) t->omega = x[2];
+    if (sh->d.tAccel > 0) {
+      if (sh->d.bForcedx && s->time < sh->d.tAccel) t->u = (s->time / sh->d.tAccel) * sh->forcedu;
+      if (sh->d.bForcedy && s->time < sh->d.tAccel) t->v = (s->time / sh->d.tAccel) * sh->forcedv;
+    }
+  }
+  free(part);
+  /* preventCollidingObstacles */
+  const int N = s->nshapes;
+  if (N < 2) return;
+  double* coll = (double*)calloc((size_t)N * 10, sizeof(double));
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      if (i == j) continue;
+      double* c = coll + 10 * (size_t)i;
+      const orc_shape_state *si = &s->shapes[i].st, *sj = &s->shapes[j].st;
+      for (int b = 0; b < s->nb; ++b) {
+        const oblock *oi = s->shapes[i].ob[b], *oj = s->shapes[j].ob[b];
+        if (!oi || !oj) continue;
+        for (int iy = 0; iy < BS; ++iy)
+          for (int ix = 0; ix < BS; ++ix) {
+            if (oi->chi[iy][ix] <= 0 || oj->chi[iy][ix] <= 0) continue;
+            double pos[2];
+            bpos(s, b, ix, iy, pos);
+            const double iUr = -si->omega * (pos[1] - si->centerOfMass[1]), iVr = si->omega * (pos[0] - si->centerOfMass[0]);
+            const double jUr = -sj->omega * (pos[1] - sj->centerOfMass[1]), jVr = sj->omega * (pos[0] - sj->centerOfMass[0]);
+            const double X = oi->chi[iy][ix], Y = oj->chi[iy][ix];
+            c[0] += X; c[1] += X * pos[0]; c[2] += X * pos[1];
+            c[3] += X * (si->u + iUr + oi->udef[iy][ix][0]);
+            c[4] += X * (si->v + iVr + oi->udef[iy][ix][1]);
+            c[5] += Y; c[6] += Y * pos[0]; c[7] += Y * pos[1];
+            c[8] += Y * (sj->u + jUr + oj->udef[iy][ix][0]);
+            c[9] += Y * (sj->v + jVr + oj->udef[iy][ix][1]);
+          }
+      }
+    }
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      if (i == j) continue;
+      const double* c = coll + 10 * (size_t)i;
+      if (c[0] < 1 || c[5] < 1) continue;
+      const double iPX = c[1] / c[0], iPY = c[2] / c[0], iUX = c[3] / c[0], iUY = c[4] / c[0];
+      const double jPX = c[6] / c[5], jPY = c[7] / c[5], jUX = c[8] / c[5], jUY = c[9] / c[5];
+      const double CX = (iPX + jPX) / 2, CY = (iPY + jPY) / 2;
+      const double dirX = iPX - jPX, dirY = iPY - jPY;
+      const double hitVelX = jUX - iUX, hitVelY = jUY - iUY;
+      const double normF = dmax(sqrt(dirX * dirX + dirY * dirY), EPS);
+      const double NX = dirX / normF, NY = dirY / normF;
+      const double projVel = hitVelX * NX + hitVelY * NY;
+      if (projVel <= 0) continue;
+      shape_t *shi = &s->shapes[i], *shj = &s->shapes[j];
+      const int iForcedX = shi->d.bForcedx && s->time < shi->d.timeForced;
+      const int iForcedY = shi->d.bForcedy && s->time < shi->d.timeForced;
+      const int iForcedA = shi->d.bBlockAng && s->time < shi->d.timeForced;
+      const int jForcedX = shj->d.bForcedx && s->time < shj->d.timeForced;
+      const int jForcedY = shj->d.bForcedy && s->time < shj->d.timeForced;
+      const double iInvMassX = iForcedX ? 0 : 1 / shi->st.M, iInvMassY = iForcedY ? 0 : 1 / shi->st.M;
+      const double jInvMassX = jForcedX ? 0 : 1 / shj->st.M, jInvMassY = jForcedY ? 0 : 1 / shj->st.M;
+      const double iInvJ = iForcedA ? 0 : 1 / shi->st.J;
+      const double meanMassX = 2 / dmax(iInvMassX + jInvMassX, EPS);
+      const double meanMassY = 2 / dmax(iInvMassY + jInvMassY, EPS);
+      const double FXdt = meanMassX * projVel * NX, FYdt = meanMassY * projVel * NY;
+      shi->st.u += FXdt * iInvMassX;
+      shi->st.v += FYdt * iInvMassY;
+      const double iCx = shi->st.centerOfMass[0], iCy = shi->st.centerOfMass[1];
+      const double RcrossF = (CX - iCx) * FYdt - (CY - iCy) * FXdt;
+      shi->st.omega += iInvJ * RcrossF;
+    }
+  free(coll);
+}
+
+/* PressureSingle::penalize, PressureSingle.cpp:69-108 */
+void orc_penalize(orc_sim* s, double dt)
+{
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int b = 0; b < s->nb; ++b)
+    for (int k = 0; k < s->nshapes; ++k) {
+      const shape_t* sh = &s->shapes[k];
+      const oblock* o = sh->ob[b];
+      if (!o) continue;
+      const double u_s = sh->st.u, v_s = sh->st.v, omega_s = sh->st.omega;
+      const double Cx = sh->st.centerOfMass[0], Cy = sh->st.centerOfMass[1];
+      const double* CHI = blk(s, ORC_CHI, b);
+      double* V = blk(s, ORC_VEL, b);
+      for (int iy = 0; iy < BS; ++iy)
+        for (int ix = 0; ix < BS; ++ix) {
+          if (CHI[iy * BS + ix] > o->chi[iy][ix]) continue;
+          if (o->chi[iy][ix] <= 0) continue;
+          double p[2];
+          bpos(s, b, ix, iy, p);
+          p[0] -= Cx; p[1] -= Cy;
+          const double alpha = s->cfg.implicit_penal ? 1 / (1 + s->cfg.lambda * dt * o->chi[iy][ix])
+                                                     : 1 - o->chi[iy][ix];
+          const double US = u_s - omega_s * p[1] + o->udef[iy][ix][0];
+          const double VS = v_s + omega_s * p[0] + o->udef[iy][ix][1];
+          V[2 * (iy * BS + ix)] = alpha * V[2 * (iy * BS + ix)] + (1 - alpha) * US;
+          V[2 * (iy * BS + ix) + 1] = alpha * V[2 * (iy * BS + ix) + 1] + (1 - alpha) * VS;
+        }
+    }
+}
+
+/* PressureSingle::operator(), PressureSingle.cpp:346-373 */
+void orc_pressure_single(orc_sim* s, double dt)
+{
+  checker(s);
+  orc_pressure_rhs(s, dt);
+  orc_poisson_solve(s);
+  checker(s);
+  orc_pressure_correction(s, dt);
+  checker(s);
+  orc_body_update(s, dt);
+  checker(s);
+  orc_penalize(s, dt);
+}
+
+/* ComputeForces::operator() (ComputeForces.cpp:16-111), Shape::computeForces
+ * (Shape.cpp:252-281) */
+void orc_compute_forces(orc_sim* s)
+{
+  const double NUoH = s->cfg.nu / s->h;
+  for (int k = 0; k < s->nshapes; ++k) {
+    shape_t* sh = &s->shapes[k];
+    orc_shape_state* t = &sh->st;
+    const double Cx = t->centerOfMass[0], Cy = t->centerOfMass[1];
+    const double vel_norm = sqrt(t->u * t->u + t->v * t->v);
+    const double vel_unit[2] = {vel_norm > 0 ? t->u / vel_norm : 0, vel_norm > 0 ? t->v / vel_norm : 0};
+    #pragma omp parallel
+    {
+      double* V = (double*)malloc(sizeof(double) * LW * LW * 2);
+      double* P = (double*)malloc(sizeof(double) * LW * LW);
+      #pragma omp for schedule(static)
+      for (int b = 0; b < s->nb; ++b) {
+        oblock* O = sh->ob[b];
+        if (!O) continue;
+        lab_load(s, s->f[ORC_VEL], 2, b, V);
+        lab_load(s, s->f[ORC_PRES], 1, b, P);
+        O->perimeter = O->forcex = O->forcey = O->forcex_P = O->forcey_P = 0;
+        O->forcex_V = O->forcey_V = O->torque = O->torque_P = O->torque_V = 0;
+        O->drag = O->thrust = O->Pout = O->PoutBnd = O->defPower = O->defPowerBnd = 0;
+        O->circulation = 0;
+        for (int q = 0; q < O->nsurf; ++q) {
+          const int ix = O->sx[q], iy = O->sy[q];
+          double p[2];
+          bpos(s, b, ix, iy, p);
+          const double D11 = NUoH * (LAB(V, 2, ix + 1, iy, 0) - LAB(V, 2, ix - 1, iy, 0));
+          const double D22 = NUoH * (LAB(V, 2, ix, iy + 1, 1) - LAB(V, 2, ix, iy - 1, 1));
+          const double D12 = NUoH / 2 * (LAB(V, 2, ix, iy + 1, 0) - LAB(V, 2, ix, iy - 1, 0) +
+                                         LAB(V, 2, ix + 1, iy, 1) - LAB(V, 2, ix - 1, iy, 1));
+          const double normX = O->dchidx[q], normY = O->dchidy[q];
+          const double Pc = LAB(P, 1, ix, iy, 0);
+          const double fXV = D11 * normX + D12 * normY, fXP = -Pc * normX;
+          const double fYV = D12 * normX + D22 * normY, fYP = -Pc * normY;
+          const double fXT = fXV + fXP, fYT = fYV + fYP;
+          const double vx = LAB(V, 2, ix, iy, 0), vy = LAB(V, 2, ix, iy, 1);
+          const double vxDef = O->udef[iy][ix][0], vyDef = O->udef[iy][ix][1];
+          O->perimeter += sqrt(normX * normX + normY * normY);
+          O->circulation += normX * vy - normY * vx;
+          O->forcex += fXT; O->forcey += fYT;
+          O->forcex_V += fXV; O->forcey_V += fYV;
+          O->forcex_P += fXP; O->forcey_P += fYP;
+          O->torque += (p[0] - Cx) * fYT - (p[1] - Cy) * fXT;
+          O->torque_P += (p[0] - Cx) * fYP - (p[1] - Cy) * fXP;
+          O->torque_V += (p[0] - Cx) * fYV - (p[1] - Cy) * fXV;
+          const double forcePar = fXT * vel_unit[0] + fYT * vel_unit[1];
+          O->thrust += .5 * (forcePar + fabs(forcePar));
+          O->drag -= .5 * (forcePar - fabs(forcePar));
+          const double powOut = fXT * vx + fYT * vy;
+          const double powDef = fXT * vxDef + fYT * vyDef;
+          O->Pout += powOut;
+          O->defPower += powDef;
+          O->PoutBnd += dmin(0, powOut);
+          O->defPowerBnd += dmin(0, powDef);
+        }
+      }
+      free(V); free(P);
+    }
+    /* Shape::computeForces */
+    t->perimeter = t->forcex = t->forcey = t->forcex_P = t->forcey_P = 0;
+    t->forcex_V = t->forcey_V = t->torque = t->torque_P = t->torque_V = 0;
+    t->drag = t->thrust = t->Pout = t->PoutBnd = t->defPower = t->defPowerBnd = 0;
+    t->circulation = 0;
+    for (int b = 0; b < s->nb; ++b) {
+      const oblock* O = sh->ob[b];
+      if (!O) continue;
+      t->circulation += O->circulation;
+      t->perimeter += O->perimeter; t->torque += O->torque;
+      t->forcex += O->forcex; t->forcey += O->forcey;
+      t->forcex_P += O->forcex_P; t->forcey_P += O->forcey_P;
+      t->forcex_V += O->forcex_V; t->forcey_V += O->forcey_V;
+      t->torque_P += O->torque_P; t->torque_V += O->torque_V;
+      t->drag += O->drag; t->thrust += O->thrust;
+      t->Pout += O->Pout; t->defPowerBnd += O->defPowerBnd;
+      t->PoutBnd += O->PoutBnd; t->defPower += O->defPower;
+    }
+    t->Pthrust = t->thrust * sqrt(t->u * t->u + t->v * t->v);
+    t->Pdrag = t->drag * sqrt(t->u * t->u + t->v * t->v);
+    const double denUnb = t->Pthrust - dmin(t->defPower, 0);
+    const double demBnd = t->Pthrust - t->defPowerBnd;
+    t->EffPDef = t->Pthrust / dmax(denUnb, EPS);
+    t->EffPDefBnd = t->Pthrust / dmax(demBnd, EPS);
+  }
+}
+
+/* Simulation::reset, Simulation.cpp:232-241 */
+void orc_reset(orc_sim* s)
+{
+  for (int k = 0; k < s->nshapes; ++k) shape_reset(s, &s->shapes[k]);
+  s->time = 0; s->step = 0; s->uinfx = 0; s->uinfy = 0;
+  ic(s);
+  orc_put_objects(s, 0);
+  orc_apply_obj_vel(s);
+  s->dt = 0;
+}
+
+/* Simulation::advance, Simulation.cpp:279-305 */
+int orc_advance(orc_sim* s, double dt)
+{
+  s->dt = dt;
+  const int err = orc_put_objects(s, dt);
+  if (err) return err;
+  orc_advdiff(s, dt);
+  orc_pressure_single(s, dt);
+  orc_compute_forces(s);
+  s->time += s->dt;
+  s->step++;
+  checker(s); /* startProfiler("Dump") */
+  return 0;
+}
+
+int orc_simulate(orc_sim* s, int nsteps)
+{
+  s->t_fft = 0;
+  const double t0 = omp_get_wtime();
+  for (int n = 0; n < nsteps; ++n) {
+    const double dt = orc_calc_max_timestep(s);
+    const int err = orc_advance(s, dt);
+    if (err) { s->t_total = omp_get_wtime() - t0; return err; }
+  }
+  s->t_total = omp_get_wtime() - t0;
+  return 0;
+}

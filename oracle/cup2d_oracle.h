@@ -1,0 +1,104 @@
+/*
+ * cup2d_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 C/OpenMP restatement of CubismUP_2D's per-timestep hot path on the
+ * uniform block grid (constant density, PressureSingle pipeline).  It is the
+ * checker for the HIP library and the timed CPU baseline; it is never linked
+ * into, called by, or shipped with the product library.
+ *
+ * Fields are stored exactly like the reference: blocks of 32x32 elements in
+ * blockID order (blockID = bx + bpdx*by), elements row-major data[iy][ix],
+ * vector components interleaved (Definitions.h:51-56,84).
+ */
+#ifndef CUP2D_ORACLE_H
+#define CUP2D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_BS 32
+
+typedef struct {
+  int bpdx, bpdy;        /* blocks per dimension (Simulation.cpp:73-74)      */
+  double extent;         /* -extent (Simulation.cpp:76)                      */
+  double nu;             /* -nu                                              */
+  double CFL;            /* -CFL                                             */
+  double lambda;         /* -lambda (read only by the implicit penalization) */
+  double fadeLen;        /* -fadeLen, fraction of extent (Simulation.cpp:97) */
+  int poisson;           /* 0 = periodic (FFTW_periodic), 1 = freespace      */
+  int fs_pad2;           /* freespace: 0 = (2N-1) cyclic like the reference, */
+                         /*            1 = 2N cyclic (same math)             */
+  int implicit_penal;    /* 0 = EXPL_INTEGRATE_MOM (reference), 1 = implicit */
+  int checker;           /* 1 = run the NaN/Inf Checker 17x per step        */
+} orc_cfg;
+
+typedef struct {
+  int kind;              /* 0 = disk                                         */
+  double radius, xpos, ypos, angle_deg, rhoS;
+  int bFixedx, bFixedy, bForcedx, bForcedy, bBlockAng;
+  double xvel, yvel, angvel, timeForced, tAccel;
+} orc_shape_desc;
+
+/* Per-shape scalar state; same field order as cup2d_shape_state. */
+typedef struct {
+  double center[2], centerOfMass[2], labCenterOfMass[2], d_gm[2], orientation;
+  double M, J, u, v, omega;
+  double fluidAngMom, fluidMomX, fluidMomY, penalDX, penalDY, penalM, penalJ;
+  double perimeter, forcex, forcey, forcex_P, forcey_P, forcex_V, forcey_V;
+  double torque, torque_P, torque_V, drag, thrust, circulation;
+  double Pout, PoutBnd, defPower, defPowerBnd, Pthrust, Pdrag, EffPDef, EffPDefBnd;
+  double n_obstacle_blocks, n_surface_points;
+} orc_shape_state;
+
+enum { ORC_VEL = 0, ORC_CHI, ORC_PRES, ORC_TMP, ORC_TMPV, ORC_UDEF, ORC_INVRHO,
+       ORC_NFIELDS };
+
+typedef struct orc_sim orc_sim;
+
+orc_sim* orc_create(const orc_cfg* cfg);
+void orc_destroy(orc_sim* s);
+int  orc_add_shape(orc_sim* s, const orc_shape_desc* d);
+int  orc_ncomp(int field);
+/* block-layout host arrays: nblocks*32*32*ncomp doubles */
+int  orc_set_field(orc_sim* s, int field, const double* src);
+int  orc_get_field(const orc_sim* s, int field, double* dst);
+void orc_get_scalars(const orc_sim* s, double* time, double* dt, int* step,
+                     double uinf[2], double* h);
+void orc_set_scalars(orc_sim* s, double time, int step, double uinfx, double uinfy);
+int  orc_get_shape(const orc_sim* s, int i, orc_shape_state* out);
+
+/* Operators: same names and order as the reference pipeline. */
+void   orc_reset(orc_sim* s);                 /* Simulation::reset           */
+double orc_find_max_u(orc_sim* s);            /* findMaxU::run               */
+double orc_calc_max_timestep(orc_sim* s);     /* Simulation::calcMaxTimestep */
+int    orc_put_objects(orc_sim* s, double dt);/* PutObjectsOnGrid            */
+void   orc_advdiff(orc_sim* s, double dt);    /* advDiff                     */
+void   orc_pressure_rhs(orc_sim* s, double dt);
+void   orc_cub2rhs(orc_sim* s);               /* fade + copy to FFT buffer   */
+void   orc_poisson_solve(orc_sim* s);         /* PoissonSolver::solve        */
+void   orc_pressure_correction(orc_sim* s, double dt);
+void   orc_body_update(orc_sim* s, double dt);/* integrateMomenta+update+coll*/
+void   orc_penalize(orc_sim* s, double dt);
+void   orc_pressure_single(orc_sim* s, double dt);
+void   orc_compute_forces(orc_sim* s);
+int    orc_advance(orc_sim* s, double dt);    /* Simulation::advance         */
+int    orc_simulate(orc_sim* s, int nsteps);  /* nsteps x (dt + advance)     */
+void   orc_apply_obj_vel(orc_sim* s);
+
+/* Poisson solve on a dense NYxNX rhs (row-major), writes dense solution.
+ * Same arithmetic as orc_poisson_solve without the boundary fade. */
+int orc_poisson_dense(orc_sim* s, const double* rhs, double* sol);
+
+/* 1D FFT used by the solver (for tests): in-place complex, sign -1 forward,
+ * +1 backward, unnormalized; interleaved re/im. */
+int orc_fft1d(double* data, int n, int sign);
+
+/* timing of the last simulate() call, seconds: total and FFT-only */
+void orc_get_timing(const orc_sim* s, double* total, double* fft);
+int  orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,226 @@
+"""Pins the oracle (CPU restatement of the reference) before it is trusted.
+
+Checks: numpy.fft and closed-form Fourier modes (FFT, periodic Poisson), the
+direct-sum free-space convolution, bit-exact numpy restatements of the
+stencils on the dense layout (pins the block/ghost indexing), geometric
+invariants of the Towers chi, Cramer's rule for the body solve, and whole-step
+sanity on the reference's cylinder case.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import numpy_ref as R
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_fft1d_matches_numpy(oracle_mod):
+    rng = np.random.default_rng(1)
+    for n in [2, 8, 64, 1024, 96, 511, 1023, 127]:  # radix-2 and Bluestein lengths
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        ref = np.fft.fft(x)
+        scale = np.abs(ref).max()
+        assert np.abs(oracle_mod.fft1d(x, -1) - ref).max() / scale < 5e-15
+        assert np.abs(oracle_mod.fft1d(x, +1) - np.fft.ifft(x) * n).max() / scale < 5e-15
+
+
+def test_periodic_poisson_fourier_mode(oracle_mod):
+    """cos modes are eigenvectors of the spectral solve: p = -r/(kx'^2+ky'^2)."""
+    o = oracle_mod.Oracle(4, 2, poisson="periodic")
+    NY, NX = o.NY, o.NX
+    x, y = np.meshgrid(np.arange(NX), np.arange(NY))
+    for kx, ky in [(1, 0), (3, 2), (NX // 2, 1), (5, NY // 2)]:
+        r = np.cos(2 * np.pi * (kx * x / NX + ky * y / NY))
+        p = o.poisson_dense(r)
+        lam = (2 * np.pi * kx / NX) ** 2 + (2 * np.pi * ky / NY) ** 2
+        assert np.abs(p + r / lam).max() < 1e-12 / lam
+
+
+def test_periodic_poisson_golden(oracle_mod):
+    g = np.load(os.path.join(GOLD, "periodic_poisson_128x64.npz"))
+    o = oracle_mod.Oracle(4, 2, poisson="periodic")
+    p = o.poisson_dense(g["rhs"])
+    assert np.linalg.norm(p - g["p"]) / np.linalg.norm(g["p"]) < 1e-14
+
+
+@pytest.mark.parametrize("pad2", [False, True])
+def test_freespace_poisson_direct_sum(oracle_mod, pad2):
+    """(2N-1) cyclic (reference) and 2N cyclic both equal the linear convolution."""
+    g = np.load(os.path.join(GOLD, "freespace_direct_64x32.npz"))
+    o = oracle_mod.Oracle(2, 1, extent=1.0, poisson="freespace", fs_pad2=pad2)
+    assert abs(o.scalars()["h"] - float(g["h"])) == 0
+    p = o.poisson_dense(g["rhs"])
+    assert np.linalg.norm(p - g["p"]) / np.linalg.norm(g["p"]) < 1e-13
+
+
+def test_block_layout_roundtrip(oracle_mod):
+    rng = np.random.default_rng(2)
+    o = oracle_mod.Oracle(3, 2)
+    d = rng.standard_normal((2, o.NY, o.NX))
+    o.set_field(oracle_mod.VEL, d)
+    blocks = o.get_field(oracle_mod.VEL, dense=False)
+    # block b=bx+bpdx*by, element (iy,ix), comp c at ((b*32+iy)*32+ix)*2+c
+    bx, by, iy, ix, c = 2, 1, 5, 7, 1
+    b = bx + 3 * by
+    assert blocks[((b * 32 + iy) * 32 + ix) * 2 + c] == d[c, by * 32 + iy, bx * 32 + ix]
+    assert np.array_equal(o.get_field(oracle_mod.VEL), d)
+
+
+def test_advdiff_golden_bitexact(oracle_mod):
+    g = np.load(os.path.join(GOLD, "advdiff_96x64.npz"))
+    NY, NX = g["u"].shape
+    o = oracle_mod.Oracle(NX // 32, NY // 32, extent=1.0, nu=float(g["nu"]))
+    assert o.scalars()["h"] == float(g["h"])
+    o.set_field(oracle_mod.VEL, np.stack([g["u"], g["v"]]))
+    o.set_scalars(uinf=tuple(g["uinf"]))
+    o.advdiff(float(g["dt"]))
+    out = o.get_field(oracle_mod.VEL)
+    interior = (slice(1, -1), slice(1, -1))
+    assert np.array_equal(out[0][interior], g["u1"][interior])
+    assert np.array_equal(out[1][interior], g["v1"][interior])
+    # boundary cells carry the flux correction (sum order differs at round-off)
+    assert np.abs(out[0] - g["u1"]).max() < 1e-15
+    assert np.abs(out[1] - g["v1"]).max() < 1e-15
+
+
+def test_advdiff_linear_closed_form(oracle_mod):
+    """u = c1*x (cells), v = 0, uinf = 0: interior u* = u - dt*c1*u exactly
+    up to round-off (central difference of a linear field is exact)."""
+    o = oracle_mod.Oracle(2, 2, extent=1.0, nu=0.3)
+    h = o.scalars()["h"]
+    x = (np.arange(o.NX) + 0.5) * h
+    u = np.tile(0.7 * x, (o.NY, 1))
+    o.set_field(oracle_mod.VEL, np.stack([u, np.zeros_like(u)]))
+    dt = 1e-3
+    o.advdiff(dt)
+    out = o.get_field(oracle_mod.VEL)
+    ref = u - dt * 0.7 * u
+    assert np.abs(out[0][2:-2, 2:-2] - ref[2:-2, 2:-2]).max() < 1e-14
+
+
+def test_rhs_and_projection_bitexact(oracle_mod):
+    rng = np.random.default_rng(3)
+    o = oracle_mod.Oracle(3, 2, extent=1.0)
+    h = o.scalars()["h"]
+    u, v = rng.standard_normal((2, o.NY, o.NX))
+    p = rng.standard_normal((o.NY, o.NX))
+    dt = 2e-3
+    o.set_field(oracle_mod.VEL, np.stack([u, v]))
+    o.pressure_rhs(dt)
+    assert np.array_equal(o.get_field(oracle_mod.TMP)[0], R.pressure_rhs(u, v, dt, h))
+    o.set_field(oracle_mod.PRES, p[None])
+    o.pressure_correction(dt)
+    u1, v1 = R.pressure_correction(u, v, p, dt, h)
+    out = o.get_field(oracle_mod.VEL)
+    assert np.array_equal(out[0], u1) and np.array_equal(out[1], v1)
+
+
+def test_rhs_linear_closed_form(oracle_mod):
+    o = oracle_mod.Oracle(2, 2, extent=1.0)
+    h = o.scalars()["h"]
+    X, Y = np.meshgrid((np.arange(o.NX) + 0.5) * h, (np.arange(o.NY) + 0.5) * h)
+    a, b, dt = 0.3, -0.8, 1e-2
+    o.set_field(oracle_mod.VEL, np.stack([a * X, b * Y]))
+    o.pressure_rhs(dt)
+    r = o.get_field(oracle_mod.TMP)[0]
+    assert np.abs(r[1:-1, 1:-1] - h * h * (a + b) / dt).max() < 1e-12
+
+
+def test_cub2rhs_fade_matches_numpy(oracle_mod):
+    rng = np.random.default_rng(4)
+    o = oracle_mod.Oracle(4, 2, extent=1.0, fadeLen=0.05, poisson="periodic")
+    r = rng.standard_normal((o.NY, o.NX))
+    o.set_field(oracle_mod.TMP, r[None])
+    o.cub2rhs()
+    out = o.get_field(oracle_mod.TMP)[0]
+    ref = R.cub2rhs(r, 1.0 / 4, 1.0 / 128, 1.0, 0.05)
+    assert np.abs(out - ref).max() < 1e-14
+    # the correction preserves the sum of the RHS inside the band
+    assert abs(out.sum() - r.sum()) < 1e-10
+
+
+def test_disk_chi_invariants(oracle_mod):
+    """Towers chi: 0<=chi<=1, area -> pi r^2, perimeter -> 2 pi r, COM -> centre."""
+    for n, tol in [(8, 3e-3), (16, 8e-4)]:
+        rad, cx, cy = 0.13, 0.41, 0.52
+        o = oracle_mod.Oracle(n, n, extent=1.0, shapes=[oracle_mod.disk(rad, cx, cy)])
+        o.reset()
+        o.compute_forces()
+        chi = o.get_field(oracle_mod.CHI)[0]
+        h = o.scalars()["h"]
+        s = o.shape()
+        assert chi.min() >= 0 and chi.max() <= 1 + 1e-15
+        assert abs(chi.sum() * h * h - np.pi * rad ** 2) / (np.pi * rad ** 2) < tol
+        assert abs(s["M"] - np.pi * rad ** 2) / (np.pi * rad ** 2) < tol
+        assert abs(s["perimeter"] - 2 * np.pi * rad) / (2 * np.pi * rad) < 10 * tol
+        assert abs(s["centerOfMass"][0] - cx) < h * 1e-2 and abs(s["centerOfMass"][1] - cy) < h * 1e-2
+        assert abs(s["J"] - np.pi * rad ** 4 / 2) / (np.pi * rad ** 4 / 2) < 3 * tol
+
+
+def test_forces_divergence_theorem(oracle_mod):
+    """-sum p n dS: p = const gives ~0, p = x gives -(area, 0) (Towers normals)."""
+    rad = 0.2
+    o = oracle_mod.Oracle(8, 8, extent=1.0, nu=1e-3,
+                          shapes=[oracle_mod.disk(rad, 0.5, 0.5)])
+    o.reset()
+    h = o.scalars()["h"]
+    X, _ = np.meshgrid((np.arange(o.NX) + 0.5) * h, (np.arange(o.NY) + 0.5) * h)
+    o.set_field(oracle_mod.VEL, np.zeros((2, o.NY, o.NX)))
+    o.set_field(oracle_mod.PRES, np.ones((1, o.NY, o.NX)))
+    o.compute_forces()
+    s = o.shape()
+    assert abs(s["forcex_P"]) < 1e-12 and abs(s["forcey_P"]) < 1e-12
+    o.set_field(oracle_mod.PRES, X[None])
+    o.compute_forces()
+    s = o.shape()
+    area = np.pi * rad ** 2
+    assert abs(s["forcex_P"] + area) / area < 2e-2
+    assert abs(s["forcey_P"]) < 1e-3 * area
+
+
+def test_body_solve_cramer(oracle_mod):
+    """A free disk in a uniform flow: the 3x3 LU solve equals Cramer's rule."""
+    o = oracle_mod.Oracle(4, 4, extent=1.0, nu=1e-3,
+                          shapes=[oracle_mod.disk(0.1, 0.45, 0.55, rhoS=2.0)])
+    o.reset()
+    rng = np.random.default_rng(5)
+    o.set_field(oracle_mod.VEL, 0.1 + 0.01 * rng.standard_normal((2, o.NY, o.NX)))
+    o.body_update(1e-3)
+    s = o.shape()
+    A = np.array([[s["penalM"], 0, -s["penalDY"]], [0, s["penalM"], s["penalDX"]],
+                  [-s["penalDY"], s["penalDX"], s["penalJ"]]])
+    b = np.array([s["fluidMomX"], s["fluidMomY"], s["fluidAngMom"]])
+    x = [np.linalg.det(np.column_stack([b if k == j else A[:, k] for k in range(3)])) /
+         np.linalg.det(A) for j in range(3)]
+    assert np.allclose([s["u"], s["v"], s["omega"]], x, rtol=1e-12, atol=1e-15)
+
+
+def test_towed_cylinder_sanity(oracle_mod):
+    """Reference cylinder case (free-space, forced, fixed): first steps."""
+    o = oracle_mod.Oracle(8, 4, extent=1.0, nu=2e-5, CFL=0.1, poisson="freespace",
+                          shapes=[oracle_mod.disk(0.05, 0.25, 0.25, xvel=0.2,
+                                                  bFixed=True, bForced=True)])
+    o.reset()
+    sc = o.scalars()
+    assert sc["uinf"] == (0.2, 0.0)
+    dt0 = o.calc_max_timestep()
+    h = sc["h"]
+    ramp = np.exp(np.log(1e-3) * 0.99 + np.log(0.1) * 0.01)
+    assert dt0 == ramp * min(h / 0.2, h * h / 2e-5)
+    for _ in range(10):
+        o.advance(o.calc_max_timestep())
+    s = o.shape()
+    assert s["u"] == -0.2 and s["v"] == 0.0 and s["omega"] == 0.0
+    assert s["drag"] > 0 and s["forcex"] > 0
+    assert o.scalars()["step"] == 10
+    v = o.get_field(oracle_mod.VEL)
+    assert np.isfinite(v).all() and np.abs(v).max() < 1.0
+
+
+def test_body_out_of_domain_is_an_error(oracle_mod):
+    o = oracle_mod.Oracle(2, 2, extent=1.0, shapes=[oracle_mod.disk(0.05, 0.05, 0.5, xvel=5.0,
+                                                                    bForced=True)])
+    o.reset()
+    assert o.put_objects(1.0) == -2
