@@ -1,0 +1,170 @@
+"""Benchmark of the CubismUP_2D per-timestep hot path on one MI355X.
+
+Default workload: periodic 2.0 x 1.0 domain at 4096 x 2048 cells (128 x 64
+blocks of 32x32), nu = 1e-5, CFL 0.1, FP64, initial velocity from a seeded
+Gaussian random vorticity (sigma 1e-3) made divergence-free by one projection,
+one free disk (r = 0.04) as the immersed body.  A step is the reference's
+calcMaxTimestep + advance: PutObjectsOnGrid, advDiff, PressureSingle (RHS,
+FFT Poisson, projection, body update, penalization) and ComputeForces, all
+enqueued on the device with dt computed on the device.
+
+Prints one JSON line (metric, value = Mcell-steps/s over the K timed steps,
+roofline of the dominant kernel measured live with HIP events, and a CPU
+baseline: the FP64 OpenMP restatement timed on the host on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+# algorithmic HBM bytes per cell and launch (s = 8 bytes, FP64), periodic solver
+BALG = {"advdiff": 4, "pressure_rhs": 3, "fft_rows_fwd": 2, "fft_cols": 2, "fft_rows_inv": 2,
+        "pressure_correction": 5, "find_max_u": 2}
+FULL_GRID = list(BALG)
+
+CONFIGS = {
+    # name: (bpdx, bpdy, extent, nu, poisson, body)
+    "periodic4096": dict(bpdx=128, bpdy=64, extent=2.0, nu=1e-5, poisson="periodic",
+                         body=dict(radius=0.04, xpos=1.0, ypos=0.5)),
+    "cylinder512": dict(bpdx=16, bpdy=8, extent=1.0, nu=2e-5, poisson="freespace",
+                        body=dict(radius=0.05, xpos=0.25, ypos=0.25, xvel=0.2, bFixed=True,
+                                  bForced=True)),
+}
+
+
+def random_vorticity_velocity(NX, NY, h, sigma=1e-3, seed=12345):
+    """Seeded Gaussian vorticity -> periodic streamfunction -> central-difference u."""
+    rng = np.random.default_rng(seed)
+    w = rng.normal(0.0, sigma, (NY, NX))
+    W = np.fft.rfft2(w)
+    kx = 2 * np.pi * np.fft.rfftfreq(NX)[None, :] / h
+    ky = 2 * np.pi * np.fft.fftfreq(NY)[:, None] / h
+    k2 = kx * kx + ky * ky
+    k2[0, 0] = 1.0
+    P = W / k2
+    P[0, 0] = 0
+    psi = np.fft.irfft2(P, s=(NY, NX))
+    u = (np.roll(psi, -1, axis=0) - np.roll(psi, 1, axis=0)) / (2 * h)
+    v = -(np.roll(psi, -1, axis=1) - np.roll(psi, 1, axis=1)) / (2 * h)
+    return np.stack([u, v])
+
+
+def setup(sim, vel):
+    sim.reset()
+    sim.set_field(0, vel)
+    sim.pressure_rhs(1.0)  # one projection makes the IC discretely divergence-free
+    sim.poisson_solve()
+    sim.pressure_correction(1.0)
+
+
+def cpu_baseline(cfg, vel, steps):
+    import oracle
+    o = oracle.Oracle(cfg["bpdx"], cfg["bpdy"], extent=cfg["extent"], nu=cfg["nu"], CFL=0.1,
+                      poisson=cfg["poisson"], fs_pad2=True,
+                      shapes=[oracle.disk(**cfg["body"])])
+    setup(o, vel)
+    o.simulate(2)  # warm-up
+    t0 = time.perf_counter()
+    o.simulate(steps)
+    t1 = time.perf_counter()
+    total, fft = o.timing()
+    cells = cfg["bpdx"] * cfg["bpdy"] * 1024
+    return dict(value=cells * steps / (t1 - t0) / 1e6, unit="Mcell-steps/s",
+                cores=oracle.num_threads(), kind="port",
+                sample=f"{steps} steps of the same workload after 2 warm-up steps, "
+                       f"FP64 OpenMP restatement without the NaN Checker; "
+                       f"FFT share {fft / total:.2f}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--config", default="periodic4096", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus must be 1 "
+              "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import cubismup_2d_amd as cup
+    cfg = CONFIGS[args.config]
+    sim = cup.Simulation(cfg["bpdx"], cfg["bpdy"], extent=cfg["extent"], nu=cfg["nu"], CFL=0.1,
+                         poisson=cfg["poisson"], shapes=[cup.disk(**cfg["body"])])
+    NX, NY = sim.NX, sim.NY
+    h = sim.scalars()["h"]
+    vel = random_vorticity_velocity(NX, NY, h)
+    setup(sim, vel)
+    cells = NX * NY
+    balg = dict(BALG)
+    if cfg["poisson"] == "freespace":
+        balg.update(fft_rows_fwd=3, fft_cols=5, fft_rows_inv=3)
+
+    # warm-up with every full-grid timer on: picks the dominant kernel
+    sim.timer_enable("*", 0)
+    for k in FULL_GRID:
+        sim.timer_enable(k, max(2 * args.warmup, 16))
+    sim.simulate(args.warmup)
+    per = {k: sim.timer_read(k) for k in FULL_GRID}
+    dominant = max(per, key=lambda k: per[k][0])
+    sim.timer_enable("*", 0)
+    sim.timer_enable(dominant, args.steps + 8)
+
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.simulate(args.steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    tot_ms, launches = sim.timer_read(dominant)
+    dt = t1 - t0
+    value = cells * args.steps / dt / 1e6
+    avg_s = tot_ms / 1e3 / max(launches, 1)
+    bytes_launch = balg[dominant] * 8 * cells
+    achieved = bytes_launch / avg_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(args.config, {}).get(dominant)
+    step_balg = sum(balg[k] for k in balg if k != "find_max_u") * 8
+    out = {
+        "metric": "Mcell-steps/s at 4096x2048 (FP64), % HBM roofline",
+        "value": round(value, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded random-vorticity initial velocity",
+        "config": {"workload": f"{args.config}: {NX}x{NY} cells ({cfg['bpdx']}x{cfg['bpdy']} "
+                               f"blocks), {cfg['poisson']} FFT Poisson, nu={cfg['nu']}, CFL 0.1, "
+                               f"one disk r={cfg['body']['radius']}",
+                   "cells": cells, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2),
+                     "alg_bytes_per_launch": bytes_launch},
+        "step_hbm_frac": round(step_balg * cells * args.steps / dt / 1e9 / PEAK_HBM_GBS, 4),
+        "kernel_us_warmup": {k: round(per[k][0] * 1e3 / max(per[k][1], 1), 2) for k in per},
+    }
+    if not args.no_cpu and args.cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(cfg, vel, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

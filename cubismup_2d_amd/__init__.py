@@ -73,8 +73,11 @@ EXPORTS = [
     "cup2d_pressure_rhs", "cup2d_poisson_solve", "cup2d_pressure_correction",
     "cup2d_body_update", "cup2d_penalize", "cup2d_pressure_single", "cup2d_compute_forces",
     "cup2d_advance", "cup2d_simulate", "cup2d_synchronize", "cup2d_poisson_dense",
-    "cup2d_grid_size",
+    "cup2d_grid_size", "cup2d_timer_enable", "cup2d_timer_read",
 ]
+TIMERS = ["find_max_u", "dt", "advdiff", "flux_fix", "pressure_rhs", "rhs_obstacles", "fade",
+          "fft_rows_fwd", "fft_cols", "fft_rows_inv", "pressure_correction", "put_objects",
+          "body_update", "penalize", "forces"]
 
 _lib = None
 
@@ -113,6 +116,8 @@ def lib():
     L.cup2d_simulate.argtypes = [P, I]
     L.cup2d_poisson_dense.argtypes = [P, dp, dp]
     L.cup2d_grid_size.argtypes = [P, C.POINTER(I), C.POINTER(I)]
+    L.cup2d_timer_enable.argtypes = [P, C.c_char_p, I]
+    L.cup2d_timer_read.argtypes = [P, C.c_char_p, dp, C.POINTER(I)]
     for n in EXPORTS:
         if n != "cup2d_last_error":
             getattr(L, n).restype = C.c_int
@@ -178,14 +183,14 @@ class Simulation:
 
     # -- data
     def set_field(self, f, arr, dense=True):
-        nc = NCOMP[f]
+        nc = NCOMP[f] if 0 <= f < len(NCOMP) else 1  # the C-ABI validates f
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
         assert a.size == nc * self.NX * self.NY
         self._chk(self.L.cup2d_upload_field(self.h, f, LAYOUT_DENSE if dense else LAYOUT_BLOCKS,
                                             _dp(a)))
 
     def get_field(self, f, dense=True):
-        nc = NCOMP[f]
+        nc = NCOMP[f] if 0 <= f < len(NCOMP) else 1  # the C-ABI validates f
         a = np.empty(nc * self.NX * self.NY)
         self._chk(self.L.cup2d_download_field(self.h, f,
                                               LAYOUT_DENSE if dense else LAYOUT_BLOCKS, _dp(a)))
@@ -264,6 +269,15 @@ class Simulation:
 
     def synchronize(self):
         self._chk(self.L.cup2d_synchronize(self.h))
+
+    def timer_enable(self, kernel="*", reserve=1024):
+        self._chk(self.L.cup2d_timer_enable(self.h, kernel.encode(), reserve))
+
+    def timer_read(self, kernel):
+        """(total_ms, launches) since the last read of this timer."""
+        t, n = C.c_double(), C.c_int()
+        self._chk(self.L.cup2d_timer_read(self.h, kernel.encode(), C.byref(t), C.byref(n)))
+        return t.value, n.value
 
     def poisson_dense(self, rhs):
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)

@@ -12,7 +12,33 @@
 
 using namespace cup2d;
 
+namespace cup2d {
+const char* const TIMER_NAMES[T_NTIMERS] = {
+    "find_max_u", "dt", "advdiff", "flux_fix", "pressure_rhs", "rhs_obstacles", "fade",
+    "fft_rows_fwd", "fft_cols", "fft_rows_inv", "pressure_correction", "put_objects",
+    "body_update", "penalize", "forces"};
+
+void timer_mark(cup2d_ctx* c, int id, bool start) {
+  Timer& t = c->timers[id];
+  if (!t.on) return;
+  const size_t need = 2 * (size_t)(t.used + 1);
+  while (t.ev.size() < need) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) { t.on = false; return; }
+    t.ev.push_back(e);
+  }
+  (void)hipEventRecord(t.ev[2 * (size_t)t.used + (start ? 0 : 1)], c->stream);
+  if (!start) ++t.used;
+}
+}  // namespace cup2d
+
 namespace {
+int timer_index(const char* name) {
+  if (!name) return -1;
+  for (int i = 0; i < T_NTIMERS; ++i)
+    if (std::strcmp(name, TIMER_NAMES[i]) == 0) return i;
+  return -1;
+}
 thread_local std::string g_create_error;
 
 int fail(cup2d_ctx* c, int code, const std::string& msg) {
@@ -98,18 +124,19 @@ int pressure_single(cup2d_ctx* c) {
   RC(enqueue_rhs_obstacles(c));
   RC(enqueue_fade(c));
   RC(enqueue_poisson_fft(c, c->rhs, c->pres));
-  RC(enqueue_pressure_correction(c));
+  RC(enqueue_pressure_correction(c, 1));
   RC(enqueue_body_update(c));
-  RC(enqueue_penalize(c));
+  RC(enqueue_penalize(c, c->maxpart + c->nwg));
   return CUP2D_OK;
 }
 
-int advance_enqueued(cup2d_ctx* c) {
-  RC(enqueue_put_objects(c));
-  RC(enqueue_advdiff(c));
+// Simulation::advance.  compute_dt: dt from the findMaxU partials of the
+// previous step (device loop) instead of the value already on the device.
+int advance_enqueued(cup2d_ctx* c, int compute_dt) {
+  RC(enqueue_put_objects(c, compute_dt));
+  RC(enqueue_advdiff(c, 1));
   RC(pressure_single(c));
-  RC(enqueue_forces(c));
-  RC(enqueue_step_end(c));
+  RC(enqueue_forces(c, 1));
   return CUP2D_OK;
 }
 }  // namespace
@@ -196,6 +223,17 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
     }
   }
   if ((size_t)3 * c->band_nblocks > c->red_n) { c->err = "grid too large for fade partials"; return bail(CUP2D_ERR_ARG); }
+  c->nwg = grid_workgroups(g.NX, g.NY);
+  c->maxpart_n = c->nwg + OBST_LIST_CAP;
+  if (hipMalloc(&c->fluxpart, sizeof(double) * c->nwg) != hipSuccess ||
+      hipMalloc(&c->maxpart, sizeof(double) * c->maxpart_n) != hipSuccess ||
+      hipMalloc(&c->obmap, (size_t)g.bpdx * g.bpdy) != hipSuccess ||
+      hipMemset(c->obmap, 0, (size_t)g.bpdx * g.bpdy) != hipSuccess ||
+      hipMemset(c->fluxpart, 0, sizeof(double) * c->nwg) != hipSuccess ||
+      hipMemset(c->maxpart, 0, sizeof(double) * c->maxpart_n) != hipSuccess) {
+    c->err = "out of device memory";
+    return bail(CUP2D_ERR_HIP);
+  }
   rc = fft_create(c);
   if (rc == CUP2D_ERR_ARG) {
     // unsupported FFT size: stencil operators still work, Poisson calls fail
@@ -218,13 +256,17 @@ int cup2d_destroy(cup2d_ctx* c) {
   fft_destroy(c);
   obstacles_free(c);
   double* ptrs[] = {c->u, c->v, c->tu, c->tv, c->pres, c->rhs, c->sdf, c->chi,
-                    c->udx, c->udy, c->invrho, c->staging, c->red, c->ramp};
+                    c->udx, c->udy, c->invrho, c->staging, c->red, c->ramp,
+                    c->fluxpart, c->maxpart};
+  if (c->obmap) (void)hipFree(c->obmap);
   for (double* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->st) (void)hipFree(c->st);
   if (c->sp) (void)hipFree(c->sp);
   if (c->ss) (void)hipFree(c->ss);
   if (c->band_blocks) (void)hipFree(c->band_blocks);
+  for (Timer& t : c->timers)
+    for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CUP2D_OK;
@@ -349,7 +391,7 @@ int cup2d_reset(cup2d_ctx* c) {
   RC(ensure_frozen(c));
   RC(enqueue_shapes_reset(c));
   RC(enqueue_ic(c));
-  RC(enqueue_put_objects(c));
+  RC(enqueue_put_objects(c, 0));
   RC(enqueue_apply_obj_vel(c));
   return sync_and_check(c);
 }
@@ -378,14 +420,14 @@ int cup2d_put_objects(cup2d_ctx* c, double dt) {
   CHECK_CTX(c);
   RC(ensure_frozen(c));
   RC(enqueue_set_dt(c, dt));
-  RC(enqueue_put_objects(c));
+  RC(enqueue_put_objects(c, 0));
   return sync_and_check(c);
 }
 
 int cup2d_advdiff(cup2d_ctx* c, double dt) {
   CHECK_CTX(c);
   RC(enqueue_set_dt(c, dt));
-  RC(enqueue_advdiff(c));
+  RC(enqueue_advdiff(c, 0));
   return sync_and_check(c);
 }
 
@@ -408,7 +450,7 @@ int cup2d_poisson_solve(cup2d_ctx* c) {
 int cup2d_pressure_correction(cup2d_ctx* c, double dt) {
   CHECK_CTX(c);
   RC(enqueue_set_dt(c, dt));
-  RC(enqueue_pressure_correction(c));
+  RC(enqueue_pressure_correction(c, 0));
   return sync_and_check(c);
 }
 
@@ -424,7 +466,7 @@ int cup2d_penalize(cup2d_ctx* c, double dt) {
   CHECK_CTX(c);
   if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
   RC(enqueue_set_dt(c, dt));
-  RC(enqueue_penalize(c));
+  RC(enqueue_penalize(c, c->maxpart + c->nwg));
   return sync_and_check(c);
 }
 
@@ -439,7 +481,7 @@ int cup2d_pressure_single(cup2d_ctx* c, double dt) {
 int cup2d_compute_forces(cup2d_ctx* c) {
   CHECK_CTX(c);
   if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
-  RC(enqueue_forces(c));
+  RC(enqueue_forces(c, 0));
   return sync_and_check(c);
 }
 
@@ -447,7 +489,7 @@ int cup2d_advance(cup2d_ctx* c, double dt) {
   CHECK_CTX(c);
   if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
   RC(enqueue_set_dt(c, dt));
-  RC(advance_enqueued(c));
+  RC(advance_enqueued(c, 0));
   return sync_and_check(c);
 }
 
@@ -455,10 +497,9 @@ int cup2d_simulate(cup2d_ctx* c, int nsteps) {
   CHECK_CTX(c);
   if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
   if (nsteps < 0) return fail(c, CUP2D_ERR_ARG, "nsteps < 0");
-  for (int s = 0; s < nsteps; ++s) {
-    RC(enqueue_dt(c));
-    RC(advance_enqueued(c));
-  }
+  if (nsteps == 0) return CUP2D_OK;
+  RC(enqueue_find_max_u_partials(c));
+  for (int s = 0; s < nsteps; ++s) RC(advance_enqueued(c, 1));
   return CUP2D_OK;
 }
 
@@ -478,6 +519,44 @@ int cup2d_poisson_dense(cup2d_ctx* c, const double* rhs, double* sol) {
   RC(enqueue_poisson_fft(c, din, dout));
   CUP2D_HIP_CHECK(c, hipMemcpyAsync(sol, dout, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return CUP2D_OK;
+}
+
+int cup2d_timer_enable(cup2d_ctx* c, const char* kernel, int reserve_launches) {
+  CHECK_CTX(c);
+  const bool all = kernel && std::strcmp(kernel, "*") == 0;
+  const int id = timer_index(kernel);
+  if (!all && id < 0) return fail(c, CUP2D_ERR_ARG, "unknown timer name");
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < T_NTIMERS; ++i) {
+    if (!all && i != id) continue;
+    Timer& t = c->timers[i];
+    t.on = reserve_launches > 0;
+    t.used = 0;
+    while (t.ev.size() < 2 * (size_t)(reserve_launches > 0 ? reserve_launches : 0)) {
+      hipEvent_t e;
+      CUP2D_HIP_CHECK(c, hipEventCreate(&e));
+      t.ev.push_back(e);
+    }
+  }
+  return CUP2D_OK;
+}
+
+int cup2d_timer_read(cup2d_ctx* c, const char* kernel, double* total_ms, int* launches) {
+  CHECK_CTX(c);
+  const int id = timer_index(kernel);
+  if (id < 0 || !total_ms || !launches) return fail(c, CUP2D_ERR_ARG, "unknown timer name");
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  Timer& t = c->timers[id];
+  double tot = 0;
+  for (int k = 0; k < t.used; ++k) {
+    float ms = 0;
+    CUP2D_HIP_CHECK(c, hipEventElapsedTime(&ms, t.ev[2 * (size_t)k], t.ev[2 * (size_t)k + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = t.used;
+  t.used = 0;
   return CUP2D_OK;
 }
 

@@ -581,22 +581,32 @@ void fft_destroy(cup2d_ctx* c) {
 int enqueue_poisson_fft(cup2d_ctx* c, const double* rhs, double* sol) {
   FFTPlan* P = c->fft;
   const int NX = c->g.NX, NY = c->g.NY;
-  int rc = rows_fwd(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r);
+  int rc;
+  {
+    TimerScope ts(c, T_FFT_ROWS_FWD);
+    rc = rows_fwd(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r);
+  }
   if (rc) return rc;
   ColParams cp{};
   cp.NX = NX; cp.NY = NY; cp.rows = NY;
-  if (!P->fs) {
-    cp.waveFactX = 2 * M_PI / NX;
-    cp.waveFactY = 2 * M_PI / NY;
-    cp.norm = 1.0 / ((double)NX * NY);
-    rc = cols<0>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
-  } else {
-    cp.ghat = P->ghat;
-    cp.ghatN = P->ghatN;
-    rc = cols<1>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
+  {
+    TimerScope ts(c, T_FFT_COLS);
+    if (!P->fs) {
+      cp.waveFactX = 2 * M_PI / NX;
+      cp.waveFactY = 2 * M_PI / NY;
+      cp.norm = 1.0 / ((double)NX * NY);
+      rc = cols<0>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
+    } else {
+      cp.ghat = P->ghat;
+      cp.ghatN = P->ghatN;
+      rc = cols<1>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
+    }
   }
   if (rc) return rc;
-  rc = rows_inv(c, P->Lr, P->spec, P->NYs, sol, NX, NX, NY, P->tw_r);
+  {
+    TimerScope ts(c, T_FFT_ROWS_INV);
+    rc = rows_inv(c, P->Lr, P->spec, P->NYs, sol, NX, NX, NY, P->tw_r);
+  }
   if (rc) return rc;
   return launch_check(c, "poisson_fft");
 }

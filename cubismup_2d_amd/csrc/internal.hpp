@@ -78,6 +78,20 @@ struct Obstacles {
 
 struct FFTPlan;  // fft.hip
 
+// Per-kernel HIP-event timers (the role of SimulationData::startProfiler /
+// stopProfiler, SimulationData.cpp:187-205): when enabled, every launch of
+// the kernel is bracketed by two events on the context's stream.
+enum TimerId {
+  T_MAXU = 0, T_DT, T_ADVDIFF, T_FLUXFIX, T_RHS, T_RHS_OBST, T_FADE, T_FFT_ROWS_FWD,
+  T_FFT_COLS, T_FFT_ROWS_INV, T_PROJECT, T_PUT_OBJECTS, T_BODY, T_PENALIZE, T_FORCES,
+  T_NTIMERS
+};
+struct Timer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // start/stop pairs
+  int used = 0;                // pairs recorded since the last read
+};
+
 }  // namespace cup2d
 
 struct cup2d_ctx {
@@ -103,6 +117,12 @@ struct cup2d_ctx {
   double* staging = nullptr;  // host-layout staging (2 comps)
   double* red = nullptr;      // reduction partials
   size_t red_n = 0;
+  int nwg = 0;                // workgroups of the stencil kernels
+  double* fluxpart = nullptr; // [nwg] advDiff boundary-flux partials
+  double* maxpart = nullptr;  // [nwg + obstacle blocks] findMaxU partials
+  int maxpart_n = 0;
+  unsigned char* obmap = nullptr;  // [blocks] 1 = obstacle block this step
+  bool flux_deferred = false; // advDiff's flux correction still to be applied
 
   cup2d::StepState* st = nullptr;          // device
   cup2d::ShapeParams* sp = nullptr;        // device [MAX_SHAPES]
@@ -115,6 +135,7 @@ struct cup2d_ctx {
   cup2d::FFTPlan* fft = nullptr;
   std::string fft_err;                     // why no FFT plan exists (size limits)
   bool shapes_frozen = false;
+  cup2d::Timer timers[cup2d::T_NTIMERS];
 };
 
 #define CUP2D_HIP_CHECK(ctx, call)                                             \
@@ -130,13 +151,24 @@ struct cup2d_ctx {
 namespace cup2d {
 int launch_check(cup2d_ctx* c, const char* what);
 
+extern const char* const TIMER_NAMES[T_NTIMERS];
+void timer_mark(cup2d_ctx* c, int id, bool start);
+struct TimerScope {
+  cup2d_ctx* c;
+  int id;
+  TimerScope(cup2d_ctx* c_, int id_) : c(c_), id(id_) { timer_mark(c, id, true); }
+  ~TimerScope() { timer_mark(c, id, false); }
+};
+
 // stencils.hip
+int grid_workgroups(int NX, int NY);
 int enqueue_find_max_u(cup2d_ctx* c);
+int enqueue_find_max_u_partials(cup2d_ctx* c);
 int enqueue_dt(cup2d_ctx* c);
-int enqueue_advdiff(cup2d_ctx* c);
+int enqueue_advdiff(cup2d_ctx* c, int defer_fix);
 int enqueue_pressure_rhs(cup2d_ctx* c);
 int enqueue_fade(cup2d_ctx* c);
-int enqueue_pressure_correction(cup2d_ctx* c);
+int enqueue_pressure_correction(cup2d_ctx* c, int want_max);
 int enqueue_set_dt(cup2d_ctx* c, double dt);
 int enqueue_ic(cup2d_ctx* c);
 int enqueue_apply_obj_vel(cup2d_ctx* c);
@@ -152,10 +184,11 @@ int enqueue_poisson_fft(cup2d_ctx* c, const double* rhs, double* sol);
 // obstacles.hip
 int obstacles_alloc(cup2d_ctx* c);
 void obstacles_free(cup2d_ctx* c);
-int enqueue_put_objects(cup2d_ctx* c);
+constexpr int OBST_LIST_CAP = 2048;  // slots handled by the list builder
+int enqueue_put_objects(cup2d_ctx* c, int compute_dt);
 int enqueue_rhs_obstacles(cup2d_ctx* c);
 int enqueue_body_update(cup2d_ctx* c);
-int enqueue_penalize(cup2d_ctx* c);
-int enqueue_forces(cup2d_ctx* c);
+int enqueue_penalize(cup2d_ctx* c, double* maxpart);
+int enqueue_forces(cup2d_ctx* c, int end_step);
 int enqueue_shapes_reset(cup2d_ctx* c);
 }  // namespace cup2d

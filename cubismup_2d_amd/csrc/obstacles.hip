@@ -21,7 +21,7 @@ constexpr double EPSD = 2.220446049250313e-16;  // std::numeric_limits<double>::
 constexpr int NT = 256;
 
 // partial-sum offsets inside a slot's NPART doubles
-enum { P_COM = 0, P_MOM = 3, P_RHS = 10, P_PEN = 13, P_FRC = 20, P_NPART_USED = 38 };
+enum { P_COM = 0, P_MOM = 3, P_RHS = 10, P_PEN = 13, P_FRC = 20, P_NSURF = 38, P_NPART_USED = 39 };
 static_assert(P_NPART_USED <= NPART + 16, "partials");
 constexpr int SLOT_PART = 40;
 
@@ -59,76 +59,215 @@ __device__ inline bool touching(const Grid& g, int bx, int by, const double bbox
   return i01 - i00 > 0 && i11 - i10 > 0;
 }
 
-__global__ void k_obst_begin(Grid g, const ShapeParams* __restrict__ sp,
-                             cup2d_shape_state* __restrict__ ss, StepState* __restrict__ st,
-                             Obstacles ob) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double dt = st->dt;
+constexpr int LCAP = OBST_LIST_CAP;  // candidate blocks / slots handled by k_obst_begin
+
+// exclusive scan of a[0..n) in LDS (n <= LCAP), returns the total
+__device__ int block_exscan(int* a, int n, int* tmp) {
+  constexpr int CH = LCAP / NT;
+  const int t = threadIdx.x;
+  const int i0 = t * CH, i1 = (t + 1) * CH < n ? (t + 1) * CH : n;
+  int s = 0;
+  for (int i = i0; i < i1; ++i) s += a[i];
+  __syncthreads();
+  tmp[t] = s;
+  __syncthreads();
+  for (int off = 1; off < NT; off <<= 1) {
+    const int v = t >= off ? tmp[t - off] : 0;
+    __syncthreads();
+    tmp[t] += v;
+    __syncthreads();
+  }
+  int base = t == 0 ? 0 : tmp[t - 1];
+  const int total = tmp[NT - 1];
+  for (int i = i0; i < i1; ++i) {
+    const int v = a[i];
+    a[i] = base;
+    base += v;
+  }
+  __syncthreads();
+  return total;
+}
+
+// One workgroup.  (1) In the device loop: dt from the findMaxU partials left
+// by the projection and penalization kernels of the previous step
+// (Simulation::calcMaxTimestep, Simulation.cpp:254-277).  (2) U_inf
+// (Shape::updateLabVelocity), body motion (Shape::updatePosition) and the
+// domain check.  (3) Obstacle slots: blocks whose cell-centre box meets a
+// shape's bounding box (FillBlocks_Cylinder::is_touching), in shape order
+// and blockID order, then the distinct blocks with their slots in shape order.
+__global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __restrict__ sp,
+                                                   cup2d_shape_state* __restrict__ ss,
+                                                   StepState* __restrict__ st, Obstacles ob,
+                                                   unsigned char* __restrict__ obmap,
+                                                   const double* __restrict__ maxpart,
+                                                   int nmaxpart,
+                                                   const double* __restrict__ ramp,
+                                                   int compute_dt) {
+  __shared__ int s_a[LCAP], s_b[LCAP], s_c[LCAP], s_d[LCAP];
+  __shared__ int s_tmp[NT];
+  __shared__ int s_cstart[MAX_SHAPES + 1];
+  __shared__ int s_rng[MAX_SHAPES][3];
+  __shared__ double s_box[MAX_SHAPES][4];
+  __shared__ double lds[8];
+  const int t = threadIdx.x;
   const int ns = g.nshapes;
-  int nSum[2] = {0, 0};
-  double uSum[2] = {0, 0};
-  for (int k = 0; k < ns; ++k) {
-    if (sp[k].bFixedx) { nSum[0]++; uSum[0] -= ss[k].u; }
-    if (sp[k].bFixedy) { nSum[1]++; uSum[1] -= ss[k].v; }
+  const int nub_prev = st->nub;
+  if (compute_dt) {
+    double m = 0;
+    const int npart = nmaxpart + nub_prev;
+    for (int i = t; i < npart; i += NT) m = dmax(m, maxpart[i]);
+    const double maxU = block_max<NT>(m, lds);
+    if (t == 0) {
+      const double h = g.h;
+      const double dtFourier = h * h / g.nu, dtCFL = maxU < 2.2e-16 ? 1 : h / maxU;
+      double maxUb = 0;
+      for (int k = 0; k < ns; ++k) maxUb = dmax(maxUb, sqrt(ss[k].u * ss[k].u + ss[k].v * ss[k].v));
+      const double dtBody = maxUb < 2.2e-16 ? 1 : h / maxUb;
+      const double mn = dmin(dmin(dtCFL, dtFourier), dtBody);
+      double dt = g.CFL * mn;
+      if (st->step < 100) dt = ramp[st->step] * mn;
+      st->dt = dt;
+      st->maxU = maxU;
+    }
   }
-  if (nSum[0] > 0) st->uinf[0] = uSum[0] / nSum[0];
-  if (nSum[1] > 0) st->uinf[1] = uSum[1] / nSum[1];
-  for (int k = 0; k < ns; ++k) {
-    cup2d_shape_state& t = ss[k];
-    t.centerOfMass[0] += dt * (t.u + st->uinf[0]);
-    t.centerOfMass[1] += dt * (t.v + st->uinf[1]);
-    t.labCenterOfMass[0] += dt * t.u;
-    t.labCenterOfMass[1] += dt * t.v;
-    t.orientation += dt * t.omega;
-    t.orientation = t.orientation > M_PI ? t.orientation - 2 * M_PI : t.orientation;
-    t.orientation = t.orientation < -M_PI ? t.orientation + 2 * M_PI : t.orientation;
-    const double cosang = cos(t.orientation), sinang = sin(t.orientation);
-    t.center[0] = t.centerOfMass[0] + cosang * t.d_gm[0] - sinang * t.d_gm[1];
-    t.center[1] = t.centerOfMass[1] + sinang * t.d_gm[0] + cosang * t.d_gm[1];
-    if (t.center[0] < 0 || t.center[0] > g.extent0 || t.center[1] < 0 ||
-        t.center[1] > g.extent1)
-      st->error = CUP2D_ERR_BODY_OUT;
+  for (int u = t; u < nub_prev; u += NT) obmap[ob.ub_block[u]] = 0;
+  __syncthreads();
+  if (t == 0) {
+    const double dt = st->dt;
+    int nSum[2] = {0, 0};
+    double uSum[2] = {0, 0};
+    for (int k = 0; k < ns; ++k) {
+      if (sp[k].bFixedx) { nSum[0]++; uSum[0] -= ss[k].u; }
+      if (sp[k].bFixedy) { nSum[1]++; uSum[1] -= ss[k].v; }
+    }
+    if (nSum[0] > 0) st->uinf[0] = uSum[0] / nSum[0];
+    if (nSum[1] > 0) st->uinf[1] = uSum[1] / nSum[1];
+    const double U0 = st->uinf[0], U1 = st->uinf[1];
+    int nc = 0;
+    for (int k = 0; k < ns; ++k) {
+      cup2d_shape_state& T = ss[k];
+      T.centerOfMass[0] += dt * (T.u + U0);
+      T.centerOfMass[1] += dt * (T.v + U1);
+      T.labCenterOfMass[0] += dt * T.u;
+      T.labCenterOfMass[1] += dt * T.v;
+      T.orientation += dt * T.omega;
+      T.orientation = T.orientation > M_PI ? T.orientation - 2 * M_PI : T.orientation;
+      T.orientation = T.orientation < -M_PI ? T.orientation + 2 * M_PI : T.orientation;
+      const double cosang = cos(T.orientation), sinang = sin(T.orientation);
+      T.center[0] = T.centerOfMass[0] + cosang * T.d_gm[0] - sinang * T.d_gm[1];
+      T.center[1] = T.centerOfMass[1] + sinang * T.d_gm[0] + cosang * T.d_gm[1];
+      if (T.center[0] < 0 || T.center[0] > g.extent0 || T.center[1] < 0 || T.center[1] > g.extent1)
+        st->error = CUP2D_ERR_BODY_OUT;
+      const double safety = 2 * g.h, R = sp[k].radius;
+      const double c0 = T.center[0], c1 = T.center[1];
+      s_box[k][0] = c0 - R - safety; s_box[k][1] = c0 + R + safety;
+      s_box[k][2] = c1 - R - safety; s_box[k][3] = c1 + R + safety;
+      int bx0 = (int)floor(s_box[k][0] / g.hb) - 1, bx1 = (int)floor(s_box[k][1] / g.hb) + 1;
+      int by0 = (int)floor(s_box[k][2] / g.hb) - 1, by1 = (int)floor(s_box[k][3] / g.hb) + 1;
+      bx0 = bx0 < 0 ? 0 : bx0; by0 = by0 < 0 ? 0 : by0;
+      bx1 = bx1 >= g.bpdx ? g.bpdx - 1 : bx1; by1 = by1 >= g.bpdy ? g.bpdy - 1 : by1;
+      const int w = bx1 >= bx0 ? bx1 - bx0 + 1 : 0, hgt = by1 >= by0 ? by1 - by0 + 1 : 0;
+      s_rng[k][0] = bx0; s_rng[k][1] = by0; s_rng[k][2] = w;
+      s_cstart[k] = nc;
+      nc += w * hgt;
+    }
+    if (nc > LCAP) {  // more candidate blocks than the list builder holds
+      st->error = CUP2D_ERR_CAPACITY;
+      for (int k = 0; k < ns; ++k) s_cstart[k] = 0;
+      nc = 0;
+    }
+    s_cstart[ns] = nc;
   }
-  int nob = 0;
-  for (int k = 0; k < ns; ++k) {
-    ob.slot_of_shape_start[k] = nob;
-    const double safety = 2 * g.h, R = sp[k].radius;
-    const double c0 = ss[k].center[0], c1 = ss[k].center[1];
-    const double bbox[2][2] = {{c0 - R - safety, c0 + R + safety},
-                               {c1 - R - safety, c1 + R + safety}};
-    int bx0 = (int)floor(bbox[0][0] / g.hb) - 1, bx1 = (int)floor(bbox[0][1] / g.hb) + 1;
-    int by0 = (int)floor(bbox[1][0] / g.hb) - 1, by1 = (int)floor(bbox[1][1] / g.hb) + 1;
-    bx0 = bx0 < 0 ? 0 : bx0; by0 = by0 < 0 ? 0 : by0;
-    bx1 = bx1 >= g.bpdx ? g.bpdx - 1 : bx1; by1 = by1 >= g.bpdy ? g.bpdy - 1 : by1;
-    for (int by = by0; by <= by1; ++by)
-      for (int bx = bx0; bx <= bx1; ++bx) {
-        if (!touching(g, bx, by, bbox)) continue;
-        if (nob >= ob.cap) { st->error = CUP2D_ERR_CAPACITY; continue; }
-        ob.slot_shape[nob] = k;
-        ob.slot_block[nob] = bx + g.bpdx * by;
-        ++nob;
-      }
+  __syncthreads();
+  const int NC = s_cstart[ns];
+  // candidates: s_a = touching flag, s_b = block, s_c = shape
+  for (int cidx = t; cidx < NC; cidx += NT) {
+    int k = 0;
+    while (k + 1 < ns && s_cstart[k + 1] <= cidx) ++k;
+    const int local = cidx - s_cstart[k], w = s_rng[k][2];
+    const int bx = s_rng[k][0] + local % w, by = s_rng[k][1] + local / w;
+    const double bbox[2][2] = {{s_box[k][0], s_box[k][1]}, {s_box[k][2], s_box[k][3]}};
+    s_a[cidx] = touching(g, bx, by, bbox) ? 1 : 0;
+    s_b[cidx] = bx + g.bpdx * by;
+    s_c[cidx] = k;
   }
-  ob.slot_of_shape_start[ns] = nob;
-  // distinct blocks, in first-appearance order; slots per block in shape order
-  int nub = 0;
-  for (int s = 0; s < nob; ++s) {
-    const int b = ob.slot_block[s];
-    bool found = false;
-    for (int u = 0; u < nub; ++u)
-      if (ob.ub_block[u] == b) { found = true; break; }
-    if (!found) ob.ub_block[nub++] = b;
+  __syncthreads();
+  // slot_of_shape_start: flags before each shape's first candidate
+  if (t <= ns) s_tmp[t] = 0;
+  __syncthreads();
+  int nob = block_exscan(s_a, NC, s_tmp);  // s_a now holds exclusive prefix
+  if (nob > ob.cap) {
+    if (t == 0) st->error = CUP2D_ERR_CAPACITY;
+    nob = 0;
   }
-  int pos = 0;
-  for (int u = 0; u < nub; ++u) {
-    ob.ub_start[u] = pos;
-    int cnt = 0;
-    for (int s = 0; s < nob; ++s)
-      if (ob.slot_block[s] == ob.ub_block[u]) { ob.ub_slots[pos++] = s; ++cnt; }
-    ob.ub_count[u] = cnt;
+  // compacted slots (s_d = block of slot)
+  for (int cidx = t; cidx < NC; cidx += NT) {
+    const int pos = s_a[cidx];
+    const int next = cidx + 1 < NC ? s_a[cidx + 1] : nob;
+    if (next > pos && pos < nob) {
+      ob.slot_shape[pos] = s_c[cidx];
+      ob.slot_block[pos] = s_b[cidx];
+      s_d[pos] = s_b[cidx];
+    }
   }
-  st->nob = nob;
-  st->nub = nub;
+  if (t <= ns) {
+    const int cs = s_cstart[t];
+    ob.slot_of_shape_start[t] = (t == ns || cs >= NC) ? nob : (nob ? s_a[cs] : 0);
+  }
+  __syncthreads();
+  // distinct blocks: s_a = first-occurrence flag, s_b = rank among same-block
+  // slots, s_c = first slot with this block, s_d keeps the block
+  for (int s = t; s < nob; s += NT) {
+    const int b = s_d[s];
+    int first = s, rank = 0;
+    for (int q = 0; q < s; ++q)
+      if (s_d[q] == b) { if (rank == 0) first = q; ++rank; }
+    s_a[s] = rank == 0 ? 1 : 0;
+    s_b[s] = rank;
+    s_c[s] = first;
+  }
+  __syncthreads();
+  const int nub = block_exscan(s_a, nob, s_tmp);  // s_a: ub index of first occurrences
+  // slot counts per distinct block, stored at the first occurrence
+  __shared__ int s_cnt[LCAP];
+  for (int s = t; s < nob; s += NT) s_cnt[s] = 0;
+  __syncthreads();
+  for (int s = t; s < nob; s += NT)
+    if (s_c[s] == s) {
+      int cnt = 0;
+      for (int q = s; q < nob; ++q) cnt += s_d[q] == s_d[s];
+      s_cnt[s] = cnt;
+    }
+  __syncthreads();
+  block_exscan(s_cnt, nob, s_tmp);  // s_cnt: ub_start at first occurrences
+  for (int s = t; s < nob; s += NT) {
+    const int f = s_c[s];
+    const int u = s_a[f];
+    ob.ub_slots[s_cnt[f] + s_b[s]] = s;
+    if (f == s) {
+      ob.ub_block[u] = s_d[s];
+      ob.ub_start[u] = s_cnt[s];
+      obmap[s_d[s]] = 1;
+    }
+  }
+  __syncthreads();
+  for (int s = t; s < nob; s += NT)
+    if (s_c[s] == s) {
+      int cnt = 0;
+      for (int q = s; q < nob; ++q) cnt += s_d[q] == s_d[s];
+      ob.ub_count[s_a[s]] = cnt;
+    }
+  if (t == 0) {
+    st->nob = nob;
+    st->nub = nub;
+  }
+}
+
+// total over the slots of shape k of partial q (fixed order, identical in
+// every workgroup)
+__device__ inline double shape_sum(const Obstacles& ob, int k, int q, double* lds) {
+  const int s0 = ob.slot_of_shape_start[k], n = ob.slot_of_shape_start[k + 1] - s0;
+  return block_sum_array<NT>(ob.part + (size_t)s0 * SLOT_PART + q, n, SLOT_PART, lds);
 }
 
 // PutObjectsOnGrid.cpp:191-197 restricted to the blocks written last step
@@ -190,7 +329,7 @@ __global__ __launch_bounds__(NT) void k_obst_chi(Grid g, const cup2d_shape_state
   SlotView o = slot(ob, s);
   const double h = g.h, i2h = 0.5 / h, fac = 0.5 * h;
   const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
-  double _x = 0, _y = 0, _m = 0;
+  double _x = 0, _y = 0, _m = 0, _ns = 0;
   for (int i = threadIdx.x; i < BS * BS; i += NT) {
     const int ix = i % BS, iy = i / BS;
     const int gx = bx * BS + ix, gy = by * BS + iy;
@@ -233,13 +372,16 @@ __global__ __launch_bounds__(NT) void k_obst_chi(Grid g, const cup2d_shape_state
       o.sd[i] = D;
       o.sx[i] = -D * gradUX;
       o.sy[i] = -D * gradUY;
+      _ns += 1;
     }
   }
   _x = block_sum<NT>(_x, lds);
   _y = block_sum<NT>(_y, lds);
   _m = block_sum<NT>(_m, lds);
+  _ns = block_sum<NT>(_ns, lds);
   if (threadIdx.x == 0) {
     o.part[P_COM] = _x; o.part[P_COM + 1] = _y; o.part[P_COM + 2] = _m;
+    o.part[P_NSURF] = _ns;
   }
 }
 
@@ -264,14 +406,12 @@ __global__ __launch_bounds__(NT) void k_obst_chigrid(Grid g, cup2d_shape_state* 
       }
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0) {
+    __shared__ double lds[8];
     for (int k = 0; k < g.nshapes; ++k) {
-      double _x = 0, _y = 0, _m = 0;
-      for (int s = ob.slot_of_shape_start[k]; s < ob.slot_of_shape_start[k + 1]; ++s) {
-        const double* p = ob.part + (size_t)s * SLOT_PART;
-        _x += p[P_COM]; _y += p[P_COM + 1]; _m += p[P_COM + 2];
-      }
-      if (_m > EPSD) {
+      const double _x = shape_sum(ob, k, P_COM, lds), _y = shape_sum(ob, k, P_COM + 1, lds);
+      const double _m = shape_sum(ob, k, P_COM + 2, lds);
+      if (threadIdx.x == 0 && _m > EPSD) {
         ss[k].centerOfMass[0] += _x / _m;
         ss[k].centerOfMass[1] += _y / _m;
         ss[k].M = _m;
@@ -324,11 +464,11 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __r
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  __shared__ double lds[8];
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
-    double I[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int t = ob.slot_of_shape_start[k]; t < ob.slot_of_shape_start[k + 1]; ++t)
-      for (int j = 0; j < 7; ++j) I[j] += ob.part[(size_t)t * SLOT_PART + P_MOM + j];
+    double I[7];
+    for (int j = 0; j < 7; ++j) I[j] = shape_sum(ob, k, P_MOM + j, lds);
     const double Iu = I[4] / I[2], Iv = I[5] / I[2], Ia = I[6] / I[3];
     const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
     SlotView o = slot(ob, s);
@@ -343,29 +483,33 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __r
       udy[d] += o.udy[i];
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0) {
     for (int k = 0; k < g.nshapes; ++k) {
-      double I[7] = {0, 0, 0, 0, 0, 0, 0};
-      for (int t = ob.slot_of_shape_start[k]; t < ob.slot_of_shape_start[k + 1]; ++t)
-        for (int j = 0; j < 7; ++j) I[j] += ob.part[(size_t)t * SLOT_PART + P_MOM + j];
-      cup2d_shape_state& T = ss[k];
-      T.M = I[2];
-      T.J = I[3];
-      const double dCx = T.center[0] - T.centerOfMass[0], dCy = T.center[1] - T.centerOfMass[1];
-      T.d_gm[0] = dCx * cos(T.orientation) + dCy * sin(T.orientation);
-      T.d_gm[1] = -dCx * sin(T.orientation) + dCy * cos(T.orientation);
+      const double m = shape_sum(ob, k, P_MOM + 2, lds), j = shape_sum(ob, k, P_MOM + 3, lds);
+      const double nsurf = shape_sum(ob, k, P_NSURF, lds);
+      if (threadIdx.x == 0) {
+        cup2d_shape_state& T = ss[k];
+        T.M = m;
+        T.J = j;
+        T.n_surface_points = nsurf;
+        T.n_obstacle_blocks = ob.slot_of_shape_start[k + 1] - ob.slot_of_shape_start[k];
+        const double dCx = T.center[0] - T.centerOfMass[0], dCy = T.center[1] - T.centerOfMass[1];
+        T.d_gm[0] = dCx * cos(T.orientation) + dCy * sin(T.orientation);
+        T.d_gm[1] = -dCx * sin(T.orientation) + dCy * cos(T.orientation);
+      }
     }
   }
 }
 
 static int grid_cap(cup2d_ctx* c) { return c->ob.cap > 0 ? c->ob.cap : 1; }
 
-int enqueue_put_objects(cup2d_ctx* c) {
+int enqueue_put_objects(cup2d_ctx* c, int compute_dt) {
   const int G = grid_cap(c);
+  TimerScope ts(c, T_PUT_OBJECTS);
   hipLaunchKernelGGL(k_obst_clear, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->sdf,
                      c->chi, c->udx, c->udy, c->invrho);
-  hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(64), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                     c->ob);
+  hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
+                     c->ob, c->obmap, c->maxpart, c->nwg, c->ramp, compute_dt);
   if (c->g.nshapes == 0) return launch_check(c, "put_objects");
   hipLaunchKernelGGL(k_obst_create, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
                      c->ob, c->sdf);
@@ -420,17 +564,15 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
 __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __restrict__ st,
                                                   Obstacles ob, const double* __restrict__ chi,
                                                   double* __restrict__ r) {
+  __shared__ double lds[8];
   if ((int)blockIdx.x >= st->nub) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
-    double sum = 0, pos = 0, neg = 0;
-    for (int t = ob.slot_of_shape_start[k]; t < ob.slot_of_shape_start[k + 1]; ++t) {
-      const double* p = ob.part + (size_t)t * SLOT_PART + P_RHS;
-      sum += p[0]; pos += p[1]; neg += p[2];
-    }
+    const double sum = shape_sum(ob, k, P_RHS, lds), pos = shape_sum(ob, k, P_RHS + 1, lds);
+    const double neg = shape_sum(ob, k, P_RHS + 2, lds);
     const double corrDenom = sum > 0 ? pos : neg;
     const double corr = sum / dmax(corrDenom, EPSD);
     SlotView o = slot(ob, s);
@@ -451,6 +593,7 @@ __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __res
 int enqueue_rhs_obstacles(cup2d_ctx* c) {
   if (c->g.nshapes == 0) return CUP2D_OK;
   const int G = grid_cap(c);
+  TimerScope ts(c, T_RHS_OBST);
   hipLaunchKernelGGL(k_rhs_obst1, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
                      c->rhs);
   hipLaunchKernelGGL(k_rhs_obst2, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
@@ -545,11 +688,10 @@ __global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __
   __shared__ double coll[MAX_SHAPES][10];
   const double dt = st->dt, time = st->time;
   const int ns = g.nshapes;
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < ns; ++k) {
-      double S[7] = {0, 0, 0, 0, 0, 0, 0};
-      for (int t = ob.slot_of_shape_start[k]; t < ob.slot_of_shape_start[k + 1]; ++t)
-        for (int j = 0; j < 7; ++j) S[j] += ob.part[(size_t)t * SLOT_PART + P_PEN + j];
+  for (int k = 0; k < ns; ++k) {
+    double S[7];
+    for (int j = 0; j < 7; ++j) S[j] = shape_sum(ob, k, P_PEN + j, lds);
+    if (threadIdx.x == 0) {
       cup2d_shape_state& T = ss[k];
       const ShapeParams& P = sp[k];
       T.penalM = S[0]; T.penalJ = S[1]; T.penalDX = S[2]; T.penalDY = S[3];
@@ -645,6 +787,7 @@ __global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __
 int enqueue_body_update(cup2d_ctx* c) {
   if (c->g.nshapes == 0) return CUP2D_OK;
   const int G = grid_cap(c);
+  TimerScope ts(c, T_BODY);
   hipLaunchKernelGGL(k_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob, c->u,
                      c->v);
   hipLaunchKernelGGL(k_body_solve, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
@@ -657,7 +800,9 @@ int enqueue_body_update(cup2d_ctx* c) {
 __global__ __launch_bounds__(NT) void k_penalize(Grid g, const cup2d_shape_state* __restrict__ ss,
                                                  const StepState* __restrict__ st, Obstacles ob,
                                                  const double* __restrict__ chi,
-                                                 double* __restrict__ u, double* __restrict__ v) {
+                                                 double* __restrict__ u, double* __restrict__ v,
+                                                 double* __restrict__ maxpart) {
+  __shared__ double lds[8];
   if ((int)blockIdx.x >= st->nub) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
@@ -682,12 +827,28 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const cup2d_shape_state
       v[d] = alpha * v[d] + (1 - alpha) * VS;
     }
   }
+  if (!maxpart) return;
+  // findMaxU (Helpers.cpp:160-175) over this block's final velocities; the
+  // projection kernel skips obstacle blocks.
+  const double UINF = st->uinf[0], VINF = st->uinf[1];
+  double m = 0;
+  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+    const double a = u[d], c = v[d];
+    m = dmax(m, fabs(a + UINF));
+    m = dmax(m, fabs(c + VINF));
+    m = dmax(m, fabs(a));
+    m = dmax(m, fabs(c));
+  }
+  m = block_max<NT>(m, lds);
+  if (threadIdx.x == 0) maxpart[blockIdx.x] = m;
 }
 
-int enqueue_penalize(cup2d_ctx* c) {
+int enqueue_penalize(cup2d_ctx* c, double* maxpart) {
   if (c->g.nshapes == 0) return CUP2D_OK;
+  TimerScope ts(c, T_PENALIZE);
   hipLaunchKernelGGL(k_penalize, dim3(grid_cap(c)), dim3(NT), 0, c->stream, c->g, c->ss, c->st,
-                     c->ob, c->chi, c->u, c->v);
+                     c->ob, c->chi, c->u, c->v, maxpart);
   return launch_check(c, "penalize");
 }
 
@@ -753,37 +914,48 @@ __global__ __launch_bounds__(NT) void k_forces(Grid g, const cup2d_shape_state* 
   }
 }
 
-// Shape::computeForces (Shape.cpp:252-281): block partials in block order.
-__global__ void k_forces_fin(Grid g, cup2d_shape_state* __restrict__ ss, Obstacles ob) {
-  const int k = threadIdx.x;
-  if (k >= g.nshapes) return;
-  double a[18];
-  for (int j = 0; j < 18; ++j) a[j] = 0;
-  const int s0 = ob.slot_of_shape_start[k], s1 = ob.slot_of_shape_start[k + 1];
-  for (int s = s0; s < s1; ++s)
-    for (int j = 0; j < 18; ++j) a[j] += ob.part[(size_t)s * SLOT_PART + P_FRC + j];
-  cup2d_shape_state& T = ss[k];
-  T.perimeter = a[0]; T.circulation = a[1];
-  T.forcex = a[2]; T.forcey = a[3]; T.forcex_V = a[4]; T.forcey_V = a[5];
-  T.forcex_P = a[6]; T.forcey_P = a[7];
-  T.torque = a[8]; T.torque_P = a[9]; T.torque_V = a[10];
-  T.thrust = a[11]; T.drag = a[12];
-  T.Pout = a[13]; T.defPower = a[14]; T.PoutBnd = a[15]; T.defPowerBnd = a[16];
-  T.n_surface_points = a[17];
-  T.n_obstacle_blocks = s1 - s0;
-  T.Pthrust = T.thrust * sqrt(T.u * T.u + T.v * T.v);
-  T.Pdrag = T.drag * sqrt(T.u * T.u + T.v * T.v);
-  const double denUnb = T.Pthrust - dmin(T.defPower, 0.0);
-  const double demBnd = T.Pthrust - T.defPowerBnd;
-  T.EffPDef = T.Pthrust / dmax(denUnb, EPSD);
-  T.EffPDefBnd = T.Pthrust / dmax(demBnd, EPSD);
+// Shape::computeForces (Shape.cpp:252-281): per-block partials added in
+// block order; optionally ends the step (time += dt, step++; Simulation.cpp:289-290).
+__global__ __launch_bounds__(NT) void k_forces_fin(Grid g, cup2d_shape_state* __restrict__ ss,
+                                                   Obstacles ob, StepState* __restrict__ st,
+                                                   int end_step) {
+  __shared__ double lds[8];
+  for (int k = 0; k < g.nshapes; ++k) {
+    double a[18];
+    for (int j = 0; j < 18; ++j) a[j] = shape_sum(ob, k, P_FRC + j, lds);
+    if (threadIdx.x != 0) continue;
+    cup2d_shape_state& T = ss[k];
+    T.perimeter = a[0]; T.circulation = a[1];
+    T.forcex = a[2]; T.forcey = a[3]; T.forcex_V = a[4]; T.forcey_V = a[5];
+    T.forcex_P = a[6]; T.forcey_P = a[7];
+    T.torque = a[8]; T.torque_P = a[9]; T.torque_V = a[10];
+    T.thrust = a[11]; T.drag = a[12];
+    T.Pout = a[13]; T.defPower = a[14]; T.PoutBnd = a[15]; T.defPowerBnd = a[16];
+    T.n_surface_points = a[17];
+    T.n_obstacle_blocks = ob.slot_of_shape_start[k + 1] - ob.slot_of_shape_start[k];
+    T.Pthrust = T.thrust * sqrt(T.u * T.u + T.v * T.v);
+    T.Pdrag = T.drag * sqrt(T.u * T.u + T.v * T.v);
+    const double denUnb = T.Pthrust - dmin(T.defPower, 0.0);
+    const double demBnd = T.Pthrust - T.defPowerBnd;
+    T.EffPDef = T.Pthrust / dmax(denUnb, EPSD);
+    T.EffPDefBnd = T.Pthrust / dmax(demBnd, EPSD);
+  }
+  if (end_step && threadIdx.x == 0) {
+    st->time += st->dt;
+    st->step += 1;
+  }
 }
 
-int enqueue_forces(cup2d_ctx* c) {
-  if (c->g.nshapes == 0) return CUP2D_OK;
+int enqueue_forces(cup2d_ctx* c, int end_step) {
+  if (c->g.nshapes == 0) {
+    if (end_step) return enqueue_step_end(c);
+    return CUP2D_OK;
+  }
+  TimerScope ts(c, T_FORCES);
   hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, c->stream, c->g, c->ss, c->st,
                      c->ob, c->u, c->v, c->pres);
-  hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(MAX_SHAPES), 0, c->stream, c->g, c->ss, c->ob);
+  hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(NT), 0, c->stream, c->g, c->ss, c->ob, c->st,
+                     end_step);
   return launch_check(c, "forces");
 }
 
@@ -823,6 +995,10 @@ int obstacles_alloc(cup2d_ctx* c) {
   cap = cap > nblocks * (int)(c->hsp.size() ? c->hsp.size() : 1)
             ? nblocks * (int)(c->hsp.size() ? c->hsp.size() : 1)
             : cap;
+  if (cap > OBST_LIST_CAP) {
+    c->err = "too many obstacle blocks for the device list builder";
+    return CUP2D_ERR_CAPACITY;
+  }
   o.cap = cap;
   const size_t cells = (size_t)cap * BS * BS;
   int* ibuf = nullptr;

@@ -17,6 +17,8 @@ namespace cup2d {
 constexpr int BC_KILL_FAC = 100;  // Definitions.h:38
 constexpr double EPSD = 2.220446049250313e-16;
 
+int grid_workgroups(int NX, int NY);
+
 int launch_check(cup2d_ctx* c, const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -31,13 +33,18 @@ static inline int strip_width(int NX) {
     if (NX % t == 0) return t;
   return 32;
 }
+static constexpr int RYS_ = 16;
+// workgroups of the stencil kernels (advDiff, RHS, projection share a grid)
+int grid_workgroups(int NX, int NY) { return (NX / strip_width(NX)) * (NY / RYS_); }
 
 // ------------------------------------------------------------ findMaxU
 // Helpers.cpp:136-176: max over cells of |u+Uinf|, |v+Vinf|, |u|, |v|.
+// Writes one partial per workgroup into part[0..gridDim.x) and clears the
+// nclear obstacle-block partials that follow.
 __global__ __launch_bounds__(256) void k_find_max_u(const double* __restrict__ u,
                                                     const double* __restrict__ v, size_t n,
                                                     const StepState* __restrict__ st,
-                                                    double* __restrict__ part) {
+                                                    double* __restrict__ part, int nclear) {
   __shared__ double lds[8];
   const double UINF = st->uinf[0], VINF = st->uinf[1];
   double m = 0;
@@ -50,6 +57,8 @@ __global__ __launch_bounds__(256) void k_find_max_u(const double* __restrict__ u
   }
   m = block_max<256>(m, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = m;
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < nclear; i += 256) part[gridDim.x + i] = 0;
 }
 
 // Simulation::calcMaxTimestep (Simulation.cpp:254-277).  rampCFL for
@@ -79,23 +88,23 @@ __global__ __launch_bounds__(256) void k_dt(const double* __restrict__ part, int
   }
 }
 
-int enqueue_find_max_u(cup2d_ctx* c) {
-  const int nb = 1024;
-  hipLaunchKernelGGL(k_find_max_u, dim3(nb), dim3(256), 0, c->stream, c->u, c->v,
-                     (size_t)c->g.NX * c->g.NY, c->st, c->red);
-  hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->g, c->ss, c->ramp,
-                     c->st, 0);
+int enqueue_find_max_u_partials(cup2d_ctx* c) {
+  TimerScope ts(c, T_MAXU);
+  hipLaunchKernelGGL(k_find_max_u, dim3(c->nwg), dim3(256), 0, c->stream, c->u, c->v,
+                     (size_t)c->g.NX * c->g.NY, c->st, c->maxpart, c->maxpart_n - c->nwg);
   return launch_check(c, "find_max_u");
 }
 
-int enqueue_dt(cup2d_ctx* c) {
-  const int nb = 1024;
-  hipLaunchKernelGGL(k_find_max_u, dim3(nb), dim3(256), 0, c->stream, c->u, c->v,
-                     (size_t)c->g.NX * c->g.NY, c->st, c->red);
-  hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, c->stream, c->red, nb, c->g, c->ss, c->ramp,
-                     c->st, 1);
+static int find_max_u_and_dt(cup2d_ctx* c, int set_dt) {
+  int rc = enqueue_find_max_u_partials(c);
+  if (rc) return rc;
+  TimerScope ts(c, T_DT);
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, c->stream, c->maxpart, c->nwg, c->g, c->ss,
+                     c->ramp, c->st, set_dt);
   return launch_check(c, "calc_max_timestep");
 }
+int enqueue_find_max_u(cup2d_ctx* c) { return find_max_u_and_dt(c, 0); }
+int enqueue_dt(cup2d_ctx* c) { return find_max_u_and_dt(c, 1); }
 
 __global__ void k_set_dt(StepState* st, double dt) { st->dt = dt; }
 int enqueue_set_dt(cup2d_ctx* c, double dt) {
@@ -113,12 +122,16 @@ int enqueue_step_end(cup2d_ctx* c) {
 }
 
 // ------------------------------------------------------------- advDiff
-// advDiff.cpp:14-34 (stencil), :47-55 (factors), :68-79 (faded lab).
+// advDiff.cpp:14-34 (stencil), :47-55 (factors), :68-79 (faded lab).  Each
+// workgroup also writes its share of the boundary flux
+// IF = sum_E u - sum_W u + sum_N v - sum_S v (advDiff.cpp:93-102).
 template <int RY>
 __global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
                                                  const double* __restrict__ v,
                                                  double* __restrict__ uo, double* __restrict__ vo,
-                                                 Grid g, const StepState* __restrict__ st) {
+                                                 Grid g, const StepState* __restrict__ st,
+                                                 double* __restrict__ fluxpart) {
+  __shared__ double lds[8];
   const int NX = g.NX, NY = g.NY;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y0 = blockIdx.y * RY;
@@ -136,6 +149,7 @@ __global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
   double us, vs;
   if (y0 == 0) { us = uc * fadeS; vs = vc * fadeS; }
   else { us = u[idx - NX]; vs = v[idx - NX]; }
+  double flux = 0;
 #pragma unroll 4
   for (int iy = 0; iy < RY; ++iy, idx += NX) {
     const int y = y0 + iy;
@@ -151,73 +165,118 @@ __global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
     const double dUdif = ue + un + uw + us - 4 * uc;
     const double dVadv = (uc + UINF0) * (ve - vw) + (vc + UINF1) * (vn - vs);
     const double dVdif = ve + vn + vw + vs - 4 * vc;
-    uo[idx] = uc + (afac * dUadv + dfac * dUdif);
-    vo[idx] = vc + (afac * dVadv + dfac * dVdif);
+    const double u1 = uc + (afac * dUadv + dfac * dUdif);
+    const double v1 = vc + (afac * dVadv + dfac * dVdif);
+    uo[idx] = u1;
+    vo[idx] = v1;
+    if (x == 0) flux -= u1;
+    if (x == NX - 1) flux += u1;
+    if (y == 0) flux -= v1;
+    if (y == NY - 1) flux += v1;
     us = uc; vs = vc;
     uc = un; vc = vn;
   }
+  const bool edge = blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
+                    blockIdx.y == gridDim.y - 1;
+  if (edge) flux = block_sum<256>(flux, lds);
+  if (threadIdx.x == 0) fluxpart[blockIdx.y * gridDim.x + blockIdx.x] = edge ? flux : 0.0;
 }
 
-// advDiff.cpp:89-116: IF = sum_E u - sum_W u + sum_N v - sum_S v over the
-// boundary cells; corr = IF / (2 NY + 2 NX); u_W += corr, u_E -= corr,
-// v_S += corr, v_N -= corr.
-__global__ __launch_bounds__(1024) void k_flux_fix(double* __restrict__ u,
-                                                   double* __restrict__ v, Grid g,
-                                                   StepState* __restrict__ st) {
-  __shared__ double lds[16];
-  const int NX = g.NX, NY = g.NY;
-  const int n = 2 * NY + 2 * NX;
+// corr = IF / (2 NY + 2 NX) (advDiff.cpp:105) from the per-workgroup partials
+// of the boundary workgroups, summed in a fixed order.
+__device__ inline double flux_corr(const double* __restrict__ fluxpart, int gx, int gy,
+                                   const Grid& g, double* lds) {
   double s = 0;
-  for (int i = threadIdx.x; i < n; i += 1024) {
-    if (i < NY) s -= u[(size_t)i * NX];
-    else if (i < 2 * NY) s += u[(size_t)(i - NY) * NX + NX - 1];
-    else if (i < 2 * NY + NX) s -= v[i - 2 * NY];
-    else s += v[(size_t)(NY - 1) * NX + (i - 2 * NY - NX)];
+  for (int w = threadIdx.x; w < gx * gy; w += blockDim.x) {
+    const int bx = w % gx, by = w / gx;
+    if (bx == 0 || bx == gx - 1 || by == 0 || by == gy - 1) s += fluxpart[w];
   }
-  const double IF = block_sum<1024>(s, lds);
-  const double corr = IF / (2 * (BS * g.bpdy - 0) + 2 * (BS * g.bpdx - 0));
-  for (int i = threadIdx.x; i < n; i += 1024) {
+  const double IF = block_sum<256>(s, lds);
+  return IF / (2 * (BS * g.bpdy - 0) + 2 * (BS * g.bpdx - 0));
+}
+
+// advDiff.cpp:107-115 applied to the stored velocity (operator-level API).
+__global__ __launch_bounds__(256) void k_flux_fix(double* __restrict__ u,
+                                                  double* __restrict__ v, Grid g,
+                                                  StepState* __restrict__ st,
+                                                  const double* __restrict__ fluxpart, int gx,
+                                                  int gy) {
+  __shared__ double lds[8];
+  const int NX = g.NX, NY = g.NY;
+  const double corr = flux_corr(fluxpart, gx, gy, g, lds);
+  const int n = 2 * NY + 2 * NX;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     if (i < NY) u[(size_t)i * NX] += corr;
     else if (i < 2 * NY) u[(size_t)(i - NY) * NX + NX - 1] -= corr;
     else if (i < 2 * NY + NX) v[i - 2 * NY] += corr;
     else v[(size_t)(NY - 1) * NX + (i - 2 * NY - NX)] -= corr;
   }
-  if (threadIdx.x == 0) st->flux_corr = corr;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->flux_corr = corr;
 }
 
-int enqueue_advdiff(cup2d_ctx* c) {
-  constexpr int RY = 16;
+static constexpr int RYS = 16;  // rows per workgroup of the stencil kernels
+
+int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
   const int tx = strip_width(c->g.NX);
-  hipLaunchKernelGGL(k_advdiff<RY>, dim3(c->g.NX / tx, c->g.NY / RY), dim3(tx), 0, c->stream,
-                     c->u, c->v, c->tu, c->tv, c->g, c->st);
+  const int gx = c->g.NX / tx, gy = c->g.NY / RYS;
+  {
+    TimerScope ts(c, T_ADVDIFF);
+    hipLaunchKernelGGL(k_advdiff<RYS>, dim3(gx, gy), dim3(tx), 0, c->stream, c->u, c->v, c->tu,
+                       c->tv, c->g, c->st, c->fluxpart);
+  }
   std::swap(c->u, c->tu);  // advDiff.cpp:81-87 copy tmpV -> vel
   std::swap(c->v, c->tv);
-  hipLaunchKernelGGL(k_flux_fix, dim3(1), dim3(1024), 0, c->stream, c->u, c->v, c->g, c->st);
+  c->flux_deferred = defer_fix != 0;
+  if (!defer_fix) {
+    TimerScope ts(c, T_FLUXFIX);
+    const int n = 2 * c->g.NX + 2 * c->g.NY;
+    hipLaunchKernelGGL(k_flux_fix, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->u, c->v,
+                       c->g, c->st, c->fluxpart, gx, gy);
+  }
   return launch_check(c, "advdiff");
 }
 
 // --------------------------------------------------------- pressure RHS
 // PressureSingle.cpp:110-128: r = (h/2dt) (u_E - u_W + v_N - v_S), zero-
-// gradient ghosts.
+// gradient ghosts.  With a deferred advDiff flux correction (fluxpart set),
+// boundary values are read as u_W + corr, u_E - corr, v_S + corr, v_N - corr,
+// exactly the values advDiff.cpp:107-115 would have stored.
 template <int RY>
 __global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
                                              const double* __restrict__ v,
                                              double* __restrict__ r, Grid g,
-                                             const StepState* __restrict__ st) {
+                                             const StepState* __restrict__ st,
+                                             const double* __restrict__ fluxpart) {
+  __shared__ double lds[8];
   const int NX = g.NX, NY = g.NY;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y0 = blockIdx.y * RY;
   const double facDiv = 0.5 * g.h / st->dt;
+  double corr = 0;
+  const bool fix = fluxpart != nullptr;
+  if (fix && (blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
+              blockIdx.y == gridDim.y - 1))
+    corr = flux_corr(fluxpart, gridDim.x, gridDim.y, g, lds);
+  auto U = [&](int xx, size_t i) {
+    const double a = u[i];
+    if (!fix) return a;
+    return xx == 0 ? a + corr : (xx == NX - 1 ? a - corr : a);
+  };
+  auto V = [&](int yy, size_t i) {
+    const double a = v[i];
+    if (!fix) return a;
+    return yy == 0 ? a + corr : (yy == NY - 1 ? a - corr : a);
+  };
   size_t idx = (size_t)y0 * NX + x;
-  double vc = v[idx];
-  double vs = (y0 == 0) ? vc : v[idx - NX];
+  double vc = V(y0, idx);
+  double vs = (y0 == 0) ? vc : V(y0 - 1, idx - NX);
 #pragma unroll 4
   for (int iy = 0; iy < RY; ++iy, idx += NX) {
     const int y = y0 + iy;
-    const double vn = (y + 1 < NY) ? v[idx + NX] : vc;
-    const double uc = u[idx];
-    const double uw = (x > 0) ? u[idx - 1] : uc;
-    const double ue = (x + 1 < NX) ? u[idx + 1] : uc;
+    const double vn = (y + 1 < NY) ? V(y + 1, idx + NX) : vc;
+    const double uc = U(x, idx);
+    const double uw = (x > 0) ? U(x - 1, idx - 1) : uc;
+    const double ue = (x + 1 < NX) ? U(x + 1, idx + 1) : uc;
     const double divVx = ue - uw;
     const double divVy = vn - vs;
     r[idx] = facDiv * (divVx + divVy);
@@ -227,10 +286,10 @@ __global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
 }
 
 int enqueue_pressure_rhs(cup2d_ctx* c) {
-  constexpr int RY = 16;
   const int tx = strip_width(c->g.NX);
-  hipLaunchKernelGGL(k_rhs<RY>, dim3(c->g.NX / tx, c->g.NY / RY), dim3(tx), 0, c->stream, c->u,
-                     c->v, c->rhs, c->g, c->st);
+  TimerScope ts(c, T_RHS);
+  hipLaunchKernelGGL(k_rhs<RYS>, dim3(c->g.NX / tx, c->g.NY / RYS), dim3(tx), 0, c->stream, c->u,
+                     c->v, c->rhs, c->g, c->st, c->flux_deferred ? c->fluxpart : nullptr);
   return launch_check(c, "pressure_rhs");
 }
 
@@ -299,6 +358,7 @@ __global__ __launch_bounds__(256) void k_fade_apply(double* __restrict__ r,
 
 int enqueue_fade(cup2d_ctx* c) {
   if (c->band_nblocks == 0) return CUP2D_OK;
+  TimerScope ts(c, T_FADE);
   hipLaunchKernelGGL(k_fade_sums, dim3(c->band_nblocks), dim3(256), 0, c->stream, c->rhs,
                      c->band_blocks, c->g, c->red);
   hipLaunchKernelGGL(k_fade_apply, dim3(c->band_nblocks), dim3(256), 0, c->stream, c->rhs,
@@ -308,35 +368,70 @@ int enqueue_fade(cup2d_ctx* c) {
 
 // ------------------------------------------------- pressure correction
 // PressureSingle.cpp:216-241: u -= (dt/2h)(p_E - p_W), v -= (dt/2h)(p_N - p_S).
+// Applies a deferred advDiff flux correction (see k_rhs) and writes the
+// findMaxU partial (Helpers.cpp:160-175) of the cells outside obstacle
+// blocks; penalization reports the obstacle blocks after it changes them.
 template <int RY>
 __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double* __restrict__ v,
                                                  const double* __restrict__ p, Grid g,
-                                                 const StepState* __restrict__ st) {
+                                                 const StepState* __restrict__ st,
+                                                 const double* __restrict__ fluxpart,
+                                                 const unsigned char* __restrict__ obmap,
+                                                 double* __restrict__ maxpart) {
+  __shared__ double lds[8];
   const int NX = g.NX, NY = g.NY;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y0 = blockIdx.y * RY;
   const double pFac = -0.5 * st->dt / g.h;
+  const double UINF = st->uinf[0], VINF = st->uinf[1];
+  double corr = 0;
+  const bool fix = fluxpart != nullptr;
+  if (fix && (blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
+              blockIdx.y == gridDim.y - 1))
+    corr = flux_corr(fluxpart, gridDim.x, gridDim.y, g, lds);
   size_t idx = (size_t)y0 * NX + x;
   double pc = p[idx];
   double ps = (y0 == 0) ? pc : p[idx - NX];
+  double m = 0;
 #pragma unroll 4
   for (int iy = 0; iy < RY; ++iy, idx += NX) {
     const int y = y0 + iy;
     const double pn = (y + 1 < NY) ? p[idx + NX] : pc;
     const double pw = (x > 0) ? p[idx - 1] : pc;
     const double pe = (x + 1 < NX) ? p[idx + 1] : pc;
-    u[idx] = u[idx] + pFac * (pe - pw);
-    v[idx] = v[idx] + pFac * (pn - ps);
+    double uu = u[idx], vv = v[idx];
+    if (fix) {
+      if (x == 0) uu = uu + corr;
+      else if (x == NX - 1) uu = uu - corr;
+      if (y == 0) vv = vv + corr;
+      else if (y == NY - 1) vv = vv - corr;
+    }
+    const double u1 = uu + pFac * (pe - pw);
+    const double v1 = vv + pFac * (pn - ps);
+    u[idx] = u1;
+    v[idx] = v1;
+    if (maxpart && !obmap[(y / BS) * g.bpdx + x / BS]) {
+      m = dmax(m, fabs(u1 + UINF));
+      m = dmax(m, fabs(v1 + VINF));
+      m = dmax(m, fabs(u1));
+      m = dmax(m, fabs(v1));
+    }
     ps = pc;
     pc = pn;
   }
+  if (maxpart) {
+    m = block_max<256>(m, lds);
+    if (threadIdx.x == 0) maxpart[blockIdx.y * gridDim.x + blockIdx.x] = m;
+  }
 }
 
-int enqueue_pressure_correction(cup2d_ctx* c) {
-  constexpr int RY = 16;
+int enqueue_pressure_correction(cup2d_ctx* c, int want_max) {
   const int tx = strip_width(c->g.NX);
-  hipLaunchKernelGGL(k_project<RY>, dim3(c->g.NX / tx, c->g.NY / RY), dim3(tx), 0, c->stream,
-                     c->u, c->v, c->pres, c->g, c->st);
+  TimerScope ts(c, T_PROJECT);
+  hipLaunchKernelGGL(k_project<RYS>, dim3(c->g.NX / tx, c->g.NY / RYS), dim3(tx), 0, c->stream,
+                     c->u, c->v, c->pres, c->g, c->st, c->flux_deferred ? c->fluxpart : nullptr,
+                     c->obmap, want_max ? c->maxpart : nullptr);
+  c->flux_deferred = false;
   return launch_check(c, "pressure_correction");
 }
 

@@ -149,8 +149,18 @@ int cup2d_synchronize(cup2d_ctx* ctx);
  * host) -> sol (host).  Same transform as cup2d_poisson_solve. */
 int cup2d_poisson_dense(cup2d_ctx* ctx, const double* rhs, double* sol);
 
-/* Cells per dimension of the grid and the number of built-in timers. */
+/* Cells per dimension of the grid. */
 int cup2d_grid_size(const cup2d_ctx* ctx, int* NX, int* NY);
+
+/* Per-kernel timers: the role of SimulationData::startProfiler/stopProfiler
+ * (SimulationData.cpp:187-205) without the Checker.  kernel is one of
+ * find_max_u dt advdiff flux_fix pressure_rhs rhs_obstacles fade fft_rows_fwd
+ * fft_cols fft_rows_inv pressure_correction put_objects body_update penalize
+ * forces, or "*".  reserve_launches > 0 enables (pre-creating that many HIP
+ * event pairs), 0 disables.  Each launch is bracketed by two events on the
+ * context's stream.  timer_read returns and clears the summed duration. */
+int cup2d_timer_enable(cup2d_ctx* ctx, const char* kernel, int reserve_launches);
+int cup2d_timer_read(cup2d_ctx* ctx, const char* kernel, double* total_ms, int* launches);
 
 #ifdef __cplusplus
 }
