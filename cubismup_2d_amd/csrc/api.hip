@@ -18,7 +18,9 @@ const char* const TIMER_NAMES[T_NTIMERS] = {
     "fft_rows_fwd", "fft_cols", "fft_rows_inv", "pressure_correction", "put_objects",
     "body_update", "penalize", "forces"};
 
-void timer_mark(cup2d_ctx* c, int id, bool start) {
+void timer_mark(cup2d_ctx* c, int id, bool start) { timer_mark_on(c, id, start, nullptr); }
+
+void timer_mark_on(cup2d_ctx* c, int id, bool start, hipStream_t s) {
   Timer& t = c->timers[id];
   if (!t.on) return;
   const size_t need = 2 * (size_t)(t.used + 1);
@@ -27,7 +29,7 @@ void timer_mark(cup2d_ctx* c, int id, bool start) {
     if (hipEventCreate(&e) != hipSuccess) { t.on = false; return; }
     t.ev.push_back(e);
   }
-  (void)hipEventRecord(t.ev[2 * (size_t)t.used + (start ? 0 : 1)], c->stream);
+  (void)hipEventRecord(t.ev[2 * (size_t)t.used + (start ? 0 : 1)], s ? s : c->stream);
   if (!start) ++t.used;
 }
 }  // namespace cup2d
@@ -103,8 +105,15 @@ int ensure_frozen(cup2d_ctx* c) {
   return CUP2D_OK;
 }
 
-int sync_and_check(cup2d_ctx* c) {
+int sync_all(cup2d_ctx* c) {
+  RC(join_obstacles(c));
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->aux));
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return CUP2D_OK;
+}
+
+int sync_and_check(cup2d_ctx* c) {
+  RC(sync_all(c));
   int err = 0;
   CUP2D_HIP_CHECK(c, hipMemcpy(&err, &c->st->error, sizeof(int), hipMemcpyDeviceToHost));
   if (err == CUP2D_ERR_BODY_OUT) return fail(c, err, "a body left the domain");
@@ -121,6 +130,7 @@ int need_fft(cup2d_ctx* c) {
 int pressure_single(cup2d_ctx* c) {
   RC(need_fft(c));
   RC(enqueue_pressure_rhs(c));
+  RC(join_obstacles(c));
   RC(enqueue_rhs_obstacles(c));
   RC(enqueue_fade(c));
   RC(enqueue_poisson_fft(c, c->rhs, c->pres));
@@ -133,10 +143,11 @@ int pressure_single(cup2d_ctx* c) {
 // Simulation::advance.  compute_dt: dt from the findMaxU partials of the
 // previous step (device loop) instead of the value already on the device.
 int advance_enqueued(cup2d_ctx* c, int compute_dt) {
-  RC(enqueue_put_objects(c, compute_dt));
+  RC(enqueue_put_objects(c, compute_dt, 1));
   RC(enqueue_advdiff(c, 1));
   RC(pressure_single(c));
-  RC(enqueue_forces(c, 1));
+  RC(enqueue_forces(c, 1, 1));
+  c->host_step += 1;
   return CUP2D_OK;
 }
 }  // namespace
@@ -188,6 +199,14 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
       return bail(CUP2D_ERR_HIP);
     }
     c->own_stream = true;
+  }
+  if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_begin, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_obst, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_pen, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc(&c->fsnap, sizeof(ForceSnap) * MAX_SHAPES) != hipSuccess) {
+    c->err = "side stream creation failed";
+    return bail(CUP2D_ERR_HIP);
   }
   const size_t n = (size_t)g.NX * g.NY;
   double** planes[] = {&c->u, &c->v, &c->tu, &c->tv, &c->pres, &c->rhs,
@@ -267,6 +286,11 @@ int cup2d_destroy(cup2d_ctx* c) {
   if (c->band_blocks) (void)hipFree(c->band_blocks);
   for (Timer& t : c->timers)
     for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
+  if (c->aux) { (void)hipStreamSynchronize(c->aux); (void)hipStreamDestroy(c->aux); }
+  if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
+  if (c->ev_obst) (void)hipEventDestroy(c->ev_obst);
+  if (c->ev_pen) (void)hipEventDestroy(c->ev_pen);
+  if (c->fsnap) (void)hipFree(c->fsnap);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CUP2D_OK;
@@ -357,7 +381,7 @@ int cup2d_field_device_ptr(cup2d_ctx* c, int f, int comp, void** dptr) {
 int cup2d_get_scalars(cup2d_ctx* c, double* time, double* dt, int* step, double uinf[2],
                       double* h) {
   CHECK_CTX(c);
-  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  RC(sync_all(c));
   StepState s;
   CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
   if (time) *time = s.time;
@@ -370,7 +394,8 @@ int cup2d_get_scalars(cup2d_ctx* c, double* time, double* dt, int* step, double 
 
 int cup2d_set_scalars(cup2d_ctx* c, double time, int step, double uinfx, double uinfy) {
   CHECK_CTX(c);
-  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  RC(sync_all(c));
+  c->host_step = step;
   StepState s;
   CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
   s.time = time; s.step = step; s.uinf[0] = uinfx; s.uinf[1] = uinfy;
@@ -381,7 +406,7 @@ int cup2d_set_scalars(cup2d_ctx* c, double time, int step, double uinfx, double 
 int cup2d_get_shape(cup2d_ctx* c, int i, cup2d_shape_state* out) {
   CHECK_CTX(c);
   if (i < 0 || i >= c->g.nshapes || !out) return fail(c, CUP2D_ERR_ARG, "bad shape id");
-  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  RC(sync_all(c));
   CUP2D_HIP_CHECK(c, hipMemcpy(out, c->ss + i, sizeof(*out), hipMemcpyDeviceToHost));
   return CUP2D_OK;
 }
@@ -389,6 +414,8 @@ int cup2d_get_shape(cup2d_ctx* c, int i, cup2d_shape_state* out) {
 int cup2d_reset(cup2d_ctx* c) {
   CHECK_CTX(c);
   RC(ensure_frozen(c));
+  RC(sync_all(c));
+  c->host_step = 0;
   RC(enqueue_shapes_reset(c));
   RC(enqueue_ic(c));
   RC(enqueue_put_objects(c, 0));

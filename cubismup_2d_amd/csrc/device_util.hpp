@@ -52,6 +52,31 @@ __device__ inline double block_max(double v, double* lds) {
   return s;
 }
 
+// The same reductions for workgroups of any power-of-two size blockDim.x
+// (32..1024): wave butterflies over min(64, blockDim.x) lanes.
+__device__ inline double block_sum_dyn(double v, double* lds) {
+  const int nt = blockDim.x, W = nt < 64 ? nt : 64;
+  for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+  const int NW = (nt + 63) / 64, w = threadIdx.x / 64, l = threadIdx.x % 64;
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  double s = lds[0];
+  for (int i = 1; i < NW; ++i) s += lds[i];
+  return s;
+}
+__device__ inline double block_max_dyn(double v, double* lds) {
+  const int nt = blockDim.x, W = nt < 64 ? nt : 64;
+  for (int o = W / 2; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, W));
+  const int NW = (nt + 63) / 64, w = threadIdx.x / 64, l = threadIdx.x % 64;
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  double s = lds[0];
+  for (int i = 1; i < NW; ++i) s = dmax(s, lds[i]);
+  return s;
+}
+
 // Sum of n values (fixed order independent of the caller): thread t sums
 // t, t+NT, ... sequentially, then block_sum.  Every workgroup that calls it
 // on the same data gets bit-identical results.

@@ -165,15 +165,69 @@ __device__ __forceinline__ void stockham_pass(double2* lds, int base, int stride
   __syncthreads();
 }
 
-template <int L, int NS, bool FWD>
-__device__ __forceinline__ void fft_passes(double2* lds, int base, int stride, int t,
-                                           const double2* __restrict__ tw) {
-  if constexpr (NS < L) {
-    constexpr int REM = L / NS;
-    constexpr int R = REM >= 16 ? 16 : REM;
-    stockham_pass<L, R, NS, FWD>(lds, base, stride, t, tw);
-    fft_passes<L, NS * R, FWD>(lds, base, stride, t, tw);
+// First pass (radix 16, span 1) straight from registers: x[q] holds element
+// t + q*NT, which is the Stockham input of butterfly j = t.  No twiddles.
+template <int L, bool FWD>
+__device__ __forceinline__ void pass_first(double2* x, double2* lds, int base, int stride, int t) {
+  dft<16, FWD>(x);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) lds[base + pad(t * 16 + r) * stride] = x[r];
+  __syncthreads();
+}
+
+// Last pass (span NS = L/R) into registers: its outputs j + r*NS land on
+// element t + q*NT with q = p + (16/R)*r, i.e. the same strided order the
+// first pass reads.  Ends with a barrier so the LDS can be reused.
+template <int L, int R, int NS, bool FWD>
+__device__ __forceinline__ void pass_last(double2* lds, int base, int stride, int t,
+                                          const double2* __restrict__ tw, double2* x) {
+  constexpr int NT = L / 16, NBF = L / R, PER = 16 / R;
+  static_assert(NS * R == L, "last pass");
+  double2 v[PER][R];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int j = t + p * NT;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[p][r] = lds[base + pad(j + r * NBF) * stride];
   }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int j = t + p * NT;
+    constexpr int TSTEP = L / (NS * R);
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      double2 w = tw[j * TSTEP * r];
+      if (!FWD) w.y = -w.y;
+      v[p][r] = cmul(v[p][r], w);
+    }
+    dft<R, FWD>(v[p]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[p + PER * r] = v[p][r];
+  }
+}
+
+template <int L, int NS, bool FWD>
+__device__ __forceinline__ void fft_tail(double2* lds, int base, int stride, int t,
+                                         const double2* __restrict__ tw, double2* x) {
+  constexpr int REM = L / NS;
+  constexpr int R = REM >= 16 ? 16 : REM;
+  if constexpr (NS * R == L) {
+    pass_last<L, R, NS, FWD>(lds, base, stride, t, tw, x);
+  } else {
+    stockham_pass<L, R, NS, FWD>(lds, base, stride, t, tw);
+    fft_tail<L, NS * R, FWD>(lds, base, stride, t, tw, x);
+  }
+}
+
+// L-point complex FFT (L >= 32, power of two) by NT = L/16 threads; x[q] is
+// element t + q*NT on entry and the transform's element t + q*NT on exit.
+// The sequence is staged in LDS at lds[base + pad(i)*stride] between passes.
+template <int L, bool FWD>
+__device__ __forceinline__ void fft_regs(double2* x, double2* lds, int base, int stride, int t,
+                                         const double2* __restrict__ tw) {
+  pass_first<L, FWD>(x, lds, base, stride, t);
+  fft_tail<L, 16, FWD>(lds, base, stride, t, tw, x);
 }
 
 // ---------------------------------------------------------- rows forward
@@ -193,30 +247,35 @@ __global__ __launch_bounds__(NP * L / 16) void k_rows_fwd(const double* __restri
   const int base = f * PADL;
   const double* r0 = in + (size_t)y0 * in_pitch;
   const double* r1 = r0 + in_pitch;
+  double2 x[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int i = t + r * NT;
+  for (int q = 0; q < 16; ++q) {
+    const int i = t + q * NT;
     double a = 0, b = 0;
     if (valid && i < ncols) { a = r0[i]; b = r1[i]; }
-    lds[base + pad(i)] = make_double2(a, b);
+    x[q] = make_double2(a, b);
   }
+  fft_regs<L, true>(x, lds, base, 1, t, tw);
+  // split z = a + i b: A_k = (Z_k + conj Z_{L-k})/2, B_k = (Z_k - conj Z_{L-k})/(2i);
+  // the partners Z_{L-k} (upper half) go through LDS
+#pragma unroll
+  for (int q = 8; q < 16; ++q) lds[base + pad(t + q * NT)] = x[q];
   __syncthreads();
-  fft_passes<L, 1, true>(lds, base, 1, t, tw);
   if (!valid) return;
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int k = t + r * NT;  // 0 .. L/2-1
-    const double2 Z = lds[base + pad(k)];
+  for (int q = 0; q < 8; ++q) {
+    const int k = t + q * NT;  // 0 .. L/2-1
+    const double2 Z = x[q];
     double2 A, B;
     if (k == 0) {
-      const double2 ZN = lds[base + pad(L / 2)];
+      const double2 ZN = x[8];  // element L/2
       A = make_double2(Z.x, ZN.x);
       B = make_double2(Z.y, ZN.y);
     } else {
       const double2 Zm = lds[base + pad(L - k)];
-      const double2 s = make_double2(Z.x + Zm.x, Z.y - Zm.y);  // Z + conj(Zm)
-      const double2 d = make_double2(Z.x - Zm.x, Z.y + Zm.y);  // Z - conj(Zm)
-      A = make_double2(0.5 * s.x, 0.5 * s.y);
+      const double2 sm = make_double2(Z.x + Zm.x, Z.y - Zm.y);  // Z + conj(Zm)
+      const double2 d = make_double2(Z.x - Zm.x, Z.y + Zm.y);   // Z - conj(Zm)
+      A = make_double2(0.5 * sm.x, 0.5 * sm.y);
       B = make_double2(0.5 * d.y, -0.5 * d.x);
     }
     const int cg = k / G, c = k % G;
@@ -238,9 +297,12 @@ __global__ __launch_bounds__(NP * L / 16) void k_rows_inv(const double2* __restr
   const int y0 = 2 * (blockIdx.x * NP + f);
   const bool valid = y0 < nrows;
   const int base = f * PADL;
+  double2 x[16];
+  // Z_k = A_k + i B_k and Z_{L-k} = conj(A_k) + i conj(B_k); DC and Nyquist
+  // of both rows are packed in slot 0 (real parts only)
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int k = t + r * NT;
+  for (int q = 0; q < 8; ++q) {
+    const int k = t + q * NT;
     double2 A = make_double2(0, 0), B = make_double2(0, 0);
     if (valid) {
       const double2* s = spec + ((size_t)(k / G) * spec_rows + y0) * G + (k % G);
@@ -248,25 +310,30 @@ __global__ __launch_bounds__(NP * L / 16) void k_rows_inv(const double2* __restr
       B = s[G];
     }
     if (k == 0) {
-      lds[base + pad(0)] = make_double2(A.x, B.x);
-      lds[base + pad(L / 2)] = make_double2(A.y, B.y);
+      x[0] = make_double2(A.x, B.x);
+      x[8] = make_double2(A.y, B.y);
     } else {
-      lds[base + pad(k)] = make_double2(A.x - B.y, A.y + B.x);
+      x[q] = make_double2(A.x - B.y, A.y + B.x);
       lds[base + pad(L - k)] = make_double2(A.x + B.y, -A.y + B.x);
     }
   }
   __syncthreads();
-  fft_passes<L, 1, false>(lds, base, 1, t, tw);
+#pragma unroll
+  for (int q = 8; q < 16; ++q) {
+    const int i = t + q * NT;
+    if (i != L / 2) x[q] = lds[base + pad(i)];
+  }
+  __syncthreads();
+  fft_regs<L, false>(x, lds, base, 1, t, tw);
   if (!valid) return;
   double* o0 = out + (size_t)y0 * out_pitch;
   double* o1 = o0 + out_pitch;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int i = t + r * NT;
+  for (int q = 0; q < 16; ++q) {
+    const int i = t + q * NT;
     if (i < ncols) {
-      const double2 z = lds[base + pad(i)];
-      o0[i] = z.x;
-      o1[i] = z.y;
+      o0[i] = x[q].x;
+      o1[i] = x[q].y;
     }
   }
 }
@@ -291,82 +358,97 @@ __device__ __forceinline__ double periodic_factor(int kx, int ky, const ColParam
 }
 
 template <int LC, int MODE>
-__global__ __launch_bounds__(G * LC / 16) void k_cols(double2* __restrict__ spec, ColParams P,
-                                                      const double2* __restrict__ tw) {
+__device__ __forceinline__ double col_symbol(int kx, int cg, int c, int ky, const ColParams& P) {
+  if constexpr (MODE == 0) {
+    return periodic_factor(kx, ky, P);
+  } else {
+    const int kyy = ky <= P.NY ? ky : LC - ky;
+    return P.ghat[((size_t)cg * (P.NY + 1) + kyy) * G + c];
+  }
+}
+
+// Columns per workgroup (CW) of a column group of G: CW = G keeps a whole
+// group per workgroup; CW = G/2 halves the LDS so two workgroups share a CU.
+// The two halves of a group run on the same XCD (block b runs on XCD b % 8)
+// so the second half of each 64 B chunk is an L2 hit.
+#ifndef CUP2D_FFT_CW
+#define CUP2D_FFT_CW 2
+#endif
+constexpr int CW = CUP2D_FFT_CW;
+static_assert(G % CW == 0, "CW");
+
+__device__ __forceinline__ void col_group_of_block(int b, int nblocks, int* cg, int* half) {
+  constexpr int H = G / CW;
+  if (H == 1 || nblocks % (8 * H) != 0) {
+    *cg = b / H;
+    *half = b % H;
+    return;
+  }
+  const int xcd = b % 8, local = b / 8;        // consecutive locals share an XCD
+  const int per_xcd = nblocks / 8;             // blocks per XCD
+  const int lin = xcd * per_xcd + local;       // XCD-contiguous numbering
+  *cg = lin / H;
+  *half = lin % H;
+}
+
+template <int LC, int MODE>
+__global__ __launch_bounds__(CW * LC / 16) void k_cols(double2* __restrict__ spec, ColParams P,
+                                                       const double2* __restrict__ tw) {
   constexpr int NT = LC / 16;
   extern __shared__ double2 lds[];
-  const int c = threadIdx.x % G, t = threadIdx.x / G;
-  const int cg = blockIdx.x;
+  const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
+  int cg, half;
+  col_group_of_block(blockIdx.x, gridDim.x, &cg, &half);
+  const int c = half * CW + cl;  // column inside the group
   double2* col = spec + (size_t)cg * P.rows * G;
+  double2 x[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int i = t + r * NT;
-    double2 z = make_double2(0, 0);
-    if (MODE != 1 || i < P.rows) z = col[(size_t)i * G + c];
-    lds[pad(i) * G + c] = z;
+  for (int q = 0; q < 16; ++q) {
+    const int i = t + q * NT;
+    x[q] = (MODE != 1 || i < P.rows) ? col[(size_t)i * G + c] : make_double2(0, 0);
   }
-  __syncthreads();
-  fft_passes<LC, 1, true>(lds, c, G, t, tw);
+  fft_regs<LC, true>(x, lds, cl, CW, t, tw);
   if constexpr (MODE == 2) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int i = t + r * NT;
-      col[(size_t)i * G + c] = lds[pad(i) * G + c];
-    }
+    for (int q = 0; q < 16; ++q) col[(size_t)(t + q * NT) * G + c] = x[q];
     return;
   } else {
     const int kx = cg * G + c;
-    if (cg == 0) {
+    if (cg == 0 && half == 0) {
       // column 0 packs the DC (kx=0) and Nyquist (kx=NX/2) columns as
       // c = dc + i nq; with real, even symbols s0, sN:
       // C'(k) = C(k) (s0+sN)/2 + conj(C(-k)) (s0-sN)/2
-      double2 o[16];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int i = t + r * NT;
-        const double2 z = lds[pad(i) * G + c];
-        double s0, sN = 0.0;
-        if constexpr (MODE == 0) {
-          s0 = periodic_factor(kx, i, P);
-          if (kx == 0) sN = periodic_factor(P.NX / 2, i, P);
-        } else {
-          const int kyy = i <= P.NY ? i : LC - i;
-          s0 = P.ghat[(size_t)kyy * G + c];
-          if (kx == 0) sN = P.ghatN[kyy];
-        }
-        if (kx == 0) {
-          const double2 m = lds[pad((LC - i) % LC) * G + c];
-          const double a = 0.5 * (s0 + sN), b = 0.5 * (s0 - sN);
-          o[r] = make_double2(z.x * a + m.x * b, z.y * a - m.y * b);
-        } else {
-          o[r] = make_double2(z.x * s0, z.y * s0);
-        }
-      }
+      for (int q = 0; q < 16; ++q) lds[pad(t + q * NT) * CW + cl] = x[q];
       __syncthreads();
 #pragma unroll
-      for (int r = 0; r < 16; ++r) lds[pad(t + r * NT) * G + c] = o[r];
+      for (int q = 0; q < 16; ++q) {
+        const int i = t + q * NT;
+        const double s0 = col_symbol<LC, MODE>(kx, cg, c, i, P);
+        if (kx == 0) {
+          double sN;
+          if constexpr (MODE == 0) sN = periodic_factor(P.NX / 2, i, P);
+          else sN = P.ghatN[i <= P.NY ? i : LC - i];
+          const double2 m = lds[pad((LC - i) % LC) * CW + cl];
+          const double a = 0.5 * (s0 + sN), b = 0.5 * (s0 - sN);
+          x[q] = make_double2(x[q].x * a + m.x * b, x[q].y * a - m.y * b);
+        } else {
+          x[q] = make_double2(x[q].x * s0, x[q].y * s0);
+        }
+      }
       __syncthreads();
     } else {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int i = t + r * NT;
-        double s;
-        if constexpr (MODE == 0) {
-          s = periodic_factor(kx, i, P);
-        } else {
-          const int kyy = i <= P.NY ? i : LC - i;
-          s = P.ghat[((size_t)cg * (P.NY + 1) + kyy) * G + c];
-        }
-        const double2 z = lds[pad(i) * G + c];
-        lds[pad(i) * G + c] = make_double2(z.x * s, z.y * s);
+      for (int q = 0; q < 16; ++q) {
+        const double s = col_symbol<LC, MODE>(kx, cg, c, t + q * NT, P);
+        x[q] = make_double2(x[q].x * s, x[q].y * s);
       }
-      __syncthreads();
     }
-    fft_passes<LC, 1, false>(lds, c, G, t, tw);
+    fft_regs<LC, false>(x, lds, cl, CW, t, tw);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int i = t + r * NT;
-      if (MODE != 1 || i < P.rows) col[(size_t)i * G + c] = lds[pad(i) * G + c];
+    for (int q = 0; q < 16; ++q) {
+      const int i = t + q * NT;
+      if (MODE != 1 || i < P.rows) col[(size_t)i * G + c] = x[q];
     }
   }
 }
@@ -420,9 +502,9 @@ static void launch_rows_inv(cup2d_ctx* c, const double2* spec, int spec_rows, do
 template <int LC, int MODE>
 static void launch_cols(cup2d_ctx* c, double2* spec, int ncg, const ColParams& P,
                         const double2* tw) {
-  const size_t shm = (size_t)(LC + LC / 16) * G * sizeof(double2);
-  hipLaunchKernelGGL((k_cols<LC, MODE>), dim3(ncg), dim3(G * LC / 16), shm, c->stream, spec, P,
-                     tw);
+  const size_t shm = (size_t)(LC + LC / 16) * CW * sizeof(double2);
+  hipLaunchKernelGGL((k_cols<LC, MODE>), dim3(ncg * (G / CW)), dim3(CW * LC / 16), shm, c->stream,
+                     spec, P, tw);
 }
 
 #define CUP2D_POW2_CASES(M, ...)                                                 \
@@ -481,7 +563,7 @@ static void set_rows_attr_L() {
 }
 template <int LC>
 static void set_cols_attr() {
-  const int shm = (LC + LC / 16) * G * (int)sizeof(double2);
+  const int shm = (LC + LC / 16) * CW * (int)sizeof(double2);
   (void)hipFuncSetAttribute((const void*)k_cols<LC, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
   (void)hipFuncSetAttribute((const void*)k_cols<LC, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
   (void)hipFuncSetAttribute((const void*)k_cols<LC, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);

@@ -45,6 +45,14 @@ struct StepState {
   double maxU;   // last findMaxU result
   double fade_corr;  // cub2rhs correction factor of the last solve
   double flux_corr;  // advDiff boundary-flux correction of the last step
+  double dt_done;    // dt of the step whose forces are pending (set by the body update)
+  double pad2_;
+};
+
+// Body state captured by the body update for ComputeForces, which may run on
+// the side stream while the next step already moves the bodies.
+struct ForceSnap {
+  double C[2], u, v, omega, pad_;
 };
 
 // Constant run parameters passed by value to kernels.
@@ -99,6 +107,13 @@ struct cup2d_ctx {
   cup2d::Grid g;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // side stream: the PutObjectsOnGrid chain after the list builder runs there,
+  // overlapping advDiff and the RHS (neither reads obstacle data)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_obst = nullptr, ev_pen = nullptr;
+  bool obst_pending = false;
+  int host_step = 0;                       // step index for the CFL ramp (Simulation.cpp:264)
+  cup2d::ForceSnap* fsnap = nullptr;       // device [MAX_SHAPES]
   int device = 0;
   std::string err;
 
@@ -153,11 +168,15 @@ int launch_check(cup2d_ctx* c, const char* what);
 
 extern const char* const TIMER_NAMES[T_NTIMERS];
 void timer_mark(cup2d_ctx* c, int id, bool start);
+void timer_mark_on(cup2d_ctx* c, int id, bool start, hipStream_t s);
 struct TimerScope {
   cup2d_ctx* c;
   int id;
-  TimerScope(cup2d_ctx* c_, int id_) : c(c_), id(id_) { timer_mark(c, id, true); }
-  ~TimerScope() { timer_mark(c, id, false); }
+  hipStream_t s;
+  TimerScope(cup2d_ctx* c_, int id_, hipStream_t s_ = nullptr) : c(c_), id(id_), s(s_) {
+    timer_mark_on(c, id, true, s);
+  }
+  ~TimerScope() { timer_mark_on(c, id, false, s); }
 };
 
 // stencils.hip
@@ -185,10 +204,11 @@ int enqueue_poisson_fft(cup2d_ctx* c, const double* rhs, double* sol);
 int obstacles_alloc(cup2d_ctx* c);
 void obstacles_free(cup2d_ctx* c);
 constexpr int OBST_LIST_CAP = 2048;  // slots handled by the list builder
-int enqueue_put_objects(cup2d_ctx* c, int compute_dt);
+int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap = 0);
+int join_obstacles(cup2d_ctx* c);
 int enqueue_rhs_obstacles(cup2d_ctx* c);
 int enqueue_body_update(cup2d_ctx* c);
 int enqueue_penalize(cup2d_ctx* c, double* maxpart);
-int enqueue_forces(cup2d_ctx* c, int end_step);
+int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux = 0);
 int enqueue_shapes_reset(cup2d_ctx* c);
 }  // namespace cup2d

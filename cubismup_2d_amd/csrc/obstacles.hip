@@ -102,7 +102,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
                                                    const double* __restrict__ maxpart,
                                                    int nmaxpart,
                                                    const double* __restrict__ ramp,
-                                                   int compute_dt) {
+                                                   int compute_dt, int step_idx) {
   __shared__ int s_a[LCAP], s_b[LCAP], s_c[LCAP], s_d[LCAP];
   __shared__ int s_tmp[NT];
   __shared__ int s_cstart[MAX_SHAPES + 1];
@@ -125,12 +125,17 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       const double dtBody = maxUb < 2.2e-16 ? 1 : h / maxUb;
       const double mn = dmin(dmin(dtCFL, dtFourier), dtBody);
       double dt = g.CFL * mn;
-      if (st->step < 100) dt = ramp[st->step] * mn;
+      if (step_idx < 100) dt = ramp[step_idx] * mn;
       st->dt = dt;
       st->maxU = maxU;
     }
   }
-  for (int u = t; u < nub_prev; u += NT) obmap[ob.ub_block[u]] = 0;
+  for (int u = t; u < nub_prev; u += NT) {
+    const int b = ob.ub_block[u];
+    obmap[b] = 0;
+    ob.prev_blocks[u] = b;  // cleared by k_obst_create unless reused
+  }
+  if (t == 0) st->nprev = nub_prev;
   __syncthreads();
   if (t == 0) {
     const double dt = st->dt;
@@ -270,13 +275,9 @@ __device__ inline double shape_sum(const Obstacles& ob, int k, int q, double* ld
   return block_sum_array<NT>(ob.part + (size_t)s0 * SLOT_PART + q, n, SLOT_PART, lds);
 }
 
-// PutObjectsOnGrid.cpp:191-197 restricted to the blocks written last step
-// (every other cell already holds the cleared values).
-__global__ __launch_bounds__(NT) void k_obst_clear(Grid g, const StepState* __restrict__ st,
-                                                   Obstacles ob, double* sdf, double* chi,
-                                                   double* udx, double* udy, double* invrho) {
-  if ((int)blockIdx.x >= st->nub) return;
-  const int b = ob.ub_block[blockIdx.x];
+// PutObjectsOnGrid.cpp:191-197 (chi=0, sdf=-1, uDef=0, invRho=1) for one block
+__device__ inline void clear_block(const Grid& g, int b, double* sdf, double* chi, double* udx,
+                                   double* udy, double* invrho) {
   const int bx = b % g.bpdx, by = b / g.bpdx;
   for (int k = threadIdx.x; k < BS * BS; k += NT) {
     const size_t d = (size_t)(by * BS + k / BS) * g.NX + bx * BS + k % BS;
@@ -284,17 +285,30 @@ __global__ __launch_bounds__(NT) void k_obst_clear(Grid g, const StepState* __re
   }
 }
 
-// Disk::create / FillBlocks_Cylinder::operator() (ShapeLibrary.cpp:21-34):
-// dist = r - |x - c|, tmp = max(tmp, dist), rho = rhoS; per distinct block,
-// shapes in order (Disk::create runs shape by shape, PutObjectsOnGrid.cpp:224).
+// Workgroups [0, nub): one per obstacle block of this step: clear the block,
+// then Disk::create / FillBlocks_Cylinder::operator() (ShapeLibrary.cpp:21-34)
+// for its shapes in order: dist = r - |x - c|, sdf = max(sdf, dist),
+// rho = rhoS.  Workgroups [nub, nub+nprev): clear the blocks of the previous
+// step that are not obstacle blocks now (every other cell already holds the
+// cleared values of PutObjectsOnGrid.cpp:191-197).
 __global__ __launch_bounds__(NT) void k_obst_create(Grid g, const ShapeParams* __restrict__ sp,
                                                     const cup2d_shape_state* __restrict__ ss,
                                                     const StepState* __restrict__ st,
-                                                    Obstacles ob, double* __restrict__ sdf) {
-  if ((int)blockIdx.x >= st->nub) return;
-  const int b = ob.ub_block[blockIdx.x];
+                                                    Obstacles ob,
+                                                    const unsigned char* __restrict__ obmap,
+                                                    double* __restrict__ sdf, double* chi,
+                                                    double* udx, double* udy, double* invrho) {
+  const int nub = st->nub, w = blockIdx.x;
+  if (w >= nub) {
+    if (w >= nub + st->nprev) return;
+    const int b = ob.prev_blocks[w - nub];
+    if (!obmap[b]) clear_block(g, b, sdf, chi, udx, udy, invrho);
+    return;
+  }
+  const int b = ob.ub_block[w];
   const int bx = b % g.bpdx, by = b / g.bpdx;
-  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  clear_block(g, b, sdf, chi, udx, udy, invrho);
+  const int s0 = ob.ub_start[w], cnt = ob.ub_count[w];
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q];
     const int k = ob.slot_shape[s];
@@ -315,112 +329,106 @@ __global__ __launch_bounds__(NT) void k_obst_create(Grid g, const ShapeParams* _
   }
 }
 
-// PutObjectsOnGrid::putChiOnGrid (PutObjectsOnGrid.cpp:17-116): Towers chi,
-// surface delta + normals (ObstacleBlock::write, ObstacleBlock.h:336-346),
-// centre-of-mass partial sums.  One workgroup per slot.
+// PutObjectsOnGrid::putChiOnGrid (PutObjectsOnGrid.cpp:17-116) per obstacle
+// block, shapes in order: Towers chi from the shape's own distance inside the
+// block and the grid SDF across block edges, surface delta + normals
+// (ObstacleBlock::write, ObstacleBlock.h:336-346), centre-of-mass partials,
+// then grid CHI = max and the invRho blend (PutObjectsOnGrid.cpp:69-73).
 __global__ __launch_bounds__(NT) void k_obst_chi(Grid g, const cup2d_shape_state* __restrict__ ss,
                                                  const StepState* __restrict__ st, Obstacles ob,
-                                                 const double* __restrict__ sdf) {
+                                                 const double* __restrict__ sdf,
+                                                 double* __restrict__ chi,
+                                                 double* __restrict__ invrho) {
   __shared__ double lds[8];
-  if ((int)blockIdx.x >= st->nob) return;
-  const int s = blockIdx.x;
-  const int k = ob.slot_shape[s], b = ob.slot_block[s];
-  const int bx = b % g.bpdx, by = b / g.bpdx;
-  SlotView o = slot(ob, s);
-  const double h = g.h, i2h = 0.5 / h, fac = 0.5 * h;
-  const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
-  double _x = 0, _y = 0, _m = 0, _ns = 0;
-  for (int i = threadIdx.x; i < BS * BS; i += NT) {
-    const int ix = i % BS, iy = i / BS;
-    const int gx = bx * BS + ix, gy = by * BS + iy;
-    const double sd = o.dist[i];
-    const double distPx = ix + 1 == BS ? lab_at(sdf, g, gx + 1, gy) : o.dist[i + 1];
-    const double distMx = ix == 0 ? lab_at(sdf, g, gx - 1, gy) : o.dist[i - 1];
-    const double distPy = iy + 1 == BS ? lab_at(sdf, g, gx, gy + 1) : o.dist[i + BS];
-    const double distMy = iy == 0 ? lab_at(sdf, g, gx, gy - 1) : o.dist[i - BS];
-    double X;
-    if (sd > +2 * h || sd < -2 * h) {
-      X = sd > 0 ? 1 : 0;
-    } else {
-      const double IplusX = distPx < 0 ? 0 : distPx, IminuX = distMx < 0 ? 0 : distMx;
-      const double IplusY = distPy < 0 ? 0 : distPy, IminuY = distMy < 0 ? 0 : distMy;
-      const double gradIX = i2h * (IplusX - IminuX), gradIY = i2h * (IplusY - IminuY);
-      const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
-      const double gradUSq = gradUX * gradUX + gradUY * gradUY;
-      const double denum = 1 / (gradUSq < EPSD ? EPSD : gradUSq);
-      X = (gradIX * gradUX + gradIY * gradUY) * denum;
-    }
-    o.chi[i] = X;
-    if (X > 0) {
-      const double p0 = cell_pos(bx, ix, g.hb, h), p1 = cell_pos(by, iy, g.hb, h);
-      const double rho = o.rho[i];
-      _x += rho * X * h * h * (p0 - C0);
-      _y += rho * X * h * h * (p1 - C1);
-      _m += rho * X * h * h;
-    }
-    if (sd > +2 * h || sd < -2 * h) continue;
-    const double HplusX = fabs(distPx) < EPSD ? 0.5 : (distPx < 0 ? 0 : 1);
-    const double HminuX = fabs(distMx) < EPSD ? 0.5 : (distMx < 0 ? 0 : 1);
-    const double HplusY = fabs(distPy) < EPSD ? 0.5 : (distPy < 0 ? 0 : 1);
-    const double HminuY = fabs(distMy) < EPSD ? 0.5 : (distMy < 0 ? 0 : 1);
-    const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
-    const double gradHX = (HplusX - HminuX), gradHY = (HplusY - HminuY);
-    const double gradUSq = gradUX * gradUX + gradUY * gradUY;
-    const double denum = 1 / (gradUSq < EPSD ? EPSD : gradUSq);
-    const double D = fac * (gradHX * gradUX + gradHY * gradUY) * denum;
-    if (D > 0) {
-      o.sd[i] = D;
-      o.sx[i] = -D * gradUX;
-      o.sy[i] = -D * gradUY;
-      _ns += 1;
-    }
-  }
-  _x = block_sum<NT>(_x, lds);
-  _y = block_sum<NT>(_y, lds);
-  _m = block_sum<NT>(_m, lds);
-  _ns = block_sum<NT>(_ns, lds);
-  if (threadIdx.x == 0) {
-    o.part[P_COM] = _x; o.part[P_COM + 1] = _y; o.part[P_COM + 2] = _m;
-    o.part[P_NSURF] = _ns;
-  }
-}
-
-// PutObjectsOnGrid.cpp:69-73 (grid CHI = max, invRho blend; per distinct
-// block, shapes in order) and :118-122 (centre-of-mass shift and mass).
-__global__ __launch_bounds__(NT) void k_obst_chigrid(Grid g, cup2d_shape_state* __restrict__ ss,
-                                                     const StepState* __restrict__ st,
-                                                     Obstacles ob, double* __restrict__ chi,
-                                                     double* __restrict__ invrho) {
   if ((int)blockIdx.x >= st->nub) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  const double h = g.h, i2h = 0.5 / h, fac = 0.5 * h;
   for (int q = 0; q < cnt; ++q) {
-    SlotView o = slot(ob, ob.ub_slots[s0 + q]);
+    const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
+    SlotView o = slot(ob, s);
+    const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
+    double _x = 0, _y = 0, _m = 0, _ns = 0;
     for (int i = threadIdx.x; i < BS * BS; i += NT) {
-      const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
-      const double X = o.chi[i];
+      const int ix = i % BS, iy = i / BS;
+      const int gx = bx * BS + ix, gy = by * BS + iy;
+      const double sd = o.dist[i];
+      const double distPx = ix + 1 == BS ? lab_at(sdf, g, gx + 1, gy) : o.dist[i + 1];
+      const double distMx = ix == 0 ? lab_at(sdf, g, gx - 1, gy) : o.dist[i - 1];
+      const double distPy = iy + 1 == BS ? lab_at(sdf, g, gx, gy + 1) : o.dist[i + BS];
+      const double distMy = iy == 0 ? lab_at(sdf, g, gx, gy - 1) : o.dist[i - BS];
+      double X;
+      if (sd > +2 * h || sd < -2 * h) {
+        X = sd > 0 ? 1 : 0;
+      } else {
+        const double IplusX = distPx < 0 ? 0 : distPx, IminuX = distMx < 0 ? 0 : distMx;
+        const double IplusY = distPy < 0 ? 0 : distPy, IminuY = distMy < 0 ? 0 : distMy;
+        const double gradIX = i2h * (IplusX - IminuX), gradIY = i2h * (IplusY - IminuY);
+        const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
+        const double gradUSq = gradUX * gradUX + gradUY * gradUY;
+        const double denum = 1 / (gradUSq < EPSD ? EPSD : gradUSq);
+        X = (gradIX * gradUX + gradIY * gradUY) * denum;
+      }
+      o.chi[i] = X;
+      const size_t d = (size_t)gy * g.NX + gx;
       if (X >= chi[d]) {
         chi[d] = X;
         invrho[d] = X / o.rho[i] + (1 - X) * invrho[d];
       }
-    }
-  }
-  if (blockIdx.x == 0) {
-    __shared__ double lds[8];
-    for (int k = 0; k < g.nshapes; ++k) {
-      const double _x = shape_sum(ob, k, P_COM, lds), _y = shape_sum(ob, k, P_COM + 1, lds);
-      const double _m = shape_sum(ob, k, P_COM + 2, lds);
-      if (threadIdx.x == 0 && _m > EPSD) {
-        ss[k].centerOfMass[0] += _x / _m;
-        ss[k].centerOfMass[1] += _y / _m;
-        ss[k].M = _m;
+      if (X > 0) {
+        const double p0 = cell_pos(bx, ix, g.hb, h), p1 = cell_pos(by, iy, g.hb, h);
+        const double rho = o.rho[i];
+        _x += rho * X * h * h * (p0 - C0);
+        _y += rho * X * h * h * (p1 - C1);
+        _m += rho * X * h * h;
       }
+      if (sd > +2 * h || sd < -2 * h) continue;
+      const double HplusX = fabs(distPx) < EPSD ? 0.5 : (distPx < 0 ? 0 : 1);
+      const double HminuX = fabs(distMx) < EPSD ? 0.5 : (distMx < 0 ? 0 : 1);
+      const double HplusY = fabs(distPy) < EPSD ? 0.5 : (distPy < 0 ? 0 : 1);
+      const double HminuY = fabs(distMy) < EPSD ? 0.5 : (distMy < 0 ? 0 : 1);
+      const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
+      const double gradHX = (HplusX - HminuX), gradHY = (HplusY - HminuY);
+      const double gradUSq = gradUX * gradUX + gradUY * gradUY;
+      const double denum = 1 / (gradUSq < EPSD ? EPSD : gradUSq);
+      const double D = fac * (gradHX * gradUX + gradHY * gradUY) * denum;
+      if (D > 0) {
+        o.sd[i] = D;
+        o.sx[i] = -D * gradUX;
+        o.sy[i] = -D * gradUY;
+        _ns += 1;
+      }
+    }
+    _x = block_sum<NT>(_x, lds);
+    _y = block_sum<NT>(_y, lds);
+    _m = block_sum<NT>(_m, lds);
+    _ns = block_sum<NT>(_ns, lds);
+    if (threadIdx.x == 0) {
+      o.part[P_COM] = _x; o.part[P_COM + 1] = _y; o.part[P_COM + 2] = _m;
+      o.part[P_NSURF] = _ns;
     }
   }
 }
 
-// Shape::integrateObstBlock (Shape.cpp:117-155): per-slot partials.
+// centre of mass after putChiOnGrid (PutObjectsOnGrid.cpp:118-122), computed
+// from the partials without touching the stored state
+__device__ inline void shape_com(const Obstacles& ob, const cup2d_shape_state* ss, int k,
+                                 double* lds, double C[2], double* mass, bool* shifted) {
+  const double _x = shape_sum(ob, k, P_COM, lds), _y = shape_sum(ob, k, P_COM + 1, lds);
+  const double _m = shape_sum(ob, k, P_COM + 2, lds);
+  C[0] = ss[k].centerOfMass[0];
+  C[1] = ss[k].centerOfMass[1];
+  *shifted = _m > EPSD;
+  *mass = _m;
+  if (*shifted) {
+    C[0] += _x / _m;
+    C[1] += _y / _m;
+  }
+}
+
+// Shape::integrateObstBlock (Shape.cpp:117-155): per-slot partials about the
+// updated centre of mass.
 __global__ __launch_bounds__(NT) void k_obst_moments(Grid g,
                                                      const cup2d_shape_state* __restrict__ ss,
                                                      const StepState* __restrict__ st,
@@ -431,13 +439,15 @@ __global__ __launch_bounds__(NT) void k_obst_moments(Grid g,
   const int bx = b % g.bpdx, by = b / g.bpdx;
   SlotView o = slot(ob, s);
   const double hsq = g.h * g.h;
-  const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
+  double C[2], mass;
+  bool shifted;
+  shape_com(ob, ss, k, lds, C, &mass, &shifted);
   double a[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int i = threadIdx.x; i < BS * BS; i += NT) {
     if (o.chi[i] <= 0) continue;
     const int ix = i % BS, iy = i / BS;
     const double rhochi = o.chi[i] * o.rho[i] * hsq;
-    const double p0 = cell_pos(bx, ix, g.hb, g.h) - C0, p1 = cell_pos(by, iy, g.hb, g.h) - C1;
+    const double p0 = cell_pos(bx, ix, g.hb, g.h) - C[0], p1 = cell_pos(by, iy, g.hb, g.h) - C[1];
     a[0] += rhochi * p0;
     a[1] += rhochi * p1;
     a[2] += rhochi;
@@ -454,73 +464,108 @@ __global__ __launch_bounds__(NT) void k_obst_moments(Grid g,
 }
 
 // Shape::removeMoments (Shape.cpp:157-212) + putObjectVelOnGrid
-// (PutObjectsOnGrid.cpp:149-184), per distinct block, shapes in order.
-__global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __restrict__ ss,
+// (PutObjectsOnGrid.cpp:149-184), per obstacle block, shapes in order.
+__global__ __launch_bounds__(NT) void k_obst_udef(Grid g, const cup2d_shape_state* __restrict__ ss,
                                                   const StepState* __restrict__ st, Obstacles ob,
                                                   const double* __restrict__ chi,
                                                   double* __restrict__ udx,
                                                   double* __restrict__ udy) {
-  if ((int)blockIdx.x >= st->nub) return;
-  const int b = ob.ub_block[blockIdx.x];
-  const int bx = b % g.bpdx, by = b / g.bpdx;
-  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
   __shared__ double lds[8];
-  for (int q = 0; q < cnt; ++q) {
-    const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
-    double I[7];
-    for (int j = 0; j < 7; ++j) I[j] = shape_sum(ob, k, P_MOM + j, lds);
-    const double Iu = I[4] / I[2], Iv = I[5] / I[2], Ia = I[6] / I[3];
-    const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
-    SlotView o = slot(ob, s);
-    for (int i = threadIdx.x; i < BS * BS; i += NT) {
-      const int ix = i % BS, iy = i / BS;
-      const double p0 = cell_pos(bx, ix, g.hb, g.h) - C0, p1 = cell_pos(by, iy, g.hb, g.h) - C1;
-      o.udx[i] -= Iu - Ia * p1;
-      o.udy[i] -= Iv + Ia * p0;
-      const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-      if (o.chi[i] < chi[d] || o.chi[i] <= 0) continue;
-      udx[d] += o.udx[i];
-      udy[d] += o.udy[i];
+  const int nub = st->nub;
+  if ((int)blockIdx.x < nub) {
+    const int b = ob.ub_block[blockIdx.x];
+    const int bx = b % g.bpdx, by = b / g.bpdx;
+    const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+    for (int q = 0; q < cnt; ++q) {
+      const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
+      double I[7];
+      for (int j = 0; j < 7; ++j) I[j] = shape_sum(ob, k, P_MOM + j, lds);
+      const double Iu = I[4] / I[2], Iv = I[5] / I[2], Ia = I[6] / I[3];
+      double C[2], mass;
+      bool shifted;
+      shape_com(ob, ss, k, lds, C, &mass, &shifted);
+      SlotView o = slot(ob, s);
+      for (int i = threadIdx.x; i < BS * BS; i += NT) {
+        const int ix = i % BS, iy = i / BS;
+        const double p0 = cell_pos(bx, ix, g.hb, g.h) - C[0], p1 = cell_pos(by, iy, g.hb, g.h) - C[1];
+        o.udx[i] -= Iu - Ia * p1;
+        o.udy[i] -= Iv + Ia * p0;
+        const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+        if (o.chi[i] < chi[d] || o.chi[i] <= 0) continue;
+        udx[d] += o.udx[i];
+        udy[d] += o.udy[i];
+      }
     }
   }
-  if (blockIdx.x == 0) {
-    for (int k = 0; k < g.nshapes; ++k) {
-      const double m = shape_sum(ob, k, P_MOM + 2, lds), j = shape_sum(ob, k, P_MOM + 3, lds);
-      const double nsurf = shape_sum(ob, k, P_NSURF, lds);
-      if (threadIdx.x == 0) {
-        cup2d_shape_state& T = ss[k];
-        T.M = m;
-        T.J = j;
-        T.n_surface_points = nsurf;
-        T.n_obstacle_blocks = ob.slot_of_shape_start[k + 1] - ob.slot_of_shape_start[k];
-        const double dCx = T.center[0] - T.centerOfMass[0], dCy = T.center[1] - T.centerOfMass[1];
-        T.d_gm[0] = dCx * cos(T.orientation) + dCy * sin(T.orientation);
-        T.d_gm[1] = -dCx * sin(T.orientation) + dCy * cos(T.orientation);
-      }
+}
+
+// one workgroup, after k_obst_udef: commit centre of mass (PutObjectsOnGrid.cpp:
+// 118-122), M, J and d_gm (Shape.cpp:165-172) once every block has used the
+// previous values
+__global__ __launch_bounds__(NT) void k_obst_commit(Grid g, cup2d_shape_state* __restrict__ ss,
+                                                    Obstacles ob) {
+  __shared__ double lds[8];
+  for (int k = 0; k < g.nshapes; ++k) {
+    double C[2], mass;
+    bool shifted;
+    shape_com(ob, ss, k, lds, C, &mass, &shifted);
+    const double m = shape_sum(ob, k, P_MOM + 2, lds), j = shape_sum(ob, k, P_MOM + 3, lds);
+    const double nsurf = shape_sum(ob, k, P_NSURF, lds);
+    if (threadIdx.x == 0) {
+      cup2d_shape_state& T = ss[k];
+      T.centerOfMass[0] = C[0];
+      T.centerOfMass[1] = C[1];
+      T.M = m;
+      T.J = j;
+      T.n_surface_points = nsurf;
+      T.n_obstacle_blocks = ob.slot_of_shape_start[k + 1] - ob.slot_of_shape_start[k];
+      const double dCx = T.center[0] - T.centerOfMass[0], dCy = T.center[1] - T.centerOfMass[1];
+      T.d_gm[0] = dCx * cos(T.orientation) + dCy * sin(T.orientation);
+      T.d_gm[1] = -dCx * sin(T.orientation) + dCy * cos(T.orientation);
     }
   }
 }
 
 static int grid_cap(cup2d_ctx* c) { return c->ob.cap > 0 ? c->ob.cap : 1; }
 
-int enqueue_put_objects(cup2d_ctx* c, int compute_dt) {
+// PutObjectsOnGrid::operator() (PutObjectsOnGrid.cpp:186-241).
+// overlap: run everything after the list builder on the side stream; the
+// caller joins with join_obstacles() before the next reader of obstacle data.
+int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   const int G = grid_cap(c);
-  TimerScope ts(c, T_PUT_OBJECTS);
-  hipLaunchKernelGGL(k_obst_clear, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->sdf,
-                     c->chi, c->udx, c->udy, c->invrho);
-  hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                     c->ob, c->obmap, c->maxpart, c->nwg, c->ramp, compute_dt);
-  if (c->g.nshapes == 0) return launch_check(c, "put_objects");
-  hipLaunchKernelGGL(k_obst_create, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                     c->ob, c->sdf);
-  hipLaunchKernelGGL(k_obst_chi, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob,
-                     c->sdf);
-  hipLaunchKernelGGL(k_obst_chigrid, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob,
-                     c->chi, c->invrho);
-  hipLaunchKernelGGL(k_obst_moments, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob);
-  hipLaunchKernelGGL(k_obst_udef, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob,
-                     c->chi, c->udx, c->udy);
+  {
+    TimerScope ts(c, T_PUT_OBJECTS);
+    hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
+                       c->ob, c->obmap, c->maxpart, c->nwg, c->ramp, compute_dt, c->host_step);
+  }
+  hipStream_t s = c->stream;
+  if (overlap) {
+    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_begin, c->stream));
+    CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_begin, 0));
+    s = c->aux;
+  }
+  hipLaunchKernelGGL(k_obst_create, dim3(2 * G), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
+                     c->obmap, c->sdf, c->chi, c->udx, c->udy, c->invrho);
+  if (c->g.nshapes > 0) {
+    hipLaunchKernelGGL(k_obst_chi, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob, c->sdf,
+                       c->chi, c->invrho);
+    hipLaunchKernelGGL(k_obst_moments, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob);
+    hipLaunchKernelGGL(k_obst_udef, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob, c->chi,
+                       c->udx, c->udy);
+    hipLaunchKernelGGL(k_obst_commit, dim3(1), dim3(NT), 0, s, c->g, c->ss, c->ob);
+  }
+  if (overlap) {
+    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_obst, c->aux));
+    c->obst_pending = true;
+  }
   return launch_check(c, "put_objects");
+}
+
+int join_obstacles(cup2d_ctx* c) {
+  if (!c->obst_pending) return CUP2D_OK;
+  CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_obst, 0));
+  c->obst_pending = false;
+  return CUP2D_OK;
 }
 
 // ------------------------------------------------ RHS obstacle correction
@@ -602,48 +647,6 @@ int enqueue_rhs_obstacles(cup2d_ctx* c) {
 }
 
 // ------------------------------------------------------------ body update
-// PressureSingle::integrateMomenta (PressureSingle.cpp:23-67), per slot.
-__global__ __launch_bounds__(NT) void k_momenta(Grid g, const cup2d_shape_state* __restrict__ ss,
-                                                const StepState* __restrict__ st, Obstacles ob,
-                                                const double* __restrict__ u,
-                                                const double* __restrict__ v) {
-  __shared__ double lds[8];
-  if ((int)blockIdx.x >= st->nob) return;
-  const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
-  const int bx = b % g.bpdx, by = b / g.bpdx;
-  SlotView o = slot(ob, s);
-  const double hsq = g.h * g.h;
-  const double lambdt = g.lambda * st->dt;
-  const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
-  double a[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < BS * BS; i += NT) {
-    if (o.chi[i] <= 0) continue;
-    const int ix = i % BS, iy = i / BS;
-    const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-    const double ud0 = u[d] - o.udx[i], ud1 = v[d] - o.udy[i];
-    double F;
-    if (!g.implicit_penal) {
-      F = hsq * o.rho[i] * o.chi[i];
-    } else {
-      const double Xlamdt = o.chi[i] * lambdt;
-      F = hsq * o.rho[i] * Xlamdt / (1 + Xlamdt);
-    }
-    const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
-    a[0] += F;
-    a[1] += F * (p0 * p0 + p1 * p1);
-    a[2] += F * p0;
-    a[3] += F * p1;
-    a[4] += F * ud0;
-    a[5] += F * ud1;
-    a[6] += F * (p0 * ud1 - p1 * ud0);
-  }
-#pragma unroll
-  for (int j = 0; j < 7; ++j) {
-    const double t = block_sum<NT>(a[j], lds);
-    if (threadIdx.x == 0) o.part[P_PEN + j] = t;
-  }
-}
-
 // 3x3 LU with partial pivoting (restates gsl_linalg_LU_decomp/LU_solve as
 // called in Shape::updateVelocity, Shape.cpp:48-54).
 __device__ void lu3_solve(double A[3][3], const double b[3], double x[3]) {
@@ -680,10 +683,10 @@ __device__ void lu3_solve(double A[3][3], const double b[3], double x[3]) {
 
 // Shape::updateVelocity (Shape.cpp:25-59), Disk::updateVelocity
 // (ShapesSimple.cpp:83-87), preventCollidingObstacles (PressureSingle.cpp:243-344).
-__global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __restrict__ sp,
-                                                   cup2d_shape_state* __restrict__ ss,
-                                                   const StepState* __restrict__ st,
-                                                   Obstacles ob) {
+__device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
+                              cup2d_shape_state* __restrict__ ss,
+                              StepState* __restrict__ st, const Obstacles& ob,
+                              ForceSnap* __restrict__ snap) {
   __shared__ double lds[8];
   __shared__ double coll[MAX_SHAPES][10];
   const double dt = st->dt, time = st->time;
@@ -784,14 +787,75 @@ __global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __
     }
 }
 
+// PressureSingle::integrateMomenta (PressureSingle.cpp:23-67), per slot.
+__global__ __launch_bounds__(NT) void k_momenta(Grid g, const ShapeParams* __restrict__ sp,
+                                                cup2d_shape_state* __restrict__ ss,
+                                                StepState* __restrict__ st, Obstacles ob,
+                                                const double* __restrict__ u,
+                                                const double* __restrict__ v) {
+  __shared__ double lds[8];
+  const int nob = st->nob;
+  if ((int)blockIdx.x >= nob) return;
+  const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  SlotView o = slot(ob, s);
+  const double hsq = g.h * g.h;
+  const double lambdt = g.lambda * st->dt;
+  const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    if (o.chi[i] <= 0) continue;
+    const int ix = i % BS, iy = i / BS;
+    const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+    const double ud0 = u[d] - o.udx[i], ud1 = v[d] - o.udy[i];
+    double F;
+    if (!g.implicit_penal) {
+      F = hsq * o.rho[i] * o.chi[i];
+    } else {
+      const double Xlamdt = o.chi[i] * lambdt;
+      F = hsq * o.rho[i] * Xlamdt / (1 + Xlamdt);
+    }
+    const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
+    a[0] += F;
+    a[1] += F * (p0 * p0 + p1 * p1);
+    a[2] += F * p0;
+    a[3] += F * p1;
+    a[4] += F * ud0;
+    a[5] += F * ud1;
+    a[6] += F * (p0 * ud1 - p1 * ud0);
+  }
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const double t = block_sum<NT>(a[j], lds);
+    if (threadIdx.x == 0) o.part[P_PEN + j] = t;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __restrict__ sp,
+                                                   cup2d_shape_state* __restrict__ ss,
+                                                   StepState* __restrict__ st, Obstacles ob,
+                                                   ForceSnap* __restrict__ snap) {
+  body_solve_wg(g, sp, ss, st, ob, snap);
+  __syncthreads();
+  // state ComputeForces uses (it may run after the next step moved the bodies)
+  if (threadIdx.x < g.nshapes) {
+    const cup2d_shape_state& T = ss[threadIdx.x];
+    ForceSnap f;
+    f.C[0] = T.centerOfMass[0]; f.C[1] = T.centerOfMass[1];
+    f.u = T.u; f.v = T.v; f.omega = T.omega; f.pad_ = 0;
+    snap[threadIdx.x] = f;
+  }
+  if (threadIdx.x == 0) st->dt_done = st->dt;
+}
+
 int enqueue_body_update(cup2d_ctx* c) {
   if (c->g.nshapes == 0) return CUP2D_OK;
   const int G = grid_cap(c);
   TimerScope ts(c, T_BODY);
-  hipLaunchKernelGGL(k_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->st, c->ob, c->u,
-                     c->v);
+  hipLaunchKernelGGL(k_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st, c->ob,
+                     c->u, c->v);
   hipLaunchKernelGGL(k_body_solve, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                     c->ob);
+                     c->ob, c->fsnap);
   return launch_check(c, "body_update");
 }
 
@@ -853,20 +917,54 @@ int enqueue_penalize(cup2d_ctx* c, double* maxpart) {
 }
 
 // ----------------------------------------------------------- ComputeForces
+// Shape::computeForces (Shape.cpp:252-281): per-block partials added in
+// block order; optionally ends the step (time += dt, step++; Simulation.cpp:289-290).
+__device__ void forces_fin_wg(const Grid& g, cup2d_shape_state* __restrict__ ss,
+                              const Obstacles& ob, StepState* __restrict__ st, int end_step,
+                              const ForceSnap* __restrict__ snap) {
+  __shared__ double lds[8];
+  for (int k = 0; k < g.nshapes; ++k) {
+    double a[18];
+    for (int j = 0; j < 18; ++j) a[j] = shape_sum(ob, k, P_FRC + j, lds);
+    if (threadIdx.x != 0) continue;
+    cup2d_shape_state& T = ss[k];
+    T.perimeter = a[0]; T.circulation = a[1];
+    T.forcex = a[2]; T.forcey = a[3]; T.forcex_V = a[4]; T.forcey_V = a[5];
+    T.forcex_P = a[6]; T.forcey_P = a[7];
+    T.torque = a[8]; T.torque_P = a[9]; T.torque_V = a[10];
+    T.thrust = a[11]; T.drag = a[12];
+    T.Pout = a[13]; T.defPower = a[14]; T.PoutBnd = a[15]; T.defPowerBnd = a[16];
+    T.n_surface_points = a[17];
+    T.n_obstacle_blocks = ob.slot_of_shape_start[k + 1] - ob.slot_of_shape_start[k];
+    const double su = snap[k].u, sv = snap[k].v;
+    T.Pthrust = T.thrust * sqrt(su * su + sv * sv);
+    T.Pdrag = T.drag * sqrt(su * su + sv * sv);
+    const double denUnb = T.Pthrust - dmin(T.defPower, 0.0);
+    const double demBnd = T.Pthrust - T.defPowerBnd;
+    T.EffPDef = T.Pthrust / dmax(denUnb, EPSD);
+    T.EffPDefBnd = T.Pthrust / dmax(demBnd, EPSD);
+  }
+  if (end_step && threadIdx.x == 0) {
+    st->time += st->dt_done;
+    st->step += 1;
+  }
+}
+
 // ComputeForces.cpp:16-107, per slot; surface points are the cells with D>0.
-__global__ __launch_bounds__(NT) void k_forces(Grid g, const cup2d_shape_state* __restrict__ ss,
+__global__ __launch_bounds__(NT) void k_forces(Grid g, const ForceSnap* __restrict__ snap,
                                                const StepState* __restrict__ st, Obstacles ob,
                                                const double* __restrict__ u,
                                                const double* __restrict__ v,
                                                const double* __restrict__ p) {
   __shared__ double lds[8];
-  if ((int)blockIdx.x >= st->nob) return;
+  const int nob = st->nob;
+  if ((int)blockIdx.x >= nob) return;
   const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   SlotView o = slot(ob, s);
   const double NUoH = g.nu / g.h;
-  const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
-  const double su = ss[k].u, sv = ss[k].v;
+  const double Cx = snap[k].C[0], Cy = snap[k].C[1];
+  const double su = snap[k].u, sv = snap[k].v;
   const double vel_norm = sqrt(su * su + sv * sv);
   const double vu0 = vel_norm > 0 ? su / vel_norm : 0, vu1 = vel_norm > 0 ? sv / vel_norm : 0;
   double a[18];
@@ -914,48 +1012,28 @@ __global__ __launch_bounds__(NT) void k_forces(Grid g, const cup2d_shape_state* 
   }
 }
 
-// Shape::computeForces (Shape.cpp:252-281): per-block partials added in
-// block order; optionally ends the step (time += dt, step++; Simulation.cpp:289-290).
 __global__ __launch_bounds__(NT) void k_forces_fin(Grid g, cup2d_shape_state* __restrict__ ss,
                                                    Obstacles ob, StepState* __restrict__ st,
-                                                   int end_step) {
-  __shared__ double lds[8];
-  for (int k = 0; k < g.nshapes; ++k) {
-    double a[18];
-    for (int j = 0; j < 18; ++j) a[j] = shape_sum(ob, k, P_FRC + j, lds);
-    if (threadIdx.x != 0) continue;
-    cup2d_shape_state& T = ss[k];
-    T.perimeter = a[0]; T.circulation = a[1];
-    T.forcex = a[2]; T.forcey = a[3]; T.forcex_V = a[4]; T.forcey_V = a[5];
-    T.forcex_P = a[6]; T.forcey_P = a[7];
-    T.torque = a[8]; T.torque_P = a[9]; T.torque_V = a[10];
-    T.thrust = a[11]; T.drag = a[12];
-    T.Pout = a[13]; T.defPower = a[14]; T.PoutBnd = a[15]; T.defPowerBnd = a[16];
-    T.n_surface_points = a[17];
-    T.n_obstacle_blocks = ob.slot_of_shape_start[k + 1] - ob.slot_of_shape_start[k];
-    T.Pthrust = T.thrust * sqrt(T.u * T.u + T.v * T.v);
-    T.Pdrag = T.drag * sqrt(T.u * T.u + T.v * T.v);
-    const double denUnb = T.Pthrust - dmin(T.defPower, 0.0);
-    const double demBnd = T.Pthrust - T.defPowerBnd;
-    T.EffPDef = T.Pthrust / dmax(denUnb, EPSD);
-    T.EffPDefBnd = T.Pthrust / dmax(demBnd, EPSD);
-  }
-  if (end_step && threadIdx.x == 0) {
-    st->time += st->dt;
-    st->step += 1;
-  }
+                                                   int end_step, const ForceSnap* __restrict__ snap) {
+  forces_fin_wg(g, ss, ob, st, end_step, snap);
 }
 
-int enqueue_forces(cup2d_ctx* c, int end_step) {
+int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux) {
   if (c->g.nshapes == 0) {
     if (end_step) return enqueue_step_end(c);
     return CUP2D_OK;
   }
-  TimerScope ts(c, T_FORCES);
-  hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, c->stream, c->g, c->ss, c->st,
-                     c->ob, c->u, c->v, c->pres);
-  hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(NT), 0, c->stream, c->g, c->ss, c->ob, c->st,
-                     end_step);
+  hipStream_t s = c->stream;
+  if (on_aux) {  // ComputeForces overlaps the next step (it reads only the snapshot)
+    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_pen, c->stream));
+    CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_pen, 0));
+    s = c->aux;
+  }
+  TimerScope ts(c, T_FORCES, s);
+  hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, s, c->g, c->fsnap, c->st, c->ob,
+                     c->u, c->v, c->pres);
+  hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(NT), 0, s, c->g, c->ss, c->ob, c->st, end_step,
+                     c->fsnap);
   return launch_check(c, "forces");
 }
 

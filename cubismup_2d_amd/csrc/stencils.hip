@@ -28,12 +28,26 @@ int launch_check(cup2d_ctx* c, const char* what) {
   return CUP2D_OK;
 }
 
+// Launch shape of the full-grid stencil kernels (compile-time tuning knobs):
+// rows per workgroup, widest column strip, unroll depth of the row march.
+#ifndef CUP2D_STENCIL_RY
+#define CUP2D_STENCIL_RY 8
+#endif
+#ifndef CUP2D_STENCIL_TX
+#define CUP2D_STENCIL_TX 256
+#endif
+#ifndef CUP2D_STENCIL_UNROLL
+#define CUP2D_STENCIL_UNROLL 4
+#endif
+#define CUP2D_PRAGMA(x) _Pragma(#x)
+#define CUP2D_UNROLL_ROWS CUP2D_PRAGMA(unroll CUP2D_STENCIL_UNROLL)
+
 static inline int strip_width(int NX) {
-  for (int t = 256; t >= 32; t >>= 1)
+  for (int t = CUP2D_STENCIL_TX; t >= 32; t >>= 1)
     if (NX % t == 0) return t;
   return 32;
 }
-static constexpr int RYS_ = 16;
+static constexpr int RYS_ = CUP2D_STENCIL_RY;
 // workgroups of the stencil kernels (advDiff, RHS, projection share a grid)
 int grid_workgroups(int NX, int NY) { return (NX / strip_width(NX)) * (NY / RYS_); }
 
@@ -121,6 +135,19 @@ int enqueue_step_end(cup2d_ctx* c) {
   return launch_check(c, "step_end");
 }
 
+// corr = IF / (2 NY + 2 NX) (advDiff.cpp:105) from the per-workgroup partials
+// of the boundary workgroups, summed in a fixed order.
+__device__ inline double flux_corr(const double* __restrict__ fluxpart, int gx, int gy,
+                                   const Grid& g, double* lds) {
+  double s = 0;
+  for (int w = threadIdx.x; w < gx * gy; w += blockDim.x) {
+    const int bx = w % gx, by = w / gx;
+    if (bx == 0 || bx == gx - 1 || by == 0 || by == gy - 1) s += fluxpart[w];
+  }
+  const double IF = block_sum_dyn(s, lds);
+  return IF / (2 * (BS * g.bpdy - 0) + 2 * (BS * g.bpdx - 0));
+}
+
 // ------------------------------------------------------------- advDiff
 // advDiff.cpp:14-34 (stencil), :47-55 (factors), :68-79 (faded lab).  Each
 // workgroup also writes its share of the boundary flux
@@ -150,7 +177,7 @@ __global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
   if (y0 == 0) { us = uc * fadeS; vs = vc * fadeS; }
   else { us = u[idx - NX]; vs = v[idx - NX]; }
   double flux = 0;
-#pragma unroll 4
+  CUP2D_UNROLL_ROWS
   for (int iy = 0; iy < RY; ++iy, idx += NX) {
     const int y = y0 + iy;
     double un, vn;
@@ -178,32 +205,25 @@ __global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
   }
   const bool edge = blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
                     blockIdx.y == gridDim.y - 1;
-  if (edge) flux = block_sum<256>(flux, lds);
-  if (threadIdx.x == 0) fluxpart[blockIdx.y * gridDim.x + blockIdx.x] = edge ? flux : 0.0;
+  if (!edge) return;
+  flux = block_sum_dyn(flux, lds);
+  if (threadIdx.x == 0) fluxpart[blockIdx.y * gridDim.x + blockIdx.x] = flux;
 }
 
-// corr = IF / (2 NY + 2 NX) (advDiff.cpp:105) from the per-workgroup partials
-// of the boundary workgroups, summed in a fixed order.
-__device__ inline double flux_corr(const double* __restrict__ fluxpart, int gx, int gy,
-                                   const Grid& g, double* lds) {
-  double s = 0;
-  for (int w = threadIdx.x; w < gx * gy; w += blockDim.x) {
-    const int bx = w % gx, by = w / gx;
-    if (bx == 0 || bx == gx - 1 || by == 0 || by == gy - 1) s += fluxpart[w];
-  }
-  const double IF = block_sum<256>(s, lds);
-  return IF / (2 * (BS * g.bpdy - 0) + 2 * (BS * g.bpdx - 0));
+// one workgroup: corr from the boundary workgroups' partials
+__global__ __launch_bounds__(256) void k_flux_reduce(const double* __restrict__ fluxpart, int gx,
+                                                     int gy, Grid g, StepState* __restrict__ st) {
+  __shared__ double lds[8];
+  const double corr = flux_corr(fluxpart, gx, gy, g, lds);
+  if (threadIdx.x == 0) st->flux_corr = corr;
 }
 
 // advDiff.cpp:107-115 applied to the stored velocity (operator-level API).
 __global__ __launch_bounds__(256) void k_flux_fix(double* __restrict__ u,
                                                   double* __restrict__ v, Grid g,
-                                                  StepState* __restrict__ st,
-                                                  const double* __restrict__ fluxpart, int gx,
-                                                  int gy) {
-  __shared__ double lds[8];
+                                                  const StepState* __restrict__ st) {
   const int NX = g.NX, NY = g.NY;
-  const double corr = flux_corr(fluxpart, gx, gy, g, lds);
+  const double corr = st->flux_corr;
   const int n = 2 * NY + 2 * NX;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     if (i < NY) u[(size_t)i * NX] += corr;
@@ -211,10 +231,9 @@ __global__ __launch_bounds__(256) void k_flux_fix(double* __restrict__ u,
     else if (i < 2 * NY + NX) v[i - 2 * NY] += corr;
     else v[(size_t)(NY - 1) * NX + (i - 2 * NY - NX)] -= corr;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->flux_corr = corr;
 }
 
-static constexpr int RYS = 16;  // rows per workgroup of the stencil kernels
+static constexpr int RYS = CUP2D_STENCIL_RY;  // rows per workgroup of the stencil kernels
 
 int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
   const int tx = strip_width(c->g.NX);
@@ -224,6 +243,11 @@ int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
     hipLaunchKernelGGL(k_advdiff<RYS>, dim3(gx, gy), dim3(tx), 0, c->stream, c->u, c->v, c->tu,
                        c->tv, c->g, c->st, c->fluxpart);
   }
+  {
+    TimerScope ts(c, T_FLUXFIX);
+    hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, c->stream, c->fluxpart, gx, gy, c->g,
+                       c->st);
+  }
   std::swap(c->u, c->tu);  // advDiff.cpp:81-87 copy tmpV -> vel
   std::swap(c->v, c->tv);
   c->flux_deferred = defer_fix != 0;
@@ -231,52 +255,57 @@ int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
     TimerScope ts(c, T_FLUXFIX);
     const int n = 2 * c->g.NX + 2 * c->g.NY;
     hipLaunchKernelGGL(k_flux_fix, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->u, c->v,
-                       c->g, c->st, c->fluxpart, gx, gy);
+                       c->g, c->st);
   }
   return launch_check(c, "advdiff");
 }
 
 // --------------------------------------------------------- pressure RHS
 // PressureSingle.cpp:110-128: r = (h/2dt) (u_E - u_W + v_N - v_S), zero-
-// gradient ghosts.  With a deferred advDiff flux correction (fluxpart set),
-// boundary values are read as u_W + corr, u_E - corr, v_S + corr, v_N - corr,
-// exactly the values advDiff.cpp:107-115 would have stored.
-template <int RY>
+// gradient ghosts.  With a deferred advDiff flux correction (FIX), boundary
+// values are read as u_W + corr, u_E - corr, v_S + corr, v_N - corr, exactly
+// the values advDiff.cpp:107-115 would have stored.
+template <int RY, bool FIX>
 __global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
                                              const double* __restrict__ v,
                                              double* __restrict__ r, Grid g,
-                                             const StepState* __restrict__ st,
-                                             const double* __restrict__ fluxpart) {
-  __shared__ double lds[8];
+                                             const StepState* __restrict__ st) {
   const int NX = g.NX, NY = g.NY;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y0 = blockIdx.y * RY;
   const double facDiv = 0.5 * g.h / st->dt;
-  double corr = 0;
-  const bool fix = fluxpart != nullptr;
-  if (fix && (blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
-              blockIdx.y == gridDim.y - 1))
-    corr = flux_corr(fluxpart, gridDim.x, gridDim.y, g, lds);
-  auto U = [&](int xx, size_t i) {
-    const double a = u[i];
-    if (!fix) return a;
-    return xx == 0 ? a + corr : (xx == NX - 1 ? a - corr : a);
-  };
-  auto V = [&](int yy, size_t i) {
-    const double a = v[i];
-    if (!fix) return a;
-    return yy == 0 ? a + corr : (yy == NY - 1 ? a - corr : a);
-  };
+  const double corr = FIX ? st->flux_corr : 0.0;
   size_t idx = (size_t)y0 * NX + x;
-  double vc = V(y0, idx);
-  double vs = (y0 == 0) ? vc : V(y0 - 1, idx - NX);
-#pragma unroll 4
+  double vc = v[idx];
+  if (FIX && y0 == 0) vc = vc + corr;
+  const double vs0 = (y0 == 0) ? vc : v[idx - NX];  // y0-1 is never a boundary row
+  double vs = vs0;
+  CUP2D_UNROLL_ROWS
   for (int iy = 0; iy < RY; ++iy, idx += NX) {
     const int y = y0 + iy;
-    const double vn = (y + 1 < NY) ? V(y + 1, idx + NX) : vc;
-    const double uc = U(x, idx);
-    const double uw = (x > 0) ? U(x - 1, idx - 1) : uc;
-    const double ue = (x + 1 < NX) ? U(x + 1, idx + 1) : uc;
+    double vn;
+    if (y + 1 < NY) {
+      vn = v[idx + NX];
+      if (FIX && y + 1 == NY - 1) vn = vn - corr;
+    } else {
+      vn = vc;
+    }
+    double uc = u[idx];
+    if (FIX && x == 0) uc = uc + corr;
+    if (FIX && x == NX - 1) uc = uc - corr;
+    double uw, ue;
+    if (x > 0) {
+      uw = u[idx - 1];
+      if (FIX && x - 1 == 0) uw = uw + corr;
+    } else {
+      uw = uc;
+    }
+    if (x + 1 < NX) {
+      ue = u[idx + 1];
+      if (FIX && x + 1 == NX - 1) ue = ue - corr;
+    } else {
+      ue = uc;
+    }
     const double divVx = ue - uw;
     const double divVy = vn - vs;
     r[idx] = facDiv * (divVx + divVy);
@@ -288,8 +317,13 @@ __global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
 int enqueue_pressure_rhs(cup2d_ctx* c) {
   const int tx = strip_width(c->g.NX);
   TimerScope ts(c, T_RHS);
-  hipLaunchKernelGGL(k_rhs<RYS>, dim3(c->g.NX / tx, c->g.NY / RYS), dim3(tx), 0, c->stream, c->u,
-                     c->v, c->rhs, c->g, c->st, c->flux_deferred ? c->fluxpart : nullptr);
+  const dim3 grid(c->g.NX / tx, c->g.NY / RYS);
+  if (c->flux_deferred)
+    hipLaunchKernelGGL((k_rhs<RYS, true>), grid, dim3(tx), 0, c->stream, c->u, c->v, c->rhs, c->g,
+                       c->st);
+  else
+    hipLaunchKernelGGL((k_rhs<RYS, false>), grid, dim3(tx), 0, c->stream, c->u, c->v, c->rhs,
+                       c->g, c->st);
   return launch_check(c, "pressure_rhs");
 }
 
@@ -374,8 +408,7 @@ int enqueue_fade(cup2d_ctx* c) {
 template <int RY>
 __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double* __restrict__ v,
                                                  const double* __restrict__ p, Grid g,
-                                                 const StepState* __restrict__ st,
-                                                 const double* __restrict__ fluxpart,
+                                                 const StepState* __restrict__ st, int fix,
                                                  const unsigned char* __restrict__ obmap,
                                                  double* __restrict__ maxpart) {
   __shared__ double lds[8];
@@ -384,16 +417,12 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
   const int y0 = blockIdx.y * RY;
   const double pFac = -0.5 * st->dt / g.h;
   const double UINF = st->uinf[0], VINF = st->uinf[1];
-  double corr = 0;
-  const bool fix = fluxpart != nullptr;
-  if (fix && (blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
-              blockIdx.y == gridDim.y - 1))
-    corr = flux_corr(fluxpart, gridDim.x, gridDim.y, g, lds);
+  const double corr = fix ? st->flux_corr : 0.0;
   size_t idx = (size_t)y0 * NX + x;
   double pc = p[idx];
   double ps = (y0 == 0) ? pc : p[idx - NX];
   double m = 0;
-#pragma unroll 4
+  CUP2D_UNROLL_ROWS
   for (int iy = 0; iy < RY; ++iy, idx += NX) {
     const int y = y0 + iy;
     const double pn = (y + 1 < NY) ? p[idx + NX] : pc;
@@ -420,7 +449,7 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
     pc = pn;
   }
   if (maxpart) {
-    m = block_max<256>(m, lds);
+    m = block_max_dyn(m, lds);
     if (threadIdx.x == 0) maxpart[blockIdx.y * gridDim.x + blockIdx.x] = m;
   }
 }
@@ -429,7 +458,7 @@ int enqueue_pressure_correction(cup2d_ctx* c, int want_max) {
   const int tx = strip_width(c->g.NX);
   TimerScope ts(c, T_PROJECT);
   hipLaunchKernelGGL(k_project<RYS>, dim3(c->g.NX / tx, c->g.NY / RYS), dim3(tx), 0, c->stream,
-                     c->u, c->v, c->pres, c->g, c->st, c->flux_deferred ? c->fluxpart : nullptr,
+                     c->u, c->v, c->pres, c->g, c->st, c->flux_deferred ? 1 : 0,
                      c->obmap, want_max ? c->maxpart : nullptr);
   c->flux_deferred = false;
   return launch_check(c, "pressure_correction");
@@ -451,6 +480,7 @@ __global__ void k_ic_state(StepState* st) {
   st->time = 0; st->dt = 0; st->step = 0; st->uinf[0] = 0; st->uinf[1] = 0;
   st->error = 0; st->nob = 0; st->nub = 0; st->nprev = 0;
   st->maxU = 0; st->fade_corr = 0; st->flux_corr = 0;
+  st->dt_done = 0;
 }
 int enqueue_ic(cup2d_ctx* c) {
   hipLaunchKernelGGL(k_ic, dim3(2048), dim3(256), 0, c->stream, c->u, c->v, c->tu, c->tv,
