@@ -1,0 +1,111 @@
+// SimulationData.cpp -- device context lifetime and host <-> device state.
+#include "SimulationData.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace cup2d {
+
+void check(cup2d_ctx* ctx, int status, const char* what) {
+  if (status == CUP2D_OK) return;
+  std::string msg = std::string(what) + ": " + cup2d_last_error(ctx);
+  throw Error(status, msg);
+}
+
+double Shape::getMaxVel() const { return std::sqrt(state.u * state.u + state.v * state.v); }
+
+SimulationData::~SimulationData() {
+  if (ctx) cup2d_destroy(ctx);
+}
+
+void SimulationData::allocateGrid() {
+  if (ctx) throw Error(CUP2D_ERR_STATE, "allocateGrid: grid already allocated");
+  if (bpdx <= 0 || bpdy <= 0) throw Error(CUP2D_ERR_ARG, "allocateGrid: bpdx, bpdy must be > 0");
+  // extents of the domain: the longer side has length `extent`
+  // (SimulationData.cpp:34-35: h is the same in both directions)
+  const int mx = std::max(bpdx, bpdy);
+  extents[0] = extent * bpdx / mx;
+  extents[1] = extent * bpdy / mx;
+  int kind;
+  if (poissonType == "periodic" || poissonType == "cpu_periodic")
+    kind = CUP2D_POISSON_PERIODIC;
+  else if (poissonType == "freespace" || poissonType == "cpu_freespace")
+    kind = CUP2D_POISSON_FREESPACE;
+  else
+    throw Error(CUP2D_ERR_ARG, "poissonType '" + poissonType +
+                                   "' has no device solver (periodic, freespace)");
+  cup2d_cfg cfg{};
+  cfg.bpdx = bpdx;
+  cfg.bpdy = bpdy;
+  cfg.extent = extent;
+  cfg.nu = nu;
+  cfg.CFL = CFL;
+  cfg.lambda = lambda;
+  cfg.fadeLen = fadeLen;
+  fadeLenX = fadeLen * extent;
+  fadeLenY = fadeLen * extent;
+  cfg.poisson = kind;
+  cfg.implicit_penal = 0;
+  cfg.stream = stream;
+  check(nullptr, cup2d_create(&cfg, &ctx), "cup2d_create");
+}
+
+void SimulationData::addShape(Shape s) {
+  int id = -1;
+  check(ctx, cup2d_add_shape(ctx, &s.desc, &id), "cup2d_add_shape");
+  s.obstacleID = id;
+  shapes.push_back(s);
+}
+
+void SimulationData::resetAll() {
+  time = 0;
+  step = 0;
+  uinfx = 0;
+  uinfy = 0;
+}
+
+bool SimulationData::bOver() const {
+  const bool timeEnd = endTime > 0 && time >= endTime;
+  const bool stepEnd = nsteps > 0 && step > nsteps;
+  return timeEnd || stepEnd;
+}
+
+double SimulationData::maxRelSpeed() const {
+  double m = 0;
+  for (const auto& s : shapes) m = std::max(m, s.getMaxVel());
+  return m;
+}
+
+double SimulationData::getH() const {
+  return extent / (double)std::max(NX(), NY());
+}
+
+void SimulationData::pullScalars() {
+  double uinf[2];
+  check(ctx, cup2d_get_scalars(ctx, &time, &dt, &step, uinf, nullptr), "cup2d_get_scalars");
+  uinfx = uinf[0];
+  uinfy = uinf[1];
+}
+
+void SimulationData::pushScalars() {
+  check(ctx, cup2d_set_scalars(ctx, time, step, uinfx, uinfy), "cup2d_set_scalars");
+}
+
+void SimulationData::refreshShapes() {
+  for (auto& s : shapes)
+    check(ctx, cup2d_get_shape(ctx, s.obstacleID, &s.state), "cup2d_get_shape");
+}
+
+void SimulationData::upload(const Field& f, const double* host, bool blocks) {
+  check(ctx,
+        cup2d_upload_field(ctx, f.id, blocks ? CUP2D_LAYOUT_BLOCKS : CUP2D_LAYOUT_DENSE, host),
+        "cup2d_upload_field");
+}
+
+void SimulationData::download(const Field& f, double* host, bool blocks) const {
+  check(ctx,
+        cup2d_download_field(ctx, f.id, blocks ? CUP2D_LAYOUT_BLOCKS : CUP2D_LAYOUT_DENSE, host),
+        "cup2d_download_field");
+}
+
+}  // namespace cup2d

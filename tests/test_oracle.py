@@ -224,3 +224,67 @@ def test_body_out_of_domain_is_an_error(oracle_mod):
                                                                     bForced=True)])
     o.reset()
     assert o.put_objects(1.0) == -2
+
+
+def test_find_max_u_and_dt_closed_form(oracle_mod):
+    """findMaxU (Helpers.cpp:136-176) and calcMaxTimestep with the CFL ramp
+    (Simulation.cpp:254-277), recomputed in numpy for step 0, 37 and 150."""
+    sh = oracle_mod.disk(radius=0.05, xpos=0.3, ypos=0.25, xvel=0.15, yvel=-0.05,
+                         bFixed=True, bForced=True)
+    o = oracle_mod.Oracle(4, 2, extent=1.0, nu=3e-4, CFL=0.3, poisson="freespace",
+                          shapes=[sh])
+    o.reset()
+    rng = np.random.default_rng(21)
+    vel = 0.1 * rng.standard_normal((2, o.NY, o.NX))
+    o.set_field(oracle_mod.VEL, vel)
+    h = o.scalars()["h"]
+    for step, uinf in [(0, (0.15, -0.05)), (37, (0.0, 0.0)), (150, (-0.2, 0.3))]:
+        o.set_scalars(time=0.0, step=step, uinf=uinf)
+        mu = max(np.abs(vel[0] + uinf[0]).max(), np.abs(vel[1] + uinf[1]).max(),
+                 np.abs(vel[0]).max(), np.abs(vel[1]).max())
+        assert o.find_max_u() == mu
+        s = o.shape()
+        ub = np.sqrt(s["u"] ** 2 + s["v"] ** 2)
+        mn = min(h / mu, h * h / 3e-4, h / ub)
+        if step < 100:
+            x = (step + 1.0) / 100
+            ref = np.exp(np.log(1e-3) * (1 - x) + np.log(0.3) * x) * mn
+        else:
+            ref = 0.3 * mn
+        assert abs(o.calc_max_timestep() - ref) <= 1e-15 * ref
+
+
+def test_penalize_closed_form(oracle_mod):
+    """PressureSingle::penalize (PressureSingle.cpp:69-108), explicit alpha =
+    1 - chi, on a translating and rotating forced disk: u <- alpha u +
+    (1 - alpha) (u_s - omega (y - Cy) + udef_x), v likewise with + omega (x - Cx)."""
+    sh = oracle_mod.disk(radius=0.07, xpos=0.31, ypos=0.26, xvel=0.2, yvel=-0.1, angvel=0.7,
+                         bFixed=False, bForced=True)
+    o = oracle_mod.Oracle(8, 4, extent=1.0, nu=1e-3, poisson="freespace", shapes=[sh])
+    o.reset()
+    rng = np.random.default_rng(22)
+    vel = 0.05 * rng.standard_normal((2, o.NY, o.NX))
+    o.set_field(oracle_mod.VEL, vel)
+    s = o.shape()
+    assert s["omega"] != 0 and s["u"] != 0 and s["v"] != 0
+    chi = o.get_field(oracle_mod.CHI)[0]
+    udef = o.get_field(oracle_mod.UDEF)
+    h = o.scalars()["h"]
+    hb = 32 * h
+    ix = np.arange(o.NX)
+    iy = np.arange(o.NY)
+    x = (ix // 32) * hb + h * (ix % 32 + 0.5)
+    y = (iy // 32) * hb + h * (iy % 32 + 0.5)
+    X, Y = np.meshgrid(x, y)
+    Cx, Cy = s["centerOfMass"]
+    US = s["u"] - s["omega"] * (Y - Cy) + udef[0]
+    VS = s["v"] + s["omega"] * (X - Cx) + udef[1]
+    a = 1 - chi
+    ref = vel.copy()
+    m = chi > 0
+    ref[0][m] = (a * vel[0] + (1 - a) * US)[m]
+    ref[1][m] = (a * vel[1] + (1 - a) * VS)[m]
+    o.penalize(1e-3)
+    out = o.get_field(oracle_mod.VEL)
+    assert m.sum() > 100
+    assert np.abs(out - ref).max() <= 1e-16
