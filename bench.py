@@ -24,9 +24,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# algorithmic HBM bytes per cell and launch (s = 8 bytes, FP64), periodic solver
-BALG = {"advdiff": 4, "pressure_rhs": 3, "fft_rows_fwd": 2, "fft_cols": 2, "fft_rows_inv": 2,
-        "pressure_correction": 5, "find_max_u": 2}
+# algorithmic HBM bytes per cell and launch of the full-grid kernels of a step,
+# in reals (x 8 bytes, FP64), periodic solver.  The forward row FFT forms the
+# pressure RHS from the advDiff output itself: it reads u*, v* (2) and writes
+# the half spectrum (1); there is no separate RHS pass.
+BALG = {"advdiff": 4, "fft_rows_fwd": 3, "fft_cols": 2, "fft_rows_inv": 2,
+        "pressure_correction": 5}
+BALG_FREESPACE = dict(BALG, fft_rows_fwd=4, fft_cols=5, fft_rows_inv=3)
+# the metric's per-step figure: each stage with its own pass (advDiff 4, RHS 3,
+# FFT 2+2+2, projection 5 = 18 reals = 144 B/cell periodic; 23 = 184 B free-space)
+STEP_BALG_METRIC = {"periodic": 18, "freespace": 23}
 FULL_GRID = list(BALG)
 
 CONFIGS = {
@@ -83,20 +90,96 @@ def cpu_baseline(cfg, vel, steps):
                        f"FFT share {fft / total:.2f}")
 
 
+KERNEL_OF = {"k_advdiff": "advdiff", "k_rhs<": "pressure_rhs", "k_rows_fwd": "fft_rows_fwd",
+             "k_cols": "fft_cols", "k_rows_inv": "fft_rows_inv", "k_project": "pressure_correction",
+             "k_find_max_u": "find_max_u"}
+
+
+def copy_calibration(mib):
+    """A plain device copy of known size (dst <- src, mib MiB each way), run
+    before the workload so one counter pass sees both."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return n * 8
+
+
+def traffic_from_counters(fetch_csv, write_csv, copy_bytes, config):
+    """HBM bytes per launch of each full-grid kernel from two rocprofv3 --pmc
+    passes (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), each scaled by
+    copy_bytes / (the same counter for the calibration copy)."""
+    import csv
+
+    def per_kernel(path, counter):
+        acc = {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] != counter:
+                    continue
+                acc.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+        return acc
+
+    fetch_all = per_kernel(fetch_csv, "FETCH_SIZE")
+    write_all = per_kernel(write_csv, "WRITE_SIZE")
+    # the calibration copy: the kernel other than ours that both reads and
+    # writes the most (a fill kernel writes as much but reads nothing)
+    others = [k for k in write_all if "cup2d" not in k and k in fetch_all]
+    if not others:
+        raise SystemExit("no calibration copy kernel in the counter files")
+    ck = max(others, key=lambda k: min(max(fetch_all[k]), max(write_all[k])))
+    if max(write_all[ck]) * 1024 < 0.5 * copy_bytes:
+        raise SystemExit("no dispatch of the calibration copy's size in the counter files")
+    f_scale = copy_bytes / (max(fetch_all[ck]) * 1024)
+    w_scale = copy_bytes / (max(write_all[ck]) * 1024)
+    fetch = {k: sum(v) / len(v) for k, v in fetch_all.items()}
+    write = {k: sum(v) / len(v) for k, v in write_all.items()}
+    out = {"_calibration": {"copy_bytes": copy_bytes, "kernel": ck[:80],
+                            "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4)}}
+    for k in fetch:
+        for pat, name in KERNEL_OF.items():
+            if pat in k and k in write:
+                out[name] = {"fetch_bytes": round(fetch[k] * 1024 * f_scale),
+                             "write_bytes": round(write[k] * 1024 * w_scale),
+                             "traffic": round((fetch[k] * f_scale + write[k] * w_scale) * 1024)}
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    allcfg = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            allcfg = json.load(f)
+    allcfg[config] = out
+    with open(path, "w") as f:
+        json.dump(allcfg, f, indent=1, sort_keys=True)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="periodic4096", choices=sorted(CONFIGS))
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run a plain MIB-MiB device copy first (counter calibration)")
+    ap.add_argument("--traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_MIB"),
+                    help="write profiles/hbm_traffic.json from two rocprofv3 --pmc passes")
     args = ap.parse_args()
+    if args.traffic:
+        traffic_from_counters(args.traffic[0], args.traffic[1],
+                              int(args.traffic[2]) * (1 << 20), args.config)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus must be 1 "
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     import cubismup_2d_amd as cup
     cfg = CONFIGS[args.config]
     sim = cup.Simulation(cfg["bpdx"], cfg["bpdy"], extent=cfg["extent"], nu=cfg["nu"], CFL=0.1,
@@ -106,9 +189,7 @@ def main():
     vel = random_vorticity_velocity(NX, NY, h)
     setup(sim, vel)
     cells = NX * NY
-    balg = dict(BALG)
-    if cfg["poisson"] == "freespace":
-        balg.update(fft_rows_fwd=3, fft_cols=5, fft_rows_inv=3)
+    balg = dict(BALG_FREESPACE if cfg["poisson"] == "freespace" else BALG)
 
     # warm-up with every full-grid timer on: picks the dominant kernel
     sim.timer_enable("*", 0)
@@ -135,10 +216,11 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get(args.config, {}).get(dominant)
-    step_balg = sum(balg[k] for k in balg if k != "find_max_u") * 8
+            t = json.load(f).get(args.config, {}).get(dominant)
+        traffic = t["traffic"] if t else None
+    step_balg = STEP_BALG_METRIC[cfg["poisson"]] * 8
     out = {
-        "metric": "Mcell-steps/s at 4096x2048 (FP64), % HBM roofline",
+        "metric": "Mcell-steps/s at 4096x2048 (FP64 vs FP32+refine), % HBM roofline, adjoint/fwd",
         "value": round(value, 2),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
@@ -153,7 +235,7 @@ def main():
         "config": {"workload": f"{args.config}: {NX}x{NY} cells ({cfg['bpdx']}x{cfg['bpdy']} "
                                f"blocks), {cfg['poisson']} FFT Poisson, nu={cfg['nu']}, CFL 0.1, "
                                f"one disk r={cfg['body']['radius']}",
-                   "cells": cells, "parallelism": "single GPU"},
+                   "cells": cells, "precision": "fp64", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2),

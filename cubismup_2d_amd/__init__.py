@@ -16,8 +16,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcup2d_hip.so")
 BS = 32
 
-VEL, CHI, PRES, TMP, TMPV, UDEF, INVRHO = range(7)
-NCOMP = [2, 1, 1, 1, 2, 2, 1]
+VEL, CHI, PRES, TMP, TMPV, UDEF, INVRHO, VFLUID = range(8)
+NCOMP = [2, 1, 1, 1, 2, 2, 1, 2]
 LAYOUT_BLOCKS, LAYOUT_DENSE = 0, 1
 POISSON = {"periodic": 0, "freespace": 1}
 
@@ -35,7 +35,7 @@ class Cfg(C.Structure):
     _fields_ = [("bpdx", C.c_int), ("bpdy", C.c_int), ("extent", C.c_double),
                 ("nu", C.c_double), ("CFL", C.c_double), ("lam", C.c_double),
                 ("fadeLen", C.c_double), ("poisson", C.c_int), ("implicit_penal", C.c_int),
-                ("stream", C.c_void_p)]
+                ("iterative_penal", C.c_int), ("stream", C.c_void_p)]
 
 
 class ShapeDesc(C.Structure):
@@ -71,7 +71,8 @@ EXPORTS = [
     "cup2d_get_scalars", "cup2d_set_scalars", "cup2d_get_shape", "cup2d_reset",
     "cup2d_find_max_u", "cup2d_calc_max_timestep", "cup2d_put_objects", "cup2d_advdiff",
     "cup2d_pressure_rhs", "cup2d_poisson_solve", "cup2d_pressure_correction",
-    "cup2d_body_update", "cup2d_penalize", "cup2d_pressure_single", "cup2d_compute_forces",
+    "cup2d_body_update", "cup2d_penalize", "cup2d_pressure_single", "cup2d_pressure_iterator",
+    "cup2d_compute_forces",
     "cup2d_advance", "cup2d_simulate", "cup2d_synchronize", "cup2d_poisson_dense",
     "cup2d_grid_size", "cup2d_timer_enable", "cup2d_timer_read",
 ]
@@ -111,6 +112,7 @@ def lib():
               "cup2d_pressure_correction", "cup2d_body_update", "cup2d_penalize",
               "cup2d_pressure_single", "cup2d_advance"]:
         getattr(L, n).argtypes = [P, D]
+    L.cup2d_pressure_iterator.argtypes = [P, D, C.POINTER(I), dp]
     L.cup2d_find_max_u.argtypes = [P, dp]
     L.cup2d_calc_max_timestep.argtypes = [P, dp]
     L.cup2d_simulate.argtypes = [P, I]
@@ -148,11 +150,12 @@ class Simulation:
     """One simulation on the GPU; methods mirror the reference's operators."""
 
     def __init__(self, bpdx, bpdy, extent=1.0, nu=1e-2, CFL=0.1, lam=None, fadeLen=0.01,
-                 poisson="periodic", implicit_penal=False, shapes=(), stream=None):
+                 poisson="periodic", implicit_penal=False, iterative_penal=False, shapes=(),
+                 stream=None):
         self.L = lib()
         lam = 1e3 / CFL if lam is None else lam
         cfg = Cfg(bpdx, bpdy, extent, nu, CFL, lam, fadeLen, POISSON[poisson],
-                  int(implicit_penal), stream)
+                  int(implicit_penal), int(iterative_penal), stream)
         h = C.c_void_p()
         rc = self.L.cup2d_create(C.byref(cfg), C.byref(h))
         if rc != OK:
@@ -254,6 +257,12 @@ class Simulation:
 
     def pressure_single(self, dt):
         self._chk(self.L.cup2d_pressure_single(self.h, dt))
+
+    def pressure_iterator(self, dt):
+        """PressureIterator_unif::operator(); returns (iterations, relDF)."""
+        n, r = C.c_int(), C.c_double()
+        self._chk(self.L.cup2d_pressure_iterator(self.h, dt, C.byref(n), C.byref(r)))
+        return n.value, r.value
 
     def compute_forces(self):
         self._chk(self.L.cup2d_compute_forces(self.h))

@@ -2,13 +2,14 @@
 // include/cup2d.h).  Host-side bookkeeping only: allocation, validation,
 // layout conversion through a device staging buffer, and the operator
 // sequence of Simulation::advance (Simulation.cpp:279-305).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
 
-#include "internal.hpp"
+#include "device_util.hpp"
 
 using namespace cup2d;
 
@@ -62,7 +63,7 @@ int fail(cup2d_ctx* c, int code, const std::string& msg) {
 
 int nfield_comps(int f) {
   switch (f) {
-    case CUP2D_VEL: case CUP2D_TMPV: case CUP2D_UDEF: return 2;
+    case CUP2D_VEL: case CUP2D_TMPV: case CUP2D_UDEF: case CUP2D_VFLUID: return 2;
     case CUP2D_CHI: case CUP2D_PRES: case CUP2D_TMP: case CUP2D_INVRHO: return 1;
     default: return 0;
   }
@@ -78,23 +79,17 @@ void field_planes(cup2d_ctx* c, int f, double** a, double** b) {
     case CUP2D_PRES: *a = c->pres; break;
     case CUP2D_TMP: *a = c->rhs; break;
     case CUP2D_INVRHO: *a = c->invrho; break;
+    case CUP2D_VFLUID: *a = c->vfu; *b = c->vfv; break;
     default: *a = nullptr;
   }
 }
 
-// PoissonSolver::cub2rhs `_is_touching` (PoissonSolver.cpp:71-79).
+// Blocks touched by the cub2rhs fade (PoissonSolver.cpp:71-79).
 std::vector<int> band_blocks(const Grid& g) {
   std::vector<int> out;
   for (int by = 0; by < g.bpdy; ++by)
-    for (int bx = 0; bx < g.bpdx; ++bx) {
-      const double mn0 = (double)bx * g.hb + g.h * (0 + 0.5);
-      const double mn1 = (double)by * g.hb + g.h * (0 + 0.5);
-      const double mx0 = (double)bx * g.hb + g.h * (BS - 1 + 0.5);
-      const double mx1 = (double)by * g.hb + g.h * (BS - 1 + 0.5);
-      const bool touchW = g.fadeLenX >= mn0, touchE = g.fadeLenX >= g.extent0 - mx0;
-      const bool touchS = g.fadeLenY >= mn1, touchN = g.fadeLenY >= g.extent1 - mx1;
-      if (touchN || touchE || touchS || touchW) out.push_back(bx + g.bpdx * by);
-    }
+    for (int bx = 0; bx < g.bpdx; ++bx)
+      if (band_block(g, bx, by)) out.push_back(bx + g.bpdx * by);
   return out;
 }
 
@@ -127,16 +122,63 @@ int need_fft(cup2d_ctx* c) {
   return fail(c, CUP2D_ERR_ARG, c->fft_err);
 }
 
-int pressure_single(cup2d_ctx* c) {
+// PressureSingle::operator() (PressureSingle.cpp:346-373).  fused (whole-step
+// path, advDiff's flux correction deferred): no separate RHS pass; the RHS of
+// the obstacle blocks is formed by the obstacle RHS kernel, every other
+// cell's by the forward row FFT, so the RHS plane holds only the obstacle
+// and boundary-band blocks afterwards.
+int pressure_single(cup2d_ctx* c, int fused) {
   RC(need_fft(c));
-  RC(enqueue_pressure_rhs(c));
-  RC(join_obstacles(c));
-  RC(enqueue_rhs_obstacles(c));
-  RC(enqueue_fade(c));
-  RC(enqueue_poisson_fft(c, c->rhs, c->pres));
+  if (fused && c->flux_deferred) {
+    RC(join_obstacles(c));
+    RC(enqueue_rhs_obstacles(c, 1));
+    RC(enqueue_fade_sums(c, 1));
+    RC(enqueue_poisson_fft(c, c->rhs, c->pres, 2));
+  } else {
+    RC(enqueue_pressure_rhs(c));
+    RC(join_obstacles(c));
+    RC(enqueue_rhs_obstacles(c));
+    RC(enqueue_fade_sums(c));
+    RC(enqueue_poisson_fft(c, c->rhs, c->pres, 1));
+  }
   RC(enqueue_pressure_correction(c, 1));
   RC(enqueue_body_update(c));
   RC(enqueue_penalize(c, c->maxpart + c->nwg));
+  return CUP2D_OK;
+}
+
+// PressureIterator_unif::operator() (PressureIterator_unif.cpp:216-283).  The
+// convergence test needs relDF on the host: one synchronization per iteration.
+int pressure_iterator(cup2d_ctx* c, int* iters_out, double* reldf_out) {
+  RC(need_fft(c));
+  RC(join_obstacles(c));
+  double* d_reldf = c->red + 4096 - 8;
+  RC(enqueue_copy_vel_to_tmpv(c));
+  int iter;
+  double relDF = 1e3;
+  for (iter = 0; iter < 1000; ++iter) {
+    RC(enqueue_iter_project(c));
+    RC(enqueue_iter_body(c));
+    RC(enqueue_iter_penalize(c, d_reldf));
+    RC(enqueue_iter_rhs(c));
+    RC(enqueue_fade_sums(c));
+    RC(enqueue_poisson_fft(c, c->rhs, c->pres, 1));
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(&relDF, d_reldf, sizeof(double), hipMemcpyDeviceToHost,
+                                      c->stream));
+    CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    const bool conv = relDF < c->iter_target || iter > 100 || iter > 2 * c->iter_old_nsteps;
+    if (conv) {
+      RC(enqueue_pressure_correction(c, 0));  // finalizePressure (:186-212)
+      break;
+    }
+  }
+  c->iter_old_nsteps = iter + 1;
+  if (c->iter_old_nsteps > 30) c->iter_target = std::max(relDF, c->iter_target);
+  if (c->iter_old_nsteps > 10) c->iter_target *= 1.01;
+  if (c->iter_old_nsteps <= 2) c->iter_target *= 0.99;
+  c->iter_target = std::min(1e-3, std::max(1e-5, c->iter_target));
+  if (iters_out) *iters_out = iter + 1;
+  if (reldf_out) *reldf_out = relDF;
   return CUP2D_OK;
 }
 
@@ -144,8 +186,14 @@ int pressure_single(cup2d_ctx* c) {
 // previous step (device loop) instead of the value already on the device.
 int advance_enqueued(cup2d_ctx* c, int compute_dt) {
   RC(enqueue_put_objects(c, compute_dt, 1));
-  RC(enqueue_advdiff(c, 1));
-  RC(pressure_single(c));
+  if (c->cfg.iterative_penal) {
+    RC(enqueue_advdiff(c, 0));
+    RC(pressure_iterator(c, nullptr, nullptr));
+    RC(enqueue_find_max_u_partials(c));  // for the next step's dt
+  } else {
+    RC(enqueue_advdiff(c, 1));
+    RC(pressure_single(c, 1));
+  }
   RC(enqueue_forces(c, 1, 1));
   c->host_step += 1;
   return CUP2D_OK;
@@ -210,7 +258,7 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
   }
   const size_t n = (size_t)g.NX * g.NY;
   double** planes[] = {&c->u, &c->v, &c->tu, &c->tv, &c->pres, &c->rhs,
-                       &c->sdf, &c->chi, &c->udx, &c->udy, &c->invrho};
+                       &c->chi, &c->udx, &c->udy, &c->invrho, &c->vfu, &c->vfv};
   for (double** p : planes)
     if (hipMalloc(p, sizeof(double) * n) != hipSuccess) { c->err = "out of device memory"; return bail(CUP2D_ERR_HIP); }
   if (hipMalloc(&c->staging, sizeof(double) * 2 * n) != hipSuccess) { c->err = "out of device memory"; return bail(CUP2D_ERR_HIP); }
@@ -274,7 +322,7 @@ int cup2d_destroy(cup2d_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   fft_destroy(c);
   obstacles_free(c);
-  double* ptrs[] = {c->u, c->v, c->tu, c->tv, c->pres, c->rhs, c->sdf, c->chi,
+  double* ptrs[] = {c->u, c->v, c->tu, c->tv, c->pres, c->rhs, c->chi, c->vfu, c->vfv,
                     c->udx, c->udy, c->invrho, c->staging, c->red, c->ramp,
                     c->fluxpart, c->maxpart};
   if (c->obmap) (void)hipFree(c->obmap);
@@ -417,6 +465,7 @@ int cup2d_reset(cup2d_ctx* c) {
   RC(sync_all(c));
   c->host_step = 0;
   RC(enqueue_shapes_reset(c));
+  RC(enqueue_obst_reset(c));
   RC(enqueue_ic(c));
   RC(enqueue_put_objects(c, 0));
   RC(enqueue_apply_obj_vel(c));
@@ -469,8 +518,8 @@ int cup2d_pressure_rhs(cup2d_ctx* c, double dt) {
 int cup2d_poisson_solve(cup2d_ctx* c) {
   CHECK_CTX(c);
   RC(need_fft(c));
-  RC(enqueue_fade(c));
-  RC(enqueue_poisson_fft(c, c->rhs, c->pres));
+  RC(enqueue_fade_sums(c));
+  RC(enqueue_poisson_fft(c, c->rhs, c->pres, 1));
   return sync_and_check(c);
 }
 
@@ -501,7 +550,15 @@ int cup2d_pressure_single(cup2d_ctx* c, double dt) {
   CHECK_CTX(c);
   if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
   RC(enqueue_set_dt(c, dt));
-  RC(pressure_single(c));
+  RC(pressure_single(c, 0));
+  return sync_and_check(c);
+}
+
+int cup2d_pressure_iterator(cup2d_ctx* c, double dt, int* iters, double* relDF) {
+  CHECK_CTX(c);
+  if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
+  RC(enqueue_set_dt(c, dt));
+  RC(pressure_iterator(c, iters, relDF));
   return sync_and_check(c);
 }
 
@@ -543,7 +600,7 @@ int cup2d_poisson_dense(cup2d_ctx* c, const double* rhs, double* sol) {
   double* din = c->staging;
   double* dout = c->staging + n;
   CUP2D_HIP_CHECK(c, hipMemcpyAsync(din, rhs, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  RC(enqueue_poisson_fft(c, din, dout));
+  RC(enqueue_poisson_fft(c, din, dout, 0));
   CUP2D_HIP_CHECK(c, hipMemcpyAsync(sol, dout, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return CUP2D_OK;

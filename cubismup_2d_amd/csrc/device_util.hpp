@@ -13,8 +13,45 @@ __host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b 
 __host__ __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
 
 // Cubism BlockInfo::pos: origin + h*(i+0.5), origin = block index * hb.
-__device__ inline double cell_pos(int blk_index, int i, double hb, double h) {
+__host__ __device__ inline double cell_pos(int blk_index, int i, double hb, double h) {
   return (double)blk_index * hb + h * (i + 0.5);
+}
+
+// PoissonSolver::cub2rhs `_is_touching` (PoissonSolver.cpp:71-79): block
+// (bx, by) lies within fadeLen of a domain face.
+__host__ __device__ inline bool band_block(const Grid& g, int bx, int by) {
+  const double mn0 = cell_pos(bx, 0, g.hb, g.h), mn1 = cell_pos(by, 0, g.hb, g.h);
+  const double mx0 = cell_pos(bx, BS - 1, g.hb, g.h), mx1 = cell_pos(by, BS - 1, g.hb, g.h);
+  const bool touchW = g.fadeLenX >= mn0, touchE = g.fadeLenX >= g.extent0 - mx0;
+  const bool touchS = g.fadeLenY >= mn1, touchN = g.fadeLenY >= g.extent1 - mx1;
+  return touchN || touchE || touchS || touchW;
+}
+
+// cub2rhs fadeArg (PoissonSolver.cpp:88-93) at cell centre (px, py).
+__host__ __device__ inline double fade_arg(const Grid& g, double px, double py) {
+  constexpr double eps = 2.220446049250313e-16;  // std::numeric_limits<double>::epsilon()
+  const double invFadeX = 1 / dmax(g.fadeLenX, eps), invFadeY = 1 / dmax(g.fadeLenY, eps);
+  const double yt = invFadeY * dmax(0.0, g.fadeLenY - g.extent1 + py);
+  const double yb = invFadeY * dmax(0.0, g.fadeLenY - py);
+  const double xt = invFadeX * dmax(0.0, g.fadeLenX - g.extent0 + px);
+  const double xb = invFadeX * dmax(0.0, g.fadeLenX - px);
+  return dmin(dmax(dmax(dmax(yt, yb), xt), xb), 1.0);
+}
+
+// cub2rhs correction factor from the totals (sum, sum of positive parts,
+// sum of negative parts) of the faded RHS (PoissonSolver.cpp:100-103).
+__host__ __device__ inline double fade_corr_factor(double sumRHS, double sumPOS, double sumNEG) {
+  constexpr double eps = 2.220446049250313e-16;
+  sumNEG = dmax(sumNEG, eps);
+  sumPOS = dmax(sumPOS, eps);
+  return sumRHS > 0 ? sumRHS / sumPOS : sumRHS / sumNEG;
+}
+
+// The faded RHS of one cell of a band block (PoissonSolver.cpp:105-121).
+__host__ __device__ inline double fade_cell(double r, double fa, double corr) {
+  const double fadeRHS = r * (fa * fa);
+  const double fadeCorrection = corr * (corr > 0 ? dmax(fadeRHS, 0.0) : -dmin(fadeRHS, 0.0));
+  return r * (1 - fa * fa) + fadeCorrection;
 }
 
 template <int NT>
@@ -77,14 +114,171 @@ __device__ inline double block_max_dyn(double v, double* lds) {
   return s;
 }
 
+// NQ workgroup sums at once: a[j] <- sum over the workgroup of a[j], with the
+// same tree as block_sum for each j (so the results are bit-identical to NQ
+// separate block_sum calls) but one pair of barriers.  lds holds NT/64*NQ.
+template <int NT, int NQ>
+__device__ inline void block_sum_multi(double* a, double* lds) {
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a[j] += __shfl_xor(a[j], o, 64);
+  }
+  constexpr int NW = NT / 64;
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  __syncthreads();
+  if (l == 0) {
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) lds[w * NQ + j] = a[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+    double s = 0;
+    if (NW > 1) {
+#pragma unroll
+      for (int i = 0; i < NW; ++i) s += lds[i * NQ + j];
+    } else {
+      s = lds[j];
+    }
+    a[j] = s;
+  }
+}
+
+// Column sums of an n x NQ table (row i at a + i*stride): thread t adds rows
+// t, t+NT, ... in order, then block_sum_multi.  Loads run U rows ahead of the
+// additions, so a long table costs one memory latency per U rows; the order of
+// the additions, and so the result, is the same in every workgroup.
+template <int NT, int NQ, int U = 4>
+__device__ inline void block_sum_table(const double* a, int n, int stride, double* out,
+                                       double* lds) {
+  double s[NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) s[j] = 0;
+  int i = threadIdx.x;
+  for (; i + (U - 1) * NT < n; i += U * NT) {
+    double v[U][NQ];
+#pragma unroll
+    for (int r = 0; r < U; ++r)
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) v[r][j] = a[(size_t)(i + r * NT) * stride + j];
+#pragma unroll
+    for (int r = 0; r < U; ++r)
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) s[j] += v[r][j];
+  }
+  for (; i < n; i += NT)
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) s[j] += a[(size_t)i * stride + j];
+  block_sum_multi<NT, NQ>(s, lds);
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) out[j] = s[j];
+}
+
 // Sum of n values (fixed order independent of the caller): thread t sums
 // t, t+NT, ... sequentially, then block_sum.  Every workgroup that calls it
 // on the same data gets bit-identical results.
 template <int NT>
 __device__ inline double block_sum_array(const double* a, int n, int stride, double* lds) {
+  double s;
+  block_sum_table<NT, 1>(a, n, stride, &s, lds);
+  return s;
+}
+
+// Max of n non-negative values with U loads in flight per thread (max is
+// independent of the order).
+template <int NT, int U = 8>
+__device__ inline double max_array(const double* a, int n) {
+  double m = 0;
+  int i = threadIdx.x;
+  for (; i + (U - 1) * NT < n; i += U * NT) {
+    double v[U];
+#pragma unroll
+    for (int r = 0; r < U; ++r) v[r] = a[i + r * NT];
+#pragma unroll
+    for (int r = 0; r < U; ++r) m = dmax(m, v[r]);
+  }
+  for (; i < n; i += NT) m = dmax(m, a[i]);
+  return m;
+}
+
+// ---------------------------------------------------------- advDiff flux
+// The workgroups on the edge of a gx x gy stencil grid, numbered in blockID
+// order (row 0, the first and last workgroup of each middle row, the last
+// row); -1 for the others.  Only they see boundary cells.
+__host__ __device__ inline int edge_count(int gx, int gy) {
+  return gy == 1 ? gx : 2 * gx + (gx == 1 ? 1 : 2) * (gy - 2);
+}
+__device__ inline int edge_index(int bx, int by, int gx, int gy) {
+  if (by == 0) return bx;
+  if (by == gy - 1) return gx + (gx == 1 ? 1 : 2) * (gy - 2) + bx;
+  if (bx == 0) return gx + (gx == 1 ? 1 : 2) * (by - 1);
+  if (bx == gx - 1) return gx + 2 * (by - 1) + 1;
+  return -1;
+}
+
+// corr = IF / (2 NY + 2 NX) (advDiff.cpp:105) from the edge workgroups'
+// partials of the boundary outflow, summed in a fixed order (any workgroup
+// size; the same size gives the same value).
+__device__ inline double flux_corr(const double* __restrict__ fluxpart, int ne, const Grid& g,
+                                   double* lds) {
+  const int nt = blockDim.x;
   double s = 0;
-  for (int i = threadIdx.x; i < n; i += NT) s += a[(size_t)i * stride];
-  return block_sum<NT>(s, lds);
+  int e = threadIdx.x;
+  for (; e + 3 * nt < ne; e += 4 * nt) {
+    const double a0 = fluxpart[e], a1 = fluxpart[e + nt], a2 = fluxpart[e + 2 * nt],
+                 a3 = fluxpart[e + 3 * nt];
+    s += a0; s += a1; s += a2; s += a3;
+  }
+  for (; e < ne; e += nt) s += fluxpart[e];
+  const double IF = block_sum_dyn(s, lds);
+  return IF / (2 * (BS * g.bpdy) + 2 * (BS * g.bpdx));
+}
+
+// corr as above, summed by the first 256 threads in the grouping of a
+// 256-thread workgroup whatever blockDim.x (>= 256) is, so every kernel that
+// needs it gets the same value.
+__device__ inline double flux_corr256(const double* __restrict__ fluxpart, int ne, const Grid& g,
+                                      double* lds) {
+  constexpr int nt = 256;
+  double s = 0;
+  if (threadIdx.x < nt) {
+    int e = threadIdx.x;
+    for (; e + 3 * nt < ne; e += 4 * nt) {
+      const double a0 = fluxpart[e], a1 = fluxpart[e + nt], a2 = fluxpart[e + 2 * nt],
+                   a3 = fluxpart[e + 3 * nt];
+      s += a0; s += a1; s += a2; s += a3;
+    }
+    for (; e < ne; e += nt) s += fluxpart[e];
+  }
+  const double IF = block_sum_dyn(s, lds);
+  return IF / (2 * (BS * g.bpdy) + 2 * (BS * g.bpdx));
+}
+
+// A velocity component as the advDiff flux correction leaves it on the
+// boundary (advDiff.cpp:107-115): + corr on the first cell of its direction,
+// - corr on the last (n cells).
+__device__ inline double flux_fixed(double val, int i, int n, double corr) {
+  if (i == 0) return val + corr;
+  if (i == n - 1) return val - corr;
+  return val;
+}
+
+// PressureSingle::updatePressureRHS's divergence term at cell (x, y)
+// (PressureSingle.cpp:112-128), r = facDiv (u_E - u_W + v_N - v_S) with
+// zero-gradient ghosts, on the advDiff output u, v whose boundary flux
+// correction corr is still to be applied; the same operations as k_rhs.
+__device__ inline double rhs_cell(const double* __restrict__ u, const double* __restrict__ v,
+                                  int NX, int NY, int x, int y, double corr, double facDiv) {
+  const int xe = x + 1 < NX ? x + 1 : x, xw = x > 0 ? x - 1 : x;
+  const int yn = y + 1 < NY ? y + 1 : y, ys = y > 0 ? y - 1 : y;
+  const double ue = flux_fixed(u[(size_t)y * NX + xe], xe, NX, corr);
+  const double uw = flux_fixed(u[(size_t)y * NX + xw], xw, NX, corr);
+  const double vn = flux_fixed(v[(size_t)yn * NX + x], yn, NY, corr);
+  const double vs = flux_fixed(v[(size_t)ys * NX + x], ys, NY, corr);
+  const double divVx = ue - uw;
+  const double divVy = vn - vs;
+  return facDiv * (divVx + divVy);
 }
 
 }  // namespace cup2d

@@ -49,7 +49,6 @@ template <bool FWD>
 __device__ __forceinline__ double2 rot(double2 a) {
   return FWD ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
-__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 
 template <bool FWD>
 __device__ __forceinline__ void dft2(double2& a, double2& b) {
@@ -128,151 +127,307 @@ __device__ __forceinline__ void dft(double2* v) {
   }
 }
 
-constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+// Elements per thread of the FFT core (the first pass is radix E): 16 gives
+// fewer LDS round trips, 8 twice the threads per transform.
+#ifndef CUP2D_FFT_E
+#define CUP2D_FFT_E 16
+#endif
+constexpr int FE = CUP2D_FFT_E;
+// minimum resident workgroups per CU requested from the compiler (caps VGPRs)
+#ifndef CUP2D_FFT_MINB
+#define CUP2D_FFT_MINB 1
+#endif
+static_assert(FE == 8 || FE == 16, "FFT elements per thread");
+
+// LDS index with one pad slot every E entries (stride-E stores hit distinct banks)
+template <int E>
+__device__ __forceinline__ int padE(int i) { return i + i / E; }
+
+// Passes after the first: span NS = E, E^2, ...; radix R = min(E, L/NS);
+// PER = E/R butterflies per thread.  Butterfly j = t + p*NT of a pass needs
+// the twiddles b^r, r = 1..R-1, with b = tw[(j % NS) * L/(NS R)]; a thread
+// keeps only the bases b (one per butterfly and pass) in registers.
+template <int L, int E, int NS>
+struct TwCount {
+  static constexpr int R = (L / NS) >= E ? E : L / NS;
+  static constexpr int n = NS >= L ? 0 : (E / R) + TwCount<L, E, (NS >= L ? L : NS * R)>::n;
+};
+template <int L, int E>
+struct TwCount<L, E, L> {
+  static constexpr int n = 0;
+};
+
+// A thread's twiddle bases for every pass (forward sign; the inverse
+// conjugates at use), loaded once per kernel.
+template <int L, int E, int NS>
+__device__ __forceinline__ void tw_load(const double2* __restrict__ tw, int t, double2* w) {
+  if constexpr (NS < L) {
+    constexpr int R = TwCount<L, E, NS>::R, PER = E / R, NT = L / E, TSTEP = L / (NS * R);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) w[p] = tw[((t + p * NT) % NS) * TSTEP];
+    tw_load<L, E, NS * R>(tw, t, w + PER);
+  }
+}
+
+// pw[r] = b^r, r = 1..R-1, by repeated squaring and products of two lower
+// powers (at most four roundings deep), conjugated for the inverse.
+template <int R, bool FWD>
+__device__ __forceinline__ void tw_powers(double2 b, double2* pw) {
+  if (!FWD) b.y = -b.y;
+  pw[1] = b;
+#pragma unroll
+  for (int r = 2; r < R; ++r) {
+    const int hi = (r & 8) ? 8 : (r & 4) ? 4 : (r & 2) ? 2 : 1;
+    pw[r] = (r == hi) ? cmul(pw[hi / 2], pw[hi / 2]) : cmul(pw[hi], pw[r - hi]);
+  }
+}
 
 // One Stockham pass of radix R at span NS over an L-point sequence held in
-// LDS at lds[base + pad(i)*stride]; NT = L/16 threads, 16/R butterflies each.
-template <int L, int R, int NS, bool FWD>
+// LDS at lds[base + padE(i)*stride]; NT = L/E threads, E/R butterflies each;
+// w holds this pass's twiddle bases.
+template <int L, int E, int R, int NS, bool FWD>
 __device__ __forceinline__ void stockham_pass(double2* lds, int base, int stride, int t,
-                                              const double2* __restrict__ tw) {
-  constexpr int NT = L / 16, NBF = L / R, PER = 16 / R;
+                                              const double2* w) {
+  constexpr int NT = L / E, NBF = L / R, PER = E / R;
   double2 v[PER][R];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int j = t + p * NT;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[p][r] = lds[base + pad(j + r * NBF) * stride];
+    for (int r = 0; r < R; ++r) v[p][r] = lds[base + padE<E>(j + r * NBF) * stride];
   }
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int j = t + p * NT;
     const int k = j % NS;
-    if constexpr (NS > 1) {
-      constexpr int TSTEP = L / (NS * R);
+    double2 pw[R];
+    tw_powers<R, FWD>(w[p], pw);
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        double2 w = tw[k * TSTEP * r];
-        if (!FWD) w.y = -w.y;
-        v[p][r] = cmul(v[p][r], w);
-      }
-    }
+    for (int r = 1; r < R; ++r) v[p][r] = cmul(v[p][r], pw[r]);
     dft<R, FWD>(v[p]);
     const int idxD = (j / NS) * NS * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) lds[base + pad(idxD + r * NS) * stride] = v[p][r];
+    for (int r = 0; r < R; ++r) lds[base + padE<E>(idxD + r * NS) * stride] = v[p][r];
   }
   __syncthreads();
 }
 
-// First pass (radix 16, span 1) straight from registers: x[q] holds element
+// First pass (radix E, span 1) straight from registers: x[q] holds element
 // t + q*NT, which is the Stockham input of butterfly j = t.  No twiddles.
-template <int L, bool FWD>
+template <int L, int E, bool FWD>
 __device__ __forceinline__ void pass_first(double2* x, double2* lds, int base, int stride, int t) {
-  dft<16, FWD>(x);
+  dft<E, FWD>(x);
 #pragma unroll
-  for (int r = 0; r < 16; ++r) lds[base + pad(t * 16 + r) * stride] = x[r];
+  for (int r = 0; r < E; ++r) lds[base + padE<E>(t * E + r) * stride] = x[r];
   __syncthreads();
 }
 
 // Last pass (span NS = L/R) into registers: its outputs j + r*NS land on
-// element t + q*NT with q = p + (16/R)*r, i.e. the same strided order the
+// element t + q*NT with q = p + (E/R)*r, i.e. the same strided order the
 // first pass reads.  Ends with a barrier so the LDS can be reused.
-template <int L, int R, int NS, bool FWD>
+template <int L, int E, int R, int NS, bool FWD>
 __device__ __forceinline__ void pass_last(double2* lds, int base, int stride, int t,
-                                          const double2* __restrict__ tw, double2* x) {
-  constexpr int NT = L / 16, NBF = L / R, PER = 16 / R;
+                                          const double2* w, double2* x) {
+  constexpr int NT = L / E, NBF = L / R, PER = E / R;
   static_assert(NS * R == L, "last pass");
   double2 v[PER][R];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int j = t + p * NT;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[p][r] = lds[base + pad(j + r * NBF) * stride];
+    for (int r = 0; r < R; ++r) v[p][r] = lds[base + padE<E>(j + r * NBF) * stride];
   }
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
-    const int j = t + p * NT;
-    constexpr int TSTEP = L / (NS * R);
+    double2 pw[R];
+    tw_powers<R, FWD>(w[p], pw);
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-      double2 w = tw[j * TSTEP * r];
-      if (!FWD) w.y = -w.y;
-      v[p][r] = cmul(v[p][r], w);
-    }
+    for (int r = 1; r < R; ++r) v[p][r] = cmul(v[p][r], pw[r]);
     dft<R, FWD>(v[p]);
 #pragma unroll
     for (int r = 0; r < R; ++r) x[p + PER * r] = v[p][r];
   }
 }
 
-template <int L, int NS, bool FWD>
+template <int L, int E, int NS, bool FWD>
 __device__ __forceinline__ void fft_tail(double2* lds, int base, int stride, int t,
-                                         const double2* __restrict__ tw, double2* x) {
+                                         const double2* w, double2* x) {
   constexpr int REM = L / NS;
-  constexpr int R = REM >= 16 ? 16 : REM;
+  constexpr int R = REM >= E ? E : REM;
   if constexpr (NS * R == L) {
-    pass_last<L, R, NS, FWD>(lds, base, stride, t, tw, x);
+    pass_last<L, E, R, NS, FWD>(lds, base, stride, t, w, x);
   } else {
-    stockham_pass<L, R, NS, FWD>(lds, base, stride, t, tw);
-    fft_tail<L, NS * R, FWD>(lds, base, stride, t, tw, x);
+    stockham_pass<L, E, R, NS, FWD>(lds, base, stride, t, w);
+    fft_tail<L, E, NS * R, FWD>(lds, base, stride, t, w + E / R, x);
   }
 }
 
-// L-point complex FFT (L >= 32, power of two) by NT = L/16 threads; x[q] is
+// L-point complex FFT (L >= 2E, power of two) by NT = L/E threads; x[q] is
 // element t + q*NT on entry and the transform's element t + q*NT on exit.
-// The sequence is staged in LDS at lds[base + pad(i)*stride] between passes.
-template <int L, bool FWD>
+// The sequence is staged in LDS at lds[base + padE(i)*stride] between
+// passes; w = this thread's twiddle bases from tw_load<L, E, E>.
+template <int L, int E, bool FWD>
 __device__ __forceinline__ void fft_regs(double2* x, double2* lds, int base, int stride, int t,
-                                         const double2* __restrict__ tw) {
-  pass_first<L, FWD>(x, lds, base, stride, t);
-  fft_tail<L, 16, FWD>(lds, base, stride, t, tw, x);
+                                         const double2* w) {
+  pass_first<L, E, FWD>(x, lds, base, stride, t);
+  fft_tail<L, E, E, FWD>(lds, base, stride, t, w, x);
 }
+
+template <int L, int E>
+constexpr int lds_len() { return L + L / E; }
+template <int L, int E>
+constexpr int rows_np() { return (L / E) >= 256 ? 1 : 256 / (L / E); }
 
 // ---------------------------------------------------------- rows forward
 // in: nrows rows of ncols valid reals (row pitch in_pitch), zero beyond;
-// out: spec[cg][y][G], y < spec_rows.
-template <int L, int NP>
-__global__ __launch_bounds__(NP * L / 16) void k_rows_fwd(const double* __restrict__ in,
-                                                          int in_pitch, int ncols, int nrows,
-                                                          double2* __restrict__ spec,
-                                                          int spec_rows,
-                                                          const double2* __restrict__ tw) {
-  constexpr int NT = L / 16, PADL = L + L / 16;
+// out: spec[cg][y][G], y < spec_rows.  NP row pairs per workgroup.
+//
+// SRC_FADE: the input is the pressure RHS on the grid; cells of band blocks
+// are faded as PoissonSolver::cub2rhs does (PoissonSolver.cpp:105-121) while
+// they are loaded, and written back (the reference modifies the source in
+// place).  The correction factor comes from the partial sums of
+// k_fade_sums, added in the same order by every workgroup.
+//
+// SRC_DIV: the RHS is formed here from the advDiff output (u, v), as
+// PressureSingle::updatePressureRHS does (PressureSingle.cpp:112-128), with
+// the deferred advDiff boundary-flux correction; the obstacle blocks take
+// the values the obstacle RHS kernels left in the RHS plane; then the fade
+// as for SRC_FADE.  u of both rows of a pair goes through LDS for the x
+// neighbours; v of rows y0-1 .. y0+2 stays in registers.
+enum { SRC_PLAIN = 0, SRC_FADE = 1, SRC_DIV = 2 };
+struct RowSource {
+  Grid g;
+  const double* part;   // fade partials: 3 sums per fade-sum workgroup
+  int npart;            // fade-sum workgroups (0: no fade)
+  double* rhs;          // RHS plane: faded band cells written back; obstacle blocks read (SRC_DIV)
+  StepState* st;        // fade_corr, flux_corr, dt
+  const double* u;      // SRC_DIV: advDiff output
+  const double* v;
+  const unsigned char* obmap;
+  const double* fluxpart;  // advDiff edge partials
+  int ne;
+};
+
+template <int L, int NP, int E, int SRC>
+__global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
+    const double* in, int in_pitch, int ncols, int nrows, double2* __restrict__ spec,
+    int spec_rows, const double2* __restrict__ tw, RowSource fd) {
+  constexpr int NT = L / E, PADL = lds_len<L, E>();
   extern __shared__ double2 lds[];
+  __shared__ double s_red[64];  // workgroup reductions (<= 16 waves x 3)
   const int f = threadIdx.x / NT, t = threadIdx.x % NT;
   const int y0 = 2 * (blockIdx.x * NP + f);
   const bool valid = y0 < nrows;
   const int base = f * PADL;
-  const double* r0 = in + (size_t)y0 * in_pitch;
-  const double* r1 = r0 + in_pitch;
-  double2 x[16];
+  double2 w[TwCount<L, E, E>::n];
+  tw_load<L, E, E>(tw, t, w);
+  double2 x[E];
+  if constexpr (SRC == SRC_DIV) {
+    const Grid& g = fd.g;
+    const int NX = ncols, NY = nrows;
+    const int yc = valid ? y0 : 0;
+    const int ym = yc > 0 ? yc - 1 : yc, yp = yc + 2 < NY ? yc + 2 : yc + 1;
+    double* ud = (double*)lds + (size_t)f * 2 * PADL;  // u of rows y0, y0+1
+    double vm[E], v0[E], v1[E], vp[E];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int i = t + q * NT;
-    double a = 0, b = 0;
-    if (valid && i < ncols) { a = r0[i]; b = r1[i]; }
-    x[q] = make_double2(a, b);
+    for (int q = 0; q < E; ++q) {
+      const int i = t + q * NT;
+      if (i < NX) {
+        ud[i] = fd.u[(size_t)yc * NX + i];
+        ud[L + i] = fd.u[(size_t)(yc + 1) * NX + i];
+        vm[q] = fd.v[(size_t)ym * NX + i];
+        v0[q] = fd.v[(size_t)yc * NX + i];
+        v1[q] = fd.v[(size_t)(yc + 1) * NX + i];
+        vp[q] = fd.v[(size_t)yp * NX + i];
+      }
+    }
+    const double fcorr = flux_corr256(fd.fluxpart, fd.ne, g, s_red);
+    if (blockIdx.x == 0 && threadIdx.x == 0) fd.st->flux_corr = fcorr;
+    const double facDiv = 0.5 * g.h / fd.st->dt;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int i = t + q * NT;
+      double r0 = 0, r1 = 0;
+      if (valid && i < NX) {
+        const int xe = i + 1 < NX ? i + 1 : i, xw = i > 0 ? i - 1 : i;
+        const double ue0 = flux_fixed(ud[xe], xe, NX, fcorr);
+        const double uw0 = flux_fixed(ud[xw], xw, NX, fcorr);
+        const double ue1 = flux_fixed(ud[L + xe], xe, NX, fcorr);
+        const double uw1 = flux_fixed(ud[L + xw], xw, NX, fcorr);
+        const double vs0 = flux_fixed(vm[q], ym, NY, fcorr);
+        const double vc0 = flux_fixed(v0[q], y0, NY, fcorr);
+        const double vc1 = flux_fixed(v1[q], y0 + 1, NY, fcorr);
+        const double vn1 = flux_fixed(vp[q], yp, NY, fcorr);
+        r0 = facDiv * ((ue0 - uw0) + (vc1 - vs0));
+        r1 = facDiv * ((ue1 - uw1) + (vn1 - vc0));
+        const int bx = i / BS;
+        if (fd.obmap[(y0 / BS) * g.bpdx + bx]) r0 = fd.rhs[(size_t)y0 * NX + i];
+        if (fd.obmap[((y0 + 1) / BS) * g.bpdx + bx]) r1 = fd.rhs[(size_t)(y0 + 1) * NX + i];
+      }
+      x[q] = make_double2(r0, r1);
+    }
+    __syncthreads();  // ud is reused below
+  } else {
+    const double* r0 = in + (size_t)(valid ? y0 : 0) * in_pitch;
+    const double* r1 = r0 + in_pitch;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int i = t + q * NT;
+      double a = 0, b = 0;
+      if (valid && i < ncols) { a = r0[i]; b = r1[i]; }
+      x[q] = make_double2(a, b);
+    }
   }
-  fft_regs<L, true>(x, lds, base, 1, t, tw);
+  if constexpr (SRC != SRC_PLAIN) {
+    if (fd.npart > 0) {
+      double sums[3];
+      block_sum_table<NP * L / E, 3>(fd.part, fd.npart, 3, sums, s_red);
+      const double corr = fade_corr_factor(sums[0], sums[1], sums[2]);
+      if (blockIdx.x == 0 && threadIdx.x == 0) fd.st->fade_corr = corr;
+      const Grid& g = fd.g;
+      const int by0 = y0 / BS, by1 = (y0 + 1) / BS;
+      if (valid) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+          const int i = t + q * NT;
+          if (i >= ncols) continue;
+          const int bx = i / BS;
+          const double px = cell_pos(bx, i % BS, g.hb, g.h);
+          if (band_block(g, bx, by0)) {
+            const double py = cell_pos(by0, y0 % BS, g.hb, g.h);
+            x[q].x = fade_cell(x[q].x, fade_arg(g, px, py), corr);
+            fd.rhs[(size_t)y0 * in_pitch + i] = x[q].x;
+          }
+          if (band_block(g, bx, by1)) {
+            const double py = cell_pos(by1, (y0 + 1) % BS, g.hb, g.h);
+            x[q].y = fade_cell(x[q].y, fade_arg(g, px, py), corr);
+            fd.rhs[(size_t)(y0 + 1) * in_pitch + i] = x[q].y;
+          }
+        }
+      }
+    }
+  }
+  fft_regs<L, E, true>(x, lds, base, 1, t, w);
   // split z = a + i b: A_k = (Z_k + conj Z_{L-k})/2, B_k = (Z_k - conj Z_{L-k})/(2i);
   // the partners Z_{L-k} (upper half) go through LDS
 #pragma unroll
-  for (int q = 8; q < 16; ++q) lds[base + pad(t + q * NT)] = x[q];
+  for (int q = E / 2; q < E; ++q) lds[base + padE<E>(t + q * NT)] = x[q];
   __syncthreads();
   if (!valid) return;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
+  for (int q = 0; q < E / 2; ++q) {
     const int k = t + q * NT;  // 0 .. L/2-1
     const double2 Z = x[q];
     double2 A, B;
     if (k == 0) {
-      const double2 ZN = x[8];  // element L/2
+      const double2 ZN = x[E / 2];  // element L/2
       A = make_double2(Z.x, ZN.x);
       B = make_double2(Z.y, ZN.y);
     } else {
-      const double2 Zm = lds[base + pad(L - k)];
+      const double2 Zm = lds[base + padE<E>(L - k)];
       const double2 sm = make_double2(Z.x + Zm.x, Z.y - Zm.y);  // Z + conj(Zm)
       const double2 d = make_double2(Z.x - Zm.x, Z.y + Zm.y);   // Z - conj(Zm)
       A = make_double2(0.5 * sm.x, 0.5 * sm.y);
@@ -286,22 +441,24 @@ __global__ __launch_bounds__(NP * L / 16) void k_rows_fwd(const double* __restri
 }
 
 // ---------------------------------------------------------- rows inverse
-template <int L, int NP>
-__global__ __launch_bounds__(NP * L / 16) void k_rows_inv(const double2* __restrict__ spec,
-                                                          int spec_rows, double* __restrict__ out,
-                                                          int out_pitch, int ncols, int nrows,
-                                                          const double2* __restrict__ tw) {
-  constexpr int NT = L / 16, PADL = L + L / 16;
+template <int L, int NP, int E>
+__global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_inv(const double2* __restrict__ spec,
+                                                         int spec_rows, double* __restrict__ out,
+                                                         int out_pitch, int ncols, int nrows,
+                                                         const double2* __restrict__ tw) {
+  constexpr int NT = L / E, PADL = lds_len<L, E>();
   extern __shared__ double2 lds[];
   const int f = threadIdx.x / NT, t = threadIdx.x % NT;
   const int y0 = 2 * (blockIdx.x * NP + f);
   const bool valid = y0 < nrows;
   const int base = f * PADL;
-  double2 x[16];
+  double2 w[TwCount<L, E, E>::n];
+  tw_load<L, E, E>(tw, t, w);
+  double2 x[E];
   // Z_k = A_k + i B_k and Z_{L-k} = conj(A_k) + i conj(B_k); DC and Nyquist
   // of both rows are packed in slot 0 (real parts only)
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
+  for (int q = 0; q < E / 2; ++q) {
     const int k = t + q * NT;
     double2 A = make_double2(0, 0), B = make_double2(0, 0);
     if (valid) {
@@ -311,25 +468,25 @@ __global__ __launch_bounds__(NP * L / 16) void k_rows_inv(const double2* __restr
     }
     if (k == 0) {
       x[0] = make_double2(A.x, B.x);
-      x[8] = make_double2(A.y, B.y);
+      x[E / 2] = make_double2(A.y, B.y);
     } else {
       x[q] = make_double2(A.x - B.y, A.y + B.x);
-      lds[base + pad(L - k)] = make_double2(A.x + B.y, -A.y + B.x);
+      lds[base + padE<E>(L - k)] = make_double2(A.x + B.y, -A.y + B.x);
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 8; q < 16; ++q) {
+  for (int q = E / 2; q < E; ++q) {
     const int i = t + q * NT;
-    if (i != L / 2) x[q] = lds[base + pad(i)];
+    if (i != L / 2) x[q] = lds[base + padE<E>(i)];
   }
   __syncthreads();
-  fft_regs<L, false>(x, lds, base, 1, t, tw);
+  fft_regs<L, E, false>(x, lds, base, 1, t, w);
   if (!valid) return;
   double* o0 = out + (size_t)y0 * out_pitch;
   double* o1 = o0 + out_pitch;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
+  for (int q = 0; q < E; ++q) {
     const int i = t + q * NT;
     if (i < ncols) {
       o0[i] = x[q].x;
@@ -391,26 +548,28 @@ __device__ __forceinline__ void col_group_of_block(int b, int nblocks, int* cg, 
   *half = lin % H;
 }
 
-template <int LC, int MODE>
-__global__ __launch_bounds__(CW * LC / 16) void k_cols(double2* __restrict__ spec, ColParams P,
-                                                       const double2* __restrict__ tw) {
-  constexpr int NT = LC / 16;
+template <int LC, int MODE, int E>
+__global__ __launch_bounds__(CW * LC / E, CUP2D_FFT_MINB) void k_cols(double2* __restrict__ spec, ColParams P,
+                                                      const double2* __restrict__ tw) {
+  constexpr int NT = LC / E;
   extern __shared__ double2 lds[];
   const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
   int cg, half;
   col_group_of_block(blockIdx.x, gridDim.x, &cg, &half);
   const int c = half * CW + cl;  // column inside the group
   double2* col = spec + (size_t)cg * P.rows * G;
-  double2 x[16];
+  double2 w[TwCount<LC, E, E>::n];
+  tw_load<LC, E, E>(tw, t, w);
+  double2 x[E];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
+  for (int q = 0; q < E; ++q) {
     const int i = t + q * NT;
     x[q] = (MODE != 1 || i < P.rows) ? col[(size_t)i * G + c] : make_double2(0, 0);
   }
-  fft_regs<LC, true>(x, lds, cl, CW, t, tw);
+  fft_regs<LC, E, true>(x, lds, cl, CW, t, w);
   if constexpr (MODE == 2) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) col[(size_t)(t + q * NT) * G + c] = x[q];
+    for (int q = 0; q < E; ++q) col[(size_t)(t + q * NT) * G + c] = x[q];
     return;
   } else {
     const int kx = cg * G + c;
@@ -419,17 +578,17 @@ __global__ __launch_bounds__(CW * LC / 16) void k_cols(double2* __restrict__ spe
       // c = dc + i nq; with real, even symbols s0, sN:
       // C'(k) = C(k) (s0+sN)/2 + conj(C(-k)) (s0-sN)/2
 #pragma unroll
-      for (int q = 0; q < 16; ++q) lds[pad(t + q * NT) * CW + cl] = x[q];
+      for (int q = 0; q < E; ++q) lds[padE<E>(t + q * NT) * CW + cl] = x[q];
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
+      for (int q = 0; q < E; ++q) {
         const int i = t + q * NT;
         const double s0 = col_symbol<LC, MODE>(kx, cg, c, i, P);
         if (kx == 0) {
           double sN;
           if constexpr (MODE == 0) sN = periodic_factor(P.NX / 2, i, P);
           else sN = P.ghatN[i <= P.NY ? i : LC - i];
-          const double2 m = lds[pad((LC - i) % LC) * CW + cl];
+          const double2 m = lds[padE<E>((LC - i) % LC) * CW + cl];
           const double a = 0.5 * (s0 + sN), b = 0.5 * (s0 - sN);
           x[q] = make_double2(x[q].x * a + m.x * b, x[q].y * a - m.y * b);
         } else {
@@ -439,14 +598,14 @@ __global__ __launch_bounds__(CW * LC / 16) void k_cols(double2* __restrict__ spe
       __syncthreads();
     } else {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const double s = col_symbol<LC, MODE>(kx, cg, c, t + q * NT, P);
-        x[q] = make_double2(x[q].x * s, x[q].y * s);
+      for (int q = 0; q < E; ++q) {
+        const double sy = col_symbol<LC, MODE>(kx, cg, c, t + q * NT, P);
+        x[q] = make_double2(x[q].x * sy, x[q].y * sy);
       }
     }
-    fft_regs<LC, false>(x, lds, cl, CW, t, tw);
+    fft_regs<LC, E, false>(x, lds, cl, CW, t, w);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
+    for (int q = 0; q < E; ++q) {
       const int i = t + q * NT;
       if (MODE != 1 || i < P.rows) col[(size_t)i * G + c] = x[q];
     }
@@ -475,36 +634,37 @@ __global__ void k_ghat_extract(const double2* __restrict__ spec, int Lc, int ncg
 
 // ----------------------------------------------------------- dispatch
 template <int L>
-static int rows_np() { return (L / 16) >= 256 ? 1 : 256 / (L / 16); }
-
-typedef void (*launch_fn)(cup2d_ctx*, const void*, void*, int, int, int, int, int);
-
-template <int L>
 static void launch_rows_fwd(cup2d_ctx* c, const double* in, int pitch, int ncols, int nrows,
-                            double2* spec, int spec_rows, const double2* tw) {
-  constexpr int NT = L / 16;
-  constexpr int NP = NT >= 256 ? 1 : 256 / NT;
-  const size_t shm = (size_t)NP * (L + L / 16) * sizeof(double2);
-  const int pairs = nrows / 2;
-  hipLaunchKernelGGL((k_rows_fwd<L, NP>), dim3((pairs + NP - 1) / NP), dim3(NP * NT), shm,
-                     c->stream, in, pitch, ncols, nrows, spec, spec_rows, tw);
+                            double2* spec, int spec_rows, const double2* tw, int src,
+                            const RowSource& fd) {
+  constexpr int NP = rows_np<L, FE>();
+  const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(double2);
+  const dim3 grid((nrows / 2 + NP - 1) / NP), block(NP * L / FE);
+  if (src == SRC_DIV)
+    hipLaunchKernelGGL((k_rows_fwd<L, NP, FE, SRC_DIV>), grid, block, shm, c->stream, in, pitch,
+                       ncols, nrows, spec, spec_rows, tw, fd);
+  else if (src == SRC_FADE)
+    hipLaunchKernelGGL((k_rows_fwd<L, NP, FE, SRC_FADE>), grid, block, shm, c->stream, in, pitch,
+                       ncols, nrows, spec, spec_rows, tw, fd);
+  else
+    hipLaunchKernelGGL((k_rows_fwd<L, NP, FE, SRC_PLAIN>), grid, block, shm, c->stream, in, pitch,
+                       ncols, nrows, spec, spec_rows, tw, fd);
 }
 template <int L>
 static void launch_rows_inv(cup2d_ctx* c, const double2* spec, int spec_rows, double* out,
                             int pitch, int ncols, int nrows, const double2* tw) {
-  constexpr int NT = L / 16;
-  constexpr int NP = NT >= 256 ? 1 : 256 / NT;
-  const size_t shm = (size_t)NP * (L + L / 16) * sizeof(double2);
+  constexpr int NP = rows_np<L, FE>();
+  const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(double2);
   const int pairs = nrows / 2;
-  hipLaunchKernelGGL((k_rows_inv<L, NP>), dim3((pairs + NP - 1) / NP), dim3(NP * NT), shm,
-                     c->stream, spec, spec_rows, out, pitch, ncols, nrows, tw);
+  hipLaunchKernelGGL((k_rows_inv<L, NP, FE>), dim3((pairs + NP - 1) / NP), dim3(NP * L / FE),
+                     shm, c->stream, spec, spec_rows, out, pitch, ncols, nrows, tw);
 }
 template <int LC, int MODE>
 static void launch_cols(cup2d_ctx* c, double2* spec, int ncg, const ColParams& P,
                         const double2* tw) {
-  const size_t shm = (size_t)(LC + LC / 16) * CW * sizeof(double2);
-  hipLaunchKernelGGL((k_cols<LC, MODE>), dim3(ncg * (G / CW)), dim3(CW * LC / 16), shm, c->stream,
-                     spec, P, tw);
+  const size_t shm = (size_t)lds_len<LC, FE>() * CW * sizeof(double2);
+  hipLaunchKernelGGL((k_cols<LC, MODE, FE>), dim3(ncg * (G / CW)), dim3(CW * LC / FE), shm,
+                     c->stream, spec, P, tw);
 }
 
 #define CUP2D_POW2_CASES(M, ...)                                                 \
@@ -522,8 +682,9 @@ static void launch_cols(cup2d_ctx* c, double2* spec, int ncg, const ColParams& P
   }
 
 static int rows_fwd(cup2d_ctx* c, int L, const double* in, int pitch, int ncols, int nrows,
-                    double2* spec, int spec_rows, const double2* tw) {
-  CUP2D_POW2_CASES(launch_rows_fwd, c, in, pitch, ncols, nrows, spec, spec_rows, tw);
+                    double2* spec, int spec_rows, const double2* tw, int src,
+                    const RowSource& fd) {
+  CUP2D_POW2_CASES(launch_rows_fwd, c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd);
   return CUP2D_OK;
 }
 static int rows_inv(cup2d_ctx* c, int L, const double2* spec, int spec_rows, double* out,
@@ -547,26 +708,25 @@ static int cols(cup2d_ctx* c, int LC, double2* spec, int ncg, const ColParams& P
   return CUP2D_OK;
 }
 
-template <int L, int NP>
-static void set_rows_attr() {
-  const int shm = NP * (L + L / 16) * (int)sizeof(double2);
-  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-}
 template <int L>
 static void set_rows_attr_L() {
-  constexpr int NT = L / 16;
-  constexpr int NP = NT >= 256 ? 1 : 256 / NT;
-  set_rows_attr<L, NP>();
+  constexpr int NP = rows_np<L, FE>();
+  const int shm = NP * lds_len<L, FE>() * (int)sizeof(double2);
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_PLAIN>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_FADE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_DIV>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, FE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
 }
 template <int LC>
 static void set_cols_attr() {
-  const int shm = (LC + LC / 16) * CW * (int)sizeof(double2);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  const int shm = lds_len<LC, FE>() * CW * (int)sizeof(double2);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 2, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
 }
 
 static std::vector<double2> twiddles(int L) {
@@ -631,7 +791,7 @@ int fft_create(cup2d_ctx* c) {
   CUP2D_HIP_CHECK(c, hipMemcpy(dG, Gk.data(), sizeof(double) * Gk.size(), hipMemcpyHostToDevice));
   CUP2D_HIP_CHECK(c, hipMalloc(&P->ghat, sizeof(double) * (size_t)P->ncg * G * (NY + 1)));
   CUP2D_HIP_CHECK(c, hipMalloc(&P->ghatN, sizeof(double) * (size_t)(NY + 1)));
-  int rc = rows_fwd(c, P->Lr, dG, MX, MX, MY, tspec, MY, P->tw_r);
+  int rc = rows_fwd(c, P->Lr, dG, MX, MX, MY, tspec, MY, P->tw_r, SRC_PLAIN, RowSource{});
   if (rc) return rc;
   ColParams cp{};
   cp.NX = NX; cp.NY = NY; cp.rows = MY;
@@ -660,13 +820,18 @@ void fft_destroy(cup2d_ctx* c) {
   c->fft = nullptr;
 }
 
-int enqueue_poisson_fft(cup2d_ctx* c, const double* rhs, double* sol) {
+// src: SRC_PLAIN (rhs as given), SRC_FADE (rhs is the grid RHS: apply
+// cub2rhs's boundary fade after enqueue_fade_sums and write the faded cells
+// back), SRC_DIV (form the RHS from the advDiff output first, see k_rows_fwd).
+int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   FFTPlan* P = c->fft;
   const int NX = c->g.NX, NY = c->g.NY;
   int rc;
   {
     TimerScope ts(c, T_FFT_ROWS_FWD);
-    rc = rows_fwd(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r);
+    RowSource fd{c->g, c->red, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
+                 c->u, c->v, c->obmap, c->fluxpart, stencil_edge_count(c)};
+    rc = rows_fwd(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r, src, fd);
   }
   if (rc) return rc;
   ColParams cp{};

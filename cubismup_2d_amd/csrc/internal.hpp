@@ -38,10 +38,6 @@ struct StepState {
   double uinf[2];
   int step;
   int error;     // sticky: CUP2D_ERR_BODY_OUT / CUP2D_ERR_CAPACITY
-  int nob;       // obstacle slots (shape, block) this step
-  int nub;       // distinct obstacle blocks this step
-  int nprev;     // blocks touched by the previous step (cleared next step)
-  int pad_;
   double maxU;   // last findMaxU result
   double fade_corr;  // cub2rhs correction factor of the last solve
   double flux_corr;  // advDiff boundary-flux correction of the last step
@@ -72,7 +68,7 @@ struct Obstacles {
   int* ub_start;      // [cap]
   int* ub_count;      // [cap]
   int* ub_slots;      // [cap] slots grouped per distinct block, shape order
-  int* prev_blocks;   // [cap] blocks to clear next step
+  int* cnt;           // [2]: slots and distinct blocks in these lists
   int* slot_of_shape_start;  // [MAX_SHAPES+1] slot range per shape
   double* chi;        // [cap][BS*BS]
   double* dist;
@@ -124,11 +120,15 @@ struct cup2d_ctx {
   double* tv = nullptr;
   double* pres = nullptr;
   double* rhs = nullptr;   // sim.tmp during PressureSingle (the Poisson RHS)
-  double* sdf = nullptr;   // sim.tmp during PutObjectsOnGrid (signed distance)
   double* chi = nullptr;
   double* udx = nullptr;   // sim.uDef
   double* udy = nullptr;
   double* invrho = nullptr;
+  double* vfu = nullptr;   // sim.vFluid (iterative penalization)
+  double* vfv = nullptr;
+  // PressureIterator_unif state (PressureIterator_unif.h:17-18)
+  int iter_old_nsteps = 10000;
+  double iter_target = 1e-4;
   double* staging = nullptr;  // host-layout staging (2 comps)
   double* red = nullptr;      // reduction partials
   size_t red_n = 0;
@@ -144,7 +144,10 @@ struct cup2d_ctx {
   cup2d_shape_state* ss = nullptr;         // device [MAX_SHAPES]
   double* ramp = nullptr;                  // device [100] CFL ramp factors
   std::vector<cup2d::ShapeParams> hsp;     // host copy of the parameters
-  cup2d::Obstacles ob{};
+  // Obstacle lists and slot data of this step (ob) and of the previous one
+  // (ob_prev), swapped at every PutObjectsOnGrid: ComputeForces of a step
+  // reads its lists on the side stream while the next step builds new ones.
+  cup2d::Obstacles ob{}, ob_prev{};
   int band_nblocks = 0;
   int* band_blocks = nullptr;              // blocks touched by the cub2rhs fade
   cup2d::FFTPlan* fft = nullptr;
@@ -186,7 +189,10 @@ int enqueue_find_max_u_partials(cup2d_ctx* c);
 int enqueue_dt(cup2d_ctx* c);
 int enqueue_advdiff(cup2d_ctx* c, int defer_fix);
 int enqueue_pressure_rhs(cup2d_ctx* c);
-int enqueue_fade(cup2d_ctx* c);
+int stencil_edge_count(const cup2d_ctx* c);
+int enqueue_fade_sums(cup2d_ctx* c, int compute = 0);
+constexpr int FADE_WG = 64;                  // workgroups (and partial triples) of the fade sums
+int fade_partials(const cup2d_ctx* c);
 int enqueue_pressure_correction(cup2d_ctx* c, int want_max);
 int enqueue_set_dt(cup2d_ctx* c, double dt);
 int enqueue_ic(cup2d_ctx* c);
@@ -198,7 +204,10 @@ int enqueue_dense_to_blocks(cup2d_ctx* c, const double* a, const double* b, doub
 // fft.hip
 int fft_create(cup2d_ctx* c);
 void fft_destroy(cup2d_ctx* c);
-int enqueue_poisson_fft(cup2d_ctx* c, const double* rhs, double* sol);
+// src: 0 = rhs as given, 1 = grid RHS with the cub2rhs fade, 2 = RHS formed
+// from the advDiff output (deferred flux correction, obstacle blocks from the
+// RHS plane) then faded.
+int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src);
 
 // obstacles.hip
 int obstacles_alloc(cup2d_ctx* c);
@@ -206,9 +215,18 @@ void obstacles_free(cup2d_ctx* c);
 constexpr int OBST_LIST_CAP = 2048;  // slots handled by the list builder
 int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap = 0);
 int join_obstacles(cup2d_ctx* c);
-int enqueue_rhs_obstacles(cup2d_ctx* c);
+// compute: form the RHS of the obstacle blocks from the advDiff output first
+// (the whole-step path, where the row FFT forms it for every other cell).
+int enqueue_rhs_obstacles(cup2d_ctx* c, int compute = 0);
 int enqueue_body_update(cup2d_ctx* c);
 int enqueue_penalize(cup2d_ctx* c, double* maxpart);
 int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux = 0);
 int enqueue_shapes_reset(cup2d_ctx* c);
+int enqueue_obst_reset(cup2d_ctx* c);
+// PressureIterator_unif pieces (PressureIterator_unif.cpp)
+int enqueue_copy_vel_to_tmpv(cup2d_ctx* c);
+int enqueue_iter_project(cup2d_ctx* c);
+int enqueue_iter_body(cup2d_ctx* c);
+int enqueue_iter_penalize(cup2d_ctx* c, double* reldf);
+int enqueue_iter_rhs(cup2d_ctx* c);
 }  // namespace cup2d

@@ -19,6 +19,12 @@ namespace cup2d {
 
 constexpr double EPSD = 2.220446049250313e-16;  // std::numeric_limits<double>::epsilon()
 constexpr int NT = 256;
+constexpr int LDS_RED = (NT / 64) * 20;  // block_sum_multi of up to 20 values
+
+// the BS*BS cells of a block, NT per iteration, with all iterations unrolled
+#define CUP2D_CELL_PRAGMA _Pragma("unroll")
+#define CELL_LOOP(i) \
+  CUP2D_CELL_PRAGMA for (int i##_it = 0, i = threadIdx.x; i##_it < BS * BS / NT; ++i##_it, i += NT)
 
 // partial-sum offsets inside a slot's NPART doubles
 enum { P_COM = 0, P_MOM = 3, P_RHS = 10, P_PEN = 13, P_FRC = 20, P_NSURF = 38, P_NPART_USED = 39 };
@@ -98,6 +104,7 @@ __device__ int block_exscan(int* a, int n, int* tmp) {
 __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __restrict__ sp,
                                                    cup2d_shape_state* __restrict__ ss,
                                                    StepState* __restrict__ st, Obstacles ob,
+                                                   Obstacles pv,
                                                    unsigned char* __restrict__ obmap,
                                                    const double* __restrict__ maxpart,
                                                    int nmaxpart,
@@ -108,15 +115,12 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
   __shared__ int s_cstart[MAX_SHAPES + 1];
   __shared__ int s_rng[MAX_SHAPES][3];
   __shared__ double s_box[MAX_SHAPES][4];
-  __shared__ double lds[8];
+  __shared__ double lds[LDS_RED];
   const int t = threadIdx.x;
   const int ns = g.nshapes;
-  const int nub_prev = st->nub;
+  const int nub_prev = pv.cnt[1];
   if (compute_dt) {
-    double m = 0;
-    const int npart = nmaxpart + nub_prev;
-    for (int i = t; i < npart; i += NT) m = dmax(m, maxpart[i]);
-    const double maxU = block_max<NT>(m, lds);
+    const double maxU = block_max<NT>(max_array<NT>(maxpart, nmaxpart + nub_prev), lds);
     if (t == 0) {
       const double h = g.h;
       const double dtFourier = h * h / g.nu, dtCFL = maxU < 2.2e-16 ? 1 : h / maxU;
@@ -130,12 +134,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       st->maxU = maxU;
     }
   }
-  for (int u = t; u < nub_prev; u += NT) {
-    const int b = ob.ub_block[u];
-    obmap[b] = 0;
-    ob.prev_blocks[u] = b;  // cleared by k_obst_create unless reused
-  }
-  if (t == 0) st->nprev = nub_prev;
+  for (int u = t; u < nub_prev; u += NT) obmap[pv.ub_block[u]] = 0;
   __syncthreads();
   if (t == 0) {
     const double dt = st->dt;
@@ -263,8 +262,8 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       ob.ub_count[s_a[s]] = cnt;
     }
   if (t == 0) {
-    st->nob = nob;
-    st->nub = nub;
+    ob.cnt[0] = nob;
+    ob.cnt[1] = nub;
   }
 }
 
@@ -275,89 +274,105 @@ __device__ inline double shape_sum(const Obstacles& ob, int k, int q, double* ld
   return block_sum_array<NT>(ob.part + (size_t)s0 * SLOT_PART + q, n, SLOT_PART, lds);
 }
 
-// PutObjectsOnGrid.cpp:191-197 (chi=0, sdf=-1, uDef=0, invRho=1) for one block
-__device__ inline void clear_block(const Grid& g, int b, double* sdf, double* chi, double* udx,
-                                   double* udy, double* invrho) {
+// totals of NQ consecutive partials q0.. over the slots of shape k, in one pass
+template <int NQ>
+__device__ inline void shape_sums(const Obstacles& ob, int k, int q0, double* out, double* lds) {
+  const int s0 = ob.slot_of_shape_start[k], n = ob.slot_of_shape_start[k + 1] - s0;
+  block_sum_table<NT, NQ>(ob.part + (size_t)s0 * SLOT_PART + q0, n, SLOT_PART, out, lds);
+}
+
+// PutObjectsOnGrid.cpp:191-197 (chi=0, uDef=0, invRho=1) for one block
+__device__ inline void clear_block(const Grid& g, int b, double* chi, double* udx, double* udy,
+                                   double* invrho) {
   const int bx = b % g.bpdx, by = b / g.bpdx;
-  for (int k = threadIdx.x; k < BS * BS; k += NT) {
+  CELL_LOOP(k) {
     const size_t d = (size_t)(by * BS + k / BS) * g.NX + bx * BS + k % BS;
-    chi[d] = 0; sdf[d] = -1; udx[d] = 0; udy[d] = 0; invrho[d] = 1;
+    chi[d] = 0; udx[d] = 0; udy[d] = 0; invrho[d] = 1;
   }
 }
 
-// Workgroups [0, nub): one per obstacle block of this step: clear the block,
-// then Disk::create / FillBlocks_Cylinder::operator() (ShapeLibrary.cpp:21-34)
-// for its shapes in order: dist = r - |x - c|, sdf = max(sdf, dist),
-// rho = rhoS.  Workgroups [nub, nub+nprev): clear the blocks of the previous
-// step that are not obstacle blocks now (every other cell already holds the
-// cleared values of PutObjectsOnGrid.cpp:191-197).
-__global__ __launch_bounds__(NT) void k_obst_create(Grid g, const ShapeParams* __restrict__ sp,
-                                                    const cup2d_shape_state* __restrict__ ss,
-                                                    const StepState* __restrict__ st,
-                                                    Obstacles ob,
-                                                    const unsigned char* __restrict__ obmap,
-                                                    double* __restrict__ sdf, double* chi,
-                                                    double* udx, double* udy, double* invrho) {
-  const int nub = st->nub, w = blockIdx.x;
+// Disk::create's distance R - |x - c| (ShapeLibrary.cpp:21-34) of shape k at
+// cell (gx, gy).
+__device__ inline double disk_dist(const Grid& g, const ShapeParams& P,
+                                   const cup2d_shape_state& S, int gx, int gy) {
+  const double p0 = cell_pos(gx / BS, gx % BS, g.hb, g.h) - S.center[0];
+  const double p1 = cell_pos(gy / BS, gy % BS, g.hb, g.h) - S.center[1];
+  return P.radius - sqrt(p0 * p0 + p1 * p1);
+}
+
+// The grid SDF the reference assembles in sim.tmp (PutObjectsOnGrid.cpp:
+// 190-200 clears it to -1, Disk::create takes the max with each shape's
+// distance on that shape's obstacle blocks) at cell (gx, gy), clamped into
+// the domain as the BlockLab does: max over the shapes whose bounding box
+// touches the cell's block of their distance, -1 if none.
+__device__ inline double grid_sdf(const Grid& g, const ShapeParams* __restrict__ sp,
+                                  const cup2d_shape_state* __restrict__ ss, int gx, int gy) {
+  gx = gx < 0 ? 0 : (gx >= g.NX ? g.NX - 1 : gx);
+  gy = gy < 0 ? 0 : (gy >= g.NY ? g.NY - 1 : gy);
+  double v = -1;
+  for (int k = 0; k < g.nshapes; ++k) {
+    const double safety = 2 * g.h, R = sp[k].radius;
+    const double c0 = ss[k].center[0], c1 = ss[k].center[1];
+    const double bbox[2][2] = {{c0 - R - safety, c0 + R + safety}, {c1 - R - safety, c1 + R + safety}};
+    if (touching(g, gx / BS, gy / BS, bbox)) v = dmax(v, disk_dist(g, sp[k], ss[k], gx, gy));
+  }
+  return v;
+}
+
+// PutObjectsOnGrid::operator() for the obstacle blocks (PutObjectsOnGrid.cpp:
+// 186-241): workgroups [0, nub), one per obstacle block of this step, clear
+// the block (PutObjectsOnGrid.cpp:191-197), then for its shapes in order:
+// Disk::create (ShapeLibrary.cpp:21-34: dist = R - |x - c|, rho = rhoS) and
+// putChiOnGrid (PutObjectsOnGrid.cpp:17-116): Towers chi from the shape's own
+// distance inside the block and the grid SDF across block edges, surface
+// delta + normals (ObstacleBlock::write, ObstacleBlock.h:336-346),
+// centre-of-mass partials, grid CHI = max and the invRho blend
+// (PutObjectsOnGrid.cpp:69-73).  Distances are evaluated where they are
+// needed, so no grid-wide SDF pass is required.  The next workgroups, one per
+// obstacle block of the previous step (list pv), clear those that are not
+// obstacle blocks now (every other cell already holds the cleared values).
+__global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __restrict__ sp,
+                                                   const cup2d_shape_state* __restrict__ ss,
+                                                   Obstacles ob, Obstacles pv,
+                                                   const unsigned char* __restrict__ obmap,
+                                                   double* __restrict__ chi,
+                                                   double* __restrict__ udx,
+                                                   double* __restrict__ udy,
+                                                   double* __restrict__ invrho) {
+  __shared__ double lds[LDS_RED];
+  const int nub = ob.cnt[1], w = blockIdx.x;
   if (w >= nub) {
-    if (w >= nub + st->nprev) return;
-    const int b = ob.prev_blocks[w - nub];
-    if (!obmap[b]) clear_block(g, b, sdf, chi, udx, udy, invrho);
+    if (w >= nub + pv.cnt[1]) return;
+    const int b = pv.ub_block[w - nub];
+    if (!obmap[b]) clear_block(g, b, chi, udx, udy, invrho);
     return;
   }
   const int b = ob.ub_block[w];
   const int bx = b % g.bpdx, by = b / g.bpdx;
-  clear_block(g, b, sdf, chi, udx, udy, invrho);
+  clear_block(g, b, chi, udx, udy, invrho);
   const int s0 = ob.ub_start[w], cnt = ob.ub_count[w];
-  for (int q = 0; q < cnt; ++q) {
-    const int s = ob.ub_slots[s0 + q];
-    const int k = ob.slot_shape[s];
-    SlotView o = slot(ob, s);
-    const double R = sp[k].radius, rhoS = sp[k].rhoS;
-    const double c0 = ss[k].center[0], c1 = ss[k].center[1];
-    for (int i = threadIdx.x; i < BS * BS; i += NT) {
-      const int ix = i % BS, iy = i / BS;
-      const double p0 = cell_pos(bx, ix, g.hb, g.h) - c0, p1 = cell_pos(by, iy, g.hb, g.h) - c1;
-      const double d = R - sqrt(p0 * p0 + p1 * p1);
-      o.dist[i] = d;
-      o.rho[i] = rhoS;
-      o.chi[i] = 0; o.udx[i] = 0; o.udy[i] = 0;
-      o.sd[i] = 0; o.sx[i] = 0; o.sy[i] = 0;
-      double* t = sdf + (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-      *t = dmax(*t, d);
-    }
-  }
-}
-
-// PutObjectsOnGrid::putChiOnGrid (PutObjectsOnGrid.cpp:17-116) per obstacle
-// block, shapes in order: Towers chi from the shape's own distance inside the
-// block and the grid SDF across block edges, surface delta + normals
-// (ObstacleBlock::write, ObstacleBlock.h:336-346), centre-of-mass partials,
-// then grid CHI = max and the invRho blend (PutObjectsOnGrid.cpp:69-73).
-__global__ __launch_bounds__(NT) void k_obst_chi(Grid g, const cup2d_shape_state* __restrict__ ss,
-                                                 const StepState* __restrict__ st, Obstacles ob,
-                                                 const double* __restrict__ sdf,
-                                                 double* __restrict__ chi,
-                                                 double* __restrict__ invrho) {
-  __shared__ double lds[8];
-  if ((int)blockIdx.x >= st->nub) return;
-  const int b = ob.ub_block[blockIdx.x];
-  const int bx = b % g.bpdx, by = b / g.bpdx;
-  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
   const double h = g.h, i2h = 0.5 / h, fac = 0.5 * h;
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
     SlotView o = slot(ob, s);
-    const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
+    const ShapeParams& P = sp[k];
+    const cup2d_shape_state& S = ss[k];
+    const double rhoS = P.rhoS;
+    const double C0 = S.centerOfMass[0], C1 = S.centerOfMass[1];
     double _x = 0, _y = 0, _m = 0, _ns = 0;
-    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    CELL_LOOP(i) {
       const int ix = i % BS, iy = i / BS;
       const int gx = bx * BS + ix, gy = by * BS + iy;
-      const double sd = o.dist[i];
-      const double distPx = ix + 1 == BS ? lab_at(sdf, g, gx + 1, gy) : o.dist[i + 1];
-      const double distMx = ix == 0 ? lab_at(sdf, g, gx - 1, gy) : o.dist[i - 1];
-      const double distPy = iy + 1 == BS ? lab_at(sdf, g, gx, gy + 1) : o.dist[i + BS];
-      const double distMy = iy == 0 ? lab_at(sdf, g, gx, gy - 1) : o.dist[i - BS];
+      const double sd = disk_dist(g, P, S, gx, gy);
+      // neighbours: the shape's own distance inside the block, the grid SDF
+      // across its edges (PutObjectsOnGrid.cpp:53-56)
+      const double distPx = ix + 1 == BS ? grid_sdf(g, sp, ss, gx + 1, gy) : disk_dist(g, P, S, gx + 1, gy);
+      const double distMx = ix == 0 ? grid_sdf(g, sp, ss, gx - 1, gy) : disk_dist(g, P, S, gx - 1, gy);
+      const double distPy = iy + 1 == BS ? grid_sdf(g, sp, ss, gx, gy + 1) : disk_dist(g, P, S, gx, gy + 1);
+      const double distMy = iy == 0 ? grid_sdf(g, sp, ss, gx, gy - 1) : disk_dist(g, P, S, gx, gy - 1);
+      o.dist[i] = sd;
+      o.rho[i] = rhoS;
+      o.udx[i] = 0; o.udy[i] = 0;
       double X;
       if (sd > +2 * h || sd < -2 * h) {
         X = sd > 0 ? 1 : 0;
@@ -374,49 +389,50 @@ __global__ __launch_bounds__(NT) void k_obst_chi(Grid g, const cup2d_shape_state
       const size_t d = (size_t)gy * g.NX + gx;
       if (X >= chi[d]) {
         chi[d] = X;
-        invrho[d] = X / o.rho[i] + (1 - X) * invrho[d];
+        invrho[d] = X / rhoS + (1 - X) * invrho[d];
       }
       if (X > 0) {
         const double p0 = cell_pos(bx, ix, g.hb, h), p1 = cell_pos(by, iy, g.hb, h);
-        const double rho = o.rho[i];
-        _x += rho * X * h * h * (p0 - C0);
-        _y += rho * X * h * h * (p1 - C1);
-        _m += rho * X * h * h;
+        _x += rhoS * X * h * h * (p0 - C0);
+        _y += rhoS * X * h * h * (p1 - C1);
+        _m += rhoS * X * h * h;
       }
-      if (sd > +2 * h || sd < -2 * h) continue;
-      const double HplusX = fabs(distPx) < EPSD ? 0.5 : (distPx < 0 ? 0 : 1);
-      const double HminuX = fabs(distMx) < EPSD ? 0.5 : (distMx < 0 ? 0 : 1);
-      const double HplusY = fabs(distPy) < EPSD ? 0.5 : (distPy < 0 ? 0 : 1);
-      const double HminuY = fabs(distMy) < EPSD ? 0.5 : (distMy < 0 ? 0 : 1);
-      const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
-      const double gradHX = (HplusX - HminuX), gradHY = (HplusY - HminuY);
-      const double gradUSq = gradUX * gradUX + gradUY * gradUY;
-      const double denum = 1 / (gradUSq < EPSD ? EPSD : gradUSq);
-      const double D = fac * (gradHX * gradUX + gradHY * gradUY) * denum;
-      if (D > 0) {
-        o.sd[i] = D;
-        o.sx[i] = -D * gradUX;
-        o.sy[i] = -D * gradUY;
-        _ns += 1;
+      double D = 0, nx = 0, ny = 0;
+      if (!(sd > +2 * h || sd < -2 * h)) {
+        const double HplusX = fabs(distPx) < EPSD ? 0.5 : (distPx < 0 ? 0 : 1);
+        const double HminuX = fabs(distMx) < EPSD ? 0.5 : (distMx < 0 ? 0 : 1);
+        const double HplusY = fabs(distPy) < EPSD ? 0.5 : (distPy < 0 ? 0 : 1);
+        const double HminuY = fabs(distMy) < EPSD ? 0.5 : (distMy < 0 ? 0 : 1);
+        const double gradUX = i2h * (distPx - distMx), gradUY = i2h * (distPy - distMy);
+        const double gradHX = (HplusX - HminuX), gradHY = (HplusY - HminuY);
+        const double gradUSq = gradUX * gradUX + gradUY * gradUY;
+        const double denum = 1 / (gradUSq < EPSD ? EPSD : gradUSq);
+        const double Dc = fac * (gradHX * gradUX + gradHY * gradUY) * denum;
+        if (Dc > 0) {
+          D = Dc;
+          nx = -Dc * gradUX;
+          ny = -Dc * gradUY;
+          _ns += 1;
+        }
       }
+      o.sd[i] = D;
+      o.sx[i] = nx;
+      o.sy[i] = ny;
     }
-    _x = block_sum<NT>(_x, lds);
-    _y = block_sum<NT>(_y, lds);
-    _m = block_sum<NT>(_m, lds);
-    _ns = block_sum<NT>(_ns, lds);
+    double a4[4] = {_x, _y, _m, _ns};
+    block_sum_multi<NT, 4>(a4, lds);
     if (threadIdx.x == 0) {
-      o.part[P_COM] = _x; o.part[P_COM + 1] = _y; o.part[P_COM + 2] = _m;
-      o.part[P_NSURF] = _ns;
+      o.part[P_COM] = a4[0]; o.part[P_COM + 1] = a4[1]; o.part[P_COM + 2] = a4[2];
+      o.part[P_NSURF] = a4[3];
     }
   }
 }
 
-// centre of mass after putChiOnGrid (PutObjectsOnGrid.cpp:118-122), computed
-// from the partials without touching the stored state
-__device__ inline void shape_com(const Obstacles& ob, const cup2d_shape_state* ss, int k,
-                                 double* lds, double C[2], double* mass, bool* shifted) {
-  const double _x = shape_sum(ob, k, P_COM, lds), _y = shape_sum(ob, k, P_COM + 1, lds);
-  const double _m = shape_sum(ob, k, P_COM + 2, lds);
+// centre of mass after putChiOnGrid (PutObjectsOnGrid.cpp:118-122) from the
+// totals t[0..3) of the COM partials, without touching the stored state
+__device__ inline void com_from_sums(const cup2d_shape_state* ss, int k, const double* t,
+                                     double C[2], double* mass, bool* shifted) {
+  const double _x = t[0], _y = t[1], _m = t[2];
   C[0] = ss[k].centerOfMass[0];
   C[1] = ss[k].centerOfMass[1];
   *shifted = _m > EPSD;
@@ -427,14 +443,21 @@ __device__ inline void shape_com(const Obstacles& ob, const cup2d_shape_state* s
   }
 }
 
+__device__ inline void shape_com(const Obstacles& ob, const cup2d_shape_state* ss, int k,
+                                 double* lds, double C[2], double* mass, bool* shifted) {
+  double t[3];
+  shape_sums<3>(ob, k, P_COM, t, lds);
+  com_from_sums(ss, k, t, C, mass, shifted);
+}
+
 // Shape::integrateObstBlock (Shape.cpp:117-155): per-slot partials about the
 // updated centre of mass.
 __global__ __launch_bounds__(NT) void k_obst_moments(Grid g,
                                                      const cup2d_shape_state* __restrict__ ss,
                                                      const StepState* __restrict__ st,
                                                      Obstacles ob) {
-  __shared__ double lds[8];
-  if ((int)blockIdx.x >= st->nob) return;
+  __shared__ double lds[LDS_RED];
+  if ((int)blockIdx.x >= ob.cnt[0]) return;
   const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   SlotView o = slot(ob, s);
@@ -443,7 +466,7 @@ __global__ __launch_bounds__(NT) void k_obst_moments(Grid g,
   bool shifted;
   shape_com(ob, ss, k, lds, C, &mass, &shifted);
   double a[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+  CELL_LOOP(i) {
     if (o.chi[i] <= 0) continue;
     const int ix = i % BS, iy = i / BS;
     const double rhochi = o.chi[i] * o.rho[i] * hsq;
@@ -456,11 +479,10 @@ __global__ __launch_bounds__(NT) void k_obst_moments(Grid g,
     a[5] += rhochi * o.udy[i];
     a[6] += rhochi * (p0 * o.udy[i] - p1 * o.udx[i]);
   }
+  block_sum_multi<NT, 7>(a, lds);
+  if (threadIdx.x == 0)
 #pragma unroll
-  for (int j = 0; j < 7; ++j) {
-    const double t = block_sum<NT>(a[j], lds);
-    if (threadIdx.x == 0) o.part[P_MOM + j] = t;
-  }
+    for (int j = 0; j < 7; ++j) o.part[P_MOM + j] = a[j];
 }
 
 // Shape::removeMoments (Shape.cpp:157-212) + putObjectVelOnGrid
@@ -470,22 +492,24 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, const cup2d_shape_stat
                                                   const double* __restrict__ chi,
                                                   double* __restrict__ udx,
                                                   double* __restrict__ udy) {
-  __shared__ double lds[8];
-  const int nub = st->nub;
+  __shared__ double lds[LDS_RED];
+  const int nub = ob.cnt[1];
   if ((int)blockIdx.x < nub) {
     const int b = ob.ub_block[blockIdx.x];
     const int bx = b % g.bpdx, by = b / g.bpdx;
     const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
     for (int q = 0; q < cnt; ++q) {
       const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
-      double I[7];
-      for (int j = 0; j < 7; ++j) I[j] = shape_sum(ob, k, P_MOM + j, lds);
+      static_assert(P_COM == 0 && P_MOM == 3, "COM and moment partials are adjacent");
+      double t[10];
+      shape_sums<10>(ob, k, P_COM, t, lds);
+      const double* I = t + P_MOM;
       const double Iu = I[4] / I[2], Iv = I[5] / I[2], Ia = I[6] / I[3];
       double C[2], mass;
       bool shifted;
-      shape_com(ob, ss, k, lds, C, &mass, &shifted);
+      com_from_sums(ss, k, t, C, &mass, &shifted);
       SlotView o = slot(ob, s);
-      for (int i = threadIdx.x; i < BS * BS; i += NT) {
+      CELL_LOOP(i) {
         const int ix = i % BS, iy = i / BS;
         const double p0 = cell_pos(bx, ix, g.hb, g.h) - C[0], p1 = cell_pos(by, iy, g.hb, g.h) - C[1];
         o.udx[i] -= Iu - Ia * p1;
@@ -504,12 +528,14 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, const cup2d_shape_stat
 // previous values
 __global__ __launch_bounds__(NT) void k_obst_commit(Grid g, cup2d_shape_state* __restrict__ ss,
                                                     Obstacles ob) {
-  __shared__ double lds[8];
+  __shared__ double lds[LDS_RED];
   for (int k = 0; k < g.nshapes; ++k) {
     double C[2], mass;
     bool shifted;
-    shape_com(ob, ss, k, lds, C, &mass, &shifted);
-    const double m = shape_sum(ob, k, P_MOM + 2, lds), j = shape_sum(ob, k, P_MOM + 3, lds);
+    double t[10];
+    shape_sums<10>(ob, k, P_COM, t, lds);
+    com_from_sums(ss, k, t, C, &mass, &shifted);
+    const double m = t[P_MOM + 2], j = t[P_MOM + 3];
     const double nsurf = shape_sum(ob, k, P_NSURF, lds);
     if (threadIdx.x == 0) {
       cup2d_shape_state& T = ss[k];
@@ -533,10 +559,12 @@ static int grid_cap(cup2d_ctx* c) { return c->ob.cap > 0 ? c->ob.cap : 1; }
 // caller joins with join_obstacles() before the next reader of obstacle data.
 int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   const int G = grid_cap(c);
+  std::swap(c->ob, c->ob_prev);
   {
     TimerScope ts(c, T_PUT_OBJECTS);
     hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                       c->ob, c->obmap, c->maxpart, c->nwg, c->ramp, compute_dt, c->host_step);
+                       c->ob, c->ob_prev, c->obmap, c->maxpart, c->nwg, c->ramp, compute_dt,
+                       c->host_step);
   }
   hipStream_t s = c->stream;
   if (overlap) {
@@ -544,14 +572,18 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_begin, 0));
     s = c->aux;
   }
-  hipLaunchKernelGGL(k_obst_create, dim3(2 * G), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
-                     c->obmap, c->sdf, c->chi, c->udx, c->udy, c->invrho);
   if (c->g.nshapes > 0) {
-    hipLaunchKernelGGL(k_obst_chi, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob, c->sdf,
-                       c->chi, c->invrho);
+    hipLaunchKernelGGL(k_obst_place, dim3(2 * G), dim3(NT), 0, s, c->g, c->sp, c->ss, c->ob,
+                       c->ob_prev, c->obmap, c->chi, c->udx, c->udy, c->invrho);
     hipLaunchKernelGGL(k_obst_moments, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob);
-    hipLaunchKernelGGL(k_obst_udef, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob, c->chi,
-                       c->udx, c->udy);
+    // Disks carry no deformation velocity (Disk::create leaves udef = 0), so
+    // removeMoments and putObjectVelOnGrid would only add zeros: skipped
+    // unless a shape kind with a deformation velocity is present.
+    bool deforming = false;
+    for (const ShapeParams& p : c->hsp) deforming |= p.kind != 0;
+    if (deforming)
+      hipLaunchKernelGGL(k_obst_udef, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob, c->chi,
+                         c->udx, c->udy);
     hipLaunchKernelGGL(k_obst_commit, dim3(1), dim3(NT), 0, s, c->g, c->ss, c->ob);
   }
   if (overlap) {
@@ -571,19 +603,34 @@ int join_obstacles(cup2d_ctx* c) {
 // ------------------------------------------------ RHS obstacle correction
 // PressureSingle.cpp:130-168 (source term + perimeter sums) and :173-212
 // (sign-selective rescale), per distinct block, shapes in order.
+// With u != nullptr the workgroup first forms the RHS of its block from the
+// advDiff output (rhs_cell: PressureSingle.cpp:112-128 with the deferred
+// boundary-flux correction), the cells the whole-step row FFT takes from the
+// RHS plane.
 __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __restrict__ st,
                                                   Obstacles ob, const double* __restrict__ chi,
-                                                  double* __restrict__ r) {
-  __shared__ double lds[8];
-  if ((int)blockIdx.x >= st->nub) return;
+                                                  double* __restrict__ r,
+                                                  const double* __restrict__ u,
+                                                  const double* __restrict__ v,
+                                                  const double* __restrict__ fluxpart, int ne) {
+  __shared__ double lds[LDS_RED];
+  if ((int)blockIdx.x >= ob.cnt[1]) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
+  if (u) {
+    const double corr = flux_corr256(fluxpart, ne, g, lds);
+    const double facDiv = 0.5 * g.h / st->dt;
+    CELL_LOOP(i) {
+      const int x = bx * BS + i % BS, y = by * BS + i / BS;
+      r[(size_t)y * g.NX + x] = rhs_cell(u, v, g.NX, g.NY, x, y, corr, facDiv);
+    }
+  }
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q];
     SlotView o = slot(ob, s);
     double sum = 0, pos = 0, neg = 0;
-    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    CELL_LOOP(i) {
       const int ix = i % BS, iy = i / BS, gx = bx * BS + ix, gy = by * BS + iy;
       const size_t d = (size_t)gy * g.NX + gx;
       if (chi[d] > o.chi[i]) continue;
@@ -597,11 +644,10 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
       pos += isPerim * T * (double)(T > 0);
       neg -= isPerim * T * (double)(T < 0);
     }
-    sum = block_sum<NT>(sum, lds);
-    pos = block_sum<NT>(pos, lds);
-    neg = block_sum<NT>(neg, lds);
+    double a3[3] = {sum, pos, neg};
+    block_sum_multi<NT, 3>(a3, lds);
     if (threadIdx.x == 0) {
-      o.part[P_RHS] = sum; o.part[P_RHS + 1] = pos; o.part[P_RHS + 2] = neg;
+      o.part[P_RHS] = a3[0]; o.part[P_RHS + 1] = a3[1]; o.part[P_RHS + 2] = a3[2];
     }
   }
 }
@@ -609,19 +655,20 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
 __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __restrict__ st,
                                                   Obstacles ob, const double* __restrict__ chi,
                                                   double* __restrict__ r) {
-  __shared__ double lds[8];
-  if ((int)blockIdx.x >= st->nub) return;
+  __shared__ double lds[LDS_RED];
+  if ((int)blockIdx.x >= ob.cnt[1]) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
-    const double sum = shape_sum(ob, k, P_RHS, lds), pos = shape_sum(ob, k, P_RHS + 1, lds);
-    const double neg = shape_sum(ob, k, P_RHS + 2, lds);
+    double t[3];
+    shape_sums<3>(ob, k, P_RHS, t, lds);
+    const double sum = t[0], pos = t[1], neg = t[2];
     const double corrDenom = sum > 0 ? pos : neg;
     const double corr = sum / dmax(corrDenom, EPSD);
     SlotView o = slot(ob, s);
-    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    CELL_LOOP(i) {
       const int ix = i % BS, iy = i / BS, gx = bx * BS + ix, gy = by * BS + iy;
       const size_t d = (size_t)gy * g.NX + gx;
       if (chi[d] > o.chi[i]) continue;
@@ -635,12 +682,12 @@ __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __res
   }
 }
 
-int enqueue_rhs_obstacles(cup2d_ctx* c) {
+int enqueue_rhs_obstacles(cup2d_ctx* c, int compute) {
   if (c->g.nshapes == 0) return CUP2D_OK;
   const int G = grid_cap(c);
   TimerScope ts(c, T_RHS_OBST);
   hipLaunchKernelGGL(k_rhs_obst1, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
-                     c->rhs);
+                     c->rhs, compute ? c->u : nullptr, c->v, c->fluxpart, stencil_edge_count(c));
   hipLaunchKernelGGL(k_rhs_obst2, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
                      c->rhs);
   return launch_check(c, "rhs_obstacles");
@@ -686,14 +733,14 @@ __device__ void lu3_solve(double A[3][3], const double b[3], double x[3]) {
 __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
                               cup2d_shape_state* __restrict__ ss,
                               StepState* __restrict__ st, const Obstacles& ob,
-                              ForceSnap* __restrict__ snap) {
-  __shared__ double lds[8];
+                              ForceSnap* __restrict__ snap, int collide) {
+  __shared__ double lds[LDS_RED];
   __shared__ double coll[MAX_SHAPES][10];
   const double dt = st->dt, time = st->time;
   const int ns = g.nshapes;
   for (int k = 0; k < ns; ++k) {
     double S[7];
-    for (int j = 0; j < 7; ++j) S[j] = shape_sum(ob, k, P_PEN + j, lds);
+    shape_sums<7>(ob, k, P_PEN, S, lds);
     if (threadIdx.x == 0) {
       cup2d_shape_state& T = ss[k];
       const ShapeParams& P = sp[k];
@@ -718,7 +765,7 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
       }
     }
   }
-  if (ns < 2) return;
+  if (ns < 2 || !collide) return;
   __syncthreads();
   // collision sums: coll[i] accumulates over every j != i (PressureSingle.cpp:255-298)
   for (int i = 0; i < ns; ++i) {
@@ -731,7 +778,7 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
           if (ob.slot_block[a] != ob.slot_block[bsl]) continue;
           const int blk = ob.slot_block[a], bx = blk % g.bpdx, by = blk / g.bpdx;
           SlotView oi = slot(ob, a), oj = slot(ob, bsl);
-          for (int c = threadIdx.x; c < BS * BS; c += NT) {
+          CELL_LOOP(c) {
             if (oi.chi[c] <= 0 || oj.chi[c] <= 0) continue;
             const double p0 = cell_pos(bx, c % BS, g.hb, g.h), p1 = cell_pos(by, c / BS, g.hb, g.h);
             const double iUr = -si.omega * (p1 - si.centerOfMass[1]), iVr = si.omega * (p0 - si.centerOfMass[0]);
@@ -746,10 +793,9 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
           }
         }
     }
-    for (int q = 0; q < 10; ++q) {
-      const double t = block_sum<NT>(acc[q], lds);
-      if (threadIdx.x == 0) coll[i][q] = t;
-    }
+    block_sum_multi<NT, 10>(acc, lds);
+    if (threadIdx.x == 0)
+      for (int q = 0; q < 10; ++q) coll[i][q] = acc[q];
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -793,8 +839,8 @@ __global__ __launch_bounds__(NT) void k_momenta(Grid g, const ShapeParams* __res
                                                 StepState* __restrict__ st, Obstacles ob,
                                                 const double* __restrict__ u,
                                                 const double* __restrict__ v) {
-  __shared__ double lds[8];
-  const int nob = st->nob;
+  __shared__ double lds[LDS_RED];
+  const int nob = ob.cnt[0];
   if ((int)blockIdx.x >= nob) return;
   const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
   const int bx = b % g.bpdx, by = b / g.bpdx;
@@ -803,7 +849,7 @@ __global__ __launch_bounds__(NT) void k_momenta(Grid g, const ShapeParams* __res
   const double lambdt = g.lambda * st->dt;
   const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
   double a[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+  CELL_LOOP(i) {
     if (o.chi[i] <= 0) continue;
     const int ix = i % BS, iy = i / BS;
     const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
@@ -824,18 +870,18 @@ __global__ __launch_bounds__(NT) void k_momenta(Grid g, const ShapeParams* __res
     a[5] += F * ud1;
     a[6] += F * (p0 * ud1 - p1 * ud0);
   }
+  block_sum_multi<NT, 7>(a, lds);
+  if (threadIdx.x == 0)
 #pragma unroll
-  for (int j = 0; j < 7; ++j) {
-    const double t = block_sum<NT>(a[j], lds);
-    if (threadIdx.x == 0) o.part[P_PEN + j] = t;
-  }
+    for (int j = 0; j < 7; ++j) o.part[P_PEN + j] = a[j];
 }
 
+// collide: also preventCollidingObstacles (PressureSingle only).
 __global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __restrict__ sp,
                                                    cup2d_shape_state* __restrict__ ss,
                                                    StepState* __restrict__ st, Obstacles ob,
-                                                   ForceSnap* __restrict__ snap) {
-  body_solve_wg(g, sp, ss, st, ob, snap);
+                                                   ForceSnap* __restrict__ snap, int collide) {
+  body_solve_wg(g, sp, ss, st, ob, snap, collide);
   __syncthreads();
   // state ComputeForces uses (it may run after the next step moved the bodies)
   if (threadIdx.x < g.nshapes) {
@@ -855,7 +901,7 @@ int enqueue_body_update(cup2d_ctx* c) {
   hipLaunchKernelGGL(k_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st, c->ob,
                      c->u, c->v);
   hipLaunchKernelGGL(k_body_solve, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                     c->ob, c->fsnap);
+                     c->ob, c->fsnap, 1);
   return launch_check(c, "body_update");
 }
 
@@ -866,8 +912,8 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const cup2d_shape_state
                                                  const double* __restrict__ chi,
                                                  double* __restrict__ u, double* __restrict__ v,
                                                  double* __restrict__ maxpart) {
-  __shared__ double lds[8];
-  if ((int)blockIdx.x >= st->nub) return;
+  __shared__ double lds[LDS_RED];
+  if ((int)blockIdx.x >= ob.cnt[1]) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
@@ -877,7 +923,7 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const cup2d_shape_state
     SlotView o = slot(ob, s);
     const double u_s = ss[k].u, v_s = ss[k].v, omega_s = ss[k].omega;
     const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
-    for (int i = threadIdx.x; i < BS * BS; i += NT) {
+    CELL_LOOP(i) {
       const int ix = i % BS, iy = i / BS;
       const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
       const double X = o.chi[i];
@@ -896,7 +942,7 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const cup2d_shape_state
   // projection kernel skips obstacle blocks.
   const double UINF = st->uinf[0], VINF = st->uinf[1];
   double m = 0;
-  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+  CELL_LOOP(i) {
     const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
     const double a = u[d], c = v[d];
     m = dmax(m, fabs(a + UINF));
@@ -916,16 +962,135 @@ int enqueue_penalize(cup2d_ctx* c, double* maxpart) {
   return launch_check(c, "penalize");
 }
 
+// ------------------------------------------------- PressureIterator_unif
+// integrateMomenta (PressureIterator_unif.cpp:77-126) with EXPL_INTEGRATE_MOM:
+// F = h^2 chi on the velocity vFluid; penalDX = penalDY = 0.
+__global__ __launch_bounds__(NT) void k_iter_momenta(Grid g, const cup2d_shape_state* __restrict__ ss,
+                                                     Obstacles ob,
+                                                     const double* __restrict__ fu,
+                                                     const double* __restrict__ fv) {
+  __shared__ double lds[LDS_RED];
+  if ((int)blockIdx.x >= ob.cnt[0]) return;
+  const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  SlotView o = slot(ob, s);
+  const double hsq = g.h * g.h;
+  const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  CELL_LOOP(i) {
+    if (o.chi[i] <= 0) continue;
+    const int ix = i % BS, iy = i / BS;
+    const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+    const double F = hsq * o.chi[i];
+    const double ud0 = fu[d], ud1 = fv[d];
+    const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
+    a[0] += F;
+    a[1] += F * (p0 * p0 + p1 * p1);
+    a[4] += F * ud0;
+    a[5] += F * ud1;
+    a[6] += F * (p0 * ud1 - p1 * ud0);
+  }
+  block_sum_multi<NT, 7>(a, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < 7; ++j) o.part[P_PEN + j] = a[j];
+}
+
+int enqueue_iter_body(cup2d_ctx* c) {
+  if (c->g.nshapes == 0) return CUP2D_OK;
+  const int G = grid_cap(c);
+  TimerScope ts(c, T_BODY);
+  hipLaunchKernelGGL(k_iter_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->ob, c->vfu,
+                     c->vfv);
+  hipLaunchKernelGGL(k_body_solve, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
+                     c->ob, c->fsnap, 0);
+  return launch_check(c, "iterator body update");
+}
+
+// penalize (PressureIterator_unif.cpp:128-184), per distinct block, shapes in
+// order: vel = tmpV + chi (U_s - vFluid) written as vel += DP; per-block sums
+// of vel^2 and DP^2 (part[4 ub + 0..3]: MX, MY, DMX, DMY).
+__global__ __launch_bounds__(NT) void k_iter_penalize(Grid g, const cup2d_shape_state* __restrict__ ss,
+                                                      Obstacles ob, const double* __restrict__ chi,
+                                                      const double* __restrict__ tu,
+                                                      const double* __restrict__ tv,
+                                                      const double* __restrict__ fu,
+                                                      const double* __restrict__ fv,
+                                                      double* __restrict__ u,
+                                                      double* __restrict__ v,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[LDS_RED];
+  if ((int)blockIdx.x >= ob.cnt[1]) return;
+  const int b = ob.ub_block[blockIdx.x];
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
+  double a[4] = {0, 0, 0, 0};
+  for (int q = 0; q < cnt; ++q) {
+    const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
+    SlotView o = slot(ob, s);
+    const double u_s = ss[k].u, v_s = ss[k].v, omega_s = ss[k].omega;
+    const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
+    CELL_LOOP(i) {
+      const int ix = i % BS, iy = i / BS;
+      const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+      const double X = o.chi[i];
+      if (chi[d] > X) continue;
+      if (X <= 0) continue;
+      const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
+      const double US = u_s - omega_s * p1 + o.udx[i];
+      const double VS = v_s + omega_s * p0 + o.udy[i];
+      const double penalFac = X;
+      const double DFX = penalFac * (US - fu[d]);
+      const double DFY = penalFac * (VS - fv[d]);
+      const double DPX = tu[d] + DFX - u[d];
+      const double DPY = tv[d] + DFY - v[d];
+      const double u1 = u[d] + DPX, v1 = v[d] + DPY;
+      u[d] = u1;
+      v[d] = v1;
+      a[0] += u1 * u1;
+      a[1] += v1 * v1;
+      a[2] += DPX * DPX;
+      a[3] += DPY * DPY;
+    }
+  }
+  block_sum_multi<NT, 4>(a, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) part[4 * blockIdx.x + j] = a[j];
+}
+
+// relDF = sqrt((DMX + DMY) / (eps + MX + MY)) (PressureIterator_unif.cpp:183)
+__global__ __launch_bounds__(NT) void k_iter_reldf(Obstacles ob, const double* __restrict__ part,
+                                                   double* __restrict__ out) {
+  __shared__ double lds[LDS_RED];
+  double t[4];
+  block_sum_table<NT, 4>(part, ob.cnt[1], 4, t, lds);
+  if (threadIdx.x == 0) out[0] = sqrt((t[2] + t[3]) / (EPSD + t[0] + t[1]));
+}
+
+int enqueue_iter_penalize(cup2d_ctx* c, double* reldf) {
+  double* part = c->red + 4096;
+  if (c->g.nshapes == 0) {  // no body cells: relDF = 0
+    hipLaunchKernelGGL(k_iter_reldf, dim3(1), dim3(NT), 0, c->stream, c->ob, part, reldf);
+    return launch_check(c, "iterator penalize");
+  }
+  TimerScope ts(c, T_PENALIZE);
+  hipLaunchKernelGGL(k_iter_penalize, dim3(grid_cap(c)), dim3(NT), 0, c->stream, c->g, c->ss,
+                     c->ob, c->chi, c->tu, c->tv, c->vfu, c->vfv, c->u, c->v, part);
+  hipLaunchKernelGGL(k_iter_reldf, dim3(1), dim3(NT), 0, c->stream, c->ob, part, reldf);
+  return launch_check(c, "iterator penalize");
+}
+
 // ----------------------------------------------------------- ComputeForces
 // Shape::computeForces (Shape.cpp:252-281): per-block partials added in
 // block order; optionally ends the step (time += dt, step++; Simulation.cpp:289-290).
 __device__ void forces_fin_wg(const Grid& g, cup2d_shape_state* __restrict__ ss,
                               const Obstacles& ob, StepState* __restrict__ st, int end_step,
                               const ForceSnap* __restrict__ snap) {
-  __shared__ double lds[8];
+  __shared__ double lds[LDS_RED];
   for (int k = 0; k < g.nshapes; ++k) {
     double a[18];
-    for (int j = 0; j < 18; ++j) a[j] = shape_sum(ob, k, P_FRC + j, lds);
+    shape_sums<18>(ob, k, P_FRC, a, lds);
     if (threadIdx.x != 0) continue;
     cup2d_shape_state& T = ss[k];
     T.perimeter = a[0]; T.circulation = a[1];
@@ -956,8 +1121,8 @@ __global__ __launch_bounds__(NT) void k_forces(Grid g, const ForceSnap* __restri
                                                const double* __restrict__ u,
                                                const double* __restrict__ v,
                                                const double* __restrict__ p) {
-  __shared__ double lds[8];
-  const int nob = st->nob;
+  __shared__ double lds[LDS_RED];
+  const int nob = ob.cnt[0];
   if ((int)blockIdx.x >= nob) return;
   const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
   const int bx = b % g.bpdx, by = b / g.bpdx;
@@ -970,7 +1135,7 @@ __global__ __launch_bounds__(NT) void k_forces(Grid g, const ForceSnap* __restri
   double a[18];
 #pragma unroll
   for (int j = 0; j < 18; ++j) a[j] = 0;
-  for (int i = threadIdx.x; i < BS * BS; i += NT) {
+  CELL_LOOP(i) {
     if (!(o.sd[i] > 0)) continue;
     const int ix = i % BS, iy = i / BS, gx = bx * BS + ix, gy = by * BS + iy;
     const double p0 = cell_pos(bx, ix, g.hb, g.h), p1 = cell_pos(by, iy, g.hb, g.h);
@@ -1005,11 +1170,10 @@ __global__ __launch_bounds__(NT) void k_forces(Grid g, const ForceSnap* __restri
     a[16] += dmin(0.0, powDef);
     a[17] += 1;
   }
+  block_sum_multi<NT, 18>(a, lds);
+  if (threadIdx.x == 0)
 #pragma unroll
-  for (int j = 0; j < 18; ++j) {
-    const double t = block_sum<NT>(a[j], lds);
-    if (threadIdx.x == 0) o.part[P_FRC + j] = t;
-  }
+    for (int j = 0; j < 18; ++j) o.part[P_FRC + j] = a[j];
 }
 
 __global__ __launch_bounds__(NT) void k_forces_fin(Grid g, cup2d_shape_state* __restrict__ ss,
@@ -1024,7 +1188,9 @@ int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux) {
     return CUP2D_OK;
   }
   hipStream_t s = c->stream;
-  if (on_aux) {  // ComputeForces overlaps the next step (it reads only the snapshot)
+  if (on_aux) {
+    // ComputeForces overlaps the next step: it reads the body snapshot and
+    // this step's obstacle lists, which the next step does not overwrite
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_pen, c->stream));
     CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_pen, 0));
     s = c->aux;
@@ -1059,8 +1225,37 @@ int enqueue_shapes_reset(cup2d_ctx* c) {
 }
 
 // ------------------------------------------------------------ allocation
+static int obstacles_alloc_one(cup2d_ctx* c, Obstacles& o, int cap) {
+  o.cap = cap;
+  const size_t cells = (size_t)cap * BS * BS;
+  const size_t ni = (size_t)6 * cap + 2 + MAX_SHAPES + 1;
+  int* ibuf = nullptr;
+  CUP2D_HIP_CHECK(c, hipMalloc(&ibuf, sizeof(int) * ni));
+  CUP2D_HIP_CHECK(c, hipMemset(ibuf, 0, sizeof(int) * ni));
+  o.slot_shape = ibuf;
+  o.slot_block = ibuf + cap;
+  o.ub_block = ibuf + 2 * cap;
+  o.ub_start = ibuf + 3 * cap;
+  o.ub_count = ibuf + 4 * cap;
+  o.ub_slots = ibuf + 5 * cap;
+  o.cnt = ibuf + 6 * cap;
+  o.slot_of_shape_start = ibuf + 6 * cap + 2;
+  double* dbuf = nullptr;
+  const size_t nd = cells * 9 + (size_t)cap * SLOT_PART;
+  CUP2D_HIP_CHECK(c, hipMalloc(&dbuf, sizeof(double) * nd));
+  CUP2D_HIP_CHECK(c, hipMemset(dbuf, 0, sizeof(double) * nd));
+  o.chi = dbuf;
+  o.dist = dbuf + cells;
+  o.rho = dbuf + 2 * cells;
+  o.udef = dbuf + 3 * cells;
+  o.sdelta = dbuf + 5 * cells;
+  o.sdx = dbuf + 6 * cells;
+  o.sdy = dbuf + 7 * cells;
+  o.part = dbuf + 9 * cells;
+  return CUP2D_OK;
+}
+
 int obstacles_alloc(cup2d_ctx* c) {
-  Obstacles& o = c->ob;
   // capacity: blocks a disk's bounding box can touch, per shape
   int cap = 0;
   for (const ShapeParams& p : c->hsp) {
@@ -1077,38 +1272,27 @@ int obstacles_alloc(cup2d_ctx* c) {
     c->err = "too many obstacle blocks for the device list builder";
     return CUP2D_ERR_CAPACITY;
   }
-  o.cap = cap;
-  const size_t cells = (size_t)cap * BS * BS;
-  int* ibuf = nullptr;
-  CUP2D_HIP_CHECK(c, hipMalloc(&ibuf, sizeof(int) * ((size_t)7 * cap + MAX_SHAPES + 1)));
-  CUP2D_HIP_CHECK(c, hipMemset(ibuf, 0, sizeof(int) * ((size_t)7 * cap + MAX_SHAPES + 1)));
-  o.slot_shape = ibuf;
-  o.slot_block = ibuf + cap;
-  o.ub_block = ibuf + 2 * cap;
-  o.ub_start = ibuf + 3 * cap;
-  o.ub_count = ibuf + 4 * cap;
-  o.ub_slots = ibuf + 5 * cap;
-  o.prev_blocks = ibuf + 6 * cap;
-  o.slot_of_shape_start = ibuf + 7 * cap;
-  double* dbuf = nullptr;
-  const size_t nd = cells * 9 + (size_t)cap * SLOT_PART;
-  CUP2D_HIP_CHECK(c, hipMalloc(&dbuf, sizeof(double) * nd));
-  CUP2D_HIP_CHECK(c, hipMemset(dbuf, 0, sizeof(double) * nd));
-  o.chi = dbuf;
-  o.dist = dbuf + cells;
-  o.rho = dbuf + 2 * cells;
-  o.udef = dbuf + 3 * cells;
-  o.sdelta = dbuf + 5 * cells;
-  o.sdx = dbuf + 6 * cells;
-  o.sdy = dbuf + 7 * cells;
-  o.part = dbuf + 9 * cells;
-  return CUP2D_OK;
+  int rc = obstacles_alloc_one(c, c->ob, cap);
+  if (rc) return rc;
+  return obstacles_alloc_one(c, c->ob_prev, cap);
 }
 
 void obstacles_free(cup2d_ctx* c) {
-  if (c->ob.slot_shape) (void)hipFree(c->ob.slot_shape);
-  if (c->ob.chi) (void)hipFree(c->ob.chi);
-  c->ob = Obstacles{};
+  for (Obstacles* o : {&c->ob, &c->ob_prev}) {
+    if (o->slot_shape) (void)hipFree(o->slot_shape);
+    if (o->chi) (void)hipFree(o->chi);
+    *o = Obstacles{};
+  }
+}
+
+// Both list buffers empty (Simulation::reset).
+__global__ void k_obst_reset(int* cnt0, int* cnt1) {
+  if (threadIdx.x < 2) { cnt0[threadIdx.x] = 0; cnt1[threadIdx.x] = 0; }
+}
+int enqueue_obst_reset(cup2d_ctx* c) {
+  if (!c->ob.cnt) return CUP2D_OK;
+  hipLaunchKernelGGL(k_obst_reset, dim3(1), dim3(64), 0, c->stream, c->ob.cnt, c->ob_prev.cnt);
+  return launch_check(c, "obstacle reset");
 }
 
 }  // namespace cup2d

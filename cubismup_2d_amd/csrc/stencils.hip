@@ -82,9 +82,7 @@ __global__ __launch_bounds__(256) void k_dt(const double* __restrict__ part, int
                                             const double* __restrict__ ramp,
                                             StepState* __restrict__ st, int set_dt) {
   __shared__ double lds[8];
-  double m = 0;
-  for (int i = threadIdx.x; i < nparts; i += 256) m = dmax(m, part[i]);
-  const double maxU = block_max<256>(m, lds);
+  const double maxU = block_max<256>(max_array<256>(part, nparts), lds);
   if (threadIdx.x == 0) {
     st->maxU = maxU;
     if (set_dt) {
@@ -135,17 +133,19 @@ int enqueue_step_end(cup2d_ctx* c) {
   return launch_check(c, "step_end");
 }
 
-// corr = IF / (2 NY + 2 NX) (advDiff.cpp:105) from the per-workgroup partials
-// of the boundary workgroups, summed in a fixed order.
-__device__ inline double flux_corr(const double* __restrict__ fluxpart, int gx, int gy,
-                                   const Grid& g, double* lds) {
-  double s = 0;
-  for (int w = threadIdx.x; w < gx * gy; w += blockDim.x) {
-    const int bx = w % gx, by = w / gx;
-    if (bx == 0 || bx == gx - 1 || by == 0 || by == gy - 1) s += fluxpart[w];
-  }
-  const double IF = block_sum_dyn(s, lds);
-  return IF / (2 * (BS * g.bpdy - 0) + 2 * (BS * g.bpdx - 0));
+// Tile (bx, by) of this workgroup in the stencil grid.  Workgroup b of a
+// launch runs on XCD b % 8; numbering the tiles so that each XCD gets one
+// contiguous band of tiles keeps the halo rows and columns that neighbouring
+// tiles share inside one L2.
+#ifndef CUP2D_XCD_REMAP
+#define CUP2D_XCD_REMAP 1
+#endif
+__device__ inline void stencil_tile(int* bx, int* by) {
+  const int gx = gridDim.x, n = gx * gridDim.y;
+  int id = blockIdx.x + gx * blockIdx.y;
+  if (CUP2D_XCD_REMAP && n % 8 == 0) id = (id % 8) * (n / 8) + id / 8;
+  *bx = id % gx;
+  *by = id / gx;
 }
 
 // ------------------------------------------------------------- advDiff
@@ -160,8 +160,10 @@ __global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
                                                  double* __restrict__ fluxpart) {
   __shared__ double lds[8];
   const int NX = g.NX, NY = g.NY;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y0 = blockIdx.y * RY;
+  int bx, by;
+  stencil_tile(&bx, &by);
+  const int x = bx * blockDim.x + threadIdx.x;
+  const int y0 = by * RY;
   const double dt = st->dt, h = g.h;
   const double UINF0 = st->uinf[0], UINF1 = st->uinf[1];
   const double dfac = (g.nu / h) * (dt / h), afac = -0.5 * dt / h;
@@ -203,18 +205,17 @@ __global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
     us = uc; vs = vc;
     uc = un; vc = vn;
   }
-  const bool edge = blockIdx.x == 0 || blockIdx.x == gridDim.x - 1 || blockIdx.y == 0 ||
-                    blockIdx.y == gridDim.y - 1;
-  if (!edge) return;
+  const int e = edge_index(bx, by, gridDim.x, gridDim.y);
+  if (e < 0) return;
   flux = block_sum_dyn(flux, lds);
-  if (threadIdx.x == 0) fluxpart[blockIdx.y * gridDim.x + blockIdx.x] = flux;
+  if (threadIdx.x == 0) fluxpart[e] = flux;
 }
 
 // one workgroup: corr from the boundary workgroups' partials
 __global__ __launch_bounds__(256) void k_flux_reduce(const double* __restrict__ fluxpart, int gx,
                                                      int gy, Grid g, StepState* __restrict__ st) {
   __shared__ double lds[8];
-  const double corr = flux_corr(fluxpart, gx, gy, g, lds);
+  const double corr = flux_corr(fluxpart, edge_count(gx, gy), g, lds);
   if (threadIdx.x == 0) st->flux_corr = corr;
 }
 
@@ -243,16 +244,13 @@ int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
     hipLaunchKernelGGL(k_advdiff<RYS>, dim3(gx, gy), dim3(tx), 0, c->stream, c->u, c->v, c->tu,
                        c->tv, c->g, c->st, c->fluxpart);
   }
-  {
-    TimerScope ts(c, T_FLUXFIX);
-    hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, c->stream, c->fluxpart, gx, gy, c->g,
-                       c->st);
-  }
   std::swap(c->u, c->tu);  // advDiff.cpp:81-87 copy tmpV -> vel
   std::swap(c->v, c->tv);
   c->flux_deferred = defer_fix != 0;
   if (!defer_fix) {
     TimerScope ts(c, T_FLUXFIX);
+    hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, c->stream, c->fluxpart, gx, gy, c->g,
+                       c->st);
     const int n = 2 * c->g.NX + 2 * c->g.NY;
     hipLaunchKernelGGL(k_flux_fix, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->u, c->v,
                        c->g, c->st);
@@ -265,16 +263,27 @@ int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
 // gradient ghosts.  With a deferred advDiff flux correction (FIX), boundary
 // values are read as u_W + corr, u_E - corr, v_S + corr, v_N - corr, exactly
 // the values advDiff.cpp:107-115 would have stored.
+// The correction is computed from the advDiff edge partials by the edge
+// workgroups, the only ones with boundary cells; workgroup (0,0) stores it
+// for the projection.
 template <int RY, bool FIX>
 __global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
                                              const double* __restrict__ v,
                                              double* __restrict__ r, Grid g,
-                                             const StepState* __restrict__ st) {
+                                             StepState* __restrict__ st,
+                                             const double* __restrict__ fluxpart) {
   const int NX = g.NX, NY = g.NY;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y0 = blockIdx.y * RY;
+  int bx, by;
+  stencil_tile(&bx, &by);
+  const int x = bx * blockDim.x + threadIdx.x;
+  const int y0 = by * RY;
   const double facDiv = 0.5 * g.h / st->dt;
-  const double corr = FIX ? st->flux_corr : 0.0;
+  double corr = 0.0;
+  if (FIX && edge_index(bx, by, gridDim.x, gridDim.y) >= 0) {
+    __shared__ double lds[8];
+    corr = flux_corr(fluxpart, edge_count(gridDim.x, gridDim.y), g, lds);
+    if (bx == 0 && by == 0 && threadIdx.x == 0) st->flux_corr = corr;
+  }
   size_t idx = (size_t)y0 * NX + x;
   double vc = v[idx];
   if (FIX && y0 == 0) vc = vc + corr;
@@ -314,89 +323,85 @@ __global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
   }
 }
 
+// stencil grid of k_advdiff / k_rhs / k_project and its edge workgroups
+int stencil_edge_count(const cup2d_ctx* c) {
+  const int tx = strip_width(c->g.NX);
+  return edge_count(c->g.NX / tx, c->g.NY / RYS);
+}
+
 int enqueue_pressure_rhs(cup2d_ctx* c) {
   const int tx = strip_width(c->g.NX);
   TimerScope ts(c, T_RHS);
   const dim3 grid(c->g.NX / tx, c->g.NY / RYS);
   if (c->flux_deferred)
     hipLaunchKernelGGL((k_rhs<RYS, true>), grid, dim3(tx), 0, c->stream, c->u, c->v, c->rhs, c->g,
-                       c->st);
+                       c->st, c->fluxpart);
   else
     hipLaunchKernelGGL((k_rhs<RYS, false>), grid, dim3(tx), 0, c->stream, c->u, c->v, c->rhs,
-                       c->g, c->st);
+                       c->g, c->st, c->fluxpart);
   return launch_check(c, "pressure_rhs");
 }
 
+
 // ------------------------------------------------------- cub2rhs fade
-// PoissonSolver.cpp:61-122: fadeArg from the distance to the four faces,
-// sums of the faded RHS, correction preserving sum(r).
-__device__ inline double fade_arg(const Grid& g, double px, double py) {
-  const double invFadeX = 1 / dmax(g.fadeLenX, EPSD), invFadeY = 1 / dmax(g.fadeLenY, EPSD);
-  const double yt = invFadeY * dmax(0.0, g.fadeLenY - g.extent1 + py);
-  const double yb = invFadeY * dmax(0.0, g.fadeLenY - py);
-  const double xt = invFadeX * dmax(0.0, g.fadeLenX - g.extent0 + px);
-  const double xb = invFadeX * dmax(0.0, g.fadeLenX - px);
-  return dmin(dmax(dmax(dmax(yt, yb), xt), xb), 1.0);
-}
-
+// PoissonSolver.cpp:81-98: the sums of the faded RHS r*fa^2 over the band
+// blocks, of its positive parts and of its negative parts.  FADE_WG
+// workgroups; workgroup w adds band blocks w, w + FADE_WG, ... and writes
+// three partials.  The correction factor and the rewrite of the band cells
+// happen in the forward row FFT, whose workgroups each add these partials.
 __global__ __launch_bounds__(256) void k_fade_sums(const double* __restrict__ r,
-                                                   const int* __restrict__ blocks, Grid g,
-                                                   double* __restrict__ part) {
-  __shared__ double lds[8];
-  const int b = blocks[blockIdx.x];
-  const int bx = b % g.bpdx, by = b / g.bpdx;
+                                                   const int* __restrict__ blocks, int nblocks,
+                                                   Grid g, double* __restrict__ part,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ v,
+                                                   const unsigned char* __restrict__ obmap,
+                                                   const double* __restrict__ fluxpart, int ne,
+                                                   const StepState* __restrict__ st) {
+  __shared__ double lds[12];
+  // u != nullptr: the RHS of cells outside obstacle blocks is formed from the
+  // advDiff output (see k_rows_fwd), the obstacle blocks' from the RHS plane
+  double corr = 0, facDiv = 0;
+  if (u) {
+    corr = flux_corr256(fluxpart, ne, g, lds);
+    facDiv = 0.5 * g.h / st->dt;
+  }
   double sR = 0, sP = 0, sN = 0;
-  for (int k = threadIdx.x; k < BS * BS; k += 256) {
-    const int ix = k % BS, iy = k / BS;
-    const double px = cell_pos(bx, ix, g.hb, g.h), py = cell_pos(by, iy, g.hb, g.h);
-    const double fa = fade_arg(g, px, py);
-    const double f = r[(size_t)(by * BS + iy) * g.NX + bx * BS + ix] * (fa * fa);
-    if (f < 0) sN -= f;
-    if (f > 0) sP += f;
-    sR += f;
+  for (int j = blockIdx.x; j < nblocks; j += gridDim.x) {
+    const int b = blocks[j];
+    const int bx = b % g.bpdx, by = b / g.bpdx;
+    const bool from_plane = !u || obmap[b];
+#pragma unroll
+    for (int it = 0, k = threadIdx.x; it < BS * BS / 256; ++it, k += 256) {
+      const int ix = k % BS, iy = k / BS, x = bx * BS + ix, y = by * BS + iy;
+      const double px = cell_pos(bx, ix, g.hb, g.h), py = cell_pos(by, iy, g.hb, g.h);
+      const double fa = fade_arg(g, px, py);
+      const double rc = from_plane ? r[(size_t)y * g.NX + x]
+                                   : rhs_cell(u, v, g.NX, g.NY, x, y, corr, facDiv);
+      const double f = rc * (fa * fa);
+      if (f < 0) sN -= f;
+      if (f > 0) sP += f;
+      sR += f;
+    }
   }
-  sR = block_sum<256>(sR, lds);
-  sP = block_sum<256>(sP, lds);
-  sN = block_sum<256>(sN, lds);
+  double a[3] = {sR, sP, sN};
+  block_sum_multi<256, 3>(a, lds);
   if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = sR;
-    part[3 * blockIdx.x + 1] = sP;
-    part[3 * blockIdx.x + 2] = sN;
+    part[3 * blockIdx.x] = a[0];
+    part[3 * blockIdx.x + 1] = a[1];
+    part[3 * blockIdx.x + 2] = a[2];
   }
 }
 
-__global__ __launch_bounds__(256) void k_fade_apply(double* __restrict__ r,
-                                                    const int* __restrict__ blocks, int nblocks,
-                                                    Grid g, const double* __restrict__ part,
-                                                    StepState* __restrict__ st) {
-  __shared__ double lds[8];
-  double sumRHS = block_sum_array<256>(part, nblocks, 3, lds);
-  double sumPOS = block_sum_array<256>(part + 1, nblocks, 3, lds);
-  double sumNEG = block_sum_array<256>(part + 2, nblocks, 3, lds);
-  sumNEG = dmax(sumNEG, EPSD);
-  sumPOS = dmax(sumPOS, EPSD);
-  const double corr = sumRHS > 0 ? sumRHS / sumPOS : sumRHS / sumNEG;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->fade_corr = corr;
-  const int b = blocks[blockIdx.x];
-  const int bx = b % g.bpdx, by = b / g.bpdx;
-  for (int k = threadIdx.x; k < BS * BS; k += 256) {
-    const int ix = k % BS, iy = k / BS;
-    const double px = cell_pos(bx, ix, g.hb, g.h), py = cell_pos(by, iy, g.hb, g.h);
-    const double fa = fade_arg(g, px, py);
-    double* p = r + (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-    const double fadeRHS = *p * (fa * fa);
-    const double fadeCorrection = corr * (corr > 0 ? dmax(fadeRHS, 0.0) : -dmin(fadeRHS, 0.0));
-    *p = *p * (1 - fa * fa) + fadeCorrection;
-  }
+int fade_partials(const cup2d_ctx* c) {
+  return c->band_nblocks < FADE_WG ? c->band_nblocks : FADE_WG;
 }
 
-int enqueue_fade(cup2d_ctx* c) {
+int enqueue_fade_sums(cup2d_ctx* c, int compute) {
   if (c->band_nblocks == 0) return CUP2D_OK;
   TimerScope ts(c, T_FADE);
-  hipLaunchKernelGGL(k_fade_sums, dim3(c->band_nblocks), dim3(256), 0, c->stream, c->rhs,
-                     c->band_blocks, c->g, c->red);
-  hipLaunchKernelGGL(k_fade_apply, dim3(c->band_nblocks), dim3(256), 0, c->stream, c->rhs,
-                     c->band_blocks, c->band_nblocks, c->g, c->red, c->st);
+  hipLaunchKernelGGL(k_fade_sums, dim3(fade_partials(c)), dim3(256), 0, c->stream, c->rhs,
+                     c->band_blocks, c->band_nblocks, c->g, c->red, compute ? c->u : nullptr,
+                     c->v, c->obmap, c->fluxpart, stencil_edge_count(c), c->st);
   return launch_check(c, "cub2rhs fade");
 }
 
@@ -413,8 +418,10 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
                                                  double* __restrict__ maxpart) {
   __shared__ double lds[8];
   const int NX = g.NX, NY = g.NY;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y0 = blockIdx.y * RY;
+  int bx, by;
+  stencil_tile(&bx, &by);
+  const int x = bx * blockDim.x + threadIdx.x;
+  const int y0 = by * RY;
   const double pFac = -0.5 * st->dt / g.h;
   const double UINF = st->uinf[0], VINF = st->uinf[1];
   const double corr = fix ? st->flux_corr : 0.0;
@@ -450,7 +457,7 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
   }
   if (maxpart) {
     m = block_max_dyn(m, lds);
-    if (threadIdx.x == 0) maxpart[blockIdx.y * gridDim.x + blockIdx.x] = m;
+    if (threadIdx.x == 0) maxpart[by * gridDim.x + bx] = m;
   }
 }
 
@@ -464,27 +471,117 @@ int enqueue_pressure_correction(cup2d_ctx* c, int want_max) {
   return launch_check(c, "pressure_correction");
 }
 
+// ------------------------------------------------- PressureIterator_unif
+// operator() start (PressureIterator_unif.cpp:219-226): tmpV = vel.
+__global__ void k_copy2(const double* __restrict__ a, const double* __restrict__ b,
+                        double* __restrict__ ca, double* __restrict__ cb, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    ca[i] = a[i];
+    cb[i] = b[i];
+  }
+}
+int enqueue_copy_vel_to_tmpv(cup2d_ctx* c) {
+  hipLaunchKernelGGL(k_copy2, dim3(2048), dim3(256), 0, c->stream, c->u, c->v, c->tu, c->tv,
+                     (size_t)c->g.NX * c->g.NY);
+  return launch_check(c, "copy vel to tmpV");
+}
+
+// pressureCorrection (PressureIterator_unif.cpp:51-75): vFluid = tmpV + pFac grad p.
+template <int RY>
+__global__ __launch_bounds__(256) void k_iter_project(const double* __restrict__ tu,
+                                                      const double* __restrict__ tv,
+                                                      const double* __restrict__ p,
+                                                      double* __restrict__ fu,
+                                                      double* __restrict__ fv, Grid g,
+                                                      const StepState* __restrict__ st) {
+  const int NX = g.NX, NY = g.NY;
+  int bx, by;
+  stencil_tile(&bx, &by);
+  const int x = bx * blockDim.x + threadIdx.x;
+  const int y0 = by * RY;
+  const double pFac = -0.5 * st->dt / g.h;
+  size_t idx = (size_t)y0 * NX + x;
+  double pc = p[idx];
+  double ps = (y0 == 0) ? pc : p[idx - NX];
+  CUP2D_UNROLL_ROWS
+  for (int iy = 0; iy < RY; ++iy, idx += NX) {
+    const int y = y0 + iy;
+    const double pn = (y + 1 < NY) ? p[idx + NX] : pc;
+    const double pw = (x > 0) ? p[idx - 1] : pc;
+    const double pe = (x + 1 < NX) ? p[idx + 1] : pc;
+    fu[idx] = tu[idx] + pFac * (pe - pw);
+    fv[idx] = tv[idx] + pFac * (pn - ps);
+    ps = pc;
+    pc = pn;
+  }
+}
+int enqueue_iter_project(cup2d_ctx* c) {
+  const int tx = strip_width(c->g.NX);
+  TimerScope ts(c, T_PROJECT);
+  hipLaunchKernelGGL(k_iter_project<RYS>, dim3(c->g.NX / tx, c->g.NY / RYS), dim3(tx), 0,
+                     c->stream, c->tu, c->tv, c->pres, c->vfu, c->vfv, c->g, c->st);
+  return launch_check(c, "iterator projection");
+}
+
+// updatePressureRHS (PressureIterator_unif.cpp:22-49):
+// r = (h/2dt) (div vel - CHI div uDef), zero-gradient ghosts.
+template <int RY>
+__global__ __launch_bounds__(256) void k_iter_rhs(const double* __restrict__ u,
+                                                  const double* __restrict__ v,
+                                                  const double* __restrict__ udx,
+                                                  const double* __restrict__ udy,
+                                                  const double* __restrict__ chi,
+                                                  double* __restrict__ r, Grid g,
+                                                  const StepState* __restrict__ st) {
+  const int NX = g.NX, NY = g.NY;
+  int bx, by;
+  stencil_tile(&bx, &by);
+  const int x = bx * blockDim.x + threadIdx.x;
+  const int y0 = by * RY;
+  const double facDiv = 0.5 * g.h / st->dt;
+  CUP2D_UNROLL_ROWS
+  for (int iy = 0; iy < RY; ++iy) {
+    const int y = y0 + iy;
+    const size_t idx = (size_t)y * NX + x;
+    const size_t ie = x + 1 < NX ? idx + 1 : idx, iw = x > 0 ? idx - 1 : idx;
+    const size_t in = y + 1 < NY ? idx + NX : idx, is = y > 0 ? idx - NX : idx;
+    const double divVx = u[ie] - u[iw];
+    const double divVy = v[in] - v[is];
+    const double divUSx = udx[ie] - udx[iw];
+    const double divUSy = udy[in] - udy[is];
+    r[idx] = facDiv * (divVx + divVy - chi[idx] * (divUSx + divUSy));
+  }
+}
+int enqueue_iter_rhs(cup2d_ctx* c) {
+  const int tx = strip_width(c->g.NX);
+  TimerScope ts(c, T_RHS);
+  hipLaunchKernelGGL(k_iter_rhs<RYS>, dim3(c->g.NX / tx, c->g.NY / RYS), dim3(tx), 0, c->stream,
+                     c->u, c->v, c->udx, c->udy, c->chi, c->rhs, c->g, c->st);
+  return launch_check(c, "iterator rhs");
+}
+
 // ------------------------------------------------------ IC / ApplyObjVel
 // IC::operator() (Helpers.cpp:38-82) plus the PutObjectsOnGrid clear values
-// (chi=0, sdf=-1, udef=0, invRho=1; PutObjectsOnGrid.cpp:191-197).
+// (chi=0, udef=0, invRho=1; PutObjectsOnGrid.cpp:191-197).
 __global__ void k_ic(double* u, double* v, double* tu, double* tv, double* pres, double* rhs,
-                     double* sdf, double* chi, double* udx, double* udy, double* invrho,
-                     size_t n) {
+                     double* chi, double* udx, double* udy, double* invrho, double* vfu,
+                     double* vfv, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     u[i] = 0; v[i] = 0; tu[i] = 0; tv[i] = 0; pres[i] = 0; rhs[i] = 0;
-    sdf[i] = -1; chi[i] = 0; udx[i] = 0; udy[i] = 0; invrho[i] = 1;
+    chi[i] = 0; udx[i] = 0; udy[i] = 0; invrho[i] = 1; vfu[i] = 0; vfv[i] = 0;
   }
 }
 __global__ void k_ic_state(StepState* st) {
   st->time = 0; st->dt = 0; st->step = 0; st->uinf[0] = 0; st->uinf[1] = 0;
-  st->error = 0; st->nob = 0; st->nub = 0; st->nprev = 0;
+  st->error = 0;
   st->maxU = 0; st->fade_corr = 0; st->flux_corr = 0;
   st->dt_done = 0;
 }
 int enqueue_ic(cup2d_ctx* c) {
   hipLaunchKernelGGL(k_ic, dim3(2048), dim3(256), 0, c->stream, c->u, c->v, c->tu, c->tv,
-                     c->pres, c->rhs, c->sdf, c->chi, c->udx, c->udy, c->invrho,
+                     c->pres, c->rhs, c->chi, c->udx, c->udy, c->invrho, c->vfu, c->vfv,
                      (size_t)c->g.NX * c->g.NY);
   hipLaunchKernelGGL(k_ic_state, dim3(1), dim3(1), 0, c->stream, c->st);
   return launch_check(c, "ic");

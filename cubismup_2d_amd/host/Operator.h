@@ -105,6 +105,18 @@ class PressureSingle : public Operator {
   std::string getName() override { return "PressureSingle"; }
 };
 
+// PressureIterator_unif (PressureIterator_unif.cpp:216-289), selected by
+// -iterativePenalization 1 (Simulation.cpp:215-216).  The Poisson solver is
+// the context's FFT solver; iterations and relDF of the last call are kept.
+class PressureIterator_unif : public Operator {
+ public:
+  int iterations = 0;
+  double relDF = 0;
+  using Operator::Operator;
+  void operator()(const double dt) override;
+  std::string getName() override { return "PressureIterator_unif"; }
+};
+
 // ComputeForces (ComputeForces.cpp:16-111).
 class ComputeForces : public Operator {
  public:

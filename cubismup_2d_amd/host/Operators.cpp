@@ -72,6 +72,11 @@ void PressureSingle::operator()(const double dt) {
   penalize(dt);
 }
 
+void PressureIterator_unif::operator()(const double dt) {
+  check(sim.ctx, cup2d_pressure_iterator(sim.ctx, dt, &iterations, &relDF),
+        "PressureIterator_unif");
+}
+
 void ComputeForces::operator()(const double) {
   check(sim.ctx, cup2d_compute_forces(sim.ctx), "ComputeForces");
 }

@@ -133,8 +133,6 @@ void Simulation::parseRuntime() {
   sim.muteAll = parser.asInt("muteAll", 0);
   if (sim.muteAll) sim.verbose = 0;
   deviceLoop = parser.asBool("deviceLoop", false);
-  if (sim.iterativePenalization)
-    throw Error(CUP2D_ERR_ARG, "-iterativePenalization 1 has no device implementation");
   if (sim.dlm > 0) throw Error(CUP2D_ERR_ARG, "-dlm > 0 has no device implementation");
   sim.allocateGrid();
 }
@@ -165,7 +163,10 @@ void Simulation::init() {
   pipeline.clear();
   pipeline.push_back(std::make_unique<PutObjectsOnGrid>(sim));
   pipeline.push_back(std::make_unique<advDiff>(sim));
-  pipeline.push_back(std::make_unique<PressureSingle>(sim));
+  if (sim.iterativePenalization)
+    pipeline.push_back(std::make_unique<PressureIterator_unif>(sim));
+  else
+    pipeline.push_back(std::make_unique<PressureSingle>(sim));
   pipeline.push_back(std::make_unique<ComputeForces>(sim));
   if (sim.verbose) {
     std::printf("Operator ordering:\n");

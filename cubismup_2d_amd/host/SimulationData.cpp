@@ -46,6 +46,7 @@ void SimulationData::allocateGrid() {
   fadeLenY = fadeLen * extent;
   cfg.poisson = kind;
   cfg.implicit_penal = 0;
+  cfg.iterative_penal = iterativePenalization ? 1 : 0;
   cfg.stream = stream;
   check(nullptr, cup2d_create(&cfg, &ctx), "cup2d_create");
 }

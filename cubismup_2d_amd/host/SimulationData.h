@@ -67,7 +67,8 @@ class SimulationData {
   void* stream = nullptr;  // hipStream_t to enqueue on; nullptr: the library's own
 
   const Field vel{CUP2D_VEL, 2}, chi{CUP2D_CHI, 1}, pres{CUP2D_PRES, 1}, tmp{CUP2D_TMP, 1},
-      tmpV{CUP2D_TMPV, 2}, uDef{CUP2D_UDEF, 2}, invRho{CUP2D_INVRHO, 1};
+      tmpV{CUP2D_TMPV, 2}, uDef{CUP2D_UDEF, 2}, invRho{CUP2D_INVRHO, 1},
+      vFluid{CUP2D_VFLUID, 2};
 
   SimulationData() = default;
   SimulationData(const SimulationData&) = delete;

@@ -40,6 +40,7 @@ enum cup2d_status {
 enum cup2d_poisson { CUP2D_POISSON_PERIODIC = 0, CUP2D_POISSON_FREESPACE = 1 };
 enum cup2d_field {
   CUP2D_VEL = 0, CUP2D_CHI, CUP2D_PRES, CUP2D_TMP, CUP2D_TMPV, CUP2D_UDEF, CUP2D_INVRHO,
+  CUP2D_VFLUID,          /* sim.vFluid (SimulationData.h:31), iterative penalization only */
   CUP2D_NFIELDS
 };
 enum cup2d_layout { CUP2D_LAYOUT_BLOCKS = 0, CUP2D_LAYOUT_DENSE = 1 };
@@ -55,6 +56,8 @@ typedef struct {
   double fadeLen;        /* -fadeLen (fraction of extent)                    */
   int poisson;           /* -poissonType periodic|freespace                  */
   int implicit_penal;    /* 0: EXPL_INTEGRATE_MOM (PressureSingle.cpp:20)    */
+  int iterative_penal;   /* -iterativePenalization: 1 = PressureIterator_unif
+                            replaces PressureSingle (Simulation.cpp:215-216) */
   void* stream;          /* hipStream_t to enqueue on; NULL = own stream     */
 } cup2d_cfg;
 
@@ -133,6 +136,11 @@ int cup2d_body_update(cup2d_ctx* ctx, double dt);
 int cup2d_penalize(cup2d_ctx* ctx, double dt);
 /* PressureSingle::operator() (PressureSingle.cpp:346-373). */
 int cup2d_pressure_single(cup2d_ctx* ctx, double dt);
+/* PressureIterator_unif::operator() (PressureIterator_unif.cpp:216-283):
+ * projection / body update / penalization fixed-point iteration with an
+ * adaptive tolerance.  iters and relDF (may be NULL) receive the iterations
+ * done and the last relative momentum change. */
+int cup2d_pressure_iterator(cup2d_ctx* ctx, double dt, int* iters, double* relDF);
 /* ComputeForces::operator() + Shape::computeForces (ComputeForces.cpp:16-111,
  * Shape.cpp:252-281). */
 int cup2d_compute_forces(cup2d_ctx* ctx);

@@ -19,15 +19,16 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libcup2d_oracle.so")
 BS = 32
-VEL, CHI, PRES, TMP, TMPV, UDEF, INVRHO = range(7)
-NCOMP = [2, 1, 1, 1, 2, 2, 1]
+VEL, CHI, PRES, TMP, TMPV, UDEF, INVRHO, VFLUID = range(8)
+NCOMP = [2, 1, 1, 1, 2, 2, 1, 2]
 
 
 class Cfg(C.Structure):
     _fields_ = [("bpdx", C.c_int), ("bpdy", C.c_int), ("extent", C.c_double),
                 ("nu", C.c_double), ("CFL", C.c_double), ("lam", C.c_double),
                 ("fadeLen", C.c_double), ("poisson", C.c_int), ("fs_pad2", C.c_int),
-                ("implicit_penal", C.c_int), ("checker", C.c_int)]
+                ("implicit_penal", C.c_int), ("checker", C.c_int),
+                ("iterative_penal", C.c_int)]
 
 
 class ShapeDesc(C.Structure):
@@ -100,6 +101,9 @@ def lib():
         L.orc_calc_max_timestep.argtypes = [P]
         L.orc_calc_max_timestep.restype = D
         L.orc_poisson_dense.argtypes = [P, dp, dp]
+        L.orc_pressure_iterator.argtypes = [P, D]
+        L.orc_pressure_iterator.restype = C.c_int
+        L.orc_iter_stats.argtypes = [P, C.POINTER(C.c_int), dp, dp]
         L.orc_fft1d.argtypes = [dp, C.c_int, C.c_int]
         L.orc_get_timing.argtypes = [P, dp, dp]
         L.orc_num_threads.restype = C.c_int
@@ -145,12 +149,12 @@ class Oracle:
 
     def __init__(self, bpdx, bpdy, extent=1.0, nu=1e-2, CFL=0.1, lam=None, fadeLen=0.01,
                  poisson="periodic", fs_pad2=False, implicit_penal=False, checker=False,
-                 shapes=()):
+                 iterative_penal=False, shapes=()):
         self.L = lib()
         lam = 1e3 / CFL if lam is None else lam
         cfg = Cfg(bpdx, bpdy, extent, nu, CFL, lam, fadeLen,
                   0 if poisson == "periodic" else 1, int(fs_pad2), int(implicit_penal),
-                  int(checker))
+                  int(checker), int(iterative_penal))
         self.h = self.L.orc_create(C.byref(cfg))
         if not self.h:
             raise ValueError("orc_create failed")
@@ -232,6 +236,14 @@ class Oracle:
 
     def pressure_single(self, dt):
         self.L.orc_pressure_single(self.h, dt)
+
+    def pressure_iterator(self, dt):
+        return self.L.orc_pressure_iterator(self.h, dt)
+
+    def iter_stats(self):
+        n, r, t = C.c_int(), C.c_double(), C.c_double()
+        self.L.orc_iter_stats(self.h, C.byref(n), C.byref(r), C.byref(t))
+        return dict(iters=n.value, relDF=r.value, target=t.value)
 
     def compute_forces(self):
         self.L.orc_compute_forces(self.h)

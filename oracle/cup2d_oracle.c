@@ -72,9 +72,14 @@ struct orc_sim {
   double* kernel;
   fft2d fft;
   double t_total, t_fft;
+  /* PressureIterator_unif state (PressureIterator_unif.h:17-18) */
+  int iter_old_nsteps;
+  double iter_target;
+  int iter_last;
+  double iter_relDF;
 };
 
-static const int NCOMP[ORC_NFIELDS] = {2, 1, 1, 1, 2, 2, 1};
+static const int NCOMP[ORC_NFIELDS] = {2, 1, 1, 1, 2, 2, 1, 2};
 int orc_ncomp(int field) { return (field >= 0 && field < ORC_NFIELDS) ? NCOMP[field] : 0; }
 int orc_num_threads(void) { return omp_get_max_threads(); }
 
@@ -175,6 +180,8 @@ orc_sim* orc_create(const orc_cfg* cfg)
   /* Simulation.cpp:97-98 */
   s->fadeLenX = cfg->fadeLen * cfg->extent;
   s->fadeLenY = cfg->fadeLen * cfg->extent;
+  s->iter_old_nsteps = 10000; /* PressureIterator_unif.h:17-18 */
+  s->iter_target = 1e-4;
   const size_t ncell = (size_t)s->nb * BS * BS;
   for (int f = 0; f < ORC_NFIELDS; ++f) s->f[f] = (double*)calloc(ncell * NCOMP[f], sizeof(double));
   s->pRHS = (double*)calloc(ncell, sizeof(double));
@@ -1002,6 +1009,9 @@ static void lu3_solve(double A[3][3], const double b[3], double x[3])
 /* PressureSingle::integrateMomenta (PressureSingle.cpp:23-67), Shape::updateVelocity
  * (Shape.cpp:25-59), Disk::updateVelocity (ShapesSimple.cpp:83-87),
  * PressureSingle::preventCollidingObstacles (PressureSingle.cpp:243-344) */
+static void update_velocity(orc_sim* s, shape_t* sh, double dt);
+static void prevent_colliding_obstacles(orc_sim* s);
+
 void orc_body_update(orc_sim* s, double dt)
 {
   const double hsq = s->h * s->h;
@@ -1047,7 +1057,17 @@ void orc_body_update(orc_sim* s, double dt)
       for (int i = 0; i < 7; ++i) S[i] += part[7 * (size_t)b + i];
     t->penalM = S[0]; t->penalJ = S[1]; t->penalDX = S[2]; t->penalDY = S[3];
     t->fluidMomX = S[4]; t->fluidMomY = S[5]; t->fluidAngMom = S[6];
-    /* Shape::updateVelocity */
+    update_velocity(s, sh, dt);
+  }
+  free(part);
+  prevent_colliding_obstacles(s);
+}
+
+/* Shape::updateVelocity (Shape.cpp:25-59) from the momenta in the state */
+static void update_velocity(orc_sim* s, shape_t* sh, double dt)
+{
+  {
+    orc_shape_state* t = &sh->st;
     double A[3][3] = {{t->penalM, 0, -t->penalDY}, {0, t->penalM, t->penalDX},
                       {-t->penalDY, t->penalDX, t->penalJ}};
     double bb[3] = {t->fluidMomX + dt * sh->appliedForceX, t->fluidMomY + dt * sh->appliedForceY,
@@ -1066,8 +1086,11 @@ void orc_body_update(orc_sim* s, double dt)
       if (sh->d.bForcedy && s->time < sh->d.tAccel) t->v = (s->time / sh->d.tAccel) * sh->forcedv;
     }
   }
-  free(part);
-  /* preventCollidingObstacles */
+}
+
+/* PressureSingle::preventCollidingObstacles (PressureSingle.cpp:243-344) */
+static void prevent_colliding_obstacles(orc_sim* s)
+{
   const int N = s->nshapes;
   if (N < 2) return;
   double* coll = (double*)calloc((size_t)N * 10, sizeof(double));
@@ -1173,6 +1196,189 @@ void orc_pressure_single(orc_sim* s, double dt)
   orc_body_update(s, dt);
   checker(s);
   orc_penalize(s, dt);
+}
+
+/* ------------------------------------------------ PressureIterator_unif */
+/* pressureCorrection (PressureIterator_unif.cpp:51-75): vFluid = tmpV + pFac grad p */
+static void iter_pressure_correction(orc_sim* s, double dt)
+{
+  const double h = s->h, pFac = -0.5 * dt / h;
+  #pragma omp parallel
+  {
+    double* P = (double*)malloc(sizeof(double) * LW * LW);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; ++b) {
+      lab_load(s, s->f[ORC_PRES], 1, b, P);
+      double* V = blk(s, ORC_VFLUID, b);
+      const double* T = blk(s, ORC_TMPV, b);
+      for (int iy = 0; iy < BS; ++iy)
+        for (int ix = 0; ix < BS; ++ix) {
+          const int c = 2 * (iy * BS + ix);
+          V[c] = T[c] + pFac * (LAB(P, 1, ix + 1, iy, 0) - LAB(P, 1, ix - 1, iy, 0));
+          V[c + 1] = T[c + 1] + pFac * (LAB(P, 1, ix, iy + 1, 0) - LAB(P, 1, ix, iy - 1, 0));
+        }
+    }
+    free(P);
+  }
+}
+
+/* integrateMomenta (PressureIterator_unif.cpp:77-126) with EXPL_INTEGRATE_MOM
+ * (F = h^2 chi, velocity vFluid, penalDX = penalDY = 0), then
+ * Shape::updateVelocity */
+static void iter_integrate_momenta(orc_sim* s, shape_t* sh, double dt)
+{
+  const double hsq = s->h * s->h;
+  orc_shape_state* t = &sh->st;
+  const double Cx = t->centerOfMass[0], Cy = t->centerOfMass[1];
+  double* part = (double*)calloc((size_t)s->nb * 7, sizeof(double));
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int b = 0; b < s->nb; ++b) {
+    const oblock* o = sh->ob[b];
+    if (!o) continue;
+    const double* V = blk(s, ORC_VFLUID, b);
+    double PM = 0, PJ = 0, PX = 0, PY = 0, UM = 0, VM = 0, AM = 0;
+    for (int iy = 0; iy < BS; ++iy)
+      for (int ix = 0; ix < BS; ++ix) {
+        if (o->chi[iy][ix] <= 0) continue;
+        const double F = hsq * o->chi[iy][ix];
+        const double udiff[2] = {V[2 * (iy * BS + ix)], V[2 * (iy * BS + ix) + 1]};
+        double p[2];
+        bpos(s, b, ix, iy, p);
+        p[0] -= Cx; p[1] -= Cy;
+        PM += F;
+        PJ += F * (p[0] * p[0] + p[1] * p[1]);
+        PX += F * p[0]; PY += F * p[1];
+        UM += F * udiff[0]; VM += F * udiff[1];
+        AM += F * (p[0] * udiff[1] - p[1] * udiff[0]);
+      }
+    double* q = part + 7 * (size_t)b;
+    q[0] = PM; q[1] = PJ; q[2] = PX; q[3] = PY; q[4] = UM; q[5] = VM; q[6] = AM;
+  }
+  double S[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < s->nb; ++b)
+    for (int i = 0; i < 7; ++i) S[i] += part[7 * (size_t)b + i];
+  free(part);
+  t->fluidAngMom = S[6]; t->fluidMomX = S[4]; t->fluidMomY = S[5];
+  t->penalDX = 0; t->penalDY = 0; t->penalM = S[0]; t->penalJ = S[1];
+  update_velocity(s, sh, dt);
+}
+
+/* penalize (PressureIterator_unif.cpp:128-184): vel = tmpV + chi (U_s - vFluid)
+ * on the bodies' cells; returns the relative L2 change of the momentum */
+static double iter_penalize(orc_sim* s, double dt)
+{
+  (void)dt;
+  double* part = (double*)calloc((size_t)s->nb * 4, sizeof(double));
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int b = 0; b < s->nb; ++b) {
+    double MX = 0, MY = 0, DMX = 0, DMY = 0;
+    for (int k = 0; k < s->nshapes; ++k) {
+      const shape_t* sh = &s->shapes[k];
+      const oblock* o = sh->ob[b];
+      if (!o) continue;
+      const double u_s = sh->st.u, v_s = sh->st.v, omega_s = sh->st.omega;
+      const double Cx = sh->st.centerOfMass[0], Cy = sh->st.centerOfMass[1];
+      const double* CHI = blk(s, ORC_CHI, b);
+      const double* T = blk(s, ORC_TMPV, b);
+      const double* UF = blk(s, ORC_VFLUID, b);
+      double* V = blk(s, ORC_VEL, b);
+      for (int iy = 0; iy < BS; ++iy)
+        for (int ix = 0; ix < BS; ++ix) {
+          const double X = o->chi[iy][ix];
+          if (CHI[iy * BS + ix] > X) continue;
+          if (X <= 0) continue;
+          double p[2];
+          bpos(s, b, ix, iy, p);
+          p[0] -= Cx; p[1] -= Cy;
+          const double US = u_s - omega_s * p[1] + o->udef[iy][ix][0];
+          const double VS = v_s + omega_s * p[0] + o->udef[iy][ix][1];
+          const double penalFac = X;
+          const int c = 2 * (iy * BS + ix);
+          const double DFX = penalFac * (US - UF[c]);
+          const double DFY = penalFac * (VS - UF[c + 1]);
+          const double DPX = T[c] + DFX - V[c];
+          const double DPY = T[c + 1] + DFY - V[c + 1];
+          V[c] += DPX;
+          V[c + 1] += DPY;
+          MX += V[c] * V[c]; DMX += DPX * DPX;
+          MY += V[c + 1] * V[c + 1]; DMY += DPY * DPY;
+        }
+    }
+    double* q = part + 4 * (size_t)b;
+    q[0] = MX; q[1] = MY; q[2] = DMX; q[3] = DMY;
+  }
+  double MX = 0, MY = 0, DMX = 0, DMY = 0;
+  for (int b = 0; b < s->nb; ++b) {
+    MX += part[4 * (size_t)b]; MY += part[4 * (size_t)b + 1];
+    DMX += part[4 * (size_t)b + 2]; DMY += part[4 * (size_t)b + 3];
+  }
+  free(part);
+  return sqrt((DMX + DMY) / (EPS + MX + MY));
+}
+
+/* updatePressureRHS (PressureIterator_unif.cpp:22-49):
+ * r = (h/2dt) (div vel - CHI div uDef) */
+static void iter_pressure_rhs(orc_sim* s, double dt)
+{
+  const double facDiv = 0.5 * s->h / dt;
+  #pragma omp parallel
+  {
+    double* V = (double*)malloc(sizeof(double) * LW * LW * 2);
+    double* U = (double*)malloc(sizeof(double) * LW * LW * 2);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; ++b) {
+      lab_load(s, s->f[ORC_VEL], 2, b, V);
+      lab_load(s, s->f[ORC_UDEF], 2, b, U);
+      const double* CHI = blk(s, ORC_CHI, b);
+      double* T = blk(s, ORC_TMP, b);
+      for (int iy = 0; iy < BS; ++iy)
+        for (int ix = 0; ix < BS; ++ix) {
+          const double divVx = LAB(V, 2, ix + 1, iy, 0) - LAB(V, 2, ix - 1, iy, 0);
+          const double divVy = LAB(V, 2, ix, iy + 1, 1) - LAB(V, 2, ix, iy - 1, 1);
+          const double divUSx = LAB(U, 2, ix + 1, iy, 0) - LAB(U, 2, ix - 1, iy, 0);
+          const double divUSy = LAB(U, 2, ix, iy + 1, 1) - LAB(U, 2, ix, iy - 1, 1);
+          T[iy * BS + ix] = facDiv * (divVx + divVy - CHI[iy * BS + ix] * (divUSx + divUSy));
+        }
+    }
+    free(V);
+    free(U);
+  }
+}
+
+int orc_pressure_iterator(orc_sim* s, double dt)
+{
+  const size_t n2 = (size_t)s->nb * BS * BS * 2;
+  memcpy(s->f[ORC_TMPV], s->f[ORC_VEL], sizeof(double) * n2);
+  int iter;
+  double relDF = 1e3;
+  for (iter = 0; iter < 1000; ++iter) {
+    iter_pressure_correction(s, dt);
+    for (int k = 0; k < s->nshapes; ++k) iter_integrate_momenta(s, &s->shapes[k], dt);
+    relDF = iter_penalize(s, dt);
+    iter_pressure_rhs(s, dt);
+    orc_poisson_solve(s);
+    const int conv = relDF < s->iter_target || iter > 100 || iter > 2 * s->iter_old_nsteps;
+    if (conv) {
+      /* finalizePressure (PressureIterator_unif.cpp:186-212) */
+      orc_pressure_correction(s, dt);
+      break;
+    }
+  }
+  s->iter_old_nsteps = iter + 1;
+  if (s->iter_old_nsteps > 30) s->iter_target = dmax(relDF, s->iter_target);
+  if (s->iter_old_nsteps > 10) s->iter_target *= 1.01;
+  if (s->iter_old_nsteps <= 2) s->iter_target *= 0.99;
+  s->iter_target = dmin(1e-3, dmax(1e-5, s->iter_target));
+  s->iter_last = iter + 1;
+  s->iter_relDF = relDF;
+  return iter + 1;
+}
+
+void orc_iter_stats(const orc_sim* s, int* iters, double* relDF, double* target)
+{
+  if (iters) *iters = s->iter_last;
+  if (relDF) *relDF = s->iter_relDF;
+  if (target) *target = s->iter_target;
 }
 
 /* ComputeForces::operator() (ComputeForces.cpp:16-111), Shape::computeForces
@@ -1281,7 +1487,8 @@ int orc_advance(orc_sim* s, double dt)
   const int err = orc_put_objects(s, dt);
   if (err) return err;
   orc_advdiff(s, dt);
-  orc_pressure_single(s, dt);
+  if (s->cfg.iterative_penal) orc_pressure_iterator(s, dt);
+  else orc_pressure_single(s, dt);
   orc_compute_forces(s);
   s->time += s->dt;
   s->step++;

@@ -31,6 +31,8 @@ typedef struct {
                          /*            1 = 2N cyclic (same math)             */
   int implicit_penal;    /* 0 = EXPL_INTEGRATE_MOM (reference), 1 = implicit */
   int checker;           /* 1 = run the NaN/Inf Checker 17x per step        */
+  int iterative_penal;   /* 1 = PressureIterator_unif instead of PressureSingle
+                            (-iterativePenalization 1, Simulation.cpp:215-216) */
 } orc_cfg;
 
 typedef struct {
@@ -52,7 +54,7 @@ typedef struct {
 } orc_shape_state;
 
 enum { ORC_VEL = 0, ORC_CHI, ORC_PRES, ORC_TMP, ORC_TMPV, ORC_UDEF, ORC_INVRHO,
-       ORC_NFIELDS };
+       ORC_VFLUID, ORC_NFIELDS };
 
 typedef struct orc_sim orc_sim;
 
@@ -81,6 +83,11 @@ void   orc_pressure_correction(orc_sim* s, double dt);
 void   orc_body_update(orc_sim* s, double dt);/* integrateMomenta+update+coll*/
 void   orc_penalize(orc_sim* s, double dt);
 void   orc_pressure_single(orc_sim* s, double dt);
+/* PressureIterator_unif::operator() (PressureIterator_unif.cpp:216-283);
+ * returns the iterations done.  orc_iter_stats: iterations and the relative
+ * momentum change of the last call, and the adaptive tolerance after it. */
+int    orc_pressure_iterator(orc_sim* s, double dt);
+void   orc_iter_stats(const orc_sim* s, int* iters, double* relDF, double* target);
 void   orc_compute_forces(orc_sim* s);
 int    orc_advance(orc_sim* s, double dt);    /* Simulation::advance         */
 int    orc_simulate(orc_sim* s, int nsteps);  /* nsteps x (dt + advance)     */

@@ -278,3 +278,73 @@ def test_bad_arguments(cup):
     assert e.value.code == cup.ERR_ARG
     with pytest.raises(cup.Cup2dError):
         g.get_field(17)
+
+
+@pytest.mark.parametrize("poisson", ["freespace", "periodic"])
+def test_disk_on_boundary_blocks(cup, oracle_mod, poisson):
+    """A forced disk whose obstacle blocks touch the domain faces and the
+    cub2rhs fade band: the obstacle RHS, the deferred advDiff flux correction
+    and the fade meet in the same blocks."""
+    shapes = [dict(radius=0.05, xpos=0.055, ypos=0.06, xvel=0.15, yvel=0.05, bFixed=True,
+                   bForced=True)]
+    g, o = _disk_pair(cup, oracle_mod, (8, 4), shapes, extent=1.0, nu=1e-3, CFL=0.2,
+                      poisson=poisson, fadeLen=0.05)
+    rng = np.random.default_rng(19)
+    g.reset()
+    o.reset()
+    vel = 0.05 * rng.standard_normal((2, g.NY, g.NX))
+    for sim in (g, o):
+        sim.set_field(0, vel)
+    for _ in range(6):
+        dtg, dto = g.calc_max_timestep(), o.calc_max_timestep()
+        g.advance(dtg)
+        o.advance(dto)
+    ev = rel(g.get_field(0), o.get_field(0))
+    ep = rel(g.get_field(2), o.get_field(2))
+    print(f"  rel L2 vel {ev:.3e} pres {ep:.3e}", flush=True)
+    assert ev < 1e-10 and ep < 1e-10
+    a, b = g.shape(), o.shape()
+    assert abs(a["drag"] - b["drag"]) <= 1e-8 * max(abs(b["drag"]), abs(b["forcex"]))
+
+
+def test_pressure_iterator_towed_cylinder(cup, oracle_mod):
+    """-iterativePenalization 1 (PressureIterator_unif): whole steps against the
+    oracle; the iteration count of every step and the adaptive tolerance
+    match, and so do the fields."""
+    shapes = [dict(radius=0.08, xpos=0.3, ypos=0.25, xvel=0.2, bFixed=True, bForced=True)]
+    g, o = _disk_pair(cup, oracle_mod, (8, 4), shapes, extent=1.0, nu=1e-3, CFL=0.1,
+                      poisson="freespace", iterative_penal=True)
+    g.reset()
+    o.reset()
+    for step in range(4):
+        dtg, dto = g.calc_max_timestep(), o.calc_max_timestep()
+        assert abs(dtg - dto) <= 1e-13 * dto
+        g.advance(dtg)
+        o.advance(dto)
+    ev = rel(g.get_field(0), o.get_field(0))
+    ep = rel(g.get_field(2), o.get_field(2))
+    print(f"  rel L2 vel {ev:.3e} pres {ep:.3e}", flush=True)
+    assert ev < 1e-10 and ep < 1e-10
+    a, b = g.shape(), o.shape()
+    assert abs(a["drag"] - b["drag"]) <= 1e-8 * max(abs(b["drag"]), abs(b["forcex"]))
+
+
+def test_pressure_iterator_entry_point(cup, oracle_mod):
+    """cup2d_pressure_iterator on the same state as the oracle's operator."""
+    shapes = [dict(radius=0.06, xpos=0.4, ypos=0.25, rhoS=1.5)]
+    g, o = _disk_pair(cup, oracle_mod, (8, 4), shapes, extent=1.0, nu=1e-3, CFL=0.2,
+                      poisson="freespace")
+    rng = np.random.default_rng(24)
+    g.reset()
+    o.reset()
+    vel = 0.05 * rng.standard_normal((2, g.NY, g.NX))
+    for sim in (g, o):
+        sim.set_field(0, vel)
+    n_g, r_g = g.pressure_iterator(1e-3)
+    n_o = o.pressure_iterator(1e-3)
+    st = o.iter_stats()
+    assert n_g == n_o
+    assert abs(r_g - st["relDF"]) <= 1e-9 * st["relDF"]
+    assert rel(g.get_field(0), o.get_field(0)) < 1e-10
+    assert rel(g.get_field(2), o.get_field(2)) < 1e-10
+    assert rel(g.get_field(cup.VFLUID), o.get_field(oracle_mod.VFLUID)) < 1e-10

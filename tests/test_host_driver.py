@@ -43,12 +43,11 @@ def test_driver_rejects_solver_without_device_path():
     assert "no device solver" in r.stderr
 
 
-def test_driver_rejects_iterative_penalization():
+def test_driver_rejects_dlm():
     _need_driver()
-    r = _run(["-bpdx", "2", "-bpdy", "2", "-poissonType", "periodic",
-              "-iterativePenalization", "1"])
+    r = _run(["-bpdx", "2", "-bpdy", "2", "-poissonType", "periodic", "-dlm", "1"])
     assert r.returncode == 2
-    assert "iterativePenalization" in r.stderr
+    assert "dlm" in r.stderr
 
 
 def rel(a, b):
