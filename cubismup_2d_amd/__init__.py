@@ -19,7 +19,7 @@ BS = 32
 VEL, CHI, PRES, TMP, TMPV, UDEF, INVRHO, VFLUID = range(8)
 NCOMP = [2, 1, 1, 1, 2, 2, 1, 2]
 LAYOUT_BLOCKS, LAYOUT_DENSE = 0, 1
-POISSON = {"periodic": 0, "freespace": 1}
+POISSON = {"periodic": 0, "freespace": 1, "cosine": 2}
 
 OK = 0
 ERR_ARG, ERR_HIP, ERR_STATE, ERR_BODY_OUT, ERR_CAPACITY, ERR_NONFINITE = -1, -2, -3, -4, -5, -6

@@ -129,7 +129,7 @@ int need_fft(cup2d_ctx* c) {
 // and boundary-band blocks afterwards.
 int pressure_single(cup2d_ctx* c, int fused) {
   RC(need_fft(c));
-  if (fused && c->flux_deferred) {
+  if (fused && c->flux_deferred && c->cfg.poisson != CUP2D_POISSON_COSINE) {
     RC(join_obstacles(c));
     RC(enqueue_rhs_obstacles(c, 1));
     RC(enqueue_fade_sums(c, 1));
@@ -182,10 +182,16 @@ int pressure_iterator(cup2d_ctx* c, int* iters_out, double* reldf_out) {
   return CUP2D_OK;
 }
 
+// Run the obstacle placement on the side stream, overlapping advDiff (1), or
+// in line before it (0).
+#ifndef CUP2D_OBST_OVERLAP
+#define CUP2D_OBST_OVERLAP 1
+#endif
+
 // Simulation::advance.  compute_dt: dt from the findMaxU partials of the
 // previous step (device loop) instead of the value already on the device.
 int advance_enqueued(cup2d_ctx* c, int compute_dt) {
-  RC(enqueue_put_objects(c, compute_dt, 1));
+  RC(enqueue_put_objects(c, compute_dt, CUP2D_OBST_OVERLAP));
   if (c->cfg.iterative_penal) {
     RC(enqueue_advdiff(c, 0));
     RC(pressure_iterator(c, nullptr, nullptr));
@@ -210,8 +216,8 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
   if (!cfg || !out) { g_create_error = "null argument"; return CUP2D_ERR_ARG; }
   *out = nullptr;
   if (cfg->bpdx <= 0 || cfg->bpdy <= 0 || !(cfg->extent > 0) || !(cfg->nu > 0) ||
-      !(cfg->CFL > 0) || (cfg->poisson != 0 && cfg->poisson != 1)) {
-    g_create_error = "invalid configuration (bpdx, bpdy, extent, nu, CFL > 0; poisson 0|1)";
+      !(cfg->CFL > 0) || cfg->poisson < 0 || cfg->poisson > 2) {
+    g_create_error = "invalid configuration (bpdx, bpdy, extent, nu, CFL > 0; poisson 0|1|2)";
     return CUP2D_ERR_ARG;
   }
   cup2d_ctx* c = new cup2d_ctx();

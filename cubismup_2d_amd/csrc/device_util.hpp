@@ -146,9 +146,10 @@ __device__ inline void block_sum_multi(double* a, double* lds) {
 }
 
 // Column sums of an n x NQ table (row i at a + i*stride): thread t adds rows
-// t, t+NT, ... in order, then block_sum_multi.  Loads run U rows ahead of the
-// additions, so a long table costs one memory latency per U rows; the order of
-// the additions, and so the result, is the same in every workgroup.
+// t, t+NT, ... in order, then block_sum_multi.  Loads are issued U rows at a
+// time ahead of the additions, so a table of up to U*NT rows costs one memory
+// latency; the order of the additions, and so the result, is the same in
+// every workgroup.
 template <int NT, int NQ, int U = 4>
 __device__ inline void block_sum_table(const double* a, int n, int stride, double* out,
                                        double* lds) {
@@ -167,9 +168,19 @@ __device__ inline void block_sum_table(const double* a, int n, int stride, doubl
 #pragma unroll
       for (int j = 0; j < NQ; ++j) s[j] += v[r][j];
   }
-  for (; i < n; i += NT)
+  if (i < n) {  // fewer than U rows left: load them all, then add in order
+    double v[U][NQ];
 #pragma unroll
-    for (int j = 0; j < NQ; ++j) s[j] += a[(size_t)i * stride + j];
+    for (int r = 0; r < U; ++r)
+#pragma unroll
+      for (int j = 0; j < NQ; ++j)
+        v[r][j] = i + r * NT < n ? a[(size_t)(i + r * NT) * stride + j] : 0.0;
+#pragma unroll
+    for (int r = 0; r < U; ++r)
+      if (i + r * NT < n)
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) s[j] += v[r][j];
+  }
   block_sum_multi<NT, NQ>(s, lds);
 #pragma unroll
   for (int j = 0; j < NQ; ++j) out[j] = s[j];
@@ -225,12 +236,16 @@ __device__ inline double flux_corr(const double* __restrict__ fluxpart, int ne, 
   const int nt = blockDim.x;
   double s = 0;
   int e = threadIdx.x;
-  for (; e + 3 * nt < ne; e += 4 * nt) {
-    const double a0 = fluxpart[e], a1 = fluxpart[e + nt], a2 = fluxpart[e + 2 * nt],
-                 a3 = fluxpart[e + 3 * nt];
-    s += a0; s += a1; s += a2; s += a3;
+  for (; e < ne; e += 4 * nt) {  // four loads in flight, added in order
+    const double a0 = fluxpart[e];
+    const double a1 = e + nt < ne ? fluxpart[e + nt] : 0.0;
+    const double a2 = e + 2 * nt < ne ? fluxpart[e + 2 * nt] : 0.0;
+    const double a3 = e + 3 * nt < ne ? fluxpart[e + 3 * nt] : 0.0;
+    s += a0;
+    if (e + nt < ne) s += a1;
+    if (e + 2 * nt < ne) s += a2;
+    if (e + 3 * nt < ne) s += a3;
   }
-  for (; e < ne; e += nt) s += fluxpart[e];
   const double IF = block_sum_dyn(s, lds);
   return IF / (2 * (BS * g.bpdy) + 2 * (BS * g.bpdx));
 }
@@ -244,12 +259,16 @@ __device__ inline double flux_corr256(const double* __restrict__ fluxpart, int n
   double s = 0;
   if (threadIdx.x < nt) {
     int e = threadIdx.x;
-    for (; e + 3 * nt < ne; e += 4 * nt) {
-      const double a0 = fluxpart[e], a1 = fluxpart[e + nt], a2 = fluxpart[e + 2 * nt],
-                   a3 = fluxpart[e + 3 * nt];
-      s += a0; s += a1; s += a2; s += a3;
+    for (; e < ne; e += 4 * nt) {  // four loads in flight, added in order
+      const double a0 = fluxpart[e];
+      const double a1 = e + nt < ne ? fluxpart[e + nt] : 0.0;
+      const double a2 = e + 2 * nt < ne ? fluxpart[e + 2 * nt] : 0.0;
+      const double a3 = e + 3 * nt < ne ? fluxpart[e + 3 * nt] : 0.0;
+      s += a0;
+      if (e + nt < ne) s += a1;
+      if (e + 2 * nt < ne) s += a2;
+      if (e + 3 * nt < ne) s += a3;
     }
-    for (; e < ne; e += nt) s += fluxpart[e];
   }
   const double IF = block_sum_dyn(s, lds);
   return IF / (2 * (BS * g.bpdy) + 2 * (BS * g.bpdx));

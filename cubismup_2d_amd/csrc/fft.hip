@@ -14,6 +14,14 @@
 //      zero-pruned: only the NY data rows are read and written.
 //   3. rows inverse: unpack two half-spectrum rows, inverse complex FFT, real
 //      part -> row y, imaginary part -> row y+1.
+// Cosine solver (FFTW_dirichlet, FFTW_dirichlet.h:23-148): a DCT-II of an
+// N-point sequence x is, up to the phase exp(i pi k / 2N), the FFT of its
+// 2N-point mirror image (x, reversed x), and a DCT-III of real coefficients
+// multiplied by a symbol s(k) is the first N points of the inverse FFT of
+// s(k) times that spectrum when s is extended evenly, s(2N-k) = s(k).  So the
+// 2D cosine solve is the free-space pipeline with the mirrored grid in place
+// of the zero padding (the row pass reads x reversed past NX, the column pass
+// reads the spectrum rows reversed past NY) and the DCT-domain symbol.
 // The FFT core is a radix-16 Stockham autosort (radix 2/4/8 for the last
 // pass), 16 complex values per thread in registers, LDS padded by one slot
 // every 16 entries so stride-16 stores are bank-conflict free.  Twiddles come
@@ -32,11 +40,14 @@ struct FFTPlan {
   int NYs = 0;            // rows stored in the spectrum
   int ncg = 0;            // column groups
   int fs = 0;             // free-space
+  int cs = 0;             // cosine (Neumann) solver on the mirrored 2NX x 2NY grid
   double2* spec = nullptr;
   double2* tw_r = nullptr;
   double2* tw_c = nullptr;
   double* ghat = nullptr;   // free-space: [ncg][NY+1][G]
   double* ghatN = nullptr;  // free-space Nyquist column [NY+1]
+  double* cosX = nullptr;   // cosine: (float) cos(2 pi i / NX), i < NX
+  double* cosY = nullptr;   // cosine: (float) cos(2 pi j / NY), j < NY
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -308,6 +319,7 @@ struct RowSource {
   const unsigned char* obmap;
   const double* fluxpart;  // advDiff edge partials
   int ne;
+  int mirror;           // cosine: columns past ncols read the row reversed
 };
 
 template <int L, int NP, int E, int SRC>
@@ -318,7 +330,11 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
   extern __shared__ double2 lds[];
   __shared__ double s_red[64];  // workgroup reductions (<= 16 waves x 3)
   const int f = threadIdx.x / NT, t = threadIdx.x % NT;
-  const int y0 = 2 * (blockIdx.x * NP + f);
+  // SRC_DIV reads the v rows next to its pair too: consecutive pairs run on
+  // one XCD (workgroup b runs on XCD b % 8) so those rows are L2 hits
+  int wg = blockIdx.x;
+  if (SRC == SRC_DIV && gridDim.x % 8 == 0) wg = (wg % 8) * (gridDim.x / 8) + wg / 8;
+  const int y0 = 2 * (wg * NP + f);
   const bool valid = y0 < nrows;
   const int base = f * PADL;
   double2 w[TwCount<L, E, E>::n];
@@ -375,7 +391,8 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
     const double* r1 = r0 + in_pitch;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const int i = t + q * NT;
+      int i = t + q * NT;
+      if (fd.mirror && i >= ncols) i = 2 * ncols - 1 - i;
       double a = 0, b = 0;
       if (valid && i < ncols) { a = r0[i]; b = r1[i]; }
       x[q] = make_double2(a, b);
@@ -392,19 +409,21 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
       if (valid) {
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-          const int i = t + q * NT;
+          const int iq = t + q * NT;
+          const int i = fd.mirror && iq >= ncols ? 2 * ncols - 1 - iq : iq;
           if (i >= ncols) continue;
+          const bool own = i == iq;  // the mirrored copy is written back once
           const int bx = i / BS;
           const double px = cell_pos(bx, i % BS, g.hb, g.h);
           if (band_block(g, bx, by0)) {
             const double py = cell_pos(by0, y0 % BS, g.hb, g.h);
             x[q].x = fade_cell(x[q].x, fade_arg(g, px, py), corr);
-            fd.rhs[(size_t)y0 * in_pitch + i] = x[q].x;
+            if (own) fd.rhs[(size_t)y0 * in_pitch + i] = x[q].x;
           }
           if (band_block(g, bx, by1)) {
             const double py = cell_pos(by1, (y0 + 1) % BS, g.hb, g.h);
             x[q].y = fade_cell(x[q].y, fade_arg(g, px, py), corr);
-            fd.rhs[(size_t)(y0 + 1) * in_pitch + i] = x[q].y;
+            if (own) fd.rhs[(size_t)(y0 + 1) * in_pitch + i] = x[q].y;
           }
         }
       }
@@ -503,6 +522,8 @@ struct ColParams {
   double waveFactX, waveFactY, norm;
   const double* ghat;    // MODE 1
   const double* ghatN;
+  const double* cosX;    // MODE 3
+  const double* cosY;
 };
 
 __device__ __forceinline__ double periodic_factor(int kx, int ky, const ColParams& P) {
@@ -514,10 +535,27 @@ __device__ __forceinline__ double periodic_factor(int kx, int ky, const ColParam
   return kinv * P.norm;
 }
 
+// FFTW_dirichlet::_solve (FFTW_dirichlet.h:45-61) at DCT indices (i, j) <
+// (NX, NY).  Line 51 declares the blend weight as `static constexpr tol =
+// 0.1;`, which has no type; it is taken as the double 0.1 the initializer
+// states.  COScoef are floats (FFTW_dirichlet.h:26-27,65-68).
+__device__ __forceinline__ double cosine_factor(int i, int j, const ColParams& P) {
+  if (i == 0 && j == 0) return 0.0;  // in_out[0] = 0
+  const double tol = 0.1;
+  const double rkx = (i + 0.5) * P.waveFactX, rky = (j + 0.5) * P.waveFactY;
+  const double denomFD = 1 - P.cosX[i] / 2 - P.cosY[j] / 2;
+  const double denomSP = rkx * rkx + rky * rky;
+  return -P.norm / ((1 - tol) * denomFD + tol * denomSP);
+}
+
 template <int LC, int MODE>
 __device__ __forceinline__ double col_symbol(int kx, int cg, int c, int ky, const ColParams& P) {
   if constexpr (MODE == 0) {
     return periodic_factor(kx, ky, P);
+  } else if constexpr (MODE == 3) {
+    // symbol extended evenly about NX and NY; zero at the mirror Nyquist
+    if (kx >= P.NX || ky == P.NY) return 0.0;
+    return cosine_factor(kx, ky < P.NY ? ky : LC - ky, P);
   } else {
     const int kyy = ky <= P.NY ? ky : LC - ky;
     return P.ghat[((size_t)cg * (P.NY + 1) + kyy) * G + c];
@@ -564,7 +602,10 @@ __global__ __launch_bounds__(CW * LC / E, CUP2D_FFT_MINB) void k_cols(double2* _
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int i = t + q * NT;
-    x[q] = (MODE != 1 || i < P.rows) ? col[(size_t)i * G + c] : make_double2(0, 0);
+    if constexpr (MODE == 3)  // mirrored grid: rows past NY are rows NY-1, NY-2, ...
+      x[q] = col[(size_t)(i < P.rows ? i : 2 * P.rows - 1 - i) * G + c];
+    else
+      x[q] = (MODE != 1 || i < P.rows) ? col[(size_t)i * G + c] : make_double2(0, 0);
   }
   fft_regs<LC, E, true>(x, lds, cl, CW, t, w);
   if constexpr (MODE == 2) {
@@ -587,6 +628,7 @@ __global__ __launch_bounds__(CW * LC / E, CUP2D_FFT_MINB) void k_cols(double2* _
         if (kx == 0) {
           double sN;
           if constexpr (MODE == 0) sN = periodic_factor(P.NX / 2, i, P);
+          else if constexpr (MODE == 3) sN = 0.0;  // mirror Nyquist kx = NX
           else sN = P.ghatN[i <= P.NY ? i : LC - i];
           const double2 m = lds[padE<E>((LC - i) % LC) * CW + cl];
           const double a = 0.5 * (s0 + sN), b = 0.5 * (s0 - sN);
@@ -607,7 +649,7 @@ __global__ __launch_bounds__(CW * LC / E, CUP2D_FFT_MINB) void k_cols(double2* _
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int i = t + q * NT;
-      if (MODE != 1 || i < P.rows) col[(size_t)i * G + c] = x[q];
+      if ((MODE != 1 && MODE != 3) || i < P.rows) col[(size_t)i * G + c] = x[q];
     }
   }
 }
@@ -727,6 +769,7 @@ static void set_cols_attr() {
   (void)hipFuncSetAttribute((const void*)k_cols<LC, 0, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
   (void)hipFuncSetAttribute((const void*)k_cols<LC, 1, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
   (void)hipFuncSetAttribute((const void*)k_cols<LC, 2, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 3, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
 }
 
 static std::vector<double2> twiddles(int L) {
@@ -744,16 +787,18 @@ static bool pow2(int n) { return n >= 32 && (n & (n - 1)) == 0; }
 int fft_create(cup2d_ctx* c) {
   const int NX = c->g.NX, NY = c->g.NY;
   const bool fs = c->cfg.poisson == CUP2D_POISSON_FREESPACE;
+  const bool cs = c->cfg.poisson == CUP2D_POISSON_COSINE;
   FFTPlan* P = new FFTPlan();
   c->fft = P;
   P->fs = fs;
-  P->Lr = fs ? 2 * NX : NX;
-  P->Lc = fs ? 2 * NY : NY;
+  P->cs = cs;
+  P->Lr = fs || cs ? 2 * NX : NX;
+  P->Lc = fs || cs ? 2 * NY : NY;
   P->NYs = NY;
   P->ncg = (P->Lr / 2) / G;
   if (!pow2(NX) || !pow2(NY) || P->Lr > 8192 || P->Lc > 2048) {
     c->err = "FFT Poisson: NX and NY must be powers of two, row length <= 8192 and column "
-             "length <= 2048 (periodic NY <= 2048, free-space NY <= 1024)";
+             "length <= 2048 (periodic NY <= 2048, free-space and cosine NY <= 1024)";
     return CUP2D_ERR_ARG;
   }
   set_rows_attr_L<32>(); set_rows_attr_L<64>(); set_rows_attr_L<128>(); set_rows_attr_L<256>();
@@ -768,6 +813,15 @@ int fft_create(cup2d_ctx* c) {
   CUP2D_HIP_CHECK(c, hipMalloc(&P->tw_c, sizeof(double2) * twc.size()));
   CUP2D_HIP_CHECK(c, hipMemcpy(P->tw_r, twr.data(), sizeof(double2) * twr.size(), hipMemcpyHostToDevice));
   CUP2D_HIP_CHECK(c, hipMemcpy(P->tw_c, twc.data(), sizeof(double2) * twc.size(), hipMemcpyHostToDevice));
+  if (cs) {  // FFTW_dirichlet.h:65-68: COScoef[i] = (float) cos(pi/M * 2 i)
+    std::vector<double> cx(NX), cy(NY);
+    for (int i = 0; i < NX; ++i) cx[i] = (double)(float)std::cos(M_PI / NX * 2.0 * i);
+    for (int j = 0; j < NY; ++j) cy[j] = (double)(float)std::cos(M_PI / NY * 2.0 * j);
+    CUP2D_HIP_CHECK(c, hipMalloc(&P->cosX, sizeof(double) * NX));
+    CUP2D_HIP_CHECK(c, hipMalloc(&P->cosY, sizeof(double) * NY));
+    CUP2D_HIP_CHECK(c, hipMemcpy(P->cosX, cx.data(), sizeof(double) * NX, hipMemcpyHostToDevice));
+    CUP2D_HIP_CHECK(c, hipMemcpy(P->cosY, cy.data(), sizeof(double) * NY, hipMemcpyHostToDevice));
+  }
   if (!fs) return CUP2D_OK;
 
   // Free-space Green's function on the 2NX x 2NY cyclic grid
@@ -816,6 +870,8 @@ void fft_destroy(cup2d_ctx* c) {
   (void)hipFree(P->tw_c);
   if (P->ghat) (void)hipFree(P->ghat);
   if (P->ghatN) (void)hipFree(P->ghatN);
+  if (P->cosX) (void)hipFree(P->cosX);
+  if (P->cosY) (void)hipFree(P->cosY);
   delete P;
   c->fft = nullptr;
 }
@@ -830,7 +886,7 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   {
     TimerScope ts(c, T_FFT_ROWS_FWD);
     RowSource fd{c->g, c->red, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
-                 c->u, c->v, c->obmap, c->fluxpart, stencil_edge_count(c)};
+                 c->u, c->v, c->obmap, c->fluxpart, stencil_edge_count(c), P->cs};
     rc = rows_fwd(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r, src, fd);
   }
   if (rc) return rc;
@@ -838,7 +894,14 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   cp.NX = NX; cp.NY = NY; cp.rows = NY;
   {
     TimerScope ts(c, T_FFT_COLS);
-    if (!P->fs) {
+    if (P->cs) {  // FFTW_dirichlet.h:47-48
+      cp.waveFactX = M_PI / NX;
+      cp.waveFactY = M_PI / NY;
+      cp.norm = 0.25 / ((double)NX * NY);
+      cp.cosX = P->cosX;
+      cp.cosY = P->cosY;
+      rc = cols<3>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
+    } else if (!P->fs) {
       cp.waveFactX = 2 * M_PI / NX;
       cp.waveFactY = 2 * M_PI / NY;
       cp.norm = 1.0 / ((double)NX * NY);

@@ -191,7 +191,6 @@ int enqueue_advdiff(cup2d_ctx* c, int defer_fix);
 int enqueue_pressure_rhs(cup2d_ctx* c);
 int stencil_edge_count(const cup2d_ctx* c);
 int enqueue_fade_sums(cup2d_ctx* c, int compute = 0);
-constexpr int FADE_WG = 64;                  // workgroups (and partial triples) of the fade sums
 int fade_partials(const cup2d_ctx* c);
 int enqueue_pressure_correction(cup2d_ctx* c, int want_max);
 int enqueue_set_dt(cup2d_ctx* c, double dt);

@@ -345,10 +345,10 @@ int enqueue_pressure_rhs(cup2d_ctx* c) {
 
 // ------------------------------------------------------- cub2rhs fade
 // PoissonSolver.cpp:81-98: the sums of the faded RHS r*fa^2 over the band
-// blocks, of its positive parts and of its negative parts.  FADE_WG
-// workgroups; workgroup w adds band blocks w, w + FADE_WG, ... and writes
-// three partials.  The correction factor and the rewrite of the band cells
-// happen in the forward row FFT, whose workgroups each add these partials.
+// blocks, of its positive parts and of its negative parts; one workgroup per
+// band block writes three partials.  The correction factor and the rewrite of
+// the band cells happen in the forward row FFT, whose workgroups each add
+// these partials in the same order.
 __global__ __launch_bounds__(256) void k_fade_sums(const double* __restrict__ r,
                                                    const int* __restrict__ blocks, int nblocks,
                                                    Grid g, double* __restrict__ part,
@@ -392,9 +392,7 @@ __global__ __launch_bounds__(256) void k_fade_sums(const double* __restrict__ r,
   }
 }
 
-int fade_partials(const cup2d_ctx* c) {
-  return c->band_nblocks < FADE_WG ? c->band_nblocks : FADE_WG;
-}
+int fade_partials(const cup2d_ctx* c) { return c->band_nblocks; }
 
 int enqueue_fade_sums(cup2d_ctx* c, int compute) {
   if (c->band_nblocks == 0) return CUP2D_OK;

@@ -44,8 +44,9 @@ class PoissonSolver {
   virtual void solve(const Field& BSRC, const Field& BDST) = 0;
   virtual std::string getName() = 0;
   // PoissonSolver::makeSolver (PoissonSolver.cpp:23-59) for the poissonType
-  // values with an FFT solver: "periodic", "freespace" (and the CUDA names
-  // "cpu_periodic", "cpu_freespace").  Other types throw cup2d::Error.
+  // values with an FFT solver: "periodic", "freespace", "cosine" or "" (the
+  // default), and the CUDA names "cpu_periodic", "cpu_freespace".  Other
+  // types throw cup2d::Error.
   static std::unique_ptr<PoissonSolver> makeSolver(SimulationData& sim);
 };
 
@@ -55,6 +56,15 @@ class HIP_periodic : public PoissonSolver {
   using PoissonSolver::PoissonSolver;
   void solve(const Field& BSRC, const Field& BDST) override;
   std::string getName() override { return "HIP_periodic"; }
+};
+
+// FFTW_dirichlet (FFTW_dirichlet.h:23-148), the reference's default solver
+// (poissonType "" or "cosine"), on the device.
+class HIP_cosine : public PoissonSolver {
+ public:
+  using PoissonSolver::PoissonSolver;
+  void solve(const Field& BSRC, const Field& BDST) override;
+  std::string getName() override { return "HIP_cosine"; }
 };
 
 // FFTW_freespace (FFTW_freespace.cpp:14-138) on the device.
@@ -117,10 +127,17 @@ class PressureIterator_unif : public Operator {
   std::string getName() override { return "PressureIterator_unif"; }
 };
 
-// ComputeForces (ComputeForces.cpp:16-111).
+// ComputeForces (ComputeForces.cpp:16-111).  Unless -muteAll, appends each
+// body's forces and powers to forceValues_<id>.dat and powerValues_<id>.dat
+// in -file's directory, with the reference's headers and columns
+// (Shape.cpp:305-328).
 class ComputeForces : public Operator {
  public:
   using Operator::Operator;
+  // log lines of the bodies' current force state, stamped with sim.time or
+  // with the given time (the device-loop driver logs after a run of steps)
+  static void writeLogs(SimulationData& sim);
+  static void writeLogs(SimulationData& sim, double time);
   void operator()(const double dt) override;
   std::string getName() override { return "ComputeForces"; }
 };

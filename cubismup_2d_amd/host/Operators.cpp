@@ -1,6 +1,8 @@
 // Operators.cpp -- the pipeline operators and Poisson solvers over the C-ABI.
 #include "Operator.h"
 
+#include <fstream>
+
 namespace cup2d {
 
 // ---------------------------------------------------------------- Poisson
@@ -8,6 +10,7 @@ std::unique_ptr<PoissonSolver> PoissonSolver::makeSolver(SimulationData& sim) {
   const std::string& t = sim.poissonType;
   if (t == "periodic" || t == "cpu_periodic") return std::make_unique<HIP_periodic>(sim);
   if (t == "freespace" || t == "cpu_freespace") return std::make_unique<HIP_freespace>(sim);
+  if (t == "" || t == "cosine" || t == "cpu_cosine") return std::make_unique<HIP_cosine>(sim);
   throw Error(CUP2D_ERR_ARG, "PoissonSolver::makeSolver: no device solver for '" + t + "'");
 }
 
@@ -23,6 +26,11 @@ void HIP_periodic::solve(const Field& BSRC, const Field& BDST) {
 }
 
 void HIP_freespace::solve(const Field& BSRC, const Field& BDST) {
+  solve_device(sim, BSRC, BDST);
+  iter = 1;
+}
+
+void HIP_cosine::solve(const Field& BSRC, const Field& BDST) {
   solve_device(sim, BSRC, BDST);
   iter = 1;
 }
@@ -79,6 +87,37 @@ void PressureIterator_unif::operator()(const double dt) {
 
 void ComputeForces::operator()(const double) {
   check(sim.ctx, cup2d_compute_forces(sim.ctx), "ComputeForces");
+  if (!sim.muteAll) {
+    sim.refreshShapes();
+    writeLogs(sim);
+  }
+}
+
+void ComputeForces::writeLogs(SimulationData& sim) { writeLogs(sim, sim.time); }
+
+void ComputeForces::writeLogs(SimulationData& sim, double time) {
+  for (const Shape& sh : sim.shapes) {
+    const cup2d_shape_state& t = sh.state;
+    const std::string id = std::to_string(sh.obstacleID);
+    std::ofstream f(sim.path2file + "/forceValues_" + id + ".dat", std::ios::app);
+    std::ofstream p(sim.path2file + "/powerValues_" + id + ".dat", std::ios::app);
+    if (!f || !p) throw Error(CUP2D_ERR_ARG, "cannot write the force logs in " + sim.path2file);
+    f.seekp(0, std::ios::end);
+    if (sim.step == 0 || f.tellp() == 0) {
+      f << "time Fx Fy FxPres FyPres FxVisc FyVisc tau tauPres tauVisc"
+           " drag thrust perimeter circulation area_penal mass_penal"
+           " forcex_penal forcey_penal torque_penal\n";
+      p << "time Pthrust Pdrag PoutBnd Pout defPowerBnd defPower"
+           " EffPDefBnd EffPDef\n";
+    }
+    f << time << " " << t.forcex << " " << t.forcey << " " << t.forcex_P << " " << t.forcey_P
+      << " " << t.forcex_V << " " << t.forcey_V << " " << t.torque << " " << t.torque_P << " "
+      << t.torque_V << " " << t.drag << " " << t.thrust << " " << t.perimeter << " "
+      << t.circulation << "\n";
+    p << time << " " << t.Pthrust << " " << t.Pdrag << " " << t.PoutBnd << " " << t.Pout
+      << " " << t.defPowerBnd << " " << t.defPower << " " << t.EffPDefBnd << " " << t.EffPDef
+      << "\n";
+  }
 }
 
 }  // namespace cup2d

@@ -131,6 +131,7 @@ void Simulation::parseRuntime() {
   sim.verbose = parser.asInt("verbose", 1);
   sim.iterativePenalization = parser.asInt("iterativePenalization", 0);
   sim.muteAll = parser.asInt("muteAll", 0);
+  sim.path2file = parser.asString("file", "./");
   if (sim.muteAll) sim.verbose = 0;
   deviceLoop = parser.asBool("deviceLoop", false);
   if (sim.dlm > 0) throw Error(CUP2D_ERR_ARG, "-dlm > 0 has no device implementation");
@@ -223,7 +224,8 @@ void Simulation::simulate() {
   }
   // whole steps enqueued back to back with dt computed on the device; the
   // stop condition is checked every `chunk` steps (the step count is exact,
-  // -tend may overshoot by up to chunk-1 steps)
+  // -tend may overshoot by up to chunk-1 steps); the force logs get one line
+  // per run of steps instead of one per step
   const int chunk = 50;
   while (!sim.bOver()) {
     int n = chunk;
@@ -232,6 +234,8 @@ void Simulation::simulate() {
     check(sim.ctx, cup2d_synchronize(sim.ctx), "cup2d_synchronize");
     sim.pullScalars();
     sim.refreshShapes();
+    // forces of the last step of the run, computed at its start time
+    if (!sim.muteAll) ComputeForces::writeLogs(sim, sim.time - sim.dt);
     if (sim.verbose)
       std::printf("step:%d, time:%f, dt=%f, uinf:[%f %f]\n", sim.step, sim.time, sim.dt,
                   sim.uinfx, sim.uinfy);

@@ -31,9 +31,11 @@ void SimulationData::allocateGrid() {
     kind = CUP2D_POISSON_PERIODIC;
   else if (poissonType == "freespace" || poissonType == "cpu_freespace")
     kind = CUP2D_POISSON_FREESPACE;
+  else if (poissonType == "" || poissonType == "cosine" || poissonType == "cpu_cosine")
+    kind = CUP2D_POISSON_COSINE;  // the default: FFTW_dirichlet (PoissonSolver.cpp:57-58)
   else
     throw Error(CUP2D_ERR_ARG, "poissonType '" + poissonType +
-                                   "' has no device solver (periodic, freespace)");
+                                   "' has no device solver (periodic, freespace, cosine)");
   cup2d_cfg cfg{};
   cfg.bpdx = bpdx;
   cfg.bpdy = bpdy;

@@ -57,6 +57,7 @@ class SimulationData {
   double fadeLen = 0.01;                 // -fadeLen, fraction of the extent
   double fadeLenX = 0.01, fadeLenY = 0.01;  // fadeLen * extent (Simulation.cpp:97-98)
   int verbose = 1, iterativePenalization = 0, muteAll = 0;
+  std::string path2file = "./";  // -file: directory of the text logs
 
   // state (SimulationData.h:346-350)
   double time = 0, dt = 0, uinfx = 0, uinfy = 0;

@@ -37,7 +37,11 @@ enum cup2d_status {
   CUP2D_ERR_NONFINITE = -6  /* NaN/Inf detected (Helpers.cpp:178-288)    */
 };
 
-enum cup2d_poisson { CUP2D_POISSON_PERIODIC = 0, CUP2D_POISSON_FREESPACE = 1 };
+enum cup2d_poisson {
+  CUP2D_POISSON_PERIODIC = 0,   /* FFTW_periodic (FFTW_periodic.h)                   */
+  CUP2D_POISSON_FREESPACE = 1,  /* FFTW_freespace (FFTW_freespace.cpp)               */
+  CUP2D_POISSON_COSINE = 2      /* FFTW_dirichlet: DCT-II/III, the reference default  */
+};
 enum cup2d_field {
   CUP2D_VEL = 0, CUP2D_CHI, CUP2D_PRES, CUP2D_TMP, CUP2D_TMPV, CUP2D_UDEF, CUP2D_INVRHO,
   CUP2D_VFLUID,          /* sim.vFluid (SimulationData.h:31), iterative penalization only */
@@ -54,7 +58,7 @@ typedef struct {
   double CFL;            /* -CFL                                             */
   double lambda;         /* -lambda (implicit penalization only)             */
   double fadeLen;        /* -fadeLen (fraction of extent)                    */
-  int poisson;           /* -poissonType periodic|freespace                  */
+  int poisson;           /* -poissonType periodic|freespace|cosine            */
   int implicit_penal;    /* 0: EXPL_INTEGRATE_MOM (PressureSingle.cpp:20)    */
   int iterative_penal;   /* -iterativePenalization: 1 = PressureIterator_unif
                             replaces PressureSingle (Simulation.cpp:215-216) */

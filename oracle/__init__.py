@@ -153,7 +153,8 @@ class Oracle:
         self.L = lib()
         lam = 1e3 / CFL if lam is None else lam
         cfg = Cfg(bpdx, bpdy, extent, nu, CFL, lam, fadeLen,
-                  0 if poisson == "periodic" else 1, int(fs_pad2), int(implicit_penal),
+                  {"periodic": 0, "freespace": 1, "cosine": 2}[poisson], int(fs_pad2),
+                  int(implicit_penal),
                   int(checker), int(iterative_penal))
         self.h = self.L.orc_create(C.byref(cfg))
         if not self.h:

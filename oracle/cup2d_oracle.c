@@ -191,6 +191,13 @@ orc_sim* orc_create(const orc_cfg* cfg)
     if (BX(s, b) == 0 || BX(s, b) == s->bpdx - 1 || BY(s, b) == 0 || BY(s, b) == s->bpdy - 1)
       s->bnd[s->nbnd++] = b;
 
+  if (cfg->poisson == 2) { /* FFTW_dirichlet.h:25,70: MX x MY reals, stride MX */
+    s->MX = s->NX; s->MY = s->NY;
+    s->MXh = s->MX / 2 + 1;
+    s->stride = (size_t)s->MX;
+    s->buffer = (double*)calloc(s->stride * (size_t)s->MY, sizeof(double));
+    return s;
+  }
   if (cfg->poisson == 0) { /* FFTW_periodic.h:25,96-97 */
     s->MX = s->NX; s->MY = s->NY;
   } else { /* FFTW_freespace.h:25-27 (2N-1), or 2N */
@@ -908,8 +915,85 @@ static void spectral(orc_sim* s)
   }
 }
 
+/* FFTW's REDFT10 (DCT-II, Y_k = 2 sum_j x_j cos(pi (j+1/2) k / n)) or REDFT01
+ * (DCT-III, y_j = X_0 + 2 sum_{k>=1} X_k cos(pi k (j+1/2) / n)) of n values at
+ * stride `stride`, by the direct sums (the FFTW documentation's definitions). */
+static void dct_direct(double* x, int n, size_t stride, int type3, const double* cs, double* tmp)
+{
+  for (int k = 0; k < n; ++k) {
+    double acc = 0;
+    if (!type3) {
+      for (int j = 0; j < n; ++j) acc += x[(size_t)j * stride] * cs[(size_t)k * n + j];
+      tmp[k] = 2 * acc;
+    } else {
+      for (int m = 1; m < n; ++m) acc += x[(size_t)m * stride] * cs[(size_t)m * n + k];
+      tmp[k] = x[0] + 2 * acc;
+    }
+  }
+  for (int k = 0; k < n; ++k) x[(size_t)k * stride] = tmp[k];
+}
+
+/* cs[k n + j] = cos(pi (j + 1/2) k / n) */
+static double* dct_table(int n)
+{
+  double* cs = (double*)malloc(sizeof(double) * (size_t)n * n);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j) cs[(size_t)k * n + j] = cos(M_PI * (j + 0.5) * k / n);
+  return cs;
+}
+
+/* 2D r2r transform of the MY x MX buffer (rows, then columns) */
+static void dct2d(orc_sim* s, int type3)
+{
+  const int MX = s->MX, MY = s->MY;
+  double* csx = dct_table(MX);
+  double* csy = dct_table(MY);
+  #pragma omp parallel
+  {
+    double* tmp = (double*)malloc(sizeof(double) * (size_t)(MX > MY ? MX : MY));
+    #pragma omp for schedule(static)
+    for (int j = 0; j < MY; ++j) dct_direct(s->buffer + (size_t)j * MX, MX, 1, type3, csx, tmp);
+    #pragma omp for schedule(static)
+    for (int i = 0; i < MX; ++i) dct_direct(s->buffer + i, MY, (size_t)MX, type3, csy, tmp);
+    free(tmp);
+  }
+  free(csx);
+  free(csy);
+}
+
+/* FFTW_dirichlet::_solve (FFTW_dirichlet.h:45-61).  Line 51 declares the
+ * blend weight as `static constexpr tol = 0.1;`, which has no type; it is
+ * taken as the double 0.1 the initializer states.  COScoef are floats
+ * (FFTW_dirichlet.h:26-27,65-68). */
+static void cosine_spectral(orc_sim* s)
+{
+  const int MX = s->MX, MY = s->MY;
+  const double waveFactX = M_PI / MX, waveFactY = M_PI / MY;
+  const double tol = 0.1;
+  const double norm_factor = 0.25 / ((double)MX * MY);
+  double* io = s->buffer;
+  #pragma omp parallel for schedule(static)
+  for (int j = 0; j < MY; ++j)
+    for (int i = 0; i < MX; ++i) {
+      const float cx = (float)cos(M_PI / MX * 2.0 * i), cy = (float)cos(M_PI / MY * 2.0 * j);
+      const double rkx = (i + 0.5) * waveFactX, rky = (j + 0.5) * waveFactY;
+      const double denomFD = 1 - (double)cx / 2 - (double)cy / 2;
+      const double denomSP = rkx * rkx + rky * rky;
+      io[(size_t)j * MX + i] *= -norm_factor / ((1 - tol) * denomFD + tol * denomSP);
+    }
+  io[0] = 0;
+}
+
 static void fft_solve_buffer(orc_sim* s)
 {
+  if (s->cfg.poisson == 2) {
+    const double t0 = omp_get_wtime();
+    dct2d(s, 0);
+    cosine_spectral(s);
+    dct2d(s, 1);
+    s->t_fft += omp_get_wtime() - t0;
+    return;
+  }
   const double t0 = omp_get_wtime();
   fft2d_exec(&s->fft, s->buffer, -1, s->cfg.poisson == 0 ? s->MY : s->NY);
   spectral(s);
@@ -920,7 +1004,7 @@ static void fft_solve_buffer(orc_sim* s)
 /* PoissonSolver::solve (FFTW_periodic.h:135-166, FFTW_freespace.cpp:14-56) */
 void orc_poisson_solve(orc_sim* s)
 {
-  if (s->cfg.poisson != 0) memset(s->buffer, 0, sizeof(double) * s->stride * (size_t)s->MY);
+  if (s->cfg.poisson == 1) memset(s->buffer, 0, sizeof(double) * s->stride * (size_t)s->MY);
   checker(s);
   orc_cub2rhs(s);
   checker(s);

@@ -26,7 +26,8 @@ typedef struct {
   double CFL;            /* -CFL                                             */
   double lambda;         /* -lambda (read only by the implicit penalization) */
   double fadeLen;        /* -fadeLen, fraction of extent (Simulation.cpp:97) */
-  int poisson;           /* 0 = periodic (FFTW_periodic), 1 = freespace      */
+  int poisson;           /* 0 = periodic (FFTW_periodic), 1 = freespace,     */
+                         /* 2 = cosine (FFTW_dirichlet, DCT-II / DCT-III)    */
   int fs_pad2;           /* freespace: 0 = (2N-1) cyclic like the reference, */
                          /*            1 = 2N cyclic (same math)             */
   int implicit_penal;    /* 0 = EXPL_INTEGRATE_MOM (reference), 1 = implicit */

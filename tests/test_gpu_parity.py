@@ -348,3 +348,23 @@ def test_pressure_iterator_entry_point(cup, oracle_mod):
     assert rel(g.get_field(0), o.get_field(0)) < 1e-10
     assert rel(g.get_field(2), o.get_field(2)) < 1e-10
     assert rel(g.get_field(cup.VFLUID), o.get_field(oracle_mod.VFLUID)) < 1e-10
+
+
+@pytest.mark.parametrize("bp", [(4, 2), (16, 8), (32, 16), (8, 8)])
+def test_poisson_cosine_vs_oracle(cup, oracle_mod, bp):
+    """Cosine (DCT) solver: the device runs the mirrored-grid FFT, the oracle the
+    direct DCT sums pinned against scipy.fft.dct."""
+    rng = np.random.default_rng(26)
+    g = cup.Simulation(*bp, extent=1.0, poisson="cosine")
+    o = oracle_mod.Oracle(*bp, extent=1.0, poisson="cosine")
+    r = rng.standard_normal((g.NY, g.NX))
+    pg, po = g.poisson_dense(r), o.poisson_dense(r)
+    assert rel(pg, po) < 1e-12
+
+
+def test_towed_cylinder_cosine(cup, oracle_mod):
+    """Whole steps with the reference's default solver (poissonType "")."""
+    shapes = [dict(radius=0.05, xpos=0.25, ypos=0.25, xvel=0.2, bFixed=True, bForced=True)]
+    g, o = _disk_pair(cup, oracle_mod, (16, 8), shapes, extent=1.0, nu=2e-5, CFL=0.1,
+                      poisson="cosine")
+    _run_and_compare(g, o, 12)

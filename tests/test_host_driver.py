@@ -38,7 +38,7 @@ def test_driver_requires_bpdx():
 
 def test_driver_rejects_solver_without_device_path():
     _need_driver()
-    r = _run(["-bpdx", "2", "-bpdy", "2", "-poissonType", "cosine"])
+    r = _run(["-bpdx", "2", "-bpdy", "2", "-poissonType", "hypre"])
     assert r.returncode == 2
     assert "no device solver" in r.stderr
 
@@ -91,3 +91,23 @@ def test_driver_towed_cylinder_matches_oracle(oracle_mod, tmp_path, device_loop)
     scale = max(abs(so["drag"]), abs(so["forcex"]))
     assert abs(sh["drag"] - so["drag"]) <= 1e-8 * scale
     assert abs(sh["forcex"] - so["forcex"]) <= 1e-8 * scale
+
+
+@pytest.mark.gpu
+def test_driver_force_logs(tmp_path):
+    """Without -muteAll the driver appends forceValues_<id>.dat and
+    powerValues_<id>.dat with the reference's header and columns
+    (Shape.cpp:305-328): one line per step, time before the step's increment."""
+    _need_driver()
+    args = [a for a in C1_SMALL if a not in ("-muteAll", "1")]
+    r = _run(args + ["-nsteps", "4", "-verbose", "0", "-file", str(tmp_path)])
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    lines = (tmp_path / "forceValues_0.dat").read_text().splitlines()
+    assert lines[0].startswith("time Fx Fy FxPres")
+    rows = [list(map(float, l.split())) for l in lines[1:]]
+    assert len(rows) == 5 and all(len(x) == 14 for x in rows)
+    assert rows[0][0] == 0.0
+    assert abs(rows[-1][10] - out["shapes"][0]["drag"]) <= 1e-5 * abs(out["shapes"][0]["drag"])
+    plines = (tmp_path / "powerValues_0.dat").read_text().splitlines()
+    assert plines[0].startswith("time Pthrust") and len(plines) == 6

@@ -333,3 +333,27 @@ def test_pressure_iterator_converges_on_towed_cylinder(oracle_mod):
     assert inside.sum() > 100
     # solid cells: tmpV + (U_s - vFluid) + pFac grad p; converged -> close to U_s
     assert np.abs(u[inside] - (-0.2)).max() < 1e-2
+
+
+def test_cosine_poisson_matches_scipy_dct(oracle_mod):
+    """FFTW_dirichlet (FFTW_dirichlet.h:45-61,118-125): REDFT10 in both
+    directions, the blended FD/spectral symbol with float COScoef, DC = 0,
+    REDFT01 back, recomputed with scipy.fft.dct (unnormalized types II and III
+    are FFTW's REDFT10 and REDFT01)."""
+    import scipy.fft as F
+    o = oracle_mod.Oracle(4, 2, extent=1.0, poisson="cosine")
+    rng = np.random.default_rng(25)
+    r = rng.standard_normal((o.NY, o.NX))
+    p = o.poisson_dense(r)
+    MX, MY = o.NX, o.NY
+    Y = F.dct(F.dct(r, type=2, axis=1), type=2, axis=0)
+    i = np.arange(MX)[None, :]
+    j = np.arange(MY)[:, None]
+    cx = np.cos(np.pi / MX * 2.0 * i).astype(np.float32).astype(np.float64)
+    cy = np.cos(np.pi / MY * 2.0 * j).astype(np.float32).astype(np.float64)
+    rkx, rky = (i + 0.5) * np.pi / MX, (j + 0.5) * np.pi / MY
+    den = 0.9 * (1 - cx / 2 - cy / 2) + 0.1 * (rkx ** 2 + rky ** 2)
+    Y = Y * (-0.25 / (MX * MY) / den)
+    Y[0, 0] = 0
+    ref = F.dct(F.dct(Y, type=3, axis=1), type=3, axis=0)
+    assert np.abs(p - ref).max() <= 1e-13 * np.abs(ref).max()
