@@ -3,7 +3,9 @@
 Default workload: periodic 2.0 x 1.0 domain at 4096 x 2048 cells (128 x 64
 blocks of 32x32), nu = 1e-5, CFL 0.1, FP64, initial velocity from a seeded
 Gaussian random vorticity (sigma 1e-3) made divergence-free by one projection,
-one free disk (r = 0.04) as the immersed body.  A step is the reference's
+one free carangiform swimmer (CarlingFish midline kinematics, length 0.2,
+lateral amplitude envelope L (0.02 + 0.08 (s/L)^2), period 1.0, wavelength 1.0 L,
+NACA 0008 half-width profile) as the immersed body.  A step is the reference's
 calcMaxTimestep + advance: PutObjectsOnGrid, advDiff, PressureSingle (RHS,
 FFT Poisson, projection, body update, penalization) and ComputeForces, all
 enqueued on the device with dt computed on the device.
@@ -36,10 +38,14 @@ BALG_FREESPACE = dict(BALG, fft_rows_fwd=4, fft_cols=5, fft_rows_inv=3)
 STEP_BALG_METRIC = {"periodic": 18, "freespace": 23}
 FULL_GRID = list(BALG)
 
+SWIMMER = dict(kind="carlingfish", xpos=1.0, ypos=0.5, L=0.2, T=1.0, envelope="quadratic",
+               env_a0=0.02, env_a2=0.08, waveLength=1.0, profile="naca", naca_t=0.08)
 CONFIGS = {
     # name: (bpdx, bpdy, extent, nu, poisson, body)
     "periodic4096": dict(bpdx=128, bpdy=64, extent=2.0, nu=1e-5, poisson="periodic",
-                         body=dict(radius=0.04, xpos=1.0, ypos=0.5)),
+                         body=SWIMMER),
+    "periodic4096_disk": dict(bpdx=128, bpdy=64, extent=2.0, nu=1e-5, poisson="periodic",
+                              body=dict(radius=0.04, xpos=1.0, ypos=0.5)),
     "cylinder512": dict(bpdx=16, bpdy=8, extent=1.0, nu=2e-5, poisson="freespace",
                         body=dict(radius=0.05, xpos=0.25, ypos=0.25, xvel=0.2, bFixed=True,
                                   bForced=True)),
@@ -71,11 +77,27 @@ def setup(sim, vel):
     sim.pressure_correction(1.0)
 
 
+def make_shape(mod, body):
+    """Shape descriptor of the workload's body for the library or the oracle."""
+    b = dict(body)
+    if b.pop("kind", "disk") == "carlingfish":
+        return mod.carlingfish(**b)
+    return mod.disk(**b)
+
+
+def body_text(body):
+    if body.get("kind") == "carlingfish":
+        return (f"one free carlingfish L={body['L']} T={body['T']}, envelope "
+                f"L({body['env_a0']}+{body['env_a2']}(s/L)^2), wavelength "
+                f"{body['waveLength']}L, NACA t={body['naca_t']}")
+    return f"one disk r={body['radius']}"
+
+
 def cpu_baseline(cfg, vel, steps):
     import oracle
     o = oracle.Oracle(cfg["bpdx"], cfg["bpdy"], extent=cfg["extent"], nu=cfg["nu"], CFL=0.1,
                       poisson=cfg["poisson"], fs_pad2=True,
-                      shapes=[oracle.disk(**cfg["body"])])
+                      shapes=[make_shape(oracle, cfg["body"])])
     setup(o, vel)
     o.simulate(2)  # warm-up
     t0 = time.perf_counter()
@@ -183,7 +205,7 @@ def main():
     import cubismup_2d_amd as cup
     cfg = CONFIGS[args.config]
     sim = cup.Simulation(cfg["bpdx"], cfg["bpdy"], extent=cfg["extent"], nu=cfg["nu"], CFL=0.1,
-                         poisson=cfg["poisson"], shapes=[cup.disk(**cfg["body"])])
+                         poisson=cfg["poisson"], shapes=[make_shape(cup, cfg["body"])])
     NX, NY = sim.NX, sim.NY
     h = sim.scalars()["h"]
     vel = random_vorticity_velocity(NX, NY, h)
@@ -234,7 +256,7 @@ def main():
         "data": "synthetic: seeded random-vorticity initial velocity",
         "config": {"workload": f"{args.config}: {NX}x{NY} cells ({cfg['bpdx']}x{cfg['bpdy']} "
                                f"blocks), {cfg['poisson']} FFT Poisson, nu={cfg['nu']}, CFL 0.1, "
-                               f"one disk r={cfg['body']['radius']}",
+                               f"{body_text(cfg['body'])}",
                    "cells": cells, "precision": "fp64", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),

@@ -44,8 +44,15 @@ class ShapeDesc(C.Structure):
                 ("bFixedx", C.c_int), ("bFixedy", C.c_int), ("bForcedx", C.c_int),
                 ("bForcedy", C.c_int), ("bBlockAng", C.c_int), ("xvel", C.c_double),
                 ("yvel", C.c_double), ("angvel", C.c_double), ("timeForced", C.c_double),
-                ("tAccel", C.c_double)]
+                ("tAccel", C.c_double), ("length", C.c_double), ("Tperiod", C.c_double),
+                ("phaseShift", C.c_double), ("amplitudeFactor", C.c_double),
+                ("envelope", C.c_int), ("env_a0", C.c_double), ("env_a2", C.c_double),
+                ("waveLength", C.c_double), ("profile", C.c_int), ("naca_t", C.c_double)]
 
+
+FISH_ARRAYS = ["rS", "rX", "rY", "vX", "vY", "norX", "norY", "vNorX", "vNorY", "width"]
+FISH_SCALARS = ["area_internal", "J_internal", "CoM_internal_x", "CoM_internal_y",
+                "vCoM_internal_x", "vCoM_internal_y", "theta_internal", "angvel_internal"]
 
 SHAPE_FIELDS = ["center", "centerOfMass", "labCenterOfMass", "d_gm", "orientation",
                 "M", "J", "u", "v", "omega",
@@ -68,7 +75,8 @@ class ShapeState(C.Structure):
 EXPORTS = [
     "cup2d_create", "cup2d_destroy", "cup2d_last_error", "cup2d_add_shape",
     "cup2d_upload_field", "cup2d_download_field", "cup2d_field_device_ptr",
-    "cup2d_get_scalars", "cup2d_set_scalars", "cup2d_get_shape", "cup2d_reset",
+    "cup2d_get_scalars", "cup2d_set_scalars", "cup2d_get_shape", "cup2d_fish_midline",
+    "cup2d_reset",
     "cup2d_find_max_u", "cup2d_calc_max_timestep", "cup2d_put_objects", "cup2d_advdiff",
     "cup2d_pressure_rhs", "cup2d_poisson_solve", "cup2d_pressure_correction",
     "cup2d_body_update", "cup2d_penalize", "cup2d_pressure_single", "cup2d_pressure_iterator",
@@ -105,6 +113,7 @@ def lib():
     L.cup2d_get_scalars.argtypes = [P, dp, dp, C.POINTER(I), dp, dp]
     L.cup2d_set_scalars.argtypes = [P, D, I, D, D]
     L.cup2d_get_shape.argtypes = [P, I, C.POINTER(ShapeState)]
+    L.cup2d_fish_midline.argtypes = [P, I, C.POINTER(I), dp]
     for n in ["cup2d_reset", "cup2d_poisson_solve", "cup2d_compute_forces",
               "cup2d_synchronize"]:
         getattr(L, n).argtypes = [P]
@@ -144,6 +153,27 @@ def disk(radius, xpos, ypos, xvel=0.0, yvel=0.0, angvel=0.0, rhoS=1.0, angle=0.0
                 bFixedx=int(bFixedx), bFixedy=int(bFixedy), bForcedx=int(bForcedx),
                 bForcedy=int(bForcedy), bBlockAng=int(bBlockAng), xvel=xvel, yvel=yvel,
                 angvel=angvel, timeForced=timeForced, tAccel=tAccel)
+
+
+def carlingfish(xpos, ypos, L=0.1, T=1.0, phi=0.0, angle=0.0, amplitudeFactor=1.0,
+                envelope="carling", env_a0=0.02, env_a2=0.08, waveLength=1.0,
+                profile="carling", naca_t=0.08, rhoS=1.0, bFixed=False, bForced=False,
+                bFixedx=None, bFixedy=None, bForcedx=None, bForcedy=None, bBlockAng=None,
+                xvel=0.0, yvel=0.0, angvel=0.0, timeForced=1.7976931348623157e308):
+    """CarlingFish descriptor (-L -T -phi -amplitudeFactor, Fish.h:27-29,
+    CarlingFish.cpp:371-376) with the reference's flag defaults
+    (Shape.cpp:331-345).  envelope "carling" is the reference's lateral
+    amplitude 4/33 (s + L/32) with wavelength L; "quadratic" is
+    L (env_a0 + env_a2 (s/L)^2) with wavelength waveLength*L.  profile
+    "carling" is AmplitudeFish::_width; "naca" is a NACA 4-digit half-thickness
+    of thickness ratio naca_t."""
+    d = disk(0.0, xpos, ypos, xvel=xvel, yvel=yvel, angvel=angvel, rhoS=rhoS, angle=angle,
+             bFixed=bFixed, bForced=bForced, bFixedx=bFixedx, bFixedy=bFixedy,
+             bForcedx=bForcedx, bForcedy=bForcedy, bBlockAng=bBlockAng, timeForced=timeForced)
+    d.update(kind=1, length=L, Tperiod=T, phaseShift=phi, amplitudeFactor=amplitudeFactor,
+             envelope={"carling": 0, "quadratic": 1}[envelope], env_a0=env_a0, env_a2=env_a2,
+             waveLength=waveLength, profile={"carling": 0, "naca": 1}[profile], naca_t=naca_t)
+    return d
 
 
 class Simulation:
@@ -214,6 +244,20 @@ class Simulation:
 
     def set_scalars(self, time=0.0, step=0, uinf=(0.0, 0.0)):
         self._chk(self.L.cup2d_set_scalars(self.h, time, step, uinf[0], uinf[1]))
+
+    def fish_midline(self, i=0):
+        """Midline arrays and internal-frame scalars of swimmer i (same keys as
+        the oracle's fish_info)."""
+        n = C.c_int()
+        self._chk(self.L.cup2d_fish_midline(self.h, i, C.byref(n), None))
+        Nm = n.value
+        out = np.empty(10 * Nm + 8)
+        self._chk(self.L.cup2d_fish_midline(self.h, i, C.byref(n),
+                                            out.ctypes.data_as(C.POINTER(C.c_double))))
+        d = {k: out[j * Nm:(j + 1) * Nm].copy() for j, k in enumerate(FISH_ARRAYS)}
+        d.update({k: float(out[10 * Nm + j]) for j, k in enumerate(FISH_SCALARS)})
+        d["Nm"] = Nm
+        return d
 
     def shape(self, i=0):
         s = ShapeState()

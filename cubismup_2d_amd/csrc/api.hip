@@ -239,6 +239,7 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
   g.fadeLenY = cfg->fadeLen * cfg->extent;
   g.implicit_penal = cfg->implicit_penal;
   g.nshapes = 0;
+  g.nfish = 0;
   int rc = CUP2D_OK;
   auto bail = [&](int code) {
     g_create_error = c->err;
@@ -302,6 +303,8 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
       hipMalloc(&c->maxpart, sizeof(double) * c->maxpart_n) != hipSuccess ||
       hipMalloc(&c->obmap, (size_t)g.bpdx * g.bpdy) != hipSuccess ||
       hipMemset(c->obmap, 0, (size_t)g.bpdx * g.bpdy) != hipSuccess ||
+      hipMalloc(&c->ubidx, sizeof(int) * g.bpdx * g.bpdy) != hipSuccess ||
+      hipMemset(c->ubidx, 0xff, sizeof(int) * g.bpdx * g.bpdy) != hipSuccess ||
       hipMemset(c->fluxpart, 0, sizeof(double) * c->nwg) != hipSuccess ||
       hipMemset(c->maxpart, 0, sizeof(double) * c->maxpart_n) != hipSuccess) {
     c->err = "out of device memory";
@@ -332,6 +335,8 @@ int cup2d_destroy(cup2d_ctx* c) {
                     c->udx, c->udy, c->invrho, c->staging, c->red, c->ramp,
                     c->fluxpart, c->maxpart};
   if (c->obmap) (void)hipFree(c->obmap);
+  if (c->ubidx) (void)hipFree(c->ubidx);
+  fish_free_all(c);
   for (double* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->st) (void)hipFree(c->st);
@@ -354,11 +359,22 @@ int cup2d_add_shape(cup2d_ctx* c, const cup2d_shape_desc* d, int* shape_id) {
   CHECK_CTX(c);
   if (!d) return fail(c, CUP2D_ERR_ARG, "null shape");
   if (c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "shapes must be added before reset");
-  if (d->kind != 0) return fail(c, CUP2D_ERR_ARG, "only disk shapes (kind 0) are supported");
-  if (!(d->radius > 0)) return fail(c, CUP2D_ERR_ARG, "radius must be > 0");
+  if (d->kind != 0 && d->kind != 1)
+    return fail(c, CUP2D_ERR_ARG, "shape kind must be 0 (disk) or 1 (carlingfish)");
+  if (d->kind == 0 && !(d->radius > 0)) return fail(c, CUP2D_ERR_ARG, "radius must be > 0");
+  if (d->kind == 1) {
+    if (!(d->length > 0) || !(d->Tperiod > 0))
+      return fail(c, CUP2D_ERR_ARG, "carlingfish needs L > 0 and T > 0");
+    if (d->envelope != 0 && d->envelope != 1) return fail(c, CUP2D_ERR_ARG, "envelope must be 0 or 1");
+    if (d->envelope == 1 && !(d->waveLength > 0))
+      return fail(c, CUP2D_ERR_ARG, "waveLength must be > 0");
+    if (d->profile != 0 && d->profile != 1) return fail(c, CUP2D_ERR_ARG, "profile must be 0 or 1");
+    if (d->profile == 1 && !(d->naca_t > 0)) return fail(c, CUP2D_ERR_ARG, "naca_t must be > 0");
+  }
   if (c->g.nshapes >= MAX_SHAPES) return fail(c, CUP2D_ERR_ARG, "too many shapes");
   ShapeParams p{};
   p.kind = d->kind;
+  p.fish = -1;
   p.bFixedx = d->bFixedx; p.bFixedy = d->bFixedy;
   p.bForcedx = d->bForcedx; p.bForcedy = d->bForcedy; p.bBlockAng = d->bBlockAng;
   p.radius = d->radius; p.rhoS = d->rhoS;
@@ -368,6 +384,18 @@ int cup2d_add_shape(cup2d_ctx* c, const cup2d_shape_desc* d, int* shape_id) {
   p.origC[0] = d->xpos; p.origC[1] = d->ypos;
   p.origAng = d->angle_deg * M_PI / 180;
   const int id = c->g.nshapes;
+  if (d->kind == 1) {
+    // radius: distance from the swimmer's centre within which it always
+    // lies (sizes the obstacle-block pool only)
+    const double amp = d->envelope == 0
+                           ? 4. / 33. * (d->length + 0.03125 * d->length)
+                           : d->length * std::max(std::fabs(d->env_a0), std::fabs(d->env_a0 + d->env_a2));
+    const double wmax = d->profile == 0 ? 0.04 * d->length : 0.6 * d->naca_t * d->length;
+    p.radius = d->length + amp + wmax;
+    p.fish = (int)c->hfish.size();
+    const int rc = fish_add(c, d, id);
+    if (rc) return rc;
+  }
   CUP2D_HIP_CHECK(c, hipMemcpy(c->sp + id, &p, sizeof(p), hipMemcpyHostToDevice));
   c->hsp.push_back(p);
   c->g.nshapes++;
@@ -462,6 +490,23 @@ int cup2d_get_shape(cup2d_ctx* c, int i, cup2d_shape_state* out) {
   if (i < 0 || i >= c->g.nshapes || !out) return fail(c, CUP2D_ERR_ARG, "bad shape id");
   RC(sync_all(c));
   CUP2D_HIP_CHECK(c, hipMemcpy(out, c->ss + i, sizeof(*out), hipMemcpyDeviceToHost));
+  return CUP2D_OK;
+}
+
+int cup2d_fish_midline(cup2d_ctx* c, int i, int* Nm, double* out) {
+  CHECK_CTX(c);
+  if (i < 0 || i >= c->g.nshapes || !Nm) return fail(c, CUP2D_ERR_ARG, "bad shape id");
+  const int f = c->hsp[i].fish;
+  if (f < 0) return fail(c, CUP2D_ERR_ARG, "shape is not a swimmer");
+  const FishDev& F = c->hfish[f];
+  *Nm = F.Nm;
+  if (!out) return CUP2D_OK;
+  RC(sync_all(c));
+  const size_t n = (size_t)F.Nm;
+  CUP2D_HIP_CHECK(c, hipMemcpy(out, F.rS, sizeof(double) * n, hipMemcpyDeviceToHost));
+  CUP2D_HIP_CHECK(c, hipMemcpy(out + n, F.mid, sizeof(double) * 8 * n, hipMemcpyDeviceToHost));
+  CUP2D_HIP_CHECK(c, hipMemcpy(out + 9 * n, F.width, sizeof(double) * n, hipMemcpyDeviceToHost));
+  CUP2D_HIP_CHECK(c, hipMemcpy(out + 10 * n, F.st, sizeof(double) * 8, hipMemcpyDeviceToHost));
   return CUP2D_OK;
 }
 

@@ -12,6 +12,32 @@ namespace cup2d {
 __host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
 __host__ __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
 
+// sin and cos on the body path: Cody-Waite reduction by pi/2 (three-part
+// constant, the first two parts with 33 significant bits so k*P1 and k*P2 are
+// exact for |k| < 2^20) and Taylor polynomials on [-pi/4, pi/4] (sine to
+// r^17, cosine to r^18; truncation below 1e-17).  Plain IEEE operations only,
+// so host and device (and any CPU restatement written the same way) round
+// identically; accuracy is about one ulp.
+__host__ __device__ inline void det_sincos(double x, double* sn, double* cs) {
+  const double P1 = 0x1.921fb544p+0, P2 = 0x1.0b4611a6p-34, P3 = 0x1.3198a2e037073p-69;
+  const double k = floor(x * 0x1.45f306dc9c883p-1 + 0.5);
+  const double r = ((x - k * P1) - k * P2) - k * P3;
+  const double z = r * r;
+  const double sp = r + (r * z) * (-1.0 / 6 + z * (1.0 / 120 + z * (-1.0 / 5040 + z * (1.0 / 362880 +
+                    z * (-1.0 / 39916800 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0 +
+                    z * (1.0 / 355687428096000.0))))))));
+  const double cp = 1 + z * (-0.5 + z * (1.0 / 24 + z * (-1.0 / 720 + z * (1.0 / 40320 +
+                    z * (-1.0 / 3628800 + z * (1.0 / 479001600 + z * (-1.0 / 87178291200.0 +
+                    z * (1.0 / 20922789888000.0 + z * (-1.0 / 6402373705728000.0)))))))));
+  const int q = (int)(k - 4 * floor(k * 0.25));
+  if (q == 0) { *sn = sp; *cs = cp; }
+  else if (q == 1) { *sn = cp; *cs = -sp; }
+  else if (q == 2) { *sn = -sp; *cs = -cp; }
+  else { *sn = -cp; *cs = sp; }
+}
+__host__ __device__ inline double det_sin(double x) { double a, b; det_sincos(x, &a, &b); return a; }
+__host__ __device__ inline double det_cos(double x) { double a, b; det_sincos(x, &a, &b); return b; }
+
 // Cubism BlockInfo::pos: origin + h*(i+0.5), origin = block index * hb.
 __host__ __device__ inline double cell_pos(int blk_index, int i, double hb, double h) {
   return (double)blk_index * hb + h * (i + 0.5);

@@ -21,10 +21,14 @@ namespace cup2d {
 constexpr int BS = CUP2D_BS;
 constexpr int MAX_SHAPES = 64;
 constexpr int NPART = 24;  // doubles of partial sums per obstacle slot
+constexpr int MAX_FISH = 8;
+constexpr int FISH_MAX_NM = 4096;  // midline points per swimmer
+constexpr int FISH_SEGD = 16;      // doubles per AreaSegment record
 
 // Constant per-shape parameters (Shape.h:22-41, ShapesSimple.h:16-17).
 struct ShapeParams {
-  int kind;
+  int kind;        // 0 = disk, 1 = carlingfish
+  int fish;        // index into the swimmer table, -1 for other kinds
   int bFixedx, bFixedy, bForcedx, bForcedy, bBlockAng;
   double radius, rhoS;
   double forcedu, forcedv, forcedomega, timeForced, tAccel;
@@ -56,7 +60,20 @@ struct Grid {
   int NX, NY, bpdx, bpdy;
   double h, hb, extent0, extent1;
   double nu, CFL, lambda, fadeLenX, fadeLenY;
-  int implicit_penal, nshapes;
+  int implicit_penal, nshapes, nfish;
+};
+
+// One swimmer (FishData, FishLibrary.h:41-147, with the Fish internal-frame
+// state, Fish.h:19-25).  Arrays are device memory.
+struct FishDev {
+  int shape, Nm, Nseg, envelope;
+  double length, Tperiod, phaseShift, env_a0, env_a2, waveLength;
+  double* rS;     // [Nm] arclength of the midline points (constant)
+  double* width;  // [Nm] half-width (constant)
+  double* mid;    // [8][Nm] rX rY vX vY norX norY vNorX vNorY of this step
+  double* seg;    // [Nseg][FISH_SEGD] AreaSegment: safe, lab box, object box, |normalI|, |normalJ|
+  double* box;    // [4] union of the segments' lab boxes (x0 x1 y0 y1)
+  double* st;     // [8] area J CoM_internal[2] vCoM_internal[2] theta_internal angvel_internal
 };
 
 // Obstacle pool (ObstacleBlock.h:266-378, one slot per (shape, block)).
@@ -148,6 +165,9 @@ struct cup2d_ctx {
   // (ob_prev), swapped at every PutObjectsOnGrid: ComputeForces of a step
   // reads its lists on the side stream while the next step builds new ones.
   cup2d::Obstacles ob{}, ob_prev{};
+  int* ubidx = nullptr;                    // [blocks] distinct-block index this step, -1 if none
+  cup2d::FishDev* fishd = nullptr;         // device [MAX_FISH]
+  std::vector<cup2d::FishDev> hfish;       // host copy (device pointers)
   int band_nblocks = 0;
   int* band_blocks = nullptr;              // blocks touched by the cub2rhs fade
   cup2d::FFTPlan* fft = nullptr;
@@ -222,6 +242,15 @@ int enqueue_penalize(cup2d_ctx* c, double* maxpart);
 int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux = 0);
 int enqueue_shapes_reset(cup2d_ctx* c);
 int enqueue_obst_reset(cup2d_ctx* c);
+
+// fish.hip
+// FishData::FishData + width table (FishLibrary.cpp:14-38); returns Nm or -1
+int fish_discretize(const cup2d_shape_desc* d, double h, std::vector<double>& rS,
+                    std::vector<double>& width);
+int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape);
+void fish_free_all(cup2d_ctx* c);
+int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s);
+int enqueue_fish_place(cup2d_ctx* c, hipStream_t s);
 // PressureIterator_unif pieces (PressureIterator_unif.cpp)
 int enqueue_copy_vel_to_tmpv(cup2d_ctx* c);
 int enqueue_iter_project(cup2d_ctx* c);

@@ -14,6 +14,7 @@
 // shape total recomputes it from the partials in the same order, so all of
 // them see bit-identical values.
 #include "device_util.hpp"
+#include "fish.hpp"
 
 namespace cup2d {
 
@@ -101,15 +102,22 @@ __device__ int block_exscan(int* a, int n, int* tmp) {
 // domain check.  (3) Obstacle slots: blocks whose cell-centre box meets a
 // shape's bounding box (FillBlocks_Cylinder::is_touching), in shape order
 // and blockID order, then the distinct blocks with their slots in shape order.
+//
+// phase 0 does everything; with swimmers present phase 1 (dt, motion) runs
+// first, then the swimmers' midline kernel, then phase 2 (lists), whose
+// candidate test for a swimmer is AreaSegment::isIntersectingWithAABB over
+// its segments (Fish.cpp:158-173).
 __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __restrict__ sp,
                                                    cup2d_shape_state* __restrict__ ss,
                                                    StepState* __restrict__ st, Obstacles ob,
                                                    Obstacles pv,
                                                    unsigned char* __restrict__ obmap,
+                                                   int* __restrict__ ubidx,
+                                                   const FishDev* __restrict__ fd,
                                                    const double* __restrict__ maxpart,
                                                    int nmaxpart,
                                                    const double* __restrict__ ramp,
-                                                   int compute_dt, int step_idx) {
+                                                   int compute_dt, int step_idx, int phase) {
   __shared__ int s_a[LCAP], s_b[LCAP], s_c[LCAP], s_d[LCAP];
   __shared__ int s_tmp[NT];
   __shared__ int s_cstart[MAX_SHAPES + 1];
@@ -119,7 +127,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
   const int t = threadIdx.x;
   const int ns = g.nshapes;
   const int nub_prev = pv.cnt[1];
-  if (compute_dt) {
+  if (compute_dt && phase != 2) {
     const double maxU = block_max<NT>(max_array<NT>(maxpart, nmaxpart + nub_prev), lds);
     if (t == 0) {
       const double h = g.h;
@@ -134,9 +142,13 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       st->maxU = maxU;
     }
   }
-  for (int u = t; u < nub_prev; u += NT) obmap[pv.ub_block[u]] = 0;
+  if (phase != 1)
+    for (int u = t; u < nub_prev; u += NT) {
+      obmap[pv.ub_block[u]] = 0;
+      ubidx[pv.ub_block[u]] = -1;
+    }
   __syncthreads();
-  if (t == 0) {
+  if (t == 0 && phase != 2) {
     const double dt = st->dt;
     int nSum[2] = {0, 0};
     double uSum[2] = {0, 0};
@@ -147,7 +159,6 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
     if (nSum[0] > 0) st->uinf[0] = uSum[0] / nSum[0];
     if (nSum[1] > 0) st->uinf[1] = uSum[1] / nSum[1];
     const double U0 = st->uinf[0], U1 = st->uinf[1];
-    int nc = 0;
     for (int k = 0; k < ns; ++k) {
       cup2d_shape_state& T = ss[k];
       T.centerOfMass[0] += dt * (T.u + U0);
@@ -157,15 +168,27 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       T.orientation += dt * T.omega;
       T.orientation = T.orientation > M_PI ? T.orientation - 2 * M_PI : T.orientation;
       T.orientation = T.orientation < -M_PI ? T.orientation + 2 * M_PI : T.orientation;
-      const double cosang = cos(T.orientation), sinang = sin(T.orientation);
+      double sinang, cosang;
+      det_sincos(T.orientation, &sinang, &cosang);
       T.center[0] = T.centerOfMass[0] + cosang * T.d_gm[0] - sinang * T.d_gm[1];
       T.center[1] = T.centerOfMass[1] + sinang * T.d_gm[0] + cosang * T.d_gm[1];
       if (T.center[0] < 0 || T.center[0] > g.extent0 || T.center[1] < 0 || T.center[1] > g.extent1)
         st->error = CUP2D_ERR_BODY_OUT;
-      const double safety = 2 * g.h, R = sp[k].radius;
-      const double c0 = T.center[0], c1 = T.center[1];
-      s_box[k][0] = c0 - R - safety; s_box[k][1] = c0 + R + safety;
-      s_box[k][2] = c1 - R - safety; s_box[k][3] = c1 + R + safety;
+    }
+  }
+  if (phase == 1) return;
+  if (t == 0) {
+    int nc = 0;
+    for (int k = 0; k < ns; ++k) {
+      if (sp[k].kind == 1) {  // union of the swimmer's segment boxes
+        const double* B = fd[sp[k].fish].box;
+        for (int q = 0; q < 4; ++q) s_box[k][q] = B[q];
+      } else {
+        const double safety = 2 * g.h, R = sp[k].radius;
+        const double c0 = ss[k].center[0], c1 = ss[k].center[1];
+        s_box[k][0] = c0 - R - safety; s_box[k][1] = c0 + R + safety;
+        s_box[k][2] = c1 - R - safety; s_box[k][3] = c1 + R + safety;
+      }
       int bx0 = (int)floor(s_box[k][0] / g.hb) - 1, bx1 = (int)floor(s_box[k][1] / g.hb) + 1;
       int by0 = (int)floor(s_box[k][2] / g.hb) - 1, by1 = (int)floor(s_box[k][3] / g.hb) + 1;
       bx0 = bx0 < 0 ? 0 : bx0; by0 = by0 < 0 ? 0 : by0;
@@ -191,7 +214,9 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
     const int local = cidx - s_cstart[k], w = s_rng[k][2];
     const int bx = s_rng[k][0] + local % w, by = s_rng[k][1] + local / w;
     const double bbox[2][2] = {{s_box[k][0], s_box[k][1]}, {s_box[k][2], s_box[k][3]}};
-    s_a[cidx] = touching(g, bx, by, bbox) ? 1 : 0;
+    const bool hit = sp[k].kind == 1 ? fish_touching(g, fd[sp[k].fish], bx, by)
+                                     : touching(g, bx, by, bbox);
+    s_a[cidx] = hit ? 1 : 0;
     s_b[cidx] = bx + g.bpdx * by;
     s_c[cidx] = k;
   }
@@ -252,6 +277,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       ob.ub_block[u] = s_d[s];
       ob.ub_start[u] = s_cnt[s];
       obmap[s_d[s]] = 1;
+      ubidx[s_d[s]] = u;
     }
   }
   __syncthreads();
@@ -305,12 +331,28 @@ __device__ inline double disk_dist(const Grid& g, const ShapeParams& P,
 // distance on that shape's obstacle blocks) at cell (gx, gy), clamped into
 // the domain as the BlockLab does: max over the shapes whose bounding box
 // touches the cell's block of their distance, -1 if none.
+//
+// Swimmers contribute the distance PutFishOnBlocks stored in their slots of
+// that block (signedDistanceSqrt, FishLibrary.cpp:431).
 __device__ inline double grid_sdf(const Grid& g, const ShapeParams* __restrict__ sp,
-                                  const cup2d_shape_state* __restrict__ ss, int gx, int gy) {
+                                  const cup2d_shape_state* __restrict__ ss, const Obstacles& ob,
+                                  const int* __restrict__ ubidx, int gx, int gy) {
   gx = gx < 0 ? 0 : (gx >= g.NX ? g.NX - 1 : gx);
   gy = gy < 0 ? 0 : (gy >= g.NY ? g.NY - 1 : gy);
   double v = -1;
+  if (g.nfish > 0) {
+    const int u = ubidx[(gy / BS) * g.bpdx + gx / BS];
+    if (u >= 0) {
+      const int s0 = ob.ub_start[u], n = ob.ub_count[u];
+      for (int q = 0; q < n; ++q) {
+        const int s = ob.ub_slots[s0 + q];
+        if (sp[ob.slot_shape[s]].kind == 1)
+          v = dmax(v, ob.dist[(size_t)s * BS * BS + (gy % BS) * BS + gx % BS]);
+      }
+    }
+  }
   for (int k = 0; k < g.nshapes; ++k) {
+    if (sp[k].kind != 0) continue;
     const double safety = 2 * g.h, R = sp[k].radius;
     const double c0 = ss[k].center[0], c1 = ss[k].center[1];
     const double bbox[2][2] = {{c0 - R - safety, c0 + R + safety}, {c1 - R - safety, c1 + R + safety}};
@@ -335,6 +377,7 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
                                                    const cup2d_shape_state* __restrict__ ss,
                                                    Obstacles ob, Obstacles pv,
                                                    const unsigned char* __restrict__ obmap,
+                                                   const int* __restrict__ ubidx,
                                                    double* __restrict__ chi,
                                                    double* __restrict__ udx,
                                                    double* __restrict__ udy,
@@ -357,22 +400,33 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
     SlotView o = slot(ob, s);
     const ShapeParams& P = sp[k];
     const cup2d_shape_state& S = ss[k];
-    const double rhoS = P.rhoS;
+    // swimmers: dist, udef and rho = 1 come from PutFishOnBlocks
+    const bool fish = P.kind == 1;
+    const double rhoS = fish ? 1.0 : P.rhoS;
     const double C0 = S.centerOfMass[0], C1 = S.centerOfMass[1];
     double _x = 0, _y = 0, _m = 0, _ns = 0;
     CELL_LOOP(i) {
       const int ix = i % BS, iy = i / BS;
       const int gx = bx * BS + ix, gy = by * BS + iy;
-      const double sd = disk_dist(g, P, S, gx, gy);
-      // neighbours: the shape's own distance inside the block, the grid SDF
-      // across its edges (PutObjectsOnGrid.cpp:53-56)
-      const double distPx = ix + 1 == BS ? grid_sdf(g, sp, ss, gx + 1, gy) : disk_dist(g, P, S, gx + 1, gy);
-      const double distMx = ix == 0 ? grid_sdf(g, sp, ss, gx - 1, gy) : disk_dist(g, P, S, gx - 1, gy);
-      const double distPy = iy + 1 == BS ? grid_sdf(g, sp, ss, gx, gy + 1) : disk_dist(g, P, S, gx, gy + 1);
-      const double distMy = iy == 0 ? grid_sdf(g, sp, ss, gx, gy - 1) : disk_dist(g, P, S, gx, gy - 1);
-      o.dist[i] = sd;
-      o.rho[i] = rhoS;
-      o.udx[i] = 0; o.udy[i] = 0;
+      // the shape's own distance inside the block, the grid SDF across its
+      // edges (PutObjectsOnGrid.cpp:53-56)
+      double sd, distPx, distMx, distPy, distMy;
+      if (fish) {
+        sd = o.dist[i];
+        distPx = ix + 1 == BS ? grid_sdf(g, sp, ss, ob, ubidx, gx + 1, gy) : o.dist[i + 1];
+        distMx = ix == 0 ? grid_sdf(g, sp, ss, ob, ubidx, gx - 1, gy) : o.dist[i - 1];
+        distPy = iy + 1 == BS ? grid_sdf(g, sp, ss, ob, ubidx, gx, gy + 1) : o.dist[i + BS];
+        distMy = iy == 0 ? grid_sdf(g, sp, ss, ob, ubidx, gx, gy - 1) : o.dist[i - BS];
+      } else {
+        sd = disk_dist(g, P, S, gx, gy);
+        distPx = ix + 1 == BS ? grid_sdf(g, sp, ss, ob, ubidx, gx + 1, gy) : disk_dist(g, P, S, gx + 1, gy);
+        distMx = ix == 0 ? grid_sdf(g, sp, ss, ob, ubidx, gx - 1, gy) : disk_dist(g, P, S, gx - 1, gy);
+        distPy = iy + 1 == BS ? grid_sdf(g, sp, ss, ob, ubidx, gx, gy + 1) : disk_dist(g, P, S, gx, gy + 1);
+        distMy = iy == 0 ? grid_sdf(g, sp, ss, ob, ubidx, gx, gy - 1) : disk_dist(g, P, S, gx, gy - 1);
+        o.dist[i] = sd;
+        o.rho[i] = rhoS;
+        o.udx[i] = 0; o.udy[i] = 0;
+      }
       double X;
       if (sd > +2 * h || sd < -2 * h) {
         X = sd > 0 ? 1 : 0;
@@ -546,8 +600,10 @@ __global__ __launch_bounds__(NT) void k_obst_commit(Grid g, cup2d_shape_state* _
       T.n_surface_points = nsurf;
       T.n_obstacle_blocks = ob.slot_of_shape_start[k + 1] - ob.slot_of_shape_start[k];
       const double dCx = T.center[0] - T.centerOfMass[0], dCy = T.center[1] - T.centerOfMass[1];
-      T.d_gm[0] = dCx * cos(T.orientation) + dCy * sin(T.orientation);
-      T.d_gm[1] = -dCx * sin(T.orientation) + dCy * cos(T.orientation);
+      double sn, cs;
+      det_sincos(T.orientation, &sn, &cs);
+      T.d_gm[0] = dCx * cs + dCy * sn;
+      T.d_gm[1] = -dCx * sn + dCy * cs;
     }
   }
 }
@@ -560,11 +616,12 @@ static int grid_cap(cup2d_ctx* c) { return c->ob.cap > 0 ? c->ob.cap : 1; }
 int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   const int G = grid_cap(c);
   std::swap(c->ob, c->ob_prev);
+  const bool fish = !c->hfish.empty();
   {
     TimerScope ts(c, T_PUT_OBJECTS);
     hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                       c->ob, c->ob_prev, c->obmap, c->maxpart, c->nwg, c->ramp, compute_dt,
-                       c->host_step);
+                       c->ob, c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg,
+                       c->ramp, compute_dt, c->host_step, fish ? 1 : 0);
   }
   hipStream_t s = c->stream;
   if (overlap) {
@@ -572,9 +629,18 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_begin, 0));
     s = c->aux;
   }
+  if (fish) {
+    int rc = enqueue_fish_midline(c, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
+                       c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp, 0,
+                       c->host_step, 2);
+    rc = enqueue_fish_place(c, s);
+    if (rc) return rc;
+  }
   if (c->g.nshapes > 0) {
     hipLaunchKernelGGL(k_obst_place, dim3(2 * G), dim3(NT), 0, s, c->g, c->sp, c->ss, c->ob,
-                       c->ob_prev, c->obmap, c->chi, c->udx, c->udy, c->invrho);
+                       c->ob_prev, c->obmap, c->ubidx, c->chi, c->udx, c->udy, c->invrho);
     hipLaunchKernelGGL(k_obst_moments, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob);
     // Disks carry no deformation velocity (Disk::create leaves udef = 0), so
     // removeMoments and putObjectVelOnGrid would only add zeros: skipped
@@ -1204,9 +1270,12 @@ int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux) {
 }
 
 // Shape::resetAll (Shape.h:64-88).
+// Fish::resetAll (Fish.cpp:227-234) clears the swimmers' internal frame.
 __global__ void k_shapes_reset(Grid g, const ShapeParams* __restrict__ sp,
-                               cup2d_shape_state* __restrict__ ss) {
+                               cup2d_shape_state* __restrict__ ss, const FishDev* __restrict__ fd) {
   const int k = threadIdx.x;
+  if (k < g.nfish)
+    for (int q = 0; q < 8; ++q) fd[k].st[q] = 0;
   if (k >= g.nshapes) return;
   cup2d_shape_state T;
   double* z = (double*)&T;
@@ -1220,7 +1289,8 @@ __global__ void k_shapes_reset(Grid g, const ShapeParams* __restrict__ sp,
 
 int enqueue_shapes_reset(cup2d_ctx* c) {
   if (c->g.nshapes == 0) return CUP2D_OK;
-  hipLaunchKernelGGL(k_shapes_reset, dim3(1), dim3(MAX_SHAPES), 0, c->stream, c->g, c->sp, c->ss);
+  hipLaunchKernelGGL(k_shapes_reset, dim3(1), dim3(MAX_SHAPES), 0, c->stream, c->g, c->sp, c->ss,
+                     c->fishd);
   return launch_check(c, "shapes_reset");
 }
 

@@ -87,12 +87,13 @@ Shape parseShape(const std::string& line_in, const double extents[2]) {
     kv[strip_dash(tok.substr(0, eq))] = tok.substr(eq + 1);
   }
   ArgumentParser p(kv);
-  if (name != "disk")
-    throw Error(CUP2D_ERR_ARG, "shape '" + name + "' is not supported on the device (disk)");
+  if (name != "disk" && name != "carlingfish")
+    throw Error(CUP2D_ERR_ARG,
+                "shape '" + name + "' is not supported on the device (disk, carlingfish)");
   Shape s;
   s.name = name;
   cup2d_shape_desc& d = s.desc;
-  d.kind = 0;
+  d.kind = name == "disk" ? 0 : 1;
   d.xpos = p.asDouble("xpos", 0.5 * extents[0]);
   d.ypos = p.asDouble("ypos", 0.5 * extents[1]);
   d.angle_deg = p.asDouble("angle", 0);
@@ -109,7 +110,52 @@ Shape parseShape(const std::string& line_in, const double extents[2]) {
   d.timeForced = p.asDouble("timeForced", std::numeric_limits<double>::max());
   d.radius = p.asDouble("radius", 0.1);
   d.tAccel = p.asDouble("tAccel", -1);
+  if (d.kind == 1) {
+    // Fish::Fish and CarlingFish::CarlingFish (Fish.h:27-29, CarlingFish.cpp:371-376)
+    d.length = p.asDouble("L", 0.1);
+    d.Tperiod = p.asDouble("T", 1);
+    d.phaseShift = p.asDouble("phi", 0);
+    d.amplitudeFactor = p.asDouble("amplitudeFactor", 1.0);
+    // midline envelope and half-width law: the reference's CarlingFish by
+    // default; envelope=quadratic gives L (envA0 + envA2 (s/L)^2) with
+    // wavelength waveLength*L, profile=naca a NACA 4-digit half-thickness
+    const std::string env = p.asString("envelope", "carling");
+    const std::string prof = p.asString("profile", "carling");
+    if (env != "carling" && env != "quadratic")
+      throw Error(CUP2D_ERR_ARG, "envelope must be carling or quadratic");
+    if (prof != "carling" && prof != "naca")
+      throw Error(CUP2D_ERR_ARG, "profile must be carling or naca");
+    d.envelope = env == "carling" ? 0 : 1;
+    d.env_a0 = p.asDouble("envA0", 0.02);
+    d.env_a2 = p.asDouble("envA2", 0.08);
+    d.waveLength = p.asDouble("waveLength", 1.0);
+    d.profile = prof == "carling" ? 0 : 1;
+    d.naca_t = p.asDouble("nacaT", 0.08);
+  }
   return s;
+}
+
+// the -shapes argument: descriptors separated by newlines or commas
+// (Simulation.cpp:106-173)
+static std::vector<Shape> parseShapes(const std::string& arg, const double extents[2]) {
+  std::vector<Shape> out;
+  std::stringstream descriptors(arg);
+  std::string lines;
+  while (std::getline(descriptors, lines)) {
+    std::stringstream ls(lines);
+    std::string line;
+    while (std::getline(ls, line, ',')) {
+      std::string t = line;
+      std::replace(t.begin(), t.end(), '_', ' ');
+      std::istringstream probe(t);
+      std::string objectName;
+      probe >> objectName;
+      if (objectName.empty() || objectName[0] == '#') continue;
+      out.push_back(parseShape(line, extents));
+    }
+  }
+  if (out.empty()) throw Error(CUP2D_ERR_ARG, "FATAL - Did not create any obstacles.");
+  return out;
 }
 
 // ---------------------------------------------------------- Simulation
@@ -135,27 +181,16 @@ void Simulation::parseRuntime() {
   if (sim.muteAll) sim.verbose = 0;
   deviceLoop = parser.asBool("deviceLoop", false);
   if (sim.dlm > 0) throw Error(CUP2D_ERR_ARG, "-dlm > 0 has no device implementation");
+  {  // shape descriptors are checked before any device allocation
+    const double mx = std::max(sim.bpdx, sim.bpdy);
+    const double ext[2] = {sim.extent * sim.bpdx / mx, sim.extent * sim.bpdy / mx};
+    (void)parseShapes(parser.asString("shapes", ""), ext);
+  }
   sim.allocateGrid();
 }
 
 void Simulation::createShapes() {
-  const std::string arg = parser.asString("shapes", "");
-  std::stringstream descriptors(arg);
-  std::string lines;
-  while (std::getline(descriptors, lines)) {
-    std::stringstream ls(lines);
-    std::string line;
-    while (std::getline(ls, line, ',')) {
-      std::string t = line;
-      std::replace(t.begin(), t.end(), '_', ' ');
-      std::istringstream probe(t);
-      std::string objectName;
-      probe >> objectName;
-      if (objectName.empty() || objectName[0] == '#') continue;
-      sim.addShape(parseShape(line, sim.extents));
-    }
-  }
-  if (sim.shapes.empty()) throw Error(CUP2D_ERR_ARG, "FATAL - Did not create any obstacles.");
+  for (const Shape& s : parseShapes(parser.asString("shapes", ""), sim.extents)) sim.addShape(s);
 }
 
 void Simulation::init() {

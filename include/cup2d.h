@@ -65,13 +65,25 @@ typedef struct {
   void* stream;          /* hipStream_t to enqueue on; NULL = own stream     */
 } cup2d_cfg;
 
-/* Shape description: the keys parsed by Shape::Shape and Disk::Disk
- * (Shape.cpp:331-345, ShapesSimple.h:19-21), defaults already resolved. */
+/* Shape description: the keys parsed by Shape::Shape, Disk::Disk and
+ * Fish / CarlingFish (Shape.cpp:331-345, ShapesSimple.h:19-21, Fish.h:27-29,
+ * CarlingFish.cpp:371-376), defaults already resolved. */
 typedef struct {
-  int kind;              /* 0 = disk                                         */
+  int kind;              /* 0 = disk, 1 = carlingfish                        */
   double radius, xpos, ypos, angle_deg, rhoS;
   int bFixedx, bFixedy, bForcedx, bForcedy, bBlockAng;
   double xvel, yvel, angvel, timeForced, tAccel;
+  /* carlingfish: -L -T -phi -amplitudeFactor and the midline / half-width
+   * laws.  envelope 0 is AmplitudeFish's 4/33 (s + L/32) lateral amplitude
+   * with wavelength L (CarlingFish.cpp:300-308); envelope 1 is
+   * L (env_a0 + env_a2 (s/L)^2) with wavelength waveLength*L.  profile 0 is
+   * AmplitudeFish::_width (CarlingFish.cpp:322-329); profile 1 is the NACA
+   * 4-digit half-thickness with closed trailing edge, ratio naca_t. */
+  double length, Tperiod, phaseShift, amplitudeFactor;
+  int envelope;
+  double env_a0, env_a2, waveLength;
+  int profile;
+  double naca_t;
 } cup2d_shape_desc;
 
 /* Body state after a step (Shape.h:22-62). */
@@ -96,7 +108,8 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out);
 int cup2d_destroy(cup2d_ctx* ctx);
 /* Message of the last error on this context ("" if none). */
 const char* cup2d_last_error(const cup2d_ctx* ctx);
-/* Replaces Simulation::createShapes for "disk" (Simulation.cpp:106-173). */
+/* Replaces Simulation::createShapes for "disk" and "carlingfish"
+ * (Simulation.cpp:106-173). */
 int cup2d_add_shape(cup2d_ctx* ctx, const cup2d_shape_desc* desc, int* shape_id);
 
 /* --- data exchange ---------------------------------------------------------
@@ -113,6 +126,14 @@ int cup2d_get_scalars(cup2d_ctx* ctx, double* time, double* dt, int* step, doubl
                       double* h);
 int cup2d_set_scalars(cup2d_ctx* ctx, double time, int step, double uinfx, double uinfy);
 int cup2d_get_shape(cup2d_ctx* ctx, int shape_id, cup2d_shape_state* out);
+/* Midline of swimmer shape_id after the last PutObjectsOnGrid: the data
+ * FishData::writeMidline2File dumps (FishLibrary.cpp:53-64) plus normals and
+ * the internal frame.  *Nm receives the number of midline points; out (if not
+ * NULL, 10*Nm + 8 doubles) receives the arrays rS rX rY vX vY norX norY vNorX
+ * vNorY width (Nm each; centre-of-mass frame rotated by theta_internal) and
+ * area J CoM_internal[2] vCoM_internal[2] theta_internal angvel_internal
+ * (Fish.h:21-23). */
+int cup2d_fish_midline(cup2d_ctx* ctx, int shape_id, int* Nm, double* out);
 
 /* --- operators (Operator::operator()(dt), Operator.h:24-26) --------------- */
 /* Simulation::reset: IC + PutObjectsOnGrid(0) + ApplyObjVel (Simulation.cpp:232-241). */

@@ -37,7 +37,10 @@ class ShapeDesc(C.Structure):
                 ("bFixedx", C.c_int), ("bFixedy", C.c_int), ("bForcedx", C.c_int),
                 ("bForcedy", C.c_int), ("bBlockAng", C.c_int), ("xvel", C.c_double),
                 ("yvel", C.c_double), ("angvel", C.c_double), ("timeForced", C.c_double),
-                ("tAccel", C.c_double)]
+                ("tAccel", C.c_double), ("length", C.c_double), ("Tperiod", C.c_double),
+                ("phaseShift", C.c_double), ("amplitudeFactor", C.c_double),
+                ("envelope", C.c_int), ("env_a0", C.c_double), ("env_a2", C.c_double),
+                ("waveLength", C.c_double), ("profile", C.c_int), ("naca_t", C.c_double)]
 
 
 SHAPE_FIELDS = ["center", "centerOfMass", "labCenterOfMass", "d_gm", "orientation",
@@ -107,6 +110,11 @@ def lib():
         L.orc_fft1d.argtypes = [dp, C.c_int, C.c_int]
         L.orc_get_timing.argtypes = [P, dp, dp]
         L.orc_num_threads.restype = C.c_int
+        L.orc_fish_info.argtypes = [P, C.c_int, dp]
+        L.orc_fish_info.restype = C.c_int
+        L.orc_det_sincos.argtypes = [D, dp, dp]
+        L.orc_lane_sum.argtypes = [dp, C.c_int]
+        L.orc_lane_sum.restype = D
         _lib = L
     return _lib
 
@@ -142,6 +150,42 @@ def disk(radius, xpos, ypos, xvel=0.0, yvel=0.0, angvel=0.0, rhoS=1.0, angle=0.0
                 bFixedx=int(bFixedx), bFixedy=int(bFixedy), bForcedx=int(bForcedx),
                 bForcedy=int(bForcedy), bBlockAng=int(bBlockAng), xvel=xvel, yvel=yvel,
                 angvel=angvel, timeForced=timeForced, tAccel=tAccel)
+
+
+def carlingfish(xpos, ypos, L=0.1, T=1.0, phi=0.0, angle=0.0, amplitudeFactor=1.0,
+                envelope="carling", env_a0=0.02, env_a2=0.08, waveLength=1.0,
+                profile="carling", naca_t=0.08, rhoS=1.0, bFixed=False, bForced=False,
+                bFixedx=None, bFixedy=None, bForcedx=None, bForcedy=None, bBlockAng=None,
+                xvel=0.0, yvel=0.0, angvel=0.0, timeForced=1.7976931348623157e308):
+    """CarlingFish descriptor (Fish.h:27-29, CarlingFish.cpp:371-376) with the
+    reference's flag defaults (Shape.cpp:331-345).  envelope "carling" is
+    4/33 (s + L/32); "quadratic" is L (env_a0 + env_a2 (s/L)^2) with wavelength
+    waveLength*L.  profile "carling" is AmplitudeFish::_width, "naca" a NACA
+    4-digit half-thickness of ratio naca_t."""
+    d = disk(0.0, xpos, ypos, xvel=xvel, yvel=yvel, angvel=angvel, rhoS=rhoS, angle=angle,
+             bFixed=bFixed, bForced=bForced, bFixedx=bFixedx, bFixedy=bFixedy,
+             bForcedx=bForcedx, bForcedy=bForcedy, bBlockAng=bBlockAng,
+             timeForced=timeForced)
+    d.update(kind=1, length=L, Tperiod=T, phaseShift=phi, amplitudeFactor=amplitudeFactor,
+             envelope={"carling": 0, "quadratic": 1}[envelope], env_a0=env_a0, env_a2=env_a2,
+             waveLength=waveLength, profile={"carling": 0, "naca": 1}[profile], naca_t=naca_t)
+    return d
+
+
+FISH_ARRAYS = ["rS", "rX", "rY", "vX", "vY", "norX", "norY", "vNorX", "vNorY", "width"]
+FISH_SCALARS = ["area_internal", "J_internal", "CoM_internal_x", "CoM_internal_y",
+                "vCoM_internal_x", "vCoM_internal_y", "theta_internal", "angvel_internal"]
+
+
+def det_sincos(x):
+    a, b = C.c_double(), C.c_double()
+    lib().orc_det_sincos(x, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def lane_sum(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().orc_lane_sum(_dp(v), len(v))
 
 
 class Oracle:
@@ -196,6 +240,18 @@ class Oracle:
 
     def set_scalars(self, time=0.0, step=0, uinf=(0.0, 0.0)):
         self.L.orc_set_scalars(self.h, time, step, uinf[0], uinf[1])
+
+    def fish_info(self, k=0):
+        """Midline arrays and internal-frame scalars of swimmer k."""
+        Nm = self.L.orc_fish_info(self.h, k, None)
+        if Nm < 0:
+            raise ValueError("shape %d is not a swimmer" % k)
+        out = np.empty(10 * Nm + 8)
+        self.L.orc_fish_info(self.h, k, _dp(out))
+        d = {n: out[i * Nm:(i + 1) * Nm].copy() for i, n in enumerate(FISH_ARRAYS)}
+        d.update({n: float(out[10 * Nm + i]) for i, n in enumerate(FISH_SCALARS)})
+        d["Nm"] = Nm
+        return d
 
     def shape(self, i=0):
         s = ShapeState()

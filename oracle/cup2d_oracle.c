@@ -44,12 +44,32 @@ typedef struct {
       defPowerBnd, circulation;
 } oblock;
 
+/* AreaSegment (FishLibrary.h:149-168): a group of 8 midline intervals with
+ * its bounding box in the fish frame and the lab-frame boxes of
+ * changeToComputationalFrame. */
+typedef struct {
+  int s0, s1;
+  double safe, w[2], c[2], nI[2], nJ[2], lab[2][2], obj[2][2];
+} fseg_t;
+
+/* FishData (FishLibrary.h:41-147) with the Fish internal-frame state
+ * (Fish.h:19-25). */
+typedef struct {
+  int Nm, Nseg, envelope, profile;
+  double length, Tperiod, phaseShift, h, env_a0, env_a2, waveLength, naca_t;
+  double *rS, *rX, *rY, *vX, *vY, *norX, *norY, *vNorX, *vNorY, *width;
+  double area_internal, J_internal, CoM_internal[2], vCoM_internal[2];
+  double theta_internal, angvel_internal, angvel_internal_prev;
+  fseg_t* seg;
+} fish_t;
+
 typedef struct {
   orc_shape_desc d;
   double origC[2], origAng, forcedu, forcedv, forcedomega;
   double appliedForceX, appliedForceY, appliedTorque;
   orc_shape_state st;
   oblock** ob; /* [nblocks], NULL where the shape does not touch the block */
+  fish_t* fish; /* swimmers only */
 } shape_t;
 
 struct orc_sim {
@@ -79,6 +99,9 @@ struct orc_sim {
   double iter_relDF;
 };
 
+static fish_t* fish_new(const orc_shape_desc* d, double h);
+static void fish_free(fish_t* f);
+
 static const int NCOMP[ORC_NFIELDS] = {2, 1, 1, 1, 2, 2, 1, 2};
 int orc_ncomp(int field) { return (field >= 0 && field < ORC_NFIELDS) ? NCOMP[field] : 0; }
 int orc_num_threads(void) { return omp_get_max_threads(); }
@@ -86,6 +109,53 @@ int orc_num_threads(void) { return omp_get_max_threads(); }
 static inline double sq(double x) { return x * x; }
 static inline double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
 static inline double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
+
+/* sin and cos on the body path: Cody-Waite reduction by pi/2 (three-part
+ * constant, the first two parts with 33 significant bits so k*P1 and k*P2
+ * are exact for |k| < 2^20) and Taylor polynomials on [-pi/4, pi/4] (sine to
+ * r^17, cosine to r^18; truncation below 1e-17).  The device library runs the
+ * same operation sequence, so both sides produce identical bits; accuracy is
+ * about one ulp. */
+void orc_det_sincos(double x, double* sn, double* cs)
+{
+  const double P1 = 0x1.921fb544p+0, P2 = 0x1.0b4611a6p-34, P3 = 0x1.3198a2e037073p-69;
+  const double k = floor(x * 0x1.45f306dc9c883p-1 + 0.5);
+  const double r = ((x - k * P1) - k * P2) - k * P3;
+  const double z = r * r;
+  const double sp = r + (r * z) * (-1.0 / 6 + z * (1.0 / 120 + z * (-1.0 / 5040 + z * (1.0 / 362880 +
+                    z * (-1.0 / 39916800 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0 +
+                    z * (1.0 / 355687428096000.0))))))));
+  const double cp = 1 + z * (-0.5 + z * (1.0 / 24 + z * (-1.0 / 720 + z * (1.0 / 40320 +
+                    z * (-1.0 / 3628800 + z * (1.0 / 479001600 + z * (-1.0 / 87178291200.0 +
+                    z * (1.0 / 20922789888000.0 + z * (-1.0 / 6402373705728000.0)))))))));
+  const int q = (int)(k - 4 * floor(k * 0.25));
+  switch (q) {
+    case 0: *sn = sp; *cs = cp; break;
+    case 1: *sn = cp; *cs = -sp; break;
+    case 2: *sn = -sp; *cs = -cp; break;
+    default: *sn = -cp; *cs = sp; break;
+  }
+}
+static inline double det_sin(double x) { double a, b; orc_det_sincos(x, &a, &b); return a; }
+static inline double det_cos(double x) { double a, b; orc_det_sincos(x, &a, &b); return b; }
+
+/* Sum of n values in a fixed order: 256 lanes, lane l adds v[l], v[l+256],
+ * ... in turn, then the lanes are combined pairwise (l += l + 128, 64, ..., 1).
+ * The reference sums these midline integrals with OpenMP reductions (no fixed
+ * order); the device library uses this order, so both sides agree bit for
+ * bit. */
+double orc_lane_sum(const double* v, int n)
+{
+  double p[256];
+  for (int l = 0; l < 256; ++l) {
+    double a = 0;
+    for (int i = l; i < n; i += 256) a += v[i];
+    p[l] = a;
+  }
+  for (int st = 128; st >= 1; st >>= 1)
+    for (int l = 0; l < st; ++l) p[l] = p[l] + p[l + st];
+  return p[0];
+}
 
 static inline int BX(const orc_sim* s, int b) { return b % s->bpdx; }
 static inline int BY(const orc_sim* s, int b) { return b / s->bpdx; }
@@ -240,6 +310,7 @@ void orc_destroy(orc_sim* s)
   for (int k = 0; k < s->nshapes; ++k) {
     for (int b = 0; b < s->nb; ++b) free(s->shapes[k].ob[b]);
     free(s->shapes[k].ob);
+    fish_free(s->shapes[k].fish);
   }
   free(s->shapes); free(s->bnd); free(s->buffer); free(s->kernel);
   fft2d_free(&s->fft);
@@ -256,11 +327,24 @@ static void shape_reset(orc_sim* s, shape_t* sh)
   t->u = sh->forcedu; t->v = sh->forcedv; t->omega = sh->forcedomega;
   sh->appliedForceX = sh->appliedForceY = sh->appliedTorque = 0;
   for (int b = 0; b < s->nb; ++b) { free(sh->ob[b]); sh->ob[b] = NULL; }
+  if (sh->fish) { /* Fish::resetAll, Fish.cpp:227-234 */
+    fish_t* f = sh->fish;
+    f->CoM_internal[0] = f->CoM_internal[1] = 0;
+    f->vCoM_internal[0] = f->vCoM_internal[1] = 0;
+    f->theta_internal = f->angvel_internal = f->angvel_internal_prev = 0;
+  }
 }
 
 int orc_add_shape(orc_sim* s, const orc_shape_desc* d)
 {
-  if (d->kind != 0) return -1;
+  if (d->kind != 0 && d->kind != 1) return -1;
+  fish_t* fish = NULL;
+  if (d->kind == 1) {
+    if (!(d->length > 0) || !(d->Tperiod > 0)) return -1;
+    if (d->envelope == 1 && !(d->waveLength > 0)) return -1;
+    fish = fish_new(d, s->h);
+    if (!fish) return -1;
+  }
   s->shapes = (shape_t*)realloc(s->shapes, sizeof(shape_t) * (size_t)(s->nshapes + 1));
   shape_t* sh = &s->shapes[s->nshapes];
   memset(sh, 0, sizeof(*sh));
@@ -270,6 +354,7 @@ int orc_add_shape(orc_sim* s, const orc_shape_desc* d)
   sh->origAng = d->angle_deg * M_PI / 180;
   sh->forcedu = -d->xvel; sh->forcedv = -d->yvel; sh->forcedomega = -d->angvel;
   sh->ob = (oblock**)calloc((size_t)s->nb, sizeof(oblock*));
+  sh->fish = fish;
   shape_reset(s, sh);
   return s->nshapes++;
 }
@@ -305,6 +390,24 @@ int orc_get_shape(const orc_sim* s, int i, orc_shape_state* out)
   out->n_surface_points = ns;
   return 0;
 }
+int orc_fish_info(const orc_sim* s, int k, double* out)
+{
+  if (k < 0 || k >= s->nshapes || !s->shapes[k].fish) return -1;
+  const fish_t* f = s->shapes[k].fish;
+  const int Nm = f->Nm;
+  if (out) {
+    const double* arr[10] = {f->rS, f->rX, f->rY, f->vX, f->vY, f->norX, f->norY,
+                             f->vNorX, f->vNorY, f->width};
+    for (int a = 0; a < 10; ++a) memcpy(out + (size_t)a * Nm, arr[a], sizeof(double) * (size_t)Nm);
+    double* q = out + (size_t)10 * Nm;
+    q[0] = f->area_internal; q[1] = f->J_internal;
+    q[2] = f->CoM_internal[0]; q[3] = f->CoM_internal[1];
+    q[4] = f->vCoM_internal[0]; q[5] = f->vCoM_internal[1];
+    q[6] = f->theta_internal; q[7] = f->angvel_internal;
+  }
+  return Nm;
+}
+
 void orc_get_timing(const orc_sim* s, double* total, double* fft) { *total = s->t_total; *fft = s->t_fft; }
 
 /* --------------------------------------------------------------- helpers */
@@ -411,6 +514,522 @@ static void disk_create(orc_sim* s, shape_t* sh)
         B[iy * BS + ix] = dmax(B[iy * BS + ix], o->dist[iy][ix]);
         o->rho[iy][ix] = sh->d.rhoS;
       }
+  }
+}
+
+/* ---------------------------------------------------------------- swimmers */
+/* AmplitudeFish::_width (CarlingFish.cpp:322-329) or a NACA 4-digit
+ * symmetric half-thickness with closed trailing edge,
+ * 5 t L (0.2969 sqrt(x) - 0.1260 x - 0.3516 x^2 + 0.2843 x^3 - 0.1036 x^4). */
+static double fish_width(const fish_t* f, double s, double L)
+{
+  if (s < 0 || s > L) return 0;
+  if (f->profile == 0) {
+    const double sb = .04 * f->length, st = .95 * f->length, wt = .01 * f->length,
+                 wh = .04 * f->length;
+    return s < sb ? sqrt(2 * wh * s - s * s)
+                  : (s < st ? wh - (wh - wt) * ((s - sb) / (st - sb)) : (wt * (L - s) / (L - st)));
+  }
+  const double x = s / L;
+  const double w = 5 * f->naca_t * L *
+                   (0.2969 * sqrt(x) - 0.1260 * x - 0.3516 * x * x + 0.2843 * x * x * x -
+                    0.1036 * x * x * x * x);
+  return w < 0 ? 0 : w;
+}
+
+/* FishData::FishData (FishLibrary.cpp:14-38, FishLibrary.h:44-60) and
+ * AmplitudeFish's width table (CarlingFish.cpp:318-319). */
+static fish_t* fish_new(const orc_shape_desc* d, double h)
+{
+  const double length = d->length;
+  const double fracRefined = 0.1, fracMid = 1 - 2 * fracRefined;
+  const double dSmid_tgt = h / sqrt(2);
+  const double dSrefine_tgt = 0.125 * h;
+  const int Nmid = (int)ceil(length * fracMid / dSmid_tgt / 8) * 8;
+  const double dSmid = length * fracMid / Nmid;
+  const int Nend = (int)ceil(fracRefined * length * 2 / (dSmid + dSrefine_tgt) / 4) * 4;
+  const double dSref = fracRefined * length * 2 / Nend - dSmid;
+  const int Nm = Nmid + 2 * Nend + 1;
+  if (!(dSref > 0) || Nend < 2 || Nm < 17) return NULL;
+  fish_t* f = (fish_t*)calloc(1, sizeof(fish_t));
+  f->Nm = Nm;
+  f->Nseg = (Nm - 1) / 8;
+  f->length = length; f->Tperiod = d->Tperiod; f->phaseShift = d->phaseShift; f->h = h;
+  f->envelope = d->envelope; f->profile = d->profile;
+  f->env_a0 = d->env_a0; f->env_a2 = d->env_a2; f->waveLength = d->waveLength;
+  f->naca_t = d->naca_t;
+  double** arr[10] = {&f->rS, &f->rX, &f->rY, &f->vX, &f->vY, &f->norX, &f->norY,
+                      &f->vNorX, &f->vNorY, &f->width};
+  for (int a = 0; a < 10; ++a) *arr[a] = (double*)calloc((size_t)Nm, sizeof(double));
+  f->seg = (fseg_t*)calloc((size_t)f->Nseg, sizeof(fseg_t));
+  double* rS = f->rS;
+  rS[0] = 0;
+  int k = 0;
+  for (int i = 0; i < Nend; ++i, k++)
+    rS[k + 1] = rS[k] + dSref + (dSmid - dSref) * i / ((double)Nend - 1.);
+  for (int i = 0; i < Nmid; ++i, k++) rS[k + 1] = rS[k] + dSmid;
+  for (int i = 0; i < Nend; ++i, k++)
+    rS[k + 1] = rS[k] + dSref + (dSmid - dSref) * (Nend - i - 1) / ((double)Nend - 1.);
+  rS[k] = dmin(rS[k], length);
+  for (int i = 0; i < Nm; ++i) f->width[i] = fish_width(f, rS[i], length);
+  return f;
+}
+
+static void fish_free(fish_t* f)
+{
+  if (!f) return;
+  free(f->rS); free(f->rX); free(f->rY); free(f->vX); free(f->vY);
+  free(f->norX); free(f->norY); free(f->vNorX); free(f->vNorY); free(f->width);
+  free(f->seg); free(f);
+}
+
+/* AmplitudeFish::midlineLatPos / midlineLatVel (CarlingFish.cpp:300-308), or
+ * the envelope L (a0 + a2 (s/L)^2) with wavelength waveLength*L. */
+static inline double lat_pos(const fish_t* f, double s, double t)
+{
+  if (f->envelope == 0) {
+    const double arg = 2 * M_PI * (s / f->length - t / f->Tperiod + f->phaseShift);
+    return 4. / 33. * (s + 0.03125 * f->length) * det_sin(arg);
+  }
+  const double x = s / f->length;
+  const double arg = 2 * M_PI * (x / f->waveLength - t / f->Tperiod + f->phaseShift);
+  return f->length * (f->env_a0 + f->env_a2 * x * x) * det_sin(arg);
+}
+static inline double lat_vel(const fish_t* f, double s, double t)
+{
+  if (f->envelope == 0) {
+    const double arg = 2 * M_PI * (s / f->length - t / f->Tperiod + f->phaseShift);
+    return -4. / 33. * (s + 0.03125 * f->length) * (2 * M_PI / f->Tperiod) * det_cos(arg);
+  }
+  const double x = s / f->length;
+  const double arg = 2 * M_PI * (x / f->waveLength - t / f->Tperiod + f->phaseShift);
+  return -f->length * (f->env_a0 + f->env_a2 * x * x) * (2 * M_PI / f->Tperiod) * det_cos(arg);
+}
+
+/* AmplitudeFish::computeMidline (CarlingFish.cpp:332-369) */
+static void fish_midline(fish_t* f, double t)
+{
+  const int Nm = f->Nm;
+  const double T = f->Tperiod;
+  const double rampFac = t < T ? det_sin(0.5 * M_PI * t / T) : 1.0;
+  const double rampFacVel = t < T ? 0.5 * M_PI / T * det_cos(0.5 * M_PI * t / T) : 0.0;
+  double *rS = f->rS, *rX = f->rX, *rY = f->rY, *vX = f->vX, *vY = f->vY;
+  rX[0] = 0.0;
+  rY[0] = rampFac * lat_pos(f, rS[0], t);
+  vX[0] = 0.0;
+  vY[0] = rampFac * lat_vel(f, rS[0], t) + rampFacVel * lat_pos(f, rS[0], t);
+  for (int i = 1; i < Nm; ++i) {
+    rY[i] = rampFac * lat_pos(f, rS[i], t);
+    vY[i] = rampFac * lat_vel(f, rS[i], t) + rampFacVel * lat_pos(f, rS[i], t);
+  }
+  for (int i = 1; i < Nm; ++i) {
+    const double dy = rY[i] - rY[i - 1], ds = rS[i] - rS[i - 1];
+    const double dx = sqrt(ds * ds - dy * dy);
+    const double dVy = vY[i] - vY[i - 1];
+    const double dVx = -dy / dx * dVy;
+    rX[i] = dx;
+    vX[i] = dVx;
+    f->norX[i - 1] = -dy / ds;
+    f->norY[i - 1] = dx / ds;
+    f->vNorX[i - 1] = -dVy / ds;
+    f->vNorY[i - 1] = dVx / ds;
+  }
+  for (int i = 1; i < Nm; ++i) { rX[i] += rX[i - 1]; vX[i] += vX[i - 1]; }
+  f->norX[Nm - 1] = f->norX[Nm - 2];
+  f->norY[Nm - 1] = f->norY[Nm - 2];
+  f->vNorX[Nm - 1] = f->vNorX[Nm - 2];
+  f->vNorY[Nm - 1] = f->vNorY[Nm - 2];
+}
+
+/* FishData::_d_ds (FishLibrary.h:102-107) */
+static inline double fish_d_ds(const fish_t* f, int idx, const double* vals)
+{
+  const double* rS = f->rS;
+  const int maxidx = f->Nm;
+  if (idx == 0) return (vals[idx + 1] - vals[idx]) / (rS[idx + 1] - rS[idx]);
+  else if (idx == maxidx - 1) return (vals[idx] - vals[idx - 1]) / (rS[idx] - rS[idx - 1]);
+  else return ((vals[idx + 1] - vals[idx]) / (rS[idx + 1] - rS[idx]) +
+               (vals[idx] - vals[idx - 1]) / (rS[idx] - rS[idx - 1])) / 2;
+}
+
+/* trapezoid weight and _integrationFac1..3 (FishLibrary.h:108-118); w^3 is
+ * written w*w*w */
+static inline void fish_facs(const fish_t* f, int i, double* ds, double* fac1, double* fac2,
+                             double* fac3)
+{
+  const int Nm = f->Nm;
+  const double* rS = f->rS;
+  *ds = (i == 0) ? rS[1] - rS[0] : ((i == Nm - 1) ? rS[Nm - 1] - rS[Nm - 2] : rS[i + 1] - rS[i - 1]);
+  const double w = f->width[i], w3 = w * w * w;
+  const double dnorXi = fish_d_ds(f, i, f->norX), dnorYi = fish_d_ds(f, i, f->norY);
+  *fac1 = 2 * w;
+  *fac2 = 2 * w3 * (dnorXi * f->norY[i] - dnorYi * f->norX[i]) / 3;
+  *fac3 = 2 * w3 / 3;
+}
+
+/* FishData::_computeMidlineNormals (FishLibrary.cpp:66-84) */
+static void fish_normals(fish_t* f)
+{
+  const int Nm = f->Nm;
+  for (int i = 0; i < Nm - 1; i++) {
+    const double ds = f->rS[i + 1] - f->rS[i];
+    const double tX = f->rX[i + 1] - f->rX[i], tY = f->rY[i + 1] - f->rY[i];
+    const double tVX = f->vX[i + 1] - f->vX[i], tVY = f->vY[i + 1] - f->vY[i];
+    f->norX[i] = -tY / ds;
+    f->norY[i] = tX / ds;
+    f->vNorX[i] = -tVY / ds;
+    f->vNorY[i] = tVX / ds;
+  }
+  f->norX[Nm - 1] = f->norX[Nm - 2];
+  f->norY[Nm - 1] = f->norY[Nm - 2];
+  f->vNorX[Nm - 1] = f->vNorX[Nm - 2];
+  f->vNorY[Nm - 1] = f->vNorY[Nm - 2];
+}
+
+/* Fish::create steps 1-3 (Fish.cpp:78-95): midline, integrateLinearMomentum
+ * (FishLibrary.cpp:86-117), changeToCoMFrameLinear (:152-159),
+ * integrateAngularMomentum (:119-150), changeToCoMFrameAngular (:161-176). */
+static void fish_frame(fish_t* f, double t)
+{
+  const int Nm = f->Nm;
+  fish_midline(f, t);
+  double* tmp = (double*)malloc(sizeof(double) * (size_t)Nm * 5);
+  for (int i = 0; i < Nm; ++i) {
+    double ds, fac1, fac2, fac3;
+    fish_facs(f, i, &ds, &fac1, &fac2, &fac3);
+    tmp[i] = fac1 * ds / 2;
+    tmp[Nm + i] = (f->rX[i] * fac1 + f->norX[i] * fac2) * ds / 2;
+    tmp[2 * Nm + i] = (f->rY[i] * fac1 + f->norY[i] * fac2) * ds / 2;
+    tmp[3 * Nm + i] = (f->vX[i] * fac1 + f->vNorX[i] * fac2) * ds / 2;
+    tmp[4 * Nm + i] = (f->vY[i] * fac1 + f->vNorY[i] * fac2) * ds / 2;
+  }
+  const double area = orc_lane_sum(tmp, Nm);
+  const double cmx = orc_lane_sum(tmp + Nm, Nm), cmy = orc_lane_sum(tmp + 2 * Nm, Nm);
+  const double lmx = orc_lane_sum(tmp + 3 * Nm, Nm), lmy = orc_lane_sum(tmp + 4 * Nm, Nm);
+  f->area_internal = area;
+  f->CoM_internal[0] = cmx / area; f->CoM_internal[1] = cmy / area;
+  f->vCoM_internal[0] = lmx / area; f->vCoM_internal[1] = lmy / area;
+  for (int i = 0; i < Nm; ++i) {
+    f->rX[i] -= f->CoM_internal[0]; f->rY[i] -= f->CoM_internal[1];
+    f->vX[i] -= f->vCoM_internal[0]; f->vY[i] -= f->vCoM_internal[1];
+  }
+  f->angvel_internal_prev = f->angvel_internal;
+  for (int i = 0; i < Nm; ++i) {
+    double ds, fac1, fac2, fac3;
+    fish_facs(f, i, &ds, &fac1, &fac2, &fac3);
+    const double rX = f->rX[i], rY = f->rY[i], vX = f->vX[i], vY = f->vY[i];
+    const double norX = f->norX[i], norY = f->norY[i], vNorX = f->vNorX[i], vNorY = f->vNorY[i];
+    const double tmp_M = (rX * vY - rY * vX) * fac1 +
+                         (rX * vNorY - rY * vNorX + vY * norX - vX * norY) * fac2 +
+                         (norX * vNorY - norY * vNorX) * fac3;
+    const double tmp_J = (rX * rX + rY * rY) * fac1 + 2 * (rX * norX + rY * norY) * fac2 + fac3;
+    tmp[i] = tmp_M * ds / 2;
+    tmp[Nm + i] = tmp_J * ds / 2;
+  }
+  const double am = orc_lane_sum(tmp, Nm), J = orc_lane_sum(tmp + Nm, Nm);
+  free(tmp);
+  f->J_internal = J;
+  f->angvel_internal = am / J;
+  double sn, cs;
+  orc_det_sincos(f->theta_internal, &sn, &cs);
+  const double R[2][2] = {{cs, -sn}, {sn, cs}};
+  const double vAin = f->angvel_internal;
+  for (int i = 0; i < Nm; ++i) {
+    f->vX[i] += vAin * f->rY[i];
+    f->vY[i] -= vAin * f->rX[i];
+    const double px = f->rX[i], py = f->rY[i];
+    f->rX[i] = R[0][0] * px + R[0][1] * py;
+    f->rY[i] = R[1][0] * px + R[1][1] * py;
+    const double qx = f->vX[i], qy = f->vY[i];
+    f->vX[i] = R[0][0] * qx + R[0][1] * qy;
+    f->vY[i] = R[1][0] * qx + R[1][1] * qy;
+  }
+  fish_normals(f);
+}
+
+/* Fish::create step 4-5 (Fish.cpp:118-150): segment boxes, AreaSegment
+ * (FishLibrary.h:160-164) and AreaSegment::changeToComputationalFrame
+ * (FishLibrary.cpp:295-342) */
+static void fish_segments(fish_t* f, const double pos[2], double angle)
+{
+  const int Nm = f->Nm, Nseg = f->Nseg;
+  double sn, cs;
+  orc_det_sincos(angle, &sn, &cs);
+  const double R[2][2] = {{cs, -sn}, {sn, cs}};
+  for (int i = 0; i < Nseg; ++i) {
+    fseg_t* S = &f->seg[i];
+    const int next_idx = (i + 1) * (Nm - 1) / Nseg, idx = i * (Nm - 1) / Nseg;
+    double bbox[2][2] = {{1e9, -1e9}, {1e9, -1e9}};
+    for (int ss = idx; ss <= next_idx; ++ss) {
+      const double w = f->width[ss];
+      const double xBnd[2] = {f->rX[ss] - f->norX[ss] * w, f->rX[ss] + f->norX[ss] * w};
+      const double yBnd[2] = {f->rY[ss] - f->norY[ss] * w, f->rY[ss] + f->norY[ss] * w};
+      const double maxX = dmax(xBnd[0], xBnd[1]), minX = dmin(xBnd[0], xBnd[1]);
+      const double maxY = dmax(yBnd[0], yBnd[1]), minY = dmin(yBnd[0], yBnd[1]);
+      bbox[0][0] = dmin(bbox[0][0], minX);
+      bbox[0][1] = dmax(bbox[0][1], maxX);
+      bbox[1][0] = dmin(bbox[1][0], minY);
+      bbox[1][1] = dmax(bbox[1][1], maxY);
+    }
+    S->s0 = idx; S->s1 = next_idx;
+    S->safe = 2 * f->h;
+    S->w[0] = (bbox[0][1] - bbox[0][0]) / 2 + S->safe;
+    S->w[1] = (bbox[1][1] - bbox[1][0]) / 2 + S->safe;
+    S->c[0] = (bbox[0][1] + bbox[0][0]) / 2;
+    S->c[1] = (bbox[1][1] + bbox[1][0]) / 2;
+    const double p[2] = {S->c[0], S->c[1]};
+    const double nx[2] = {1, 0}, ny[2] = {0, 1};
+    for (int d = 0; d < 2; ++d) {
+      S->c[d] = R[d][0] * p[0] + R[d][1] * p[1];
+      S->nI[d] = R[d][0] * nx[0] + R[d][1] * nx[1];
+      S->nJ[d] = R[d][0] * ny[0] + R[d][1] * ny[1];
+    }
+    S->c[0] += pos[0];
+    S->c[1] += pos[1];
+    const double magI = sqrt(S->nI[0] * S->nI[0] + S->nI[1] * S->nI[1]);
+    const double magJ = sqrt(S->nJ[0] * S->nJ[0] + S->nJ[1] * S->nJ[1]);
+    const double invMagI = 1 / magI, invMagJ = 1 / magJ;
+    for (int d = 0; d < 2; ++d) {
+      S->nI[d] = fabs(S->nI[d]) * invMagI;
+      S->nJ[d] = fabs(S->nJ[d]) * invMagJ;
+    }
+    const double wX[2] = {S->w[0] * S->nI[0], S->w[0] * S->nI[1]};
+    const double wY[2] = {S->w[1] * S->nJ[0], S->w[1] * S->nJ[1]};
+    for (int d = 0; d < 2; ++d) {
+      S->lab[d][0] = S->c[d] - wX[d] - wY[d];
+      S->lab[d][1] = S->c[d] + wX[d] + wY[d];
+      S->obj[d][0] = S->c[d] - S->w[d];
+      S->obj[d][1] = S->c[d] + S->w[d];
+    }
+  }
+}
+
+/* AreaSegment::isIntersectingWithAABB (FishLibrary.cpp:344-401) */
+static int seg_intersects(const fseg_t* S, const double start[2], const double end[2])
+{
+  const double AABB_w[2] = {(end[0] - start[0]) / 2 + S->safe, (end[1] - start[1]) / 2 + S->safe};
+  const double AABB_c[2] = {(end[0] + start[0]) / 2, (end[1] + start[1]) / 2};
+  const double AABB_box[2][2] = {{AABB_c[0] - AABB_w[0], AABB_c[0] + AABB_w[0]},
+                                 {AABB_c[1] - AABB_w[1], AABB_c[1] + AABB_w[1]}};
+  const double iL[2][2] = {{dmax(S->lab[0][0], AABB_box[0][0]), dmin(S->lab[0][1], AABB_box[0][1])},
+                           {dmax(S->lab[1][0], AABB_box[1][0]), dmin(S->lab[1][1], AABB_box[1][1])}};
+  if (iL[0][1] - iL[0][0] < 0 || iL[1][1] - iL[1][0] < 0) return 0;
+  const double widthXbox[2] = {AABB_w[0] * S->nI[0], AABB_w[0] * S->nJ[0]};
+  const double widthYbox[2] = {AABB_w[1] * S->nI[1], AABB_w[1] * S->nJ[1]};
+  const double boxBox[2][2] = {
+      {AABB_c[0] - widthXbox[0] - widthYbox[0], AABB_c[0] + widthXbox[0] + widthYbox[0]},
+      {AABB_c[1] - widthXbox[1] - widthYbox[1], AABB_c[1] + widthXbox[1] + widthYbox[1]}};
+  const double iF[2][2] = {{dmax(boxBox[0][0], S->obj[0][0]), dmin(boxBox[0][1], S->obj[0][1])},
+                           {dmax(boxBox[1][0], S->obj[1][0]), dmin(boxBox[1][1], S->obj[1][1])}};
+  if (iF[0][1] - iF[0][0] < 0 || iF[1][1] - iF[1][0] < 0) return 0;
+  return 1;
+}
+
+/* PutFishOnBlocks (FishLibrary.h:170-214): the frame of the fish at
+ * (position, angle) */
+typedef struct {
+  double pos[2], R[2][2];
+} fframe_t;
+
+static inline void to_comp(const fframe_t* F, double x[2])
+{ /* changeToComputationalFrame */
+  const double p[2] = {x[0], x[1]};
+  x[0] = F->R[0][0] * p[0] + F->R[0][1] * p[1];
+  x[1] = F->R[1][0] * p[0] + F->R[1][1] * p[1];
+  x[0] += F->pos[0];
+  x[1] += F->pos[1];
+}
+static inline void vel_to_comp(const fframe_t* F, double x[2])
+{ /* changeVelocityToComputationalFrame */
+  const double p[2] = {x[0], x[1]};
+  x[0] = F->R[0][0] * p[0] + F->R[0][1] * p[1];
+  x[1] = F->R[1][0] * p[0] + F->R[1][1] * p[1];
+}
+static inline void from_comp(const fframe_t* F, double x[2])
+{ /* changeFromComputationalFrame */
+  const double p[2] = {x[0] - F->pos[0], x[1] - F->pos[1]};
+  x[0] = F->R[0][0] * p[0] + F->R[1][0] * p[1];
+  x[1] = F->R[0][1] * p[0] + F->R[1][1] * p[1];
+}
+static inline double dist_sq(const double a[2], const double b[2])
+{
+  return (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]);
+}
+
+/* PutFishOnBlocks::constructSurface, constructInternl and signedDistanceSqrt
+ * (FishLibrary.cpp:419-638) for block b and the segments sl[0..ns) that
+ * intersect it, in segment order; B is the block of the grid SDF (sim.tmp). */
+static void put_fish_on_block(const orc_sim* s, const fish_t* f, const fframe_t* F, int b,
+                              const int* sl, int ns, oblock* o, double* B)
+{
+  double org[2];
+  bpos(s, b, 0, 0, org);
+  const double h = s->h, invh = 1.0 / h;
+  const int Nm = f->Nm;
+  const double *rX = f->rX, *rY = f->rY, *norX = f->norX, *norY = f->norY;
+  const double *vX = f->vX, *vY = f->vY, *vNorX = f->vNorX, *vNorY = f->vNorY;
+  const double* width = f->width;
+  for (int iy = 0; iy < BS; ++iy)
+    for (int ix = 0; ix < BS; ++ix) { o->dist[iy][ix] = -1; o->chi[iy][ix] = 0; }
+  /* constructSurface */
+  for (int i = 0; i < ns; ++i) {
+    const fseg_t* S = &f->seg[sl[i]];
+    const int firstSegm = S->s0 > 1 ? S->s0 : 1;
+    const int lastSegm = S->s1 < Nm - 2 ? S->s1 : Nm - 2;
+    for (int ss = firstSegm; ss <= lastSegm; ++ss) {
+      for (int signp = -1; signp <= 1; signp += 2) {
+        double myP[2] = {rX[ss + 0] + width[ss + 0] * signp * norX[ss + 0],
+                         rY[ss + 0] + width[ss + 0] * signp * norY[ss + 0]};
+        to_comp(F, myP);
+        const int iap[2] = {(int)floor((myP[0] - org[0]) * invh), (int)floor((myP[1] - org[1]) * invh)};
+        if (iap[0] + 3 <= 0 || iap[0] - 1 >= BS) continue;
+        if (iap[1] + 3 <= 0 || iap[1] - 1 >= BS) continue;
+        double pP[2] = {rX[ss + 1] + width[ss + 1] * signp * norX[ss + 1],
+                        rY[ss + 1] + width[ss + 1] * signp * norY[ss + 1]};
+        to_comp(F, pP);
+        double pM[2] = {rX[ss - 1] + width[ss - 1] * signp * norX[ss - 1],
+                        rY[ss - 1] + width[ss - 1] * signp * norY[ss - 1]};
+        to_comp(F, pM);
+        double udef[2] = {vX[ss + 0] + width[ss + 0] * signp * vNorX[ss + 0],
+                          vY[ss + 0] + width[ss + 0] * signp * vNorY[ss + 0]};
+        vel_to_comp(F, udef);
+        const int sy0 = iap[1] - 1 > 0 ? iap[1] - 1 : 0, sy1 = iap[1] + 3 < BS ? iap[1] + 3 : BS;
+        const int sx0 = iap[0] - 1 > 0 ? iap[0] - 1 : 0, sx1 = iap[0] + 3 < BS ? iap[0] + 3 : BS;
+        for (int sy = sy0; sy < sy1; ++sy)
+          for (int sx = sx0; sx < sx1; ++sx) {
+            double p[2];
+            bpos(s, b, sx, sy, p);
+            const double dist0 = dist_sq(p, myP);
+            const double distP = dist_sq(p, pP);
+            const double distM = dist_sq(p, pM);
+            if (fabs(o->dist[sy][sx]) < dmin(dmin(dist0, distP), distM)) continue;
+            from_comp(F, p);
+            int close_s = ss, secnd_s = ss + (distP < distM ? 1 : -1);
+            double dist1 = dist0, dist2 = distP < distM ? distP : distM;
+            if (distP < dist0 || distM < dist0) {
+              dist1 = dist2; dist2 = dist0;
+              close_s = secnd_s; secnd_s = ss;
+            }
+            const double dSsq = (rX[close_s] - rX[secnd_s]) * (rX[close_s] - rX[secnd_s]) +
+                                (rY[close_s] - rY[secnd_s]) * (rY[close_s] - rY[secnd_s]);
+            const double cnt2ML = width[close_s] * width[close_s];
+            const double nxt2ML = width[secnd_s] * width[secnd_s];
+            const double safeW = dmax(width[close_s], width[secnd_s]) + 2 * h;
+            const double xMidl[2] = {rX[close_s], rY[close_s]};
+            const double grd2ML = dist_sq(p, xMidl);
+            const double diffH = fabs(width[close_s] - width[secnd_s]);
+            double sign2d = 0;
+            if (dSsq > diffH * diffH || grd2ML > safeW * safeW) {
+              sign2d = grd2ML > cnt2ML ? -1 : 1;
+            } else {
+              const double corr = 2 * sqrt(cnt2ML * nxt2ML);
+              const double Rsq = (cnt2ML + nxt2ML - corr + dSsq) * (cnt2ML + nxt2ML + corr + dSsq) / 4 / dSsq;
+              const double maxAx = dmax(cnt2ML, nxt2ML);
+              const int idAx1 = cnt2ML > nxt2ML ? close_s : secnd_s;
+              const int idAx2 = idAx1 == close_s ? secnd_s : close_s;
+              const double d = sqrt((Rsq - maxAx) / dSsq);
+              const double xCentr[2] = {rX[idAx1] + (rX[idAx1] - rX[idAx2]) * d,
+                                        rY[idAx1] + (rY[idAx1] - rY[idAx2]) * d};
+              const double grd2Core = dist_sq(p, xCentr);
+              sign2d = grd2Core > Rsq ? -1 : 1;
+            }
+            if (fabs(o->dist[sy][sx]) > dist1) {
+              const double W = 1 - sqrt(dist1) * (invh / 3);
+              o->udef[sy][sx][0] = W * udef[0];
+              o->udef[sy][sx][1] = W * udef[1];
+              o->dist[sy][sx] = sign2d * dist1;
+              o->chi[sy][sx] = W;
+              o->rho[sy][sx] = 1;
+            }
+          }
+      }
+    }
+  }
+  /* constructInternl */
+  for (int i = 0; i < ns; ++i) {
+    const fseg_t* S = &f->seg[sl[i]];
+    const int firstSegm = S->s0 > 1 ? S->s0 : 1;
+    const int lastSegm = S->s1 < Nm - 2 ? S->s1 : Nm - 2;
+    for (int ss = firstSegm; ss <= lastSegm; ++ss) {
+      const double myWidth = width[ss];
+      const int Nw = (int)floor(myWidth / h);
+      for (int iw = -Nw + 1; iw < Nw; ++iw) {
+        const double offsetW = iw * h;
+        double xp[2] = {rX[ss] + offsetW * norX[ss], rY[ss] + offsetW * norY[ss]};
+        to_comp(F, xp);
+        xp[0] = (xp[0] - org[0]) * invh;
+        xp[1] = (xp[1] - org[1]) * invh;
+        const double ap[2] = {floor(xp[0]), floor(xp[1])};
+        const int iap[2] = {(int)ap[0], (int)ap[1]};
+        if (iap[0] + 2 <= 0 || iap[0] >= BS) continue;
+        if (iap[1] + 2 <= 0 || iap[1] >= BS) continue;
+        double udef[2] = {vX[ss] + offsetW * vNorX[ss], vY[ss] + offsetW * vNorY[ss]};
+        vel_to_comp(F, udef);
+        double wghts[2][2];
+        for (int c = 0; c < 2; ++c) {
+          const double t[2] = {fabs(xp[c] - ap[c]), fabs(xp[c] - (ap[c] + 1))};
+          wghts[c][0] = 1 - t[0];
+          wghts[c][1] = 1 - t[1];
+        }
+        const int y0 = iap[1] > 0 ? iap[1] : 0, y1 = iap[1] + 2 < BS ? iap[1] + 2 : BS;
+        const int x0 = iap[0] > 0 ? iap[0] : 0, x1 = iap[0] + 2 < BS ? iap[0] + 2 : BS;
+        for (int idy = y0; idy < y1; ++idy)
+          for (int idx = x0; idx < x1; ++idx) {
+            const int sx = idx - iap[0], sy = idy - iap[1];
+            const double wxwy = wghts[1][sy] * wghts[0][sx];
+            o->udef[idy][idx][0] += wxwy * udef[0];
+            o->udef[idy][idx][1] += wxwy * udef[1];
+            o->chi[idy][idx] += wxwy;
+            o->rho[idy][idx] = 1;
+            if (fabs(o->dist[idy][idx] + 1) < EPS) o->dist[idy][idx] = 1;
+          }
+      }
+    }
+  }
+  /* signedDistanceSqrt */
+  for (int iy = 0; iy < BS; iy++)
+    for (int ix = 0; ix < BS; ix++) {
+      const double normfac = o->chi[iy][ix] > EPS ? o->chi[iy][ix] : 1;
+      o->udef[iy][ix][0] /= normfac;
+      o->udef[iy][ix][1] /= normfac;
+      o->dist[iy][ix] = o->dist[iy][ix] >= 0 ? sqrt(o->dist[iy][ix]) : -sqrt(-o->dist[iy][ix]);
+      B[iy * BS + ix] = dmax(B[iy * BS + ix], o->dist[iy][ix]);
+    }
+  for (int iy = 0; iy < BS; iy++)
+    for (int ix = 0; ix < BS; ix++) { o->chi[iy][ix] = 0; o->rho[iy][ix] = 1; }
+}
+
+/* Fish::create (Fish.cpp:60-218) */
+static void fish_create(orc_sim* s, shape_t* sh)
+{
+  fish_t* f = sh->fish;
+  fish_frame(f, s->time);
+  fish_segments(f, sh->st.center, sh->st.orientation);
+  for (int b = 0; b < s->nb; ++b) { free(sh->ob[b]); sh->ob[b] = NULL; }
+  fframe_t F;
+  F.pos[0] = sh->st.center[0];
+  F.pos[1] = sh->st.center[1];
+  double sn, cs;
+  orc_det_sincos(sh->st.orientation, &sn, &cs);
+  F.R[0][0] = cs; F.R[0][1] = -sn; F.R[1][0] = sn; F.R[1][1] = cs;
+  double* TMP = s->f[ORC_TMP];
+  #pragma omp parallel
+  {
+    int* sl = (int*)malloc(sizeof(int) * (size_t)f->Nseg);
+    #pragma omp for schedule(dynamic, 1)
+    for (int b = 0; b < s->nb; ++b) {
+      double pStart[2], pEnd[2];
+      bpos(s, b, 0, 0, pStart);
+      bpos(s, b, BS - 1, BS - 1, pEnd);
+      int ns = 0;
+      for (int q = 0; q < f->Nseg; ++q)
+        if (seg_intersects(&f->seg[q], pStart, pEnd)) sl[ns++] = q;
+      if (!ns) continue;
+      oblock* o = (oblock*)calloc(1, sizeof(oblock));
+      sh->ob[b] = o;
+      put_fish_on_block(s, f, &F, b, sl, ns, o, TMP + (size_t)b * BS * BS);
+    }
+    free(sl);
   }
 }
 
@@ -535,8 +1154,10 @@ static void remove_moments(orc_sim* s, shape_t* sh)
   t->M = I[2]; t->J = I[3];
   const double dCx = t->center[0] - t->centerOfMass[0];
   const double dCy = t->center[1] - t->centerOfMass[1];
-  t->d_gm[0] = dCx * cos(t->orientation) + dCy * sin(t->orientation);
-  t->d_gm[1] = -dCx * sin(t->orientation) + dCy * cos(t->orientation);
+  double sn, cs;
+  orc_det_sincos(t->orientation, &sn, &cs);
+  t->d_gm[0] = dCx * cs + dCy * sn;
+  t->d_gm[1] = -dCx * sn + dCy * cs;
   #pragma omp parallel for schedule(dynamic)
   for (int b = 0; b < s->nb; ++b) {
     oblock* o = sh->ob[b];
@@ -582,9 +1203,12 @@ static void update_position(orc_sim* s, shape_t* sh, double dt)
   t->orientation += dt * t->omega;
   t->orientation = t->orientation > M_PI ? t->orientation - 2 * M_PI : t->orientation;
   t->orientation = t->orientation < -M_PI ? t->orientation + 2 * M_PI : t->orientation;
-  const double cosang = cos(t->orientation), sinang = sin(t->orientation);
+  double sinang, cosang;
+  orc_det_sincos(t->orientation, &sinang, &cosang);
   t->center[0] = t->centerOfMass[0] + cosang * t->d_gm[0] - sinang * t->d_gm[1];
   t->center[1] = t->centerOfMass[1] + sinang * t->d_gm[0] + cosang * t->d_gm[1];
+  /* Fish::updatePosition, Fish.cpp:220-225 */
+  if (sh->fish) sh->fish->theta_internal -= dt * sh->fish->angvel_internal;
 }
 
 /* PutObjectsOnGrid::operator(), PutObjectsOnGrid.cpp:186-241 */
@@ -619,7 +1243,10 @@ int orc_put_objects(orc_sim* s, double dt)
   }
   if (err) return err; /* reference: printf + exit(0), PutObjectsOnGrid.cpp:213-218 */
   checker(s);
-  for (int k = 0; k < s->nshapes; ++k) disk_create(s, &s->shapes[k]);
+  for (int k = 0; k < s->nshapes; ++k) {
+    if (s->shapes[k].fish) fish_create(s, &s->shapes[k]);
+    else disk_create(s, &s->shapes[k]);
+  }
   checker(s);
   for (int k = 0; k < s->nshapes; ++k) put_chi_on_grid(s, &s->shapes[k]);
   checker(s);

@@ -37,10 +37,20 @@ typedef struct {
 } orc_cfg;
 
 typedef struct {
-  int kind;              /* 0 = disk                                         */
+  int kind;              /* 0 = disk, 1 = carlingfish (CarlingFish.cpp)      */
   double radius, xpos, ypos, angle_deg, rhoS;
   int bFixedx, bFixedy, bForcedx, bForcedy, bBlockAng;
   double xvel, yvel, angvel, timeForced, tAccel;
+  /* carlingfish: -L -T -phi -amplitudeFactor (Fish.h:27-29,
+   * CarlingFish.cpp:371-376) and the midline / half-width laws */
+  double length, Tperiod, phaseShift, amplitudeFactor;
+  int envelope;          /* 0 = 4/33 (s + L/32) (CarlingFish.cpp:300-308),
+                            1 = L (env_a0 + env_a2 (s/L)^2), wavelength
+                            waveLength*L                                     */
+  double env_a0, env_a2, waveLength;
+  int profile;           /* 0 = AmplitudeFish::_width (CarlingFish.cpp:322-329),
+                            1 = NACA 4-digit half-thickness, ratio naca_t    */
+  double naca_t;
 } orc_shape_desc;
 
 /* Per-shape scalar state; same field order as cup2d_shape_state. */
@@ -101,6 +111,18 @@ int orc_poisson_dense(orc_sim* s, const double* rhs, double* sol);
 /* 1D FFT used by the solver (for tests): in-place complex, sign -1 forward,
  * +1 backward, unnormalized; interleaved re/im. */
 int orc_fft1d(double* data, int n, int sign);
+
+/* Swimmer state of shape k (carlingfish): Nm midline points; out (if not
+ * NULL) receives 10 arrays of Nm doubles in the order rS rX rY vX vY norX
+ * norY vNorX vNorY width (after the last PutObjectsOnGrid: CoM frame,
+ * rotated by theta_internal), then area_internal J_internal CoM_internal[2]
+ * vCoM_internal[2] theta_internal angvel_internal.  Returns Nm, -1 if shape k
+ * is not a swimmer. */
+int  orc_fish_info(const orc_sim* s, int k, double* out);
+/* sin and cos as used on the swimmer path (for tests) */
+void orc_det_sincos(double x, double* sn, double* cs);
+/* sum of n values in the fixed order used on the swimmer path (for tests) */
+double orc_lane_sum(const double* v, int n);
 
 /* timing of the last simulate() call, seconds: total and FFT-only */
 void orc_get_timing(const orc_sim* s, double* total, double* fft);

@@ -368,3 +368,87 @@ def test_towed_cylinder_cosine(cup, oracle_mod):
     g, o = _disk_pair(cup, oracle_mod, (16, 8), shapes, extent=1.0, nu=2e-5, CFL=0.1,
                       poisson="cosine")
     _run_and_compare(g, o, 12)
+
+
+# ------------------------------------------------------------------ swimmers
+FISH_VARIANTS = {
+    "carling": dict(L=0.2, T=1.0),
+    "quadratic_naca": dict(L=0.2, T=1.0, envelope="quadratic", profile="naca", phi=0.25),
+}
+
+
+def _fish_pair(cup, oracle_mod, bp, fish, extent=1.0, poisson="freespace", **kw):
+    kw = dict(extent=extent, nu=kw.pop("nu", 1e-4), CFL=kw.pop("CFL", 0.1), poisson=poisson, **kw)
+    g = cup.Simulation(*bp, shapes=[cup.carlingfish(**f) for f in fish], **kw)
+    o = oracle_mod.Oracle(*bp, fs_pad2=True, shapes=[oracle_mod.carlingfish(**f) for f in fish],
+                          **kw)
+    return g, o
+
+
+@pytest.mark.parametrize("variant", sorted(FISH_VARIANTS))
+@pytest.mark.parametrize("time", [0.0, 1.37])
+def test_fish_midline_bitexact(cup, oracle_mod, variant, time):
+    """Midline, normals, internal frame and momenta of the swimmer kernel equal
+    the oracle's bit for bit (same operation sequence, same sin/cos, same
+    summation order)."""
+    fish = [dict(xpos=0.5, ypos=0.25, angle=17.0, **FISH_VARIANTS[variant])]
+    g, o = _fish_pair(cup, oracle_mod, (16, 8), fish)
+    g.reset()
+    o.reset()
+    if time:
+        for sim in (g, o):
+            sim.set_scalars(time=time)
+            sim.put_objects(0.0)
+    a, b = g.fish_midline(0), o.fish_info(0)
+    assert a["Nm"] == b["Nm"]
+    for k in cup.FISH_ARRAYS + cup.FISH_SCALARS:
+        assert np.array_equal(a[k], b[k]), (k, np.max(np.abs(np.asarray(a[k]) - np.asarray(b[k]))))
+
+
+@pytest.mark.parametrize("variant", sorted(FISH_VARIANTS))
+def test_fish_put_objects(cup, oracle_mod, variant):
+    """PutObjectsOnGrid with a swimmer (Fish::create + PutFishOnBlocks gathered
+    per cell): chi and invRho bit-exact, udef after removeMoments within
+    round-off of the momentum sums."""
+    fish = [dict(xpos=0.47, ypos=0.26, angle=-23.0, **FISH_VARIANTS[variant])]
+    g, o = _fish_pair(cup, oracle_mod, (16, 8), fish)
+    g.reset()
+    o.reset()
+    for sim in (g, o):
+        sim.set_scalars(time=0.83)
+        sim.put_objects(0.0)
+    cg, co = g.get_field(cup.CHI), o.get_field(oracle_mod.CHI)
+    assert np.array_equal(cg, co), np.abs(cg - co).max()
+    assert np.array_equal(g.get_field(cup.INVRHO), o.get_field(oracle_mod.INVRHO))
+    ug, uo = g.get_field(cup.UDEF), o.get_field(oracle_mod.UDEF)
+    assert np.abs(ug - uo).max() <= 1e-13 * np.abs(uo).max()
+    sg, so = g.shape(), o.shape()
+    for k in ["M", "J"]:
+        assert abs(sg[k] - so[k]) <= 1e-13 * abs(so[k]), k
+    for k in ["centerOfMass", "center"]:
+        assert np.abs(np.array(sg[k]) - np.array(so[k])).max() < 1e-14
+    assert sg["n_obstacle_blocks"] == so["n_obstacle_blocks"]
+    assert sg["n_surface_points"] == so["n_surface_points"]
+
+
+def test_free_swimmer_steps(cup, oracle_mod):
+    fish = [dict(xpos=0.5, ypos=0.25, **FISH_VARIANTS["carling"])]
+    g, o = _fish_pair(cup, oracle_mod, (16, 8), fish, nu=1e-4, CFL=0.2)
+    _run_and_compare(g, o, 15)
+    a, b = g.fish_midline(0), o.fish_info(0)
+    assert abs(a["theta_internal"] - b["theta_internal"]) <= 1e-12
+
+
+def test_free_swimmer_device_loop_periodic(cup, oracle_mod):
+    """The C2-style swimmer (quadratic envelope, NACA profile) in a periodic
+    box next to a towed disk, advanced by the device loop."""
+    fish = [dict(xpos=0.9, ypos=0.5, angle=8.0, **FISH_VARIANTS["quadratic_naca"])]
+    g = cup.Simulation(32, 16, extent=2.0, nu=1e-4, CFL=0.2, poisson="periodic",
+                       shapes=[cup.carlingfish(**fish[0]),
+                               cup.disk(radius=0.05, xpos=1.5, ypos=0.4, xvel=0.1, bFixed=True,
+                                        bForced=True)])
+    o = oracle_mod.Oracle(32, 16, extent=2.0, nu=1e-4, CFL=0.2, poisson="periodic",
+                          shapes=[oracle_mod.carlingfish(**fish[0]),
+                                  oracle_mod.disk(radius=0.05, xpos=1.5, ypos=0.4, xvel=0.1,
+                                                  bFixed=True, bForced=True)])
+    _run_and_compare(g, o, 10, nshapes=2, device_loop=True)

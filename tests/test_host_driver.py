@@ -38,7 +38,7 @@ def test_driver_requires_bpdx():
 
 def test_driver_rejects_solver_without_device_path():
     _need_driver()
-    r = _run(["-bpdx", "2", "-bpdy", "2", "-poissonType", "hypre"])
+    r = _run(["-bpdx", "2", "-bpdy", "2", "-poissonType", "hypre", "-shapes", "disk_radius=0.1"])
     assert r.returncode == 2
     assert "no device solver" in r.stderr
 
@@ -111,3 +111,48 @@ def test_driver_force_logs(tmp_path):
     assert abs(rows[-1][10] - out["shapes"][0]["drag"]) <= 1e-5 * abs(out["shapes"][0]["drag"])
     plines = (tmp_path / "powerValues_0.dat").read_text().splitlines()
     assert plines[0].startswith("time Pthrust") and len(plines) == 6
+
+
+def test_driver_rejects_unknown_shape():
+    _need_driver()
+    r = _run(["-bpdx", "2", "-bpdy", "2", "-poissonType", "periodic", "-shapes",
+              "stefanfish_L=0.2_xpos=0.5_ypos=0.5"])
+    assert r.returncode == 2
+    assert "not supported" in r.stderr
+
+
+def test_driver_rejects_bad_fish_option():
+    _need_driver()
+    r = _run(["-bpdx", "2", "-bpdy", "2", "-poissonType", "periodic", "-shapes",
+              "carlingfish_L=0.2_xpos=0.5_ypos=0.5_envelope=cubic"])
+    assert r.returncode == 2
+    assert "envelope" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_carlingfish_matches_oracle(oracle_mod, tmp_path):
+    """A free CarlingFish from the -shapes line, 12 steps, periodic."""
+    _need_driver()
+    dump = str(tmp_path / "final.bin")
+    args = ["-bpdx", "16", "-bpdy", "8", "-CFL", "0.2", "-nu", "0.0001",
+            "-poissonType", "periodic", "-muteAll", "1", "-nsteps", "11", "-dumpFinal", dump,
+            "-shapes", "carlingfish_L=0.2_T=1.0_xpos=0.5_ypos=0.25_angle=10"]
+    print("cup2d_run carlingfish", flush=True)
+    r = _run(args)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["step"] == 12
+    o = oracle_mod.Oracle(16, 8, extent=1.0, nu=1e-4, CFL=0.2, poisson="periodic",
+                          shapes=[oracle_mod.carlingfish(0.5, 0.25, L=0.2, T=1.0, angle=10.0)])
+    o.reset()
+    for _ in range(12):
+        o.advance(o.calc_max_timestep())
+    NX, NY = 512, 256
+    buf = np.fromfile(dump, dtype=np.float64)
+    vel = buf[:2 * NX * NY].reshape(2, NY, NX)
+    ev = rel(vel, o.get_field(oracle_mod.VEL))
+    print(f"  rel L2 vel {ev:.3e}", flush=True)
+    assert ev < 1e-10
+    so = o.shape()
+    for key in ["u", "v", "omega"]:
+        assert abs(out["shapes"][0][key] - so[key]) <= 1e-10 * max(abs(so[key]), 1e-3), key
