@@ -82,7 +82,7 @@ EXPORTS = [
     "cup2d_body_update", "cup2d_penalize", "cup2d_pressure_single", "cup2d_pressure_iterator",
     "cup2d_compute_forces",
     "cup2d_advance", "cup2d_simulate", "cup2d_synchronize", "cup2d_poisson_dense",
-    "cup2d_grid_size", "cup2d_timer_enable", "cup2d_timer_read",
+    "cup2d_grid_size", "cup2d_timer_enable", "cup2d_timer_read", "cup2d_timer_sample",
 ]
 TIMERS = ["find_max_u", "dt", "advdiff", "flux_fix", "pressure_rhs", "rhs_obstacles", "fade",
           "fft_rows_fwd", "fft_cols", "fft_rows_inv", "pressure_correction", "put_objects",
@@ -129,6 +129,7 @@ def lib():
     L.cup2d_grid_size.argtypes = [P, C.POINTER(I), C.POINTER(I)]
     L.cup2d_timer_enable.argtypes = [P, C.c_char_p, I]
     L.cup2d_timer_read.argtypes = [P, C.c_char_p, dp, C.POINTER(I)]
+    L.cup2d_timer_sample.argtypes = [P, C.c_char_p, I]
     for n in EXPORTS:
         if n != "cup2d_last_error":
             getattr(L, n).restype = C.c_int
@@ -325,6 +326,10 @@ class Simulation:
 
     def timer_enable(self, kernel="*", reserve=1024):
         self._chk(self.L.cup2d_timer_enable(self.h, kernel.encode(), reserve))
+
+    def timer_sample(self, kernel="*", every=1):
+        """Time only every n-th launch of the kernel."""
+        self._chk(self.L.cup2d_timer_sample(self.h, kernel.encode(), every))
 
     def timer_read(self, kernel):
         """(total_ms, launches) since the last read of this timer."""

@@ -24,6 +24,13 @@ void timer_mark(cup2d_ctx* c, int id, bool start) { timer_mark_on(c, id, start, 
 void timer_mark_on(cup2d_ctx* c, int id, bool start, hipStream_t s) {
   Timer& t = c->timers[id];
   if (!t.on) return;
+  if (start) {
+    t.open = t.seen++ % t.every == 0;
+    if (!t.open) return;
+  } else {
+    if (!t.open) return;
+    t.open = false;
+  }
   const size_t need = 2 * (size_t)(t.used + 1);
   while (t.ev.size() < need) {
     hipEvent_t e;
@@ -49,10 +56,14 @@ int fail(cup2d_ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+// every entry point first enqueues a ComputeForces still pending from the
+// last whole step, so nothing else it enqueues can overtake it
 #define CHECK_CTX(c)                                        \
   do {                                                      \
     if (!(c)) return CUP2D_ERR_ARG;                         \
     (void)hipSetDevice((c)->device);                        \
+    const int frc_ = flush_forces(c);                       \
+    if (frc_) return frc_;                                  \
   } while (0)
 
 #define RC(x)                 \
@@ -101,7 +112,9 @@ int ensure_frozen(cup2d_ctx* c) {
 }
 
 int sync_all(cup2d_ctx* c) {
+  RC(flush_forces(c));
   RC(join_obstacles(c));
+  RC(join_forces(c));
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->aux));
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return CUP2D_OK;
@@ -259,6 +272,7 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
       hipEventCreateWithFlags(&c->ev_begin, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_obst, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_pen, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_frc, hipEventDisableTiming) != hipSuccess ||
       hipMalloc(&c->fsnap, sizeof(ForceSnap) * MAX_SHAPES) != hipSuccess) {
     c->err = "side stream creation failed";
     return bail(CUP2D_ERR_HIP);
@@ -349,6 +363,7 @@ int cup2d_destroy(cup2d_ctx* c) {
   if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
   if (c->ev_obst) (void)hipEventDestroy(c->ev_obst);
   if (c->ev_pen) (void)hipEventDestroy(c->ev_pen);
+  if (c->ev_frc) (void)hipEventDestroy(c->ev_frc);
   if (c->fsnap) (void)hipFree(c->fsnap);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -668,12 +683,28 @@ int cup2d_timer_enable(cup2d_ctx* c, const char* kernel, int reserve_launches) {
     Timer& t = c->timers[i];
     t.on = reserve_launches > 0;
     t.used = 0;
+    t.seen = 0;
+    t.open = false;
     while (t.ev.size() < 2 * (size_t)(reserve_launches > 0 ? reserve_launches : 0)) {
       hipEvent_t e;
       CUP2D_HIP_CHECK(c, hipEventCreate(&e));
       t.ev.push_back(e);
     }
   }
+  return CUP2D_OK;
+}
+
+int cup2d_timer_sample(cup2d_ctx* c, const char* kernel, int every) {
+  CHECK_CTX(c);
+  const bool all = kernel && std::strcmp(kernel, "*") == 0;
+  const int id = timer_index(kernel);
+  if ((!all && id < 0) || every < 1) return fail(c, CUP2D_ERR_ARG, "timer_sample: bad name or n");
+  for (int i = 0; i < T_NTIMERS; ++i)
+    if (all || i == id) {
+      c->timers[i].every = every;
+      c->timers[i].seen = 0;
+      c->timers[i].open = false;
+    }
   return CUP2D_OK;
 }
 

@@ -914,6 +914,10 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   }
   if (rc) return rc;
   {
+    // a ComputeForces still reading the previous pressure on the side stream
+    // must finish before it is overwritten
+    rc = join_forces(c);
+    if (rc) return rc;
     TimerScope ts(c, T_FFT_ROWS_INV);
     rc = rows_inv(c, P->Lr, P->spec, P->NYs, sol, NX, NX, NY, P->tw_r);
   }

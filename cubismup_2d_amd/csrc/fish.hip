@@ -6,9 +6,10 @@
 // trigonometry uses det_sincos and the midline integrals use a fixed
 // 256-lane summation order, so a CPU restatement that does the same produces
 // identical bits.  PutFishOnBlocks is a scatter in the reference (each
-// midline point writes into the cells around it); here every cell gathers the
-// points in the reference's loop order, which gives the same minimum-distance
-// winner and the same sums without atomics.
+// midline point writes into the cells around it); here the minimum-distance
+// surface winner comes from exact LDS atomic minima with the reference's
+// first-wins order, and each cell gathers its internal contributions in the
+// reference's loop order, so the sums are the same without float atomics.
 #include <cmath>
 #include <vector>
 
@@ -61,16 +62,22 @@ __device__ inline Mid mid_of(const FishDev& F) {
 }
 
 // trapezoid weight and _integrationFac1..3 (FishLibrary.h:108-118, w^3 as w*w*w)
-__device__ inline void facs(const FishDev& F, const Mid& M, int i, double* ds, double* fac1,
-                            double* fac2, double* fac3) {
-  const int Nm = F.Nm;
-  const double* rS = F.rS;
+__device__ inline void facs(const double* rS, const double* width, int Nm, const Mid& M, int i,
+                            double* ds, double* fac1, double* fac2, double* fac3) {
   *ds = (i == 0) ? rS[1] - rS[0] : ((i == Nm - 1) ? rS[Nm - 1] - rS[Nm - 2] : rS[i + 1] - rS[i - 1]);
-  const double w = F.width[i], w3 = w * w * w;
+  const double w = width[i], w3 = w * w * w;
   const double dnorXi = d_ds(rS, Nm, i, M.norX), dnorYi = d_ds(rS, Nm, i, M.norY);
   *fac1 = 2 * w;
   *fac2 = 2 * w3 * (dnorXi * M.norY[i] - dnorYi * M.norX[i]) / 3;
   *fac3 = 2 * w3 / 3;
+}
+
+// value of v in lane `lane` (uniform) of this wave
+__device__ inline double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // pairwise combination of NQ rows of NT lane partials: lane l += lane l + 128,
@@ -95,21 +102,29 @@ __device__ inline void lane_tree(double* red) {
 // integrateLinearMomentum, changeToCoMFrameLinear, integrateAngularMomentum,
 // changeToCoMFrameAngular (FishLibrary.cpp:86-176) and the AreaSegment boxes
 // in the computational frame (Fish.cpp:118-150, FishLibrary.cpp:295-342).
+// The midline arrays live in LDS (fish_midline_lds bytes) until the end.
 __global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__ fd,
                                                      const cup2d_shape_state* __restrict__ ss,
                                                      const StepState* __restrict__ st,
                                                      double h) {
-  __shared__ double red[5 * NT];
-  __shared__ double scan[2][FISH_MAX_NM];
+  extern __shared__ double lds[];
   const FishDev F = fd[blockIdx.x];
   const int Nm = F.Nm, t = threadIdx.x;
-  const Mid M = mid_of(F);
-  const double* rS = F.rS;
+  double* rS = lds;
+  double* width = lds + Nm;
+  const Mid M{lds + 2 * Nm, lds + 3 * Nm, lds + 4 * Nm, lds + 5 * Nm,
+              lds + 6 * Nm, lds + 7 * Nm, lds + 8 * Nm, lds + 9 * Nm};
+  double* red = lds + 10 * Nm;
+  for (int i = t; i < Nm; i += NT) {
+    rS[i] = F.rS[i];
+    width[i] = F.width[i];
+  }
   const double time = st->time, dt = st->dt;
   const double theta = F.st[6] - dt * F.st[7];
   const double T = F.Tperiod;
   const double rampFac = time < T ? det_sin(0.5 * M_PI * time / T) : 1.0;
   const double rampFacVel = time < T ? 0.5 * M_PI / T * det_cos(0.5 * M_PI * time / T) : 0.0;
+  __syncthreads();
   // 1. midline
   for (int i = t; i < Nm; i += NT) {
     const double lp = lat_pos(F, rS[i], time);
@@ -122,24 +137,39 @@ __global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__
     const double dx = sqrt(ds * ds - dy * dy);
     const double dVy = M.vY[i] - M.vY[i - 1];
     const double dVx = -dy / dx * dVy;
-    scan[0][i] = dx;
-    scan[1][i] = dVx;
+    M.rX[i] = dx;
+    M.vX[i] = dVx;
     M.norX[i - 1] = -dy / ds;
     M.norY[i - 1] = dx / ds;
     M.vNorX[i - 1] = -dVy / ds;
     M.vNorY[i - 1] = dVx / ds;
   }
   __syncthreads();
-  // running sums rX[i] += rX[i-1] in index order, one lane per array
-  if (t == 0 || t == 64) {
-    const int a = t == 0 ? 0 : 1;
-    double* out = a == 0 ? M.rX : M.vX;
-    double acc = 0;
-    out[0] = 0;
-    for (int i = 1; i < Nm; ++i) {
-      acc = scan[a][i] + acc;
-      out[i] = acc;
+  // running sums of the increments dx, dVx (rX[i] += rX[i-1]) in a fixed
+  // two-level order, one wave per array: lane l sums its chunk of
+  // c = ceil((Nm-1)/64) consecutive increments in index order, the lane totals
+  // give the chunk offsets one lane after the other, and each element is its
+  // chunk-local sum plus the chunk offset.  (The reference adds strictly in
+  // index order; the two orders round differently in the last bit.)
+  if (t < 128) {
+    const int l = t & 63;
+    double* a = t < 64 ? M.rX : M.vX;
+    const int c = (Nm - 1 + 63) / 64;
+    const int i0 = 1 + l * c, i1 = i0 + c < Nm ? i0 + c : Nm;
+    double p = 0;
+    for (int i = i0; i < i1; ++i) {
+      p = a[i] + p;
+      a[i] = p;
     }
+    const double total = i0 < i1 ? p : 0.0;
+    double off = 0, mine = 0;
+    for (int j = 0; j < 64; ++j) {  // offset of lane j: totals of lanes < j
+      mine = l == j ? off : mine;
+      off = readlane_d(total, j) + off;
+    }
+    if (l > 0)
+      for (int i = i0; i < i1; ++i) a[i] = a[i] + mine;
+    if (l == 0) a[0] = 0;
   }
   if (t == 128) {
     M.norX[Nm - 1] = M.norX[Nm - 2];
@@ -153,7 +183,7 @@ __global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__
     double a[5] = {0, 0, 0, 0, 0};
     for (int i = t; i < Nm; i += NT) {
       double ds, fac1, fac2, fac3;
-      facs(F, M, i, &ds, &fac1, &fac2, &fac3);
+      facs(rS, width, Nm, M, i, &ds, &fac1, &fac2, &fac3);
       a[0] += fac1 * ds / 2;
       a[1] += (M.rX[i] * fac1 + M.norX[i] * fac2) * ds / 2;
       a[2] += (M.rY[i] * fac1 + M.norY[i] * fac2) * ds / 2;
@@ -179,7 +209,7 @@ __global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__
     double am = 0, jj = 0;
     for (int i = t; i < Nm; i += NT) {
       double ds, fac1, fac2, fac3;
-      facs(F, M, i, &ds, &fac1, &fac2, &fac3);
+      facs(rS, width, Nm, M, i, &ds, &fac1, &fac2, &fac3);
       const double rX = M.rX[i], rY = M.rY[i], vX = M.vX[i], vY = M.vY[i];
       const double norX = M.norX[i], norY = M.norY[i], vNorX = M.vNorX[i], vNorY = M.vNorY[i];
       const double tmp_M = (rX * vY - rY * vX) * fac1 +
@@ -230,17 +260,24 @@ __global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__
     M.vNorY[Nm - 1] = M.vNorY[Nm - 2];
   }
   __syncthreads();
+  // the step's midline for PutFishOnBlocks (rX rY vX vY norX norY vNorX vNorY)
+  for (int i = t; i < 8 * Nm; i += NT) F.mid[i] = M.rX[i];
   // 4-5. AreaSegments in the computational frame of (center, orientation)
   const cup2d_shape_state& S = ss[F.shape];
   double sn, cs;
   det_sincos(S.orientation, &sn, &cs);
   const double R[2][2] = {{cs, -sn}, {sn, cs}};
   double ub[4] = {1e300, -1e300, 1e300, -1e300};
+  // bound on any component of a deformation velocity of PutFishOnBlocks,
+  // |R (v + o vNor)| <= |vX| + |vY| + w (|vNorX| + |vNorY|), for its fixed-point sums
+  double vmax = 0;
+  for (int i = t; i < Nm; i += NT)
+    vmax = dmax(vmax, fabs(M.vX[i]) + fabs(M.vY[i]) + width[i] * (fabs(M.vNorX[i]) + fabs(M.vNorY[i])));
   for (int i = t; i < F.Nseg; i += NT) {
     const int next_idx = (i + 1) * (Nm - 1) / F.Nseg, idx = i * (Nm - 1) / F.Nseg;
     double bbox[2][2] = {{1e9, -1e9}, {1e9, -1e9}};
     for (int s = idx; s <= next_idx; ++s) {
-      const double w = F.width[s];
+      const double w = width[s];
       const double xB0 = M.rX[s] - M.norX[s] * w, xB1 = M.rX[s] + M.norX[s] * w;
       const double yB0 = M.rY[s] - M.norY[s] * w, yB1 = M.rY[s] + M.norY[s] * w;
       const double maxX = dmax(xB0, xB1), minX = dmin(xB0, xB1);
@@ -286,8 +323,10 @@ __global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__
     ub[2] = dmin(ub[2], out[SG_LAB + 2]);
     ub[3] = dmax(ub[3], out[SG_LAB + 3]);
   }
+  __syncthreads();
 #pragma unroll
   for (int q = 0; q < 4; ++q) red[q * NT + t] = ub[q];
+  red[4 * NT + t] = vmax;
   for (int s = NT / 2; s >= 1; s >>= 1) {
     __syncthreads();
     if (t < s) {
@@ -295,11 +334,15 @@ __global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__
       red[NT + t] = dmax(red[NT + t], red[NT + t + s]);
       red[2 * NT + t] = dmin(red[2 * NT + t], red[2 * NT + t + s]);
       red[3 * NT + t] = dmax(red[3 * NT + t], red[3 * NT + t + s]);
+      red[4 * NT + t] = dmax(red[4 * NT + t], red[4 * NT + t + s]);
     }
   }
   __syncthreads();
   if (t == 0) {
     for (int q = 0; q < 4; ++q) F.box[q] = red[q * NT];
+    int e = 0;
+    (void)frexp(red[4 * NT], &e);  // vmax < 2^e
+    F.st[8] = ldexp(1.0, 52 - e);  // velocity scale Su: |w ud Su| < 2^52
     F.st[0] = area; F.st[1] = J;
     F.st[2] = C0; F.st[3] = C1;
     F.st[4] = V0; F.st[5] = V1;
@@ -369,24 +412,38 @@ constexpr int FISH_MAX_SEG = FISH_MAX_NM / 8;
 }  // namespace
 
 // PutFishOnBlocks::operator() (FishLibrary.cpp:403-417) for every obstacle
-// slot that belongs to a swimmer: one workgroup per slot, four cells of one
-// column per thread.
-//   constructSurface (:438-573): the cell's signed squared distance, chi weight
-//     W and W*udef come from the first surface point, in loop order, with the
-//     smallest squared distance among those whose 4x4 support covers the cell;
-//   constructInternl (:575-638): hat-kernel P2M of the internal slices, summed
-//     in loop order;
+// slot that belongs to a swimmer: one workgroup per slot.
+//   constructSurface (:438-573): a cell takes its signed squared distance,
+//     chi weight W and W*udef from the first surface point, in loop order,
+//     with the smallest squared distance among those whose 4x4 support covers
+//     it.  Found in two parallel passes over (point, cell) pairs: an LDS
+//     atomic minimum of the distance (non-negative doubles order like their
+//     bit patterns), then an atomic minimum of the loop index among the points
+//     that attain it exactly.
+//   constructInternl (:575-638): hat-kernel P2M of the internal slices, each
+//     cell summing its contributions in loop order (a gather: slices whose
+//     extent misses a wave's rows are skipped by the whole wave).
 //   signedDistanceSqrt (:419-436): udef /= chi, signed square root.
 // Writes the slot's dist, udef and rho (chi is set by the Towers pass).
+namespace {
+enum { E_NW = 0, E_RX, E_RY, E_NX, E_NY, E_VX, E_VY, E_VNX, E_VNY, ENTD };
+union PlaceLds {
+  SurfPt pts[NT];
+  double ent[NT][ENTD];
+};
+}  // namespace
+
 __global__ __launch_bounds__(NT) void k_fish_place(Grid g, const ShapeParams* __restrict__ sp,
                                                    const cup2d_shape_state* __restrict__ ss,
                                                    const FishDev* __restrict__ fd,
                                                    Obstacles ob) {
   __shared__ int sl[FISH_MAX_SEG];
   __shared__ int s_tmp[NT];
-  __shared__ SurfPt pts[NT];
-  __shared__ double ent[NT][5];  // internal slice: m0 m1 n0 n1 (index units), Nw
-  __shared__ int ent_ss[NT];
+  __shared__ unsigned long long s_best[BS * BS];
+  __shared__ int s_win[BS * BS];
+  __shared__ unsigned long long s_qw[BS * BS], s_qu[BS * BS], s_qv[BS * BS];
+  __shared__ int s_touch[BS * BS];
+  __shared__ PlaceLds U;
   const int s = blockIdx.x;
   if (s >= ob.cnt[0]) return;
   const int k = ob.slot_shape[s];
@@ -398,6 +455,11 @@ __global__ __launch_bounds__(NT) void k_fish_place(Grid g, const ShapeParams* __
   const int b = ob.slot_block[s], bx = b % g.bpdx, by = b / g.bpdx;
   const double h = g.h, invh = 1.0 / h;
   const double org0 = cell_pos(bx, 0, g.hb, h), org1 = cell_pos(by, 0, g.hb, h);
+  const unsigned long long ONE = (unsigned long long)__double_as_longlong(1.0);
+  for (int i = t; i < BS * BS; i += NT) {
+    s_best[i] = ONE;  // |dist| starts at 1 (dist = -1)
+    s_win[i] = 0x7fffffff;
+  }
   // segments meeting this block, in segment order (Fish.cpp:158-173)
   int nsl;
   {
@@ -430,67 +492,65 @@ __global__ __launch_bounds__(NT) void k_fish_place(Grid g, const ShapeParams* __
     Fr.pos1 = S.center[1];
     Fr.R00 = cs; Fr.R01 = -sn; Fr.R10 = sn; Fr.R11 = cs;
   }
-  const int cx = t % BS, cy0 = (t / BS) * 4;
-  double px[4][2];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    px[c][0] = cell_pos(bx, cx, g.hb, h);
-    px[c][1] = cell_pos(by, cy0 + c, g.hb, h);
-  }
-  // ---- constructSurface: first point with the smallest squared distance
-  double best[4] = {1, 1, 1, 1};  // |dist| starts at 1 (dist = -1)
-  int win[4] = {-1, -1, -1, -1};
-  for (int j0 = 0; j0 < nsl; j0 += SEG_PER_CHUNK) {
-    const int nj = nsl - j0 < SEG_PER_CHUNK ? nsl - j0 : SEG_PER_CHUNK;
-    if (t < nj * PTS_PER_SEG) {
-      const int j = j0 + t / PTS_PER_SEG, r = t % PTS_PER_SEG;
-      const int s0 = 8 * sl[j], s1 = 8 * sl[j] + 8;
-      const int first = s0 > 1 ? s0 : 1, last = s1 < Nm - 2 ? s1 : Nm - 2;
-      const int sidx = first + r / 2, signp = (r & 1) ? 1 : -1;
-      SurfPt P;
-      if (sidx <= last) {
-        surf_point(F, M, Fr, sidx, signp, org0, org1, invh, P);
-        P.valid = 1;
-      } else {
-        P.valid = 0;
+  // ---- constructSurface
+  const int nchunk = (nsl + SEG_PER_CHUNK - 1) / SEG_PER_CHUNK;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int ch = 0; ch < nchunk; ++ch) {
+      const int j0 = ch * SEG_PER_CHUNK;
+      const int nj = nsl - j0 < SEG_PER_CHUNK ? nsl - j0 : SEG_PER_CHUNK;
+      const int np = nj * PTS_PER_SEG;
+      if (pass == 0 || nchunk > 1) {  // (one chunk: pass 1 reuses the points)
+        __syncthreads();
+        if (t < np) {
+          const int j = j0 + t / PTS_PER_SEG, r = t % PTS_PER_SEG;
+          const int s0 = 8 * sl[j], s1 = 8 * sl[j] + 8;
+          const int first = s0 > 1 ? s0 : 1, last = s1 < Nm - 2 ? s1 : Nm - 2;
+          const int sidx = first + r / 2, signp = (r & 1) ? 1 : -1;
+          SurfPt P;
+          if (sidx <= last) {
+            surf_point(F, M, Fr, sidx, signp, org0, org1, invh, P);
+            P.valid = 1;
+          } else {
+            P.valid = 0;
+          }
+          U.pts[t] = P;
+        }
+        __syncthreads();
       }
-      pts[t] = P;
-    }
-    __syncthreads();
-    const int np = nj * PTS_PER_SEG;
-    for (int p = 0; p < np; ++p) {
-      if (!pts[p].valid) continue;
-      const int a0 = pts[p].iap[0], a1 = pts[p].iap[1];
-      if (cx < a0 - 1 || cx > a0 + 2 || cy0 + 3 < a1 - 1 || cy0 > a1 + 2) continue;
-      const double m0 = pts[p].myP[0], m1 = pts[p].myP[1];
-      const double P0 = pts[p].pP[0], P1 = pts[p].pP[1], Q0 = pts[p].pM[0], Q1 = pts[p].pM[1];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int cy = cy0 + c;
-        if (cy < a1 - 1 || cy > a1 + 2) continue;
-        const double my[2] = {m0, m1}, pp[2] = {P0, P1}, pm[2] = {Q0, Q1};
-        const double dist0 = dist_sq(px[c], my), distP = dist_sq(px[c], pp), distM = dist_sq(px[c], pm);
+      for (int idx = t; idx < np * 16; idx += NT) {
+        const int p = idx >> 4, q = idx & 15;
+        if (!U.pts[p].valid) continue;
+        const int sx = U.pts[p].iap[0] - 1 + (q & 3), sy = U.pts[p].iap[1] - 1 + (q >> 2);
+        if (sx < 0 || sx >= BS || sy < 0 || sy >= BS) continue;
+        const double pc[2] = {cell_pos(bx, sx, g.hb, h), cell_pos(by, sy, g.hb, h)};
+        const double dist0 = dist_sq(pc, U.pts[p].myP);
+        const double distP = dist_sq(pc, U.pts[p].pP);
+        const double distM = dist_sq(pc, U.pts[p].pM);
         const double d1 = dmin(dmin(dist0, distP), distM);
-        if (d1 < best[c]) {
-          best[c] = d1;
-          win[c] = (j0 * PTS_PER_SEG) + p;
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(d1);
+        if (pass == 0) {
+          atomicMin(&s_best[sy * BS + sx], bits);
+        } else if (d1 < 1.0 && bits == s_best[sy * BS + sx]) {
+          atomicMin(&s_win[sy * BS + sx], j0 * PTS_PER_SEG + p);
         }
       }
     }
     __syncthreads();
   }
+  const int cx = t % BS, cy0 = (t / BS) * 4;
   double dist[4], chi[4], udx[4], udy[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     dist[c] = -1; chi[c] = 0; udx[c] = 0; udy[c] = 0;
-    if (win[c] < 0) continue;
-    const int j = win[c] / PTS_PER_SEG, r = win[c] % PTS_PER_SEG;
+    const int win = s_win[(cy0 + c) * BS + cx];
+    if (win == 0x7fffffff) continue;
+    const int j = win / PTS_PER_SEG, r = win % PTS_PER_SEG;
     const int s0 = 8 * sl[j];
     const int first = s0 > 1 ? s0 : 1;
     const int sidx = first + r / 2, signp = (r & 1) ? 1 : -1;
     SurfPt P;
     surf_point(F, M, Fr, sidx, signp, org0, org1, invh, P);
-    double p[2] = {px[c][0], px[c][1]};
+    double p[2] = {cell_pos(bx, cx, g.hb, h), cell_pos(by, cy0 + c, g.hb, h)};
     const double dist0 = dist_sq(p, P.myP), distP = dist_sq(p, P.pP), distM = dist_sq(p, P.pM);
     Fr.from_comp(p);
     int close_s = sidx, secnd_s = sidx + (distP < distM ? 1 : -1);
@@ -532,72 +592,89 @@ __global__ __launch_bounds__(NT) void k_fish_place(Grid g, const ShapeParams* __
     dist[c] = sign2d * dist1;
     chi[c] = W;
   }
-  // ---- constructInternl: hat P2M of the slices at offsets iw*h
+  // ---- constructInternl: hat P2M of the slice points at offsets iw*h,
+  // every (point, patch cell) pair in parallel, summed in 64-bit fixed point
+  // (weights x 2^52, weighted velocities x Su) so the sums do not depend on
+  // the order of the additions
+  __syncthreads();
+  for (int i = t; i < BS * BS; i += NT) {
+    s_qw[i] = 0; s_qu[i] = 0; s_qv[i] = 0; s_touch[i] = 0;
+  }
+  const double Su = F.st[8];
   for (int j0 = 0; j0 < nsl; j0 += SEG_PER_ICHUNK) {
     const int nj = nsl - j0 < SEG_PER_ICHUNK ? nsl - j0 : SEG_PER_ICHUNK;
+    const int ne = nj * ENT_PER_SEG;
     __syncthreads();
-    if (t < nj * ENT_PER_SEG) {
+    int npts = 0;
+    if (t < ne) {
       const int j = j0 + t / ENT_PER_SEG, r = t % ENT_PER_SEG;
       const int s0 = 8 * sl[j], s1 = 8 * sl[j] + 8;
       const int first = s0 > 1 ? s0 : 1, last = s1 < Nm - 2 ? s1 : Nm - 2;
       const int sidx = first + r;
       if (sidx <= last) {
-        double xp[2] = {M.rX[sidx], M.rY[sidx]};
-        Fr.to_comp(xp);
-        ent[t][0] = (xp[0] - org0) * invh;
-        ent[t][1] = (xp[1] - org1) * invh;
-        ent[t][2] = Fr.R00 * M.norX[sidx] + Fr.R01 * M.norY[sidx];
-        ent[t][3] = Fr.R10 * M.norX[sidx] + Fr.R11 * M.norY[sidx];
-        ent[t][4] = floor(F.width[sidx] / h);
-        ent_ss[t] = sidx;
-      } else {
-        ent_ss[t] = -1;
+        const int Nw = (int)floor(F.width[sidx] / h);
+        npts = Nw >= 1 ? 2 * Nw - 1 : 0;
+        double* E = U.ent[t];
+        E[E_NW] = Nw;
+        E[E_RX] = M.rX[sidx]; E[E_RY] = M.rY[sidx];
+        E[E_NX] = M.norX[sidx]; E[E_NY] = M.norY[sidx];
+        E[E_VX] = M.vX[sidx]; E[E_VY] = M.vY[sidx];
+        E[E_VNX] = M.vNorX[sidx]; E[E_VNY] = M.vNorY[sidx];
       }
     }
+    s_tmp[t] = npts;
     __syncthreads();
-    const int ne = nj * ENT_PER_SEG;
-    for (int e = 0; e < ne; ++e) {
-      const int sidx = ent_ss[e];
-      if (sidx < 0) continue;
-      const double m0 = ent[e][0], m1 = ent[e][1], n0 = ent[e][2], n1 = ent[e][3];
-      const int Nw = (int)ent[e][4];
-      const double d0 = cx - m0;
-      // cells within sqrt(2) of the slice line, in index units
-      if (fabs((cy0 - m1) * n0 - d0 * n1) > 1.5 && fabs((cy0 + 3 - m1) * n0 - d0 * n1) > 1.5 &&
-          ((cy0 - m1) * n0 - d0 * n1) * ((cy0 + 3 - m1) * n0 - d0 * n1) > 0)
-        continue;
+    for (int off = 1; off < NT; off <<= 1) {  // inclusive scan of the point counts
+      const int v = t >= off ? s_tmp[t - off] : 0;
+      __syncthreads();
+      s_tmp[t] += v;
+      __syncthreads();
+    }
+    const int P = s_tmp[NT - 1];
+    for (int idx = t; idx < P; idx += NT) {
+      int lo = 0, hi = ne - 1;  // first entry whose inclusive count exceeds idx
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_tmp[mid] > idx) hi = mid; else lo = mid + 1;
+      }
+      const double* E = U.ent[lo];
+      const int Nw = (int)E[E_NW];
+      const int iw = -Nw + 1 + (idx - (lo ? s_tmp[lo - 1] : 0));
+      const double offsetW = iw * h;
+      double xp[2] = {E[E_RX] + offsetW * E[E_NX], E[E_RY] + offsetW * E[E_NY]};
+      Fr.to_comp(xp);
+      xp[0] = (xp[0] - org0) * invh;
+      xp[1] = (xp[1] - org1) * invh;
+      const double ap0 = floor(xp[0]), ap1 = floor(xp[1]);
+      const int iap0 = (int)ap0, iap1 = (int)ap1;
+      if (iap0 + 2 <= 0 || iap0 >= BS || iap1 + 2 <= 0 || iap1 >= BS) continue;
+      double ud[2] = {E[E_VX] + offsetW * E[E_VNX], E[E_VY] + offsetW * E[E_VNY]};
+      Fr.vel_to_comp(ud);
+      const double wx[2] = {1 - fabs(xp[0] - ap0), 1 - fabs(xp[0] - (ap0 + 1))};
+      const double wy[2] = {1 - fabs(xp[1] - ap1), 1 - fabs(xp[1] - (ap1 + 1))};
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int cy = cy0 + c;
-        const double d1 = cy - m1;
-        const double perp = d1 * n0 - d0 * n1;
-        if (fabs(perp) > 1.5) continue;
-        const double tp = d0 * n0 + d1 * n1;
-        int lo = (int)ceil(tp - 1.5), hi = (int)floor(tp + 1.5);
-        lo = lo > -Nw + 1 ? lo : -Nw + 1;
-        hi = hi < Nw - 1 ? hi : Nw - 1;
-        for (int iw = lo; iw <= hi; ++iw) {
-          const double offsetW = iw * h;
-          double xp[2] = {M.rX[sidx] + offsetW * M.norX[sidx], M.rY[sidx] + offsetW * M.norY[sidx]};
-          Fr.to_comp(xp);
-          xp[0] = (xp[0] - org0) * invh;
-          xp[1] = (xp[1] - org1) * invh;
-          const double ap0 = floor(xp[0]), ap1 = floor(xp[1]);
-          const int iap0 = (int)ap0, iap1 = (int)ap1;
-          const int sx = cx - iap0, sy = cy - iap1;
-          if (sx < 0 || sx > 1 || sy < 0 || sy > 1) continue;
-          double ud[2] = {M.vX[sidx] + offsetW * M.vNorX[sidx], M.vY[sidx] + offsetW * M.vNorY[sidx]};
-          Fr.vel_to_comp(ud);
-          const double wx = 1 - fabs(xp[0] - (sx == 0 ? ap0 : ap0 + 1));
-          const double wy = 1 - fabs(xp[1] - (sy == 0 ? ap1 : ap1 + 1));
-          const double wxwy = wy * wx;
-          udx[c] += wxwy * ud[0];
-          udy[c] += wxwy * ud[1];
-          chi[c] += wxwy;
-          if (fabs(dist[c] + 1) < EPSD) dist[c] = 1;
-        }
+      for (int q = 0; q < 4; ++q) {
+        const int sx = q & 1, sy = q >> 1;
+        const int cxq = iap0 + sx, cyq = iap1 + sy;
+        if (cxq < 0 || cxq >= BS || cyq < 0 || cyq >= BS) continue;
+        const double wxwy = wy[sy] * wx[sx];
+        const int cell = cyq * BS + cxq;
+        atomicAdd(&s_qw[cell], (unsigned long long)__double2ll_rn(wxwy * 0x1p52));
+        atomicAdd(&s_qu[cell], (unsigned long long)__double2ll_rn(wxwy * ud[0] * Su));
+        atomicAdd(&s_qv[cell], (unsigned long long)__double2ll_rn(wxwy * ud[1] * Su));
+        s_touch[cell] = 1;
       }
     }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int cell = (cy0 + c) * BS + cx;
+    if (!s_touch[cell]) continue;
+    if (fabs(dist[c] + 1) < EPSD) dist[c] = 1;
+    chi[c] += (double)(long long)s_qw[cell] * 0x1p-52;
+    udx[c] += (double)(long long)s_qu[cell] / Su;
+    udy[c] += (double)(long long)s_qv[cell] / Su;
   }
   // ---- signedDistanceSqrt
   const size_t base = (size_t)s * BS * BS;
@@ -662,6 +739,8 @@ int fish_discretize(const cup2d_shape_desc* d, double h, std::vector<double>& rS
   return Nm;
 }
 
+static size_t fish_midline_lds(int Nm) { return sizeof(double) * ((size_t)10 * Nm + 5 * NT); }
+
 int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape) {
   if ((int)c->hfish.size() >= MAX_FISH) {
     c->err = "too many swimmers";
@@ -684,7 +763,7 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape) {
   F.env_a0 = d->env_a0;
   F.env_a2 = d->env_a2;
   F.waveLength = d->waveLength;
-  const size_t nd = (size_t)Nm * 10 + (size_t)F.Nseg * FISH_SEGD + 4 + 8;
+  const size_t nd = (size_t)Nm * 10 + (size_t)F.Nseg * FISH_SEGD + 4 + 16;
   double* buf = nullptr;
   CUP2D_HIP_CHECK(c, hipMalloc(&buf, sizeof(double) * nd));
   CUP2D_HIP_CHECK(c, hipMemset(buf, 0, sizeof(double) * nd));
@@ -697,6 +776,9 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape) {
   CUP2D_HIP_CHECK(c, hipMemcpy(F.rS, rS.data(), sizeof(double) * Nm, hipMemcpyHostToDevice));
   CUP2D_HIP_CHECK(c, hipMemcpy(F.width, width.data(), sizeof(double) * Nm, hipMemcpyHostToDevice));
   if (!c->fishd) CUP2D_HIP_CHECK(c, hipMalloc(&c->fishd, sizeof(FishDev) * MAX_FISH));
+  CUP2D_HIP_CHECK(c, hipFuncSetAttribute((const void*)k_fish_midline,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)fish_midline_lds(FISH_MAX_NM)));
   const int idx = (int)c->hfish.size();
   CUP2D_HIP_CHECK(c, hipMemcpy(c->fishd + idx, &F, sizeof(F), hipMemcpyHostToDevice));
   c->hfish.push_back(F);
@@ -714,8 +796,10 @@ void fish_free_all(cup2d_ctx* c) {
 
 int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s) {
   if (c->hfish.empty()) return CUP2D_OK;
-  hipLaunchKernelGGL(k_fish_midline, dim3((unsigned)c->hfish.size()), dim3(NT), 0, s, c->fishd,
-                     c->ss, c->st, c->g.h);
+  // one launch per swimmer: the LDS size depends on its number of points
+  for (size_t f = 0; f < c->hfish.size(); ++f)
+    hipLaunchKernelGGL(k_fish_midline, dim3(1), dim3(NT), fish_midline_lds(c->hfish[f].Nm), s,
+                       c->fishd + f, c->ss, c->st, c->g.h);
   return launch_check(c, "fish midline");
 }
 

@@ -22,7 +22,7 @@ constexpr int BS = CUP2D_BS;
 constexpr int MAX_SHAPES = 64;
 constexpr int NPART = 24;  // doubles of partial sums per obstacle slot
 constexpr int MAX_FISH = 8;
-constexpr int FISH_MAX_NM = 4096;  // midline points per swimmer
+constexpr int FISH_MAX_NM = 1536;  // midline points per swimmer (LDS of the midline kernel)
 constexpr int FISH_SEGD = 16;      // doubles per AreaSegment record
 
 // Constant per-shape parameters (Shape.h:22-41, ShapesSimple.h:16-17).
@@ -73,7 +73,8 @@ struct FishDev {
   double* mid;    // [8][Nm] rX rY vX vY norX norY vNorX vNorY of this step
   double* seg;    // [Nseg][FISH_SEGD] AreaSegment: safe, lab box, object box, |normalI|, |normalJ|
   double* box;    // [4] union of the segments' lab boxes (x0 x1 y0 y1)
-  double* st;     // [8] area J CoM_internal[2] vCoM_internal[2] theta_internal angvel_internal
+  double* st;     // [16] area J CoM_internal[2] vCoM_internal[2] theta_internal angvel_internal,
+                  // velocity scale of the fixed-point P2M sums
 };
 
 // Obstacle pool (ObstacleBlock.h:266-378, one slot per (shape, block)).
@@ -111,6 +112,9 @@ struct Timer {
   bool on = false;
   std::vector<hipEvent_t> ev;  // start/stop pairs
   int used = 0;                // pairs recorded since the last read
+  int every = 1;               // time every n-th launch
+  long long seen = 0;          // launches since enabled
+  bool open = false;           // a start event of this launch was recorded
 };
 
 }  // namespace cup2d
@@ -123,8 +127,16 @@ struct cup2d_ctx {
   // side stream: the PutObjectsOnGrid chain after the list builder runs there,
   // overlapping advDiff and the RHS (neither reads obstacle data)
   hipStream_t aux = nullptr;
-  hipEvent_t ev_begin = nullptr, ev_obst = nullptr, ev_pen = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_obst = nullptr, ev_pen = nullptr, ev_frc = nullptr;
   bool obst_pending = false;
+  // ComputeForces and the step end (time += dt) of the last whole step, not
+  // enqueued yet.  The next whole step applies the step end in its list
+  // builder and runs ComputeForces on the side stream after its obstacle
+  // chain (joined before the pressure is overwritten); any other call first
+  // enqueues both on the main stream (flush_forces).
+  bool forces_pending = false;
+  bool end_pending = false;
+  bool frc_join = false;  // the main stream must wait for ev_frc
   int host_step = 0;                       // step index for the CFL ramp (Simulation.cpp:264)
   cup2d::ForceSnap* fsnap = nullptr;       // device [MAX_SHAPES]
   int device = 0;
@@ -240,6 +252,10 @@ int enqueue_rhs_obstacles(cup2d_ctx* c, int compute = 0);
 int enqueue_body_update(cup2d_ctx* c);
 int enqueue_penalize(cup2d_ctx* c, double* maxpart);
 int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux = 0);
+// enqueue pending ComputeForces and step end on the main stream
+int flush_forces(cup2d_ctx* c);
+// main stream waits for a ComputeForces running on the side stream
+int join_forces(cup2d_ctx* c);
 int enqueue_shapes_reset(cup2d_ctx* c);
 int enqueue_obst_reset(cup2d_ctx* c);
 

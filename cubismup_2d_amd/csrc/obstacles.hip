@@ -18,6 +18,12 @@
 
 namespace cup2d {
 
+#define RC_OB(x)              \
+  do {                        \
+    const int rc_ = (x);      \
+    if (rc_) return rc_;      \
+  } while (0)
+
 constexpr double EPSD = 2.220446049250313e-16;  // std::numeric_limits<double>::epsilon()
 constexpr int NT = 256;
 constexpr int LDS_RED = (NT / 64) * 20;  // block_sum_multi of up to 20 values
@@ -117,50 +123,68 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
                                                    const double* __restrict__ maxpart,
                                                    int nmaxpart,
                                                    const double* __restrict__ ramp,
-                                                   int compute_dt, int step_idx, int phase) {
+                                                   int compute_dt, int step_idx, int phase,
+                                                   int end_prev) {
   __shared__ int s_a[LCAP], s_b[LCAP], s_c[LCAP], s_d[LCAP];
   __shared__ int s_tmp[NT];
   __shared__ int s_cstart[MAX_SHAPES + 1];
   __shared__ int s_rng[MAX_SHAPES][3];
   __shared__ double s_box[MAX_SHAPES][4];
   __shared__ double lds[LDS_RED];
+  __shared__ double s_seg[(FISH_MAX_NM / 8) * FISH_SEGD];
   const int t = threadIdx.x;
   const int ns = g.nshapes;
   const int nub_prev = pv.cnt[1];
-  if (compute_dt && phase != 2) {
-    const double maxU = block_max<NT>(max_array<NT>(maxpart, nmaxpart + nub_prev), lds);
-    if (t == 0) {
-      const double h = g.h;
-      const double dtFourier = h * h / g.nu, dtCFL = maxU < 2.2e-16 ? 1 : h / maxU;
-      double maxUb = 0;
-      for (int k = 0; k < ns; ++k) maxUb = dmax(maxUb, sqrt(ss[k].u * ss[k].u + ss[k].v * ss[k].v));
-      const double dtBody = maxUb < 2.2e-16 ? 1 : h / maxUb;
-      const double mn = dmin(dmin(dtCFL, dtFourier), dtBody);
-      double dt = g.CFL * mn;
-      if (step_idx < 100) dt = ramp[step_idx] * mn;
-      st->dt = dt;
-      st->maxU = maxU;
-    }
+  __shared__ double s_uv[MAX_SHAPES][2];
+  __shared__ double s_step[3];  // dt, U_inf
+  if (end_prev && t == 0) {  // the previous whole step ends (Simulation.cpp:289-290)
+    st->time += st->dt_done;
+    st->step += 1;
   }
-  if (phase != 1)
-    for (int u = t; u < nub_prev; u += NT) {
-      obmap[pv.ub_block[u]] = 0;
-      ubidx[pv.ub_block[u]] = -1;
+  if (phase != 2) {
+    if (t < ns) {
+      s_uv[t][0] = ss[t].u;
+      s_uv[t][1] = ss[t].v;
     }
-  __syncthreads();
-  if (t == 0 && phase != 2) {
-    const double dt = st->dt;
-    int nSum[2] = {0, 0};
-    double uSum[2] = {0, 0};
-    for (int k = 0; k < ns; ++k) {
-      if (sp[k].bFixedx) { nSum[0]++; uSum[0] -= ss[k].u; }
-      if (sp[k].bFixedy) { nSum[1]++; uSum[1] -= ss[k].v; }
+    __syncthreads();
+    if (compute_dt) {
+      const double maxU = block_max<NT>(max_array<NT>(maxpart, nmaxpart + nub_prev), lds);
+      if (t == 0) {
+        const double h = g.h;
+        const double dtFourier = h * h / g.nu, dtCFL = maxU < 2.2e-16 ? 1 : h / maxU;
+        double maxUb = 0;
+        for (int k = 0; k < ns; ++k)
+          maxUb = dmax(maxUb, sqrt(s_uv[k][0] * s_uv[k][0] + s_uv[k][1] * s_uv[k][1]));
+        const double dtBody = maxUb < 2.2e-16 ? 1 : h / maxUb;
+        const double mn = dmin(dmin(dtCFL, dtFourier), dtBody);
+        double dt = g.CFL * mn;
+        if (step_idx < 100) dt = ramp[step_idx] * mn;
+        st->dt = dt;
+        st->maxU = maxU;
+        s_step[0] = dt;
+      }
+    } else if (t == 0) {
+      s_step[0] = st->dt;
     }
-    if (nSum[0] > 0) st->uinf[0] = uSum[0] / nSum[0];
-    if (nSum[1] > 0) st->uinf[1] = uSum[1] / nSum[1];
-    const double U0 = st->uinf[0], U1 = st->uinf[1];
-    for (int k = 0; k < ns; ++k) {
-      cup2d_shape_state& T = ss[k];
+    if (t == 0) {  // U_inf, in shape order
+      int nSum[2] = {0, 0};
+      double uSum[2] = {0, 0};
+      for (int k = 0; k < ns; ++k) {
+        if (sp[k].bFixedx) { nSum[0]++; uSum[0] -= s_uv[k][0]; }
+        if (sp[k].bFixedy) { nSum[1]++; uSum[1] -= s_uv[k][1]; }
+      }
+      double U0 = st->uinf[0], U1 = st->uinf[1];
+      if (nSum[0] > 0) U0 = uSum[0] / nSum[0];
+      if (nSum[1] > 0) U1 = uSum[1] / nSum[1];
+      st->uinf[0] = U0;
+      st->uinf[1] = U1;
+      s_step[1] = U0;
+      s_step[2] = U1;
+    }
+    __syncthreads();
+    if (t < ns) {  // each shape moves on its own lane
+      const double dt = s_step[0], U0 = s_step[1], U1 = s_step[2];
+      cup2d_shape_state& T = ss[t];
       T.centerOfMass[0] += dt * (T.u + U0);
       T.centerOfMass[1] += dt * (T.v + U1);
       T.labCenterOfMass[0] += dt * T.u;
@@ -176,6 +200,12 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
         st->error = CUP2D_ERR_BODY_OUT;
     }
   }
+  if (phase != 1)
+    for (int u = t; u < nub_prev; u += NT) {
+      obmap[pv.ub_block[u]] = 0;
+      ubidx[pv.ub_block[u]] = -1;
+    }
+  __syncthreads();
   if (phase == 1) return;
   if (t == 0) {
     int nc = 0;
@@ -214,11 +244,27 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
     const int local = cidx - s_cstart[k], w = s_rng[k][2];
     const int bx = s_rng[k][0] + local % w, by = s_rng[k][1] + local / w;
     const double bbox[2][2] = {{s_box[k][0], s_box[k][1]}, {s_box[k][2], s_box[k][3]}};
-    const bool hit = sp[k].kind == 1 ? fish_touching(g, fd[sp[k].fish], bx, by)
-                                     : touching(g, bx, by, bbox);
-    s_a[cidx] = hit ? 1 : 0;
+    s_a[cidx] = sp[k].kind == 0 && touching(g, bx, by, bbox) ? 1 : 0;
     s_b[cidx] = bx + g.bpdx * by;
     s_c[cidx] = k;
+  }
+  // swimmers: any of the segments (staged in LDS) meets the block; all
+  // (candidate, segment) pairs in parallel
+  for (int k = 0; k < ns; ++k) {
+    if (sp[k].kind != 1) continue;
+    const FishDev& F = fd[sp[k].fish];
+    const int nseg = F.Nseg;
+    __syncthreads();
+    for (int i = t; i < nseg * FISH_SEGD; i += NT) s_seg[i] = F.seg[i];
+    __syncthreads();
+    const int c0 = s_cstart[k], nck = s_cstart[k + 1] - c0;
+    for (int pr = t; pr < nck * nseg; pr += NT) {
+      const int cidx = c0 + pr / nseg, i = pr % nseg;
+      const int b = s_b[cidx];
+      double start[2], end[2];
+      block_corners(g, b % g.bpdx, b / g.bpdx, start, end);
+      if (seg_intersects(s_seg + i * FISH_SEGD, start, end)) s_a[cidx] = 1;
+    }
   }
   __syncthreads();
   // slot_of_shape_start: flags before each shape's first candidate
@@ -334,23 +380,16 @@ __device__ inline double disk_dist(const Grid& g, const ShapeParams& P,
 //
 // Swimmers contribute the distance PutFishOnBlocks stored in their slots of
 // that block (signedDistanceSqrt, FishLibrary.cpp:431).
+// nbs[0..nnb) are the swimmer slots of the block that holds the (clamped)
+// cell.
 __device__ inline double grid_sdf(const Grid& g, const ShapeParams* __restrict__ sp,
                                   const cup2d_shape_state* __restrict__ ss, const Obstacles& ob,
-                                  const int* __restrict__ ubidx, int gx, int gy) {
+                                  const int* nbs, int nnb, int gx, int gy) {
   gx = gx < 0 ? 0 : (gx >= g.NX ? g.NX - 1 : gx);
   gy = gy < 0 ? 0 : (gy >= g.NY ? g.NY - 1 : gy);
   double v = -1;
-  if (g.nfish > 0) {
-    const int u = ubidx[(gy / BS) * g.bpdx + gx / BS];
-    if (u >= 0) {
-      const int s0 = ob.ub_start[u], n = ob.ub_count[u];
-      for (int q = 0; q < n; ++q) {
-        const int s = ob.ub_slots[s0 + q];
-        if (sp[ob.slot_shape[s]].kind == 1)
-          v = dmax(v, ob.dist[(size_t)s * BS * BS + (gy % BS) * BS + gx % BS]);
-      }
-    }
-  }
+  for (int q = 0; q < nnb; ++q)
+    v = dmax(v, ob.dist[(size_t)nbs[q] * BS * BS + (gy % BS) * BS + gx % BS]);
   for (int k = 0; k < g.nshapes; ++k) {
     if (sp[k].kind != 0) continue;
     const double safety = 2 * g.h, R = sp[k].radius;
@@ -383,6 +422,8 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
                                                    double* __restrict__ udy,
                                                    double* __restrict__ invrho) {
   __shared__ double lds[LDS_RED];
+  __shared__ int s_nbs[4][MAX_FISH];  // swimmer slots of the E, W, N, S neighbour blocks
+  __shared__ int s_nbn[4];
   const int nub = ob.cnt[1], w = blockIdx.x;
   if (w >= nub) {
     if (w >= nub + pv.cnt[1]) return;
@@ -392,6 +433,21 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
   }
   const int b = ob.ub_block[w];
   const int bx = b % g.bpdx, by = b / g.bpdx;
+  if (threadIdx.x < 4) {
+    const int d = threadIdx.x;
+    int nx = bx + (d == 0) - (d == 1), ny = by + (d == 2) - (d == 3);
+    nx = nx < 0 ? 0 : (nx >= g.bpdx ? g.bpdx - 1 : nx);  // clamped like the cells
+    ny = ny < 0 ? 0 : (ny >= g.bpdy ? g.bpdy - 1 : ny);
+    int n = 0;
+    const int u = g.nfish > 0 ? ubidx[ny * g.bpdx + nx] : -1;
+    if (u >= 0)
+      for (int q = 0; q < ob.ub_count[u]; ++q) {
+        const int s = ob.ub_slots[ob.ub_start[u] + q];
+        if (sp[ob.slot_shape[s]].kind == 1 && n < MAX_FISH) s_nbs[d][n++] = s;
+      }
+    s_nbn[d] = n;
+  }
+  __syncthreads();
   clear_block(g, b, chi, udx, udy, invrho);
   const int s0 = ob.ub_start[w], cnt = ob.ub_count[w];
   const double h = g.h, i2h = 0.5 / h, fac = 0.5 * h;
@@ -411,18 +467,20 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
       // the shape's own distance inside the block, the grid SDF across its
       // edges (PutObjectsOnGrid.cpp:53-56)
       double sd, distPx, distMx, distPy, distMy;
+#define GSDF(d, x, y) grid_sdf(g, sp, ss, ob, s_nbs[d], s_nbn[d], x, y)
       if (fish) {
         sd = o.dist[i];
-        distPx = ix + 1 == BS ? grid_sdf(g, sp, ss, ob, ubidx, gx + 1, gy) : o.dist[i + 1];
-        distMx = ix == 0 ? grid_sdf(g, sp, ss, ob, ubidx, gx - 1, gy) : o.dist[i - 1];
-        distPy = iy + 1 == BS ? grid_sdf(g, sp, ss, ob, ubidx, gx, gy + 1) : o.dist[i + BS];
-        distMy = iy == 0 ? grid_sdf(g, sp, ss, ob, ubidx, gx, gy - 1) : o.dist[i - BS];
+        distPx = ix + 1 == BS ? GSDF(0, gx + 1, gy) : o.dist[i + 1];
+        distMx = ix == 0 ? GSDF(1, gx - 1, gy) : o.dist[i - 1];
+        distPy = iy + 1 == BS ? GSDF(2, gx, gy + 1) : o.dist[i + BS];
+        distMy = iy == 0 ? GSDF(3, gx, gy - 1) : o.dist[i - BS];
       } else {
         sd = disk_dist(g, P, S, gx, gy);
-        distPx = ix + 1 == BS ? grid_sdf(g, sp, ss, ob, ubidx, gx + 1, gy) : disk_dist(g, P, S, gx + 1, gy);
-        distMx = ix == 0 ? grid_sdf(g, sp, ss, ob, ubidx, gx - 1, gy) : disk_dist(g, P, S, gx - 1, gy);
-        distPy = iy + 1 == BS ? grid_sdf(g, sp, ss, ob, ubidx, gx, gy + 1) : disk_dist(g, P, S, gx, gy + 1);
-        distMy = iy == 0 ? grid_sdf(g, sp, ss, ob, ubidx, gx, gy - 1) : disk_dist(g, P, S, gx, gy - 1);
+        distPx = ix + 1 == BS ? GSDF(0, gx + 1, gy) : disk_dist(g, P, S, gx + 1, gy);
+        distMx = ix == 0 ? GSDF(1, gx - 1, gy) : disk_dist(g, P, S, gx - 1, gy);
+        distPy = iy + 1 == BS ? GSDF(2, gx, gy + 1) : disk_dist(g, P, S, gx, gy + 1);
+        distMy = iy == 0 ? GSDF(3, gx, gy - 1) : disk_dist(g, P, S, gx, gy - 1);
+#undef GSDF
         o.dist[i] = sd;
         o.rho[i] = rhoS;
         o.udx[i] = 0; o.udy[i] = 0;
@@ -506,12 +564,18 @@ __device__ inline void shape_com(const Obstacles& ob, const cup2d_shape_state* s
 
 // Shape::integrateObstBlock (Shape.cpp:117-155): per-slot partials about the
 // updated centre of mass.
-__global__ __launch_bounds__(NT) void k_obst_moments(Grid g,
-                                                     const cup2d_shape_state* __restrict__ ss,
+__device__ void obst_moments_wg(const Grid& g, const cup2d_shape_state* __restrict__ ss,
+                                const Obstacles& ob, double* lds);
+
+__global__ __launch_bounds__(NT) void k_obst_moments(Grid g, const cup2d_shape_state* __restrict__ ss,
                                                      const StepState* __restrict__ st,
                                                      Obstacles ob) {
   __shared__ double lds[LDS_RED];
-  if ((int)blockIdx.x >= ob.cnt[0]) return;
+  if ((int)blockIdx.x < ob.cnt[0]) obst_moments_wg(g, ss, ob, lds);
+}
+
+__device__ void obst_moments_wg(const Grid& g, const cup2d_shape_state* __restrict__ ss,
+                                const Obstacles& ob, double* lds) {
   const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   SlotView o = slot(ob, s);
@@ -541,7 +605,7 @@ __global__ __launch_bounds__(NT) void k_obst_moments(Grid g,
 
 // Shape::removeMoments (Shape.cpp:157-212) + putObjectVelOnGrid
 // (PutObjectsOnGrid.cpp:149-184), per obstacle block, shapes in order.
-__global__ __launch_bounds__(NT) void k_obst_udef(Grid g, const cup2d_shape_state* __restrict__ ss,
+__global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __restrict__ ss,
                                                   const StepState* __restrict__ st, Obstacles ob,
                                                   const double* __restrict__ chi,
                                                   double* __restrict__ udx,
@@ -577,11 +641,12 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, const cup2d_shape_stat
   }
 }
 
+
 // one workgroup, after k_obst_udef: commit centre of mass (PutObjectsOnGrid.cpp:
 // 118-122), M, J and d_gm (Shape.cpp:165-172) once every block has used the
 // previous values
-__global__ __launch_bounds__(NT) void k_obst_commit(Grid g, cup2d_shape_state* __restrict__ ss,
-                                                    Obstacles ob) {
+__device__ void obst_commit_wg(const Grid& g, cup2d_shape_state* __restrict__ ss,
+                               const Obstacles& ob) {
   __shared__ double lds[LDS_RED];
   for (int k = 0; k < g.nshapes; ++k) {
     double C[2], mass;
@@ -608,20 +673,42 @@ __global__ __launch_bounds__(NT) void k_obst_commit(Grid g, cup2d_shape_state* _
   }
 }
 
+__global__ __launch_bounds__(NT) void k_obst_commit(Grid g, cup2d_shape_state* __restrict__ ss,
+                                                    Obstacles ob) {
+  obst_commit_wg(g, ss, ob);
+}
+
 static int grid_cap(cup2d_ctx* c) { return c->ob.cap > 0 ? c->ob.cap : 1; }
 
 // PutObjectsOnGrid::operator() (PutObjectsOnGrid.cpp:186-241).
 // overlap: run everything after the list builder on the side stream; the
 // caller joins with join_obstacles() before the next reader of obstacle data.
+static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, int end_step);
+
+// ComputeForces of a whole step: 1 = behind the next step's list builder on
+// the side stream, 0 = on the side stream behind an event after the
+// penalization
+#ifndef CUP2D_FORCES_DEFER
+#define CUP2D_FORCES_DEFER 1
+#endif
+
 int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   const int G = grid_cap(c);
+  // the previous whole step's ComputeForces runs after this step's obstacle
+  // chain on the side stream, its step end in this step's list builder
+  const bool defer_forces = overlap && c->forces_pending;
+  if (!defer_forces) RC_OB(flush_forces(c));
+  const int end_prev = c->end_pending ? 1 : 0;
+  c->forces_pending = false;
+  c->end_pending = false;
+  RC_OB(join_forces(c));  // its lists are about to be rebuilt
   std::swap(c->ob, c->ob_prev);
   const bool fish = !c->hfish.empty();
   {
     TimerScope ts(c, T_PUT_OBJECTS);
     hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
                        c->ob, c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg,
-                       c->ramp, compute_dt, c->host_step, fish ? 1 : 0);
+                       c->ramp, compute_dt, c->host_step, fish ? 1 : 0, end_prev);
   }
   hipStream_t s = c->stream;
   if (overlap) {
@@ -634,19 +721,20 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
                        c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp, 0,
-                       c->host_step, 2);
+                       c->host_step, 2, 0);
     rc = enqueue_fish_place(c, s);
     if (rc) return rc;
   }
   if (c->g.nshapes > 0) {
     hipLaunchKernelGGL(k_obst_place, dim3(2 * G), dim3(NT), 0, s, c->g, c->sp, c->ss, c->ob,
                        c->ob_prev, c->obmap, c->ubidx, c->chi, c->udx, c->udy, c->invrho);
-    hipLaunchKernelGGL(k_obst_moments, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob);
     // Disks carry no deformation velocity (Disk::create leaves udef = 0), so
     // removeMoments and putObjectVelOnGrid would only add zeros: skipped
-    // unless a shape kind with a deformation velocity is present.
+    // unless a shape kind with a deformation velocity is present.  The last
+    // workgroup of the last kernel commits the shapes.
     bool deforming = false;
     for (const ShapeParams& p : c->hsp) deforming |= p.kind != 0;
+    hipLaunchKernelGGL(k_obst_moments, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob);
     if (deforming)
       hipLaunchKernelGGL(k_obst_udef, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob, c->chi,
                          c->udx, c->udy);
@@ -655,6 +743,11 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   if (overlap) {
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_obst, c->aux));
     c->obst_pending = true;
+    if (defer_forces) {  // the previous step's ComputeForces, on its own lists
+      RC_OB(launch_forces(c, c->aux, c->ob_prev, 0));
+      CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_frc, c->aux));
+      c->frc_join = true;
+    }
   }
   return launch_check(c, "put_objects");
 }
@@ -900,14 +993,10 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
 }
 
 // PressureSingle::integrateMomenta (PressureSingle.cpp:23-67), per slot.
-__global__ __launch_bounds__(NT) void k_momenta(Grid g, const ShapeParams* __restrict__ sp,
-                                                cup2d_shape_state* __restrict__ ss,
-                                                StepState* __restrict__ st, Obstacles ob,
-                                                const double* __restrict__ u,
-                                                const double* __restrict__ v) {
-  __shared__ double lds[LDS_RED];
-  const int nob = ob.cnt[0];
-  if ((int)blockIdx.x >= nob) return;
+__device__ void momenta_wg(const Grid& g, const cup2d_shape_state* __restrict__ ss,
+                           const StepState* __restrict__ st, const Obstacles& ob,
+                           const double* __restrict__ u, const double* __restrict__ v,
+                           double* lds) {
   const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   SlotView o = slot(ob, s);
@@ -943,10 +1032,9 @@ __global__ __launch_bounds__(NT) void k_momenta(Grid g, const ShapeParams* __res
 }
 
 // collide: also preventCollidingObstacles (PressureSingle only).
-__global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __restrict__ sp,
-                                                   cup2d_shape_state* __restrict__ ss,
-                                                   StepState* __restrict__ st, Obstacles ob,
-                                                   ForceSnap* __restrict__ snap, int collide) {
+__device__ void body_update_tail(const Grid& g, const ShapeParams* __restrict__ sp,
+                                 cup2d_shape_state* __restrict__ ss, StepState* __restrict__ st,
+                                 const Obstacles& ob, ForceSnap* __restrict__ snap, int collide) {
   body_solve_wg(g, sp, ss, st, ob, snap, collide);
   __syncthreads();
   // state ComputeForces uses (it may run after the next step moved the bodies)
@@ -958,6 +1046,23 @@ __global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __
     snap[threadIdx.x] = f;
   }
   if (threadIdx.x == 0) st->dt_done = st->dt;
+}
+
+__global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __restrict__ sp,
+                                                   cup2d_shape_state* __restrict__ ss,
+                                                   StepState* __restrict__ st, Obstacles ob,
+                                                   ForceSnap* __restrict__ snap, int collide) {
+  body_update_tail(g, sp, ss, st, ob, snap, collide);
+}
+
+// PressureSingle::integrateMomenta (PressureSingle.cpp:23-67) per slot.
+__global__ __launch_bounds__(NT) void k_momenta(Grid g, const ShapeParams* __restrict__ sp,
+                                                cup2d_shape_state* __restrict__ ss,
+                                                StepState* __restrict__ st, Obstacles ob,
+                                                const double* __restrict__ u,
+                                                const double* __restrict__ v) {
+  __shared__ double lds[LDS_RED];
+  if ((int)blockIdx.x < ob.cnt[0]) momenta_wg(g, ss, st, ob, u, v, lds);
 }
 
 int enqueue_body_update(cup2d_ctx* c) {
@@ -1182,14 +1287,9 @@ __device__ void forces_fin_wg(const Grid& g, cup2d_shape_state* __restrict__ ss,
 }
 
 // ComputeForces.cpp:16-107, per slot; surface points are the cells with D>0.
-__global__ __launch_bounds__(NT) void k_forces(Grid g, const ForceSnap* __restrict__ snap,
-                                               const StepState* __restrict__ st, Obstacles ob,
-                                               const double* __restrict__ u,
-                                               const double* __restrict__ v,
-                                               const double* __restrict__ p) {
-  __shared__ double lds[LDS_RED];
-  const int nob = ob.cnt[0];
-  if ((int)blockIdx.x >= nob) return;
+__device__ void forces_wg(const Grid& g, const ForceSnap* __restrict__ snap, const Obstacles& ob,
+                          const double* __restrict__ u, const double* __restrict__ v,
+                          const double* __restrict__ p, double* lds) {
   const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   SlotView o = slot(ob, s);
@@ -1242,31 +1342,84 @@ __global__ __launch_bounds__(NT) void k_forces(Grid g, const ForceSnap* __restri
     for (int j = 0; j < 18; ++j) o.part[P_FRC + j] = a[j];
 }
 
+// the step end of a whole step (Simulation.cpp:289-290) with the dt its body
+// update used
+__global__ void k_step_end_done(StepState* st) {
+  st->time += st->dt_done;
+  st->step += 1;
+}
+
+// ComputeForces per slot (ComputeForces.cpp:16-107)
+__global__ __launch_bounds__(NT) void k_forces(Grid g, const ForceSnap* __restrict__ snap,
+                                               Obstacles ob, const double* __restrict__ u,
+                                               const double* __restrict__ v,
+                                               const double* __restrict__ p) {
+  __shared__ double lds[LDS_RED];
+  if ((int)blockIdx.x < ob.cnt[0]) forces_wg(g, snap, ob, u, v, p, lds);
+}
+
+// shape totals (Shape::computeForces, Shape.cpp:252-300) and, with end_step,
+// the step end
 __global__ __launch_bounds__(NT) void k_forces_fin(Grid g, cup2d_shape_state* __restrict__ ss,
                                                    Obstacles ob, StepState* __restrict__ st,
                                                    int end_step, const ForceSnap* __restrict__ snap) {
   forces_fin_wg(g, ss, ob, st, end_step, snap);
 }
 
+static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, int end_step) {
+  TimerScope ts(c, T_FORCES, s);
+  hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, s, c->g, c->fsnap, lists, c->u,
+                     c->v, c->pres);
+  hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(NT), 0, s, c->g, c->ss, lists, c->st, end_step,
+                     c->fsnap);
+  return launch_check(c, "forces");
+}
+
+// ComputeForces::operator() + the step end.  on_aux: defer it to the side
+// stream of the next step's PutObjectsOnGrid, where it overlaps advDiff: it
+// reads the body snapshot of the body update, this step's obstacle lists
+// (the next step builds the other list buffer) and this step's velocity and
+// pressure, none of which is overwritten before the next step joins the side
+// stream.  The step end adds dt_done (set by the body update), not dt.
 int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux) {
+  RC_OB(flush_forces(c));
   if (c->g.nshapes == 0) {
     if (end_step) return enqueue_step_end(c);
     return CUP2D_OK;
   }
-  hipStream_t s = c->stream;
-  if (on_aux) {
-    // ComputeForces overlaps the next step: it reads the body snapshot and
-    // this step's obstacle lists, which the next step does not overwrite
+  if (on_aux && end_step) {
+#if CUP2D_FORCES_DEFER
+    c->forces_pending = true;
+    c->end_pending = true;
+    return CUP2D_OK;
+#else
+    // on the side stream right after the penalization (one more event)
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_pen, c->stream));
     CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_pen, 0));
-    s = c->aux;
+    return launch_forces(c, c->aux, c->ob, end_step);
+#endif
   }
-  TimerScope ts(c, T_FORCES, s);
-  hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, s, c->g, c->fsnap, c->st, c->ob,
-                     c->u, c->v, c->pres);
-  hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(NT), 0, s, c->g, c->ss, c->ob, c->st, end_step,
-                     c->fsnap);
-  return launch_check(c, "forces");
+  return launch_forces(c, c->stream, c->ob, end_step);
+}
+
+int flush_forces(cup2d_ctx* c) {
+  if (c->forces_pending) {
+    c->forces_pending = false;
+    RC_OB(launch_forces(c, c->stream, c->ob, 0));
+  }
+  if (c->end_pending) {
+    c->end_pending = false;
+    hipLaunchKernelGGL(k_step_end_done, dim3(1), dim3(1), 0, c->stream, c->st);
+    return launch_check(c, "step end");
+  }
+  return CUP2D_OK;
+}
+
+int join_forces(cup2d_ctx* c) {
+  if (!c->frc_join) return CUP2D_OK;
+  CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_frc, 0));
+  c->frc_join = false;
+  return CUP2D_OK;
 }
 
 // Shape::resetAll (Shape.h:64-88).

@@ -194,6 +194,9 @@ int cup2d_grid_size(const cup2d_ctx* ctx, int* NX, int* NY);
  * context's stream.  timer_read returns and clears the summed duration. */
 int cup2d_timer_enable(cup2d_ctx* ctx, const char* kernel, int reserve_launches);
 int cup2d_timer_read(cup2d_ctx* ctx, const char* kernel, double* total_ms, int* launches);
+/* Time only every n-th launch of the kernel ("*": all timers; n >= 1, default
+ * 1), so a long device loop can be timed with few event pairs on its stream. */
+int cup2d_timer_sample(cup2d_ctx* ctx, const char* kernel, int every);
 
 #ifdef __cplusplus
 }

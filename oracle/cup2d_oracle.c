@@ -60,6 +60,7 @@ typedef struct {
   double *rS, *rX, *rY, *vX, *vY, *norX, *norY, *vNorX, *vNorY, *width;
   double area_internal, J_internal, CoM_internal[2], vCoM_internal[2];
   double theta_internal, angvel_internal, angvel_internal_prev;
+  double Su; /* scale of the fixed-point deformation-velocity sums */
   fseg_t* seg;
 } fish_t;
 
@@ -634,7 +635,27 @@ static void fish_midline(fish_t* f, double t)
     f->vNorX[i - 1] = -dVy / ds;
     f->vNorY[i - 1] = dVx / ds;
   }
-  for (int i = 1; i < Nm; ++i) { rX[i] += rX[i - 1]; vX[i] += vX[i - 1]; }
+  /* running sums rX[i] += rX[i-1]: the reference adds in index order; here
+   * (as on the device) in a fixed two-level order: 64 chunks of
+   * c = ceil((Nm-1)/64) increments, each summed in index order, chunk offsets
+   * summed in chunk order, element = chunk-local sum + offset */
+  {
+    const int c = (Nm - 1 + 63) / 64;
+    double* arr[2] = {rX, vX};
+    for (int q = 0; q < 2; ++q) {
+      double* a = arr[q];
+      double off = 0;
+      for (int l = 0; l < 64; ++l) {
+        const int i0 = 1 + l * c, i1 = i0 + c < Nm ? i0 + c : Nm;
+        double p = 0;
+        for (int i = i0; i < i1; ++i) { p = a[i] + p; a[i] = p; }
+        const double total = i0 < i1 ? p : 0.0;
+        if (l > 0)
+          for (int i = i0; i < i1; ++i) a[i] = a[i] + off;
+        off = total + off;
+      }
+    }
+  }
   f->norX[Nm - 1] = f->norX[Nm - 2];
   f->norY[Nm - 1] = f->norY[Nm - 2];
   f->vNorX[Nm - 1] = f->vNorX[Nm - 2];
@@ -745,6 +766,16 @@ static void fish_frame(fish_t* f, double t)
     f->vY[i] = R[1][0] * qx + R[1][1] * qy;
   }
   fish_normals(f);
+  /* bound on any component of a deformation velocity of PutFishOnBlocks,
+   * |R (v + o vNor)| <= |vX| + |vY| + w (|vNorX| + |vNorY|): the internal
+   * P2M sums below use the power-of-two scale Su with |w ud Su| < 2^52 */
+  double vmax = 0;
+  for (int i = 0; i < Nm; ++i)
+    vmax = dmax(vmax, fabs(f->vX[i]) + fabs(f->vY[i]) +
+                          f->width[i] * (fabs(f->vNorX[i]) + fabs(f->vNorY[i])));
+  int e = 0;
+  (void)frexp(vmax, &e);
+  f->Su = ldexp(1.0, 52 - e);
 }
 
 /* Fish::create step 4-5 (Fish.cpp:118-150): segment boxes, AreaSegment
@@ -945,7 +976,17 @@ static void put_fish_on_block(const orc_sim* s, const fish_t* f, const fframe_t*
       }
     }
   }
-  /* constructInternl */
+  /* constructInternl.  The reference adds each contribution to udef and chi
+   * in loop order; here they are summed in 64-bit fixed point (weights
+   * x 2^52, weighted velocities x Su, each rounded to the nearest integer)
+   * and added to the surface values at the end, so the result does not depend
+   * on the summation order (the device sums in parallel).  This differs from
+   * the reference's sequential double sums by rounding only. */
+  static long long qw[BS][BS], qu[BS][BS], qv[BS][BS];
+  static char touched[BS][BS];
+  #pragma omp threadprivate(qw, qu, qv, touched)
+  memset(qw, 0, sizeof(qw)); memset(qu, 0, sizeof(qu)); memset(qv, 0, sizeof(qv));
+  memset(touched, 0, sizeof(touched));
   for (int i = 0; i < ns; ++i) {
     const fseg_t* S = &f->seg[sl[i]];
     const int firstSegm = S->s0 > 1 ? S->s0 : 1;
@@ -977,15 +1018,23 @@ static void put_fish_on_block(const orc_sim* s, const fish_t* f, const fframe_t*
           for (int idx = x0; idx < x1; ++idx) {
             const int sx = idx - iap[0], sy = idy - iap[1];
             const double wxwy = wghts[1][sy] * wghts[0][sx];
-            o->udef[idy][idx][0] += wxwy * udef[0];
-            o->udef[idy][idx][1] += wxwy * udef[1];
-            o->chi[idy][idx] += wxwy;
+            qu[idy][idx] += llrint(wxwy * udef[0] * f->Su);
+            qv[idy][idx] += llrint(wxwy * udef[1] * f->Su);
+            qw[idy][idx] += llrint(wxwy * 0x1p52);
+            touched[idy][idx] = 1;
             o->rho[idy][idx] = 1;
             if (fabs(o->dist[idy][idx] + 1) < EPS) o->dist[idy][idx] = 1;
           }
       }
     }
   }
+  for (int iy = 0; iy < BS; iy++)
+    for (int ix = 0; ix < BS; ix++) {
+      if (!touched[iy][ix]) continue;
+      o->chi[iy][ix] += (double)qw[iy][ix] * 0x1p-52;
+      o->udef[iy][ix][0] += (double)qu[iy][ix] / f->Su;
+      o->udef[iy][ix][1] += (double)qv[iy][ix] / f->Su;
+    }
   /* signedDistanceSqrt */
   for (int iy = 0; iy < BS; iy++)
     for (int ix = 0; ix < BS; ix++) {
