@@ -205,6 +205,9 @@ int pressure_iterator(cup2d_ctx* c, int* iters_out, double* reldf_out) {
 // previous step (device loop) instead of the value already on the device.
 int advance_enqueued(cup2d_ctx* c, int compute_dt) {
   RC(enqueue_put_objects(c, compute_dt, CUP2D_OBST_OVERLAP));
+  // the swimmers' time-only midline part of the next step, behind this step's
+  // obstacle chain on the side stream (off the critical path)
+  if (CUP2D_OBST_OVERLAP) RC(enqueue_fish_mid_next(c, c->aux, c->host_step & 1));
   if (c->cfg.iterative_penal) {
     RC(enqueue_advdiff(c, 0));
     RC(pressure_iterator(c, nullptr, nullptr));
@@ -493,6 +496,7 @@ int cup2d_set_scalars(cup2d_ctx* c, double time, int step, double uinfx, double 
   CHECK_CTX(c);
   RC(sync_all(c));
   c->host_step = step;
+  c->midA_ready = false;  // computed for the old time
   StepState s;
   CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
   s.time = time; s.step = step; s.uinf[0] = uinfx; s.uinf[1] = uinfy;
@@ -530,6 +534,7 @@ int cup2d_reset(cup2d_ctx* c) {
   RC(ensure_frozen(c));
   RC(sync_all(c));
   c->host_step = 0;
+  c->midA_ready = false;
   RC(enqueue_shapes_reset(c));
   RC(enqueue_obst_reset(c));
   RC(enqueue_ic(c));

@@ -96,17 +96,17 @@ __device__ inline void lane_tree(double* red) {
 
 }  // namespace
 
-// One workgroup per swimmer, after the bodies have moved:
-// Fish::updatePosition's theta_internal update (Fish.cpp:220-225),
-// AmplitudeFish::computeMidline (CarlingFish.cpp:332-369),
-// integrateLinearMomentum, changeToCoMFrameLinear, integrateAngularMomentum,
-// changeToCoMFrameAngular (FishLibrary.cpp:86-176) and the AreaSegment boxes
-// in the computational frame (Fish.cpp:118-150, FishLibrary.cpp:295-342).
-// The midline arrays live in LDS (fish_midline_lds bytes) until the end.
-__global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__ fd,
-                                                     const cup2d_shape_state* __restrict__ ss,
-                                                     const StepState* __restrict__ st,
-                                                     double h) {
+// Fish::create steps 1-3 (Fish.cpp:78-95), first part, one workgroup per
+// swimmer: AmplitudeFish::computeMidline (CarlingFish.cpp:332-369),
+// integrateLinearMomentum, changeToCoMFrameLinear, integrateAngularMomentum
+// and the angular-velocity correction of changeToCoMFrameAngular
+// (FishLibrary.cpp:86-176), i.e. everything that depends only on the time.
+// next = 0: at the current time; next = 1: at the end of the current step
+// (time + dt, the value the step end forms), so the next step finds it ready.
+// Arrays live in LDS (fish_midline_lds bytes); results go to midA / stA.
+__global__ __launch_bounds__(NT) void k_fish_mid_a(const FishDev* __restrict__ fd,
+                                                   const StepState* __restrict__ st, int next,
+                                                   int parity) {
   extern __shared__ double lds[];
   const FishDev F = fd[blockIdx.x];
   const int Nm = F.Nm, t = threadIdx.x;
@@ -119,8 +119,7 @@ __global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__
     rS[i] = F.rS[i];
     width[i] = F.width[i];
   }
-  const double time = st->time, dt = st->dt;
-  const double theta = F.st[6] - dt * F.st[7];
+  const double time = next ? st->tnext[parity] : st->time;
   const double T = F.Tperiod;
   const double rampFac = time < T ? det_sin(0.5 * M_PI * time / T) : 1.0;
   const double rampFacVel = time < T ? 0.5 * M_PI / T * det_cos(0.5 * M_PI * time / T) : 0.0;
@@ -226,14 +225,57 @@ __global__ __launch_bounds__(NT) void k_fish_midline(const FishDev* __restrict__
   const double J = red[NT];
   const double angvel = red[0] / J;
   __syncthreads();
-  // changeToCoMFrameAngular(theta_internal, angvel_internal)
+  // changeToCoMFrameAngular, first half: v += angvel x r (before the rotation)
+  for (int i = t; i < Nm; i += NT) {
+    F.midA[i] = M.rX[i];
+    F.midA[Nm + i] = M.rY[i];
+    F.midA[2 * Nm + i] = M.vX[i] + angvel * M.rY[i];
+    F.midA[3 * Nm + i] = M.vY[i] - angvel * M.rX[i];
+  }
+  if (t == 0) {
+    F.stA[0] = area; F.stA[1] = J;
+    F.stA[2] = C0; F.stA[3] = C1;
+    F.stA[4] = V0; F.stA[5] = V1;
+    F.stA[6] = angvel;
+  }
+}
+
+// Fish::updatePosition's theta_internal update (Fish.cpp:220-225) and
+// Fish::create steps 3-5, second part, one workgroup per swimmer, after the
+// bodies have moved: rotation by theta_internal (changeToCoMFrameAngular),
+// _computeMidlineNormals (FishLibrary.cpp:66-84), and the AreaSegment boxes
+// in the computational frame (Fish.cpp:118-150, FishLibrary.cpp:295-342).
+__global__ __launch_bounds__(NT) void k_fish_mid_b(const FishDev* __restrict__ fd,
+                                                   const cup2d_shape_state* __restrict__ ss,
+                                                   const StepState* __restrict__ st, double h) {
+  extern __shared__ double lds[];
+  const FishDev F = fd[blockIdx.x];
+  const int Nm = F.Nm, t = threadIdx.x;
+  double* rS = lds;
+  double* width = lds + Nm;
+  const Mid M{lds + 2 * Nm, lds + 3 * Nm, lds + 4 * Nm, lds + 5 * Nm,
+              lds + 6 * Nm, lds + 7 * Nm, lds + 8 * Nm, lds + 9 * Nm};
+  double* red = lds + 10 * Nm;
+  for (int i = t; i < Nm; i += NT) {
+    rS[i] = F.rS[i];
+    width[i] = F.width[i];
+    M.rX[i] = F.midA[i];
+    M.rY[i] = F.midA[Nm + i];
+    M.vX[i] = F.midA[2 * Nm + i];
+    M.vY[i] = F.midA[3 * Nm + i];
+  }
+  const double theta = F.st[6] - st->dt * F.st[7];
+  const double angvel = F.stA[6];
+  const double area = F.stA[0], J = F.stA[1];
+  const double C0 = F.stA[2], C1 = F.stA[3], V0 = F.stA[4], V1 = F.stA[5];
+  __syncthreads();
+  // changeToCoMFrameAngular(theta_internal, angvel_internal), second half
   {
     double sn, cs;
     det_sincos(theta, &sn, &cs);
     const double R00 = cs, R01 = -sn, R10 = sn, R11 = cs;
     for (int i = t; i < Nm; i += NT) {
-      double vX = M.vX[i] + angvel * M.rY[i];
-      double vY = M.vY[i] - angvel * M.rX[i];
+      const double vX = M.vX[i], vY = M.vY[i];
       const double px = M.rX[i], py = M.rY[i];
       M.rX[i] = R00 * px + R01 * py;
       M.rY[i] = R10 * px + R11 * py;
@@ -763,7 +805,7 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape) {
   F.env_a0 = d->env_a0;
   F.env_a2 = d->env_a2;
   F.waveLength = d->waveLength;
-  const size_t nd = (size_t)Nm * 10 + (size_t)F.Nseg * FISH_SEGD + 4 + 16;
+  const size_t nd = (size_t)Nm * 14 + (size_t)F.Nseg * FISH_SEGD + 4 + 16 + 8;
   double* buf = nullptr;
   CUP2D_HIP_CHECK(c, hipMalloc(&buf, sizeof(double) * nd));
   CUP2D_HIP_CHECK(c, hipMemset(buf, 0, sizeof(double) * nd));
@@ -773,10 +815,15 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape) {
   F.seg = buf + 10 * (size_t)Nm;
   F.box = F.seg + (size_t)F.Nseg * FISH_SEGD;
   F.st = F.box + 4;
+  F.stA = F.st + 16;
+  F.midA = F.stA + 8;
   CUP2D_HIP_CHECK(c, hipMemcpy(F.rS, rS.data(), sizeof(double) * Nm, hipMemcpyHostToDevice));
   CUP2D_HIP_CHECK(c, hipMemcpy(F.width, width.data(), sizeof(double) * Nm, hipMemcpyHostToDevice));
   if (!c->fishd) CUP2D_HIP_CHECK(c, hipMalloc(&c->fishd, sizeof(FishDev) * MAX_FISH));
-  CUP2D_HIP_CHECK(c, hipFuncSetAttribute((const void*)k_fish_midline,
+  CUP2D_HIP_CHECK(c, hipFuncSetAttribute((const void*)k_fish_mid_a,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)fish_midline_lds(FISH_MAX_NM)));
+  CUP2D_HIP_CHECK(c, hipFuncSetAttribute((const void*)k_fish_mid_b,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)fish_midline_lds(FISH_MAX_NM)));
   const int idx = (int)c->hfish.size();
@@ -797,10 +844,27 @@ void fish_free_all(cup2d_ctx* c) {
 int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s) {
   if (c->hfish.empty()) return CUP2D_OK;
   // one launch per swimmer: the LDS size depends on its number of points
-  for (size_t f = 0; f < c->hfish.size(); ++f)
-    hipLaunchKernelGGL(k_fish_midline, dim3(1), dim3(NT), fish_midline_lds(c->hfish[f].Nm), s,
-                       c->fishd + f, c->ss, c->st, c->g.h);
+  for (size_t f = 0; f < c->hfish.size(); ++f) {
+    const size_t lds = fish_midline_lds(c->hfish[f].Nm);
+    if (!c->midA_ready)
+      hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), lds, s, c->fishd + f, c->st, 0, 0);
+    hipLaunchKernelGGL(k_fish_mid_b, dim3(1), dim3(NT), lds, s, c->fishd + f, c->ss, c->st,
+                       c->g.h);
+  }
+  c->midA_ready = false;
   return launch_check(c, "fish midline");
+}
+
+// parity: that of the current step, whose list builder stored time + dt in
+// st->tnext[parity] (the next step's list builder writes the other slot, so
+// this may run concurrently with it)
+int enqueue_fish_mid_next(cup2d_ctx* c, hipStream_t s, int parity) {
+  if (c->hfish.empty()) return CUP2D_OK;
+  for (size_t f = 0; f < c->hfish.size(); ++f)
+    hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), fish_midline_lds(c->hfish[f].Nm), s,
+                       c->fishd + f, c->st, 1, parity);
+  c->midA_ready = true;
+  return launch_check(c, "fish midline (next step)");
 }
 
 int enqueue_fish_place(cup2d_ctx* c, hipStream_t s) {

@@ -46,7 +46,7 @@ struct StepState {
   double fade_corr;  // cub2rhs correction factor of the last solve
   double flux_corr;  // advDiff boundary-flux correction of the last step
   double dt_done;    // dt of the step whose forces are pending (set by the body update)
-  double pad2_;
+  double tnext[2];   // time + dt of the step with that parity, set by its list builder
 };
 
 // Body state captured by the body update for ComputeForces, which may run on
@@ -75,6 +75,8 @@ struct FishDev {
   double* box;    // [4] union of the segments' lab boxes (x0 x1 y0 y1)
   double* st;     // [16] area J CoM_internal[2] vCoM_internal[2] theta_internal angvel_internal,
                   // velocity scale of the fixed-point P2M sums
+  double* midA;   // [4][Nm] rX rY vX vY of the time-only part of the midline (k_fish_mid_a)
+  double* stA;    // [8] area J CoM_internal[2] vCoM_internal[2] angvel_internal of midA
 };
 
 // Obstacle pool (ObstacleBlock.h:266-378, one slot per (shape, block)).
@@ -137,6 +139,9 @@ struct cup2d_ctx {
   bool forces_pending = false;
   bool end_pending = false;
   bool frc_join = false;  // the main stream must wait for ev_frc
+  // the swimmers' time-only midline part for the next whole step is computed
+  // (valid for the next advance only; any other call clears it)
+  bool midA_ready = false;
   int host_step = 0;                       // step index for the CFL ramp (Simulation.cpp:264)
   cup2d::ForceSnap* fsnap = nullptr;       // device [MAX_SHAPES]
   int device = 0;
@@ -266,6 +271,8 @@ int fish_discretize(const cup2d_shape_desc* d, double h, std::vector<double>& rS
 int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape);
 void fish_free_all(cup2d_ctx* c);
 int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s);
+// the time-only part of the next whole step's midline (time + dt)
+int enqueue_fish_mid_next(cup2d_ctx* c, hipStream_t s, int parity);
 int enqueue_fish_place(cup2d_ctx* c, hipStream_t s);
 // PressureIterator_unif pieces (PressureIterator_unif.cpp)
 int enqueue_copy_vel_to_tmpv(cup2d_ctx* c);

@@ -181,6 +181,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       s_step[1] = U0;
       s_step[2] = U1;
     }
+    if (t == 0) st->tnext[step_idx & 1] = st->time + s_step[0];  // this step's end time
     __syncthreads();
     if (t < ns) {  // each shape moves on its own lane
       const double dt = s_step[0], U0 = s_step[1], U1 = s_step[2];
