@@ -21,6 +21,27 @@ const char* const TIMER_NAMES[T_NTIMERS] = {
 
 void timer_mark(cup2d_ctx* c, int id, bool start) { timer_mark_on(c, id, start, nullptr); }
 
+void kernel_timer_begin(cup2d_ctx* c, int id) {
+  c->lt_start = c->lt_stop = nullptr;
+  Timer& t = c->timers[id];
+  if (!t.on || t.seen++ % t.every != 0) return;
+  const size_t need = 2 * (size_t)(t.used + 1);
+  while (t.ev.size() < need) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) { t.on = false; return; }
+    t.ev.push_back(e);
+  }
+  c->lt_start = t.ev[2 * (size_t)t.used];
+  c->lt_stop = t.ev[2 * (size_t)t.used + 1];
+  ++t.used;
+}
+
+void kernel_timer_end(cup2d_ctx* c, int id) {
+  // a scope left before its launch (an error return) recorded nothing
+  if (c->lt_start) --c->timers[id].used;
+  c->lt_start = c->lt_stop = nullptr;
+}
+
 void timer_mark_on(cup2d_ctx* c, int id, bool start, hipStream_t s) {
   Timer& t = c->timers[id];
   if (!t.on) return;
@@ -195,6 +216,14 @@ int pressure_iterator(cup2d_ctx* c, int* iters_out, double* reldf_out) {
   return CUP2D_OK;
 }
 
+// Flags of the events that order the main and side streams.  They only order
+// device work between two streams of one device and the host never waits on
+// them, so recording them needs no system-scope fence (which would write the
+// L2 back on every record).
+#ifndef CUP2D_SYNC_EVENT_FLAGS
+#define CUP2D_SYNC_EVENT_FLAGS (hipEventDisableTiming | hipEventDisableSystemFence)
+#endif
+
 // Run the obstacle placement on the side stream, overlapping advDiff (1), or
 // in line before it (0).
 #ifndef CUP2D_OBST_OVERLAP
@@ -272,10 +301,10 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
     c->own_stream = true;
   }
   if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_begin, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_obst, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_pen, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_frc, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_begin, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_obst, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_pen, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_frc, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipMalloc(&c->fsnap, sizeof(ForceSnap) * MAX_SHAPES) != hipSuccess) {
     c->err = "side stream creation failed";
     return bail(CUP2D_ERR_HIP);

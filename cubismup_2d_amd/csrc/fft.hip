@@ -683,14 +683,14 @@ static void launch_rows_fwd(cup2d_ctx* c, const double* in, int pitch, int ncols
   const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(double2);
   const dim3 grid((nrows / 2 + NP - 1) / NP), block(NP * L / FE);
   if (src == SRC_DIV)
-    hipLaunchKernelGGL((k_rows_fwd<L, NP, FE, SRC_DIV>), grid, block, shm, c->stream, in, pitch,
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_DIV>), grid, block, shm, c->stream, in, pitch,
                        ncols, nrows, spec, spec_rows, tw, fd);
   else if (src == SRC_FADE)
-    hipLaunchKernelGGL((k_rows_fwd<L, NP, FE, SRC_FADE>), grid, block, shm, c->stream, in, pitch,
-                       ncols, nrows, spec, spec_rows, tw, fd);
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_FADE>), grid, block, shm, c->stream, in,
+                       pitch, ncols, nrows, spec, spec_rows, tw, fd);
   else
-    hipLaunchKernelGGL((k_rows_fwd<L, NP, FE, SRC_PLAIN>), grid, block, shm, c->stream, in, pitch,
-                       ncols, nrows, spec, spec_rows, tw, fd);
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_PLAIN>), grid, block, shm, c->stream, in,
+                       pitch, ncols, nrows, spec, spec_rows, tw, fd);
 }
 template <int L>
 static void launch_rows_inv(cup2d_ctx* c, const double2* spec, int spec_rows, double* out,
@@ -698,14 +698,14 @@ static void launch_rows_inv(cup2d_ctx* c, const double2* spec, int spec_rows, do
   constexpr int NP = rows_np<L, FE>();
   const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(double2);
   const int pairs = nrows / 2;
-  hipLaunchKernelGGL((k_rows_inv<L, NP, FE>), dim3((pairs + NP - 1) / NP), dim3(NP * L / FE),
+  CUP2D_TIMED_LAUNCH(c, (k_rows_inv<L, NP, FE>), dim3((pairs + NP - 1) / NP), dim3(NP * L / FE),
                      shm, c->stream, spec, spec_rows, out, pitch, ncols, nrows, tw);
 }
 template <int LC, int MODE>
 static void launch_cols(cup2d_ctx* c, double2* spec, int ncg, const ColParams& P,
                         const double2* tw) {
   const size_t shm = (size_t)lds_len<LC, FE>() * CW * sizeof(double2);
-  hipLaunchKernelGGL((k_cols<LC, MODE, FE>), dim3(ncg * (G / CW)), dim3(CW * LC / FE), shm,
+  CUP2D_TIMED_LAUNCH(c, (k_cols<LC, MODE, FE>), dim3(ncg * (G / CW)), dim3(CW * LC / FE), shm,
                      c->stream, spec, P, tw);
 }
 
@@ -884,7 +884,7 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   const int NX = c->g.NX, NY = c->g.NY;
   int rc;
   {
-    TimerScope ts(c, T_FFT_ROWS_FWD);
+    KernelTimer kt(c, T_FFT_ROWS_FWD);
     RowSource fd{c->g, c->red, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
                  c->u, c->v, c->obmap, c->fluxpart, stencil_edge_count(c), P->cs};
     rc = rows_fwd(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r, src, fd);
@@ -893,7 +893,7 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   ColParams cp{};
   cp.NX = NX; cp.NY = NY; cp.rows = NY;
   {
-    TimerScope ts(c, T_FFT_COLS);
+    KernelTimer kt(c, T_FFT_COLS);
     if (P->cs) {  // FFTW_dirichlet.h:47-48
       cp.waveFactX = M_PI / NX;
       cp.waveFactY = M_PI / NY;
@@ -918,7 +918,7 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
     // must finish before it is overwritten
     rc = join_forces(c);
     if (rc) return rc;
-    TimerScope ts(c, T_FFT_ROWS_INV);
+    KernelTimer kt(c, T_FFT_ROWS_INV);
     rc = rows_inv(c, P->Lr, P->spec, P->NYs, sol, NX, NX, NY, P->tw_r);
   }
   if (rc) return rc;

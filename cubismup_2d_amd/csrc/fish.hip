@@ -103,22 +103,21 @@ __device__ inline void lane_tree(double* red) {
 // (FishLibrary.cpp:86-176), i.e. everything that depends only on the time.
 // next = 0: at the current time; next = 1: at the end of the current step
 // (time + dt, the value the step end forms), so the next step finds it ready.
-// Arrays live in LDS (fish_midline_lds bytes); results go to midA / stA.
+// The midline arrays live in the swimmer's global work area (wk, L2-resident)
+// and only the reduction scratch in LDS: this kernel runs beside the FFT row
+// pass, whose workgroups need 2 x 68 KB of a CU's 160 KB LDS, and a larger
+// LDS footprint here would take both of that pass's slots on one CU.
+// Results go to midA / stA.
 __global__ __launch_bounds__(NT) void k_fish_mid_a(const FishDev* __restrict__ fd,
                                                    const StepState* __restrict__ st, int next,
                                                    int parity) {
-  extern __shared__ double lds[];
+  __shared__ double red[5 * NT];
   const FishDev F = fd[blockIdx.x];
   const int Nm = F.Nm, t = threadIdx.x;
-  double* rS = lds;
-  double* width = lds + Nm;
-  const Mid M{lds + 2 * Nm, lds + 3 * Nm, lds + 4 * Nm, lds + 5 * Nm,
-              lds + 6 * Nm, lds + 7 * Nm, lds + 8 * Nm, lds + 9 * Nm};
-  double* red = lds + 10 * Nm;
-  for (int i = t; i < Nm; i += NT) {
-    rS[i] = F.rS[i];
-    width[i] = F.width[i];
-  }
+  const double* rS = F.rS;
+  const double* width = F.width;
+  double* w = F.wk;
+  const Mid M{w, w + Nm, w + 2 * Nm, w + 3 * Nm, w + 4 * Nm, w + 5 * Nm, w + 6 * Nm, w + 7 * Nm};
   const double time = next ? st->tnext[parity] : st->time;
   const double T = F.Tperiod;
   const double rampFac = time < T ? det_sin(0.5 * M_PI * time / T) : 1.0;
@@ -805,7 +804,7 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape) {
   F.env_a0 = d->env_a0;
   F.env_a2 = d->env_a2;
   F.waveLength = d->waveLength;
-  const size_t nd = (size_t)Nm * 14 + (size_t)F.Nseg * FISH_SEGD + 4 + 16 + 8;
+  const size_t nd = (size_t)Nm * 22 + (size_t)F.Nseg * FISH_SEGD + 4 + 16 + 8;
   double* buf = nullptr;
   CUP2D_HIP_CHECK(c, hipMalloc(&buf, sizeof(double) * nd));
   CUP2D_HIP_CHECK(c, hipMemset(buf, 0, sizeof(double) * nd));
@@ -817,12 +816,10 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape) {
   F.st = F.box + 4;
   F.stA = F.st + 16;
   F.midA = F.stA + 8;
+  F.wk = F.midA + 4 * (size_t)Nm;
   CUP2D_HIP_CHECK(c, hipMemcpy(F.rS, rS.data(), sizeof(double) * Nm, hipMemcpyHostToDevice));
   CUP2D_HIP_CHECK(c, hipMemcpy(F.width, width.data(), sizeof(double) * Nm, hipMemcpyHostToDevice));
   if (!c->fishd) CUP2D_HIP_CHECK(c, hipMalloc(&c->fishd, sizeof(FishDev) * MAX_FISH));
-  CUP2D_HIP_CHECK(c, hipFuncSetAttribute((const void*)k_fish_mid_a,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)fish_midline_lds(FISH_MAX_NM)));
   CUP2D_HIP_CHECK(c, hipFuncSetAttribute((const void*)k_fish_mid_b,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)fish_midline_lds(FISH_MAX_NM)));
@@ -847,7 +844,7 @@ int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s) {
   for (size_t f = 0; f < c->hfish.size(); ++f) {
     const size_t lds = fish_midline_lds(c->hfish[f].Nm);
     if (!c->midA_ready)
-      hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), lds, s, c->fishd + f, c->st, 0, 0);
+      hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), 0, s, c->fishd + f, c->st, 0, 0);
     hipLaunchKernelGGL(k_fish_mid_b, dim3(1), dim3(NT), lds, s, c->fishd + f, c->ss, c->st,
                        c->g.h);
   }
@@ -861,8 +858,7 @@ int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s) {
 int enqueue_fish_mid_next(cup2d_ctx* c, hipStream_t s, int parity) {
   if (c->hfish.empty()) return CUP2D_OK;
   for (size_t f = 0; f < c->hfish.size(); ++f)
-    hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), fish_midline_lds(c->hfish[f].Nm), s,
-                       c->fishd + f, c->st, 1, parity);
+    hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), 0, s, c->fishd + f, c->st, 1, parity);
   c->midA_ready = true;
   return launch_check(c, "fish midline (next step)");
 }

@@ -8,6 +8,7 @@
 // zero-gradient rule (Definitions.h:125-157).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -77,6 +78,7 @@ struct FishDev {
                   // velocity scale of the fixed-point P2M sums
   double* midA;   // [4][Nm] rX rY vX vY of the time-only part of the midline (k_fish_mid_a)
   double* stA;    // [8] area J CoM_internal[2] vCoM_internal[2] angvel_internal of midA
+  double* wk;     // [8][Nm] working arrays of k_fish_mid_a
 };
 
 // Obstacle pool (ObstacleBlock.h:266-378, one slot per (shape, block)).
@@ -191,6 +193,8 @@ struct cup2d_ctx {
   std::string fft_err;                     // why no FFT plan exists (size limits)
   bool shapes_frozen = false;
   cup2d::Timer timers[cup2d::T_NTIMERS];
+  // events of the next timed launch (KernelTimer), recorded by its dispatch
+  hipEvent_t lt_start = nullptr, lt_stop = nullptr;
 };
 
 #define CUP2D_HIP_CHECK(ctx, call)                                             \
@@ -218,6 +222,23 @@ struct TimerScope {
   }
   ~TimerScope() { timer_mark_on(c, id, false, s); }
 };
+// Timer of a single kernel launched with CUP2D_TIMED_LAUNCH inside the scope:
+// the dispatch itself records the start and stop events (no marker packets
+// around it on the stream).
+void kernel_timer_begin(cup2d_ctx* c, int id);
+void kernel_timer_end(cup2d_ctx* c, int id);
+struct KernelTimer {
+  cup2d_ctx* c;
+  int id;
+  KernelTimer(cup2d_ctx* c_, int id_) : c(c_), id(id_) { kernel_timer_begin(c, id); }
+  ~KernelTimer() { kernel_timer_end(c, id); }
+};
+#define CUP2D_TIMED_LAUNCH(ctx, kernel, grid, block, shm, stream, ...)                     \
+  do {                                                                                    \
+    hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, (ctx)->lt_start, (ctx)->lt_stop, \
+                          0, __VA_ARGS__);                                                \
+    (ctx)->lt_start = (ctx)->lt_stop = nullptr;                                           \
+  } while (0)
 
 // stencils.hip
 int grid_workgroups(int NX, int NY);

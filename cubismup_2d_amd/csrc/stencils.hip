@@ -153,15 +153,12 @@ __device__ inline void stencil_tile(int* bx, int* by) {
 // workgroup also writes its share of the boundary flux
 // IF = sum_E u - sum_W u + sum_N v - sum_S v (advDiff.cpp:93-102).
 template <int RY>
-__global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
-                                                 const double* __restrict__ v,
-                                                 double* __restrict__ uo, double* __restrict__ vo,
-                                                 Grid g, const StepState* __restrict__ st,
-                                                 double* __restrict__ fluxpart) {
-  __shared__ double lds[8];
+__device__ inline void advdiff_strip(const double* __restrict__ u, const double* __restrict__ v,
+                                     double* __restrict__ uo, double* __restrict__ vo,
+                                     const Grid& g, const StepState* __restrict__ st,
+                                     double* __restrict__ fluxpart, int bx, int by, int gx, int gy,
+                                     double* lds) {
   const int NX = g.NX, NY = g.NY;
-  int bx, by;
-  stencil_tile(&bx, &by);
   const int x = bx * blockDim.x + threadIdx.x;
   const int y0 = by * RY;
   const double dt = st->dt, h = g.h;
@@ -205,10 +202,28 @@ __global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
     us = uc; vs = vc;
     uc = un; vc = vn;
   }
-  const int e = edge_index(bx, by, gridDim.x, gridDim.y);
+  const int e = edge_index(bx, by, gx, gy);
   if (e < 0) return;
   flux = block_sum_dyn(flux, lds);
   if (threadIdx.x == 0) fluxpart[e] = flux;
+}
+
+// Strips of RY rows x blockDim.x columns.  nstrip_x x nstrip_y strips; with a
+// grid smaller than that each workgroup takes the strips id, id + gridDim.x,
+// ... (the grid is a multiple of 8, so a workgroup keeps its XCD), which
+// leaves workgroup slots free for the obstacle kernels of the side stream.
+template <int RY>
+__global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
+                                                 const double* __restrict__ v,
+                                                 double* __restrict__ uo, double* __restrict__ vo,
+                                                 Grid g, const StepState* __restrict__ st,
+                                                 double* __restrict__ fluxpart, int gx, int gy) {
+  __shared__ double lds[8];
+  const int n = gx * gy;
+  for (int id = blockIdx.x; id < n; id += gridDim.x) {
+    const int t = (CUP2D_XCD_REMAP && n % 8 == 0) ? (id % 8) * (n / 8) + id / 8 : id;
+    advdiff_strip<RY>(u, v, uo, vo, g, st, fluxpart, t % gx, t / gx, gx, gy, lds);
+  }
 }
 
 // one workgroup: corr from the boundary workgroups' partials
@@ -235,14 +250,21 @@ __global__ __launch_bounds__(256) void k_flux_fix(double* __restrict__ u,
 }
 
 static constexpr int RYS = CUP2D_STENCIL_RY;  // rows per workgroup of the stencil kernels
+// at most this many advDiff workgroups (a multiple of 8; 0: one per strip)
+#ifndef CUP2D_ADVDIFF_WG_CAP
+#define CUP2D_ADVDIFF_WG_CAP 0
+#endif
 
 int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
   const int tx = strip_width(c->g.NX);
   const int gx = c->g.NX / tx, gy = c->g.NY / RYS;
   {
-    TimerScope ts(c, T_ADVDIFF);
-    hipLaunchKernelGGL(k_advdiff<RYS>, dim3(gx, gy), dim3(tx), 0, c->stream, c->u, c->v, c->tu,
-                       c->tv, c->g, c->st, c->fluxpart);
+    int nwg = gx * gy;
+    if (CUP2D_ADVDIFF_WG_CAP > 0 && nwg > CUP2D_ADVDIFF_WG_CAP && nwg % 8 == 0)
+      nwg = CUP2D_ADVDIFF_WG_CAP;
+    KernelTimer kt(c, T_ADVDIFF);
+    CUP2D_TIMED_LAUNCH(c, k_advdiff<RYS>, dim3(nwg), dim3(tx), 0, c->stream, c->u, c->v, c->tu,
+                       c->tv, c->g, (const StepState*)c->st, c->fluxpart, gx, gy);
   }
   std::swap(c->u, c->tu);  // advDiff.cpp:81-87 copy tmpV -> vel
   std::swap(c->v, c->tv);
@@ -461,10 +483,11 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
 
 int enqueue_pressure_correction(cup2d_ctx* c, int want_max) {
   const int tx = strip_width(c->g.NX);
-  TimerScope ts(c, T_PROJECT);
-  hipLaunchKernelGGL(k_project<RYS>, dim3(c->g.NX / tx, c->g.NY / RYS), dim3(tx), 0, c->stream,
-                     c->u, c->v, c->pres, c->g, c->st, c->flux_deferred ? 1 : 0,
-                     c->obmap, want_max ? c->maxpart : nullptr);
+  KernelTimer kt(c, T_PROJECT);
+  CUP2D_TIMED_LAUNCH(c, k_project<RYS>, dim3(c->g.NX / tx, c->g.NY / RYS), dim3(tx), 0, c->stream,
+                     c->u, c->v, (const double*)c->pres, c->g, (const StepState*)c->st,
+                     c->flux_deferred ? 1 : 0, (const unsigned char*)c->obmap,
+                     want_max ? c->maxpart : nullptr);
   c->flux_deferred = false;
   return launch_check(c, "pressure_correction");
 }
