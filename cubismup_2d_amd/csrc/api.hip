@@ -136,6 +136,7 @@ int sync_all(cup2d_ctx* c) {
   RC(flush_forces(c));
   RC(join_obstacles(c));
   RC(join_forces(c));
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->mids));
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->aux));
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return CUP2D_OK;
@@ -236,7 +237,7 @@ int advance_enqueued(cup2d_ctx* c, int compute_dt) {
   RC(enqueue_put_objects(c, compute_dt, CUP2D_OBST_OVERLAP));
   // the swimmers' time-only midline part of the next step, behind this step's
   // obstacle chain on the side stream (off the critical path)
-  if (CUP2D_OBST_OVERLAP) RC(enqueue_fish_mid_next(c, c->aux, c->host_step & 1));
+  if (CUP2D_OBST_OVERLAP) RC(enqueue_fish_mid_next(c, c->host_step & 1));
   if (c->cfg.iterative_penal) {
     RC(enqueue_advdiff(c, 0));
     RC(pressure_iterator(c, nullptr, nullptr));
@@ -301,6 +302,8 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
     c->own_stream = true;
   }
   if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->mids, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_mid, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_begin, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_obst, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_pen, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
@@ -374,6 +377,8 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
 int cup2d_destroy(cup2d_ctx* c) {
   if (!c) return CUP2D_ERR_ARG;
   (void)hipSetDevice(c->device);
+  if (c->mids) (void)hipStreamSynchronize(c->mids);
+  if (c->aux) (void)hipStreamSynchronize(c->aux);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   fft_destroy(c);
   obstacles_free(c);
@@ -392,6 +397,8 @@ int cup2d_destroy(cup2d_ctx* c) {
   for (Timer& t : c->timers)
     for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
   if (c->aux) { (void)hipStreamSynchronize(c->aux); (void)hipStreamDestroy(c->aux); }
+  if (c->mids) (void)hipStreamDestroy(c->mids);
+  if (c->ev_mid) (void)hipEventDestroy(c->ev_mid);
   if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
   if (c->ev_obst) (void)hipEventDestroy(c->ev_obst);
   if (c->ev_pen) (void)hipEventDestroy(c->ev_pen);

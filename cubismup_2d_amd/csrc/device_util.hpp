@@ -7,6 +7,15 @@
 
 namespace cup2d {
 
+// Issue priority of the waves of the obstacle-placement kernels, which share
+// the CUs with advDiff when they run on the side stream (0: default).
+#ifndef CUP2D_CHAIN_PRIO
+#define CUP2D_CHAIN_PRIO 0
+#endif
+__device__ inline void chain_prio() {
+  if (CUP2D_CHAIN_PRIO > 0) __builtin_amdgcn_s_setprio(CUP2D_CHAIN_PRIO);
+}
+
 // std::max / std::min semantics ((a<b)?b:a), so NaN handling and signed
 // zeros follow the reference's calls exactly.
 __host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }

@@ -346,13 +346,16 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
     const int yc = valid ? y0 : 0;
     const int ym = yc > 0 ? yc - 1 : yc, yp = yc + 2 < NY ? yc + 2 : yc + 1;
     double* ud = (double*)lds + (size_t)f * 2 * PADL;  // u of rows y0, y0+1
-    double vm[E], v0[E], v1[E], vp[E];
+    // every load of the six rows is issued before the first use, so a wave
+    // has all of them in flight (storing u into LDS inside this loop would
+    // wait for each group in turn)
+    double u0[E], u1[E], vm[E], v0[E], v1[E], vp[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int i = t + q * NT;
       if (i < NX) {
-        ud[i] = fd.u[(size_t)yc * NX + i];
-        ud[L + i] = fd.u[(size_t)(yc + 1) * NX + i];
+        u0[q] = fd.u[(size_t)yc * NX + i];
+        u1[q] = fd.u[(size_t)(yc + 1) * NX + i];
         vm[q] = fd.v[(size_t)ym * NX + i];
         v0[q] = fd.v[(size_t)yc * NX + i];
         v1[q] = fd.v[(size_t)(yc + 1) * NX + i];
@@ -362,6 +365,14 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
     const double fcorr = flux_corr256(fd.fluxpart, fd.ne, g, s_red);
     if (blockIdx.x == 0 && threadIdx.x == 0) fd.st->flux_corr = fcorr;
     const double facDiv = 0.5 * g.h / fd.st->dt;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int i = t + q * NT;
+      if (i < NX) {
+        ud[i] = u0[q];
+        ud[L + i] = u1[q];
+      }
+    }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -475,16 +486,24 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_inv(const d
   tw_load<L, E, E>(tw, t, w);
   double2 x[E];
   // Z_k = A_k + i B_k and Z_{L-k} = conj(A_k) + i conj(B_k); DC and Nyquist
-  // of both rows are packed in slot 0 (real parts only)
+  // of both rows are packed in slot 0 (real parts only).  All loads are
+  // issued before the first LDS store, so they are in flight together.
+  double2 As[E / 2], Bs[E / 2];
 #pragma unroll
   for (int q = 0; q < E / 2; ++q) {
     const int k = t + q * NT;
-    double2 A = make_double2(0, 0), B = make_double2(0, 0);
+    As[q] = make_double2(0, 0);
+    Bs[q] = make_double2(0, 0);
     if (valid) {
       const double2* s = spec + ((size_t)(k / G) * spec_rows + y0) * G + (k % G);
-      A = s[0];
-      B = s[G];
+      As[q] = s[0];
+      Bs[q] = s[G];
     }
+  }
+#pragma unroll
+  for (int q = 0; q < E / 2; ++q) {
+    const int k = t + q * NT;
+    const double2 A = As[q], B = Bs[q];
     if (k == 0) {
       x[0] = make_double2(A.x, B.x);
       x[E / 2] = make_double2(A.y, B.y);

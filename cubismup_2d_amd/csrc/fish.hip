@@ -110,7 +110,7 @@ __device__ inline void lane_tree(double* red) {
 // Results go to midA / stA.
 __global__ __launch_bounds__(NT) void k_fish_mid_a(const FishDev* __restrict__ fd,
                                                    const StepState* __restrict__ st, int next,
-                                                   int parity) {
+                                                   int parity, int buf) {
   __shared__ double red[5 * NT];
   const FishDev F = fd[blockIdx.x];
   const int Nm = F.Nm, t = threadIdx.x;
@@ -226,16 +226,18 @@ __global__ __launch_bounds__(NT) void k_fish_mid_a(const FishDev* __restrict__ f
   __syncthreads();
   // changeToCoMFrameAngular, first half: v += angvel x r (before the rotation)
   for (int i = t; i < Nm; i += NT) {
-    F.midA[i] = M.rX[i];
-    F.midA[Nm + i] = M.rY[i];
-    F.midA[2 * Nm + i] = M.vX[i] + angvel * M.rY[i];
-    F.midA[3 * Nm + i] = M.vY[i] - angvel * M.rX[i];
+    double* mA = F.midA + (size_t)buf * 4 * Nm;
+    mA[i] = M.rX[i];
+    mA[Nm + i] = M.rY[i];
+    mA[2 * Nm + i] = M.vX[i] + angvel * M.rY[i];
+    mA[3 * Nm + i] = M.vY[i] - angvel * M.rX[i];
   }
   if (t == 0) {
-    F.stA[0] = area; F.stA[1] = J;
-    F.stA[2] = C0; F.stA[3] = C1;
-    F.stA[4] = V0; F.stA[5] = V1;
-    F.stA[6] = angvel;
+    double* sA = F.stA + buf * 8;
+    sA[0] = area; sA[1] = J;
+    sA[2] = C0; sA[3] = C1;
+    sA[4] = V0; sA[5] = V1;
+    sA[6] = angvel;
   }
 }
 
@@ -246,7 +248,9 @@ __global__ __launch_bounds__(NT) void k_fish_mid_a(const FishDev* __restrict__ f
 // in the computational frame (Fish.cpp:118-150, FishLibrary.cpp:295-342).
 __global__ __launch_bounds__(NT) void k_fish_mid_b(const FishDev* __restrict__ fd,
                                                    const cup2d_shape_state* __restrict__ ss,
-                                                   const StepState* __restrict__ st, double h) {
+                                                   const StepState* __restrict__ st, double h,
+                                                   int buf) {
+  chain_prio();
   extern __shared__ double lds[];
   const FishDev F = fd[blockIdx.x];
   const int Nm = F.Nm, t = threadIdx.x;
@@ -258,15 +262,17 @@ __global__ __launch_bounds__(NT) void k_fish_mid_b(const FishDev* __restrict__ f
   for (int i = t; i < Nm; i += NT) {
     rS[i] = F.rS[i];
     width[i] = F.width[i];
-    M.rX[i] = F.midA[i];
-    M.rY[i] = F.midA[Nm + i];
-    M.vX[i] = F.midA[2 * Nm + i];
-    M.vY[i] = F.midA[3 * Nm + i];
+    const double* mA = F.midA + (size_t)buf * 4 * Nm;
+    M.rX[i] = mA[i];
+    M.rY[i] = mA[Nm + i];
+    M.vX[i] = mA[2 * Nm + i];
+    M.vY[i] = mA[3 * Nm + i];
   }
   const double theta = F.st[6] - st->dt * F.st[7];
-  const double angvel = F.stA[6];
-  const double area = F.stA[0], J = F.stA[1];
-  const double C0 = F.stA[2], C1 = F.stA[3], V0 = F.stA[4], V1 = F.stA[5];
+  const double* sA = F.stA + buf * 8;
+  const double angvel = sA[6];
+  const double area = sA[0], J = sA[1];
+  const double C0 = sA[2], C1 = sA[3], V0 = sA[4], V1 = sA[5];
   __syncthreads();
   // changeToCoMFrameAngular(theta_internal, angvel_internal), second half
   {
@@ -478,6 +484,7 @@ __global__ __launch_bounds__(NT) void k_fish_place(Grid g, const ShapeParams* __
                                                    const cup2d_shape_state* __restrict__ ss,
                                                    const FishDev* __restrict__ fd,
                                                    Obstacles ob) {
+  chain_prio();
   __shared__ int sl[FISH_MAX_SEG];
   __shared__ int s_tmp[NT];
   __shared__ unsigned long long s_best[BS * BS];
@@ -804,7 +811,7 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape) {
   F.env_a0 = d->env_a0;
   F.env_a2 = d->env_a2;
   F.waveLength = d->waveLength;
-  const size_t nd = (size_t)Nm * 22 + (size_t)F.Nseg * FISH_SEGD + 4 + 16 + 8;
+  const size_t nd = (size_t)Nm * 26 + (size_t)F.Nseg * FISH_SEGD + 4 + 16 + 16;
   double* buf = nullptr;
   CUP2D_HIP_CHECK(c, hipMalloc(&buf, sizeof(double) * nd));
   CUP2D_HIP_CHECK(c, hipMemset(buf, 0, sizeof(double) * nd));
@@ -815,8 +822,8 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape) {
   F.box = F.seg + (size_t)F.Nseg * FISH_SEGD;
   F.st = F.box + 4;
   F.stA = F.st + 16;
-  F.midA = F.stA + 8;
-  F.wk = F.midA + 4 * (size_t)Nm;
+  F.midA = F.stA + 16;
+  F.wk = F.midA + 8 * (size_t)Nm;
   CUP2D_HIP_CHECK(c, hipMemcpy(F.rS, rS.data(), sizeof(double) * Nm, hipMemcpyHostToDevice));
   CUP2D_HIP_CHECK(c, hipMemcpy(F.width, width.data(), sizeof(double) * Nm, hipMemcpyHostToDevice));
   if (!c->fishd) CUP2D_HIP_CHECK(c, hipMalloc(&c->fishd, sizeof(FishDev) * MAX_FISH));
@@ -838,27 +845,36 @@ void fish_free_all(cup2d_ctx* c) {
   c->fishd = nullptr;
 }
 
+// The time-only part lives in buffer host_step & 1: computed here when it is
+// not ready, else by enqueue_fish_mid_next of the previous step.
 int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s) {
   if (c->hfish.empty()) return CUP2D_OK;
+  const int buf = c->host_step & 1;
   // one launch per swimmer: the LDS size depends on its number of points
   for (size_t f = 0; f < c->hfish.size(); ++f) {
     const size_t lds = fish_midline_lds(c->hfish[f].Nm);
     if (!c->midA_ready)
-      hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), 0, s, c->fishd + f, c->st, 0, 0);
+      hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), 0, s, c->fishd + f, c->st, 0, 0, buf);
     hipLaunchKernelGGL(k_fish_mid_b, dim3(1), dim3(NT), lds, s, c->fishd + f, c->ss, c->st,
-                       c->g.h);
+                       c->g.h, buf);
   }
   c->midA_ready = false;
   return launch_check(c, "fish midline");
 }
 
-// parity: that of the current step, whose list builder stored time + dt in
-// st->tnext[parity] (the next step's list builder writes the other slot, so
-// this may run concurrently with it)
-int enqueue_fish_mid_next(cup2d_ctx* c, hipStream_t s, int parity) {
+// The next step's time-only part, on the midline stream once this step's
+// list builder has stored time + dt in st->tnext[parity] (event ev_begin;
+// the next step's list builder writes the other slot), into the other
+// buffer than the one this step's k_fish_mid_b reads.  The side stream
+// waits for it behind this step's obstacle chain, where it is idle.
+int enqueue_fish_mid_next(cup2d_ctx* c, int parity) {
   if (c->hfish.empty()) return CUP2D_OK;
+  CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->mids, c->ev_begin, 0));
   for (size_t f = 0; f < c->hfish.size(); ++f)
-    hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), 0, s, c->fishd + f, c->st, 1, parity);
+    hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), 0, c->mids, c->fishd + f, c->st, 1,
+                       parity, parity ^ 1);
+  CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_mid, c->mids));
+  CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_mid, 0));
   c->midA_ready = true;
   return launch_check(c, "fish midline (next step)");
 }

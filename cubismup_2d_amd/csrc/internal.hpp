@@ -76,8 +76,9 @@ struct FishDev {
   double* box;    // [4] union of the segments' lab boxes (x0 x1 y0 y1)
   double* st;     // [16] area J CoM_internal[2] vCoM_internal[2] theta_internal angvel_internal,
                   // velocity scale of the fixed-point P2M sums
-  double* midA;   // [4][Nm] rX rY vX vY of the time-only part of the midline (k_fish_mid_a)
-  double* stA;    // [8] area J CoM_internal[2] vCoM_internal[2] angvel_internal of midA
+  double* midA;   // [2][4][Nm] rX rY vX vY of the time-only part of the midline (k_fish_mid_a),
+                  // one buffer per step parity
+  double* stA;    // [2][8] area J CoM_internal[2] vCoM_internal[2] angvel_internal of midA
   double* wk;     // [8][Nm] working arrays of k_fish_mid_a
 };
 
@@ -132,6 +133,11 @@ struct cup2d_ctx {
   // overlapping advDiff and the RHS (neither reads obstacle data)
   hipStream_t aux = nullptr;
   hipEvent_t ev_begin = nullptr, ev_obst = nullptr, ev_pen = nullptr, ev_frc = nullptr;
+  // third stream: the swimmers' time-only midline part of the next step runs
+  // there beside advDiff (a kernel with many workgroups, so one more resident
+  // workgroup costs it nothing); the side stream joins it (ev_mid) while idle
+  hipStream_t mids = nullptr;
+  hipEvent_t ev_mid = nullptr;
   bool obst_pending = false;
   // ComputeForces and the step end (time += dt) of the last whole step, not
   // enqueued yet.  The next whole step applies the step end in its list
@@ -293,7 +299,7 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape);
 void fish_free_all(cup2d_ctx* c);
 int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s);
 // the time-only part of the next whole step's midline (time + dt)
-int enqueue_fish_mid_next(cup2d_ctx* c, hipStream_t s, int parity);
+int enqueue_fish_mid_next(cup2d_ctx* c, int parity);
 int enqueue_fish_place(cup2d_ctx* c, hipStream_t s);
 // PressureIterator_unif pieces (PressureIterator_unif.cpp)
 int enqueue_copy_vel_to_tmpv(cup2d_ctx* c);

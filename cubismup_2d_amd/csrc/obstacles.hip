@@ -125,6 +125,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
                                                    const double* __restrict__ ramp,
                                                    int compute_dt, int step_idx, int phase,
                                                    int end_prev) {
+  chain_prio();
   __shared__ int s_a[LCAP], s_b[LCAP], s_c[LCAP], s_d[LCAP];
   __shared__ int s_tmp[NT];
   __shared__ int s_cstart[MAX_SHAPES + 1];
@@ -422,6 +423,7 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
                                                    double* __restrict__ udx,
                                                    double* __restrict__ udy,
                                                    double* __restrict__ invrho) {
+  chain_prio();
   __shared__ double lds[LDS_RED];
   __shared__ int s_nbs[4][MAX_FISH];  // swimmer slots of the E, W, N, S neighbour blocks
   __shared__ int s_nbn[4];
@@ -571,6 +573,7 @@ __device__ void obst_moments_wg(const Grid& g, const cup2d_shape_state* __restri
 __global__ __launch_bounds__(NT) void k_obst_moments(Grid g, const cup2d_shape_state* __restrict__ ss,
                                                      const StepState* __restrict__ st,
                                                      Obstacles ob) {
+  chain_prio();
   __shared__ double lds[LDS_RED];
   if ((int)blockIdx.x < ob.cnt[0]) obst_moments_wg(g, ss, ob, lds);
 }
@@ -611,6 +614,7 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __r
                                                   const double* __restrict__ chi,
                                                   double* __restrict__ udx,
                                                   double* __restrict__ udy) {
+  chain_prio();
   __shared__ double lds[LDS_RED];
   const int nub = ob.cnt[1];
   if ((int)blockIdx.x < nub) {
@@ -676,6 +680,7 @@ __device__ void obst_commit_wg(const Grid& g, cup2d_shape_state* __restrict__ ss
 
 __global__ __launch_bounds__(NT) void k_obst_commit(Grid g, cup2d_shape_state* __restrict__ ss,
                                                     Obstacles ob) {
+  chain_prio();
   obst_commit_wg(g, ss, ob);
 }
 

@@ -170,35 +170,57 @@ __device__ inline void advdiff_strip(const double* __restrict__ u, const double*
   const double fadeW = 1 - tW * tW / BC_KILL_FAC, fadeS = 1 - tS * tS / BC_KILL_FAC;
   const double fadeE = 1 - tE * tE / BC_KILL_FAC, fadeN = 1 - tN * tN / BC_KILL_FAC;
 
-  size_t idx = (size_t)y0 * NX + x;
-  double uc = u[idx], vc = v[idx];
-  double us, vs;
-  if (y0 == 0) { us = uc * fadeS; vs = vc * fadeS; }
-  else { us = u[idx - NX]; vs = v[idx - NX]; }
+  // All loads of the strip are issued first (clamped indices, so every load
+  // is unconditional), then the rows are computed with selects of the
+  // boundary values: the wave keeps the whole strip's loads in flight.
+  const size_t i0 = (size_t)y0 * NX + x;
+  const bool hasW = x > 0, hasE = x + 1 < NX;
+  const size_t dW = hasW ? 1 : 0, dE = hasE ? 1 : 0;
+  double uC[RY + 2], vC[RY + 2], uW[RY], vW[RY], uE[RY], vE[RY];
+  {
+    const size_t is = y0 > 0 ? i0 - NX : i0;
+    uC[0] = u[is];
+    vC[0] = v[is];
+#pragma unroll
+    for (int iy = 0; iy <= RY; ++iy) {
+      const size_t i = y0 + iy < NY ? i0 + (size_t)iy * NX : i0 + (size_t)(iy - 1) * NX;
+      uC[iy + 1] = u[i];
+      vC[iy + 1] = v[i];
+    }
+#pragma unroll
+    for (int iy = 0; iy < RY; ++iy) {
+      const size_t i = i0 + (size_t)iy * NX;
+      uW[iy] = u[i - dW];
+      vW[iy] = v[i - dW];
+      uE[iy] = u[i + dE];
+      vE[iy] = v[i + dE];
+    }
+  }
+  double uc = uC[1], vc = vC[1];
+  double us = y0 == 0 ? uc * fadeS : uC[0];
+  double vs = y0 == 0 ? vc * fadeS : vC[0];
   double flux = 0;
-  CUP2D_UNROLL_ROWS
-  for (int iy = 0; iy < RY; ++iy, idx += NX) {
+#pragma unroll
+  for (int iy = 0; iy < RY; ++iy) {
     const int y = y0 + iy;
-    double un, vn;
-    if (y + 1 < NY) { un = u[idx + NX]; vn = v[idx + NX]; }
-    else { un = uc * fadeN; vn = vc * fadeN; }
-    double uw, vw, ue, ve;
-    if (x > 0) { uw = u[idx - 1]; vw = v[idx - 1]; }
-    else { uw = uc * fadeW; vw = vc * fadeW; }
-    if (x + 1 < NX) { ue = u[idx + 1]; ve = v[idx + 1]; }
-    else { ue = uc * fadeE; ve = vc * fadeE; }
+    const bool hasN = y + 1 < NY;
+    const double un = hasN ? uC[iy + 2] : uc * fadeN, vn = hasN ? vC[iy + 2] : vc * fadeN;
+    const double uw = hasW ? uW[iy] : uc * fadeW, vw = hasW ? vW[iy] : vc * fadeW;
+    const double ue = hasE ? uE[iy] : uc * fadeE, ve = hasE ? vE[iy] : vc * fadeE;
     const double dUadv = (uc + UINF0) * (ue - uw) + (vc + UINF1) * (un - us);
     const double dUdif = ue + un + uw + us - 4 * uc;
     const double dVadv = (uc + UINF0) * (ve - vw) + (vc + UINF1) * (vn - vs);
     const double dVdif = ve + vn + vw + vs - 4 * vc;
     const double u1 = uc + (afac * dUadv + dfac * dUdif);
     const double v1 = vc + (afac * dVadv + dfac * dVdif);
+    const size_t idx = i0 + (size_t)iy * NX;
     uo[idx] = u1;
     vo[idx] = v1;
-    if (x == 0) flux -= u1;
-    if (x == NX - 1) flux += u1;
-    if (y == 0) flux -= v1;
-    if (y == NY - 1) flux += v1;
+    // selects, not branches, so the row loop stays one basic block
+    flux = x == 0 ? flux - u1 : flux;
+    flux = x == NX - 1 ? flux + u1 : flux;
+    flux = y == 0 ? flux - v1 : flux;
+    flux = y == NY - 1 ? flux + v1 : flux;
     us = uc; vs = vc;
     uc = un; vc = vn;
   }
@@ -445,33 +467,50 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
   const double pFac = -0.5 * st->dt / g.h;
   const double UINF = st->uinf[0], VINF = st->uinf[1];
   const double corr = fix ? st->flux_corr : 0.0;
-  size_t idx = (size_t)y0 * NX + x;
-  double pc = p[idx];
-  double ps = (y0 == 0) ? pc : p[idx - NX];
+  // All loads of the strip first (clamped indices), then the rows with
+  // selects of the boundary values; the RY rows lie in one block row.
+  static_assert(BS % RY == 0, "a strip lies in one block row");
+  const size_t i0 = (size_t)y0 * NX + x;
+  const bool hasW = x > 0, hasE = x + 1 < NX;
+  const size_t dW = hasW ? 1 : 0, dE = hasE ? 1 : 0;
+  double pC[RY + 2], pW[RY], pE[RY], uu[RY], vv[RY];
+  pC[0] = p[y0 > 0 ? i0 - NX : i0];
+#pragma unroll
+  for (int iy = 0; iy <= RY; ++iy)
+    pC[iy + 1] = p[y0 + iy < NY ? i0 + (size_t)iy * NX : i0 + (size_t)(iy - 1) * NX];
+#pragma unroll
+  for (int iy = 0; iy < RY; ++iy) {
+    const size_t i = i0 + (size_t)iy * NX;
+    pW[iy] = p[i - dW];
+    pE[iy] = p[i + dE];
+    uu[iy] = u[i];
+    vv[iy] = v[i];
+  }
+  const unsigned char inob = obmap[(y0 / BS) * g.bpdx + x / BS];  // always allocated
+  const bool count = (maxpart != nullptr) & (inob == 0);
+  double pc = pC[1];
+  double ps = y0 == 0 ? pc : pC[0];
   double m = 0;
-  CUP2D_UNROLL_ROWS
-  for (int iy = 0; iy < RY; ++iy, idx += NX) {
+#pragma unroll
+  for (int iy = 0; iy < RY; ++iy) {
     const int y = y0 + iy;
-    const double pn = (y + 1 < NY) ? p[idx + NX] : pc;
-    const double pw = (x > 0) ? p[idx - 1] : pc;
-    const double pe = (x + 1 < NX) ? p[idx + 1] : pc;
-    double uu = u[idx], vv = v[idx];
-    if (fix) {
-      if (x == 0) uu = uu + corr;
-      else if (x == NX - 1) uu = uu - corr;
-      if (y == 0) vv = vv + corr;
-      else if (y == NY - 1) vv = vv - corr;
-    }
-    const double u1 = uu + pFac * (pe - pw);
-    const double v1 = vv + pFac * (pn - ps);
+    const double pn = y + 1 < NY ? pC[iy + 2] : pc;
+    const double pw = hasW ? pW[iy] : pc;
+    const double pe = hasE ? pE[iy] : pc;
+    // selects, not branches, so the row loop stays one basic block
+    double ux = uu[iy], vy = vv[iy];
+    ux = fix && x == 0 ? ux + corr : (fix && x == NX - 1 ? ux - corr : ux);
+    vy = fix && y == 0 ? vy + corr : (fix && y == NY - 1 ? vy - corr : vy);
+    const double u1 = ux + pFac * (pe - pw);
+    const double v1 = vy + pFac * (pn - ps);
+    const size_t idx = i0 + (size_t)iy * NX;
     u[idx] = u1;
     v[idx] = v1;
-    if (maxpart && !obmap[(y / BS) * g.bpdx + x / BS]) {
-      m = dmax(m, fabs(u1 + UINF));
-      m = dmax(m, fabs(v1 + VINF));
-      m = dmax(m, fabs(u1));
-      m = dmax(m, fabs(v1));
-    }
+    double mc = dmax(m, fabs(u1 + UINF));
+    mc = dmax(mc, fabs(v1 + VINF));
+    mc = dmax(mc, fabs(u1));
+    mc = dmax(mc, fabs(v1));
+    m = count ? mc : m;
     ps = pc;
     pc = pn;
   }
