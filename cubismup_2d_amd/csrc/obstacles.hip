@@ -365,27 +365,24 @@ __device__ inline void clear_block(const Grid& g, int b, double* chi, double* ud
   }
 }
 
-// Disk::create's distance R - |x - c| (ShapeLibrary.cpp:21-34) of shape k at
-// cell (gx, gy).
-__device__ inline double disk_dist(const Grid& g, const ShapeParams& P,
-                                   const cup2d_shape_state& S, int gx, int gy) {
-  const double p0 = cell_pos(gx / BS, gx % BS, g.hb, g.h) - S.center[0];
-  const double p1 = cell_pos(gy / BS, gy % BS, g.hb, g.h) - S.center[1];
-  return P.radius - sqrt(p0 * p0 + p1 * p1);
-}
+// The shapes' kind, radius and centre, staged in LDS once per workgroup.
+struct DiskLds {
+  int kind[MAX_SHAPES];
+  double R[MAX_SHAPES], c0[MAX_SHAPES], c1[MAX_SHAPES];
+};
 
 // The grid SDF the reference assembles in sim.tmp (PutObjectsOnGrid.cpp:
 // 190-200 clears it to -1, Disk::create takes the max with each shape's
-// distance on that shape's obstacle blocks) at cell (gx, gy), clamped into
-// the domain as the BlockLab does: max over the shapes whose bounding box
-// touches the cell's block of their distance, -1 if none.
+// distance R - |x - c| (ShapeLibrary.cpp:21-34) on that shape's obstacle
+// blocks) at cell (gx, gy), clamped into the domain as the BlockLab does: max
+// over the shapes whose bounding box touches the cell's block of their
+// distance, -1 if none.
 //
 // Swimmers contribute the distance PutFishOnBlocks stored in their slots of
 // that block (signedDistanceSqrt, FishLibrary.cpp:431).
 // nbs[0..nnb) are the swimmer slots of the block that holds the (clamped)
 // cell.
-__device__ inline double grid_sdf(const Grid& g, const ShapeParams* __restrict__ sp,
-                                  const cup2d_shape_state* __restrict__ ss, const Obstacles& ob,
+__device__ inline double grid_sdf(const Grid& g, const DiskLds& D, const Obstacles& ob,
                                   const int* nbs, int nnb, int gx, int gy) {
   gx = gx < 0 ? 0 : (gx >= g.NX ? g.NX - 1 : gx);
   gy = gy < 0 ? 0 : (gy >= g.NY ? g.NY - 1 : gy);
@@ -393,11 +390,15 @@ __device__ inline double grid_sdf(const Grid& g, const ShapeParams* __restrict__
   for (int q = 0; q < nnb; ++q)
     v = dmax(v, ob.dist[(size_t)nbs[q] * BS * BS + (gy % BS) * BS + gx % BS]);
   for (int k = 0; k < g.nshapes; ++k) {
-    if (sp[k].kind != 0) continue;
-    const double safety = 2 * g.h, R = sp[k].radius;
-    const double c0 = ss[k].center[0], c1 = ss[k].center[1];
+    if (D.kind[k] != 0) continue;
+    const double safety = 2 * g.h, R = D.R[k];
+    const double c0 = D.c0[k], c1 = D.c1[k];
     const double bbox[2][2] = {{c0 - R - safety, c0 + R + safety}, {c1 - R - safety, c1 + R + safety}};
-    if (touching(g, gx / BS, gy / BS, bbox)) v = dmax(v, disk_dist(g, sp[k], ss[k], gx, gy));
+    if (touching(g, gx / BS, gy / BS, bbox)) {
+      const double p0 = cell_pos(gx / BS, gx % BS, g.hb, g.h) - c0;
+      const double p1 = cell_pos(gy / BS, gy % BS, g.hb, g.h) - c1;
+      v = dmax(v, R - sqrt(p0 * p0 + p1 * p1));
+    }
   }
   return v;
 }
@@ -414,6 +415,10 @@ __device__ inline double grid_sdf(const Grid& g, const ShapeParams* __restrict__
 // needed, so no grid-wide SDF pass is required.  The next workgroups, one per
 // obstacle block of the previous step (list pv), clear those that are not
 // obstacle blocks now (every other cell already holds the cleared values).
+//
+// The shapes' data is staged in LDS once, and the grid CHI and invRho of the
+// block stay in LDS across the block's slots: the block is cleared and
+// written once, with no read back of what was just stored.
 __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __restrict__ sp,
                                                    const cup2d_shape_state* __restrict__ ss,
                                                    Obstacles ob, Obstacles pv,
@@ -427,6 +432,7 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
   __shared__ double lds[LDS_RED];
   __shared__ int s_nbs[4][MAX_FISH];  // swimmer slots of the E, W, N, S neighbour blocks
   __shared__ int s_nbn[4];
+  __shared__ DiskLds D;
   const int nub = ob.cnt[1], w = blockIdx.x;
   if (w >= nub) {
     if (w >= nub + pv.cnt[1]) return;
@@ -436,6 +442,12 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
   }
   const int b = ob.ub_block[w];
   const int bx = b % g.bpdx, by = b / g.bpdx;
+  for (int k = threadIdx.x; k < g.nshapes; k += NT) {
+    D.kind[k] = sp[k].kind;
+    D.R[k] = sp[k].radius;
+    D.c0[k] = ss[k].center[0];
+    D.c1[k] = ss[k].center[1];
+  }
   if (threadIdx.x < 4) {
     const int d = threadIdx.x;
     int nx = bx + (d == 0) - (d == 1), ny = by + (d == 2) - (d == 3);
@@ -451,26 +463,34 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
     s_nbn[d] = n;
   }
   __syncthreads();
-  clear_block(g, b, chi, udx, udy, invrho);
+  // PutObjectsOnGrid.cpp:191-197: chi = 0, invRho = 1 (uDef = 0 is stored at
+  // the end); each thread only touches its own cells of s_chi / s_inv
+  __shared__ double s_chi[BS * BS], s_inv[BS * BS];
+  for (int i = threadIdx.x; i < BS * BS; i += NT) { s_chi[i] = 0; s_inv[i] = 1; }
   const int s0 = ob.ub_start[w], cnt = ob.ub_count[w];
   const double h = g.h, i2h = 0.5 / h, fac = 0.5 * h;
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
     SlotView o = slot(ob, s);
-    const ShapeParams& P = sp[k];
-    const cup2d_shape_state& S = ss[k];
     // swimmers: dist, udef and rho = 1 come from PutFishOnBlocks
-    const bool fish = P.kind == 1;
-    const double rhoS = fish ? 1.0 : P.rhoS;
-    const double C0 = S.centerOfMass[0], C1 = S.centerOfMass[1];
+    const bool fish = D.kind[k] == 1;
+    const double rhoS = fish ? 1.0 : sp[k].rhoS;
+    const double R = D.R[k], Sc0 = D.c0[k], Sc1 = D.c1[k];
+    const double C0 = ss[k].centerOfMass[0], C1 = ss[k].centerOfMass[1];
     double _x = 0, _y = 0, _m = 0, _ns = 0;
-    CELL_LOOP(i) {
+#pragma unroll 1
+    for (int i = threadIdx.x; i < BS * BS; i += NT) {
       const int ix = i % BS, iy = i / BS;
       const int gx = bx * BS + ix, gy = by * BS + iy;
       // the shape's own distance inside the block, the grid SDF across its
       // edges (PutObjectsOnGrid.cpp:53-56)
       double sd, distPx, distMx, distPy, distMy;
-#define GSDF(d, x, y) grid_sdf(g, sp, ss, ob, s_nbs[d], s_nbn[d], x, y)
+#define GSDF(d, x, y) grid_sdf(g, D, ob, s_nbs[d], s_nbn[d], x, y)
+#define DDIST(x, y)                                                              \
+  (R - sqrt((cell_pos((x) / BS, (x) % BS, g.hb, g.h) - Sc0) *                   \
+                (cell_pos((x) / BS, (x) % BS, g.hb, g.h) - Sc0) +                \
+            (cell_pos((y) / BS, (y) % BS, g.hb, g.h) - Sc1) *                   \
+                (cell_pos((y) / BS, (y) % BS, g.hb, g.h) - Sc1)))
       if (fish) {
         sd = o.dist[i];
         distPx = ix + 1 == BS ? GSDF(0, gx + 1, gy) : o.dist[i + 1];
@@ -478,16 +498,17 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
         distPy = iy + 1 == BS ? GSDF(2, gx, gy + 1) : o.dist[i + BS];
         distMy = iy == 0 ? GSDF(3, gx, gy - 1) : o.dist[i - BS];
       } else {
-        sd = disk_dist(g, P, S, gx, gy);
-        distPx = ix + 1 == BS ? GSDF(0, gx + 1, gy) : disk_dist(g, P, S, gx + 1, gy);
-        distMx = ix == 0 ? GSDF(1, gx - 1, gy) : disk_dist(g, P, S, gx - 1, gy);
-        distPy = iy + 1 == BS ? GSDF(2, gx, gy + 1) : disk_dist(g, P, S, gx, gy + 1);
-        distMy = iy == 0 ? GSDF(3, gx, gy - 1) : disk_dist(g, P, S, gx, gy - 1);
-#undef GSDF
+        sd = DDIST(gx, gy);
+        distPx = ix + 1 == BS ? GSDF(0, gx + 1, gy) : DDIST(gx + 1, gy);
+        distMx = ix == 0 ? GSDF(1, gx - 1, gy) : DDIST(gx - 1, gy);
+        distPy = iy + 1 == BS ? GSDF(2, gx, gy + 1) : DDIST(gx, gy + 1);
+        distMy = iy == 0 ? GSDF(3, gx, gy - 1) : DDIST(gx, gy - 1);
         o.dist[i] = sd;
         o.rho[i] = rhoS;
         o.udx[i] = 0; o.udy[i] = 0;
       }
+#undef DDIST
+#undef GSDF
       double X;
       if (sd > +2 * h || sd < -2 * h) {
         X = sd > 0 ? 1 : 0;
@@ -501,10 +522,10 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
         X = (gradIX * gradUX + gradIY * gradUY) * denum;
       }
       o.chi[i] = X;
-      const size_t d = (size_t)gy * g.NX + gx;
-      if (X >= chi[d]) {
-        chi[d] = X;
-        invrho[d] = X / rhoS + (1 - X) * invrho[d];
+      // grid CHI = max and the invRho blend (PutObjectsOnGrid.cpp:69-73)
+      if (X >= s_chi[i]) {
+        s_chi[i] = X;
+        s_inv[i] = X / rhoS + (1 - X) * s_inv[i];
       }
       if (X > 0) {
         const double p0 = cell_pos(bx, ix, g.hb, h), p1 = cell_pos(by, iy, g.hb, h);
@@ -512,7 +533,7 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
         _y += rhoS * X * h * h * (p1 - C1);
         _m += rhoS * X * h * h;
       }
-      double D = 0, nx = 0, ny = 0;
+      double Dd = 0, nx = 0, ny = 0;
       if (!(sd > +2 * h || sd < -2 * h)) {
         const double HplusX = fabs(distPx) < EPSD ? 0.5 : (distPx < 0 ? 0 : 1);
         const double HminuX = fabs(distMx) < EPSD ? 0.5 : (distMx < 0 ? 0 : 1);
@@ -524,13 +545,13 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
         const double denum = 1 / (gradUSq < EPSD ? EPSD : gradUSq);
         const double Dc = fac * (gradHX * gradUX + gradHY * gradUY) * denum;
         if (Dc > 0) {
-          D = Dc;
+          Dd = Dc;
           nx = -Dc * gradUX;
           ny = -Dc * gradUY;
           _ns += 1;
         }
       }
-      o.sd[i] = D;
+      o.sd[i] = Dd;
       o.sx[i] = nx;
       o.sy[i] = ny;
     }
@@ -540,6 +561,13 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
       o.part[P_COM] = a4[0]; o.part[P_COM + 1] = a4[1]; o.part[P_COM + 2] = a4[2];
       o.part[P_NSURF] = a4[3];
     }
+  }
+  CELL_LOOP(i) {
+    const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+    chi[d] = s_chi[i];
+    invrho[d] = s_inv[i];
+    udx[d] = 0;
+    udy[d] = 0;
   }
 }
 
