@@ -726,6 +726,14 @@ static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, in
 #define CUP2D_FORCES_DEFER 1
 #endif
 
+// Swimmers: their midline (k_fish_mid_b) and the list builder run on the
+// main stream before advDiff (1) or first in the side-stream chain (0).
+// Beside advDiff these latency-bound single-workgroup kernels run 2-3x slower
+// than alone, which costs the chain more than running them alone first.
+#ifndef CUP2D_FISH_HEAD_MAIN
+#define CUP2D_FISH_HEAD_MAIN 1
+#endif
+
 int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   const int G = grid_cap(c);
   // the previous whole step's ComputeForces runs after this step's obstacle
@@ -745,19 +753,25 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
                        c->ramp, compute_dt, c->host_step, fish ? 1 : 0, end_prev);
   }
   hipStream_t s = c->stream;
+  if (fish && (!overlap || CUP2D_FISH_HEAD_MAIN)) {
+    RC_OB(enqueue_fish_midline(c, c->stream));
+    hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
+                       c->ob, c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg,
+                       c->ramp, 0, c->host_step, 2, 0);
+  }
   if (overlap) {
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_begin, c->stream));
     CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_begin, 0));
     s = c->aux;
   }
   if (fish) {
-    int rc = enqueue_fish_midline(c, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
-                       c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp, 0,
-                       c->host_step, 2, 0);
-    rc = enqueue_fish_place(c, s);
-    if (rc) return rc;
+    if (overlap && !CUP2D_FISH_HEAD_MAIN) {
+      RC_OB(enqueue_fish_midline(c, s));
+      hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
+                         c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp,
+                         0, c->host_step, 2, 0);
+    }
+    RC_OB(enqueue_fish_place(c, s));
   }
   if (c->g.nshapes > 0) {
     hipLaunchKernelGGL(k_obst_place, dim3(2 * G), dim3(NT), 0, s, c->g, c->sp, c->ss, c->ob,
