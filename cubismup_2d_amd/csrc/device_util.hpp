@@ -54,12 +54,17 @@ __host__ __device__ inline double cell_pos(int blk_index, int i, double hb, doub
 
 // PoissonSolver::cub2rhs `_is_touching` (PoissonSolver.cpp:71-79): block
 // (bx, by) lies within fadeLen of a domain face.
+// band_block = band_col || band_row: the x and y tests separately
+__host__ __device__ inline bool band_col(const Grid& g, int bx) {
+  const double mn0 = cell_pos(bx, 0, g.hb, g.h), mx0 = cell_pos(bx, BS - 1, g.hb, g.h);
+  return g.fadeLenX >= g.extent0 - mx0 || g.fadeLenX >= mn0;  // touchE || touchW
+}
+__host__ __device__ inline bool band_row(const Grid& g, int by) {
+  const double mn1 = cell_pos(by, 0, g.hb, g.h), mx1 = cell_pos(by, BS - 1, g.hb, g.h);
+  return g.fadeLenY >= g.extent1 - mx1 || g.fadeLenY >= mn1;  // touchN || touchS
+}
 __host__ __device__ inline bool band_block(const Grid& g, int bx, int by) {
-  const double mn0 = cell_pos(bx, 0, g.hb, g.h), mn1 = cell_pos(by, 0, g.hb, g.h);
-  const double mx0 = cell_pos(bx, BS - 1, g.hb, g.h), mx1 = cell_pos(by, BS - 1, g.hb, g.h);
-  const bool touchW = g.fadeLenX >= mn0, touchE = g.fadeLenX >= g.extent0 - mx0;
-  const bool touchS = g.fadeLenY >= mn1, touchN = g.fadeLenY >= g.extent1 - mx1;
-  return touchN || touchE || touchS || touchW;
+  return band_row(g, by) || band_col(g, bx);
 }
 
 // cub2rhs fadeArg (PoissonSolver.cpp:88-93) at cell centre (px, py).

@@ -298,27 +298,24 @@ constexpr int rows_np() { return (L / E) >= 256 ? 1 : 256 / (L / E); }
 // SRC_FADE: the input is the pressure RHS on the grid; cells of band blocks
 // are faded as PoissonSolver::cub2rhs does (PoissonSolver.cpp:105-121) while
 // they are loaded, and written back (the reference modifies the source in
-// place).  The correction factor comes from the partial sums of
-// k_fade_sums, added in the same order by every workgroup.
+// place).  The correction factor is st->fade_corr (k_fade_reduce).
 //
 // SRC_DIV: the RHS is formed here from the advDiff output (u, v), as
 // PressureSingle::updatePressureRHS does (PressureSingle.cpp:112-128), with
-// the deferred advDiff boundary-flux correction; the obstacle blocks take
+// the deferred advDiff boundary-flux correction st->flux_corr (k_flux_reduce,
+// right after advDiff); the obstacle blocks take
 // the values the obstacle RHS kernels left in the RHS plane; then the fade
 // as for SRC_FADE.  u of both rows of a pair goes through LDS for the x
 // neighbours; v of rows y0-1 .. y0+2 stays in registers.
 enum { SRC_PLAIN = 0, SRC_FADE = 1, SRC_DIV = 2 };
 struct RowSource {
   Grid g;
-  const double* part;   // fade partials: 3 sums per fade-sum workgroup
   int npart;            // fade-sum workgroups (0: no fade)
   double* rhs;          // RHS plane: faded band cells written back; obstacle blocks read (SRC_DIV)
-  StepState* st;        // fade_corr, flux_corr, dt
+  const StepState* st;  // fade_corr, flux_corr, dt
   const double* u;      // SRC_DIV: advDiff output
   const double* v;
   const unsigned char* obmap;
-  const double* fluxpart;  // advDiff edge partials
-  int ne;
   int mirror;           // cosine: columns past ncols read the row reversed
 };
 
@@ -328,7 +325,6 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
     int spec_rows, const double2* __restrict__ tw, RowSource fd) {
   constexpr int NT = L / E, PADL = lds_len<L, E>();
   extern __shared__ double2 lds[];
-  __shared__ double s_red[64];  // workgroup reductions (<= 16 waves x 3)
   const int f = threadIdx.x / NT, t = threadIdx.x % NT;
   // SRC_DIV reads the v rows next to its pair too: consecutive pairs run on
   // one XCD (workgroup b runs on XCD b % 8) so those rows are L2 hits
@@ -362,8 +358,7 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
         vp[q] = fd.v[(size_t)yp * NX + i];
       }
     }
-    const double fcorr = flux_corr256(fd.fluxpart, fd.ne, g, s_red);
-    if (blockIdx.x == 0 && threadIdx.x == 0) fd.st->flux_corr = fcorr;
+    const double fcorr = fd.st->flux_corr;  // k_flux_reduce, after advDiff
     const double facDiv = 0.5 * g.h / fd.st->dt;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -411,12 +406,10 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
   }
   if constexpr (SRC != SRC_PLAIN) {
     if (fd.npart > 0) {
-      double sums[3];
-      block_sum_table<NP * L / E, 3>(fd.part, fd.npart, 3, sums, s_red);
-      const double corr = fade_corr_factor(sums[0], sums[1], sums[2]);
-      if (blockIdx.x == 0 && threadIdx.x == 0) fd.st->fade_corr = corr;
+      const double corr = fd.st->fade_corr;  // k_fade_reduce
       const Grid& g = fd.g;
       const int by0 = y0 / BS, by1 = (y0 + 1) / BS;
+      const bool row0 = band_row(g, by0), row1 = band_row(g, by1);
       if (valid) {
 #pragma unroll
         for (int q = 0; q < E; ++q) {
@@ -425,13 +418,15 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
           if (i >= ncols) continue;
           const bool own = i == iq;  // the mirrored copy is written back once
           const int bx = i / BS;
+          const bool col = band_col(g, bx);
+          if (!(row0 || row1 || col)) continue;
           const double px = cell_pos(bx, i % BS, g.hb, g.h);
-          if (band_block(g, bx, by0)) {
+          if (row0 || col) {
             const double py = cell_pos(by0, y0 % BS, g.hb, g.h);
             x[q].x = fade_cell(x[q].x, fade_arg(g, px, py), corr);
             if (own) fd.rhs[(size_t)y0 * in_pitch + i] = x[q].x;
           }
-          if (band_block(g, bx, by1)) {
+          if (row1 || col) {
             const double py = cell_pos(by1, (y0 + 1) % BS, g.hb, g.h);
             x[q].y = fade_cell(x[q].y, fade_arg(g, px, py), corr);
             if (own) fd.rhs[(size_t)(y0 + 1) * in_pitch + i] = x[q].y;
@@ -904,8 +899,8 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   int rc;
   {
     KernelTimer kt(c, T_FFT_ROWS_FWD);
-    RowSource fd{c->g, c->red, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
-                 c->u, c->v, c->obmap, c->fluxpart, stencil_edge_count(c), P->cs};
+    RowSource fd{c->g, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
+                 c->u, c->v, c->obmap, P->cs};
     rc = rows_fwd(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r, src, fd);
   }
   if (rc) return rc;

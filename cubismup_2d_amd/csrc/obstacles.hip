@@ -818,14 +818,13 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
                                                   Obstacles ob, const double* __restrict__ chi,
                                                   double* __restrict__ r,
                                                   const double* __restrict__ u,
-                                                  const double* __restrict__ v,
-                                                  const double* __restrict__ fluxpart, int ne) {
+                                                  const double* __restrict__ v) {
   __shared__ double lds[LDS_RED];
   if ((int)blockIdx.x >= ob.cnt[1]) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   if (u) {
-    const double corr = flux_corr256(fluxpart, ne, g, lds);
+    const double corr = st->flux_corr;  // deferred advDiff flux correction
     const double facDiv = 0.5 * g.h / st->dt;
     CELL_LOOP(i) {
       const int x = bx * BS + i % BS, y = by * BS + i / BS;
@@ -894,7 +893,7 @@ int enqueue_rhs_obstacles(cup2d_ctx* c, int compute) {
   const int G = grid_cap(c);
   TimerScope ts(c, T_RHS_OBST);
   hipLaunchKernelGGL(k_rhs_obst1, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
-                     c->rhs, compute ? c->u : nullptr, c->v, c->fluxpart, stencil_edge_count(c));
+                     c->rhs, compute ? c->u : nullptr, c->v);
   hipLaunchKernelGGL(k_rhs_obst2, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
                      c->rhs);
   return launch_check(c, "rhs_obstacles");

@@ -291,7 +291,11 @@ int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
   std::swap(c->u, c->tu);  // advDiff.cpp:81-87 copy tmpV -> vel
   std::swap(c->v, c->tv);
   c->flux_deferred = defer_fix != 0;
-  if (!defer_fix) {
+  if (defer_fix) {
+    // the correction once, for every reader of the deferred fix
+    hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, c->stream, c->fluxpart, gx, gy, c->g,
+                       c->st);
+  } else {
     TimerScope ts(c, T_FLUXFIX);
     hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, c->stream, c->fluxpart, gx, gy, c->g,
                        c->st);
@@ -399,14 +403,14 @@ __global__ __launch_bounds__(256) void k_fade_sums(const double* __restrict__ r,
                                                    const double* __restrict__ u,
                                                    const double* __restrict__ v,
                                                    const unsigned char* __restrict__ obmap,
-                                                   const double* __restrict__ fluxpart, int ne,
                                                    const StepState* __restrict__ st) {
   __shared__ double lds[12];
   // u != nullptr: the RHS of cells outside obstacle blocks is formed from the
-  // advDiff output (see k_rows_fwd), the obstacle blocks' from the RHS plane
+  // advDiff output (see k_rows_fwd) with the deferred flux correction, the
+  // obstacle blocks' from the RHS plane
   double corr = 0, facDiv = 0;
   if (u) {
-    corr = flux_corr256(fluxpart, ne, g, lds);
+    corr = st->flux_corr;
     facDiv = 0.5 * g.h / st->dt;
   }
   double sR = 0, sP = 0, sN = 0;
@@ -438,12 +442,24 @@ __global__ __launch_bounds__(256) void k_fade_sums(const double* __restrict__ r,
 
 int fade_partials(const cup2d_ctx* c) { return c->band_nblocks; }
 
+// the correction factor of the fade (PoissonSolver.cpp:105-121) from the
+// partials of k_fade_sums, once for every reader
+__global__ __launch_bounds__(256) void k_fade_reduce(const double* __restrict__ part, int npart,
+                                                     StepState* __restrict__ st) {
+  __shared__ double lds[64];
+  double sums[3];
+  block_sum_table<256, 3>(part, npart, 3, sums, lds);
+  if (threadIdx.x == 0) st->fade_corr = fade_corr_factor(sums[0], sums[1], sums[2]);
+}
+
 int enqueue_fade_sums(cup2d_ctx* c, int compute) {
   if (c->band_nblocks == 0) return CUP2D_OK;
   TimerScope ts(c, T_FADE);
   hipLaunchKernelGGL(k_fade_sums, dim3(fade_partials(c)), dim3(256), 0, c->stream, c->rhs,
                      c->band_blocks, c->band_nblocks, c->g, c->red, compute ? c->u : nullptr,
-                     c->v, c->obmap, c->fluxpart, stencil_edge_count(c), c->st);
+                     c->v, c->obmap, c->st);
+  hipLaunchKernelGGL(k_fade_reduce, dim3(1), dim3(256), 0, c->stream, c->red, fade_partials(c),
+                     c->st);
   return launch_check(c, "cub2rhs fade");
 }
 
