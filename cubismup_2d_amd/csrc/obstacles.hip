@@ -137,19 +137,63 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
   const int ns = g.nshapes;
   const int nub_prev = pv.cnt[1];
   __shared__ double s_uv[MAX_SHAPES][2];
+  __shared__ int s_fix[MAX_SHAPES][2];
   __shared__ double s_step[3];  // dt, U_inf
-  if (end_prev && t == 0) {  // the previous whole step ends (Simulation.cpp:289-290)
-    st->time += st->dt_done;
-    st->step += 1;
+  // Every global value of the dt / motion part is loaded first, so the
+  // kernel waits for one round trip instead of one per dependent step.
+  double mA = 0;  // the max of the projection's partials (count known upfront)
+  if (phase != 2 && compute_dt) mA = max_array<NT>(maxpart, nmaxpart);
+  double Tu = 0, Tv = 0, Tom = 0, Tcm0 = 0, Tcm1 = 0, Tl0 = 0, Tl1 = 0, Tor = 0, Tg0 = 0, Tg1 = 0;
+  // the bounding-box inputs of the list part, one lane per shape
+  int kind = 0;
+  double R = 0, Cx = 0, Cy = 0, fbox[4] = {0, 0, 0, 0};
+  if (phase != 1 && t < ns) {
+    kind = sp[t].kind;
+    if (kind == 1) {
+      const double* B = fd[sp[t].fish].box;
+      for (int q = 0; q < 4; ++q) fbox[q] = B[q];
+    } else {
+      R = sp[t].radius;
+      if (phase == 2) {  // phase 0 moves the shape below, on this lane
+        Cx = ss[t].center[0];
+        Cy = ss[t].center[1];
+      }
+    }
+  }
+  if (phase != 2 && t < ns) {
+    const cup2d_shape_state& T = ss[t];
+    Tu = T.u; Tv = T.v; Tom = T.omega;
+    Tcm0 = T.centerOfMass[0]; Tcm1 = T.centerOfMass[1];
+    Tl0 = T.labCenterOfMass[0]; Tl1 = T.labCenterOfMass[1];
+    Tor = T.orientation; Tg0 = T.d_gm[0]; Tg1 = T.d_gm[1];
+    s_fix[t][0] = sp[t].bFixedx;
+    s_fix[t][1] = sp[t].bFixedy;
+  }
+  double sTime = 0, sDt = 0, sU0 = 0, sU1 = 0, sRamp = 0;
+  if (t == 0) {
+    sTime = st->time;
+    sDt = st->dt;
+    sU0 = st->uinf[0];
+    sU1 = st->uinf[1];
+    if (phase != 2 && compute_dt && step_idx < 100) sRamp = ramp[step_idx];
+    if (end_prev) {  // the previous whole step ends (Simulation.cpp:289-290)
+      sTime += st->dt_done;
+      st->time = sTime;
+      st->step += 1;
+    }
   }
   if (phase != 2) {
     if (t < ns) {
-      s_uv[t][0] = ss[t].u;
-      s_uv[t][1] = ss[t].v;
+      s_uv[t][0] = Tu;
+      s_uv[t][1] = Tv;
     }
+    // the partials of the previous step's obstacle blocks follow the
+    // projection's (max does not depend on the order)
+    if (compute_dt)
+      for (int i = nmaxpart + t; i < nmaxpart + nub_prev; i += NT) mA = dmax(mA, maxpart[i]);
     __syncthreads();
     if (compute_dt) {
-      const double maxU = block_max<NT>(max_array<NT>(maxpart, nmaxpart + nub_prev), lds);
+      const double maxU = block_max<NT>(mA, lds);
       if (t == 0) {
         const double h = g.h;
         const double dtFourier = h * h / g.nu, dtCFL = maxU < 2.2e-16 ? 1 : h / maxU;
@@ -159,47 +203,53 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
         const double dtBody = maxUb < 2.2e-16 ? 1 : h / maxUb;
         const double mn = dmin(dmin(dtCFL, dtFourier), dtBody);
         double dt = g.CFL * mn;
-        if (step_idx < 100) dt = ramp[step_idx] * mn;
+        if (step_idx < 100) dt = sRamp * mn;
         st->dt = dt;
         st->maxU = maxU;
         s_step[0] = dt;
       }
     } else if (t == 0) {
-      s_step[0] = st->dt;
+      s_step[0] = sDt;
     }
     if (t == 0) {  // U_inf, in shape order
       int nSum[2] = {0, 0};
       double uSum[2] = {0, 0};
       for (int k = 0; k < ns; ++k) {
-        if (sp[k].bFixedx) { nSum[0]++; uSum[0] -= s_uv[k][0]; }
-        if (sp[k].bFixedy) { nSum[1]++; uSum[1] -= s_uv[k][1]; }
+        if (s_fix[k][0]) { nSum[0]++; uSum[0] -= s_uv[k][0]; }
+        if (s_fix[k][1]) { nSum[1]++; uSum[1] -= s_uv[k][1]; }
       }
-      double U0 = st->uinf[0], U1 = st->uinf[1];
+      double U0 = sU0, U1 = sU1;
       if (nSum[0] > 0) U0 = uSum[0] / nSum[0];
       if (nSum[1] > 0) U1 = uSum[1] / nSum[1];
       st->uinf[0] = U0;
       st->uinf[1] = U1;
       s_step[1] = U0;
       s_step[2] = U1;
+      st->tnext[step_idx & 1] = sTime + s_step[0];  // this step's end time
     }
-    if (t == 0) st->tnext[step_idx & 1] = st->time + s_step[0];  // this step's end time
     __syncthreads();
     if (t < ns) {  // each shape moves on its own lane
       const double dt = s_step[0], U0 = s_step[1], U1 = s_step[2];
       cup2d_shape_state& T = ss[t];
-      T.centerOfMass[0] += dt * (T.u + U0);
-      T.centerOfMass[1] += dt * (T.v + U1);
-      T.labCenterOfMass[0] += dt * T.u;
-      T.labCenterOfMass[1] += dt * T.v;
-      T.orientation += dt * T.omega;
-      T.orientation = T.orientation > M_PI ? T.orientation - 2 * M_PI : T.orientation;
-      T.orientation = T.orientation < -M_PI ? T.orientation + 2 * M_PI : T.orientation;
+      const double cm0 = Tcm0 + dt * (Tu + U0);
+      const double cm1 = Tcm1 + dt * (Tv + U1);
+      T.centerOfMass[0] = cm0;
+      T.centerOfMass[1] = cm1;
+      T.labCenterOfMass[0] = Tl0 + dt * Tu;
+      T.labCenterOfMass[1] = Tl1 + dt * Tv;
+      double o = Tor + dt * Tom;
+      o = o > M_PI ? o - 2 * M_PI : o;
+      o = o < -M_PI ? o + 2 * M_PI : o;
+      T.orientation = o;
       double sinang, cosang;
-      det_sincos(T.orientation, &sinang, &cosang);
-      T.center[0] = T.centerOfMass[0] + cosang * T.d_gm[0] - sinang * T.d_gm[1];
-      T.center[1] = T.centerOfMass[1] + sinang * T.d_gm[0] + cosang * T.d_gm[1];
-      if (T.center[0] < 0 || T.center[0] > g.extent0 || T.center[1] < 0 || T.center[1] > g.extent1)
-        st->error = CUP2D_ERR_BODY_OUT;
+      det_sincos(o, &sinang, &cosang);
+      const double c0 = cm0 + cosang * Tg0 - sinang * Tg1;
+      const double c1 = cm1 + sinang * Tg0 + cosang * Tg1;
+      T.center[0] = c0;
+      T.center[1] = c1;
+      Cx = c0;
+      Cy = c1;
+      if (c0 < 0 || c0 > g.extent0 || c1 < 0 || c1 > g.extent1) st->error = CUP2D_ERR_BODY_OUT;
     }
   }
   if (phase != 1)
@@ -209,26 +259,29 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
     }
   __syncthreads();
   if (phase == 1) return;
+  if (t < ns) {  // each shape's candidate block range on its own lane
+    const int k = t;
+    if (kind == 1) {  // union of the swimmer's segment boxes
+      for (int q = 0; q < 4; ++q) s_box[k][q] = fbox[q];
+    } else {
+      const double safety = 2 * g.h;
+      s_box[k][0] = Cx - R - safety; s_box[k][1] = Cx + R + safety;
+      s_box[k][2] = Cy - R - safety; s_box[k][3] = Cy + R + safety;
+    }
+    int bx0 = (int)floor(s_box[k][0] / g.hb) - 1, bx1 = (int)floor(s_box[k][1] / g.hb) + 1;
+    int by0 = (int)floor(s_box[k][2] / g.hb) - 1, by1 = (int)floor(s_box[k][3] / g.hb) + 1;
+    bx0 = bx0 < 0 ? 0 : bx0; by0 = by0 < 0 ? 0 : by0;
+    bx1 = bx1 >= g.bpdx ? g.bpdx - 1 : bx1; by1 = by1 >= g.bpdy ? g.bpdy - 1 : by1;
+    const int w = bx1 >= bx0 ? bx1 - bx0 + 1 : 0, hgt = by1 >= by0 ? by1 - by0 + 1 : 0;
+    s_rng[k][0] = bx0; s_rng[k][1] = by0; s_rng[k][2] = w;
+    s_tmp[k] = w * hgt;
+  }
+  __syncthreads();
   if (t == 0) {
     int nc = 0;
     for (int k = 0; k < ns; ++k) {
-      if (sp[k].kind == 1) {  // union of the swimmer's segment boxes
-        const double* B = fd[sp[k].fish].box;
-        for (int q = 0; q < 4; ++q) s_box[k][q] = B[q];
-      } else {
-        const double safety = 2 * g.h, R = sp[k].radius;
-        const double c0 = ss[k].center[0], c1 = ss[k].center[1];
-        s_box[k][0] = c0 - R - safety; s_box[k][1] = c0 + R + safety;
-        s_box[k][2] = c1 - R - safety; s_box[k][3] = c1 + R + safety;
-      }
-      int bx0 = (int)floor(s_box[k][0] / g.hb) - 1, bx1 = (int)floor(s_box[k][1] / g.hb) + 1;
-      int by0 = (int)floor(s_box[k][2] / g.hb) - 1, by1 = (int)floor(s_box[k][3] / g.hb) + 1;
-      bx0 = bx0 < 0 ? 0 : bx0; by0 = by0 < 0 ? 0 : by0;
-      bx1 = bx1 >= g.bpdx ? g.bpdx - 1 : bx1; by1 = by1 >= g.bpdy ? g.bpdy - 1 : by1;
-      const int w = bx1 >= bx0 ? bx1 - bx0 + 1 : 0, hgt = by1 >= by0 ? by1 - by0 + 1 : 0;
-      s_rng[k][0] = bx0; s_rng[k][1] = by0; s_rng[k][2] = w;
       s_cstart[k] = nc;
-      nc += w * hgt;
+      nc += s_tmp[k];
     }
     if (nc > LCAP) {  // more candidate blocks than the list builder holds
       st->error = CUP2D_ERR_CAPACITY;
