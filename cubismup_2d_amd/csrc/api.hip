@@ -177,8 +177,7 @@ int pressure_single(cup2d_ctx* c, int fused) {
     RC(enqueue_poisson_fft(c, c->rhs, c->pres, 1));
   }
   RC(enqueue_pressure_correction(c, 1));
-  RC(enqueue_body_update(c));
-  RC(enqueue_penalize(c, c->maxpart + c->nwg));
+  RC(enqueue_body_penalize(c, c->maxpart + c->nwg));
   return CUP2D_OK;
 }
 

@@ -283,6 +283,7 @@ int join_obstacles(cup2d_ctx* c);
 int enqueue_rhs_obstacles(cup2d_ctx* c, int compute = 0);
 int enqueue_body_update(cup2d_ctx* c);
 int enqueue_penalize(cup2d_ctx* c, double* maxpart);
+int enqueue_body_penalize(cup2d_ctx* c, double* maxpart);
 int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux = 0);
 // enqueue pending ComputeForces and step end on the main stream
 int flush_forces(cup2d_ctx* c);

@@ -989,6 +989,30 @@ __device__ void lu3_solve(double A[3][3], const double b[3], double x[3]) {
 
 // Shape::updateVelocity (Shape.cpp:25-59), Disk::updateVelocity
 // (ShapesSimple.cpp:83-87), preventCollidingObstacles (PressureSingle.cpp:243-344).
+// Shape::updateVelocity (Shape.cpp:25-59) from the penalization sums
+// S = penalM penalJ penalDX penalDY fluidMomX fluidMomY fluidAngMom; uvw holds
+// the current u v omega on entry (kept where forced) and the new ones on exit.
+__device__ inline void shape_velocity(const ShapeParams& P, const double S[7], double dt,
+                                      double time, double uvw[3]) {
+  const double penalM = S[0], penalJ = S[1], penalDX = S[2], penalDY = S[3];
+  double A[3][3] = {{penalM, 0, -penalDY}, {0, penalM, penalDX}, {-penalDY, penalDX, penalJ}};
+  double bb[3] = {S[4] + dt * P.appliedForceX, S[5] + dt * P.appliedForceY,
+                  S[6] + dt * P.appliedTorque};
+  const double tf = P.timeForced;
+  if (P.bForcedx && time < tf) { A[0][1] = 0; A[0][2] = 0; bb[0] = penalM * P.forcedu; }
+  if (P.bForcedy && time < tf) { A[1][0] = 0; A[1][2] = 0; bb[1] = penalM * P.forcedv; }
+  if (P.bBlockAng && time < tf) { A[2][0] = 0; A[2][1] = 0; bb[2] = penalJ * P.forcedomega; }
+  double x[3];
+  lu3_solve(A, bb, x);
+  if (!P.bForcedy || time > tf) uvw[0] = x[0];  // Shape.cpp:56 tests bForcedy for u
+  if (!P.bForcedy || time > tf) uvw[1] = x[1];
+  if (!P.bBlockAng || time > tf) uvw[2] = x[2];
+  if (P.tAccel > 0) {
+    if (P.bForcedx && time < P.tAccel) uvw[0] = (time / P.tAccel) * P.forcedu;
+    if (P.bForcedy && time < P.tAccel) uvw[1] = (time / P.tAccel) * P.forcedv;
+  }
+}
+
 __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
                               cup2d_shape_state* __restrict__ ss,
                               StepState* __restrict__ st, const Obstacles& ob,
@@ -1002,26 +1026,11 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
     shape_sums<7>(ob, k, P_PEN, S, lds);
     if (threadIdx.x == 0) {
       cup2d_shape_state& T = ss[k];
-      const ShapeParams& P = sp[k];
+      double uvw[3] = {T.u, T.v, T.omega};
+      shape_velocity(sp[k], S, dt, time, uvw);
       T.penalM = S[0]; T.penalJ = S[1]; T.penalDX = S[2]; T.penalDY = S[3];
       T.fluidMomX = S[4]; T.fluidMomY = S[5]; T.fluidAngMom = S[6];
-      double A[3][3] = {{T.penalM, 0, -T.penalDY}, {0, T.penalM, T.penalDX},
-                        {-T.penalDY, T.penalDX, T.penalJ}};
-      double bb[3] = {T.fluidMomX + dt * P.appliedForceX, T.fluidMomY + dt * P.appliedForceY,
-                      T.fluidAngMom + dt * P.appliedTorque};
-      const double tf = P.timeForced;
-      if (P.bForcedx && time < tf) { A[0][1] = 0; A[0][2] = 0; bb[0] = T.penalM * P.forcedu; }
-      if (P.bForcedy && time < tf) { A[1][0] = 0; A[1][2] = 0; bb[1] = T.penalM * P.forcedv; }
-      if (P.bBlockAng && time < tf) { A[2][0] = 0; A[2][1] = 0; bb[2] = T.penalJ * P.forcedomega; }
-      double x[3];
-      lu3_solve(A, bb, x);
-      if (!P.bForcedy || time > tf) T.u = x[0];  // Shape.cpp:56 tests bForcedy for u
-      if (!P.bForcedy || time > tf) T.v = x[1];
-      if (!P.bBlockAng || time > tf) T.omega = x[2];
-      if (P.tAccel > 0) {
-        if (P.bForcedx && time < P.tAccel) T.u = (time / P.tAccel) * P.forcedu;
-        if (P.bForcedy && time < P.tAccel) T.v = (time / P.tAccel) * P.forcedv;
-      }
+      T.u = uvw[0]; T.v = uvw[1]; T.omega = uvw[2];
     }
   }
   if (ns < 2 || !collide) return;
@@ -1177,49 +1186,108 @@ int enqueue_body_update(cup2d_ctx* c) {
 }
 
 // PressureSingle::penalize (PressureSingle.cpp:69-108), per distinct block,
-// shapes in order.
-__global__ __launch_bounds__(NT) void k_penalize(Grid g, const cup2d_shape_state* __restrict__ ss,
-                                                 const StepState* __restrict__ st, Obstacles ob,
+// shapes in order.  The block's velocities stay in registers across its
+// slots and are stored once; the findMaxU partial of the block comes from
+// the same registers.
+//
+// SOLVE: the body update's solve (Shape::updateVelocity) is done here by
+// every workgroup, from the momenta partials in the same fixed order, so the
+// separate single-workgroup solve kernel and its launch are saved (used with
+// fewer than two shapes, where no collision step follows the solve).
+// Workgroup 0 stores the shape state, the ComputeForces snapshot and
+// dt_done; where a shape is forced it rewrites the value every workgroup read.
+template <bool SOLVE>
+__global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __restrict__ sp,
+                                                 cup2d_shape_state* __restrict__ ss,
+                                                 StepState* __restrict__ st, Obstacles ob,
                                                  const double* __restrict__ chi,
                                                  double* __restrict__ u, double* __restrict__ v,
-                                                 double* __restrict__ maxpart) {
+                                                 double* __restrict__ maxpart,
+                                                 ForceSnap* __restrict__ snap) {
   __shared__ double lds[LDS_RED];
-  if ((int)blockIdx.x >= ob.cnt[1]) return;
+  __shared__ double s_uvw[MAX_SHAPES][3];
+  const int nub = ob.cnt[1];
+  if ((int)blockIdx.x >= nub && !(SOLVE && blockIdx.x == 0)) return;
+  const double dt = st->dt;
+  if (SOLVE) {
+    const double time = st->time;
+    for (int k = 0; k < g.nshapes; ++k) {
+      double S[7];
+      shape_sums<7>(ob, k, P_PEN, S, lds);
+      if (threadIdx.x == 0) {
+        cup2d_shape_state& T = ss[k];
+        double uvw[3] = {T.u, T.v, T.omega};
+        shape_velocity(sp[k], S, dt, time, uvw);
+        s_uvw[k][0] = uvw[0]; s_uvw[k][1] = uvw[1]; s_uvw[k][2] = uvw[2];
+        if (blockIdx.x == 0) {
+          T.penalM = S[0]; T.penalJ = S[1]; T.penalDX = S[2]; T.penalDY = S[3];
+          T.fluidMomX = S[4]; T.fluidMomY = S[5]; T.fluidAngMom = S[6];
+          T.u = uvw[0]; T.v = uvw[1]; T.omega = uvw[2];
+        }
+      }
+    }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+      // state ComputeForces uses (it may run after the next step moved the bodies)
+      if ((int)threadIdx.x < g.nshapes) {
+        const int k = threadIdx.x;
+        ForceSnap f;
+        f.C[0] = ss[k].centerOfMass[0]; f.C[1] = ss[k].centerOfMass[1];
+        f.u = s_uvw[k][0]; f.v = s_uvw[k][1]; f.omega = s_uvw[k][2]; f.pad_ = 0;
+        snap[k] = f;
+      }
+      if (threadIdx.x == 0) st->dt_done = dt;
+      if (nub == 0) return;
+    }
+  }
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
-  const double dt = st->dt;
+  constexpr int NC = BS * BS / NT;
+  double uR[NC], vR[NC], cR[NC];
+  CELL_LOOP(i) {
+    const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+    uR[i_it] = u[d];
+    vR[i_it] = v[d];
+    cR[i_it] = chi[d];
+  }
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
     SlotView o = slot(ob, s);
-    const double u_s = ss[k].u, v_s = ss[k].v, omega_s = ss[k].omega;
+    const double u_s = SOLVE ? s_uvw[k][0] : ss[k].u;
+    const double v_s = SOLVE ? s_uvw[k][1] : ss[k].v;
+    const double omega_s = SOLVE ? s_uvw[k][2] : ss[k].omega;
     const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
     CELL_LOOP(i) {
       const int ix = i % BS, iy = i / BS;
-      const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
       const double X = o.chi[i];
-      if (chi[d] > X) continue;
+      if (cR[i_it] > X) continue;
       if (X <= 0) continue;
       const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
       const double alpha = g.implicit_penal ? 1 / (1 + g.lambda * dt * X) : 1 - X;
       const double US = u_s - omega_s * p1 + o.udx[i];
       const double VS = v_s + omega_s * p0 + o.udy[i];
-      u[d] = alpha * u[d] + (1 - alpha) * US;
-      v[d] = alpha * v[d] + (1 - alpha) * VS;
+      uR[i_it] = alpha * uR[i_it] + (1 - alpha) * US;
+      vR[i_it] = alpha * vR[i_it] + (1 - alpha) * VS;
     }
+  }
+  CELL_LOOP(i) {
+    const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+    u[d] = uR[i_it];
+    v[d] = vR[i_it];
   }
   if (!maxpart) return;
   // findMaxU (Helpers.cpp:160-175) over this block's final velocities; the
   // projection kernel skips obstacle blocks.
   const double UINF = st->uinf[0], VINF = st->uinf[1];
   double m = 0;
-  CELL_LOOP(i) {
-    const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
-    const double a = u[d], c = v[d];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const double a = uR[c], w = vR[c];
     m = dmax(m, fabs(a + UINF));
-    m = dmax(m, fabs(c + VINF));
+    m = dmax(m, fabs(w + VINF));
     m = dmax(m, fabs(a));
-    m = dmax(m, fabs(c));
+    m = dmax(m, fabs(w));
   }
   m = block_max<NT>(m, lds);
   if (threadIdx.x == 0) maxpart[blockIdx.x] = m;
@@ -1228,9 +1296,29 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const cup2d_shape_state
 int enqueue_penalize(cup2d_ctx* c, double* maxpart) {
   if (c->g.nshapes == 0) return CUP2D_OK;
   TimerScope ts(c, T_PENALIZE);
-  hipLaunchKernelGGL(k_penalize, dim3(grid_cap(c)), dim3(NT), 0, c->stream, c->g, c->ss, c->st,
-                     c->ob, c->chi, c->u, c->v, maxpart);
+  hipLaunchKernelGGL(k_penalize<false>, dim3(grid_cap(c)), dim3(NT), 0, c->stream, c->g, c->sp,
+                     c->ss, c->st, c->ob, c->chi, c->u, c->v, maxpart, c->fsnap);
   return launch_check(c, "penalize");
+}
+
+// PressureSingle's body update + penalization (PressureSingle.cpp:360-368):
+// with fewer than two shapes the solve runs inside the penalization kernel.
+int enqueue_body_penalize(cup2d_ctx* c, double* maxpart) {
+  if (c->g.nshapes == 0) return CUP2D_OK;
+  if (c->g.nshapes > 1) {
+    RC_OB(enqueue_body_update(c));
+    return enqueue_penalize(c, maxpart);
+  }
+  const int G = grid_cap(c);
+  {
+    TimerScope ts(c, T_BODY);
+    hipLaunchKernelGGL(k_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
+                       c->ob, c->u, c->v);
+  }
+  TimerScope ts(c, T_PENALIZE);
+  hipLaunchKernelGGL(k_penalize<true>, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss,
+                     c->st, c->ob, c->chi, c->u, c->v, maxpart, c->fsnap);
+  return launch_check(c, "body update + penalize");
 }
 
 // ------------------------------------------------- PressureIterator_unif
