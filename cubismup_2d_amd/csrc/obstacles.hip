@@ -779,12 +779,16 @@ static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, in
 #define CUP2D_FORCES_DEFER 1
 #endif
 
-// Swimmers: their midline (k_fish_mid_b) and the list builder run on the
-// main stream before advDiff (1) or first in the side-stream chain (0).
-// Beside advDiff these latency-bound single-workgroup kernels run 2-3x slower
-// than alone, which costs the chain more than running them alone first.
-#ifndef CUP2D_FISH_HEAD_MAIN
-#define CUP2D_FISH_HEAD_MAIN 1
+// How much of the placement chain runs on the main stream before advDiff
+// (stages of enqueue_put_objects: midline + list builder, PutFishOnBlocks,
+// placement, moments, removeMoments, commit), with and without swimmers.
+// Beside advDiff, which saturates HBM, these latency-bound kernels run 2-3x
+// slower than alone; the split balances that against the main stream's wait.
+#ifndef CUP2D_SPLIT_FISH
+#define CUP2D_SPLIT_FISH 1
+#endif
+#ifndef CUP2D_SPLIT_DISKS
+#define CUP2D_SPLIT_DISKS 0
 #endif
 
 int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
@@ -805,42 +809,53 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
                        c->ob, c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg,
                        c->ramp, compute_dt, c->host_step, fish ? 1 : 0, end_prev);
   }
-  hipStream_t s = c->stream;
-  if (fish && (!overlap || CUP2D_FISH_HEAD_MAIN)) {
-    RC_OB(enqueue_fish_midline(c, c->stream));
-    hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                       c->ob, c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg,
-                       c->ramp, 0, c->host_step, 2, 0);
-  }
-  if (overlap) {
-    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_begin, c->stream));
-    CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_begin, 0));
-    s = c->aux;
-  }
-  if (fish) {
-    if (overlap && !CUP2D_FISH_HEAD_MAIN) {
-      RC_OB(enqueue_fish_midline(c, s));
-      hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
-                         c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp,
-                         0, c->host_step, 2, 0);
+  // The chain's stages in order; the first `split` of them run on the main
+  // stream before advDiff, the rest on the side stream beside it.
+  enum { ST_MIDLINE, ST_FISH_PLACE, ST_PLACE, ST_MOMENTS, ST_UDEF, ST_COMMIT, ST_N };
+  const int split = !overlap ? ST_N : (fish ? CUP2D_SPLIT_FISH : CUP2D_SPLIT_DISKS);
+  bool on_aux = false;
+  auto stream_for = [&](int stage) -> hipStream_t {
+    if (stage < split) return c->stream;
+    if (!on_aux) {  // first side-stream stage: the side stream joins here
+      on_aux = true;
+      if (hipEventRecord(c->ev_begin, c->stream) != hipSuccess ||
+          hipStreamWaitEvent(c->aux, c->ev_begin, 0) != hipSuccess)
+        return nullptr;
     }
+    return c->aux;
+  };
+  hipStream_t s;
+  if (fish) {
+    if (!(s = stream_for(ST_MIDLINE))) return CUP2D_ERR_HIP;
+    RC_OB(enqueue_fish_midline(c, s));
+    hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
+                       c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp, 0,
+                       c->host_step, 2, 0);
+    if (!(s = stream_for(ST_FISH_PLACE))) return CUP2D_ERR_HIP;
     RC_OB(enqueue_fish_place(c, s));
   }
   if (c->g.nshapes > 0) {
+    if (!(s = stream_for(ST_PLACE))) return CUP2D_ERR_HIP;
     hipLaunchKernelGGL(k_obst_place, dim3(2 * G), dim3(NT), 0, s, c->g, c->sp, c->ss, c->ob,
                        c->ob_prev, c->obmap, c->ubidx, c->chi, c->udx, c->udy, c->invrho);
     // Disks carry no deformation velocity (Disk::create leaves udef = 0), so
     // removeMoments and putObjectVelOnGrid would only add zeros: skipped
-    // unless a shape kind with a deformation velocity is present.  The last
-    // workgroup of the last kernel commits the shapes.
+    // unless a shape kind with a deformation velocity is present.
     bool deforming = false;
     for (const ShapeParams& p : c->hsp) deforming |= p.kind != 0;
+    if (!(s = stream_for(ST_MOMENTS))) return CUP2D_ERR_HIP;
     hipLaunchKernelGGL(k_obst_moments, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob);
-    if (deforming)
+    if (deforming) {
+      if (!(s = stream_for(ST_UDEF))) return CUP2D_ERR_HIP;
       hipLaunchKernelGGL(k_obst_udef, dim3(G), dim3(NT), 0, s, c->g, c->ss, c->st, c->ob, c->chi,
                          c->udx, c->udy);
+    }
+    if (!(s = stream_for(ST_COMMIT))) return CUP2D_ERR_HIP;
     hipLaunchKernelGGL(k_obst_commit, dim3(1), dim3(NT), 0, s, c->g, c->ss, c->ob);
   }
+  // the side stream always joins (the deferred ComputeForces and the next
+  // step's midline are ordered after this step's list builder through it)
+  if (overlap && !stream_for(ST_N)) return CUP2D_ERR_HIP;
   if (overlap) {
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_obst, c->aux));
     c->obst_pending = true;
