@@ -1,0 +1,117 @@
+"""Parity at the benchmark size: 4096x2048 cells (128x64 blocks of 32x32),
+periodic, the workloads bench.py measures.  The oracle runs a few steps of
+this size in about a second, so the whole step is compared directly, and the
+FFT row and column lengths of this size (4096 and 2048) are pinned by closed
+forms."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+import bench  # noqa: E402  (the workload definitions and initial condition)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu case] {request.node.name}", flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _numpy_periodic_solve(r):
+    """FFTW_periodic.h:77-92 restated with numpy's FFT: c^ *= -1/(kx~^2+ky~^2),
+    k~ = 2 pi min(i, N-i)/N (index units), DC = 0."""
+    NY, NX = r.shape
+    R = np.fft.rfft2(r)
+    kx = 2 * np.pi * np.minimum(np.arange(NX // 2 + 1), NX - np.arange(NX // 2 + 1)) / NX
+    ky = 2 * np.pi * np.minimum(np.arange(NY), NY - np.arange(NY)) / NY
+    lam = ky[:, None] ** 2 + kx[None, :] ** 2
+    lam[0, 0] = 1
+    P = -R / lam
+    P[0, 0] = 0
+    return np.fft.irfft2(P, s=(NY, NX))
+
+
+def test_fullsize_poisson_fourier_modes(cup):
+    """cos modes are eigenvectors of the periodic solve (FFTW_periodic.h:
+    77-92): p = -r / (kx~^2 + ky~^2) in index units.  The closed form holds to
+    the round-off of r itself, amplified by lambda / lambda_min through the
+    lowest modes (numpy's FFT on the same r shows the same); the GPU solve
+    matches numpy's to the pressure tolerance, 1e-10 relative L2 (observed
+    <= 1e-12)."""
+    g = cup.Simulation(128, 64, extent=2.0, poisson="periodic")
+    assert (g.NX, g.NY) == (4096, 2048)
+    x, y = np.meshgrid(np.arange(g.NX), np.arange(g.NY))
+    for kx, ky in [(1, 0), (0, 1), (g.NX // 2, g.NY // 2), (1234, 567), (g.NX - 3, 1000)]:
+        # phase reduced exactly in integers, so r is a Fourier mode to round-off
+        # (cos of a large unreduced argument carries a phase error that grows
+        # with x and y, which the solve amplifies through the lowest modes)
+        ph = ((kx * x) % g.NX) * g.NY + ((ky * y) % g.NY) * g.NX
+        r = np.cos(2 * np.pi * ph / (g.NX * g.NY))
+        kxt = 2 * np.pi * min(kx, g.NX - kx) / g.NX
+        kyt = 2 * np.pi * min(ky, g.NY - ky) / g.NY
+        lam = kxt * kxt + kyt * kyt
+        p = g.poisson_dense(r)
+        err = np.abs(p + r / lam).max() * lam
+        dnp = rel(p, _numpy_periodic_solve(r))
+        print(f"  mode ({kx},{ky}) closed form {err:.2e}, vs numpy {dnp:.2e}", flush=True)
+        assert err < 1e-10
+        assert dnp < 1e-10
+
+
+def test_fullsize_advdiff_bitexact(cup, oracle_mod):
+    """advDiff over the full grid (tile numbering, XCD remap, strip loads)
+    is bit-identical to the oracle away from the flux-corrected boundary."""
+    rng = np.random.default_rng(41)
+    kw = dict(extent=2.0, nu=1e-5)
+    g = cup.Simulation(128, 64, **kw)
+    o = oracle_mod.Oracle(128, 64, **kw)
+    vel = 0.3 * rng.standard_normal((2, g.NY, g.NX))
+    for sim in (g, o):
+        sim.set_field(0, vel)
+        sim.set_scalars(uinf=(0.05, -0.02))
+        sim.advdiff(2.5e-5)
+    a, b = g.get_field(cup.VEL), o.get_field(oracle_mod.VEL)
+    inner = (slice(None), slice(1, -1), slice(1, -1))
+    assert np.array_equal(a[inner], b[inner])
+    assert np.abs(a - b).max() < 1e-15
+
+
+@pytest.mark.parametrize("config", ["periodic4096", "periodic4096_disk"])
+def test_fullsize_bench_workload_vs_oracle(cup, oracle_mod, config):
+    """The benchmarked workload itself (same seeded initial condition and
+    projection as bench.py), three whole steps on the device loop against
+    the oracle: velocity and pressure to 1e-10 relative L2, body velocity and
+    position."""
+    cfg = bench.CONFIGS[config]
+    kw = dict(extent=cfg["extent"], nu=cfg["nu"], CFL=0.1, poisson=cfg["poisson"])
+    g = cup.Simulation(cfg["bpdx"], cfg["bpdy"], shapes=[bench.make_shape(cup, cfg["body"])],
+                       **kw)
+    o = oracle_mod.Oracle(cfg["bpdx"], cfg["bpdy"],
+                          shapes=[bench.make_shape(oracle_mod, cfg["body"])], **kw)
+    vel = bench.random_vorticity_velocity(g.NX, g.NY, g.scalars()["h"])
+    bench.setup(g, vel)
+    bench.setup(o, vel)
+    assert rel(g.get_field(0), o.get_field(0)) < 1e-12
+    g.simulate(3)
+    o.simulate(3)
+    sg, so = g.scalars(), o.scalars()
+    assert sg["step"] == so["step"] == 3
+    assert abs(sg["time"] - so["time"]) <= 1e-12 * so["time"]
+    ev = rel(g.get_field(0), o.get_field(0))
+    ep = rel(g.get_field(2), o.get_field(2))
+    print(f"  rel L2 vel {ev:.3e} pres {ep:.3e}", flush=True)
+    assert ev < 1e-10 and ep < 1e-10
+    a, b = g.shape(0), o.shape(0)
+    for key in ["u", "v", "omega"]:
+        assert abs(a[key] - b[key]) <= 1e-10 * max(abs(b[key]), 1e-3), (key, a[key], b[key])
+    assert np.allclose(a["centerOfMass"], b["centerOfMass"], rtol=1e-12, atol=1e-14)
