@@ -759,6 +759,7 @@ static int cols(cup2d_ctx* c, int LC, double2* spec, int ncg, const ColParams& P
     case 512: launch_cols<512, MODE>(c, spec, ncg, P, tw); break;
     case 1024: launch_cols<1024, MODE>(c, spec, ncg, P, tw); break;
     case 2048: launch_cols<2048, MODE>(c, spec, ncg, P, tw); break;
+    case 4096: launch_cols<4096, MODE>(c, spec, ncg, P, tw); break;
     default: return CUP2D_ERR_ARG;
   }
   return CUP2D_OK;
@@ -810,16 +811,17 @@ int fft_create(cup2d_ctx* c) {
   P->Lc = fs || cs ? 2 * NY : NY;
   P->NYs = NY;
   P->ncg = (P->Lr / 2) / G;
-  if (!pow2(NX) || !pow2(NY) || P->Lr > 8192 || P->Lc > 2048) {
+  if (!pow2(NX) || !pow2(NY) || P->Lr > 8192 || P->Lc > 4096) {
     c->err = "FFT Poisson: NX and NY must be powers of two, row length <= 8192 and column "
-             "length <= 2048 (periodic NY <= 2048, free-space and cosine NY <= 1024)";
+             "length <= 4096 (periodic NX <= 8192, NY <= 4096; free-space and cosine "
+             "NX <= 4096, NY <= 2048)";
     return CUP2D_ERR_ARG;
   }
   set_rows_attr_L<32>(); set_rows_attr_L<64>(); set_rows_attr_L<128>(); set_rows_attr_L<256>();
   set_rows_attr_L<512>(); set_rows_attr_L<1024>(); set_rows_attr_L<2048>();
   set_rows_attr_L<4096>(); set_rows_attr_L<8192>();
   set_cols_attr<32>(); set_cols_attr<64>(); set_cols_attr<128>(); set_cols_attr<256>();
-  set_cols_attr<512>(); set_cols_attr<1024>(); set_cols_attr<2048>();
+  set_cols_attr<512>(); set_cols_attr<1024>(); set_cols_attr<2048>(); set_cols_attr<4096>();
 
   CUP2D_HIP_CHECK(c, hipMalloc(&P->spec, sizeof(double2) * (size_t)P->ncg * G * P->NYs));
   const std::vector<double2> twr = twiddles(P->Lr), twc = twiddles(P->Lc);

@@ -115,3 +115,76 @@ def test_fullsize_bench_workload_vs_oracle(cup, oracle_mod, config):
     for key in ["u", "v", "omega"]:
         assert abs(a[key] - b[key]) <= 1e-10 * max(abs(b[key]), 1e-3), (key, a[key], b[key])
     assert np.allclose(a["centerOfMass"], b["centerOfMass"], rtol=1e-12, atol=1e-14)
+
+
+def test_freespace_4096x2048_vs_oracle(cup, oracle_mod):
+    """Free-space solve at 4096x2048 (padded rows of 8192, columns of 4096)
+    against the oracle's 2N x 2M Hockney solve (FFTW_freespace.cpp:78-138)."""
+    rng = np.random.default_rng(19)
+    g = cup.Simulation(128, 64, extent=2.0, poisson="freespace")
+    o = oracle_mod.Oracle(128, 64, extent=2.0, poisson="freespace", fs_pad2=True)
+    r = rng.standard_normal((g.NY, g.NX))
+    e = rel(g.poisson_dense(r), o.poisson_dense(r))
+    print(f"  rel L2 {e:.3e}", flush=True)
+    assert e < 1e-12
+
+
+def mixed_layout(mod, nfish, ndisks, seed=2024):
+    """A seeded layout of the mixed scale-up configuration: swimmers of the
+    benchmark's kind with seeded phases and positions, static disks of radius
+    U[0.01, 0.03] that keep clear of the swimmers and of each other."""
+    rng = np.random.default_rng(seed)
+    fish = []
+    while len(fish) < nfish:
+        x, y = rng.uniform(0.3, 1.7), rng.uniform(0.25, 0.75)
+        if all((x - a) ** 2 + (y - b) ** 2 > 0.3 ** 2 for a, b, _ in fish):
+            fish.append((x, y, rng.uniform(0.0, 1.0)))
+    disks = []
+    while len(disks) < ndisks:
+        r = rng.uniform(0.01, 0.03)
+        x, y = rng.uniform(0.1, 1.9), rng.uniform(0.1, 0.9)
+        if all((x - a) ** 2 + (y - b) ** 2 > (r + 0.16) ** 2 for a, b, _ in fish) and \
+           all((x - a) ** 2 + (y - b) ** 2 > (r + q + 0.02) ** 2 for a, b, q in disks):
+            disks.append((x, y, r))
+    sw = {k: v for k, v in bench.SWIMMER.items() if k not in ("kind", "xpos", "ypos")}
+    shapes = [mod.carlingfish(x, y, phi=phi, **sw) for x, y, phi in fish]
+    shapes += [mod.disk(radius=r, xpos=x, ypos=y, bForced=True) for x, y, r in disks]
+    return shapes
+
+
+def _steps_vs_oracle(cup, oracle_mod, bp, shapes_of, nsteps, **kw):
+    g = cup.Simulation(*bp, shapes=shapes_of(cup), **kw)
+    o = oracle_mod.Oracle(*bp, shapes=shapes_of(oracle_mod), **kw)
+    vel = bench.random_vorticity_velocity(g.NX, g.NY, g.scalars()["h"])
+    bench.setup(g, vel)
+    bench.setup(o, vel)
+    del vel
+    g.simulate(nsteps)
+    o.simulate(nsteps)
+    ev = rel(g.get_field(0), o.get_field(0))
+    ep = rel(g.get_field(2), o.get_field(2))
+    print(f"  {g.NX}x{g.NY}: rel L2 vel {ev:.3e} pres {ep:.3e}", flush=True)
+    assert ev < 1e-10 and ep < 1e-10
+    for k in range(len(shapes_of(cup))):
+        a, b = g.shape(k), o.shape(k)
+        for key in ["u", "v", "omega"]:
+            assert abs(a[key] - b[key]) <= 1e-10 * max(abs(b[key]), 1e-3), (k, key, a[key], b[key])
+        assert np.allclose(a["centerOfMass"], b["centerOfMass"], rtol=1e-12, atol=1e-14), k
+
+
+def test_8192x4096_disks_vs_oracle(cup, oracle_mod):
+    """The scale-up grid (periodic 8192x4096, rows of 8192 and columns of
+    4096 in the FFT) with the sixteen static disks, nu = 5e-6, seeded
+    random-vorticity initial condition; two device-loop steps."""
+    _steps_vs_oracle(cup, oracle_mod, (256, 128), lambda m: mixed_layout(m, 0, 16), 2,
+                     extent=2.0, nu=5e-6, CFL=0.1, poisson="periodic")
+
+
+def test_mixed_swimmers_and_disks_vs_oracle(cup, oracle_mod):
+    """The scale-up body mix (four swimmers with seeded phases and positions,
+    sixteen static disks) at 2048x1024; three device-loop steps.  At
+    8192x4096 the swimmers' candidate blocks exceed the list builder's 2048
+    (it reports CUP2D_ERR_CAPACITY)."""
+    _steps_vs_oracle(cup, oracle_mod, (64, 32), lambda m: mixed_layout(m, 4, 16), 3,
+                     extent=2.0, nu=5e-6, CFL=0.1, poisson="periodic")
+
