@@ -129,6 +129,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
   __shared__ int s_a[LCAP], s_b[LCAP], s_c[LCAP], s_d[LCAP];
   __shared__ int s_tmp[NT];
   __shared__ int s_cstart[MAX_SHAPES + 1];
+  __shared__ int s_first[MAX_SHAPES + 1];  // first slot of each shape
   __shared__ int s_rng[MAX_SHAPES][3];
   __shared__ double s_box[MAX_SHAPES][4];
   __shared__ double lds[LDS_RED];
@@ -283,67 +284,78 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       s_cstart[k] = nc;
       nc += s_tmp[k];
     }
-    if (nc > LCAP) {  // more candidate blocks than the list builder holds
-      st->error = CUP2D_ERR_CAPACITY;
-      for (int k = 0; k < ns; ++k) s_cstart[k] = 0;
-      nc = 0;
-    }
     s_cstart[ns] = nc;
   }
   __syncthreads();
   const int NC = s_cstart[ns];
-  // candidates: s_a = touching flag, s_b = block, s_c = shape
-  for (int cidx = t; cidx < NC; cidx += NT) {
-    int k = 0;
-    while (k + 1 < ns && s_cstart[k + 1] <= cidx) ++k;
-    const int local = cidx - s_cstart[k], w = s_rng[k][2];
-    const int bx = s_rng[k][0] + local % w, by = s_rng[k][1] + local / w;
-    const double bbox[2][2] = {{s_box[k][0], s_box[k][1]}, {s_box[k][2], s_box[k][3]}};
-    s_a[cidx] = sp[k].kind == 0 && touching(g, bx, by, bbox) ? 1 : 0;
-    s_b[cidx] = bx + g.bpdx * by;
-    s_c[cidx] = k;
-  }
-  // swimmers: any of the segments (staged in LDS) meets the block; all
-  // (candidate, segment) pairs in parallel
-  for (int k = 0; k < ns; ++k) {
-    if (sp[k].kind != 1) continue;
-    const FishDev& F = fd[sp[k].fish];
-    const int nseg = F.Nseg;
-    __syncthreads();
-    for (int i = t; i < nseg * FISH_SEGD; i += NT) s_seg[i] = F.seg[i];
-    __syncthreads();
-    const int c0 = s_cstart[k], nck = s_cstart[k + 1] - c0;
-    for (int pr = t; pr < nck * nseg; pr += NT) {
-      const int cidx = c0 + pr / nseg, i = pr % nseg;
-      const int b = s_b[cidx];
-      double start[2], end[2];
-      block_corners(g, b % g.bpdx, b / g.bpdx, start, end);
-      if (seg_intersects(s_seg + i * FISH_SEGD, start, end)) s_a[cidx] = 1;
+  // The candidates are taken in windows of at most LCAP, in shape and block
+  // order; each window's selected blocks are appended to the slot list.
+  const int cap = ob.cap < LCAP ? ob.cap : LCAP;
+  int nob = 0;  // slots so far (the same in every thread)
+  bool over = false;
+  for (int w0 = 0; w0 < NC; w0 += LCAP) {
+    const int nw = NC - w0 < LCAP ? NC - w0 : LCAP;
+    // candidates: s_a = touching flag, s_b = block, s_c = shape
+    for (int c = t; c < nw; c += NT) {
+      const int cidx = w0 + c;
+      int k = 0;
+      while (k + 1 < ns && s_cstart[k + 1] <= cidx) ++k;
+      const int local = cidx - s_cstart[k], w = s_rng[k][2];
+      const int bx = s_rng[k][0] + local % w, by = s_rng[k][1] + local / w;
+      const double bbox[2][2] = {{s_box[k][0], s_box[k][1]}, {s_box[k][2], s_box[k][3]}};
+      s_a[c] = sp[k].kind == 0 && touching(g, bx, by, bbox) ? 1 : 0;
+      s_b[c] = bx + g.bpdx * by;
+      s_c[c] = k;
     }
+    // swimmers: any of the segments (staged in LDS) meets the block; all
+    // (candidate, segment) pairs of the window in parallel
+    for (int k = 0; k < ns; ++k) {
+      if (sp[k].kind != 1) continue;
+      const int c0 = s_cstart[k] > w0 ? s_cstart[k] - w0 : 0;
+      const int c1 = s_cstart[k + 1] - w0 < nw ? s_cstart[k + 1] - w0 : nw;
+      if (c1 <= c0) continue;
+      const FishDev& F = fd[sp[k].fish];
+      const int nseg = F.Nseg;
+      __syncthreads();
+      for (int i = t; i < nseg * FISH_SEGD; i += NT) s_seg[i] = F.seg[i];
+      __syncthreads();
+      const int nck = c1 - c0;
+      for (int pr = t; pr < nck * nseg; pr += NT) {
+        const int c = c0 + pr / nseg, i = pr % nseg;
+        const int b = s_b[c];
+        double start[2], end[2];
+        block_corners(g, b % g.bpdx, b / g.bpdx, start, end);
+        if (seg_intersects(s_seg + i * FISH_SEGD, start, end)) s_a[c] = 1;
+      }
+    }
+    __syncthreads();  // the window's flags are complete
+    const int nsel = block_exscan(s_a, nw, s_tmp);  // s_a now holds exclusive prefix
+    // each shape's first slot: the slots before its first candidate
+    if (t <= ns) {
+      const int cs = s_cstart[t];
+      if (cs >= w0 && cs < w0 + nw) s_first[t] = nob + s_a[cs - w0];
+    }
+    if (nob + nsel > cap) over = true;
+    // compacted slots (s_d = block of slot)
+    for (int c = t; c < nw; c += NT) {
+      const int pos = s_a[c];
+      const int next = c + 1 < nw ? s_a[c + 1] : nsel;
+      if (next > pos && nob + pos < cap) {
+        ob.slot_shape[nob + pos] = s_c[c];
+        ob.slot_block[nob + pos] = s_b[c];
+        s_d[nob + pos] = s_b[c];
+      }
+    }
+    nob += nsel;
+    __syncthreads();  // s_a, s_b, s_c are refilled by the next window
   }
-  __syncthreads();
-  // slot_of_shape_start: flags before each shape's first candidate
-  if (t <= ns) s_tmp[t] = 0;
-  __syncthreads();
-  int nob = block_exscan(s_a, NC, s_tmp);  // s_a now holds exclusive prefix
-  if (nob > ob.cap) {
+  if (over) {  // more obstacle blocks than the slot lists hold
     if (t == 0) st->error = CUP2D_ERR_CAPACITY;
     nob = 0;
+    if (t <= ns) s_first[t] = 0;
+    __syncthreads();
   }
-  // compacted slots (s_d = block of slot)
-  for (int cidx = t; cidx < NC; cidx += NT) {
-    const int pos = s_a[cidx];
-    const int next = cidx + 1 < NC ? s_a[cidx + 1] : nob;
-    if (next > pos && pos < nob) {
-      ob.slot_shape[pos] = s_c[cidx];
-      ob.slot_block[pos] = s_b[cidx];
-      s_d[pos] = s_b[cidx];
-    }
-  }
-  if (t <= ns) {
-    const int cs = s_cstart[t];
-    ob.slot_of_shape_start[t] = (t == ns || cs >= NC) ? nob : (nob ? s_a[cs] : 0);
-  }
+  if (t <= ns) ob.slot_of_shape_start[t] = s_cstart[t] >= NC ? nob : s_first[t];
   __syncthreads();
   // distinct blocks: s_a = first-occurrence flag, s_b = rank among same-block
   // slots, s_c = first slot with this block, s_d keeps the block
@@ -1694,10 +1706,10 @@ int obstacles_alloc(cup2d_ctx* c) {
   cap = cap > nblocks * (int)(c->hsp.size() ? c->hsp.size() : 1)
             ? nblocks * (int)(c->hsp.size() ? c->hsp.size() : 1)
             : cap;
-  if (cap > OBST_LIST_CAP) {
-    c->err = "too many obstacle blocks for the device list builder";
-    return CUP2D_ERR_CAPACITY;
-  }
+  // the worst case (every shape's bound fully covered) can exceed what the
+  // list builder holds while the actual obstacle blocks fit: the list
+  // builder reports CUP2D_ERR_CAPACITY when they do not
+  cap = cap > OBST_LIST_CAP ? OBST_LIST_CAP : cap;
   int rc = obstacles_alloc_one(c, c->ob, cap);
   if (rc) return rc;
   return obstacles_alloc_one(c, c->ob_prev, cap);
