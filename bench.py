@@ -49,6 +49,9 @@ CONFIGS = {
     "cylinder512": dict(bpdx=16, bpdy=8, extent=1.0, nu=2e-5, poisson="freespace",
                         body=dict(radius=0.05, xpos=0.25, ypos=0.25, xvel=0.2, bFixed=True,
                                   bForced=True)),
+    # the mixed scale-up configuration in FP64: four swimmers, sixteen static disks
+    "scaleup8192": dict(bpdx=256, bpdy=128, extent=2.0, nu=5e-6, poisson="periodic",
+                        bodies=(4, 16)),
 }
 
 
@@ -85,6 +88,43 @@ def make_shape(mod, body):
     return mod.disk(**b)
 
 
+def mixed_layout(mod, nfish, ndisks, seed=2024):
+    """A seeded layout of the mixed scale-up configuration: swimmers of the
+    benchmark's kind with seeded phases and positions, static disks of radius
+    U[0.01, 0.03] that keep clear of the swimmers and of each other."""
+    rng = np.random.default_rng(seed)
+    fish = []
+    while len(fish) < nfish:
+        x, y = rng.uniform(0.3, 1.7), rng.uniform(0.25, 0.75)
+        if all((x - a) ** 2 + (y - b) ** 2 > 0.3 ** 2 for a, b, _ in fish):
+            fish.append((x, y, rng.uniform(0.0, 1.0)))
+    disks = []
+    while len(disks) < ndisks:
+        r = rng.uniform(0.01, 0.03)
+        x, y = rng.uniform(0.1, 1.9), rng.uniform(0.1, 0.9)
+        if all((x - a) ** 2 + (y - b) ** 2 > (r + 0.16) ** 2 for a, b, _ in fish) and \
+           all((x - a) ** 2 + (y - b) ** 2 > (r + q + 0.02) ** 2 for a, b, q in disks):
+            disks.append((x, y, r))
+    sw = {k: v for k, v in SWIMMER.items() if k not in ("kind", "xpos", "ypos")}
+    shapes = [mod.carlingfish(x, y, phi=phi, **sw) for x, y, phi in fish]
+    shapes += [mod.disk(radius=r, xpos=x, ypos=y, bForced=True) for x, y, r in disks]
+    return shapes
+
+
+def make_shapes(mod, cfg):
+    """Shape descriptors of a workload for the library or the oracle."""
+    if "bodies" in cfg:
+        return mixed_layout(mod, *cfg["bodies"])
+    return [make_shape(mod, cfg["body"])]
+
+
+def bodies_text(cfg):
+    if "bodies" in cfg:
+        return (f"{cfg['bodies'][0]} free carlingfish of the default kind and "
+                f"{cfg['bodies'][1]} static disks r=U[0.01,0.03], seeded layout")
+    return body_text(cfg["body"])
+
+
 def body_text(body):
     if body.get("kind") == "carlingfish":
         return (f"one free carlingfish L={body['L']} T={body['T']}, envelope "
@@ -97,7 +137,7 @@ def cpu_baseline(cfg, vel, steps):
     import oracle
     o = oracle.Oracle(cfg["bpdx"], cfg["bpdy"], extent=cfg["extent"], nu=cfg["nu"], CFL=0.1,
                       poisson=cfg["poisson"], fs_pad2=True,
-                      shapes=[make_shape(oracle, cfg["body"])])
+                      shapes=make_shapes(oracle, cfg))
     setup(o, vel)
     o.simulate(2)  # warm-up
     t0 = time.perf_counter()
@@ -213,7 +253,7 @@ def main():
     import cubismup_2d_amd as cup
     cfg = CONFIGS[args.config]
     sim = cup.Simulation(cfg["bpdx"], cfg["bpdy"], extent=cfg["extent"], nu=cfg["nu"], CFL=0.1,
-                         poisson=cfg["poisson"], shapes=[make_shape(cup, cfg["body"])])
+                         poisson=cfg["poisson"], shapes=make_shapes(cup, cfg))
     NX, NY = sim.NX, sim.NY
     h = sim.scalars()["h"]
     vel = random_vorticity_velocity(NX, NY, h)
@@ -267,7 +307,7 @@ def main():
         "data": "synthetic: seeded random-vorticity initial velocity",
         "config": {"workload": f"{args.config}: {NX}x{NY} cells ({cfg['bpdx']}x{cfg['bpdy']} "
                                f"blocks), {cfg['poisson']} FFT Poisson, nu={cfg['nu']}, CFL 0.1, "
-                               f"{body_text(cfg['body'])}",
+                               f"{bodies_text(cfg)}",
                    "cells": cells, "precision": "fp64", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),

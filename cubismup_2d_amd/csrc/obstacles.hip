@@ -1043,7 +1043,8 @@ __device__ inline void shape_velocity(const ShapeParams& P, const double S[7], d
 __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
                               cup2d_shape_state* __restrict__ ss,
                               StepState* __restrict__ st, const Obstacles& ob,
-                              ForceSnap* __restrict__ snap, int collide) {
+                              const int* __restrict__ ubidx, ForceSnap* __restrict__ snap,
+                              int collide) {
   __shared__ double lds[LDS_RED];
   __shared__ double coll[MAX_SHAPES][10];
   const double dt = st->dt, time = st->time;
@@ -1062,29 +1063,48 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
   }
   if (ns < 2 || !collide) return;
   __syncthreads();
-  // collision sums: coll[i] accumulates over every j != i (PressureSingle.cpp:255-298)
+  // collision sums: coll[i] accumulates over every j != i (PressureSingle.cpp:255-298),
+  // over the blocks the two share.  No block holding two slots means no
+  // overlap at all; otherwise the slots of shape i on a shared block are
+  // found NT at a time (a ballot per wave) and each one finds the other
+  // shapes' slots of its block through the distinct-block lists.
+  if (ob.cnt[0] == ob.cnt[1]) return;
+  __shared__ unsigned long long s_mask[NT / 64];
   for (int i = 0; i < ns; ++i) {
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int j = 0; j < ns; ++j) {
-      if (i == j) continue;
-      const cup2d_shape_state &si = ss[i], &sj = ss[j];
-      for (int a = ob.slot_of_shape_start[i]; a < ob.slot_of_shape_start[i + 1]; ++a)
-        for (int bsl = ob.slot_of_shape_start[j]; bsl < ob.slot_of_shape_start[j + 1]; ++bsl) {
-          if (ob.slot_block[a] != ob.slot_block[bsl]) continue;
+    const cup2d_shape_state& si = ss[i];
+    const int a0 = ob.slot_of_shape_start[i], a1 = ob.slot_of_shape_start[i + 1];
+    for (int base = a0; base < a1; base += NT) {
+      const int mine = base + threadIdx.x;
+      const bool shared = mine < a1 && ob.ub_count[ubidx[ob.slot_block[mine]]] > 1;
+      const unsigned long long m = __ballot(shared);
+      __syncthreads();  // the previous chunk's masks are read
+      if (threadIdx.x % 64 == 0) s_mask[threadIdx.x / 64] = m;
+      __syncthreads();
+      for (int w = 0; w < NT / 64; ++w)
+        for (unsigned long long mw = s_mask[w]; mw != 0; mw &= mw - 1) {
+          const int a = base + 64 * w + __builtin_ctzll(mw);
           const int blk = ob.slot_block[a], bx = blk % g.bpdx, by = blk / g.bpdx;
-          SlotView oi = slot(ob, a), oj = slot(ob, bsl);
-          CELL_LOOP(c) {
-            if (oi.chi[c] <= 0 || oj.chi[c] <= 0) continue;
-            const double p0 = cell_pos(bx, c % BS, g.hb, g.h), p1 = cell_pos(by, c / BS, g.hb, g.h);
-            const double iUr = -si.omega * (p1 - si.centerOfMass[1]), iVr = si.omega * (p0 - si.centerOfMass[0]);
-            const double jUr = -sj.omega * (p1 - sj.centerOfMass[1]), jVr = sj.omega * (p0 - sj.centerOfMass[0]);
-            const double X = oi.chi[c], Y = oj.chi[c];
-            acc[0] += X; acc[1] += X * p0; acc[2] += X * p1;
-            acc[3] += X * (si.u + iUr + oi.udx[c]);
-            acc[4] += X * (si.v + iVr + oi.udy[c]);
-            acc[5] += Y; acc[6] += Y * p0; acc[7] += Y * p1;
-            acc[8] += Y * (sj.u + jUr + oj.udx[c]);
-            acc[9] += Y * (sj.v + jVr + oj.udy[c]);
+          const int u = ubidx[blk], q0 = ob.ub_start[u], nq = ob.ub_count[u];
+          SlotView oi = slot(ob, a);
+          for (int q = 0; q < nq; ++q) {
+            const int bsl = ob.ub_slots[q0 + q], j = ob.slot_shape[bsl];
+            if (j == i) continue;
+            const cup2d_shape_state& sj = ss[j];
+            SlotView oj = slot(ob, bsl);
+            CELL_LOOP(c) {
+              if (oi.chi[c] <= 0 || oj.chi[c] <= 0) continue;
+              const double p0 = cell_pos(bx, c % BS, g.hb, g.h), p1 = cell_pos(by, c / BS, g.hb, g.h);
+              const double iUr = -si.omega * (p1 - si.centerOfMass[1]), iVr = si.omega * (p0 - si.centerOfMass[0]);
+              const double jUr = -sj.omega * (p1 - sj.centerOfMass[1]), jVr = sj.omega * (p0 - sj.centerOfMass[0]);
+              const double X = oi.chi[c], Y = oj.chi[c];
+              acc[0] += X; acc[1] += X * p0; acc[2] += X * p1;
+              acc[3] += X * (si.u + iUr + oi.udx[c]);
+              acc[4] += X * (si.v + iVr + oi.udy[c]);
+              acc[5] += Y; acc[6] += Y * p0; acc[7] += Y * p1;
+              acc[8] += Y * (sj.u + jUr + oj.udx[c]);
+              acc[9] += Y * (sj.v + jVr + oj.udy[c]);
+            }
           }
         }
     }
@@ -1170,8 +1190,9 @@ __device__ void momenta_wg(const Grid& g, const cup2d_shape_state* __restrict__ 
 // collide: also preventCollidingObstacles (PressureSingle only).
 __device__ void body_update_tail(const Grid& g, const ShapeParams* __restrict__ sp,
                                  cup2d_shape_state* __restrict__ ss, StepState* __restrict__ st,
-                                 const Obstacles& ob, ForceSnap* __restrict__ snap, int collide) {
-  body_solve_wg(g, sp, ss, st, ob, snap, collide);
+                                 const Obstacles& ob, const int* __restrict__ ubidx,
+                                 ForceSnap* __restrict__ snap, int collide) {
+  body_solve_wg(g, sp, ss, st, ob, ubidx, snap, collide);
   __syncthreads();
   // state ComputeForces uses (it may run after the next step moved the bodies)
   if (threadIdx.x < g.nshapes) {
@@ -1187,8 +1208,9 @@ __device__ void body_update_tail(const Grid& g, const ShapeParams* __restrict__ 
 __global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __restrict__ sp,
                                                    cup2d_shape_state* __restrict__ ss,
                                                    StepState* __restrict__ st, Obstacles ob,
+                                                   const int* __restrict__ ubidx,
                                                    ForceSnap* __restrict__ snap, int collide) {
-  body_update_tail(g, sp, ss, st, ob, snap, collide);
+  body_update_tail(g, sp, ss, st, ob, ubidx, snap, collide);
 }
 
 // PressureSingle::integrateMomenta (PressureSingle.cpp:23-67) per slot.
@@ -1208,7 +1230,7 @@ int enqueue_body_update(cup2d_ctx* c) {
   hipLaunchKernelGGL(k_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st, c->ob,
                      c->u, c->v);
   hipLaunchKernelGGL(k_body_solve, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                     c->ob, c->fsnap, 1);
+                     c->ob, c->ubidx, c->fsnap, 1);
   return launch_check(c, "body_update");
 }
 
@@ -1389,7 +1411,7 @@ int enqueue_iter_body(cup2d_ctx* c) {
   hipLaunchKernelGGL(k_iter_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->ob, c->vfu,
                      c->vfv);
   hipLaunchKernelGGL(k_body_solve, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                     c->ob, c->fsnap, 0);
+                     c->ob, c->ubidx, c->fsnap, 0);
   return launch_check(c, "iterator body update");
 }
 

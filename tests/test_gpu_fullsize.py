@@ -129,29 +129,6 @@ def test_freespace_4096x2048_vs_oracle(cup, oracle_mod):
     assert e < 1e-12
 
 
-def mixed_layout(mod, nfish, ndisks, seed=2024):
-    """A seeded layout of the mixed scale-up configuration: swimmers of the
-    benchmark's kind with seeded phases and positions, static disks of radius
-    U[0.01, 0.03] that keep clear of the swimmers and of each other."""
-    rng = np.random.default_rng(seed)
-    fish = []
-    while len(fish) < nfish:
-        x, y = rng.uniform(0.3, 1.7), rng.uniform(0.25, 0.75)
-        if all((x - a) ** 2 + (y - b) ** 2 > 0.3 ** 2 for a, b, _ in fish):
-            fish.append((x, y, rng.uniform(0.0, 1.0)))
-    disks = []
-    while len(disks) < ndisks:
-        r = rng.uniform(0.01, 0.03)
-        x, y = rng.uniform(0.1, 1.9), rng.uniform(0.1, 0.9)
-        if all((x - a) ** 2 + (y - b) ** 2 > (r + 0.16) ** 2 for a, b, _ in fish) and \
-           all((x - a) ** 2 + (y - b) ** 2 > (r + q + 0.02) ** 2 for a, b, q in disks):
-            disks.append((x, y, r))
-    sw = {k: v for k, v in bench.SWIMMER.items() if k not in ("kind", "xpos", "ypos")}
-    shapes = [mod.carlingfish(x, y, phi=phi, **sw) for x, y, phi in fish]
-    shapes += [mod.disk(radius=r, xpos=x, ypos=y, bForced=True) for x, y, r in disks]
-    return shapes
-
-
 def _steps_vs_oracle(cup, oracle_mod, bp, shapes_of, nsteps, **kw):
     g = cup.Simulation(*bp, shapes=shapes_of(cup), **kw)
     o = oracle_mod.Oracle(*bp, shapes=shapes_of(oracle_mod), **kw)
@@ -176,7 +153,7 @@ def test_8192x4096_disks_vs_oracle(cup, oracle_mod):
     """The scale-up grid (periodic 8192x4096, rows of 8192 and columns of
     4096 in the FFT) with the sixteen static disks, nu = 5e-6, seeded
     random-vorticity initial condition; two device-loop steps."""
-    _steps_vs_oracle(cup, oracle_mod, (256, 128), lambda m: mixed_layout(m, 0, 16), 2,
+    _steps_vs_oracle(cup, oracle_mod, (256, 128), lambda m: bench.mixed_layout(m, 0, 16), 2,
                      extent=2.0, nu=5e-6, CFL=0.1, poisson="periodic")
 
 
@@ -186,5 +163,5 @@ def test_8192x4096_swimmers_and_disks_vs_oracle(cup, oracle_mod):
     nu = 5e-6; two device-loop steps.  The swimmers' bounds (disks of radius
     L + amplitude + half-width) cover more blocks than the list builder holds
     at once, so its candidates are taken in windows."""
-    _steps_vs_oracle(cup, oracle_mod, (256, 128), lambda m: mixed_layout(m, 4, 16), 2,
+    _steps_vs_oracle(cup, oracle_mod, (256, 128), lambda m: bench.mixed_layout(m, 4, 16), 2,
                      extent=2.0, nu=5e-6, CFL=0.1, poisson="periodic")
