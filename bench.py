@@ -232,8 +232,9 @@ def main():
     ap.add_argument("--config", default="periodic4096", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-steps", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--timer-every", type=int, default=1, metavar="K",
-                    help="time the dominant kernel on every K-th step of the timed region")
+    ap.add_argument("--timer-every", type=int, default=8, metavar="K",
+                    help="time the dominant kernel on every K-th step of the timed region "
+                         "(each timed launch costs the stream about 5 us)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain MIB-MiB device copy first (counter calibration)")
     ap.add_argument("--traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_MIB"),

@@ -185,24 +185,20 @@ __device__ inline void block_sum_multi(double* a, double* lds) {
   }
 }
 
-// Column sums of an n x NQ table (row i at a + i*stride): thread t adds rows
-// t, t+NT, ... in order, then block_sum_multi.  Loads are issued U rows at a
-// time ahead of the additions, so a table of up to U*NT rows costs one memory
-// latency; the order of the additions, and so the result, is the same in
-// every workgroup.
-template <int NT, int NQ, int U = 4>
-__device__ inline void block_sum_table(const double* a, int n, int stride, double* out,
-                                       double* lds) {
-  double s[NQ];
+// Partial column sums of an n x NQ table (row i at a + i*stride) over the
+// rows i0, i0+NL, ... in order.  Loads are issued U rows at a time ahead of
+// the additions, so up to U*NL rows cost one memory latency.
+template <int NL, int NQ, int U>
+__device__ inline void strided_table_sum(const double* a, int n, int stride, int i0, double* s) {
 #pragma unroll
   for (int j = 0; j < NQ; ++j) s[j] = 0;
-  int i = threadIdx.x;
-  for (; i + (U - 1) * NT < n; i += U * NT) {
+  int i = i0;
+  for (; i + (U - 1) * NL < n; i += U * NL) {
     double v[U][NQ];
 #pragma unroll
     for (int r = 0; r < U; ++r)
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) v[r][j] = a[(size_t)(i + r * NT) * stride + j];
+      for (int j = 0; j < NQ; ++j) v[r][j] = a[(size_t)(i + r * NL) * stride + j];
 #pragma unroll
     for (int r = 0; r < U; ++r)
 #pragma unroll
@@ -214,16 +210,43 @@ __device__ inline void block_sum_table(const double* a, int n, int stride, doubl
     for (int r = 0; r < U; ++r)
 #pragma unroll
       for (int j = 0; j < NQ; ++j)
-        v[r][j] = i + r * NT < n ? a[(size_t)(i + r * NT) * stride + j] : 0.0;
+        v[r][j] = i + r * NL < n ? a[(size_t)(i + r * NL) * stride + j] : 0.0;
 #pragma unroll
     for (int r = 0; r < U; ++r)
-      if (i + r * NT < n)
+      if (i + r * NL < n)
 #pragma unroll
         for (int j = 0; j < NQ; ++j) s[j] += v[r][j];
   }
+}
+
+// Column sums of an n x NQ table (row i at a + i*stride): thread t adds rows
+// t, t+NT, ... in order, then block_sum_multi.  A table of up to U*NT rows
+// costs one memory latency; the order of the additions, and so the result,
+// is the same in every workgroup.
+template <int NT, int NQ, int U = 4>
+__device__ inline void block_sum_table(const double* a, int n, int stride, double* out,
+                                       double* lds) {
+  double s[NQ];
+  strided_table_sum<NT, NQ, U>(a, n, stride, threadIdx.x, s);
   block_sum_multi<NT, NQ>(s, lds);
 #pragma unroll
   for (int j = 0; j < NQ; ++j) out[j] = s[j];
+}
+
+// The same column sums by one wave (lane l adds rows l, l+64, ..., then a
+// butterfly): no barrier, so the waves of a workgroup can reduce different
+// tables at once.  Every lane gets the totals; the result is the same in
+// every workgroup (it is not bit-identical to block_sum_table's).
+template <int NQ, int U = 4>
+__device__ inline void wave_sum_table(const double* a, int n, int stride, double* out) {
+  double s[NQ];
+  strided_table_sum<64, NQ, U>(a, n, stride, threadIdx.x % 64, s);
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s[j] += __shfl_xor(s[j], o, 64);
+    out[j] = s[j];
+  }
 }
 
 // Sum of n values (fixed order independent of the caller): thread t sums

@@ -850,14 +850,14 @@ void fish_free_all(cup2d_ctx* c) {
 int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s) {
   if (c->hfish.empty()) return CUP2D_OK;
   const int buf = c->host_step & 1;
-  // one launch per swimmer: the LDS size depends on its number of points
-  for (size_t f = 0; f < c->hfish.size(); ++f) {
-    const size_t lds = fish_midline_lds(c->hfish[f].Nm);
-    if (!c->midA_ready)
-      hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), 0, s, c->fishd + f, c->st, 0, 0, buf);
-    hipLaunchKernelGGL(k_fish_mid_b, dim3(1), dim3(NT), lds, s, c->fishd + f, c->ss, c->st,
-                       c->g.h, buf);
-  }
+  // one workgroup per swimmer, LDS sized for the largest number of points
+  const int nf = (int)c->hfish.size();
+  size_t lds = 0;
+  for (const auto& F : c->hfish) lds = fish_midline_lds(F.Nm) > lds ? fish_midline_lds(F.Nm) : lds;
+  if (!c->midA_ready)
+    hipLaunchKernelGGL(k_fish_mid_a, dim3(nf), dim3(NT), 0, s, c->fishd, c->st, 0, 0, buf);
+  hipLaunchKernelGGL(k_fish_mid_b, dim3(nf), dim3(NT), lds, s, c->fishd, c->ss, c->st, c->g.h,
+                     buf);
   c->midA_ready = false;
   return launch_check(c, "fish midline");
 }
@@ -870,9 +870,8 @@ int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s) {
 int enqueue_fish_mid_next(cup2d_ctx* c, int parity) {
   if (c->hfish.empty()) return CUP2D_OK;
   CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->mids, c->ev_begin, 0));
-  for (size_t f = 0; f < c->hfish.size(); ++f)
-    hipLaunchKernelGGL(k_fish_mid_a, dim3(1), dim3(NT), 0, c->mids, c->fishd + f, c->st, 1,
-                       parity, parity ^ 1);
+  hipLaunchKernelGGL(k_fish_mid_a, dim3((int)c->hfish.size()), dim3(NT), 0, c->mids, c->fishd,
+                     c->st, 1, parity, parity ^ 1);
   CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_mid, c->mids));
   CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_mid, 0));
   c->midA_ready = true;

@@ -355,32 +355,45 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
     if (t <= ns) s_first[t] = 0;
     __syncthreads();
   }
-  if (t <= ns) ob.slot_of_shape_start[t] = s_cstart[t] >= NC ? nob : s_first[t];
+  if (t <= ns) {
+    const int v = s_cstart[t] >= NC ? nob : s_first[t];
+    ob.slot_of_shape_start[t] = v;
+    s_first[t] = v;
+  }
   __syncthreads();
-  // distinct blocks: s_a = first-occurrence flag, s_b = rank among same-block
-  // slots, s_c = first slot with this block, s_d keeps the block
+  // distinct blocks.  Each shape's slots are in increasing block order, so the
+  // other shapes holding the block of slot s are found by a range test and a
+  // binary search in their runs.  s_a = first-occurrence flag, s_b = rank
+  // among the block's slots (shape order), s_c = the block's first slot,
+  // s_cnt = the block's slot count (at first occurrences), s_d keeps the block.
+  __shared__ int s_cnt[LCAP];
+  __shared__ int s_n[LCAP];
   for (int s = t; s < nob; s += NT) {
     const int b = s_d[s];
-    int first = s, rank = 0;
-    for (int q = 0; q < s; ++q)
-      if (s_d[q] == b) { if (rank == 0) first = q; ++rank; }
+    int first = s, rank = 0, cnt = 1;
+    for (int k = 0; k < ns; ++k) {
+      int lo = s_first[k], hi = s_first[k + 1];
+      if (lo >= hi || (s >= lo && s < hi)) continue;  // no slots, or the own shape
+      if (b < s_d[lo] || b > s_d[hi - 1]) continue;
+      while (hi - lo > 1) {  // last slot of the run with block <= b
+        const int mid = (lo + hi) >> 1;
+        if (s_d[mid] <= b) lo = mid; else hi = mid;
+      }
+      if (s_d[lo] != b) continue;
+      ++cnt;
+      if (lo < s) {
+        if (rank == 0) first = lo;
+        ++rank;
+      }
+    }
     s_a[s] = rank == 0 ? 1 : 0;
     s_b[s] = rank;
     s_c[s] = first;
+    s_cnt[s] = rank == 0 ? cnt : 0;
+    s_n[s] = cnt;
   }
   __syncthreads();
-  const int nub = block_exscan(s_a, nob, s_tmp);  // s_a: ub index of first occurrences
-  // slot counts per distinct block, stored at the first occurrence
-  __shared__ int s_cnt[LCAP];
-  for (int s = t; s < nob; s += NT) s_cnt[s] = 0;
-  __syncthreads();
-  for (int s = t; s < nob; s += NT)
-    if (s_c[s] == s) {
-      int cnt = 0;
-      for (int q = s; q < nob; ++q) cnt += s_d[q] == s_d[s];
-      s_cnt[s] = cnt;
-    }
-  __syncthreads();
+  const int nub = block_exscan(s_a, nob, s_tmp);    // s_a: ub index of first occurrences
   block_exscan(s_cnt, nob, s_tmp);  // s_cnt: ub_start at first occurrences
   for (int s = t; s < nob; s += NT) {
     const int f = s_c[s];
@@ -389,35 +402,26 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
     if (f == s) {
       ob.ub_block[u] = s_d[s];
       ob.ub_start[u] = s_cnt[s];
+      ob.ub_count[u] = s_n[s];
       obmap[s_d[s]] = 1;
       ubidx[s_d[s]] = u;
     }
   }
-  __syncthreads();
-  for (int s = t; s < nob; s += NT)
-    if (s_c[s] == s) {
-      int cnt = 0;
-      for (int q = s; q < nob; ++q) cnt += s_d[q] == s_d[s];
-      ob.ub_count[s_a[s]] = cnt;
-    }
   if (t == 0) {
     ob.cnt[0] = nob;
     ob.cnt[1] = nub;
   }
 }
 
-// total over the slots of shape k of partial q (fixed order, identical in
-// every workgroup)
-__device__ inline double shape_sum(const Obstacles& ob, int k, int q, double* lds) {
-  const int s0 = ob.slot_of_shape_start[k], n = ob.slot_of_shape_start[k + 1] - s0;
-  return block_sum_array<NT>(ob.part + (size_t)s0 * SLOT_PART + q, n, SLOT_PART, lds);
-}
-
-// totals of NQ consecutive partials q0.. over the slots of shape k, in one pass
+// Totals of NQ consecutive partials q0.. over the slots of shape k, reduced
+// by each wave on its own (every lane gets the totals; no barrier).  The
+// order of the additions is fixed, so every wave of every workgroup gets
+// bit-identical totals, and the waves of one workgroup can take different
+// shapes at once.
 template <int NQ>
-__device__ inline void shape_sums(const Obstacles& ob, int k, int q0, double* out, double* lds) {
+__device__ inline void shape_sums(const Obstacles& ob, int k, int q0, double* out) {
   const int s0 = ob.slot_of_shape_start[k], n = ob.slot_of_shape_start[k + 1] - s0;
-  block_sum_table<NT, NQ>(ob.part + (size_t)s0 * SLOT_PART + q0, n, SLOT_PART, out, lds);
+  wave_sum_table<NQ>(ob.part + (size_t)s0 * SLOT_PART + q0, n, SLOT_PART, out);
 }
 
 // PutObjectsOnGrid.cpp:191-197 (chi=0, uDef=0, invRho=1) for one block
@@ -652,9 +656,9 @@ __device__ inline void com_from_sums(const cup2d_shape_state* ss, int k, const d
 }
 
 __device__ inline void shape_com(const Obstacles& ob, const cup2d_shape_state* ss, int k,
-                                 double* lds, double C[2], double* mass, bool* shifted) {
+                                 double C[2], double* mass, bool* shifted) {
   double t[3];
-  shape_sums<3>(ob, k, P_COM, t, lds);
+  shape_sums<3>(ob, k, P_COM, t);
   com_from_sums(ss, k, t, C, mass, shifted);
 }
 
@@ -679,7 +683,7 @@ __device__ void obst_moments_wg(const Grid& g, const cup2d_shape_state* __restri
   const double hsq = g.h * g.h;
   double C[2], mass;
   bool shifted;
-  shape_com(ob, ss, k, lds, C, &mass, &shifted);
+  shape_com(ob, ss, k, C, &mass, &shifted);
   double a[7] = {0, 0, 0, 0, 0, 0, 0};
   CELL_LOOP(i) {
     if (o.chi[i] <= 0) continue;
@@ -708,7 +712,6 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __r
                                                   double* __restrict__ udx,
                                                   double* __restrict__ udy) {
   chain_prio();
-  __shared__ double lds[LDS_RED];
   const int nub = ob.cnt[1];
   if ((int)blockIdx.x < nub) {
     const int b = ob.ub_block[blockIdx.x];
@@ -718,7 +721,7 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __r
       const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
       static_assert(P_COM == 0 && P_MOM == 3, "COM and moment partials are adjacent");
       double t[10];
-      shape_sums<10>(ob, k, P_COM, t, lds);
+      shape_sums<10>(ob, k, P_COM, t);
       const double* I = t + P_MOM;
       const double Iu = I[4] / I[2], Iv = I[5] / I[2], Ia = I[6] / I[3];
       double C[2], mass;
@@ -745,16 +748,16 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __r
 // previous values
 __device__ void obst_commit_wg(const Grid& g, cup2d_shape_state* __restrict__ ss,
                                const Obstacles& ob) {
-  __shared__ double lds[LDS_RED];
-  for (int k = 0; k < g.nshapes; ++k) {
+  for (int k = threadIdx.x / 64; k < g.nshapes; k += NT / 64) {  // a shape per wave
     double C[2], mass;
     bool shifted;
     double t[10];
-    shape_sums<10>(ob, k, P_COM, t, lds);
+    shape_sums<10>(ob, k, P_COM, t);
     com_from_sums(ss, k, t, C, &mass, &shifted);
     const double m = t[P_MOM + 2], j = t[P_MOM + 3];
-    const double nsurf = shape_sum(ob, k, P_NSURF, lds);
-    if (threadIdx.x == 0) {
+    double nsurf;
+    shape_sums<1>(ob, k, P_NSURF, &nsurf);
+    if (threadIdx.x % 64 == 0) {
       cup2d_shape_state& T = ss[k];
       T.centerOfMass[0] = C[0];
       T.centerOfMass[1] = C[1];
@@ -941,7 +944,6 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
 __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __restrict__ st,
                                                   Obstacles ob, const double* __restrict__ chi,
                                                   double* __restrict__ r) {
-  __shared__ double lds[LDS_RED];
   if ((int)blockIdx.x >= ob.cnt[1]) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
@@ -949,7 +951,7 @@ __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __res
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
     double t[3];
-    shape_sums<3>(ob, k, P_RHS, t, lds);
+    shape_sums<3>(ob, k, P_RHS, t);
     const double sum = t[0], pos = t[1], neg = t[2];
     const double corrDenom = sum > 0 ? pos : neg;
     const double corr = sum / dmax(corrDenom, EPSD);
@@ -1049,10 +1051,10 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
   __shared__ double coll[MAX_SHAPES][10];
   const double dt = st->dt, time = st->time;
   const int ns = g.nshapes;
-  for (int k = 0; k < ns; ++k) {
+  for (int k = threadIdx.x / 64; k < ns; k += NT / 64) {  // a shape per wave
     double S[7];
-    shape_sums<7>(ob, k, P_PEN, S, lds);
-    if (threadIdx.x == 0) {
+    shape_sums<7>(ob, k, P_PEN, S);
+    if (threadIdx.x % 64 == 0) {
       cup2d_shape_state& T = ss[k];
       double uvw[3] = {T.u, T.v, T.omega};
       shape_velocity(sp[k], S, dt, time, uvw);
@@ -1262,7 +1264,7 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
     const double time = st->time;
     for (int k = 0; k < g.nshapes; ++k) {
       double S[7];
-      shape_sums<7>(ob, k, P_PEN, S, lds);
+      shape_sums<7>(ob, k, P_PEN, S);
       if (threadIdx.x == 0) {
         cup2d_shape_state& T = ss[k];
         double uvw[3] = {T.u, T.v, T.omega};
@@ -1495,11 +1497,10 @@ int enqueue_iter_penalize(cup2d_ctx* c, double* reldf) {
 __device__ void forces_fin_wg(const Grid& g, cup2d_shape_state* __restrict__ ss,
                               const Obstacles& ob, StepState* __restrict__ st, int end_step,
                               const ForceSnap* __restrict__ snap) {
-  __shared__ double lds[LDS_RED];
-  for (int k = 0; k < g.nshapes; ++k) {
+  for (int k = threadIdx.x / 64; k < g.nshapes; k += NT / 64) {  // a shape per wave
     double a[18];
-    shape_sums<18>(ob, k, P_FRC, a, lds);
-    if (threadIdx.x != 0) continue;
+    shape_sums<18>(ob, k, P_FRC, a);
+    if (threadIdx.x % 64 != 0) continue;
     cup2d_shape_state& T = ss[k];
     T.perimeter = a[0]; T.circulation = a[1];
     T.forcex = a[2]; T.forcey = a[3]; T.forcex_V = a[4]; T.forcey_V = a[5];
