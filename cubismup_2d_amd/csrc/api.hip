@@ -229,20 +229,26 @@ int pressure_iterator(cup2d_ctx* c, int* iters_out, double* reldf_out) {
 #ifndef CUP2D_OBST_OVERLAP
 #define CUP2D_OBST_OVERLAP 1
 #endif
+// with swimmers: the placement chain on the main stream, advDiff beside it
+#ifndef CUP2D_CHAIN_MAIN
+#define CUP2D_CHAIN_MAIN 1
+#endif
 
 // Simulation::advance.  compute_dt: dt from the findMaxU partials of the
 // previous step (device loop) instead of the value already on the device.
 int advance_enqueued(cup2d_ctx* c, int compute_dt) {
-  RC(enqueue_put_objects(c, compute_dt, CUP2D_OBST_OVERLAP));
-  // the swimmers' time-only midline part of the next step, behind this step's
-  // obstacle chain on the side stream (off the critical path)
-  if (CUP2D_OBST_OVERLAP) RC(enqueue_fish_mid_next(c, c->host_step & 1));
+  // (with overlap, also the swimmers' time-only midline part of the next
+  // step, off the critical path).  With swimmers the placement chain is
+  // longer than advDiff: it keeps the main stream and advDiff goes beside it.
+  const bool chain_main = CUP2D_OBST_OVERLAP && CUP2D_CHAIN_MAIN && !c->hfish.empty() &&
+                          !c->cfg.iterative_penal;
+  RC(enqueue_put_objects(c, compute_dt, chain_main ? 2 : CUP2D_OBST_OVERLAP));
   if (c->cfg.iterative_penal) {
     RC(enqueue_advdiff(c, 0));
     RC(pressure_iterator(c, nullptr, nullptr));
     RC(enqueue_find_max_u_partials(c));  // for the next step's dt
   } else {
-    RC(enqueue_advdiff(c, 1));
+    if (!chain_main) RC(enqueue_advdiff(c, 1));
     RC(pressure_single(c, 1));
   }
   RC(enqueue_forces(c, 1, 1));
@@ -307,6 +313,8 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
       hipEventCreateWithFlags(&c->ev_obst, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_pen, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_frc, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_chain, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_lists, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipMalloc(&c->fsnap, sizeof(ForceSnap) * MAX_SHAPES) != hipSuccess) {
     c->err = "side stream creation failed";
     return bail(CUP2D_ERR_HIP);
@@ -402,6 +410,8 @@ int cup2d_destroy(cup2d_ctx* c) {
   if (c->ev_obst) (void)hipEventDestroy(c->ev_obst);
   if (c->ev_pen) (void)hipEventDestroy(c->ev_pen);
   if (c->ev_frc) (void)hipEventDestroy(c->ev_frc);
+  if (c->ev_chain) (void)hipEventDestroy(c->ev_chain);
+  if (c->ev_lists) (void)hipEventDestroy(c->ev_lists);
   if (c->fsnap) (void)hipFree(c->fsnap);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -785,7 +785,8 @@ static int grid_cap(cup2d_ctx* c) { return c->ob.cap > 0 ? c->ob.cap : 1; }
 // PutObjectsOnGrid::operator() (PutObjectsOnGrid.cpp:186-241).
 // overlap: run everything after the list builder on the side stream; the
 // caller joins with join_obstacles() before the next reader of obstacle data.
-static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, int end_step);
+static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, int end_step,
+                         const double* u, const double* v);
 
 // ComputeForces of a whole step: 1 = behind the next step's list builder on
 // the side stream, 0 = on the side stream behind an event after the
@@ -808,6 +809,8 @@ static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, in
 
 int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   const int G = grid_cap(c);
+  // overlap 2: the chain on the main stream and advDiff beside it
+  const bool chain_main = overlap == 2;
   // the previous whole step's ComputeForces runs after this step's obstacle
   // chain on the side stream, its step end in this step's list builder
   const bool defer_forces = overlap && c->forces_pending;
@@ -824,10 +827,23 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
                        c->ob, c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg,
                        c->ramp, compute_dt, c->host_step, fish ? 1 : 0, end_prev);
   }
+  // overlap 2: advDiff needs only dt and U_inf from the first phase of the
+  // list builder; it runs on the side stream beside the whole chain, which
+  // stays on the main stream, so that no cross-stream hop lies on the chain.
+  // The previous step's velocity stays with the deferred ComputeForces.
+  const double* u_prev = c->u;
+  const double* v_prev = c->v;
+  if (chain_main) {
+    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_begin, c->stream));
+    CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_begin, 0));
+    RC_OB(enqueue_advdiff_on(c, 1, c->aux));
+    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_obst, c->aux));  // joined before the RHS
+    c->obst_pending = true;
+  }
   // The chain's stages in order; the first `split` of them run on the main
   // stream before advDiff, the rest on the side stream beside it.
   enum { ST_MIDLINE, ST_FISH_PLACE, ST_PLACE, ST_MOMENTS, ST_UDEF, ST_COMMIT, ST_N };
-  const int split = !overlap ? ST_N : (fish ? CUP2D_SPLIT_FISH : CUP2D_SPLIT_DISKS);
+  const int split = !overlap || chain_main ? ST_N : (fish ? CUP2D_SPLIT_FISH : CUP2D_SPLIT_DISKS);
   bool on_aux = false;
   auto stream_for = [&](int stage) -> hipStream_t {
     if (stage < split) return c->stream;
@@ -846,6 +862,7 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
                        c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp, 0,
                        c->host_step, 2, 0);
+    if (chain_main) CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_lists, c->stream));
     if (!(s = stream_for(ST_FISH_PLACE))) return CUP2D_ERR_HIP;
     RC_OB(enqueue_fish_place(c, s));
   }
@@ -868,14 +885,33 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     if (!(s = stream_for(ST_COMMIT))) return CUP2D_ERR_HIP;
     hipLaunchKernelGGL(k_obst_commit, dim3(1), dim3(NT), 0, s, c->g, c->ss, c->ob);
   }
+  if (chain_main) {
+    // the next step's midline part on the midline stream (joined by the side
+    // stream), then the previous step's ComputeForces on the side stream
+    // after this step's chain; ev_frc orders both before the main stream
+    // overwrites the pressure
+    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_chain, c->stream));
+    RC_OB(enqueue_fish_mid_next(c, c->host_step & 1, c->ev_lists));
+    CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_chain, 0));
+    if (defer_forces) RC_OB(launch_forces(c, c->aux, c->ob_prev, 0, u_prev, v_prev));
+    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_frc, c->aux));
+    c->frc_join = true;
+    return launch_check(c, "put_objects");
+  }
   // the side stream always joins (the deferred ComputeForces and the next
   // step's midline are ordered after this step's list builder through it)
   if (overlap && !stream_for(ST_N)) return CUP2D_ERR_HIP;
   if (overlap) {
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_obst, c->aux));
     c->obst_pending = true;
-    if (defer_forces) {  // the previous step's ComputeForces, on its own lists
-      RC_OB(launch_forces(c, c->aux, c->ob_prev, 0));
+    // the swimmers' time-only midline part of the next step on the midline
+    // stream, joined here by the side stream, so that ev_frc (which the main
+    // stream waits for before the pressure is overwritten) also orders the
+    // next step's midline kernel after it
+    RC_OB(enqueue_fish_mid_next(c, c->host_step & 1, c->ev_begin));
+    if (defer_forces)  // the previous step's ComputeForces, on its own lists
+      RC_OB(launch_forces(c, c->aux, c->ob_prev, 0, u_prev, v_prev));
+    if (defer_forces || !c->hfish.empty()) {
       CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_frc, c->aux));
       c->frc_join = true;
     }
@@ -1604,10 +1640,11 @@ __global__ __launch_bounds__(NT) void k_forces_fin(Grid g, cup2d_shape_state* __
   forces_fin_wg(g, ss, ob, st, end_step, snap);
 }
 
-static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, int end_step) {
+static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, int end_step,
+                         const double* u, const double* v) {
   TimerScope ts(c, T_FORCES, s);
-  hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, s, c->g, c->fsnap, lists, c->u,
-                     c->v, c->pres);
+  hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, s, c->g, c->fsnap, lists, u, v,
+                     c->pres);
   hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(NT), 0, s, c->g, c->ss, lists, c->st, end_step,
                      c->fsnap);
   return launch_check(c, "forces");
@@ -1634,16 +1671,16 @@ int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux) {
     // on the side stream right after the penalization (one more event)
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_pen, c->stream));
     CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_pen, 0));
-    return launch_forces(c, c->aux, c->ob, end_step);
+    return launch_forces(c, c->aux, c->ob, end_step, c->u, c->v);
 #endif
   }
-  return launch_forces(c, c->stream, c->ob, end_step);
+  return launch_forces(c, c->stream, c->ob, end_step, c->u, c->v);
 }
 
 int flush_forces(cup2d_ctx* c) {
   if (c->forces_pending) {
     c->forces_pending = false;
-    RC_OB(launch_forces(c, c->stream, c->ob, 0));
+    RC_OB(launch_forces(c, c->stream, c->ob, 0, c->u, c->v));
   }
   if (c->end_pending) {
     c->end_pending = false;

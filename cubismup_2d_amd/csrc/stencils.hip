@@ -278,6 +278,10 @@ static constexpr int RYS = CUP2D_STENCIL_RY;  // rows per workgroup of the stenc
 #endif
 
 int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
+  return enqueue_advdiff_on(c, defer_fix, c->stream);
+}
+
+int enqueue_advdiff_on(cup2d_ctx* c, int defer_fix, hipStream_t s) {
   const int tx = strip_width(c->g.NX);
   const int gx = c->g.NX / tx, gy = c->g.NY / RYS;
   {
@@ -285,7 +289,7 @@ int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
     if (CUP2D_ADVDIFF_WG_CAP > 0 && nwg > CUP2D_ADVDIFF_WG_CAP && nwg % 8 == 0)
       nwg = CUP2D_ADVDIFF_WG_CAP;
     KernelTimer kt(c, T_ADVDIFF);
-    CUP2D_TIMED_LAUNCH(c, k_advdiff<RYS>, dim3(nwg), dim3(tx), 0, c->stream, c->u, c->v, c->tu,
+    CUP2D_TIMED_LAUNCH(c, k_advdiff<RYS>, dim3(nwg), dim3(tx), 0, s, c->u, c->v, c->tu,
                        c->tv, c->g, (const StepState*)c->st, c->fluxpart, gx, gy);
   }
   std::swap(c->u, c->tu);  // advDiff.cpp:81-87 copy tmpV -> vel
@@ -293,14 +297,14 @@ int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
   c->flux_deferred = defer_fix != 0;
   if (defer_fix) {
     // the correction once, for every reader of the deferred fix
-    hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, c->stream, c->fluxpart, gx, gy, c->g,
+    hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, s, c->fluxpart, gx, gy, c->g,
                        c->st);
   } else {
-    TimerScope ts(c, T_FLUXFIX);
-    hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, c->stream, c->fluxpart, gx, gy, c->g,
+    TimerScope ts(c, T_FLUXFIX, s);
+    hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, s, c->fluxpart, gx, gy, c->g,
                        c->st);
     const int n = 2 * c->g.NX + 2 * c->g.NY;
-    hipLaunchKernelGGL(k_flux_fix, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->u, c->v,
+    hipLaunchKernelGGL(k_flux_fix, dim3((n + 255) / 256), dim3(256), 0, s, c->u, c->v,
                        c->g, c->st);
   }
   return launch_check(c, "advdiff");
