@@ -313,7 +313,6 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
       hipEventCreateWithFlags(&c->ev_obst, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_pen, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_frc, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_chain, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_lists, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipMalloc(&c->fsnap, sizeof(ForceSnap) * MAX_SHAPES) != hipSuccess) {
     c->err = "side stream creation failed";
@@ -410,7 +409,6 @@ int cup2d_destroy(cup2d_ctx* c) {
   if (c->ev_obst) (void)hipEventDestroy(c->ev_obst);
   if (c->ev_pen) (void)hipEventDestroy(c->ev_pen);
   if (c->ev_frc) (void)hipEventDestroy(c->ev_frc);
-  if (c->ev_chain) (void)hipEventDestroy(c->ev_chain);
   if (c->ev_lists) (void)hipEventDestroy(c->ev_lists);
   if (c->fsnap) (void)hipFree(c->fsnap);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

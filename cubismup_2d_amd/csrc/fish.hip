@@ -865,17 +865,20 @@ int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s) {
 // The next step's time-only part, on the midline stream once this step's
 // list builder has stored time + dt in st->tnext[parity] (event ev_begin;
 // the next step's list builder writes the other slot), into the other
-// buffer than the one this step's k_fish_mid_b reads.  The side stream
-// waits for it behind this step's obstacle chain, where it is idle; the main
-// stream reaches it through the side stream's next event.
-int enqueue_fish_mid_next(cup2d_ctx* c, int parity, hipEvent_t after) {
+// buffer than the one this step's k_fish_mid_b reads.  With join_aux the
+// side stream waits for it behind this step's obstacle chain, where it is
+// idle, and the main stream reaches it through the side stream's next event;
+// otherwise the caller orders the main stream after the midline stream.
+int enqueue_fish_mid_next(cup2d_ctx* c, int parity, hipEvent_t after, bool join_aux) {
   if (c->hfish.empty()) return CUP2D_OK;
   // `after` follows this step's midline kernels, which share the work arrays
   CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->mids, after, 0));
   hipLaunchKernelGGL(k_fish_mid_a, dim3((int)c->hfish.size()), dim3(NT), 0, c->mids, c->fishd,
                      c->st, 1, parity, parity ^ 1);
-  CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_mid, c->mids));
-  CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_mid, 0));
+  if (join_aux) {
+    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_mid, c->mids));
+    CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_mid, 0));
+  }
   c->midA_ready = true;
   return launch_check(c, "fish midline (next step)");
 }

@@ -133,7 +133,6 @@ struct cup2d_ctx {
   // overlapping advDiff and the RHS (neither reads obstacle data)
   hipStream_t aux = nullptr;
   hipEvent_t ev_begin = nullptr, ev_obst = nullptr, ev_pen = nullptr, ev_frc = nullptr;
-  hipEvent_t ev_chain = nullptr;  // end of the placement chain on the main stream
   hipEvent_t ev_lists = nullptr;  // this step's midline and lists done (main stream)
   // third stream: the swimmers' time-only midline part of the next step runs
   // there beside advDiff (a kernel with many workgroups, so one more resident
@@ -303,7 +302,7 @@ int fish_add(cup2d_ctx* c, const cup2d_shape_desc* d, int shape);
 void fish_free_all(cup2d_ctx* c);
 int enqueue_fish_midline(cup2d_ctx* c, hipStream_t s);
 // the time-only part of the next whole step's midline (time + dt)
-int enqueue_fish_mid_next(cup2d_ctx* c, int parity, hipEvent_t after);
+int enqueue_fish_mid_next(cup2d_ctx* c, int parity, hipEvent_t after, bool join_aux);
 int enqueue_fish_place(cup2d_ctx* c, hipStream_t s);
 // PressureIterator_unif pieces (PressureIterator_unif.cpp)
 int enqueue_copy_vel_to_tmpv(cup2d_ctx* c);

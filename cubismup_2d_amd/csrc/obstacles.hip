@@ -856,13 +856,16 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     return c->aux;
   };
   hipStream_t s;
+  // this step's time-only midline part runs here (on the main stream) unless
+  // the previous step left it ready
+  const bool mid_a_here = !c->midA_ready;
   if (fish) {
     if (!(s = stream_for(ST_MIDLINE))) return CUP2D_ERR_HIP;
     RC_OB(enqueue_fish_midline(c, s));
     hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
                        c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp, 0,
                        c->host_step, 2, 0);
-    if (chain_main) CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_lists, c->stream));
+    if (chain_main && mid_a_here) CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_lists, c->stream));
     if (!(s = stream_for(ST_FISH_PLACE))) return CUP2D_ERR_HIP;
     RC_OB(enqueue_fish_place(c, s));
   }
@@ -886,15 +889,18 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     hipLaunchKernelGGL(k_obst_commit, dim3(1), dim3(NT), 0, s, c->g, c->ss, c->ob);
   }
   if (chain_main) {
-    // the next step's midline part on the midline stream (joined by the side
-    // stream), then the previous step's ComputeForces on the side stream
-    // after this step's chain; ev_frc orders both before the main stream
-    // overwrites the pressure
-    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_chain, c->stream));
-    RC_OB(enqueue_fish_mid_next(c, c->host_step & 1, c->ev_lists));
-    CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_chain, 0));
-    if (defer_forces) RC_OB(launch_forces(c, c->aux, c->ob_prev, 0, u_prev, v_prev));
-    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_frc, c->aux));
+    // The next step's midline part, then the previous step's ComputeForces,
+    // on the midline stream; ev_frc, which the main stream waits for before
+    // the pressure is overwritten, orders both before the rest of the step.
+    // The midline part shares work arrays only with this step's own (ev_lists,
+    // when that ran here); ComputeForces reads the previous velocity (which
+    // advDiff only reads), pressure and lists, and writes only force fields
+    // of the shape state besides the two block/surface counts, which the
+    // commit also sets and the step's own ComputeForces sets last.
+    RC_OB(enqueue_fish_mid_next(c, c->host_step & 1, mid_a_here ? c->ev_lists : c->ev_begin,
+                                false));
+    if (defer_forces) RC_OB(launch_forces(c, c->mids, c->ob_prev, 0, u_prev, v_prev));
+    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_frc, c->mids));
     c->frc_join = true;
     return launch_check(c, "put_objects");
   }
@@ -908,7 +914,7 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     // stream, joined here by the side stream, so that ev_frc (which the main
     // stream waits for before the pressure is overwritten) also orders the
     // next step's midline kernel after it
-    RC_OB(enqueue_fish_mid_next(c, c->host_step & 1, c->ev_begin));
+    RC_OB(enqueue_fish_mid_next(c, c->host_step & 1, c->ev_begin, true));
     if (defer_forces)  // the previous step's ComputeForces, on its own lists
       RC_OB(launch_forces(c, c->aux, c->ob_prev, 0, u_prev, v_prev));
     if (defer_forces || !c->hfish.empty()) {
