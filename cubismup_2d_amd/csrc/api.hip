@@ -345,8 +345,12 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
   const std::vector<int> band = band_blocks(g);
   c->band_nblocks = (int)band.size();
   if (c->band_nblocks > 0) {
+    std::vector<int> index((size_t)g.bpdx * g.bpdy, -1);
+    for (size_t j = 0; j < band.size(); ++j) index[band[j]] = (int)j;
     if (hipMalloc(&c->band_blocks, sizeof(int) * band.size()) != hipSuccess ||
-        hipMemcpy(c->band_blocks, band.data(), sizeof(int) * band.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(c->band_blocks, band.data(), sizeof(int) * band.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&c->band_index, sizeof(int) * index.size()) != hipSuccess ||
+        hipMemcpy(c->band_index, index.data(), sizeof(int) * index.size(), hipMemcpyHostToDevice) != hipSuccess) {
       c->err = "out of device memory";
       return bail(CUP2D_ERR_HIP);
     }
@@ -400,6 +404,7 @@ int cup2d_destroy(cup2d_ctx* c) {
   if (c->sp) (void)hipFree(c->sp);
   if (c->ss) (void)hipFree(c->ss);
   if (c->band_blocks) (void)hipFree(c->band_blocks);
+  if (c->band_index) (void)hipFree(c->band_index);
   for (Timer& t : c->timers)
     for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
   if (c->aux) { (void)hipStreamSynchronize(c->aux); (void)hipStreamDestroy(c->aux); }

@@ -363,4 +363,31 @@ __device__ inline double rhs_cell(const double* __restrict__ u, const double* __
   return facDiv * (divVx + divVy);
 }
 
+// The cub2rhs fade sums of one band block (PoissonSolver.cpp:81-98): of the
+// faded RHS r*fa^2, of its positive parts and of its negative parts, by a
+// 256-thread workgroup in a fixed order, so the partial is the same whichever
+// kernel forms it.  from_plane: the RHS is read from r, else formed from the
+// advDiff output u, v with the deferred flux correction (rhs_cell).
+__device__ inline void fade_block_sums(const Grid& g, int b, const double* __restrict__ r,
+                                       const double* __restrict__ u,
+                                       const double* __restrict__ v, bool from_plane,
+                                       double corr, double facDiv, double* out, double* lds) {
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  double sR = 0, sP = 0, sN = 0;
+#pragma unroll
+  for (int it = 0, k = threadIdx.x; it < BS * BS / 256; ++it, k += 256) {
+    const int ix = k % BS, iy = k / BS, x = bx * BS + ix, y = by * BS + iy;
+    const double px = cell_pos(bx, ix, g.hb, g.h), py = cell_pos(by, iy, g.hb, g.h);
+    const double fa = fade_arg(g, px, py);
+    const double rc = from_plane ? r[(size_t)y * g.NX + x]
+                                 : rhs_cell(u, v, g.NX, g.NY, x, y, corr, facDiv);
+    const double f = rc * (fa * fa);
+    if (f < 0) sN -= f;
+    if (f > 0) sP += f;
+    sR += f;
+  }
+  out[0] = sR; out[1] = sP; out[2] = sN;
+  block_sum_multi<256, 3>(out, lds);
+}
+
 }  // namespace cup2d

@@ -196,6 +196,8 @@ struct cup2d_ctx {
   std::vector<cup2d::FishDev> hfish;       // host copy (device pointers)
   int band_nblocks = 0;
   int* band_blocks = nullptr;              // blocks touched by the cub2rhs fade
+  int* band_index = nullptr;               // [blocks] position in band_blocks, -1 if none
+  bool fade_early = false;                 // this step's fade sums already launched
   cup2d::FFTPlan* fft = nullptr;
   std::string fft_err;                     // why no FFT plan exists (size limits)
   bool shapes_frozen = false;
@@ -257,6 +259,7 @@ int enqueue_advdiff_on(cup2d_ctx* c, int defer_fix, hipStream_t s);
 int enqueue_pressure_rhs(cup2d_ctx* c);
 int stencil_edge_count(const cup2d_ctx* c);
 int enqueue_fade_sums(cup2d_ctx* c, int compute = 0);
+int enqueue_fade_sums_early(cup2d_ctx* c, hipStream_t s);
 int fade_partials(const cup2d_ctx* c);
 int enqueue_pressure_correction(cup2d_ctx* c, int want_max);
 int enqueue_set_dt(cup2d_ctx* c, double dt);

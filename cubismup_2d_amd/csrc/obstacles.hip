@@ -837,6 +837,7 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_begin, c->stream));
     CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_begin, 0));
     RC_OB(enqueue_advdiff_on(c, 1, c->aux));
+    RC_OB(enqueue_fade_sums_early(c, c->aux));
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_obst, c->aux));  // joined before the RHS
     c->obst_pending = true;
   }
@@ -983,9 +984,16 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
   }
 }
 
+// band_index != nullptr: the fade sums were formed from the advDiff output
+// before the obstacle RHS; the partial of an obstacle block in the fade band
+// is formed again here from its final RHS.
 __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __restrict__ st,
                                                   Obstacles ob, const double* __restrict__ chi,
-                                                  double* __restrict__ r) {
+                                                  double* __restrict__ r,
+                                                  const int* __restrict__ band_index,
+                                                  double* __restrict__ fadepart) {
+  static_assert(NT == 256, "fade_block_sums is a 256-thread reduction");
+  __shared__ double lds[12];
   if ((int)blockIdx.x >= ob.cnt[1]) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
@@ -1010,6 +1018,16 @@ __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __res
       else if (isPerim && T < 0 && corr < 0) r[d] = T + corr * T;
     }
   }
+  const int j = band_index ? band_index[b] : -1;
+  if (j < 0) return;
+  __syncthreads();  // the block's RHS is final
+  double a[3];
+  fade_block_sums(g, b, r, nullptr, nullptr, true, 0.0, 0.0, a, lds);
+  if (threadIdx.x == 0) {
+    fadepart[3 * j] = a[0];
+    fadepart[3 * j + 1] = a[1];
+    fadepart[3 * j + 2] = a[2];
+  }
 }
 
 int enqueue_rhs_obstacles(cup2d_ctx* c, int compute) {
@@ -1019,7 +1037,7 @@ int enqueue_rhs_obstacles(cup2d_ctx* c, int compute) {
   hipLaunchKernelGGL(k_rhs_obst1, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
                      c->rhs, compute ? c->u : nullptr, c->v);
   hipLaunchKernelGGL(k_rhs_obst2, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
-                     c->rhs);
+                     c->rhs, c->fade_early ? c->band_index : nullptr, c->red);
   return launch_check(c, "rhs_obstacles");
 }
 

@@ -409,38 +409,26 @@ __global__ __launch_bounds__(256) void k_fade_sums(const double* __restrict__ r,
                                                    const unsigned char* __restrict__ obmap,
                                                    const StepState* __restrict__ st) {
   __shared__ double lds[12];
+  const int j = blockIdx.x;
+  if (j >= nblocks) return;
   // u != nullptr: the RHS of cells outside obstacle blocks is formed from the
   // advDiff output (see k_rows_fwd) with the deferred flux correction, the
-  // obstacle blocks' from the RHS plane
+  // obstacle blocks' from the RHS plane.  obmap == nullptr: every block from
+  // the advDiff output (the obstacle blocks' partials are then replaced once
+  // their RHS is final, by the obstacle RHS kernel).
   double corr = 0, facDiv = 0;
   if (u) {
     corr = st->flux_corr;
     facDiv = 0.5 * g.h / st->dt;
   }
-  double sR = 0, sP = 0, sN = 0;
-  for (int j = blockIdx.x; j < nblocks; j += gridDim.x) {
-    const int b = blocks[j];
-    const int bx = b % g.bpdx, by = b / g.bpdx;
-    const bool from_plane = !u || obmap[b];
-#pragma unroll
-    for (int it = 0, k = threadIdx.x; it < BS * BS / 256; ++it, k += 256) {
-      const int ix = k % BS, iy = k / BS, x = bx * BS + ix, y = by * BS + iy;
-      const double px = cell_pos(bx, ix, g.hb, g.h), py = cell_pos(by, iy, g.hb, g.h);
-      const double fa = fade_arg(g, px, py);
-      const double rc = from_plane ? r[(size_t)y * g.NX + x]
-                                   : rhs_cell(u, v, g.NX, g.NY, x, y, corr, facDiv);
-      const double f = rc * (fa * fa);
-      if (f < 0) sN -= f;
-      if (f > 0) sP += f;
-      sR += f;
-    }
-  }
-  double a[3] = {sR, sP, sN};
-  block_sum_multi<256, 3>(a, lds);
+  const int b = blocks[j];
+  const bool from_plane = !u || (obmap && obmap[b]);
+  double a[3];
+  fade_block_sums(g, b, r, u, v, from_plane, corr, facDiv, a, lds);
   if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = a[0];
-    part[3 * blockIdx.x + 1] = a[1];
-    part[3 * blockIdx.x + 2] = a[2];
+    part[3 * j] = a[0];
+    part[3 * j + 1] = a[1];
+    part[3 * j + 2] = a[2];
   }
 }
 
@@ -456,12 +444,26 @@ __global__ __launch_bounds__(256) void k_fade_reduce(const double* __restrict__ 
   if (threadIdx.x == 0) st->fade_corr = fade_corr_factor(sums[0], sums[1], sums[2]);
 }
 
+// The fade sums of every band block from the advDiff output, on stream s
+// right after advDiff (the whole-step RHS); the obstacle RHS kernel replaces
+// the partials of obstacle blocks, and enqueue_fade_sums then only reduces.
+int enqueue_fade_sums_early(cup2d_ctx* c, hipStream_t s) {
+  c->fade_early = false;
+  if (c->band_nblocks == 0) return CUP2D_OK;
+  hipLaunchKernelGGL(k_fade_sums, dim3(fade_partials(c)), dim3(256), 0, s, c->rhs,
+                     c->band_blocks, c->band_nblocks, c->g, c->red, c->u, c->v, nullptr, c->st);
+  c->fade_early = true;
+  return launch_check(c, "cub2rhs fade (early)");
+}
+
 int enqueue_fade_sums(cup2d_ctx* c, int compute) {
   if (c->band_nblocks == 0) return CUP2D_OK;
   TimerScope ts(c, T_FADE);
-  hipLaunchKernelGGL(k_fade_sums, dim3(fade_partials(c)), dim3(256), 0, c->stream, c->rhs,
-                     c->band_blocks, c->band_nblocks, c->g, c->red, compute ? c->u : nullptr,
-                     c->v, c->obmap, c->st);
+  if (!c->fade_early || !compute)
+    hipLaunchKernelGGL(k_fade_sums, dim3(fade_partials(c)), dim3(256), 0, c->stream, c->rhs,
+                       c->band_blocks, c->band_nblocks, c->g, c->red, compute ? c->u : nullptr,
+                       c->v, c->obmap, c->st);
+  c->fade_early = false;
   hipLaunchKernelGGL(k_fade_reduce, dim3(1), dim3(256), 0, c->stream, c->red, fade_partials(c),
                      c->st);
   return launch_check(c, "cub2rhs fade");

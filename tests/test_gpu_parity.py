@@ -452,3 +452,17 @@ def test_free_swimmer_device_loop_periodic(cup, oracle_mod):
                                   oracle_mod.disk(radius=0.05, xpos=1.5, ypos=0.4, xvel=0.1,
                                                   bFixed=True, bForced=True)])
     _run_and_compare(g, o, 10, nshapes=2, device_loop=True)
+
+
+@pytest.mark.parametrize("poisson", ["periodic", "freespace"])
+def test_swimmer_in_fade_band_device_loop(cup, oracle_mod, poisson):
+    """A swimmer whose obstacle blocks reach into the cub2rhs fade band
+    (PoissonSolver.cpp:71-98: the outermost block row here), advanced by the
+    device loop: the fade sums of band blocks are formed from the advDiff
+    output beside the placement chain, and those of obstacle blocks again
+    from their final RHS, which must give the oracle's solve."""
+    fish = [dict(xpos=1.0, ypos=0.08, angle=0.0, **FISH_VARIANTS["carling"])]
+    kw = dict(extent=2.0, nu=1e-4, CFL=0.2, poisson=poisson, fadeLen=0.05)
+    g = cup.Simulation(32, 16, shapes=[cup.carlingfish(**fish[0])], **kw)
+    o = oracle_mod.Oracle(32, 16, fs_pad2=True, shapes=[oracle_mod.carlingfish(**fish[0])], **kw)
+    _run_and_compare(g, o, 8, device_loop=True)
