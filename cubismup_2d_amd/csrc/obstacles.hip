@@ -803,6 +803,11 @@ static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, in
 #ifndef CUP2D_SPLIT_FISH
 #define CUP2D_SPLIT_FISH 1
 #endif
+// chain on the main stream: advDiff starts after the dt/motion phase (0) or
+// after the midline rotation and the list phase (1)
+#ifndef CUP2D_ADV_AFTER_LISTS
+#define CUP2D_ADV_AFTER_LISTS 0
+#endif
 #ifndef CUP2D_SPLIT_DISKS
 #define CUP2D_SPLIT_DISKS 0
 #endif
@@ -833,14 +838,16 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   // The previous step's velocity stays with the deferred ComputeForces.
   const double* u_prev = c->u;
   const double* v_prev = c->v;
-  if (chain_main) {
+  auto launch_side = [&]() -> int {
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_begin, c->stream));
     CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_begin, 0));
     RC_OB(enqueue_advdiff_on(c, 1, c->aux));
     RC_OB(enqueue_fade_sums_early(c, c->aux));
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_obst, c->aux));  // joined before the RHS
     c->obst_pending = true;
-  }
+    return CUP2D_OK;
+  };
+  if (chain_main && !CUP2D_ADV_AFTER_LISTS) RC_OB(launch_side());
   // The chain's stages in order; the first `split` of them run on the main
   // stream before advDiff, the rest on the side stream beside it.
   enum { ST_MIDLINE, ST_FISH_PLACE, ST_PLACE, ST_MOMENTS, ST_UDEF, ST_COMMIT, ST_N };
@@ -867,6 +874,7 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
                        c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp, 0,
                        c->host_step, 2, 0);
     if (chain_main && mid_a_here) CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_lists, c->stream));
+    if (chain_main && CUP2D_ADV_AFTER_LISTS) RC_OB(launch_side());
     if (!(s = stream_for(ST_FISH_PLACE))) return CUP2D_ERR_HIP;
     RC_OB(enqueue_fish_place(c, s));
   }
