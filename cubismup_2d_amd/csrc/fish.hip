@@ -370,25 +370,34 @@ __global__ __launch_bounds__(NT) void k_fish_mid_b(const FishDev* __restrict__ f
     ub[2] = dmin(ub[2], out[SG_LAB + 2]);
     ub[3] = dmax(ub[3], out[SG_LAB + 3]);
   }
-  __syncthreads();
+  // min / max over the workgroup (exact, so the order does not matter): a
+  // butterfly per wave, then the waves' values through LDS
 #pragma unroll
-  for (int q = 0; q < 4; ++q) red[q * NT + t] = ub[q];
-  red[4 * NT + t] = vmax;
-  for (int s = NT / 2; s >= 1; s >>= 1) {
-    __syncthreads();
-    if (t < s) {
-      red[t] = dmin(red[t], red[t + s]);
-      red[NT + t] = dmax(red[NT + t], red[NT + t + s]);
-      red[2 * NT + t] = dmin(red[2 * NT + t], red[2 * NT + t + s]);
-      red[3 * NT + t] = dmax(red[3 * NT + t], red[3 * NT + t + s]);
-      red[4 * NT + t] = dmax(red[4 * NT + t], red[4 * NT + t + s]);
-    }
+  for (int o = 32; o > 0; o >>= 1) {
+    ub[0] = dmin(ub[0], __shfl_xor(ub[0], o, 64));
+    ub[1] = dmax(ub[1], __shfl_xor(ub[1], o, 64));
+    ub[2] = dmin(ub[2], __shfl_xor(ub[2], o, 64));
+    ub[3] = dmax(ub[3], __shfl_xor(ub[3], o, 64));
+    vmax = dmax(vmax, __shfl_xor(vmax, o, 64));
+  }
+  __syncthreads();
+  if (t % 64 == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[(t / 64) * 5 + q] = ub[q];
+    red[(t / 64) * 5 + 4] = vmax;
   }
   __syncthreads();
   if (t == 0) {
-    for (int q = 0; q < 4; ++q) F.box[q] = red[q * NT];
+    for (int w = 1; w < NT / 64; ++w) {
+      red[0] = dmin(red[0], red[w * 5]);
+      red[1] = dmax(red[1], red[w * 5 + 1]);
+      red[2] = dmin(red[2], red[w * 5 + 2]);
+      red[3] = dmax(red[3], red[w * 5 + 3]);
+      red[4] = dmax(red[4], red[w * 5 + 4]);
+    }
+    for (int q = 0; q < 4; ++q) F.box[q] = red[q];
     int e = 0;
-    (void)frexp(red[4 * NT], &e);  // vmax < 2^e
+    (void)frexp(red[4], &e);  // vmax < 2^e
     F.st[8] = ldexp(1.0, 52 - e);  // velocity scale Su: |w ud Su| < 2^52
     F.st[0] = area; F.st[1] = J;
     F.st[2] = C0; F.st[3] = C1;
