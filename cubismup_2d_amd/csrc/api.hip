@@ -133,6 +133,8 @@ int ensure_frozen(cup2d_ctx* c) {
 }
 
 int sync_all(cup2d_ctx* c) {
+  RC(join_obstacles(c));
+  RC(flush_commit(c));
   RC(flush_forces(c));
   RC(join_obstacles(c));
   RC(join_forces(c));

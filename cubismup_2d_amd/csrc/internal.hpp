@@ -140,6 +140,7 @@ struct cup2d_ctx {
   hipStream_t mids = nullptr;
   hipEvent_t ev_mid = nullptr;
   bool obst_pending = false;
+  bool commit_pending = false;  // the placement commit rides in the next obstacle RHS launch
   // ComputeForces and the step end (time += dt) of the last whole step, not
   // enqueued yet.  The next whole step applies the step end in its list
   // builder and runs ComputeForces on the side stream after its obstacle
@@ -283,6 +284,7 @@ void obstacles_free(cup2d_ctx* c);
 constexpr int OBST_LIST_CAP = 2048;  // slots handled by the list builder
 int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap = 0);
 int join_obstacles(cup2d_ctx* c);
+int flush_commit(cup2d_ctx* c);
 // compute: form the RHS of the obstacle blocks from the advDiff output first
 // (the whole-step path, where the row FFT forms it for every other cell).
 int enqueue_rhs_obstacles(cup2d_ctx* c, int compute = 0);

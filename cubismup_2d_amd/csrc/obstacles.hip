@@ -895,7 +895,12 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
                          c->udx, c->udy);
     }
     if (!(s = stream_for(ST_COMMIT))) return CUP2D_ERR_HIP;
-    hipLaunchKernelGGL(k_obst_commit, dim3(1), dim3(NT), 0, s, c->g, c->ss, c->ob);
+    // chain on the main stream: the commit rides in the obstacle RHS launch,
+    // the next kernel that reads nothing it writes
+    if (chain_main)
+      c->commit_pending = true;
+    else
+      hipLaunchKernelGGL(k_obst_commit, dim3(1), dim3(NT), 0, s, c->g, c->ss, c->ob);
   }
   if (chain_main) {
     // The next step's midline part, then the previous step's ComputeForces,
@@ -934,6 +939,14 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   return launch_check(c, "put_objects");
 }
 
+// a placement commit still waiting for its obstacle RHS launch, on its own
+int flush_commit(cup2d_ctx* c) {
+  if (!c->commit_pending) return CUP2D_OK;
+  c->commit_pending = false;
+  hipLaunchKernelGGL(k_obst_commit, dim3(1), dim3(NT), 0, c->stream, c->g, c->ss, c->ob);
+  return launch_check(c, "placement commit");
+}
+
 int join_obstacles(cup2d_ctx* c) {
   if (!c->obst_pending) return CUP2D_OK;
   CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_obst, 0));
@@ -952,8 +965,13 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
                                                   Obstacles ob, const double* __restrict__ chi,
                                                   double* __restrict__ r,
                                                   const double* __restrict__ u,
-                                                  const double* __restrict__ v) {
+                                                  const double* __restrict__ v,
+                                                  cup2d_shape_state* __restrict__ commit_ss) {
   __shared__ double lds[LDS_RED];
+  if (commit_ss && blockIdx.x == gridDim.x - 1) {  // the placement commit (k_obst_commit)
+    obst_commit_wg(g, commit_ss, ob);
+    return;
+  }
   if ((int)blockIdx.x >= ob.cnt[1]) return;
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
@@ -1042,8 +1060,12 @@ int enqueue_rhs_obstacles(cup2d_ctx* c, int compute) {
   if (c->g.nshapes == 0) return CUP2D_OK;
   const int G = grid_cap(c);
   TimerScope ts(c, T_RHS_OBST);
-  hipLaunchKernelGGL(k_rhs_obst1, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
-                     c->rhs, compute ? c->u : nullptr, c->v);
+  // a pending placement commit takes one more workgroup
+  const int G1 = c->commit_pending ? G + 1 : G;
+  hipLaunchKernelGGL(k_rhs_obst1, dim3(G1), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
+                     c->rhs, compute ? c->u : nullptr, c->v,
+                     c->commit_pending ? c->ss : nullptr);
+  c->commit_pending = false;
   hipLaunchKernelGGL(k_rhs_obst2, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
                      c->rhs, c->fade_early ? c->band_index : nullptr, c->red);
   return launch_check(c, "rhs_obstacles");
