@@ -83,7 +83,12 @@ EXPORTS = [
     "cup2d_compute_forces",
     "cup2d_advance", "cup2d_simulate", "cup2d_synchronize", "cup2d_poisson_dense",
     "cup2d_grid_size", "cup2d_timer_enable", "cup2d_timer_read", "cup2d_timer_sample",
+    "cup2d_force_log_enable", "cup2d_force_log_read",
 ]
+FORCE_LOG_COLUMNS = ["time", "step", "Fx", "Fy", "FxPres", "FyPres", "FxVisc", "FyVisc", "tau",
+                     "tauPres", "tauVisc", "drag", "thrust", "perimeter", "circulation",
+                     "Pthrust", "Pdrag", "PoutBnd", "Pout", "defPowerBnd", "defPower",
+                     "EffPDefBnd", "EffPDef", "n_surface_points"]
 TIMERS = ["find_max_u", "dt", "advdiff", "flux_fix", "pressure_rhs", "rhs_obstacles", "fade",
           "fft_rows_fwd", "fft_cols", "fft_rows_inv", "pressure_correction", "put_objects",
           "body_update", "penalize", "forces"]
@@ -130,6 +135,8 @@ def lib():
     L.cup2d_timer_enable.argtypes = [P, C.c_char_p, I]
     L.cup2d_timer_read.argtypes = [P, C.c_char_p, dp, C.POINTER(I)]
     L.cup2d_timer_sample.argtypes = [P, C.c_char_p, I]
+    L.cup2d_force_log_enable.argtypes = [P, I]
+    L.cup2d_force_log_read.argtypes = [P, I, I, dp, C.POINTER(I)]
     for n in EXPORTS:
         if n != "cup2d_last_error":
             getattr(L, n).restype = C.c_int
@@ -162,7 +169,7 @@ def carlingfish(xpos, ypos, L=0.1, T=1.0, phi=0.0, angle=0.0, amplitudeFactor=1.
                 bFixedx=None, bFixedy=None, bForcedx=None, bForcedy=None, bBlockAng=None,
                 xvel=0.0, yvel=0.0, angvel=0.0, timeForced=1.7976931348623157e308):
     """CarlingFish descriptor (-L -T -phi -amplitudeFactor, Fish.h:27-29,
-    CarlingFish.cpp:371-376) with the reference's flag defaults
+    CarlingFish.cpp:90-95) with the reference's flag defaults
     (Shape.cpp:331-345).  envelope "carling" is the reference's lateral
     amplitude 4/33 (s + L/32) with wavelength L; "quadratic" is
     L (env_a0 + env_a2 (s/L)^2) with wavelength waveLength*L.  profile
@@ -336,6 +343,21 @@ class Simulation:
         t, n = C.c_double(), C.c_int()
         self._chk(self.L.cup2d_timer_read(self.h, kernel.encode(), C.byref(t), C.byref(n)))
         return t.value, n.value
+
+    def force_log_enable(self, capacity):
+        """Keep the last `capacity` per-step force records of every shape on
+        the device (0 disables)."""
+        self._chk(self.L.cup2d_force_log_enable(self.h, capacity))
+        self._flog_cap = capacity
+
+    def force_log(self, i=0, max_records=None):
+        """The recorded forces of shape i, oldest first: an (n, 24) array whose
+        columns are FORCE_LOG_COLUMNS."""
+        n = C.c_int()
+        cap = getattr(self, "_flog_cap", 0) if max_records is None else max_records
+        out = np.empty((cap, len(FORCE_LOG_COLUMNS)))
+        self._chk(self.L.cup2d_force_log_read(self.h, i, cap, _dp(out), C.byref(n)))
+        return out[:n.value].copy()
 
     def poisson_dense(self, rhs):
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)

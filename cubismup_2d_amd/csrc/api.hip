@@ -150,6 +150,8 @@ int sync_and_check(cup2d_ctx* c) {
   CUP2D_HIP_CHECK(c, hipMemcpy(&err, &c->st->error, sizeof(int), hipMemcpyDeviceToHost));
   if (err == CUP2D_ERR_BODY_OUT) return fail(c, err, "a body left the domain");
   if (err == CUP2D_ERR_CAPACITY) return fail(c, err, "obstacle-block pool exhausted");
+  if (err == CUP2D_ERR_NONFINITE)
+    return fail(c, err, "non-finite velocity (NaN or Inf reached findMaxU)");
   if (err) return fail(c, err, "device error");
   return CUP2D_OK;
 }
@@ -172,8 +174,10 @@ int pressure_single(cup2d_ctx* c, int fused) {
     RC(enqueue_fade_sums(c, 1));
     RC(enqueue_poisson_fft(c, c->rhs, c->pres, 2));
   } else {
-    RC(enqueue_pressure_rhs(c));
+    // the side stream may still be writing the advDiff output that the RHS
+    // reads (swimmer layout: advDiff runs beside the placement chain)
     RC(join_obstacles(c));
+    RC(enqueue_pressure_rhs(c));
     RC(enqueue_rhs_obstacles(c));
     RC(enqueue_fade_sums(c));
     RC(enqueue_poisson_fft(c, c->rhs, c->pres, 1));
@@ -222,19 +226,8 @@ int pressure_iterator(cup2d_ctx* c, int* iters_out, double* reldf_out) {
 // device work between two streams of one device and the host never waits on
 // them, so recording them needs no system-scope fence (which would write the
 // L2 back on every record).
-#ifndef CUP2D_SYNC_EVENT_FLAGS
 #define CUP2D_SYNC_EVENT_FLAGS (hipEventDisableTiming | hipEventDisableSystemFence)
-#endif
 
-// Run the obstacle placement on the side stream, overlapping advDiff (1), or
-// in line before it (0).
-#ifndef CUP2D_OBST_OVERLAP
-#define CUP2D_OBST_OVERLAP 1
-#endif
-// with swimmers: the placement chain on the main stream, advDiff beside it
-#ifndef CUP2D_CHAIN_MAIN
-#define CUP2D_CHAIN_MAIN 1
-#endif
 
 // Simulation::advance.  compute_dt: dt from the findMaxU partials of the
 // previous step (device loop) instead of the value already on the device.
@@ -242,9 +235,8 @@ int advance_enqueued(cup2d_ctx* c, int compute_dt) {
   // (with overlap, also the swimmers' time-only midline part of the next
   // step, off the critical path).  With swimmers the placement chain is
   // longer than advDiff: it keeps the main stream and advDiff goes beside it.
-  const bool chain_main = CUP2D_OBST_OVERLAP && CUP2D_CHAIN_MAIN && !c->hfish.empty() &&
-                          !c->cfg.iterative_penal;
-  RC(enqueue_put_objects(c, compute_dt, chain_main ? 2 : CUP2D_OBST_OVERLAP));
+  const bool chain_main = !c->hfish.empty() && !c->cfg.iterative_penal;
+  RC(enqueue_put_objects(c, compute_dt, chain_main ? 2 : 1));
   if (c->cfg.iterative_penal) {
     RC(enqueue_advdiff(c, 0));
     RC(pressure_iterator(c, nullptr, nullptr));
@@ -418,6 +410,8 @@ int cup2d_destroy(cup2d_ctx* c) {
   if (c->ev_frc) (void)hipEventDestroy(c->ev_frc);
   if (c->ev_lists) (void)hipEventDestroy(c->ev_lists);
   if (c->fsnap) (void)hipFree(c->fsnap);
+  if (c->flog) (void)hipFree(c->flog);
+  if (c->flog_cnt) (void)hipFree(c->flog_cnt);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CUP2D_OK;
@@ -427,6 +421,7 @@ int cup2d_add_shape(cup2d_ctx* c, const cup2d_shape_desc* d, int* shape_id) {
   CHECK_CTX(c);
   if (!d) return fail(c, CUP2D_ERR_ARG, "null shape");
   if (c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "shapes must be added before reset");
+  if (c->flog) return fail(c, CUP2D_ERR_STATE, "shapes must be added before the force log is enabled");
   if (d->kind != 0 && d->kind != 1)
     return fail(c, CUP2D_ERR_ARG, "shape kind must be 0 (disk) or 1 (carlingfish)");
   if (d->kind == 0 && !(d->radius > 0)) return fail(c, CUP2D_ERR_ARG, "radius must be > 0");
@@ -600,7 +595,7 @@ int cup2d_find_max_u(cup2d_ctx* c, double* maxU) {
   StepState s;
   CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
   if (maxU) *maxU = s.maxU;
-  return CUP2D_OK;
+  return sync_and_check(c);
 }
 
 int cup2d_calc_max_timestep(cup2d_ctx* c, double* dt) {
@@ -610,7 +605,7 @@ int cup2d_calc_max_timestep(cup2d_ctx* c, double* dt) {
   StepState s;
   CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
   if (dt) *dt = s.dt;
-  return CUP2D_OK;
+  return sync_and_check(c);
 }
 
 int cup2d_put_objects(cup2d_ctx* c, double dt) {
@@ -724,6 +719,49 @@ int cup2d_poisson_dense(cup2d_ctx* c, const double* rhs, double* sol) {
   RC(enqueue_poisson_fft(c, din, dout, 0));
   CUP2D_HIP_CHECK(c, hipMemcpyAsync(sol, dout, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return CUP2D_OK;
+}
+
+int cup2d_force_log_enable(cup2d_ctx* c, int capacity) {
+  CHECK_CTX(c);
+  if (capacity < 0) return fail(c, CUP2D_ERR_ARG, "force log capacity < 0");
+  RC(sync_all(c));
+  if (c->flog) (void)hipFree(c->flog);
+  if (c->flog_cnt) (void)hipFree(c->flog_cnt);
+  c->flog = nullptr;
+  c->flog_cnt = nullptr;
+  c->flog_cap = 0;
+  if (capacity == 0 || c->g.nshapes == 0) return CUP2D_OK;
+  const size_t n = (size_t)capacity * c->g.nshapes * FLOG_REC;
+  CUP2D_HIP_CHECK(c, hipMalloc(&c->flog, sizeof(double) * n));
+  CUP2D_HIP_CHECK(c, hipMalloc(&c->flog_cnt, sizeof(unsigned long long)));
+  CUP2D_HIP_CHECK(c, hipMemset(c->flog_cnt, 0, sizeof(unsigned long long)));
+  c->flog_cap = capacity;
+  return CUP2D_OK;
+}
+
+int cup2d_force_log_read(cup2d_ctx* c, int shape_id, int max_records, double* out, int* nrecords) {
+  CHECK_CTX(c);
+  if (!nrecords || max_records < 0 || (max_records > 0 && !out))
+    return fail(c, CUP2D_ERR_ARG, "force_log_read: bad arguments");
+  if (shape_id < 0 || shape_id >= c->g.nshapes) return fail(c, CUP2D_ERR_ARG, "bad shape id");
+  if (!c->flog) return fail(c, CUP2D_ERR_STATE, "force log not enabled");
+  RC(sync_all(c));
+  unsigned long long cnt = 0;
+  CUP2D_HIP_CHECK(c, hipMemcpy(&cnt, c->flog_cnt, sizeof(cnt), hipMemcpyDeviceToHost));
+  const unsigned long long held = cnt < (unsigned long long)c->flog_cap ? cnt : c->flog_cap;
+  const unsigned long long n = held < (unsigned long long)max_records ? held : max_records;
+  const size_t ns = (size_t)c->g.nshapes;
+  std::vector<double> all((size_t)c->flog_cap * ns * FLOG_REC);
+  CUP2D_HIP_CHECK(c, hipMemcpy(all.data(), c->flog, sizeof(double) * all.size(),
+                               hipMemcpyDeviceToHost));
+  // the n most recent records, oldest first
+  for (unsigned long long j = 0; j < n; ++j) {
+    const unsigned long long r = cnt - n + j;
+    const double* src = all.data() + ((r % c->flog_cap) * ns + shape_id) * FLOG_REC;
+    std::memcpy(out + j * FLOG_REC, src, sizeof(double) * FLOG_REC);
+  }
+  *nrecords = (int)n;
   return CUP2D_OK;
 }
 

@@ -7,19 +7,19 @@
 
 namespace cup2d {
 
-// Issue priority of the waves of the obstacle-placement kernels, which share
-// the CUs with advDiff when they run on the side stream (0: default).
-#ifndef CUP2D_CHAIN_PRIO
-#define CUP2D_CHAIN_PRIO 0
-#endif
-__device__ inline void chain_prio() {
-  if (CUP2D_CHAIN_PRIO > 0) __builtin_amdgcn_s_setprio(CUP2D_CHAIN_PRIO);
-}
-
 // std::max / std::min semantics ((a<b)?b:a), so NaN handling and signed
 // zeros follow the reference's calls exactly.
 __host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
 __host__ __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
+// The maximum of the findMaxU reductions: equal to dmax for numbers, but a
+// NaN in either argument is kept, so a non-finite velocity reaches maxU
+// (where the step raises CUP2D_ERR_NONFINITE, the role of the reference's
+// Checker, Helpers.cpp:178-288) instead of being dropped.
+__host__ __device__ inline double nmax(double a, double b) {
+  return (a < b || b != b) ? b : a;
+}
+// maxU is finite (not NaN, not +inf)
+__host__ __device__ inline bool finite_max(double m) { return m == m && m <= 1.7976931348623157e308; }
 
 // sin and cos on the body path: Cody-Waite reduction by pi/2 (three-part
 // constant, the first two parts with 33 significant bits so k*P1 and k*P2 are
@@ -117,7 +117,7 @@ __device__ inline double block_sum(double v, double* lds) {
 template <int NT>
 __device__ inline double block_max(double v, double* lds) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o, 64));
   constexpr int NW = NT / 64;
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   __syncthreads();
@@ -125,7 +125,7 @@ __device__ inline double block_max(double v, double* lds) {
   __syncthreads();
   double s = lds[0];
 #pragma unroll
-  for (int i = 1; i < NW; ++i) s = dmax(s, lds[i]);
+  for (int i = 1; i < NW; ++i) s = nmax(s, lds[i]);
   return s;
 }
 
@@ -144,13 +144,13 @@ __device__ inline double block_sum_dyn(double v, double* lds) {
 }
 __device__ inline double block_max_dyn(double v, double* lds) {
   const int nt = blockDim.x, W = nt < 64 ? nt : 64;
-  for (int o = W / 2; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, W));
+  for (int o = W / 2; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o, W));
   const int NW = (nt + 63) / 64, w = threadIdx.x / 64, l = threadIdx.x % 64;
   __syncthreads();
   if (l == 0) lds[w] = v;
   __syncthreads();
   double s = lds[0];
-  for (int i = 1; i < NW; ++i) s = dmax(s, lds[i]);
+  for (int i = 1; i < NW; ++i) s = nmax(s, lds[i]);
   return s;
 }
 
@@ -260,7 +260,7 @@ __device__ inline double block_sum_array(const double* a, int n, int stride, dou
 }
 
 // Max of n non-negative values with U loads in flight per thread (max is
-// independent of the order).
+// independent of the order; a NaN is kept).
 template <int NT, int U = 8>
 __device__ inline double max_array(const double* a, int n) {
   double m = 0;
@@ -270,9 +270,9 @@ __device__ inline double max_array(const double* a, int n) {
 #pragma unroll
     for (int r = 0; r < U; ++r) v[r] = a[i + r * NT];
 #pragma unroll
-    for (int r = 0; r < U; ++r) m = dmax(m, v[r]);
+    for (int r = 0; r < U; ++r) m = nmax(m, v[r]);
   }
-  for (; i < n; i += NT) m = dmax(m, a[i]);
+  for (; i < n; i += NT) m = nmax(m, a[i]);
   return m;
 }
 

@@ -139,16 +139,8 @@ __device__ __forceinline__ void dft(double2* v) {
 }
 
 // Elements per thread of the FFT core (the first pass is radix E): 16 gives
-// fewer LDS round trips, 8 twice the threads per transform.
-#ifndef CUP2D_FFT_E
-#define CUP2D_FFT_E 16
-#endif
-constexpr int FE = CUP2D_FFT_E;
-// minimum resident workgroups per CU requested from the compiler (caps VGPRs)
-#ifndef CUP2D_FFT_MINB
-#define CUP2D_FFT_MINB 1
-#endif
-static_assert(FE == 8 || FE == 16, "FFT elements per thread");
+// fewer LDS round trips than 8 (measured: 8 was slower on every length).
+constexpr int FE = 16;
 
 // LDS index with one pad slot every E entries (stride-E stores hit distinct banks)
 template <int E>
@@ -320,7 +312,7 @@ struct RowSource {
 };
 
 template <int L, int NP, int E, int SRC>
-__global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
+__global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
     const double* in, int in_pitch, int ncols, int nrows, double2* __restrict__ spec,
     int spec_rows, const double2* __restrict__ tw, RowSource fd) {
   constexpr int NT = L / E, PADL = lds_len<L, E>();
@@ -467,7 +459,7 @@ __global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_fwd(
 
 // ---------------------------------------------------------- rows inverse
 template <int L, int NP, int E>
-__global__ __launch_bounds__(NP * L / E, CUP2D_FFT_MINB) void k_rows_inv(const double2* __restrict__ spec,
+__global__ __launch_bounds__(NP * L / E) void k_rows_inv(const double2* __restrict__ spec,
                                                          int spec_rows, double* __restrict__ out,
                                                          int out_pitch, int ncols, int nrows,
                                                          const double2* __restrict__ tw) {
@@ -576,14 +568,11 @@ __device__ __forceinline__ double col_symbol(int kx, int cg, int c, int ky, cons
   }
 }
 
-// Columns per workgroup (CW) of a column group of G: CW = G keeps a whole
-// group per workgroup; CW = G/2 halves the LDS so two workgroups share a CU.
-// The two halves of a group run on the same XCD (block b runs on XCD b % 8)
-// so the second half of each 64 B chunk is an L2 hit.
-#ifndef CUP2D_FFT_CW
-#define CUP2D_FFT_CW 2
-#endif
-constexpr int CW = CUP2D_FFT_CW;
+// Columns per workgroup (CW) of a column group of G: CW = G/2 halves the LDS
+// of a whole group so two workgroups share a CU.  The two halves of a group
+// run on the same XCD (block b runs on XCD b % 8) so the second half of each
+// 64 B chunk is an L2 hit.
+constexpr int CW = 2;
 static_assert(G % CW == 0, "CW");
 
 __device__ __forceinline__ void col_group_of_block(int b, int nblocks, int* cg, int* half) {
@@ -601,7 +590,7 @@ __device__ __forceinline__ void col_group_of_block(int b, int nblocks, int* cg, 
 }
 
 template <int LC, int MODE, int E>
-__global__ __launch_bounds__(CW * LC / E, CUP2D_FFT_MINB) void k_cols(double2* __restrict__ spec, ColParams P,
+__global__ __launch_bounds__(CW * LC / E) void k_cols(double2* __restrict__ spec, ColParams P,
                                                       const double2* __restrict__ tw) {
   constexpr int NT = LC / E;
   extern __shared__ double2 lds[];
@@ -841,7 +830,7 @@ int fft_create(cup2d_ctx* c) {
   if (!fs) return CUP2D_OK;
 
   // Free-space Green's function on the 2NX x 2NY cyclic grid
-  // (FFTW_freespace.cpp:321-358; with 2N padding the first NX x NY outputs of
+  // (FFTW_freespace.cpp:100-137; with 2N padding the first NX x NY outputs of
   // the cyclic convolution equal the reference's (2N-1) result).
   const int MX = 2 * NX, MY = 2 * NY;
   std::vector<double> Gk((size_t)MX * MY);

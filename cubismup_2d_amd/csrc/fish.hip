@@ -23,7 +23,7 @@ namespace {
 constexpr int NT = 256;
 constexpr double EPSD = 2.220446049250313e-16;  // std::numeric_limits<double>::epsilon()
 
-// AmplitudeFish::midlineLatPos / midlineLatVel (CarlingFish.cpp:300-308), or
+// AmplitudeFish::midlineLatPos / midlineLatVel (CarlingFish.cpp:19-27), or
 // the envelope L (a0 + a2 (s/L)^2) with wavelength waveLength*L.
 __device__ inline double lat_pos(const FishDev& F, double s, double t) {
   if (F.envelope == 0) {
@@ -97,7 +97,7 @@ __device__ inline void lane_tree(double* red) {
 }  // namespace
 
 // Fish::create steps 1-3 (Fish.cpp:78-95), first part, one workgroup per
-// swimmer: AmplitudeFish::computeMidline (CarlingFish.cpp:332-369),
+// swimmer: AmplitudeFish::computeMidline (CarlingFish.cpp:51-88),
 // integrateLinearMomentum, changeToCoMFrameLinear, integrateAngularMomentum
 // and the angular-velocity correction of changeToCoMFrameAngular
 // (FishLibrary.cpp:86-176), i.e. everything that depends only on the time.
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(NT) void k_fish_mid_a(const FishDev* __restrict__ f
   }
 }
 
-// Fish::updatePosition's theta_internal update (Fish.cpp:220-225) and
+// Fish::updatePosition's theta_internal update (Fish.cpp:180-185) and
 // Fish::create steps 3-5, second part, one workgroup per swimmer, after the
 // bodies have moved: rotation by theta_internal (changeToCoMFrameAngular),
 // _computeMidlineNormals (FishLibrary.cpp:66-84), and the AreaSegment boxes
@@ -250,7 +250,6 @@ __global__ __launch_bounds__(NT) void k_fish_mid_b(const FishDev* __restrict__ f
                                                    const cup2d_shape_state* __restrict__ ss,
                                                    const StepState* __restrict__ st, double h,
                                                    int buf) {
-  chain_prio();
   extern __shared__ double lds[];
   const FishDev F = fd[blockIdx.x];
   const int Nm = F.Nm, t = threadIdx.x;
@@ -493,7 +492,6 @@ __global__ __launch_bounds__(NT) void k_fish_place(Grid g, const ShapeParams* __
                                                    const cup2d_shape_state* __restrict__ ss,
                                                    const FishDev* __restrict__ fd,
                                                    Obstacles ob) {
-  chain_prio();
   __shared__ int sl[FISH_MAX_SEG];
   __shared__ int s_tmp[NT];
   __shared__ unsigned long long s_best[BS * BS];
@@ -749,7 +747,7 @@ __global__ __launch_bounds__(NT) void k_fish_place(Grid g, const ShapeParams* __
 
 // ------------------------------------------------------------------- host
 // FishData::FishData (FishLibrary.cpp:14-38, FishLibrary.h:44-60) and the
-// half-width table (AmplitudeFish::_width, CarlingFish.cpp:322-329, or the
+// half-width table (AmplitudeFish::_width, CarlingFish.cpp:41-48, or the
 // NACA 4-digit half-thickness 5 t L (0.2969 sqrt(x) - 0.1260 x - 0.3516 x^2 +
 // 0.2843 x^3 - 0.1036 x^4)).
 int fish_discretize(const cup2d_shape_desc* d, double h, std::vector<double>& rS,

@@ -37,7 +37,7 @@ struct ShapeParams {
   double appliedForceX, appliedForceY, appliedTorque;
 };
 
-// Step scalars held on the device (SimulationData.h:346-368).
+// Step scalars held on the device (SimulationData.h:41-70).
 struct StepState {
   double time, dt;
   double uinf[2];
@@ -53,8 +53,17 @@ struct StepState {
 // Body state captured by the body update for ComputeForces, which may run on
 // the side stream while the next step already moves the bodies.
 struct ForceSnap {
-  double C[2], u, v, omega, pad_;
+  double C[2], u, v, omega;
+  double time;  // sim.time of the step (the step end may run before ComputeForces)
+  int step, pad_;
 };
+
+// Per-step force records (the columns of forceValues_<id>.dat and
+// powerValues_<id>.dat, Shape.cpp:305-328): time step Fx Fy FxPres FyPres
+// FxVisc FyVisc tau tauPres tauVisc drag thrust perimeter circulation
+// Pthrust Pdrag PoutBnd Pout defPowerBnd defPower EffPDefBnd EffPDef
+// n_surface_points.
+constexpr int FLOG_REC = CUP2D_FORCE_LOG_REC;
 
 // Constant run parameters passed by value to kernels.
 struct Grid {
@@ -82,7 +91,7 @@ struct FishDev {
   double* wk;     // [8][Nm] working arrays of k_fish_mid_a
 };
 
-// Obstacle pool (ObstacleBlock.h:266-378, one slot per (shape, block)).
+// Obstacle pool (ObstacleBlock.h:25-137, one slot per (shape, block)).
 struct Obstacles {
   int cap;            // slot capacity
   int* slot_shape;    // [cap]
@@ -154,6 +163,11 @@ struct cup2d_ctx {
   bool midA_ready = false;
   int host_step = 0;                       // step index for the CFL ramp (Simulation.cpp:264)
   cup2d::ForceSnap* fsnap = nullptr;       // device [MAX_SHAPES]
+  // device ring of force records: [flog_cap][nshapes][FLOG_REC], filled by
+  // every ComputeForces in order; flog_cnt counts the records written
+  double* flog = nullptr;
+  int flog_cap = 0;
+  unsigned long long* flog_cnt = nullptr;
   int device = 0;
   std::string err;
 

@@ -125,7 +125,6 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
                                                    const double* __restrict__ ramp,
                                                    int compute_dt, int step_idx, int phase,
                                                    int end_prev) {
-  chain_prio();
   __shared__ int s_a[LCAP], s_b[LCAP], s_c[LCAP], s_d[LCAP];
   __shared__ int s_tmp[NT];
   __shared__ int s_cstart[MAX_SHAPES + 1];
@@ -191,7 +190,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
     // the partials of the previous step's obstacle blocks follow the
     // projection's (max does not depend on the order)
     if (compute_dt)
-      for (int i = nmaxpart + t; i < nmaxpart + nub_prev; i += NT) mA = dmax(mA, maxpart[i]);
+      for (int i = nmaxpart + t; i < nmaxpart + nub_prev; i += NT) mA = nmax(mA, maxpart[i]);
     __syncthreads();
     if (compute_dt) {
       const double maxU = block_max<NT>(mA, lds);
@@ -207,6 +206,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
         if (step_idx < 100) dt = sRamp * mn;
         st->dt = dt;
         st->maxU = maxU;
+        if (!finite_max(maxU) && st->error == 0) st->error = CUP2D_ERR_NONFINITE;
         s_step[0] = dt;
       }
     } else if (t == 0) {
@@ -478,7 +478,7 @@ __device__ inline double grid_sdf(const Grid& g, const DiskLds& D, const Obstacl
 // Disk::create (ShapeLibrary.cpp:21-34: dist = R - |x - c|, rho = rhoS) and
 // putChiOnGrid (PutObjectsOnGrid.cpp:17-116): Towers chi from the shape's own
 // distance inside the block and the grid SDF across block edges, surface
-// delta + normals (ObstacleBlock::write, ObstacleBlock.h:336-346),
+// delta + normals (ObstacleBlock::write, ObstacleBlock.h:95-105),
 // centre-of-mass partials, grid CHI = max and the invRho blend
 // (PutObjectsOnGrid.cpp:69-73).  Distances are evaluated where they are
 // needed, so no grid-wide SDF pass is required.  The next workgroups, one per
@@ -497,7 +497,6 @@ __global__ __launch_bounds__(NT) void k_obst_place(Grid g, const ShapeParams* __
                                                    double* __restrict__ udx,
                                                    double* __restrict__ udy,
                                                    double* __restrict__ invrho) {
-  chain_prio();
   __shared__ double lds[LDS_RED];
   __shared__ int s_nbs[4][MAX_FISH];  // swimmer slots of the E, W, N, S neighbour blocks
   __shared__ int s_nbn[4];
@@ -670,7 +669,6 @@ __device__ void obst_moments_wg(const Grid& g, const cup2d_shape_state* __restri
 __global__ __launch_bounds__(NT) void k_obst_moments(Grid g, const cup2d_shape_state* __restrict__ ss,
                                                      const StepState* __restrict__ st,
                                                      Obstacles ob) {
-  chain_prio();
   __shared__ double lds[LDS_RED];
   if ((int)blockIdx.x < ob.cnt[0]) obst_moments_wg(g, ss, ob, lds);
 }
@@ -711,7 +709,6 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __r
                                                   const double* __restrict__ chi,
                                                   double* __restrict__ udx,
                                                   double* __restrict__ udy) {
-  chain_prio();
   const int nub = ob.cnt[1];
   if ((int)blockIdx.x < nub) {
     const int b = ob.ub_block[blockIdx.x];
@@ -776,7 +773,6 @@ __device__ void obst_commit_wg(const Grid& g, cup2d_shape_state* __restrict__ ss
 
 __global__ __launch_bounds__(NT) void k_obst_commit(Grid g, cup2d_shape_state* __restrict__ ss,
                                                     Obstacles ob) {
-  chain_prio();
   obst_commit_wg(g, ss, ob);
 }
 
@@ -786,31 +782,15 @@ static int grid_cap(cup2d_ctx* c) { return c->ob.cap > 0 ? c->ob.cap : 1; }
 // overlap: run everything after the list builder on the side stream; the
 // caller joins with join_obstacles() before the next reader of obstacle data.
 static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, int end_step,
-                         const double* u, const double* v);
+                         const double* u, const double* v, int deferred);
 
-// ComputeForces of a whole step: 1 = behind the next step's list builder on
-// the side stream, 0 = on the side stream behind an event after the
-// penalization
-#ifndef CUP2D_FORCES_DEFER
-#define CUP2D_FORCES_DEFER 1
-#endif
-
-// How much of the placement chain runs on the main stream before advDiff
-// (stages of enqueue_put_objects: midline + list builder, PutFishOnBlocks,
-// placement, moments, removeMoments, commit), with and without swimmers.
+// How much of the placement chain runs on the main stream before advDiff when
+// the chain runs beside it on the side stream (stages of enqueue_put_objects:
+// midline + list builder, PutFishOnBlocks, placement, moments, removeMoments,
+// commit): with swimmers the midline and list builder, with disks nothing.
 // Beside advDiff, which saturates HBM, these latency-bound kernels run 2-3x
 // slower than alone; the split balances that against the main stream's wait.
-#ifndef CUP2D_SPLIT_FISH
-#define CUP2D_SPLIT_FISH 1
-#endif
-// chain on the main stream: advDiff starts after the dt/motion phase (0) or
-// after the midline rotation and the list phase (1)
-#ifndef CUP2D_ADV_AFTER_LISTS
-#define CUP2D_ADV_AFTER_LISTS 0
-#endif
-#ifndef CUP2D_SPLIT_DISKS
-#define CUP2D_SPLIT_DISKS 0
-#endif
+constexpr int SPLIT_FISH = 1, SPLIT_DISKS = 0;
 
 int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
   const int G = grid_cap(c);
@@ -847,11 +827,11 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     c->obst_pending = true;
     return CUP2D_OK;
   };
-  if (chain_main && !CUP2D_ADV_AFTER_LISTS) RC_OB(launch_side());
+  if (chain_main) RC_OB(launch_side());
   // The chain's stages in order; the first `split` of them run on the main
   // stream before advDiff, the rest on the side stream beside it.
   enum { ST_MIDLINE, ST_FISH_PLACE, ST_PLACE, ST_MOMENTS, ST_UDEF, ST_COMMIT, ST_N };
-  const int split = !overlap || chain_main ? ST_N : (fish ? CUP2D_SPLIT_FISH : CUP2D_SPLIT_DISKS);
+  const int split = !overlap || chain_main ? ST_N : (fish ? SPLIT_FISH : SPLIT_DISKS);
   bool on_aux = false;
   auto stream_for = [&](int stage) -> hipStream_t {
     if (stage < split) return c->stream;
@@ -874,7 +854,6 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
                        c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp, 0,
                        c->host_step, 2, 0);
     if (chain_main && mid_a_here) CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_lists, c->stream));
-    if (chain_main && CUP2D_ADV_AFTER_LISTS) RC_OB(launch_side());
     if (!(s = stream_for(ST_FISH_PLACE))) return CUP2D_ERR_HIP;
     RC_OB(enqueue_fish_place(c, s));
   }
@@ -913,7 +892,7 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     // commit also sets and the step's own ComputeForces sets last.
     RC_OB(enqueue_fish_mid_next(c, c->host_step & 1, mid_a_here ? c->ev_lists : c->ev_begin,
                                 false));
-    if (defer_forces) RC_OB(launch_forces(c, c->mids, c->ob_prev, 0, u_prev, v_prev));
+    if (defer_forces) RC_OB(launch_forces(c, c->mids, c->ob_prev, 0, u_prev, v_prev, 1));
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_frc, c->mids));
     c->frc_join = true;
     return launch_check(c, "put_objects");
@@ -930,7 +909,7 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     // next step's midline kernel after it
     RC_OB(enqueue_fish_mid_next(c, c->host_step & 1, c->ev_begin, true));
     if (defer_forces)  // the previous step's ComputeForces, on its own lists
-      RC_OB(launch_forces(c, c->aux, c->ob_prev, 0, u_prev, v_prev));
+      RC_OB(launch_forces(c, c->aux, c->ob_prev, 0, u_prev, v_prev, 1));
     if (defer_forces || !c->hfish.empty()) {
       CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_frc, c->aux));
       c->frc_join = true;
@@ -1291,7 +1270,8 @@ __device__ void body_update_tail(const Grid& g, const ShapeParams* __restrict__ 
     const cup2d_shape_state& T = ss[threadIdx.x];
     ForceSnap f;
     f.C[0] = T.centerOfMass[0]; f.C[1] = T.centerOfMass[1];
-    f.u = T.u; f.v = T.v; f.omega = T.omega; f.pad_ = 0;
+    f.u = T.u; f.v = T.v; f.omega = T.omega;
+    f.time = st->time; f.step = st->step; f.pad_ = 0;
     snap[threadIdx.x] = f;
   }
   if (threadIdx.x == 0) st->dt_done = st->dt;
@@ -1374,7 +1354,8 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
         const int k = threadIdx.x;
         ForceSnap f;
         f.C[0] = ss[k].centerOfMass[0]; f.C[1] = ss[k].centerOfMass[1];
-        f.u = s_uvw[k][0]; f.v = s_uvw[k][1]; f.omega = s_uvw[k][2]; f.pad_ = 0;
+        f.u = s_uvw[k][0]; f.v = s_uvw[k][1]; f.omega = s_uvw[k][2];
+        f.time = st->time; f.step = st->step; f.pad_ = 0;
         snap[k] = f;
       }
       if (threadIdx.x == 0) st->dt_done = dt;
@@ -1425,10 +1406,10 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const double a = uR[c], w = vR[c];
-    m = dmax(m, fabs(a + UINF));
-    m = dmax(m, fabs(w + VINF));
-    m = dmax(m, fabs(a));
-    m = dmax(m, fabs(w));
+    m = nmax(m, fabs(a + UINF));
+    m = nmax(m, fabs(w + VINF));
+    m = nmax(m, fabs(a));
+    m = nmax(m, fabs(w));
   }
   m = block_max<NT>(m, lds);
   if (threadIdx.x == 0) maxpart[blockIdx.x] = m;
@@ -1584,9 +1565,19 @@ int enqueue_iter_penalize(cup2d_ctx* c, double* reldf) {
 // ----------------------------------------------------------- ComputeForces
 // Shape::computeForces (Shape.cpp:252-281): per-block partials added in
 // block order; optionally ends the step (time += dt, step++; Simulation.cpp:289-290).
+// With a force log, each shape's record of this ComputeForces goes to the
+// ring (the reference appends it to its .dat logs at every step); deferred:
+// the record's time and step are those of the snapshot, since the next
+// step's list builder may already have ended this step.
+struct ForceLog {
+  double* rec;               // [cap][nshapes][FLOG_REC], nullptr: off
+  unsigned long long* cnt;   // records written so far
+  int cap;
+};
 __device__ void forces_fin_wg(const Grid& g, cup2d_shape_state* __restrict__ ss,
                               const Obstacles& ob, StepState* __restrict__ st, int end_step,
-                              const ForceSnap* __restrict__ snap) {
+                              const ForceSnap* __restrict__ snap, ForceLog flog, int deferred) {
+  const unsigned long long nrec = flog.rec ? *flog.cnt : 0ull;
   for (int k = threadIdx.x / 64; k < g.nshapes; k += NT / 64) {  // a shape per wave
     double a[18];
     shape_sums<18>(ob, k, P_FRC, a);
@@ -1607,7 +1598,20 @@ __device__ void forces_fin_wg(const Grid& g, cup2d_shape_state* __restrict__ ss,
     const double demBnd = T.Pthrust - T.defPowerBnd;
     T.EffPDef = T.Pthrust / dmax(denUnb, EPSD);
     T.EffPDefBnd = T.Pthrust / dmax(demBnd, EPSD);
+    if (flog.rec) {
+      double* r = flog.rec + ((nrec % flog.cap) * g.nshapes + k) * FLOG_REC;
+      r[0] = deferred ? snap[k].time : st->time;
+      r[1] = deferred ? snap[k].step : st->step;
+      r[2] = T.forcex; r[3] = T.forcey; r[4] = T.forcex_P; r[5] = T.forcey_P;
+      r[6] = T.forcex_V; r[7] = T.forcey_V; r[8] = T.torque; r[9] = T.torque_P;
+      r[10] = T.torque_V; r[11] = T.drag; r[12] = T.thrust; r[13] = T.perimeter;
+      r[14] = T.circulation; r[15] = T.Pthrust; r[16] = T.Pdrag; r[17] = T.PoutBnd;
+      r[18] = T.Pout; r[19] = T.defPowerBnd; r[20] = T.defPower; r[21] = T.EffPDefBnd;
+      r[22] = T.EffPDef; r[23] = T.n_surface_points;
+    }
   }
+  __syncthreads();  // every wave has read the count
+  if (flog.rec && threadIdx.x == 0) *flog.cnt = nrec + 1;
   if (end_step && threadIdx.x == 0) {
     st->time += st->dt_done;
     st->step += 1;
@@ -1690,17 +1694,19 @@ __global__ __launch_bounds__(NT) void k_forces(Grid g, const ForceSnap* __restri
 // the step end
 __global__ __launch_bounds__(NT) void k_forces_fin(Grid g, cup2d_shape_state* __restrict__ ss,
                                                    Obstacles ob, StepState* __restrict__ st,
-                                                   int end_step, const ForceSnap* __restrict__ snap) {
-  forces_fin_wg(g, ss, ob, st, end_step, snap);
+                                                   int end_step, const ForceSnap* __restrict__ snap,
+                                                   ForceLog flog, int deferred) {
+  forces_fin_wg(g, ss, ob, st, end_step, snap, flog, deferred);
 }
 
 static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, int end_step,
-                         const double* u, const double* v) {
+                         const double* u, const double* v, int deferred) {
   TimerScope ts(c, T_FORCES, s);
   hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, s, c->g, c->fsnap, lists, u, v,
                      c->pres);
+  const ForceLog flog{c->flog, c->flog_cnt, c->flog_cap};
   hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(NT), 0, s, c->g, c->ss, lists, c->st, end_step,
-                     c->fsnap);
+                     c->fsnap, flog, deferred);
   return launch_check(c, "forces");
 }
 
@@ -1717,24 +1723,17 @@ int enqueue_forces(cup2d_ctx* c, int end_step, int on_aux) {
     return CUP2D_OK;
   }
   if (on_aux && end_step) {
-#if CUP2D_FORCES_DEFER
     c->forces_pending = true;
     c->end_pending = true;
     return CUP2D_OK;
-#else
-    // on the side stream right after the penalization (one more event)
-    CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_pen, c->stream));
-    CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_pen, 0));
-    return launch_forces(c, c->aux, c->ob, end_step, c->u, c->v);
-#endif
   }
-  return launch_forces(c, c->stream, c->ob, end_step, c->u, c->v);
+  return launch_forces(c, c->stream, c->ob, end_step, c->u, c->v, 0);
 }
 
 int flush_forces(cup2d_ctx* c) {
   if (c->forces_pending) {
     c->forces_pending = false;
-    RC_OB(launch_forces(c, c->stream, c->ob, 0, c->u, c->v));
+    RC_OB(launch_forces(c, c->stream, c->ob, 0, c->u, c->v, 1));
   }
   if (c->end_pending) {
     c->end_pending = false;
@@ -1752,7 +1751,7 @@ int join_forces(cup2d_ctx* c) {
 }
 
 // Shape::resetAll (Shape.h:64-88).
-// Fish::resetAll (Fish.cpp:227-234) clears the swimmers' internal frame.
+// Fish::resetAll (Fish.cpp:187-194) clears the swimmers' internal frame.
 __global__ void k_shapes_reset(Grid g, const ShapeParams* __restrict__ sp,
                                cup2d_shape_state* __restrict__ ss, const FishDev* __restrict__ fd) {
   const int k = threadIdx.x;

@@ -28,26 +28,18 @@ int launch_check(cup2d_ctx* c, const char* what) {
   return CUP2D_OK;
 }
 
-// Launch shape of the full-grid stencil kernels (compile-time tuning knobs):
-// rows per workgroup, widest column strip, unroll depth of the row march.
-#ifndef CUP2D_STENCIL_RY
-#define CUP2D_STENCIL_RY 8
-#endif
-#ifndef CUP2D_STENCIL_TX
-#define CUP2D_STENCIL_TX 256
-#endif
-#ifndef CUP2D_STENCIL_UNROLL
-#define CUP2D_STENCIL_UNROLL 4
-#endif
-#define CUP2D_PRAGMA(x) _Pragma(#x)
-#define CUP2D_UNROLL_ROWS CUP2D_PRAGMA(unroll CUP2D_STENCIL_UNROLL)
+// Launch shape of the full-grid stencil kernels (chosen by a sweep over
+// RY in {8,16,32}, TX in {128,256}, unroll in {2,4,8}): rows per workgroup,
+// widest column strip, unroll depth of the row march.
+constexpr int STENCIL_RY = 8, STENCIL_TX = 256;
+#define CUP2D_UNROLL_ROWS _Pragma("unroll 4")
 
 static inline int strip_width(int NX) {
-  for (int t = CUP2D_STENCIL_TX; t >= 32; t >>= 1)
+  for (int t = STENCIL_TX; t >= 32; t >>= 1)
     if (NX % t == 0) return t;
   return 32;
 }
-static constexpr int RYS_ = CUP2D_STENCIL_RY;
+static constexpr int RYS_ = STENCIL_RY;
 // workgroups of the stencil kernels (advDiff, RHS, projection share a grid)
 int grid_workgroups(int NX, int NY) { return (NX / strip_width(NX)) * (NY / RYS_); }
 
@@ -64,10 +56,10 @@ __global__ __launch_bounds__(256) void k_find_max_u(const double* __restrict__ u
   double m = 0;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
     const double a = u[i], b = v[i];
-    m = dmax(m, fabs(a + UINF));
-    m = dmax(m, fabs(b + VINF));
-    m = dmax(m, fabs(a));
-    m = dmax(m, fabs(b));
+    m = nmax(m, fabs(a + UINF));
+    m = nmax(m, fabs(b + VINF));
+    m = nmax(m, fabs(a));
+    m = nmax(m, fabs(b));
   }
   m = block_max<256>(m, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = m;
@@ -85,6 +77,7 @@ __global__ __launch_bounds__(256) void k_dt(const double* __restrict__ part, int
   const double maxU = block_max<256>(max_array<256>(part, nparts), lds);
   if (threadIdx.x == 0) {
     st->maxU = maxU;
+    if (!finite_max(maxU) && st->error == 0) st->error = CUP2D_ERR_NONFINITE;
     if (set_dt) {
       const double h = g.h;
       const double dtFourier = h * h / g.nu, dtCFL = maxU < 2.2e-16 ? 1 : h / maxU;
@@ -137,13 +130,10 @@ int enqueue_step_end(cup2d_ctx* c) {
 // launch runs on XCD b % 8; numbering the tiles so that each XCD gets one
 // contiguous band of tiles keeps the halo rows and columns that neighbouring
 // tiles share inside one L2.
-#ifndef CUP2D_XCD_REMAP
-#define CUP2D_XCD_REMAP 1
-#endif
 __device__ inline void stencil_tile(int* bx, int* by) {
   const int gx = gridDim.x, n = gx * gridDim.y;
   int id = blockIdx.x + gx * blockIdx.y;
-  if (CUP2D_XCD_REMAP && n % 8 == 0) id = (id % 8) * (n / 8) + id / 8;
+  if (n % 8 == 0) id = (id % 8) * (n / 8) + id / 8;
   *bx = id % gx;
   *by = id / gx;
 }
@@ -243,7 +233,7 @@ __global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
   __shared__ double lds[8];
   const int n = gx * gy;
   for (int id = blockIdx.x; id < n; id += gridDim.x) {
-    const int t = (CUP2D_XCD_REMAP && n % 8 == 0) ? (id % 8) * (n / 8) + id / 8 : id;
+    const int t = (n % 8 == 0) ? (id % 8) * (n / 8) + id / 8 : id;
     advdiff_strip<RY>(u, v, uo, vo, g, st, fluxpart, t % gx, t / gx, gx, gy, lds);
   }
 }
@@ -271,11 +261,7 @@ __global__ __launch_bounds__(256) void k_flux_fix(double* __restrict__ u,
   }
 }
 
-static constexpr int RYS = CUP2D_STENCIL_RY;  // rows per workgroup of the stencil kernels
-// at most this many advDiff workgroups (a multiple of 8; 0: one per strip)
-#ifndef CUP2D_ADVDIFF_WG_CAP
-#define CUP2D_ADVDIFF_WG_CAP 0
-#endif
+static constexpr int RYS = STENCIL_RY;  // rows per workgroup of the stencil kernels
 
 int enqueue_advdiff(cup2d_ctx* c, int defer_fix) {
   return enqueue_advdiff_on(c, defer_fix, c->stream);
@@ -285,9 +271,7 @@ int enqueue_advdiff_on(cup2d_ctx* c, int defer_fix, hipStream_t s) {
   const int tx = strip_width(c->g.NX);
   const int gx = c->g.NX / tx, gy = c->g.NY / RYS;
   {
-    int nwg = gx * gy;
-    if (CUP2D_ADVDIFF_WG_CAP > 0 && nwg > CUP2D_ADVDIFF_WG_CAP && nwg % 8 == 0)
-      nwg = CUP2D_ADVDIFF_WG_CAP;
+    const int nwg = gx * gy;
     KernelTimer kt(c, T_ADVDIFF);
     CUP2D_TIMED_LAUNCH(c, k_advdiff<RYS>, dim3(nwg), dim3(tx), 0, s, c->u, c->v, c->tu,
                        c->tv, c->g, (const StepState*)c->st, c->fluxpart, gx, gy);
@@ -528,10 +512,10 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
     const size_t idx = i0 + (size_t)iy * NX;
     u[idx] = u1;
     v[idx] = v1;
-    double mc = dmax(m, fabs(u1 + UINF));
-    mc = dmax(mc, fabs(v1 + VINF));
-    mc = dmax(mc, fabs(u1));
-    mc = dmax(mc, fabs(v1));
+    double mc = nmax(m, fabs(u1 + UINF));
+    mc = nmax(mc, fabs(v1 + VINF));
+    mc = nmax(mc, fabs(u1));
+    mc = nmax(mc, fabs(v1));
     m = count ? mc : m;
     ps = pc;
     pc = pn;

@@ -111,7 +111,7 @@ Shape parseShape(const std::string& line_in, const double extents[2]) {
   d.radius = p.asDouble("radius", 0.1);
   d.tAccel = p.asDouble("tAccel", -1);
   if (d.kind == 1) {
-    // Fish::Fish and CarlingFish::CarlingFish (Fish.h:27-29, CarlingFish.cpp:371-376)
+    // Fish::Fish and CarlingFish::CarlingFish (Fish.h:27-29, CarlingFish.cpp:90-95)
     d.length = p.asDouble("L", 0.1);
     d.Tperiod = p.asDouble("T", 1);
     d.phaseShift = p.asDouble("phi", 0);

@@ -1,4 +1,4 @@
-// Simulation.h -- the reference's Simulation driver (Simulation.h:16-41,
+// Simulation.h -- the reference's Simulation driver (Simulation.h:16-40,
 // Simulation.cpp:49-305) over the device operators.
 #pragma once
 

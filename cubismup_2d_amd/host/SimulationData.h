@@ -1,6 +1,6 @@
 // SimulationData.h -- host-side state of one simulation on the GPU.
 //
-// Mirrors the members of SimulationData (SimulationData.h:14-120) that the
+// Mirrors the members of SimulationData (SimulationData.h:17-116) that the
 // per-timestep operators read: grid size, extent, nu, CFL, lambda, fadeLen,
 // poissonType, time, dt, step, U_inf and the shapes.  The grids themselves
 // live on the device inside the cup2d_ctx; `vel`, `chi`, `pres`, `tmp`,
@@ -59,7 +59,7 @@ class SimulationData {
   int verbose = 1, iterativePenalization = 0, muteAll = 0;
   std::string path2file = "./";  // -file: directory of the text logs
 
-  // state (SimulationData.h:346-350)
+  // state (SimulationData.h:41-45)
   double time = 0, dt = 0, uinfx = 0, uinfy = 0;
   int step = 0;
 

@@ -67,7 +67,7 @@ typedef struct {
 
 /* Shape description: the keys parsed by Shape::Shape, Disk::Disk and
  * Fish / CarlingFish (Shape.cpp:331-345, ShapesSimple.h:19-21, Fish.h:27-29,
- * CarlingFish.cpp:371-376), defaults already resolved. */
+ * CarlingFish.cpp:90-95), defaults already resolved. */
 typedef struct {
   int kind;              /* 0 = disk, 1 = carlingfish                        */
   double radius, xpos, ypos, angle_deg, rhoS;
@@ -75,9 +75,9 @@ typedef struct {
   double xvel, yvel, angvel, timeForced, tAccel;
   /* carlingfish: -L -T -phi -amplitudeFactor and the midline / half-width
    * laws.  envelope 0 is AmplitudeFish's 4/33 (s + L/32) lateral amplitude
-   * with wavelength L (CarlingFish.cpp:300-308); envelope 1 is
+   * with wavelength L (CarlingFish.cpp:19-27); envelope 1 is
    * L (env_a0 + env_a2 (s/L)^2) with wavelength waveLength*L.  profile 0 is
-   * AmplitudeFish::_width (CarlingFish.cpp:322-329); profile 1 is the NACA
+   * AmplitudeFish::_width (CarlingFish.cpp:41-48); profile 1 is the NACA
    * 4-digit half-thickness with closed trailing edge, ratio naca_t. */
   double length, Tperiod, phaseShift, amplitudeFactor;
   int envelope;
@@ -102,7 +102,7 @@ typedef struct cup2d_ctx cup2d_ctx;
 /* --- lifetime ------------------------------------------------------------
  * Replaces SimulationData::allocateGrid (SimulationData.cpp:17-47) and the
  * solver constructors FFTW_periodic::FFTW_periodic (FFTW_periodic.h:99-130),
- * FFTW_freespace::FFTW_freespace (FFTW_freespace.cpp:299-359). */
+ * FFTW_freespace::FFTW_freespace (FFTW_freespace.cpp:78-138). */
 int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out);
 /* Replaces SimulationData::~SimulationData (SimulationData.cpp:150-170). */
 int cup2d_destroy(cup2d_ctx* ctx);
@@ -121,7 +121,7 @@ int cup2d_upload_field(cup2d_ctx* ctx, int field, int layout, const double* host
 int cup2d_download_field(cup2d_ctx* ctx, int field, int layout, double* host);
 /* Device pointer of one component plane (dense, row-major, pitch NX). */
 int cup2d_field_device_ptr(cup2d_ctx* ctx, int field, int comp, void** dptr);
-/* time, dt, step, U_inf (SimulationData.h:346-350) and grid spacing h. */
+/* time, dt, step, U_inf (SimulationData.h:41-45) and grid spacing h. */
 int cup2d_get_scalars(cup2d_ctx* ctx, double* time, double* dt, int* step, double uinf[2],
                       double* h);
 int cup2d_set_scalars(cup2d_ctx* ctx, double time, int step, double uinfx, double uinfy);
@@ -184,6 +184,20 @@ int cup2d_poisson_dense(cup2d_ctx* ctx, const double* rhs, double* sol);
 
 /* Cells per dimension of the grid. */
 int cup2d_grid_size(const cup2d_ctx* ctx, int* NX, int* NY);
+
+/* Per-step force log: the rows ComputeForces appends to forceValues_<id>.dat
+ * and powerValues_<id>.dat at every step (Shape.cpp:305-328), kept on the
+ * device so a device loop (cup2d_simulate) records every step.
+ * cup2d_force_log_enable keeps the last `capacity` records per shape
+ * (0 disables; call it after the shapes are added).  cup2d_force_log_read
+ * copies the min(max_records, held) most recent records of one shape, oldest
+ * first, CUP2D_FORCE_LOG_REC doubles each: time step Fx Fy FxPres FyPres FxVisc
+ * FyVisc tau tauPres tauVisc drag thrust perimeter circulation Pthrust Pdrag
+ * PoutBnd Pout defPowerBnd defPower EffPDefBnd EffPDef n_surface_points. */
+#define CUP2D_FORCE_LOG_REC 24
+int cup2d_force_log_enable(cup2d_ctx* ctx, int capacity);
+int cup2d_force_log_read(cup2d_ctx* ctx, int shape_id, int max_records, double* out,
+                         int* nrecords);
 
 /* Per-kernel timers: the role of SimulationData::startProfiler/stopProfiler
  * (SimulationData.cpp:187-205) without the Checker.  kernel is one of

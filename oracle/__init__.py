@@ -107,6 +107,7 @@ def lib():
         L.orc_pressure_iterator.argtypes = [P, D]
         L.orc_pressure_iterator.restype = C.c_int
         L.orc_iter_stats.argtypes = [P, C.POINTER(C.c_int), dp, dp]
+        L.orc_set_collisions.argtypes = [P, C.c_int]
         L.orc_fft1d.argtypes = [dp, C.c_int, C.c_int]
         L.orc_get_timing.argtypes = [P, dp, dp]
         L.orc_num_threads.restype = C.c_int
@@ -157,7 +158,7 @@ def carlingfish(xpos, ypos, L=0.1, T=1.0, phi=0.0, angle=0.0, amplitudeFactor=1.
                 profile="carling", naca_t=0.08, rhoS=1.0, bFixed=False, bForced=False,
                 bFixedx=None, bFixedy=None, bForcedx=None, bForcedy=None, bBlockAng=None,
                 xvel=0.0, yvel=0.0, angvel=0.0, timeForced=1.7976931348623157e308):
-    """CarlingFish descriptor (Fish.h:27-29, CarlingFish.cpp:371-376) with the
+    """CarlingFish descriptor (Fish.h:27-29, CarlingFish.cpp:90-95) with the
     reference's flag defaults (Shape.cpp:331-345).  envelope "carling" is
     4/33 (s + L/32); "quadratic" is L (env_a0 + env_a2 (s/L)^2) with wavelength
     waveLength*L.  profile "carling" is AmplitudeFish::_width, "naca" a NACA
@@ -304,6 +305,11 @@ class Oracle:
 
     def compute_forces(self):
         self.L.orc_compute_forces(self.h)
+
+    def set_collisions(self, enabled):
+        """Test hook: with enabled=False the body update skips
+        preventCollidingObstacles."""
+        self.L.orc_set_collisions(self.h, int(enabled))
 
     def advance(self, dt):
         return self.L.orc_advance(self.h, dt)

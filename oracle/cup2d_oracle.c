@@ -38,7 +38,7 @@ typedef struct {
   int nsurf;
   int sx[BS * BS], sy[BS * BS];
   double dchidx[BS * BS], dchidy[BS * BS], delta[BS * BS];
-  /* additive per-block quantities (ObstacleBlock.h:285-288) */
+  /* additive per-block quantities (ObstacleBlock.h:44-47) */
   double perimeter, forcex, forcey, forcex_P, forcey_P, forcex_V, forcey_V;
   double torque, torque_P, torque_V, drag, thrust, Pout, PoutBnd, defPower,
       defPowerBnd, circulation;
@@ -98,6 +98,7 @@ struct orc_sim {
   double iter_target;
   int iter_last;
   double iter_relDF;
+  int no_collide; /* test hook: skip preventCollidingObstacles */
 };
 
 static fish_t* fish_new(const orc_shape_desc* d, double h);
@@ -280,7 +281,7 @@ orc_sim* orc_create(const orc_cfg* cfg)
   s->buffer = (double*)calloc(s->stride * (size_t)s->MY, sizeof(double));
   fft2d_init(&s->fft, s->MX, s->MY, s->stride);
   if (cfg->poisson != 0) {
-    /* FFTW_freespace.cpp:321-358 */
+    /* FFTW_freespace.cpp:100-137 */
     const double h = s->h, fac = 1 / (2 * M_PI);
     const int totNx = s->NX, totNy = s->NY, MX = s->MX, MY = s->MY;
     #pragma omp parallel for schedule(static)
@@ -328,7 +329,7 @@ static void shape_reset(orc_sim* s, shape_t* sh)
   t->u = sh->forcedu; t->v = sh->forcedv; t->omega = sh->forcedomega;
   sh->appliedForceX = sh->appliedForceY = sh->appliedTorque = 0;
   for (int b = 0; b < s->nb; ++b) { free(sh->ob[b]); sh->ob[b] = NULL; }
-  if (sh->fish) { /* Fish::resetAll, Fish.cpp:227-234 */
+  if (sh->fish) { /* Fish::resetAll, Fish.cpp:187-194 */
     fish_t* f = sh->fish;
     f->CoM_internal[0] = f->CoM_internal[1] = 0;
     f->vCoM_internal[0] = f->vCoM_internal[1] = 0;
@@ -519,7 +520,7 @@ static void disk_create(orc_sim* s, shape_t* sh)
 }
 
 /* ---------------------------------------------------------------- swimmers */
-/* AmplitudeFish::_width (CarlingFish.cpp:322-329) or a NACA 4-digit
+/* AmplitudeFish::_width (CarlingFish.cpp:41-48) or a NACA 4-digit
  * symmetric half-thickness with closed trailing edge,
  * 5 t L (0.2969 sqrt(x) - 0.1260 x - 0.3516 x^2 + 0.2843 x^3 - 0.1036 x^4). */
 static double fish_width(const fish_t* f, double s, double L)
@@ -539,7 +540,7 @@ static double fish_width(const fish_t* f, double s, double L)
 }
 
 /* FishData::FishData (FishLibrary.cpp:14-38, FishLibrary.h:44-60) and
- * AmplitudeFish's width table (CarlingFish.cpp:318-319). */
+ * AmplitudeFish's width table (CarlingFish.cpp:37-38). */
 static fish_t* fish_new(const orc_shape_desc* d, double h)
 {
   const double length = d->length;
@@ -584,7 +585,7 @@ static void fish_free(fish_t* f)
   free(f->seg); free(f);
 }
 
-/* AmplitudeFish::midlineLatPos / midlineLatVel (CarlingFish.cpp:300-308), or
+/* AmplitudeFish::midlineLatPos / midlineLatVel (CarlingFish.cpp:19-27), or
  * the envelope L (a0 + a2 (s/L)^2) with wavelength waveLength*L. */
 static inline double lat_pos(const fish_t* f, double s, double t)
 {
@@ -607,7 +608,7 @@ static inline double lat_vel(const fish_t* f, double s, double t)
   return -f->length * (f->env_a0 + f->env_a2 * x * x) * (2 * M_PI / f->Tperiod) * det_cos(arg);
 }
 
-/* AmplitudeFish::computeMidline (CarlingFish.cpp:332-369) */
+/* AmplitudeFish::computeMidline (CarlingFish.cpp:51-88) */
 static void fish_midline(fish_t* f, double t)
 {
   const int Nm = f->Nm;
@@ -1144,7 +1145,7 @@ static void put_chi_on_grid(orc_sim* s, shape_t* sh)
             const double gradUSq = gradUX * gradUX + gradUY * gradUY;
             const double denum = 1 / (gradUSq < EPS ? EPS : gradUSq);
             const double D = fac * (gradHX * gradUX + gradHY * gradUY) * denum;
-            if (D > 0) { /* ObstacleBlock::write, ObstacleBlock.h:336-346 */
+            if (D > 0) { /* ObstacleBlock::write, ObstacleBlock.h:95-105 */
               const int k = o->nsurf++;
               o->sx[k] = ix; o->sy[k] = iy;
               o->dchidx[k] = -D * gradUX; o->dchidy[k] = -D * gradUY; o->delta[k] = D;
@@ -1256,7 +1257,7 @@ static void update_position(orc_sim* s, shape_t* sh, double dt)
   orc_det_sincos(t->orientation, &sinang, &cosang);
   t->center[0] = t->centerOfMass[0] + cosang * t->d_gm[0] - sinang * t->d_gm[1];
   t->center[1] = t->centerOfMass[1] + sinang * t->d_gm[0] + cosang * t->d_gm[1];
-  /* Fish::updatePosition, Fish.cpp:220-225 */
+  /* Fish::updatePosition, Fish.cpp:180-185 */
   if (sh->fish) sh->fish->theta_internal -= dt * sh->fish->angvel_internal;
 }
 
@@ -1562,7 +1563,7 @@ static void sol2cub(orc_sim* s)
 }
 
 /* spectral step: FFTW_periodic::_solve_spectral (FFTW_periodic.h:77-92) or
- * the Green's function product (FFTW_freespace.cpp:252-263) */
+ * the Green's function product (FFTW_freespace.cpp:31-42) */
 static void spectral(orc_sim* s)
 {
   double* io = s->buffer;
@@ -1820,8 +1821,10 @@ void orc_body_update(orc_sim* s, double dt)
     update_velocity(s, sh, dt);
   }
   free(part);
-  prevent_colliding_obstacles(s);
+  if (!s->no_collide) prevent_colliding_obstacles(s);
 }
+
+void orc_set_collisions(orc_sim* s, int enabled) { s->no_collide = !enabled; }
 
 /* Shape::updateVelocity (Shape.cpp:25-59) from the momenta in the state */
 static void update_velocity(orc_sim* s, shape_t* sh, double dt)

@@ -42,13 +42,13 @@ typedef struct {
   int bFixedx, bFixedy, bForcedx, bForcedy, bBlockAng;
   double xvel, yvel, angvel, timeForced, tAccel;
   /* carlingfish: -L -T -phi -amplitudeFactor (Fish.h:27-29,
-   * CarlingFish.cpp:371-376) and the midline / half-width laws */
+   * CarlingFish.cpp:90-95) and the midline / half-width laws */
   double length, Tperiod, phaseShift, amplitudeFactor;
-  int envelope;          /* 0 = 4/33 (s + L/32) (CarlingFish.cpp:300-308),
+  int envelope;          /* 0 = 4/33 (s + L/32) (CarlingFish.cpp:19-27),
                             1 = L (env_a0 + env_a2 (s/L)^2), wavelength
                             waveLength*L                                     */
   double env_a0, env_a2, waveLength;
-  int profile;           /* 0 = AmplitudeFish::_width (CarlingFish.cpp:322-329),
+  int profile;           /* 0 = AmplitudeFish::_width (CarlingFish.cpp:41-48),
                             1 = NACA 4-digit half-thickness, ratio naca_t    */
   double naca_t;
 } orc_shape_desc;
@@ -99,6 +99,9 @@ void   orc_pressure_single(orc_sim* s, double dt);
  * momentum change of the last call, and the adaptive tolerance after it. */
 int    orc_pressure_iterator(orc_sim* s, double dt);
 void   orc_iter_stats(const orc_sim* s, int* iters, double* relDF, double* target);
+/* Test hook: enabled = 0 skips preventCollidingObstacles in the body update,
+ * so a test can see the velocities the impulse starts from. */
+void   orc_set_collisions(orc_sim* s, int enabled);
 void   orc_compute_forces(orc_sim* s);
 int    orc_advance(orc_sim* s, double dt);    /* Simulation::advance         */
 int    orc_simulate(orc_sim* s, int nsteps);  /* nsteps x (dt + advance)     */

@@ -4,7 +4,7 @@ The reference ships no tests or golden vectors and cannot be built here, so
 the fixtures restate its algorithms with independent tools:
   periodic_poisson_128x64.npz : FFTW_periodic.h:77-92 spectral solve done with
                                 numpy.fft (pocketfft) on a seeded RHS;
-  freespace_direct_64x32.npz  : FFTW_freespace.cpp:321-358 Green's function,
+  freespace_direct_64x32.npz  : FFTW_freespace.cpp:100-137 Green's function,
                                 evaluated as a direct O(N^2) sum (no FFT);
   advdiff_96x64.npz           : advDiff.cpp:14-116 on seeded fields, written
                                 from the reference formulas in numpy.

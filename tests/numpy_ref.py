@@ -129,7 +129,7 @@ def poisson_periodic(r):
 
 
 def green(NX, NY, h):
-    """Free-space kernel G(dx, dy) on |dx|<NX, |dy|<NY (FFTW_freespace.cpp:329-340)."""
+    """Free-space kernel G(dx, dy) on |dx|<NX, |dy|<NY (FFTW_freespace.cpp:108-119)."""
     dx = np.arange(-(NX - 1), NX)[None, :]
     dy = np.arange(-(NY - 1), NY)[:, None]
     with np.errstate(divide="ignore"):

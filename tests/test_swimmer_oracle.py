@@ -68,7 +68,7 @@ def test_discretisation_and_width(oracle_mod, profile):
     assert abs(ds[0] - dSref) < 1e-12 and abs(ds[-1] - dSref) < 1e-12
     assert abs(fi["rS"][-1] - L) < 1e-12 and fi["rS"][-1] <= L
     s, w = fi["rS"], fi["width"]
-    if profile == "carling":  # CarlingFish.cpp:322-329
+    if profile == "carling":  # CarlingFish.cpp:41-48
         sb, st, wt, wh = .04 * L, .95 * L, .01 * L, .04 * L
         ref = np.where(s < sb, np.sqrt(np.maximum(2 * wh * s - s * s, 0)),
                        np.where(s < st, wh - (wh - wt) * (s - sb) / (st - sb), wt * (L - s) / (L - st)))
