@@ -269,11 +269,22 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       s_box[k][0] = Cx - R - safety; s_box[k][1] = Cx + R + safety;
       s_box[k][2] = Cy - R - safety; s_box[k][3] = Cy + R + safety;
     }
-    int bx0 = (int)floor(s_box[k][0] / g.hb) - 1, bx1 = (int)floor(s_box[k][1] / g.hb) + 1;
-    int by0 = (int)floor(s_box[k][2] / g.hb) - 1, by1 = (int)floor(s_box[k][3] / g.hb) + 1;
+    // a box that is not finite (a body moved by a non-finite velocity or dt;
+    // CUP2D_ERR_NONFINITE is raised by the dt part) selects no block: the
+    // block range is clamped in floating point before any integer conversion
+    bool fin = true;
+    double bb[4];
+    for (int q = 0; q < 4; ++q) {
+      const double x = s_box[k][q] / g.hb;
+      fin = fin && x == x && fabs(x) < 1e9;
+      bb[q] = fin ? x : 0.0;
+    }
+    int bx0 = (int)floor(bb[0]) - 1, bx1 = (int)floor(bb[1]) + 1;
+    int by0 = (int)floor(bb[2]) - 1, by1 = (int)floor(bb[3]) + 1;
     bx0 = bx0 < 0 ? 0 : bx0; by0 = by0 < 0 ? 0 : by0;
     bx1 = bx1 >= g.bpdx ? g.bpdx - 1 : bx1; by1 = by1 >= g.bpdy ? g.bpdy - 1 : by1;
-    const int w = bx1 >= bx0 ? bx1 - bx0 + 1 : 0, hgt = by1 >= by0 ? by1 - by0 + 1 : 0;
+    const int w = fin && bx1 >= bx0 ? bx1 - bx0 + 1 : 0;
+    const int hgt = fin && by1 >= by0 ? by1 - by0 + 1 : 0;
     s_rng[k][0] = bx0; s_rng[k][1] = by0; s_rng[k][2] = w;
     s_tmp[k] = w * hgt;
   }

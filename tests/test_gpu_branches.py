@@ -105,8 +105,9 @@ def test_collision_body_update(cup, oracle_mod):
         sim.body_update(1e-3)
     a, b = g.shape(0), o.shape(0)
     assert abs(a["u"] - (-2 * speed)) <= 1e-12 * speed, a["u"]
-    for key in ["u", "v", "omega"]:
-        assert abs(a[key] - b[key]) <= 1e-12 * speed, (key, a[key], b[key])
+    # omega is zero up to round-off; its scale is speed / r
+    for key, scale in (("u", speed), ("v", speed), ("omega", speed / r)):
+        assert abs(a[key] - b[key]) <= 1e-12 * scale, (key, a[key], b[key])
 
 
 def test_collision_whole_steps(cup, oracle_mod):
@@ -142,8 +143,9 @@ def test_collision_whole_steps(cup, oracle_mod):
     assert ev < 1e-10 and ep < 1e-10
     for k in range(2):
         a, b = g.shape(k), o.shape(k)
-        for key in ["u", "v", "omega"]:
-            assert abs(a[key] - b[key]) <= 1e-10 * max(abs(b[key]), 1e-3), (k, key, a[key], b[key])
+        # omega is round-off for these head-on disks: its scale is speed / r
+        for key, scale in (("u", speed), ("v", speed), ("omega", speed / r)):
+            assert abs(a[key] - b[key]) <= 1e-10 * max(abs(b[key]), scale), (k, key, a[key], b[key])
 
 
 @pytest.mark.parametrize("device_loop", [False, True])
@@ -245,9 +247,12 @@ def test_force_log_every_step_swimmer(cup, oracle_mod):
 
 @pytest.mark.parametrize("poisson", ["periodic", "freespace", "cosine"])
 def test_divergence_after_projection(cup, oracle_mod, poisson):
-    """The divergence the projection leaves (central differences of a
-    spectrally projected field are not exactly divergence-free) is the
-    oracle's: same field to 1e-9 relative L2, same norm."""
+    """The divergence the projection leaves is the oracle's: same field to
+    1e-9 relative L2.  Central differences of a spectrally projected field are
+    not exactly divergence-free (the wide-stencil divergence of the wide
+    gradient is not the spectral Laplacian, and the penalization acts after
+    the projection), so the initial field is smooth; away from the body and
+    the domain faces its divergence must drop by two orders of magnitude."""
     shapes = [dict(radius=0.05, xpos=0.3, ypos=0.25, xvel=0.2, bFixed=True, bForced=True)]
     kw = dict(extent=1.0, nu=1e-4, CFL=0.2, poisson=poisson)
     g = cup.Simulation(16, 8, shapes=[cup.disk(**s) for s in shapes], **kw)
@@ -255,17 +260,24 @@ def test_divergence_after_projection(cup, oracle_mod, poisson):
                           **kw)
     g.reset()
     o.reset()
-    rng = np.random.default_rng(52)
-    vel = 0.05 * rng.standard_normal((2, g.NY, g.NX))
+    h = g.scalars()["h"]
+    X, Y = np.meshgrid((np.arange(g.NX) + 0.5) * h, (np.arange(g.NY) + 0.5) * h)
+    vel = np.stack([0.05 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+                    + 0.03 * np.cos(4 * np.pi * Y),
+                    0.05 * np.cos(2 * np.pi * X) * np.sin(4 * np.pi * Y)])
     g.set_field(0, vel)
     o.set_field(0, vel)
     g.simulate(5)
     o.simulate(5)
-    h = g.scalars()["h"]
     dg, do = divergence(g.get_field(0), h), divergence(o.get_field(0), h)
     d0 = divergence(vel, h)
     e = rel(dg, do)
+    far = (X - 0.3) ** 2 + (Y - 0.25) ** 2 > 0.15 ** 2
+    inner = np.zeros_like(far)
+    inner[8:-8, 8:-8] = True
+    far &= inner
+    drop = np.linalg.norm(do[far]) / np.linalg.norm(d0[far])
     print(f"  |div| {np.linalg.norm(do):.4e} (initial {np.linalg.norm(d0):.4e}), "
-          f"rel L2 vs oracle {e:.2e}", flush=True)
+          f"away from body and faces x{drop:.2e}, rel L2 vs oracle {e:.2e}", flush=True)
     assert e < 1e-9
-    assert np.linalg.norm(do) < 0.2 * np.linalg.norm(d0)  # the projection did its work
+    assert drop < 1e-2  # the projection did its work
