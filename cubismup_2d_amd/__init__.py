@@ -84,6 +84,7 @@ EXPORTS = [
     "cup2d_advance", "cup2d_simulate", "cup2d_synchronize", "cup2d_poisson_dense",
     "cup2d_grid_size", "cup2d_timer_enable", "cup2d_timer_read", "cup2d_timer_sample",
     "cup2d_force_log_enable", "cup2d_force_log_read",
+    "cup2d_find_max_u_noalloc",
 ]
 FORCE_LOG_COLUMNS = ["time", "step", "Fx", "Fy", "FxPres", "FyPres", "FxVisc", "FyVisc", "tau",
                      "tauPres", "tauVisc", "drag", "thrust", "perimeter", "circulation",
@@ -137,6 +138,7 @@ def lib():
     L.cup2d_timer_sample.argtypes = [P, C.c_char_p, I]
     L.cup2d_force_log_enable.argtypes = [P, I]
     L.cup2d_force_log_read.argtypes = [P, I, I, dp, C.POINTER(I)]
+    L.cup2d_find_max_u_noalloc.argtypes = [P, P, C.c_longlong, D, D, P, P]
     for n in EXPORTS:
         if n != "cup2d_last_error":
             getattr(L, n).restype = C.c_int
@@ -364,3 +366,16 @@ class Simulation:
         out = np.empty_like(rhs)
         self._chk(self.L.cup2d_poisson_dense(self.h, _dp(rhs), _dp(out)))
         return out
+
+
+def find_max_u_noalloc(u_ptr, v_ptr, ncells, uinf, out_ptr, stream=None):
+    """cup2d_find_max_u_noalloc: findMaxU over device planes u_ptr, v_ptr
+    (ncells doubles each) into the device double at out_ptr, enqueued on
+    `stream` (a raw hipStream_t; None = the null stream).  No allocation."""
+    L = lib()
+    rc = L.cup2d_find_max_u_noalloc(C.c_void_p(u_ptr), C.c_void_p(v_ptr), int(ncells),
+                                    float(uinf[0]), float(uinf[1]), C.c_void_p(out_ptr),
+                                    C.c_void_p(stream) if stream else None)
+    if rc != 0:
+        raise Cup2dError(rc, L.cup2d_last_error(None).decode())
+

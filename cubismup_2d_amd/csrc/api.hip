@@ -71,7 +71,14 @@ int timer_index(const char* name) {
   return -1;
 }
 thread_local std::string g_create_error;
+}  // namespace
 
+namespace cup2d {
+// message of a failed call that has no context (cup2d_last_error(NULL))
+void set_ctxfree_error(const char* msg) { g_create_error = msg; }
+}  // namespace cup2d
+
+namespace {
 int fail(cup2d_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;

@@ -212,6 +212,19 @@ int cup2d_timer_read(cup2d_ctx* ctx, const char* kernel, double* total_ms, int* 
  * 1), so a long device loop can be timed with few event pairs on its stream. */
 int cup2d_timer_sample(cup2d_ctx* ctx, const char* kernel, int every);
 
+/* findMaxU over caller-owned device planes, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops).  u and v are device
+ * planes of ncells doubles (ncells <= 2^28); *maxU (device memory) receives
+ * max over cells of |u+uinfx|, |v+uinfy|, |u|, |v| (NaN kept), the value
+ * cup2d_find_max_u computes.  Enqueued on `stream` (a hipStream_t; NULL = the
+ * null stream), no synchronization.  The partial maxima live in one
+ * library-held device array per GPU (32 KiB), shared by every call on that
+ * GPU: calls that overlap on one GPU race; calls ordered on one stream do
+ * not.  Context-free: errors are read with cup2d_last_error(NULL). */
+int cup2d_find_max_u_noalloc(const double* u, const double* v, long long ncells, double uinfx,
+                             double uinfy, double* maxU, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
