@@ -24,6 +24,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads: one per core, packed (set before the
+# oracle library starts its OpenMP runtime)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 PEAK_HBM_GBS = 8000.0
 # algorithmic HBM bytes per cell and launch of the full-grid kernels of a step,
@@ -36,6 +40,15 @@ BALG_FREESPACE = dict(BALG, fft_rows_fwd=4, fft_cols=5, fft_rows_inv=3)
 # the metric's per-step figure: each stage with its own pass (advDiff 4, RHS 3,
 # FFT 2+2+2, projection 5 = 18 reals = 144 B/cell periodic; 23 = 184 B free-space)
 STEP_BALG_METRIC = {"periodic": 18, "freespace": 23}
+# FP32 precision, periodic, bytes per cell of one step's full-grid passes:
+# advDiff 16 (R u,v; W u*,v* in float), forward rows 16 (R u*,v*; W the
+# float RHS b for the refinement; W the float half spectrum), columns 8,
+# inverse rows 12 (R float spectrum; W the double pressure), projection 24
+# (R double p, float u*,v*; W float u,v) = 76; each FP64 refinement sweep adds
+# 84: rows 16 (R p; W double spectrum), apply columns 16, inverse rows 16
+# (R spectrum, b; W float residual), FP32 solve of the residual 8 + 8 and
+# its inverse rows 20 (R float spectrum, R and W p).
+STEP_BYTES_FP32 = {"base": 76, "sweep": 84}
 FULL_GRID = list(BALG)
 
 SWIMMER = dict(kind="carlingfish", xpos=1.0, ypos=0.5, L=0.2, T=1.0, envelope="quadratic",
@@ -53,6 +66,26 @@ CONFIGS = {
     "scaleup8192": dict(bpdx=256, bpdy=128, extent=2.0, nu=5e-6, poisson="periodic",
                         bodies=(4, 16)),
 }
+
+
+def ensemble_members(n=64, seed=3):
+    """The batched-ensemble workload: n independent towed cylinders on grids
+    drawn from the seed between 256x128 and 1024x512 cells (NX a multiple of
+    64, NY = NX/2, domain 1.0 x 0.5), radius U[0.03, 0.08], centre in the
+    left half, towing speed U[0.1, 0.3], nu = 1e-4, free-space Poisson.
+    Returns (bpdx, bpdy, disk keyword arguments) per member."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        bpdx = 2 * int(rng.integers(4, 17))
+        shape = dict(radius=float(rng.uniform(0.03, 0.08)), xpos=float(rng.uniform(0.15, 0.5)),
+                     ypos=float(rng.uniform(0.15, 0.35)), xvel=float(rng.uniform(0.1, 0.3)),
+                     bFixed=True, bForced=True)
+        out.append((bpdx, bpdx // 2, shape))
+    return out
+
+
+ENSEMBLE_KW = dict(extent=1.0, nu=1e-4, CFL=0.1, poisson="freespace")
 
 
 def random_vorticity_velocity(NX, NY, h, sigma=1e-3, seed=12345):
@@ -133,23 +166,60 @@ def body_text(body):
     return f"one disk r={body['radius']}"
 
 
-def cpu_baseline(cfg, vel, steps):
+def host_cpu():
+    """nproc, the CPU model (lscpu) and the OpenMP binding of this host."""
+    model = ""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return dict(nproc=os.cpu_count(), model=model,
+                omp=f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', '')} "
+                    f"OMP_PLACES={os.environ.get('OMP_PLACES', '')} "
+                    f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', '')}")
+
+
+def cpu_step_times(cfg, vel, steps, checker):
+    """Per-step wall times of calcMaxTimestep + advance of the FP64 OpenMP
+    restatement on the same workload (checker: the reference's NaN/Inf scan
+    before every operator, SimulationData.cpp:187-194)."""
     import oracle
     o = oracle.Oracle(cfg["bpdx"], cfg["bpdy"], extent=cfg["extent"], nu=cfg["nu"], CFL=0.1,
-                      poisson=cfg["poisson"], fs_pad2=True,
+                      poisson=cfg["poisson"], fs_pad2=True, checker=checker,
                       shapes=make_shapes(oracle, cfg))
     setup(o, vel)
-    o.simulate(2)  # warm-up
-    t0 = time.perf_counter()
-    o.simulate(steps)
-    t1 = time.perf_counter()
+    t = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        o.simulate(1)
+        t.append(time.perf_counter() - t0)
     total, fft = o.timing()
+    return np.array(t), fft / max(total, 1e-300)
+
+
+def cpu_baseline(cfg, vel, steps):
+    """The CPU baseline: both modes of the oracle, median step time over steps
+    6..N (the first 5 are warm-up).  value = the faster (lean) mode."""
+    import oracle
     cells = cfg["bpdx"] * cfg["bpdy"] * 1024
-    return dict(value=cells * steps / (t1 - t0) / 1e6, unit="Mcell-steps/s",
-                cores=oracle.num_threads(), kind="port",
-                sample=f"{steps} steps of the same workload after 2 warm-up steps, "
-                       f"FP64 OpenMP restatement without the NaN Checker; "
-                       f"FFT share {fft / total:.2f}")
+    res = {}
+    for mode, chk in (("lean", False), ("faithful", True)):
+        t, fft_share = cpu_step_times(cfg, vel, steps, chk)
+        med = float(np.median(t[5:]))
+        res[mode] = dict(value=round(cells / med / 1e6, 2), median_step_ms=round(med * 1e3, 3),
+                         fft_share=round(fft_share, 3))
+    best = max(res, key=lambda k: res[k]["value"])
+    return dict(value=res[best]["value"], unit="Mcell-steps/s", cores=oracle.num_threads(),
+                kind="port", host=host_cpu(), modes=res,
+                sample=f"{steps} steps of the same workload per mode, median step time over "
+                       f"steps 6..{steps}; value is the {best} mode (FP64 OpenMP "
+                       f"restatement, no NaN checker); faithful adds the reference's NaN/Inf "
+                       f"Checker before every operator")
 
 
 KERNEL_OF = {"k_advdiff": "advdiff", "k_rhs<": "pressure_rhs", "k_rows_fwd": "fft_rows_fwd",
@@ -224,14 +294,76 @@ def traffic_from_counters(fetch_csv, write_csv, copy_bytes, config):
     print(json.dumps(out, indent=1))
 
 
+def fp32_leg(cup, cfg, vel, args, sweeps):
+    """The same workload at FP32 precision (float stencils and FFT Poisson,
+    FP64 pressure with `sweeps` FP64 defect-correction sweeps per solve),
+    timed like the FP64 leg."""
+    sim = cup.Simulation(cfg["bpdx"], cfg["bpdy"], extent=cfg["extent"], nu=cfg["nu"], CFL=0.1,
+                         poisson=cfg["poisson"], shapes=make_shapes(cup, cfg), precision="fp32",
+                         refine_sweeps=sweeps)
+    setup(sim, vel)
+    sim.simulate(args.warmup)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.simulate(args.steps)
+    sim.synchronize()
+    dt = time.perf_counter() - t0
+    res = sim.refine_residuals()  # of the last step's solve, before each sweep
+    sim.close()
+    cells = cfg["bpdx"] * cfg["bpdy"] * 1024
+    step_bytes = STEP_BYTES_FP32["base"] + sweeps * STEP_BYTES_FP32["sweep"]
+    achieved = step_bytes * cells * args.steps / dt / 1e9
+    return {"value": round(cells * args.steps / dt / 1e6, 2), "unit": "Mcell-steps/s",
+            "ms_per_step": round(dt / args.steps * 1e3, 5), "refine_sweeps": sweeps,
+            "residual_before_sweeps": [float(f"{r:.3e}") for r in res],
+            "dtype": "f32 (FP64 pressure)",
+            "roofline": {"bound": "hbm", "scope": "whole step", "achieved": round(achieved, 1),
+                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(achieved / PEAK_HBM_GBS, 4),
+                         "alg_bytes_per_cell_step": step_bytes}}
+
+
+def ensemble_leg(cup, steps, precision):
+    """The batched ensemble (64 towed cylinders, grids 256x128 .. 1024x512,
+    free-space): one captured graph advances every member two steps; the
+    lane count is autotuned on the live run first (those steps are not
+    timed).  value = total member cell-steps per second."""
+    members = ensemble_members()
+    sims = []
+    for bpdx, bpdy, shape in members:
+        g = cup.Simulation(bpdx, bpdy, shapes=[cup.disk(**shape)], precision=precision,
+                           **ENSEMBLE_KW)
+        g.reset()
+        sims.append(g)
+    ens = cup.Ensemble(sims)
+    lanes, ms_tuned = ens.tune(launches=2)
+    ens.simulate(2)  # the tuned graph captured and warm
+    t0 = time.perf_counter()
+    ens.simulate(steps)
+    dt = time.perf_counter() - t0
+    cells = sum(bpdx * bpdy * 1024 for bpdx, bpdy, _ in members)
+    ens.close()
+    for g in sims:
+        g.close()
+    return {"value": round(cells * steps / dt / 1e6, 2), "unit": "Mcell-steps/s",
+            "members": len(members), "cells_total": cells, "steps": steps,
+            "ms_per_step": round(dt / steps * 1e3, 4), "lanes": lanes,
+            "ms_per_launch_tuned": round(ms_tuned, 4), "precision": precision,
+            "launches_per_step": 0.5}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="periodic4096", choices=sorted(CONFIGS))
-    ap.add_argument("--cpu-steps", type=int, default=500)
+    ap.add_argument("--cpu-steps", type=int, default=120,
+                    help="steps of each CPU-baseline mode (lean, faithful)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-precision legs")
+    ap.add_argument("--ensemble-steps", type=int, default=100,
+                    help="steps of the 64-member ensemble legs (0 skips them)")
     ap.add_argument("--timer-every", type=int, default=8, metavar="K",
                     help="time the dominant kernel on every K-th step of the timed region "
                          "(each timed launch costs the stream about 5 us)")
@@ -318,6 +450,12 @@ def main():
         "step_hbm_frac": round(step_balg * cells * args.steps / dt / 1e9 / PEAK_HBM_GBS, 4),
         "kernel_us_warmup": {k: round(per[k][0] * 1e3 / max(per[k][1], 1), 2) for k in per},
     }
+    sim.close()
+    if cfg["poisson"] == "periodic" and not args.no_fp32:
+        out["fp32"] = {f"sweeps{k}": fp32_leg(cup, cfg, vel, args, k) for k in (0, 1, 2, 3)}
+    if args.ensemble_steps > 0:
+        out["ensemble64"] = {p: ensemble_leg(cup, args.ensemble_steps, p)
+                             for p in ("fp64", "fp32")}
     if not args.no_cpu and args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cfg, vel, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -20,6 +20,7 @@ VEL, CHI, PRES, TMP, TMPV, UDEF, INVRHO, VFLUID = range(8)
 NCOMP = [2, 1, 1, 1, 2, 2, 1, 2]
 LAYOUT_BLOCKS, LAYOUT_DENSE = 0, 1
 POISSON = {"periodic": 0, "freespace": 1, "cosine": 2}
+PRECISION = {"fp64": 0, "fp32": 1}
 
 OK = 0
 ERR_ARG, ERR_HIP, ERR_STATE, ERR_BODY_OUT, ERR_CAPACITY, ERR_NONFINITE = -1, -2, -3, -4, -5, -6
@@ -35,7 +36,8 @@ class Cfg(C.Structure):
     _fields_ = [("bpdx", C.c_int), ("bpdy", C.c_int), ("extent", C.c_double),
                 ("nu", C.c_double), ("CFL", C.c_double), ("lam", C.c_double),
                 ("fadeLen", C.c_double), ("poisson", C.c_int), ("implicit_penal", C.c_int),
-                ("iterative_penal", C.c_int), ("stream", C.c_void_p)]
+                ("iterative_penal", C.c_int), ("stream", C.c_void_p),
+                ("precision", C.c_int), ("refine_sweeps", C.c_int)]
 
 
 class ShapeDesc(C.Structure):
@@ -84,7 +86,11 @@ EXPORTS = [
     "cup2d_advance", "cup2d_simulate", "cup2d_synchronize", "cup2d_poisson_dense",
     "cup2d_grid_size", "cup2d_timer_enable", "cup2d_timer_read", "cup2d_timer_sample",
     "cup2d_force_log_enable", "cup2d_force_log_read",
-    "cup2d_find_max_u_noalloc",
+    "cup2d_find_max_u_noalloc", "cup2d_simulate_graph", "cup2d_refine_residuals",
+    "cup2d_set_surface_velocity",
+    "cup2d_ensemble_create", "cup2d_ensemble_destroy", "cup2d_ensemble_last_error",
+    "cup2d_ensemble_set_lanes", "cup2d_ensemble_simulate", "cup2d_ensemble_synchronize",
+    "cup2d_ensemble_tune",
 ]
 FORCE_LOG_COLUMNS = ["time", "step", "Fx", "Fy", "FxPres", "FyPres", "FxVisc", "FyVisc", "tau",
                      "tauPres", "tauVisc", "drag", "thrust", "perimeter", "circulation",
@@ -92,7 +98,7 @@ FORCE_LOG_COLUMNS = ["time", "step", "Fx", "Fy", "FxPres", "FyPres", "FxVisc", "
                      "EffPDefBnd", "EffPDef", "n_surface_points"]
 TIMERS = ["find_max_u", "dt", "advdiff", "flux_fix", "pressure_rhs", "rhs_obstacles", "fade",
           "fft_rows_fwd", "fft_cols", "fft_rows_inv", "pressure_correction", "put_objects",
-          "body_update", "penalize", "forces"]
+          "body_update", "penalize", "forces", "refine"]
 
 _lib = None
 
@@ -139,8 +145,19 @@ def lib():
     L.cup2d_force_log_enable.argtypes = [P, I]
     L.cup2d_force_log_read.argtypes = [P, I, I, dp, C.POINTER(I)]
     L.cup2d_find_max_u_noalloc.argtypes = [P, P, C.c_longlong, D, D, P, P]
+    L.cup2d_simulate_graph.argtypes = [P, I]
+    L.cup2d_refine_residuals.argtypes = [P, dp, C.POINTER(I)]
+    L.cup2d_set_surface_velocity.argtypes = [P, I, I, dp]
+    L.cup2d_ensemble_create.argtypes = [C.POINTER(C.c_void_p), I, C.POINTER(C.c_void_p)]
+    L.cup2d_ensemble_destroy.argtypes = [P]
+    L.cup2d_ensemble_last_error.argtypes = [P]
+    L.cup2d_ensemble_last_error.restype = C.c_char_p
+    L.cup2d_ensemble_set_lanes.argtypes = [P, I]
+    L.cup2d_ensemble_simulate.argtypes = [P, I]
+    L.cup2d_ensemble_synchronize.argtypes = [P]
+    L.cup2d_ensemble_tune.argtypes = [P, I, C.POINTER(I), dp]
     for n in EXPORTS:
-        if n != "cup2d_last_error":
+        if n not in ("cup2d_last_error", "cup2d_ensemble_last_error"):
             getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -191,11 +208,15 @@ class Simulation:
 
     def __init__(self, bpdx, bpdy, extent=1.0, nu=1e-2, CFL=0.1, lam=None, fadeLen=0.01,
                  poisson="periodic", implicit_penal=False, iterative_penal=False, shapes=(),
-                 stream=None):
+                 stream=None, precision="fp64", refine_sweeps=None):
         self.L = lib()
         lam = 1e3 / CFL if lam is None else lam
+        prec = PRECISION[precision]
+        if refine_sweeps is None:  # FP32: one FP64 sweep where the solver takes one
+            refine_sweeps = 1 if prec == PRECISION["fp32"] and poisson != "freespace" else 0
         cfg = Cfg(bpdx, bpdy, extent, nu, CFL, lam, fadeLen, POISSON[poisson],
-                  int(implicit_penal), int(iterative_penal), stream)
+                  int(implicit_penal), int(iterative_penal), stream, prec, int(refine_sweeps))
+        self.precision = precision
         h = C.c_void_p()
         rc = self.L.cup2d_create(C.byref(cfg), C.byref(h))
         if rc != OK:
@@ -330,6 +351,31 @@ class Simulation:
             self.synchronize()
         return 0
 
+    def set_surface_velocity(self, coef, shape=0):
+        """Fourier-mode surface velocity of a disk: coef = (a, b, c, d) rows of
+        M normal-cos, normal-sin, tangential-cos, tangential-sin amplitudes
+        (or the flat 4M vector); None removes it."""
+        if coef is None:
+            self._chk(self.L.cup2d_set_surface_velocity(self.h, shape, 0, None))
+            return
+        c = np.ascontiguousarray(np.asarray(coef, dtype=np.float64).ravel())
+        assert c.size % 4 == 0
+        self._chk(self.L.cup2d_set_surface_velocity(self.h, shape, c.size // 4, _dp(c)))
+
+    def refine_residuals(self):
+        """FP32: relative residual at the start of each refinement sweep of
+        the last Poisson solve."""
+        out = np.zeros(4)
+        n = C.c_int()
+        self._chk(self.L.cup2d_refine_residuals(self.h, _dp(out), C.byref(n)))
+        return out[:n.value].copy()
+
+    def simulate_graph(self, nsteps, sync=True):
+        """cup2d_simulate_graph: the steps replayed from a captured HIP graph."""
+        self._chk(self.L.cup2d_simulate_graph(self.h, nsteps))
+        if sync:
+            self.synchronize()
+
     def synchronize(self):
         self._chk(self.L.cup2d_synchronize(self.h))
 
@@ -379,3 +425,50 @@ def find_max_u_noalloc(u_ptr, v_ptr, ncells, uinf, out_ptr, stream=None):
     if rc != 0:
         raise Cup2dError(rc, L.cup2d_last_error(None).decode())
 
+
+class Ensemble:
+    """Independent simulations advanced by one captured graph per two steps
+    (cup2d_ensemble_*).  Members are Simulation objects (reset done)."""
+
+    def __init__(self, sims, lanes=0):
+        self.L = lib()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
+        h = C.c_void_p()
+        rc = self.L.cup2d_ensemble_create(arr, len(self.sims), C.byref(h))
+        if rc != OK:
+            raise Cup2dError(rc, self.L.cup2d_ensemble_last_error(None).decode())
+        self.h = h
+        if lanes:
+            self._chk(self.L.cup2d_ensemble_set_lanes(self.h, lanes))
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise Cup2dError(rc, self.L.cup2d_ensemble_last_error(self.h).decode())
+        return rc
+
+    def simulate(self, nsteps, sync=True):
+        self._chk(self.L.cup2d_ensemble_simulate(self.h, nsteps))
+        if sync:
+            self.synchronize()
+
+    def synchronize(self):
+        self._chk(self.L.cup2d_ensemble_synchronize(self.h))
+
+    def tune(self, launches=3):
+        """Autotune the lane count on the live run (2 * launches steps per
+        candidate, real steps); returns (lanes, ms per two-step launch)."""
+        lanes, ms = C.c_int(), C.c_double()
+        self._chk(self.L.cup2d_ensemble_tune(self.h, launches, C.byref(lanes), C.byref(ms)))
+        return lanes.value, ms.value
+
+    def set_lanes(self, lanes):
+        self._chk(self.L.cup2d_ensemble_set_lanes(self.h, lanes))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cup2d_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -17,7 +17,7 @@ namespace cup2d {
 const char* const TIMER_NAMES[T_NTIMERS] = {
     "find_max_u", "dt", "advdiff", "flux_fix", "pressure_rhs", "rhs_obstacles", "fade",
     "fft_rows_fwd", "fft_cols", "fft_rows_inv", "pressure_correction", "put_objects",
-    "body_update", "penalize", "forces"};
+    "body_update", "penalize", "forces", "refine"};
 
 void timer_mark(cup2d_ctx* c, int id, bool start) { timer_mark_on(c, id, start, nullptr); }
 
@@ -106,6 +106,11 @@ int nfield_comps(int f) {
     case CUP2D_CHI: case CUP2D_PRES: case CUP2D_TMP: case CUP2D_INVRHO: return 1;
     default: return 0;
   }
+}
+
+// the field's planes hold float (FP32 precision: velocity, tmpV, RHS)
+bool f32_field(const cup2d_ctx* c, int f) {
+  return c->f32 && (f == CUP2D_VEL || f == CUP2D_TMPV || f == CUP2D_TMP);
 }
 
 void field_planes(cup2d_ctx* c, int f, double** a, double** b) {
@@ -256,7 +261,101 @@ int advance_enqueued(cup2d_ctx* c, int compute_dt) {
   c->host_step += 1;
   return CUP2D_OK;
 }
+
+// The quiescent state: everything the last step left pending (the
+// placement commit, its ComputeForces and step end, the side-stream chain)
+// enqueued, and both side streams joined into the main stream, with no host
+// wait, so it can end a captured graph.  The next step's precomputed
+// midline part is dropped (the step recomputes it).
+int enqueue_quiesce(cup2d_ctx* c) {
+  RC(join_obstacles(c));
+  RC(flush_commit(c));
+  RC(flush_forces(c));
+  RC(join_obstacles(c));
+  RC(join_forces(c));
+  CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_qaux, c->aux));
+  CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_qaux, 0));
+  CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_qmid, c->mids));
+  CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_qmid, 0));
+  c->midA_ready = false;
+  return CUP2D_OK;
+}
+
+// Two whole steps (Simulation::advance with dt from the device) from the
+// quiescent state back to it, enqueued into the capture that c->stream is
+// part of.  The side streams are forked from the main stream first, so every
+// stream the steps use belongs to the capture.  Host-side state is as before
+// (two buffer swaps each); host_step is restored, the replays advance it.
+int capture_two_steps(cup2d_ctx* c) {
+  CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_fork, c->stream));
+  CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+  CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->mids, c->ev_fork, 0));
+  c->graph_capture = true;
+  int rc = advance_enqueued(c, 1);
+  if (!rc) rc = advance_enqueued(c, 1);
+  if (!rc) rc = enqueue_quiesce(c);
+  c->graph_capture = false;
+  c->host_step -= 2;
+  return rc;
+}
+
+bool timers_on(const cup2d_ctx* c) {
+  for (const Timer& t : c->timers)
+    if (t.on) return true;
+  return false;
+}
+
+int graph_eligible(cup2d_ctx* c) {
+  if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
+  if (c->cfg.iterative_penal)
+    return fail(c, CUP2D_ERR_STATE, "captured steps need PressureSingle (the iterator waits "
+                                    "on the host)");
+  if (timers_on(c)) return fail(c, CUP2D_ERR_STATE, "captured steps need the kernel timers off");
+  RC(need_fft(c));
+  return CUP2D_OK;
+}
+
+void drop_graphs(cup2d_ctx* c) {
+  for (auto& gr : c->graphs) {
+    if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+    gr = cup2d_ctx::StepGraph{};
+  }
+}
+
+// The cached two-step graph of the current parity and buffers (captured now
+// if there is none).  Needs the quiescent state.
+int step_graph(cup2d_ctx* c, hipGraphExec_t* out) {
+  cup2d_ctx::StepGraph& gr = c->graphs[c->host_step & 1];
+  if (gr.exec && gr.u0 == c->u && gr.ob0 == c->ob.cnt) {
+    *out = gr.exec;
+    return CUP2D_OK;
+  }
+  if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+  gr = cup2d_ctx::StepGraph{};
+  CUP2D_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = capture_two_steps(c);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  CUP2D_HIP_CHECK(c, e);
+  const hipError_t ei = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  CUP2D_HIP_CHECK(c, ei);
+  gr.u0 = c->u;
+  gr.ob0 = c->ob.cnt;
+  *out = gr.exec;
+  return CUP2D_OK;
+}
 }  // namespace
+
+namespace cup2d {
+// graphs bake buffer pointers and launch arguments: dropped whenever those
+// may change (reset, force log, shapes)
+void invalidate_graphs(cup2d_ctx* c) { drop_graphs(c); }
+}  // namespace cup2d
 
 extern "C" {
 
@@ -272,8 +371,27 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
     g_create_error = "invalid configuration (bpdx, bpdy, extent, nu, CFL > 0; poisson 0|1|2)";
     return CUP2D_ERR_ARG;
   }
+  if (cfg->precision != CUP2D_PRECISION_FP64 && cfg->precision != CUP2D_PRECISION_FP32) {
+    g_create_error = "precision must be CUP2D_PRECISION_FP64 or CUP2D_PRECISION_FP32";
+    return CUP2D_ERR_ARG;
+  }
+  if (cfg->refine_sweeps < 0 || cfg->refine_sweeps > 4) {
+    g_create_error = "refine_sweeps must be 0..4";
+    return CUP2D_ERR_ARG;
+  }
+  if (cfg->precision == CUP2D_PRECISION_FP32 && cfg->iterative_penal) {
+    g_create_error = "iterative penalization needs FP64 precision";
+    return CUP2D_ERR_ARG;
+  }
+  if (cfg->precision == CUP2D_PRECISION_FP32 && cfg->refine_sweeps > 0 &&
+      cfg->poisson == CUP2D_POISSON_FREESPACE) {
+    g_create_error = "FP64 refinement sweeps need the periodic or cosine solver";
+    return CUP2D_ERR_ARG;
+  }
   cup2d_ctx* c = new cup2d_ctx();
   c->cfg = *cfg;
+  c->f32 = cfg->precision == CUP2D_PRECISION_FP32;
+  c->refine_sweeps = c->f32 ? cfg->refine_sweeps : 0;
   if (hipGetDevice(&c->device) != hipSuccess) {
     g_create_error = "no HIP device";
     delete c;
@@ -315,7 +433,11 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
       hipEventCreateWithFlags(&c->ev_pen, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_frc, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_lists, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
-      hipMalloc(&c->fsnap, sizeof(ForceSnap) * MAX_SHAPES) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_fork, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_qaux, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_qmid, CUP2D_SYNC_EVENT_FLAGS) != hipSuccess ||
+      hipMalloc(&c->fsnap, sizeof(ForceSnap) * MAX_SHAPES) != hipSuccess ||
+      hipMalloc(&c->surf_coef, sizeof(double) * 4 * SURF_MAX_MODES) != hipSuccess) {
     c->err = "side stream creation failed";
     return bail(CUP2D_ERR_HIP);
   }
@@ -325,6 +447,11 @@ int cup2d_create(const cup2d_cfg* cfg, cup2d_ctx** out) {
   for (double** p : planes)
     if (hipMalloc(p, sizeof(double) * n) != hipSuccess) { c->err = "out of device memory"; return bail(CUP2D_ERR_HIP); }
   if (hipMalloc(&c->staging, sizeof(double) * 2 * n) != hipSuccess) { c->err = "out of device memory"; return bail(CUP2D_ERR_HIP); }
+  if (c->f32 && (hipMalloc(&c->rres, sizeof(float) * n) != hipSuccess ||
+                 hipMalloc(&c->respart, sizeof(double) * 5 * g.NY) != hipSuccess)) {
+    c->err = "out of device memory";
+    return bail(CUP2D_ERR_HIP);
+  }
   c->red_n = 1 << 16;
   if (hipMalloc(&c->red, sizeof(double) * c->red_n) != hipSuccess) { c->err = "out of device memory"; return bail(CUP2D_ERR_HIP); }
   if (hipMalloc(&c->st, sizeof(StepState)) != hipSuccess ||
@@ -395,7 +522,7 @@ int cup2d_destroy(cup2d_ctx* c) {
   obstacles_free(c);
   double* ptrs[] = {c->u, c->v, c->tu, c->tv, c->pres, c->rhs, c->chi, c->vfu, c->vfv,
                     c->udx, c->udy, c->invrho, c->staging, c->red, c->ramp,
-                    c->fluxpart, c->maxpart};
+                    c->fluxpart, c->maxpart, c->rres, c->respart};
   if (c->obmap) (void)hipFree(c->obmap);
   if (c->ubidx) (void)hipFree(c->ubidx);
   fish_free_all(c);
@@ -416,7 +543,12 @@ int cup2d_destroy(cup2d_ctx* c) {
   if (c->ev_pen) (void)hipEventDestroy(c->ev_pen);
   if (c->ev_frc) (void)hipEventDestroy(c->ev_frc);
   if (c->ev_lists) (void)hipEventDestroy(c->ev_lists);
+  drop_graphs(c);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_qaux) (void)hipEventDestroy(c->ev_qaux);
+  if (c->ev_qmid) (void)hipEventDestroy(c->ev_qmid);
   if (c->fsnap) (void)hipFree(c->fsnap);
+  if (c->surf_coef) (void)hipFree(c->surf_coef);
   if (c->flog) (void)hipFree(c->flog);
   if (c->flog_cnt) (void)hipFree(c->flog_cnt);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -488,13 +620,18 @@ int cup2d_upload_field(cup2d_ctx* c, int f, int layout, const double* host) {
   double *a, *b;
   field_planes(c, f, &a, &b);
   const size_t n = (size_t)c->g.NX * c->g.NY;
-  if (layout == CUP2D_LAYOUT_DENSE) {
+  const bool f32 = f32_field(c, f);
+  if (layout == CUP2D_LAYOUT_DENSE && f32) {
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(c->staging, host, sizeof(double) * n * nc, hipMemcpyHostToDevice, c->stream));
+    RC(enqueue_stage_to_f32(c, c->staging, a));
+    if (nc > 1) RC(enqueue_stage_to_f32(c, c->staging + n, b));
+  } else if (layout == CUP2D_LAYOUT_DENSE) {
     CUP2D_HIP_CHECK(c, hipMemcpyAsync(a, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     if (nc > 1)
       CUP2D_HIP_CHECK(c, hipMemcpyAsync(b, host + n, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   } else {
     CUP2D_HIP_CHECK(c, hipMemcpyAsync(c->staging, host, sizeof(double) * n * nc, hipMemcpyHostToDevice, c->stream));
-    RC(enqueue_blocks_to_dense(c, c->staging, a, b, nc));
+    RC(enqueue_blocks_to_dense(c, c->staging, a, b, nc, f32));
   }
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return CUP2D_OK;
@@ -508,12 +645,17 @@ int cup2d_download_field(cup2d_ctx* c, int f, int layout, double* host) {
   double *a, *b;
   field_planes(c, f, &a, &b);
   const size_t n = (size_t)c->g.NX * c->g.NY;
-  if (layout == CUP2D_LAYOUT_DENSE) {
+  const bool f32 = f32_field(c, f);
+  if (layout == CUP2D_LAYOUT_DENSE && f32) {
+    RC(enqueue_f32_to_stage(c, a, c->staging));
+    if (nc > 1) RC(enqueue_f32_to_stage(c, b, c->staging + n));
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(host, c->staging, sizeof(double) * n * nc, hipMemcpyDeviceToHost, c->stream));
+  } else if (layout == CUP2D_LAYOUT_DENSE) {
     CUP2D_HIP_CHECK(c, hipMemcpyAsync(host, a, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     if (nc > 1)
       CUP2D_HIP_CHECK(c, hipMemcpyAsync(host + n, b, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   } else {
-    RC(enqueue_dense_to_blocks(c, a, b, c->staging, nc));
+    RC(enqueue_dense_to_blocks(c, a, b, c->staging, nc, f32));
     CUP2D_HIP_CHECK(c, hipMemcpyAsync(host, c->staging, sizeof(double) * n * nc, hipMemcpyDeviceToHost, c->stream));
   }
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -585,6 +727,7 @@ int cup2d_reset(cup2d_ctx* c) {
   CHECK_CTX(c);
   RC(ensure_frozen(c));
   RC(sync_all(c));
+  drop_graphs(c);
   c->host_step = 0;
   c->midA_ready = false;
   RC(enqueue_shapes_reset(c));
@@ -710,6 +853,61 @@ int cup2d_simulate(cup2d_ctx* c, int nsteps) {
   return CUP2D_OK;
 }
 
+int cup2d_set_surface_velocity(cup2d_ctx* c, int shape_id, int nmodes, const double* coef) {
+  CHECK_CTX(c);
+  if (nmodes < 0 || nmodes > SURF_MAX_MODES || (nmodes > 0 && !coef))
+    return fail(c, CUP2D_ERR_ARG, "surface velocity: nmodes must be 0..32 with 4*nmodes coefficients");
+  if (nmodes > 0 && (shape_id < 0 || shape_id >= c->g.nshapes))
+    return fail(c, CUP2D_ERR_ARG, "bad shape id");
+  if (nmodes > 0 && c->hsp[shape_id].kind != 0)
+    return fail(c, CUP2D_ERR_ARG, "the surface velocity is defined on disks");
+  if (c->cfg.iterative_penal)
+    return fail(c, CUP2D_ERR_STATE, "the surface velocity needs PressureSingle");
+  RC(sync_all(c));
+  drop_graphs(c);
+  if (nmodes > 0)
+    CUP2D_HIP_CHECK(c, hipMemcpy(c->surf_coef, coef, sizeof(double) * 4 * nmodes,
+                                 hipMemcpyHostToDevice));
+  c->surf_shape = nmodes > 0 ? shape_id : -1;
+  c->surf_modes = nmodes;
+  return CUP2D_OK;
+}
+
+int cup2d_refine_residuals(cup2d_ctx* c, double* res, int* nsweeps) {
+  CHECK_CTX(c);
+  if (!res || !nsweeps) return fail(c, CUP2D_ERR_ARG, "null pointer");
+  RC(sync_all(c));
+  StepState s;
+  CUP2D_HIP_CHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  const int n = c->fft && !c->fft_err.size() && c->cfg.poisson != CUP2D_POISSON_FREESPACE
+                    ? c->refine_sweeps : 0;
+  for (int k = 0; k < n; ++k) res[k] = s.res[k];
+  *nsweeps = n;
+  return CUP2D_OK;
+}
+
+int cup2d_simulate_graph(cup2d_ctx* c, int nsteps) {
+  CHECK_CTX(c);
+  if (nsteps < 0) return fail(c, CUP2D_ERR_ARG, "nsteps < 0");
+  RC(graph_eligible(c));
+  if (nsteps == 0) return CUP2D_OK;
+  RC(enqueue_quiesce(c));
+  RC(enqueue_find_max_u_partials(c));
+  for (int left = nsteps; left > 0;) {
+    if (left == 1) {
+      RC(advance_enqueued(c, 1));
+      RC(enqueue_quiesce(c));
+      break;
+    }
+    hipGraphExec_t ex = nullptr;
+    RC(step_graph(c, &ex));
+    CUP2D_HIP_CHECK(c, hipGraphLaunch(ex, c->stream));
+    c->host_step += 2;
+    left -= 2;
+  }
+  return launch_check(c, "simulate_graph");
+}
+
 int cup2d_synchronize(cup2d_ctx* c) {
   CHECK_CTX(c);
   return sync_and_check(c);
@@ -723,7 +921,15 @@ int cup2d_poisson_dense(cup2d_ctx* c, const double* rhs, double* sol) {
   double* din = c->staging;
   double* dout = c->staging + n;
   CUP2D_HIP_CHECK(c, hipMemcpyAsync(din, rhs, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  RC(enqueue_poisson_fft(c, din, dout, 0));
+  if (c->f32) {
+    // the float RHS in the second half of the staging buffer; the solution
+    // (double) replaces the uploaded RHS
+    RC(enqueue_stage_to_f32(c, din, dout));
+    RC(enqueue_poisson_fft(c, dout, din, 0));
+    std::swap(din, dout);
+  } else {
+    RC(enqueue_poisson_fft(c, din, dout, 0));
+  }
   CUP2D_HIP_CHECK(c, hipMemcpyAsync(sol, dout, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return CUP2D_OK;
@@ -733,6 +939,7 @@ int cup2d_force_log_enable(cup2d_ctx* c, int capacity) {
   CHECK_CTX(c);
   if (capacity < 0) return fail(c, CUP2D_ERR_ARG, "force log capacity < 0");
   RC(sync_all(c));
+  drop_graphs(c);  // they hold the log's buffers as launch arguments
   if (c->flog) (void)hipFree(c->flog);
   if (c->flog_cnt) (void)hipFree(c->flog_cnt);
   c->flog = nullptr;
@@ -823,6 +1030,306 @@ int cup2d_timer_read(cup2d_ctx* c, const char* kernel, double* total_ms, int* la
   *total_ms = tot;
   *launches = t.used;
   t.used = 0;
+  return CUP2D_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- ensembles
+// Many independent simulations (the reference runs one Simulation per
+// process under an RL driver, main_RL_smartCyl.cpp:102-205) advanced by
+// captured graphs: two whole steps of every member per launch.  The members
+// are spread over `lanes` lanes; each lane is one captured graph (its
+// members one after another) launched on its own stream, the lanes side by
+// side.  The autotuner times every lane count on the live run and keeps the
+// fastest.
+struct cup2d_ensemble {
+  std::vector<cup2d_ctx*> m;
+  hipStream_t stream = nullptr;     // orders the lanes after and before the members
+  std::vector<hipStream_t> lane_stream;
+  hipEvent_t ev_start = nullptr, ev_end = nullptr;
+  std::vector<hipEvent_t> ev_lane, ev_done;
+  hipEvent_t t0 = nullptr, t1 = nullptr;  // timing (autotuner)
+  int lanes = 1;
+  std::vector<hipGraphExec_t> exec;
+  int exec_lanes = -1, exec_parity = -1;
+  std::vector<const double*> u0;
+  std::vector<const int*> ob0;
+  std::string err;
+};
+
+namespace {
+int efail(cup2d_ensemble* e, int code, const std::string& msg) {
+  e->err = msg;
+  return code;
+}
+int member_error(cup2d_ensemble* e, size_t i, int rc) {
+  e->err = "member " + std::to_string(i) + ": " + e->m[i]->err;
+  return rc;
+}
+#define EHIP(e, call)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) return efail((e), CUP2D_ERR_HIP, std::string(#call) + ": " + \
+                                       hipGetErrorString(e_));            \
+  } while (0)
+
+bool ens_graph_valid(const cup2d_ensemble* e, int lanes) {
+  if (e->exec.empty() || e->exec_lanes != lanes ||
+      e->exec_parity != (e->m[0]->host_step & 1))
+    return false;
+  for (size_t i = 0; i < e->m.size(); ++i)
+    if (e->u0[i] != e->m[i]->u || e->ob0[i] != e->m[i]->ob.cnt) return false;
+  return true;
+}
+
+void ens_drop(cup2d_ensemble* e) {
+  for (hipGraphExec_t x : e->exec)
+    if (x) (void)hipGraphExecDestroy(x);
+  e->exec.clear();
+  e->exec_lanes = -1;
+}
+
+// One graph per lane: lane l holds members l, l + L, l + 2L, ... one after
+// another, each forked from the lane's stream (or the previous member) and
+// joined back.
+int ens_capture(cup2d_ensemble* e, int lanes) {
+  ens_drop(e);
+  const int n = (int)e->m.size();
+  const int L = lanes <= 0 || lanes > n ? n : lanes;
+  while ((int)e->lane_stream.size() < L) {
+    hipStream_t st = nullptr;
+    EHIP(e, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    e->lane_stream.push_back(st);
+  }
+  for (int l = 0; l < L; ++l) {
+    hipStream_t ls = e->lane_stream[l];
+    EHIP(e, hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
+    int rc = CUP2D_OK;
+    if (hipEventRecord(e->ev_lane[l], ls) != hipSuccess) rc = CUP2D_ERR_HIP;
+    for (int i = l; i < n && !rc; i += L) {
+      cup2d_ctx* c = e->m[i];
+      if (hipStreamWaitEvent(c->stream, e->ev_lane[l], 0) != hipSuccess) {
+        rc = CUP2D_ERR_HIP;
+        break;
+      }
+      rc = capture_two_steps(c);
+      if (rc) {
+        member_error(e, i, rc);
+        break;
+      }
+      if (hipEventRecord(e->ev_lane[l], c->stream) != hipSuccess) rc = CUP2D_ERR_HIP;
+    }
+    if (!rc && hipStreamWaitEvent(ls, e->ev_lane[l], 0) != hipSuccess) rc = CUP2D_ERR_HIP;
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(ls, &graph);
+    if (rc || ee != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      ens_drop(e);
+      if (!rc) return efail(e, CUP2D_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                                   hipGetErrorString(ee));
+      if (e->err.empty()) e->err = "capture failed";
+      return rc;
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      ens_drop(e);
+      return efail(e, CUP2D_ERR_HIP, std::string("hipGraphInstantiate: ") +
+                                         hipGetErrorString(ei));
+    }
+    e->exec.push_back(x);
+  }
+  e->exec_lanes = lanes;
+  e->exec_parity = e->m[0]->host_step & 1;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    e->u0[i] = e->m[i]->u;
+    e->ob0[i] = e->m[i]->ob.cnt;
+  }
+  return CUP2D_OK;
+}
+
+// members quiescent, with their findMaxU partials, and the ensemble stream
+// ordered after their streams
+int ens_prepare(cup2d_ensemble* e) {
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    cup2d_ctx* c = e->m[i];
+    int rc = flush_forces(c);
+    if (!rc) rc = graph_eligible(c);
+    if (!rc) rc = enqueue_quiesce(c);
+    if (!rc) rc = enqueue_find_max_u_partials(c);
+    if (rc) return member_error(e, i, rc);
+    EHIP(e, hipEventRecord(e->ev_lane[0], c->stream));
+    EHIP(e, hipStreamWaitEvent(e->stream, e->ev_lane[0], 0));
+  }
+  return CUP2D_OK;
+}
+
+// every member's stream ordered after the ensemble stream
+int ens_release(cup2d_ensemble* e) {
+  EHIP(e, hipEventRecord(e->ev_end, e->stream));
+  for (cup2d_ctx* c : e->m) EHIP(e, hipStreamWaitEvent(c->stream, e->ev_end, 0));
+  return CUP2D_OK;
+}
+
+int ens_launch(cup2d_ensemble* e, int lanes) {
+  if (!ens_graph_valid(e, lanes)) {
+    const int rc = ens_capture(e, lanes);
+    if (rc) return rc;
+  }
+  EHIP(e, hipEventRecord(e->ev_start, e->stream));
+  for (size_t l = 0; l < e->exec.size(); ++l) {
+    hipStream_t ls = e->lane_stream[l];
+    EHIP(e, hipStreamWaitEvent(ls, e->ev_start, 0));
+    EHIP(e, hipGraphLaunch(e->exec[l], ls));
+    EHIP(e, hipEventRecord(e->ev_done[l], ls));
+    EHIP(e, hipStreamWaitEvent(e->stream, e->ev_done[l], 0));
+  }
+  for (cup2d_ctx* c : e->m) c->host_step += 2;
+  return CUP2D_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cup2d_ensemble_create(cup2d_ctx* const* ctxs, int n, cup2d_ensemble** out) {
+  if (!ctxs || n <= 0 || !out) {
+    g_create_error = "ensemble: null members or n <= 0";
+    return CUP2D_ERR_ARG;
+  }
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device) {
+      g_create_error = "ensemble: null member or members on different devices";
+      return CUP2D_ERR_ARG;
+    }
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) {
+        g_create_error = "ensemble: a context appears twice";
+        return CUP2D_ERR_ARG;
+      }
+  }
+  (void)hipSetDevice(ctxs[0]->device);
+  cup2d_ensemble* e = new cup2d_ensemble();
+  e->m.assign(ctxs, ctxs + n);
+  e->u0.assign(n, nullptr);
+  e->ob0.assign(n, nullptr);
+  e->ev_lane.assign(n, nullptr);
+  e->ev_done.assign(n, nullptr);
+  bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&e->ev_start, CUP2D_SYNC_EVENT_FLAGS) == hipSuccess &&
+            hipEventCreateWithFlags(&e->ev_end, CUP2D_SYNC_EVENT_FLAGS) == hipSuccess &&
+            hipEventCreate(&e->t0) == hipSuccess && hipEventCreate(&e->t1) == hipSuccess;
+  for (int i = 0; i < n && ok; ++i)
+    ok = hipEventCreateWithFlags(&e->ev_lane[i], CUP2D_SYNC_EVENT_FLAGS) == hipSuccess &&
+         hipEventCreateWithFlags(&e->ev_done[i], CUP2D_SYNC_EVENT_FLAGS) == hipSuccess;
+  if (!ok) {
+    g_create_error = "ensemble: stream or event creation failed";
+    cup2d_ensemble_destroy(e);
+    return CUP2D_ERR_HIP;
+  }
+  *out = e;
+  return CUP2D_OK;
+}
+
+int cup2d_ensemble_destroy(cup2d_ensemble* e) {
+  if (!e) return CUP2D_ERR_ARG;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (hipStream_t ls : e->lane_stream) (void)hipStreamSynchronize(ls);
+  ens_drop(e);
+  for (hipStream_t ls : e->lane_stream) (void)hipStreamDestroy(ls);
+  for (hipEvent_t v : e->ev_lane)
+    if (v) (void)hipEventDestroy(v);
+  for (hipEvent_t v : e->ev_done)
+    if (v) (void)hipEventDestroy(v);
+  for (hipEvent_t v : {e->ev_start, e->ev_end, e->t0, e->t1})
+    if (v) (void)hipEventDestroy(v);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return CUP2D_OK;
+}
+
+const char* cup2d_ensemble_last_error(const cup2d_ensemble* e) {
+  return e ? e->err.c_str() : g_create_error.c_str();
+}
+
+int cup2d_ensemble_set_lanes(cup2d_ensemble* e, int lanes) {
+  if (!e) return CUP2D_ERR_ARG;
+  if (lanes < 0 || lanes > (int)e->m.size())
+    return efail(e, CUP2D_ERR_ARG, "lanes must be 0..members (0: one lane per member)");
+  e->lanes = lanes;
+  return CUP2D_OK;
+}
+
+int cup2d_ensemble_simulate(cup2d_ensemble* e, int nsteps) {
+  if (!e) return CUP2D_ERR_ARG;
+  if (nsteps < 0) return efail(e, CUP2D_ERR_ARG, "nsteps < 0");
+  if (nsteps == 0) return CUP2D_OK;
+  (void)hipSetDevice(e->m[0]->device);
+  const int parity = e->m[0]->host_step & 1;
+  for (cup2d_ctx* c : e->m)
+    if ((c->host_step & 1) != parity)
+      return efail(e, CUP2D_ERR_STATE, "members must be at steps of the same parity");
+  RC(ens_prepare(e));
+  for (int left = nsteps; left >= 2; left -= 2) RC(ens_launch(e, e->lanes));
+  RC(ens_release(e));
+  if (nsteps % 2) {
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      cup2d_ctx* c = e->m[i];
+      int rc = advance_enqueued(c, 1);
+      if (!rc) rc = enqueue_quiesce(c);
+      if (rc) return member_error(e, i, rc);
+    }
+  }
+  return CUP2D_OK;
+}
+
+int cup2d_ensemble_synchronize(cup2d_ensemble* e) {
+  if (!e) return CUP2D_ERR_ARG;
+  EHIP(e, hipStreamSynchronize(e->stream));
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    cup2d_ctx* c = e->m[i];
+    (void)hipSetDevice(c->device);
+    const int rc = sync_and_check(c);
+    if (rc) return member_error(e, i, rc);
+  }
+  return CUP2D_OK;
+}
+
+int cup2d_ensemble_tune(cup2d_ensemble* e, int launches_per_candidate, int* lanes_out,
+                        double* ms_per_launch_out) {
+  if (!e || launches_per_candidate < 1) return CUP2D_ERR_ARG;
+  (void)hipSetDevice(e->m[0]->device);
+  const int n = (int)e->m.size();
+  std::vector<int> cand;
+  for (int l = 1; l < n; l *= 2) cand.push_back(l);
+  cand.push_back(n);
+  const int parity = e->m[0]->host_step & 1;
+  for (cup2d_ctx* c : e->m)
+    if ((c->host_step & 1) != parity)
+      return efail(e, CUP2D_ERR_STATE, "members must be at steps of the same parity");
+  RC(ens_prepare(e));
+  double best = 1e300;
+  int best_l = n;
+  for (int l : cand) {
+    RC(ens_launch(e, l));  // captures; the first replay warms up
+    EHIP(e, hipEventRecord(e->t0, e->stream));
+    for (int k = 0; k < launches_per_candidate; ++k) RC(ens_launch(e, l));
+    EHIP(e, hipEventRecord(e->t1, e->stream));
+    EHIP(e, hipEventSynchronize(e->t1));
+    float ms = 0;
+    EHIP(e, hipEventElapsedTime(&ms, e->t0, e->t1));
+    const double per = ms / launches_per_candidate;
+    if (per < best) {
+      best = per;
+      best_l = l;
+    }
+  }
+  e->lanes = best_l;
+  RC(ens_release(e));
+  if (lanes_out) *lanes_out = best_l;
+  if (ms_per_launch_out) *ms_per_launch_out = best;
   return CUP2D_OK;
 }
 

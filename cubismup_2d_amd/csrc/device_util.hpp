@@ -340,7 +340,10 @@ __device__ inline double flux_corr256(const double* __restrict__ fluxpart, int n
 // A velocity component as the advDiff flux correction leaves it on the
 // boundary (advDiff.cpp:107-115): + corr on the first cell of its direction,
 // - corr on the last (n cells).
-__device__ inline double flux_fixed(double val, int i, int n, double corr) {
+// T is the velocity planes' type (double, or float at FP32 precision, where
+// the correction is applied in float as well).
+template <typename T>
+__device__ inline T flux_fixed(T val, int i, int n, T corr) {
   if (i == 0) return val + corr;
   if (i == n - 1) return val - corr;
   return val;
@@ -350,16 +353,17 @@ __device__ inline double flux_fixed(double val, int i, int n, double corr) {
 // (PressureSingle.cpp:112-128), r = facDiv (u_E - u_W + v_N - v_S) with
 // zero-gradient ghosts, on the advDiff output u, v whose boundary flux
 // correction corr is still to be applied; the same operations as k_rhs.
-__device__ inline double rhs_cell(const double* __restrict__ u, const double* __restrict__ v,
-                                  int NX, int NY, int x, int y, double corr, double facDiv) {
+template <typename T>
+__device__ inline T rhs_cell(const T* __restrict__ u, const T* __restrict__ v, int NX, int NY,
+                             int x, int y, T corr, T facDiv) {
   const int xe = x + 1 < NX ? x + 1 : x, xw = x > 0 ? x - 1 : x;
   const int yn = y + 1 < NY ? y + 1 : y, ys = y > 0 ? y - 1 : y;
-  const double ue = flux_fixed(u[(size_t)y * NX + xe], xe, NX, corr);
-  const double uw = flux_fixed(u[(size_t)y * NX + xw], xw, NX, corr);
-  const double vn = flux_fixed(v[(size_t)yn * NX + x], yn, NY, corr);
-  const double vs = flux_fixed(v[(size_t)ys * NX + x], ys, NY, corr);
-  const double divVx = ue - uw;
-  const double divVy = vn - vs;
+  const T ue = flux_fixed<T>(u[(size_t)y * NX + xe], xe, NX, corr);
+  const T uw = flux_fixed<T>(u[(size_t)y * NX + xw], xw, NX, corr);
+  const T vn = flux_fixed<T>(v[(size_t)yn * NX + x], yn, NY, corr);
+  const T vs = flux_fixed<T>(v[(size_t)ys * NX + x], ys, NY, corr);
+  const T divVx = ue - uw;
+  const T divVy = vn - vs;
   return facDiv * (divVx + divVy);
 }
 
@@ -368,10 +372,11 @@ __device__ inline double rhs_cell(const double* __restrict__ u, const double* __
 // 256-thread workgroup in a fixed order, so the partial is the same whichever
 // kernel forms it.  from_plane: the RHS is read from r, else formed from the
 // advDiff output u, v with the deferred flux correction (rhs_cell).
-__device__ inline void fade_block_sums(const Grid& g, int b, const double* __restrict__ r,
-                                       const double* __restrict__ u,
-                                       const double* __restrict__ v, bool from_plane,
-                                       double corr, double facDiv, double* out, double* lds) {
+template <typename T>
+__device__ inline void fade_block_sums(const Grid& g, int b, const T* __restrict__ r,
+                                       const T* __restrict__ u, const T* __restrict__ v,
+                                       bool from_plane, T corr, T facDiv, double* out,
+                                       double* lds) {
   const int bx = b % g.bpdx, by = b / g.bpdx;
   double sR = 0, sP = 0, sN = 0;
 #pragma unroll
@@ -379,8 +384,8 @@ __device__ inline void fade_block_sums(const Grid& g, int b, const double* __res
     const int ix = k % BS, iy = k / BS, x = bx * BS + ix, y = by * BS + iy;
     const double px = cell_pos(bx, ix, g.hb, g.h), py = cell_pos(by, iy, g.hb, g.h);
     const double fa = fade_arg(g, px, py);
-    const double rc = from_plane ? r[(size_t)y * g.NX + x]
-                                 : rhs_cell(u, v, g.NX, g.NY, x, y, corr, facDiv);
+    const double rc = from_plane ? (double)r[(size_t)y * g.NX + x]
+                                 : (double)rhs_cell<T>(u, v, g.NX, g.NY, x, y, corr, facDiv);
     const double f = rc * (fa * fa);
     if (f < 0) sN -= f;
     if (f > 0) sP += f;
@@ -388,6 +393,27 @@ __device__ inline void fade_block_sums(const Grid& g, int b, const double* __res
   }
   out[0] = sR; out[1] = sP; out[2] = sN;
   block_sum_multi<256, 3>(out, lds);
+}
+
+// The imposed surface velocity (SurfVel) at offset (p0, p1) from the
+// shape's centre: z = e^{i phi} = (p0, p1)/|p|, its powers by complex
+// multiplication (no transcendental calls), normal n = (cos, sin), tangent
+// t = (-sin, cos).  At the centre itself phi = 0.
+__device__ inline void surf_vel(const SurfVel& sv, double p0, double p1, double* ux,
+                                double* uy) {
+  const double r = sqrt(p0 * p0 + p1 * p1);
+  const double zr = r > 0 ? p0 / r : 1.0, zi = r > 0 ? p1 / r : 0.0;
+  double wr = 1, wi = 0, un = 0, ut = 0;
+  const int M = sv.M;
+  for (int m = 0; m < M; ++m) {
+    const double tr = wr * zr - wi * zi;
+    wi = wr * zi + wi * zr;
+    wr = tr;
+    un += sv.coef[m] * wr + sv.coef[M + m] * wi;
+    ut += sv.coef[2 * M + m] * wr + sv.coef[3 * M + m] * wi;
+  }
+  *ux = un * zr - ut * zi;
+  *uy = un * zi + ut * zr;
 }
 
 }  // namespace cup2d

@@ -44,68 +44,93 @@ struct FFTPlan {
   double2* spec = nullptr;
   double2* tw_r = nullptr;
   double2* tw_c = nullptr;
-  double* ghat = nullptr;   // free-space: [ncg][NY+1][G]
-  double* ghatN = nullptr;  // free-space Nyquist column [NY+1]
+  // FP32 precision: the FP32 solve's spectrum and twiddle tables (float2)
+  void* spec32 = nullptr;
+  void* tw_r32 = nullptr;
+  void* tw_c32 = nullptr;
+  double* ghat = nullptr;   // free-space: [ncg][Lc/2+1][G]
+  double* ghatN = nullptr;  // free-space Nyquist column [Lc/2+1]
   double* cosX = nullptr;   // cosine: (float) cos(2 pi i / NX), i < NX
   double* cosY = nullptr;   // cosine: (float) cos(2 pi j / NY), j < NY
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// The FFT core works on V = double2 (FP64) or float2 (FP32) with S the
+// matching scalar; every kernel below is templated on S.
+template <typename S> struct CplxOf;
+template <> struct CplxOf<double> { using type = double2; };
+template <> struct CplxOf<float> { using type = float2; };
+template <typename S> using cplx = typename CplxOf<S>::type;
+template <typename V> struct ScalarOf;
+template <> struct ScalarOf<double2> { using type = double; };
+template <> struct ScalarOf<float2> { using type = float; };
+
+template <typename V>
+__device__ __forceinline__ V mk(typename ScalarOf<V>::type a, typename ScalarOf<V>::type b) {
+  V r;
+  r.x = a;
+  r.y = b;
+  return r;
 }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) {
+  return mk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) { return mk<V>(a.x + b.x, a.y + b.y); }
+template <typename V>
+__device__ __forceinline__ V csub(V a, V b) { return mk<V>(a.x - b.x, a.y - b.y); }
 // multiply by -i (forward) or +i (inverse)
-template <bool FWD>
-__device__ __forceinline__ double2 rot(double2 a) {
-  return FWD ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+template <bool FWD, typename V>
+__device__ __forceinline__ V rot(V a) {
+  return FWD ? mk<V>(a.y, -a.x) : mk<V>(-a.y, a.x);
 }
 
-template <bool FWD>
-__device__ __forceinline__ void dft2(double2& a, double2& b) {
-  const double2 t = a;
+template <bool FWD, typename V>
+__device__ __forceinline__ void dft2(V& a, V& b) {
+  const V t = a;
   a = cadd(t, b);
   b = csub(t, b);
 }
 
-template <bool FWD>
-__device__ __forceinline__ void dft4(double2& x0, double2& x1, double2& x2, double2& x3) {
-  const double2 a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3);
-  const double2 a3 = rot<FWD>(csub(x1, x3));
+template <bool FWD, typename V>
+__device__ __forceinline__ void dft4(V& x0, V& x1, V& x2, V& x3) {
+  const V a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3);
+  const V a3 = rot<FWD>(csub(x1, x3));
   x0 = cadd(a0, a2);
   x1 = cadd(a1, a3);
   x2 = csub(a0, a2);
   x3 = csub(a1, a3);
 }
 
-// W_N^m = exp(-2 pi i m/N) (forward) constants
-template <bool FWD>
-__device__ __forceinline__ double2 w16(int m) {
-  constexpr double c1 = 0.92387953251128673848, s1 = 0.38268343236508978178;
-  constexpr double c2 = 0.70710678118654752440;
-  double2 w;
+// W_N^m = exp(-2 pi i m/N) (forward) constants, rounded to S
+template <typename V, bool FWD>
+__device__ __forceinline__ V w16(int m) {
+  using S = typename ScalarOf<V>::type;
+  constexpr S c1 = (S)0.92387953251128673848, s1 = (S)0.38268343236508978178;
+  constexpr S c2 = (S)0.70710678118654752440;
+  V w;
   switch (m) {
-    case 0: w = make_double2(1, 0); break;
-    case 1: w = make_double2(c1, -s1); break;
-    case 2: w = make_double2(c2, -c2); break;
-    case 3: w = make_double2(s1, -c1); break;
-    case 4: w = make_double2(0, -1); break;
-    case 6: w = make_double2(-c2, -c2); break;
-    case 9: w = make_double2(-c1, s1); break;
-    default: w = make_double2(1, 0); break;
+    case 0: w = mk<V>(1, 0); break;
+    case 1: w = mk<V>(c1, -s1); break;
+    case 2: w = mk<V>(c2, -c2); break;
+    case 3: w = mk<V>(s1, -c1); break;
+    case 4: w = mk<V>(0, -1); break;
+    case 6: w = mk<V>(-c2, -c2); break;
+    case 9: w = mk<V>(-c1, s1); break;
+    default: w = mk<V>(1, 0); break;
   }
   if (!FWD) w.y = -w.y;
   return w;
 }
 
-template <int R, bool FWD>
-__device__ __forceinline__ void dft(double2* v) {
+template <int R, bool FWD, typename V>
+__device__ __forceinline__ void dft(V* v) {
   if constexpr (R == 2) {
     dft2<FWD>(v[0], v[1]);
   } else if constexpr (R == 4) {
     dft4<FWD>(v[0], v[1], v[2], v[3]);
   } else if constexpr (R == 8) {
-    double2 y[2][4];
+    V y[2][4];
 #pragma unroll
     for (int n2 = 0; n2 < 2; ++n2) {
       y[n2][0] = v[n2]; y[n2][1] = v[n2 + 2]; y[n2][2] = v[n2 + 4]; y[n2][3] = v[n2 + 6];
@@ -113,14 +138,14 @@ __device__ __forceinline__ void dft(double2* v) {
     }
     // W8^k1 = W16^(2 k1)
 #pragma unroll
-    for (int k1 = 1; k1 < 4; ++k1) y[1][k1] = cmul(y[1][k1], w16<FWD>(2 * k1));
+    for (int k1 = 1; k1 < 4; ++k1) y[1][k1] = cmul(y[1][k1], w16<V, FWD>(2 * k1));
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) {
       v[k1] = cadd(y[0][k1], y[1][k1]);
       v[k1 + 4] = csub(y[0][k1], y[1][k1]);
     }
   } else {  // 16
-    double2 y[4][4];
+    V y[4][4];
 #pragma unroll
     for (int n2 = 0; n2 < 4; ++n2) {
       y[n2][0] = v[n2]; y[n2][1] = v[n2 + 4]; y[n2][2] = v[n2 + 8]; y[n2][3] = v[n2 + 12];
@@ -129,7 +154,7 @@ __device__ __forceinline__ void dft(double2* v) {
 #pragma unroll
     for (int n2 = 1; n2 < 4; ++n2)
 #pragma unroll
-      for (int k1 = 1; k1 < 4; ++k1) y[n2][k1] = cmul(y[n2][k1], w16<FWD>(n2 * k1));
+      for (int k1 = 1; k1 < 4; ++k1) y[n2][k1] = cmul(y[n2][k1], w16<V, FWD>(n2 * k1));
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) {
       dft4<FWD>(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
@@ -162,8 +187,8 @@ struct TwCount<L, E, L> {
 
 // A thread's twiddle bases for every pass (forward sign; the inverse
 // conjugates at use), loaded once per kernel.
-template <int L, int E, int NS>
-__device__ __forceinline__ void tw_load(const double2* __restrict__ tw, int t, double2* w) {
+template <int L, int E, int NS, typename V>
+__device__ __forceinline__ void tw_load(const V* __restrict__ tw, int t, V* w) {
   if constexpr (NS < L) {
     constexpr int R = TwCount<L, E, NS>::R, PER = E / R, NT = L / E, TSTEP = L / (NS * R);
 #pragma unroll
@@ -174,8 +199,8 @@ __device__ __forceinline__ void tw_load(const double2* __restrict__ tw, int t, d
 
 // pw[r] = b^r, r = 1..R-1, by repeated squaring and products of two lower
 // powers (at most four roundings deep), conjugated for the inverse.
-template <int R, bool FWD>
-__device__ __forceinline__ void tw_powers(double2 b, double2* pw) {
+template <int R, bool FWD, typename V>
+__device__ __forceinline__ void tw_powers(V b, V* pw) {
   if (!FWD) b.y = -b.y;
   pw[1] = b;
 #pragma unroll
@@ -188,11 +213,10 @@ __device__ __forceinline__ void tw_powers(double2 b, double2* pw) {
 // One Stockham pass of radix R at span NS over an L-point sequence held in
 // LDS at lds[base + padE(i)*stride]; NT = L/E threads, E/R butterflies each;
 // w holds this pass's twiddle bases.
-template <int L, int E, int R, int NS, bool FWD>
-__device__ __forceinline__ void stockham_pass(double2* lds, int base, int stride, int t,
-                                              const double2* w) {
+template <int L, int E, int R, int NS, bool FWD, typename V>
+__device__ __forceinline__ void stockham_pass(V* lds, int base, int stride, int t, const V* w) {
   constexpr int NT = L / E, NBF = L / R, PER = E / R;
-  double2 v[PER][R];
+  V v[PER][R];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int j = t + p * NT;
@@ -204,7 +228,7 @@ __device__ __forceinline__ void stockham_pass(double2* lds, int base, int stride
   for (int p = 0; p < PER; ++p) {
     const int j = t + p * NT;
     const int k = j % NS;
-    double2 pw[R];
+    V pw[R];
     tw_powers<R, FWD>(w[p], pw);
 #pragma unroll
     for (int r = 1; r < R; ++r) v[p][r] = cmul(v[p][r], pw[r]);
@@ -218,8 +242,8 @@ __device__ __forceinline__ void stockham_pass(double2* lds, int base, int stride
 
 // First pass (radix E, span 1) straight from registers: x[q] holds element
 // t + q*NT, which is the Stockham input of butterfly j = t.  No twiddles.
-template <int L, int E, bool FWD>
-__device__ __forceinline__ void pass_first(double2* x, double2* lds, int base, int stride, int t) {
+template <int L, int E, bool FWD, typename V>
+__device__ __forceinline__ void pass_first(V* x, V* lds, int base, int stride, int t) {
   dft<E, FWD>(x);
 #pragma unroll
   for (int r = 0; r < E; ++r) lds[base + padE<E>(t * E + r) * stride] = x[r];
@@ -229,12 +253,11 @@ __device__ __forceinline__ void pass_first(double2* x, double2* lds, int base, i
 // Last pass (span NS = L/R) into registers: its outputs j + r*NS land on
 // element t + q*NT with q = p + (E/R)*r, i.e. the same strided order the
 // first pass reads.  Ends with a barrier so the LDS can be reused.
-template <int L, int E, int R, int NS, bool FWD>
-__device__ __forceinline__ void pass_last(double2* lds, int base, int stride, int t,
-                                          const double2* w, double2* x) {
+template <int L, int E, int R, int NS, bool FWD, typename V>
+__device__ __forceinline__ void pass_last(V* lds, int base, int stride, int t, const V* w, V* x) {
   constexpr int NT = L / E, NBF = L / R, PER = E / R;
   static_assert(NS * R == L, "last pass");
-  double2 v[PER][R];
+  V v[PER][R];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int j = t + p * NT;
@@ -244,7 +267,7 @@ __device__ __forceinline__ void pass_last(double2* lds, int base, int stride, in
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
-    double2 pw[R];
+    V pw[R];
     tw_powers<R, FWD>(w[p], pw);
 #pragma unroll
     for (int r = 1; r < R; ++r) v[p][r] = cmul(v[p][r], pw[r]);
@@ -254,9 +277,8 @@ __device__ __forceinline__ void pass_last(double2* lds, int base, int stride, in
   }
 }
 
-template <int L, int E, int NS, bool FWD>
-__device__ __forceinline__ void fft_tail(double2* lds, int base, int stride, int t,
-                                         const double2* w, double2* x) {
+template <int L, int E, int NS, bool FWD, typename V>
+__device__ __forceinline__ void fft_tail(V* lds, int base, int stride, int t, const V* w, V* x) {
   constexpr int REM = L / NS;
   constexpr int R = REM >= E ? E : REM;
   if constexpr (NS * R == L) {
@@ -271,9 +293,8 @@ __device__ __forceinline__ void fft_tail(double2* lds, int base, int stride, int
 // element t + q*NT on entry and the transform's element t + q*NT on exit.
 // The sequence is staged in LDS at lds[base + padE(i)*stride] between
 // passes; w = this thread's twiddle bases from tw_load<L, E, E>.
-template <int L, int E, bool FWD>
-__device__ __forceinline__ void fft_regs(double2* x, double2* lds, int base, int stride, int t,
-                                         const double2* w) {
+template <int L, int E, bool FWD, typename V>
+__device__ __forceinline__ void fft_regs(V* x, V* lds, int base, int stride, int t, const V* w) {
   pass_first<L, E, FWD>(x, lds, base, stride, t);
   fft_tail<L, E, E, FWD>(lds, base, stride, t, w, x);
 }
@@ -285,7 +306,8 @@ constexpr int rows_np() { return (L / E) >= 256 ? 1 : 256 / (L / E); }
 
 // ---------------------------------------------------------- rows forward
 // in: nrows rows of ncols valid reals (row pitch in_pitch), zero beyond;
-// out: spec[cg][y][G], y < spec_rows.  NP row pairs per workgroup.
+// out: spec[cg][y][G], y < spec_rows.  NP row pairs per workgroup.  S is the
+// transform's real type (double or float); every plane it reads is S.
 //
 // SRC_FADE: the input is the pressure RHS on the grid; cells of band blocks
 // are faded as PoissonSolver::cub2rhs does (PoissonSolver.cpp:105-121) while
@@ -299,25 +321,32 @@ constexpr int rows_np() { return (L / E) >= 256 ? 1 : 256 / (L / E); }
 // the values the obstacle RHS kernels left in the RHS plane; then the fade
 // as for SRC_FADE.  u of both rows of a pair goes through LDS for the x
 // neighbours; v of rows y0-1 .. y0+2 stays in registers.
+//
+// store_all (SRC_FADE, SRC_DIV): every cell's final RHS is written to the
+// RHS plane, not only the faded band cells (the FP64 refinement reads it).
 enum { SRC_PLAIN = 0, SRC_FADE = 1, SRC_DIV = 2 };
 struct RowSource {
   Grid g;
   int npart;            // fade-sum workgroups (0: no fade)
-  double* rhs;          // RHS plane: faded band cells written back; obstacle blocks read (SRC_DIV)
+  void* rhs;            // RHS plane (S): faded band cells written back; obstacle blocks read (SRC_DIV)
   const StepState* st;  // fade_corr, flux_corr, dt
-  const double* u;      // SRC_DIV: advDiff output
-  const double* v;
+  const void* u;        // SRC_DIV: advDiff output (S)
+  const void* v;
   const unsigned char* obmap;
   int mirror;           // cosine: columns past ncols read the row reversed
+  int store_all;
 };
 
-template <int L, int NP, int E, int SRC>
+template <int L, int NP, int E, int SRC, typename S>
 __global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
-    const double* in, int in_pitch, int ncols, int nrows, double2* __restrict__ spec,
-    int spec_rows, const double2* __restrict__ tw, RowSource fd) {
+    const S* in, int in_pitch, int ncols, int nrows, cplx<S>* __restrict__ spec,
+    int spec_rows, const cplx<S>* __restrict__ tw, RowSource fd) {
+  using V = cplx<S>;
   constexpr int NT = L / E, PADL = lds_len<L, E>();
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* lds = reinterpret_cast<V*>(smem_raw);
   const int f = threadIdx.x / NT, t = threadIdx.x % NT;
+  S* rhsp = static_cast<S*>(fd.rhs);
   // SRC_DIV reads the v rows next to its pair too: consecutive pairs run on
   // one XCD (workgroup b runs on XCD b % 8) so those rows are L2 hits
   int wg = blockIdx.x;
@@ -325,33 +354,35 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
   const int y0 = 2 * (wg * NP + f);
   const bool valid = y0 < nrows;
   const int base = f * PADL;
-  double2 w[TwCount<L, E, E>::n];
+  V w[TwCount<L, E, E>::n];
   tw_load<L, E, E>(tw, t, w);
-  double2 x[E];
+  V x[E];
   if constexpr (SRC == SRC_DIV) {
     const Grid& g = fd.g;
+    const S* fu = static_cast<const S*>(fd.u);
+    const S* fv = static_cast<const S*>(fd.v);
     const int NX = ncols, NY = nrows;
     const int yc = valid ? y0 : 0;
     const int ym = yc > 0 ? yc - 1 : yc, yp = yc + 2 < NY ? yc + 2 : yc + 1;
-    double* ud = (double*)lds + (size_t)f * 2 * PADL;  // u of rows y0, y0+1
+    S* ud = (S*)lds + (size_t)f * 2 * PADL;  // u of rows y0, y0+1
     // every load of the six rows is issued before the first use, so a wave
     // has all of them in flight (storing u into LDS inside this loop would
     // wait for each group in turn)
-    double u0[E], u1[E], vm[E], v0[E], v1[E], vp[E];
+    S u0[E], u1[E], vm[E], v0[E], v1[E], vp[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int i = t + q * NT;
       if (i < NX) {
-        u0[q] = fd.u[(size_t)yc * NX + i];
-        u1[q] = fd.u[(size_t)(yc + 1) * NX + i];
-        vm[q] = fd.v[(size_t)ym * NX + i];
-        v0[q] = fd.v[(size_t)yc * NX + i];
-        v1[q] = fd.v[(size_t)(yc + 1) * NX + i];
-        vp[q] = fd.v[(size_t)yp * NX + i];
+        u0[q] = fu[(size_t)yc * NX + i];
+        u1[q] = fu[(size_t)(yc + 1) * NX + i];
+        vm[q] = fv[(size_t)ym * NX + i];
+        v0[q] = fv[(size_t)yc * NX + i];
+        v1[q] = fv[(size_t)(yc + 1) * NX + i];
+        vp[q] = fv[(size_t)yp * NX + i];
       }
     }
-    const double fcorr = fd.st->flux_corr;  // k_flux_reduce, after advDiff
-    const double facDiv = 0.5 * g.h / fd.st->dt;
+    const S fcorr = (S)fd.st->flux_corr;  // k_flux_reduce, after advDiff
+    const S facDiv = (S)(0.5 * g.h / fd.st->dt);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int i = t + q * NT;
@@ -364,36 +395,36 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int i = t + q * NT;
-      double r0 = 0, r1 = 0;
+      S r0 = 0, r1 = 0;
       if (valid && i < NX) {
         const int xe = i + 1 < NX ? i + 1 : i, xw = i > 0 ? i - 1 : i;
-        const double ue0 = flux_fixed(ud[xe], xe, NX, fcorr);
-        const double uw0 = flux_fixed(ud[xw], xw, NX, fcorr);
-        const double ue1 = flux_fixed(ud[L + xe], xe, NX, fcorr);
-        const double uw1 = flux_fixed(ud[L + xw], xw, NX, fcorr);
-        const double vs0 = flux_fixed(vm[q], ym, NY, fcorr);
-        const double vc0 = flux_fixed(v0[q], y0, NY, fcorr);
-        const double vc1 = flux_fixed(v1[q], y0 + 1, NY, fcorr);
-        const double vn1 = flux_fixed(vp[q], yp, NY, fcorr);
+        const S ue0 = flux_fixed<S>(ud[xe], xe, NX, fcorr);
+        const S uw0 = flux_fixed<S>(ud[xw], xw, NX, fcorr);
+        const S ue1 = flux_fixed<S>(ud[L + xe], xe, NX, fcorr);
+        const S uw1 = flux_fixed<S>(ud[L + xw], xw, NX, fcorr);
+        const S vs0 = flux_fixed<S>(vm[q], ym, NY, fcorr);
+        const S vc0 = flux_fixed<S>(v0[q], y0, NY, fcorr);
+        const S vc1 = flux_fixed<S>(v1[q], y0 + 1, NY, fcorr);
+        const S vn1 = flux_fixed<S>(vp[q], yp, NY, fcorr);
         r0 = facDiv * ((ue0 - uw0) + (vc1 - vs0));
         r1 = facDiv * ((ue1 - uw1) + (vn1 - vc0));
         const int bx = i / BS;
-        if (fd.obmap[(y0 / BS) * g.bpdx + bx]) r0 = fd.rhs[(size_t)y0 * NX + i];
-        if (fd.obmap[((y0 + 1) / BS) * g.bpdx + bx]) r1 = fd.rhs[(size_t)(y0 + 1) * NX + i];
+        if (fd.obmap[(y0 / BS) * g.bpdx + bx]) r0 = rhsp[(size_t)y0 * NX + i];
+        if (fd.obmap[((y0 + 1) / BS) * g.bpdx + bx]) r1 = rhsp[(size_t)(y0 + 1) * NX + i];
       }
-      x[q] = make_double2(r0, r1);
+      x[q] = mk<V>(r0, r1);
     }
     __syncthreads();  // ud is reused below
   } else {
-    const double* r0 = in + (size_t)(valid ? y0 : 0) * in_pitch;
-    const double* r1 = r0 + in_pitch;
+    const S* r0 = in + (size_t)(valid ? y0 : 0) * in_pitch;
+    const S* r1 = r0 + in_pitch;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       int i = t + q * NT;
       if (fd.mirror && i >= ncols) i = 2 * ncols - 1 - i;
-      double a = 0, b = 0;
+      S a = 0, b = 0;
       if (valid && i < ncols) { a = r0[i]; b = r1[i]; }
-      x[q] = make_double2(a, b);
+      x[q] = mk<V>(a, b);
     }
   }
   if constexpr (SRC != SRC_PLAIN) {
@@ -415,14 +446,24 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
           const double px = cell_pos(bx, i % BS, g.hb, g.h);
           if (row0 || col) {
             const double py = cell_pos(by0, y0 % BS, g.hb, g.h);
-            x[q].x = fade_cell(x[q].x, fade_arg(g, px, py), corr);
-            if (own) fd.rhs[(size_t)y0 * in_pitch + i] = x[q].x;
+            x[q].x = (S)fade_cell((double)x[q].x, fade_arg(g, px, py), corr);
+            if (own && !fd.store_all) rhsp[(size_t)y0 * in_pitch + i] = x[q].x;
           }
           if (row1 || col) {
             const double py = cell_pos(by1, (y0 + 1) % BS, g.hb, g.h);
-            x[q].y = fade_cell(x[q].y, fade_arg(g, px, py), corr);
-            if (own) fd.rhs[(size_t)(y0 + 1) * in_pitch + i] = x[q].y;
+            x[q].y = (S)fade_cell((double)x[q].y, fade_arg(g, px, py), corr);
+            if (own && !fd.store_all) rhsp[(size_t)(y0 + 1) * in_pitch + i] = x[q].y;
           }
+        }
+      }
+    }
+    if (fd.store_all && valid) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int i = t + q * NT;
+        if (i < ncols) {
+          rhsp[(size_t)y0 * in_pitch + i] = x[q].x;
+          rhsp[(size_t)(y0 + 1) * in_pitch + i] = x[q].y;
         }
       }
     }
@@ -434,55 +475,73 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
   for (int q = E / 2; q < E; ++q) lds[base + padE<E>(t + q * NT)] = x[q];
   __syncthreads();
   if (!valid) return;
+  const S half = (S)0.5;
 #pragma unroll
   for (int q = 0; q < E / 2; ++q) {
     const int k = t + q * NT;  // 0 .. L/2-1
-    const double2 Z = x[q];
-    double2 A, B;
+    const V Z = x[q];
+    V A, B;
     if (k == 0) {
-      const double2 ZN = x[E / 2];  // element L/2
-      A = make_double2(Z.x, ZN.x);
-      B = make_double2(Z.y, ZN.y);
+      const V ZN = x[E / 2];  // element L/2
+      A = mk<V>(Z.x, ZN.x);
+      B = mk<V>(Z.y, ZN.y);
     } else {
-      const double2 Zm = lds[base + padE<E>(L - k)];
-      const double2 sm = make_double2(Z.x + Zm.x, Z.y - Zm.y);  // Z + conj(Zm)
-      const double2 d = make_double2(Z.x - Zm.x, Z.y + Zm.y);   // Z - conj(Zm)
-      A = make_double2(0.5 * sm.x, 0.5 * sm.y);
-      B = make_double2(0.5 * d.y, -0.5 * d.x);
+      const V Zm = lds[base + padE<E>(L - k)];
+      const V sm = mk<V>(Z.x + Zm.x, Z.y - Zm.y);  // Z + conj(Zm)
+      const V d = mk<V>(Z.x - Zm.x, Z.y + Zm.y);   // Z - conj(Zm)
+      A = mk<V>(half * sm.x, half * sm.y);
+      B = mk<V>(half * d.y, -half * d.x);
     }
     const int cg = k / G, c = k % G;
-    double2* o = spec + ((size_t)cg * spec_rows + y0) * G + c;
+    V* o = spec + ((size_t)cg * spec_rows + y0) * G + c;
     o[0] = A;
     o[G] = B;
   }
 }
 
 // ---------------------------------------------------------- rows inverse
-template <int L, int NP, int E>
-__global__ __launch_bounds__(NP * L / E) void k_rows_inv(const double2* __restrict__ spec,
-                                                         int spec_rows, double* __restrict__ out,
+// Output of the inverse row pass (x = the transform's real output):
+// OUT_SET: out (double) = x; OUT_ADD: out (double) += x (the FP64 update of a
+// refinement sweep); OUT_RES: out (float) = b - x, b the float RHS plane
+// (the residual of a refinement sweep, x = the FP64 operator applied to p).
+// With S = double and OUT_SET the pass is the plain FP64 one.
+enum { OUT_SET = 0, OUT_ADD = 1, OUT_RES = 2 };
+// OUT_ADD also subtracts *mean (the solution's mean before the update, which
+// the spectral operator cannot see: the solve's convention is a zero mean).
+// OUT_RES also writes the workgroup's statistics of r and b to
+// respart[5 wg .. 5 wg + 4]: cell count, sum r, sum (r - mean_wg r)^2, sum b,
+// sum (b - mean_wg b)^2 (the residual norm of the sweep without the mean,
+// which the operator does not see; k_res_reduce merges them).
+template <int L, int NP, int E, typename S, int OUT>
+__global__ __launch_bounds__(NP * L / E) void k_rows_inv(const cplx<S>* __restrict__ spec,
+                                                         int spec_rows, void* __restrict__ outv,
+                                                         const float* __restrict__ bres,
                                                          int out_pitch, int ncols, int nrows,
-                                                         const double2* __restrict__ tw) {
+                                                         const cplx<S>* __restrict__ tw,
+                                                         const double* __restrict__ mean,
+                                                         double* __restrict__ respart) {
+  using V = cplx<S>;
   constexpr int NT = L / E, PADL = lds_len<L, E>();
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* lds = reinterpret_cast<V*>(smem_raw);
   const int f = threadIdx.x / NT, t = threadIdx.x % NT;
   const int y0 = 2 * (blockIdx.x * NP + f);
   const bool valid = y0 < nrows;
   const int base = f * PADL;
-  double2 w[TwCount<L, E, E>::n];
+  V w[TwCount<L, E, E>::n];
   tw_load<L, E, E>(tw, t, w);
-  double2 x[E];
+  V x[E];
   // Z_k = A_k + i B_k and Z_{L-k} = conj(A_k) + i conj(B_k); DC and Nyquist
   // of both rows are packed in slot 0 (real parts only).  All loads are
   // issued before the first LDS store, so they are in flight together.
-  double2 As[E / 2], Bs[E / 2];
+  V As[E / 2], Bs[E / 2];
 #pragma unroll
   for (int q = 0; q < E / 2; ++q) {
     const int k = t + q * NT;
-    As[q] = make_double2(0, 0);
-    Bs[q] = make_double2(0, 0);
+    As[q] = mk<V>(0, 0);
+    Bs[q] = mk<V>(0, 0);
     if (valid) {
-      const double2* s = spec + ((size_t)(k / G) * spec_rows + y0) * G + (k % G);
+      const V* s = spec + ((size_t)(k / G) * spec_rows + y0) * G + (k % G);
       As[q] = s[0];
       Bs[q] = s[G];
     }
@@ -490,13 +549,13 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_inv(const double2* __restri
 #pragma unroll
   for (int q = 0; q < E / 2; ++q) {
     const int k = t + q * NT;
-    const double2 A = As[q], B = Bs[q];
+    const V A = As[q], B = Bs[q];
     if (k == 0) {
-      x[0] = make_double2(A.x, B.x);
-      x[E / 2] = make_double2(A.y, B.y);
+      x[0] = mk<V>(A.x, B.x);
+      x[E / 2] = mk<V>(A.y, B.y);
     } else {
-      x[q] = make_double2(A.x - B.y, A.y + B.x);
-      lds[base + padE<E>(L - k)] = make_double2(A.x + B.y, -A.y + B.x);
+      x[q] = mk<V>(A.x - B.y, A.y + B.x);
+      lds[base + padE<E>(L - k)] = mk<V>(A.x + B.y, -A.y + B.x);
     }
   }
   __syncthreads();
@@ -507,29 +566,126 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_inv(const double2* __restri
   }
   __syncthreads();
   fft_regs<L, E, false>(x, lds, base, 1, t, w);
-  if (!valid) return;
-  double* o0 = out + (size_t)y0 * out_pitch;
-  double* o1 = o0 + out_pitch;
+  if constexpr (OUT == OUT_RES) {
+    constexpr int NTB = NP * L / E;
+    double rr[2 * E], bv[2 * E];
+    double rs = 0, bs = 0, cnt = 0;
+    if (valid) {
+      float* o0 = static_cast<float*>(outv) + (size_t)y0 * out_pitch;
+      float* o1 = o0 + out_pitch;
+      const float* b0 = bres + (size_t)y0 * out_pitch;
+      const float* b1 = b0 + out_pitch;
 #pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int i = t + q * NT;
-    if (i < ncols) {
-      o0[i] = x[q].x;
-      o1[i] = x[q].y;
+      for (int q = 0; q < E; ++q) {
+        const int i = t + q * NT;
+        rr[2 * q] = rr[2 * q + 1] = bv[2 * q] = bv[2 * q + 1] = 0;
+        if (i < ncols) {
+          const double bb0 = b0[i], bb1 = b1[i];
+          const double r0 = bb0 - (double)x[q].x, r1 = bb1 - (double)x[q].y;
+          o0[i] = (float)r0;
+          o1[i] = (float)r1;
+          rr[2 * q] = r0; rr[2 * q + 1] = r1;
+          bv[2 * q] = bb0; bv[2 * q + 1] = bb1;
+          rs += r0 + r1;
+          bs += bb0 + bb1;
+          cnt += 2;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 2 * E; ++q) rr[q] = bv[q] = 0;
+    }
+    // every thread reaches the reductions (no early return in this mode)
+    double* red = reinterpret_cast<double*>(smem_raw);
+    const double n_wg = block_sum<NTB>(cnt, red);
+    rs = block_sum<NTB>(rs, red);
+    bs = block_sum<NTB>(bs, red);
+    const double mr = n_wg > 0 ? rs / n_wg : 0, mb = n_wg > 0 ? bs / n_wg : 0;
+    double r2 = 0, b2 = 0;
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        if (t + q * NT < ncols) {
+          const double d0 = rr[2 * q] - mr, d1 = rr[2 * q + 1] - mr;
+          const double e0 = bv[2 * q] - mb, e1 = bv[2 * q + 1] - mb;
+          r2 += d0 * d0 + d1 * d1;
+          b2 += e0 * e0 + e1 * e1;
+        }
+      }
+    }
+    r2 = block_sum<NTB>(r2, red);
+    b2 = block_sum<NTB>(b2, red);
+    if (threadIdx.x == 0) {
+      double* o = respart + 5 * blockIdx.x;
+      o[0] = n_wg; o[1] = rs; o[2] = r2; o[3] = bs; o[4] = b2;
+    }
+  } else {
+    if (!valid) return;
+    double* o0 = static_cast<double*>(outv) + (size_t)y0 * out_pitch;
+    double* o1 = o0 + out_pitch;
+    const double pm = OUT == OUT_ADD && mean ? *mean : 0.0;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int i = t + q * NT;
+      if (i < ncols) {
+        if constexpr (OUT == OUT_ADD) {
+          o0[i] += (double)x[q].x - pm;
+          o1[i] += (double)x[q].y - pm;
+        } else {
+          o0[i] = (double)x[q].x;
+          o1[i] = (double)x[q].y;
+        }
+      }
     }
   }
 }
 
+// relative residual norm |r - mean r| / |b - mean b| (r = b - A p) of a
+// refinement sweep from the inverse row pass's per-workgroup statistics,
+// merged by Chan's formula (M2 = sum M2_wg + sum n_wg (mean_wg - mean)^2),
+// into *out
+__global__ __launch_bounds__(256) void k_res_reduce(const double* __restrict__ respart, int n,
+                                                    double* __restrict__ out) {
+  __shared__ double lds[4];
+  double cnt = 0, rs = 0, bs = 0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    cnt += respart[5 * i];
+    rs += respart[5 * i + 1];
+    bs += respart[5 * i + 3];
+  }
+  cnt = block_sum<256>(cnt, lds);
+  rs = block_sum<256>(rs, lds);
+  bs = block_sum<256>(bs, lds);
+  const double mr = cnt > 0 ? rs / cnt : 0, mb = cnt > 0 ? bs / cnt : 0;
+  double r2 = 0, b2 = 0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double nw = respart[5 * i];
+    if (nw <= 0) continue;
+    const double dr = respart[5 * i + 1] / nw - mr, db = respart[5 * i + 3] / nw - mb;
+    r2 += respart[5 * i + 2] + nw * dr * dr;
+    b2 += respart[5 * i + 4] + nw * db * db;
+  }
+  r2 = block_sum<256>(r2, lds);
+  b2 = block_sum<256>(b2, lds);
+  if (threadIdx.x == 0) *out = b2 > 0 ? sqrt(r2 / b2) : sqrt(r2);
+}
+
 // --------------------------------------------------------------- columns
-// MODE 0: periodic solve, 1: free-space solve (zero-pruned), 2: forward only.
+// MODE 0: periodic solve, 1: free-space solve (zero-pruned), 2: forward only,
+// 3: cosine solve, 4: periodic apply, 5: cosine apply.  The apply modes
+// multiply by the inverse of the solve symbol (zero at the DC mode, which
+// the solve discards): the FP64 spectral operator whose residual the
+// refinement sweeps correct.
 struct ColParams {
   int NX, NY;            // physical grid
   int rows;              // data rows stored per column (NY, or Lc for MODE 2)
   double waveFactX, waveFactY, norm;
-  const double* ghat;    // MODE 1
+  const double* ghat;    // MODE 1: the Green's function spectrum, even-folded at GH = Lc/2
   const double* ghatN;
-  const double* cosX;    // MODE 3
+  int GH;
+  const double* cosX;    // MODE 3, 5
   const double* cosY;
+  double* dc_out;        // MODE 4, 5: receives the input's mean (its DC coefficient * norm)
 };
 
 __device__ __forceinline__ double periodic_factor(int kx, int ky, const ColParams& P) {
@@ -540,31 +696,48 @@ __device__ __forceinline__ double periodic_factor(int kx, int ky, const ColParam
   const double kinv = -1 / (rkx * rkx + rky * rky);
   return kinv * P.norm;
 }
+// the operator whose pseudo-inverse periodic_factor applies: -k^2 norm
+__device__ __forceinline__ double periodic_apply(int kx, int ky, const ColParams& P) {
+  const int kyy = ky <= P.NY / 2 ? ky : P.NY - ky;
+  if (kx == 0 && kyy == 0) return 0.0;
+  const double rkx = kx * P.waveFactX, rky = kyy * P.waveFactY;
+  return -(rkx * rkx + rky * rky) * P.norm;
+}
 
 // FFTW_dirichlet::_solve (FFTW_dirichlet.h:45-61) at DCT indices (i, j) <
 // (NX, NY).  Line 51 declares the blend weight as `static constexpr tol =
 // 0.1;`, which has no type; it is taken as the double 0.1 the initializer
 // states.  COScoef are floats (FFTW_dirichlet.h:26-27,65-68).
-__device__ __forceinline__ double cosine_factor(int i, int j, const ColParams& P) {
-  if (i == 0 && j == 0) return 0.0;  // in_out[0] = 0
+__device__ __forceinline__ double cosine_denom(int i, int j, const ColParams& P) {
   const double tol = 0.1;
   const double rkx = (i + 0.5) * P.waveFactX, rky = (j + 0.5) * P.waveFactY;
   const double denomFD = 1 - P.cosX[i] / 2 - P.cosY[j] / 2;
   const double denomSP = rkx * rkx + rky * rky;
-  return -P.norm / ((1 - tol) * denomFD + tol * denomSP);
+  return (1 - tol) * denomFD + tol * denomSP;
+}
+__device__ __forceinline__ double cosine_factor(int i, int j, const ColParams& P) {
+  if (i == 0 && j == 0) return 0.0;  // in_out[0] = 0
+  return -P.norm / cosine_denom(i, j, P);
+}
+__device__ __forceinline__ double cosine_apply(int i, int j, const ColParams& P) {
+  if (i == 0 && j == 0) return 0.0;
+  return -P.norm * cosine_denom(i, j, P);
 }
 
 template <int LC, int MODE>
 __device__ __forceinline__ double col_symbol(int kx, int cg, int c, int ky, const ColParams& P) {
   if constexpr (MODE == 0) {
     return periodic_factor(kx, ky, P);
-  } else if constexpr (MODE == 3) {
+  } else if constexpr (MODE == 4) {
+    return periodic_apply(kx, ky, P);
+  } else if constexpr (MODE == 3 || MODE == 5) {
     // symbol extended evenly about NX and NY; zero at the mirror Nyquist
     if (kx >= P.NX || ky == P.NY) return 0.0;
-    return cosine_factor(kx, ky < P.NY ? ky : LC - ky, P);
+    const int kyy = ky < P.NY ? ky : LC - ky;
+    return MODE == 3 ? cosine_factor(kx, kyy, P) : cosine_apply(kx, kyy, P);
   } else {
-    const int kyy = ky <= P.NY ? ky : LC - ky;
-    return P.ghat[((size_t)cg * (P.NY + 1) + kyy) * G + c];
+    const int kyy = ky <= P.GH ? ky : LC - ky;
+    return P.ghat[((size_t)cg * (P.GH + 1) + kyy) * G + c];
   }
 }
 
@@ -589,26 +762,29 @@ __device__ __forceinline__ void col_group_of_block(int b, int nblocks, int* cg, 
   *half = lin % H;
 }
 
-template <int LC, int MODE, int E>
-__global__ __launch_bounds__(CW * LC / E) void k_cols(double2* __restrict__ spec, ColParams P,
-                                                      const double2* __restrict__ tw) {
+template <int LC, int MODE, int E, typename S>
+__global__ __launch_bounds__(CW * LC / E) void k_cols(cplx<S>* __restrict__ spec, ColParams P,
+                                                      const cplx<S>* __restrict__ tw) {
+  using V = cplx<S>;
   constexpr int NT = LC / E;
-  extern __shared__ double2 lds[];
+  constexpr bool MIRROR = MODE == 3 || MODE == 5;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* lds = reinterpret_cast<V*>(smem_raw);
   const int cl = threadIdx.x % CW, t = threadIdx.x / CW;
   int cg, half;
   col_group_of_block(blockIdx.x, gridDim.x, &cg, &half);
   const int c = half * CW + cl;  // column inside the group
-  double2* col = spec + (size_t)cg * P.rows * G;
-  double2 w[TwCount<LC, E, E>::n];
+  V* col = spec + (size_t)cg * P.rows * G;
+  V w[TwCount<LC, E, E>::n];
   tw_load<LC, E, E>(tw, t, w);
-  double2 x[E];
+  V x[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int i = t + q * NT;
-    if constexpr (MODE == 3)  // mirrored grid: rows past NY are rows NY-1, NY-2, ...
+    if constexpr (MIRROR)  // mirrored grid: rows past NY are rows NY-1, NY-2, ...
       x[q] = col[(size_t)(i < P.rows ? i : 2 * P.rows - 1 - i) * G + c];
     else
-      x[q] = (MODE != 1 || i < P.rows) ? col[(size_t)i * G + c] : make_double2(0, 0);
+      x[q] = (MODE != 1 || i < P.rows) ? col[(size_t)i * G + c] : mk<V>(0, 0);
   }
   fft_regs<LC, E, true>(x, lds, cl, CW, t, w);
   if constexpr (MODE == 2) {
@@ -623,6 +799,11 @@ __global__ __launch_bounds__(CW * LC / E) void k_cols(double2* __restrict__ spec
       // C'(k) = C(k) (s0+sN)/2 + conj(C(-k)) (s0-sN)/2
 #pragma unroll
       for (int q = 0; q < E; ++q) lds[padE<E>(t + q * NT) * CW + cl] = x[q];
+      // apply modes: the DC coefficient (real part of the packed DC column
+      // at ky = 0) is the input's sum, times 4 on the mirrored grid; the
+      // operator discards it, the refinement removes it from the solution
+      if constexpr (MODE == 4 || MODE == 5)
+        if (t == 0 && cl == 0 && P.dc_out) *P.dc_out = (double)x[0].x * P.norm;
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < E; ++q) {
@@ -631,40 +812,42 @@ __global__ __launch_bounds__(CW * LC / E) void k_cols(double2* __restrict__ spec
         if (kx == 0) {
           double sN;
           if constexpr (MODE == 0) sN = periodic_factor(P.NX / 2, i, P);
-          else if constexpr (MODE == 3) sN = 0.0;  // mirror Nyquist kx = NX
-          else sN = P.ghatN[i <= P.NY ? i : LC - i];
-          const double2 m = lds[padE<E>((LC - i) % LC) * CW + cl];
-          const double a = 0.5 * (s0 + sN), b = 0.5 * (s0 - sN);
-          x[q] = make_double2(x[q].x * a + m.x * b, x[q].y * a - m.y * b);
+          else if constexpr (MODE == 4) sN = periodic_apply(P.NX / 2, i, P);
+          else if constexpr (MIRROR) sN = 0.0;  // mirror Nyquist kx = NX
+          else sN = P.ghatN[i <= P.GH ? i : LC - i];
+          const V m = lds[padE<E>((LC - i) % LC) * CW + cl];
+          const S a = (S)(0.5 * (s0 + sN)), b = (S)(0.5 * (s0 - sN));
+          x[q] = mk<V>(x[q].x * a + m.x * b, x[q].y * a - m.y * b);
         } else {
-          x[q] = make_double2(x[q].x * s0, x[q].y * s0);
+          const S ss = (S)s0;
+          x[q] = mk<V>(x[q].x * ss, x[q].y * ss);
         }
       }
       __syncthreads();
     } else {
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        const double sy = col_symbol<LC, MODE>(kx, cg, c, t + q * NT, P);
-        x[q] = make_double2(x[q].x * sy, x[q].y * sy);
+        const S sy = (S)col_symbol<LC, MODE>(kx, cg, c, t + q * NT, P);
+        x[q] = mk<V>(x[q].x * sy, x[q].y * sy);
       }
     }
     fft_regs<LC, E, false>(x, lds, cl, CW, t, w);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int i = t + q * NT;
-      if ((MODE != 1 && MODE != 3) || i < P.rows) col[(size_t)i * G + c] = x[q];
+      if ((MODE != 1 && !MIRROR) || i < P.rows) col[(size_t)i * G + c] = x[q];
     }
   }
 }
 
-// Green's function spectrum: Re(FFT G) * norm, even-folded to ky <= NY.
-// spec holds the full forward transform of the (2NX)x(2NY) kernel.
-__global__ void k_ghat_extract(const double2* __restrict__ spec, int Lc, int ncg, int NY,
+// Green's function spectrum: Re(FFT G) * norm, even-folded to ky <= GH =
+// Lc/2.  spec holds the full forward transform of the Lr x Lc kernel.
+__global__ void k_ghat_extract(const double2* __restrict__ spec, int Lc, int ncg, int GH,
                                double norm, double* __restrict__ ghat,
                                double* __restrict__ ghatN) {
-  const int n = ncg * G * (NY + 1);
+  const int n = ncg * G * (GH + 1);
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const int c = q % G, ky = (q / G) % (NY + 1), cg = q / (G * (NY + 1));
+    const int c = q % G, ky = (q / G) % (GH + 1), cg = q / (G * (GH + 1));
     const double2 C = spec[((size_t)cg * Lc + ky) * G + c];
     if (cg == 0 && c == 0) {
       const double2 Cm = spec[((size_t)0 * Lc + (Lc - ky) % Lc) * G + 0];
@@ -678,102 +861,134 @@ __global__ void k_ghat_extract(const double2* __restrict__ spec, int Lc, int ncg
 }
 
 // ----------------------------------------------------------- dispatch
-template <int L>
-static void launch_rows_fwd(cup2d_ctx* c, const double* in, int pitch, int ncols, int nrows,
-                            double2* spec, int spec_rows, const double2* tw, int src,
+template <int L, typename S>
+static void launch_rows_fwd(cup2d_ctx* c, const S* in, int pitch, int ncols, int nrows,
+                            cplx<S>* spec, int spec_rows, const cplx<S>* tw, int src,
                             const RowSource& fd) {
   constexpr int NP = rows_np<L, FE>();
-  const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(double2);
+  const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(cplx<S>);
   const dim3 grid((nrows / 2 + NP - 1) / NP), block(NP * L / FE);
   if (src == SRC_DIV)
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_DIV>), grid, block, shm, c->stream, in, pitch,
-                       ncols, nrows, spec, spec_rows, tw, fd);
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_DIV, S>), grid, block, shm, c->stream, in,
+                       pitch, ncols, nrows, spec, spec_rows, tw, fd);
   else if (src == SRC_FADE)
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_FADE>), grid, block, shm, c->stream, in,
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_FADE, S>), grid, block, shm, c->stream, in,
                        pitch, ncols, nrows, spec, spec_rows, tw, fd);
   else
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_PLAIN>), grid, block, shm, c->stream, in,
-                       pitch, ncols, nrows, spec, spec_rows, tw, fd);
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_PLAIN, S>), grid, block, shm, c->stream,
+                       in, pitch, ncols, nrows, spec, spec_rows, tw, fd);
 }
-template <int L>
-static void launch_rows_inv(cup2d_ctx* c, const double2* spec, int spec_rows, double* out,
-                            int pitch, int ncols, int nrows, const double2* tw) {
+template <int L, typename S, int OUT>
+static void launch_rows_inv(cup2d_ctx* c, const cplx<S>* spec, int spec_rows, void* out,
+                            const float* bres, int pitch, int ncols, int nrows,
+                            const cplx<S>* tw, const double* mean) {
+  c->rows_inv_wgs = (nrows / 2 + rows_np<L, FE>() - 1) / rows_np<L, FE>();
   constexpr int NP = rows_np<L, FE>();
-  const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(double2);
+  const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(cplx<S>);
   const int pairs = nrows / 2;
-  CUP2D_TIMED_LAUNCH(c, (k_rows_inv<L, NP, FE>), dim3((pairs + NP - 1) / NP), dim3(NP * L / FE),
-                     shm, c->stream, spec, spec_rows, out, pitch, ncols, nrows, tw);
+  CUP2D_TIMED_LAUNCH(c, (k_rows_inv<L, NP, FE, S, OUT>), dim3((pairs + NP - 1) / NP),
+                     dim3(NP * L / FE), shm, c->stream, spec, spec_rows, out, bres, pitch,
+                     ncols, nrows, tw, mean, c->respart);
 }
-template <int LC, int MODE>
-static void launch_cols(cup2d_ctx* c, double2* spec, int ncg, const ColParams& P,
-                        const double2* tw) {
-  const size_t shm = (size_t)lds_len<LC, FE>() * CW * sizeof(double2);
-  CUP2D_TIMED_LAUNCH(c, (k_cols<LC, MODE, FE>), dim3(ncg * (G / CW)), dim3(CW * LC / FE), shm,
+template <int LC, int MODE, typename S>
+static void launch_cols(cup2d_ctx* c, cplx<S>* spec, int ncg, const ColParams& P,
+                        const cplx<S>* tw) {
+  const size_t shm = (size_t)lds_len<LC, FE>() * CW * sizeof(cplx<S>);
+  CUP2D_TIMED_LAUNCH(c, (k_cols<LC, MODE, FE, S>), dim3(ncg * (G / CW)), dim3(CW * LC / FE), shm,
                      c->stream, spec, P, tw);
 }
 
-#define CUP2D_POW2_CASES(M, ...)                                                 \
-  switch (L) {                                                                   \
-    case 32: M<32>(__VA_ARGS__); break;                                          \
-    case 64: M<64>(__VA_ARGS__); break;                                          \
-    case 128: M<128>(__VA_ARGS__); break;                                        \
-    case 256: M<256>(__VA_ARGS__); break;                                        \
-    case 512: M<512>(__VA_ARGS__); break;                                        \
-    case 1024: M<1024>(__VA_ARGS__); break;                                      \
-    case 2048: M<2048>(__VA_ARGS__); break;                                      \
-    case 4096: M<4096>(__VA_ARGS__); break;                                      \
-    case 8192: M<8192>(__VA_ARGS__); break;                                      \
-    default: return CUP2D_ERR_ARG;                                               \
-  }
-
-static int rows_fwd(cup2d_ctx* c, int L, const double* in, int pitch, int ncols, int nrows,
-                    double2* spec, int spec_rows, const double2* tw, int src,
+template <typename S>
+static int rows_fwd(cup2d_ctx* c, int L, const S* in, int pitch, int ncols, int nrows,
+                    cplx<S>* spec, int spec_rows, const cplx<S>* tw, int src,
                     const RowSource& fd) {
-  CUP2D_POW2_CASES(launch_rows_fwd, c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd);
+  switch (L) {
+    case 32: launch_rows_fwd<32, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    case 64: launch_rows_fwd<64, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    case 128: launch_rows_fwd<128, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    case 256: launch_rows_fwd<256, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    case 512: launch_rows_fwd<512, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    case 1024: launch_rows_fwd<1024, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    case 2048: launch_rows_fwd<2048, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    case 4096: launch_rows_fwd<4096, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    case 8192: launch_rows_fwd<8192, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    default: return CUP2D_ERR_ARG;
+  }
   return CUP2D_OK;
 }
-static int rows_inv(cup2d_ctx* c, int L, const double2* spec, int spec_rows, double* out,
-                    int pitch, int ncols, int nrows, const double2* tw) {
-  CUP2D_POW2_CASES(launch_rows_inv, c, spec, spec_rows, out, pitch, ncols, nrows, tw);
+template <typename S, int OUT>
+static int rows_inv(cup2d_ctx* c, int L, const cplx<S>* spec, int spec_rows, void* out,
+                    const float* bres, int pitch, int ncols, int nrows, const cplx<S>* tw,
+                    const double* mean = nullptr) {
+  switch (L) {
+    case 32: launch_rows_inv<32, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    case 64: launch_rows_inv<64, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    case 128: launch_rows_inv<128, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    case 256: launch_rows_inv<256, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    case 512: launch_rows_inv<512, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    case 1024: launch_rows_inv<1024, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    case 2048: launch_rows_inv<2048, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    case 4096: launch_rows_inv<4096, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    case 8192: launch_rows_inv<8192, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    default: return CUP2D_ERR_ARG;
+  }
   return CUP2D_OK;
 }
-template <int MODE>
-static int cols(cup2d_ctx* c, int LC, double2* spec, int ncg, const ColParams& P,
-                const double2* tw) {
+template <int MODE, typename S>
+static int cols(cup2d_ctx* c, int LC, cplx<S>* spec, int ncg, const ColParams& P,
+                const cplx<S>* tw) {
   switch (LC) {
-    case 32: launch_cols<32, MODE>(c, spec, ncg, P, tw); break;
-    case 64: launch_cols<64, MODE>(c, spec, ncg, P, tw); break;
-    case 128: launch_cols<128, MODE>(c, spec, ncg, P, tw); break;
-    case 256: launch_cols<256, MODE>(c, spec, ncg, P, tw); break;
-    case 512: launch_cols<512, MODE>(c, spec, ncg, P, tw); break;
-    case 1024: launch_cols<1024, MODE>(c, spec, ncg, P, tw); break;
-    case 2048: launch_cols<2048, MODE>(c, spec, ncg, P, tw); break;
-    case 4096: launch_cols<4096, MODE>(c, spec, ncg, P, tw); break;
+    case 32: launch_cols<32, MODE, S>(c, spec, ncg, P, tw); break;
+    case 64: launch_cols<64, MODE, S>(c, spec, ncg, P, tw); break;
+    case 128: launch_cols<128, MODE, S>(c, spec, ncg, P, tw); break;
+    case 256: launch_cols<256, MODE, S>(c, spec, ncg, P, tw); break;
+    case 512: launch_cols<512, MODE, S>(c, spec, ncg, P, tw); break;
+    case 1024: launch_cols<1024, MODE, S>(c, spec, ncg, P, tw); break;
+    case 2048: launch_cols<2048, MODE, S>(c, spec, ncg, P, tw); break;
+    case 4096: launch_cols<4096, MODE, S>(c, spec, ncg, P, tw); break;
     default: return CUP2D_ERR_ARG;
   }
   return CUP2D_OK;
 }
 
+template <int L, typename S>
+static void set_rows_attr_LS() {
+  constexpr int NP = rows_np<L, FE>();
+  const int shm = NP * lds_len<L, FE>() * (int)sizeof(cplx<S>);
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_PLAIN, S>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_FADE, S>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_DIV, S>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, FE, S, OUT_SET>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  if constexpr (sizeof(S) == 4)
+    (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, FE, S, OUT_ADD>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  else
+    (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, FE, S, OUT_RES>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+}
 template <int L>
 static void set_rows_attr_L() {
-  constexpr int NP = rows_np<L, FE>();
-  const int shm = NP * lds_len<L, FE>() * (int)sizeof(double2);
-  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_PLAIN>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_FADE>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_DIV>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, FE>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  set_rows_attr_LS<L, double>();
+  set_rows_attr_LS<L, float>();
 }
 template <int LC>
 static void set_cols_attr() {
   const int shm = lds_len<LC, FE>() * CW * (int)sizeof(double2);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 2, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 3, FE>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  const int shm32 = lds_len<LC, FE>() * CW * (int)sizeof(float2);
+  const hipFuncAttribute A = hipFuncAttributeMaxDynamicSharedMemorySize;
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0, FE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1, FE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 2, FE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 3, FE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 4, FE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 5, FE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0, FE, float>, A, shm32);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1, FE, float>, A, shm32);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 3, FE, float>, A, shm32);
 }
 
 static std::vector<double2> twiddles(int L) {
@@ -785,8 +1000,22 @@ static std::vector<double2> twiddles(int L) {
   }
   return w;
 }
+static std::vector<float2> twiddles32(int L) {
+  std::vector<float2> w(L);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < L; ++k) {
+    const long double a = -2.0L * pi * (long double)k / (long double)L;
+    w[k] = make_float2((float)cosl(a), (float)sinl(a));
+  }
+  return w;
+}
 
 static bool pow2(int n) { return n >= 32 && (n & (n - 1)) == 0; }
+static int next_pow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
 
 int fft_create(cup2d_ctx* c) {
   const int NX = c->g.NX, NY = c->g.NY;
@@ -796,14 +1025,20 @@ int fft_create(cup2d_ctx* c) {
   c->fft = P;
   P->fs = fs;
   P->cs = cs;
-  P->Lr = fs || cs ? 2 * NX : NX;
-  P->Lc = fs || cs ? 2 * NY : NY;
+  // Free-space: any cyclic length >= 2N - 1 gives the linear convolution on
+  // the N data points (FFTW_freespace.h:25-27 uses 2N - 1), so the rows and
+  // columns take the next power of two >= 2N and any grid of 32-cell blocks
+  // is solved.  Periodic (N) and cosine (mirrored 2N) lengths must be powers
+  // of two themselves.
+  P->Lr = fs ? next_pow2(2 * NX) : cs ? 2 * NX : NX;
+  P->Lc = fs ? next_pow2(2 * NY) : cs ? 2 * NY : NY;
   P->NYs = NY;
   P->ncg = (P->Lr / 2) / G;
-  if (!pow2(NX) || !pow2(NY) || P->Lr > 8192 || P->Lc > 4096) {
-    c->err = "FFT Poisson: NX and NY must be powers of two, row length <= 8192 and column "
-             "length <= 4096 (periodic NX <= 8192, NY <= 4096; free-space and cosine "
-             "NX <= 4096, NY <= 2048)";
+  if ((!fs && (!pow2(NX) || !pow2(NY))) || P->Lr > 8192 || P->Lc > 4096) {
+    c->err = "FFT Poisson: periodic and cosine NX and NY must be powers of two; row length "
+             "<= 8192 and column length <= 4096 (periodic NX <= 8192, NY <= 4096; cosine "
+             "NX <= 4096, NY <= 2048; free-space next_pow2(2 NX) <= 8192, next_pow2(2 NY) "
+             "<= 4096)";
     return CUP2D_ERR_ARG;
   }
   set_rows_attr_L<32>(); set_rows_attr_L<64>(); set_rows_attr_L<128>(); set_rows_attr_L<256>();
@@ -812,12 +1047,25 @@ int fft_create(cup2d_ctx* c) {
   set_cols_attr<32>(); set_cols_attr<64>(); set_cols_attr<128>(); set_cols_attr<256>();
   set_cols_attr<512>(); set_cols_attr<1024>(); set_cols_attr<2048>(); set_cols_attr<4096>();
 
-  CUP2D_HIP_CHECK(c, hipMalloc(&P->spec, sizeof(double2) * (size_t)P->ncg * G * P->NYs));
+  const size_t nspec = (size_t)P->ncg * G * P->NYs;
+  CUP2D_HIP_CHECK(c, hipMalloc(&P->spec, sizeof(double2) * nspec));
   const std::vector<double2> twr = twiddles(P->Lr), twc = twiddles(P->Lc);
   CUP2D_HIP_CHECK(c, hipMalloc(&P->tw_r, sizeof(double2) * twr.size()));
   CUP2D_HIP_CHECK(c, hipMalloc(&P->tw_c, sizeof(double2) * twc.size()));
   CUP2D_HIP_CHECK(c, hipMemcpy(P->tw_r, twr.data(), sizeof(double2) * twr.size(), hipMemcpyHostToDevice));
   CUP2D_HIP_CHECK(c, hipMemcpy(P->tw_c, twc.data(), sizeof(double2) * twc.size(), hipMemcpyHostToDevice));
+  if (c->f32) {
+    // the FP32 solve's spectrum, tables rounded to float; P->spec (FP64)
+    // serves the refinement's operator apply
+    CUP2D_HIP_CHECK(c, hipMalloc(&P->spec32, sizeof(float2) * nspec));
+    const std::vector<float2> twr32 = twiddles32(P->Lr), twc32 = twiddles32(P->Lc);
+    CUP2D_HIP_CHECK(c, hipMalloc(&P->tw_r32, sizeof(float2) * twr32.size()));
+    CUP2D_HIP_CHECK(c, hipMalloc(&P->tw_c32, sizeof(float2) * twc32.size()));
+    CUP2D_HIP_CHECK(c, hipMemcpy(P->tw_r32, twr32.data(), sizeof(float2) * twr32.size(),
+                                 hipMemcpyHostToDevice));
+    CUP2D_HIP_CHECK(c, hipMemcpy(P->tw_c32, twc32.data(), sizeof(float2) * twc32.size(),
+                                 hipMemcpyHostToDevice));
+  }
   if (cs) {  // FFTW_dirichlet.h:65-68: COScoef[i] = (float) cos(pi/M * 2 i)
     std::vector<double> cx(NX), cy(NY);
     for (int i = 0; i < NX; ++i) cx[i] = (double)(float)std::cos(M_PI / NX * 2.0 * i);
@@ -829,16 +1077,17 @@ int fft_create(cup2d_ctx* c) {
   }
   if (!fs) return CUP2D_OK;
 
-  // Free-space Green's function on the 2NX x 2NY cyclic grid
-  // (FFTW_freespace.cpp:100-137; with 2N padding the first NX x NY outputs of
-  // the cyclic convolution equal the reference's (2N-1) result).
-  const int MX = 2 * NX, MY = 2 * NY;
+  // Free-space Green's function on the Lr x Lc cyclic grid
+  // (FFTW_freespace.cpp:100-137), even in both directions; with any cyclic
+  // size >= 2N - 1 the first NX x NY outputs of the cyclic convolution equal
+  // the reference's (2N-1) result.
+  const int MX = P->Lr, MY = P->Lc, GH = MY / 2;
   std::vector<double> Gk((size_t)MX * MY);
   const double h = c->g.h, fac = 1 / (2 * M_PI);
   for (int j = 0; j < MY; ++j)
     for (int i = 0; i < MX; ++i) {
-      const double yi = j >= NY ? MY - j : j;
-      const double xi = i >= NX ? MX - i : i;
+      const double yi = j > MY - j ? MY - j : j;
+      const double xi = i > MX - i ? MX - i : i;
       const double r = std::sqrt(xi * xi + yi * yi);
       Gk[(size_t)j * MX + i] = fac * std::log(h * r);
     }
@@ -848,16 +1097,17 @@ int fft_create(cup2d_ctx* c) {
   CUP2D_HIP_CHECK(c, hipMalloc(&dG, sizeof(double) * Gk.size()));
   CUP2D_HIP_CHECK(c, hipMalloc(&tspec, sizeof(double2) * (size_t)P->ncg * G * MY));
   CUP2D_HIP_CHECK(c, hipMemcpy(dG, Gk.data(), sizeof(double) * Gk.size(), hipMemcpyHostToDevice));
-  CUP2D_HIP_CHECK(c, hipMalloc(&P->ghat, sizeof(double) * (size_t)P->ncg * G * (NY + 1)));
-  CUP2D_HIP_CHECK(c, hipMalloc(&P->ghatN, sizeof(double) * (size_t)(NY + 1)));
-  int rc = rows_fwd(c, P->Lr, dG, MX, MX, MY, tspec, MY, P->tw_r, SRC_PLAIN, RowSource{});
+  CUP2D_HIP_CHECK(c, hipMalloc(&P->ghat, sizeof(double) * (size_t)P->ncg * G * (GH + 1)));
+  CUP2D_HIP_CHECK(c, hipMalloc(&P->ghatN, sizeof(double) * (size_t)(GH + 1)));
+  int rc = rows_fwd<double>(c, P->Lr, dG, MX, MX, MY, tspec, MY, P->tw_r, SRC_PLAIN,
+                            RowSource{});
   if (rc) return rc;
   ColParams cp{};
   cp.NX = NX; cp.NY = NY; cp.rows = MY;
-  rc = cols<2>(c, P->Lc, tspec, P->ncg, cp, P->tw_c);
+  rc = cols<2, double>(c, P->Lc, tspec, P->ncg, cp, P->tw_c);
   if (rc) return rc;
   const double norm = 1.0 / ((double)MY * MX);
-  hipLaunchKernelGGL(k_ghat_extract, dim3(256), dim3(256), 0, c->stream, tspec, MY, P->ncg, NY,
+  hipLaunchKernelGGL(k_ghat_extract, dim3(256), dim3(256), 0, c->stream, tspec, MY, P->ncg, GH,
                      norm, P->ghat, P->ghatN);
   rc = launch_check(c, "green kernel");
   if (rc) return rc;
@@ -873,6 +1123,9 @@ void fft_destroy(cup2d_ctx* c) {
   (void)hipFree(P->spec);
   (void)hipFree(P->tw_r);
   (void)hipFree(P->tw_c);
+  if (P->spec32) (void)hipFree(P->spec32);
+  if (P->tw_r32) (void)hipFree(P->tw_r32);
+  if (P->tw_c32) (void)hipFree(P->tw_c32);
   if (P->ghat) (void)hipFree(P->ghat);
   if (P->ghatN) (void)hipFree(P->ghatN);
   if (P->cosX) (void)hipFree(P->cosX);
@@ -881,52 +1134,131 @@ void fft_destroy(cup2d_ctx* c) {
   c->fft = nullptr;
 }
 
+// The column pass's parameters and mode (solve or apply) for this plan.
+static ColParams col_params(const cup2d_ctx* c, bool apply, int* mode) {
+  const FFTPlan* P = c->fft;
+  const int NX = c->g.NX, NY = c->g.NY;
+  ColParams cp{};
+  cp.NX = NX; cp.NY = NY; cp.rows = NY;
+  if (P->cs) {  // FFTW_dirichlet.h:47-48
+    cp.waveFactX = M_PI / NX;
+    cp.waveFactY = M_PI / NY;
+    cp.norm = 0.25 / ((double)NX * NY);
+    cp.cosX = P->cosX;
+    cp.cosY = P->cosY;
+    *mode = apply ? 5 : 3;
+  } else if (!P->fs) {
+    cp.waveFactX = 2 * M_PI / NX;
+    cp.waveFactY = 2 * M_PI / NY;
+    cp.norm = 1.0 / ((double)NX * NY);
+    *mode = apply ? 4 : 0;
+  } else {
+    cp.ghat = P->ghat;
+    cp.ghatN = P->ghatN;
+    cp.GH = P->Lc / 2;
+    *mode = 1;
+  }
+  return cp;
+}
+
+template <typename S>
+static int cols_solve(cup2d_ctx* c, cplx<S>* spec, const cplx<S>* tw) {
+  int mode;
+  const ColParams cp = col_params(c, false, &mode);
+  const FFTPlan* P = c->fft;
+  if (mode == 3) return cols<3, S>(c, P->Lc, spec, P->ncg, cp, tw);
+  if (mode == 0) return cols<0, S>(c, P->Lc, spec, P->ncg, cp, tw);
+  return cols<1, S>(c, P->Lc, spec, P->ncg, cp, tw);
+}
+
 // src: SRC_PLAIN (rhs as given), SRC_FADE (rhs is the grid RHS: apply
 // cub2rhs's boundary fade after enqueue_fade_sums and write the faded cells
 // back), SRC_DIV (form the RHS from the advDiff output first, see k_rows_fwd).
+// At FP32 precision rhs is the float RHS plane, the solve runs in float and
+// its result goes to the double pressure, then refine_sweeps sweeps of
+//   r = b - A p  (A: the FP64 spectral operator; rows, apply column pass,
+//                 inverse rows writing the float residual),
+//   p += S32(r)  (the FP32 solve of the residual added in FP64),
+// with b the RHS the FP32 solve transformed (stored whole by its row pass).
 int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   FFTPlan* P = c->fft;
   const int NX = c->g.NX, NY = c->g.NY;
   int rc;
+  if (!c->f32) {
+    {
+      KernelTimer kt(c, T_FFT_ROWS_FWD);
+      RowSource fd{c->g, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
+                   c->u, c->v, c->obmap, P->cs, 0};
+      rc = rows_fwd<double>(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r, src, fd);
+    }
+    if (rc) return rc;
+    {
+      KernelTimer kt(c, T_FFT_COLS);
+      rc = cols_solve<double>(c, P->spec, P->tw_c);
+    }
+    if (rc) return rc;
+    {
+      // a ComputeForces still reading the previous pressure on the side stream
+      // must finish before it is overwritten
+      rc = join_forces(c);
+      if (rc) return rc;
+      KernelTimer kt(c, T_FFT_ROWS_INV);
+      rc = rows_inv<double, OUT_SET>(c, P->Lr, P->spec, P->NYs, sol, nullptr, NX, NX, NY,
+                                     P->tw_r);
+    }
+    if (rc) return rc;
+    return launch_check(c, "poisson_fft");
+  }
+  // FP32 solve
+  float* b = tp<float>(rhs);
+  float2* s32 = reinterpret_cast<float2*>(P->spec32);
+  const float2* twr32 = reinterpret_cast<const float2*>(P->tw_r32);
+  const float2* twc32 = reinterpret_cast<const float2*>(P->tw_c32);
   {
     KernelTimer kt(c, T_FFT_ROWS_FWD);
     RowSource fd{c->g, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
-                 c->u, c->v, c->obmap, P->cs};
-    rc = rows_fwd(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r, src, fd);
+                 c->u, c->v, c->obmap, P->cs, src != SRC_PLAIN ? 1 : 0};
+    rc = rows_fwd<float>(c, P->Lr, b, NX, NX, NY, s32, P->NYs, twr32, src, fd);
   }
   if (rc) return rc;
-  ColParams cp{};
-  cp.NX = NX; cp.NY = NY; cp.rows = NY;
   {
     KernelTimer kt(c, T_FFT_COLS);
-    if (P->cs) {  // FFTW_dirichlet.h:47-48
-      cp.waveFactX = M_PI / NX;
-      cp.waveFactY = M_PI / NY;
-      cp.norm = 0.25 / ((double)NX * NY);
-      cp.cosX = P->cosX;
-      cp.cosY = P->cosY;
-      rc = cols<3>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
-    } else if (!P->fs) {
-      cp.waveFactX = 2 * M_PI / NX;
-      cp.waveFactY = 2 * M_PI / NY;
-      cp.norm = 1.0 / ((double)NX * NY);
-      rc = cols<0>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
-    } else {
-      cp.ghat = P->ghat;
-      cp.ghatN = P->ghatN;
-      rc = cols<1>(c, P->Lc, P->spec, P->ncg, cp, P->tw_c);
-    }
+    rc = cols_solve<float>(c, s32, twc32);
   }
+  if (rc) return rc;
+  rc = join_forces(c);
   if (rc) return rc;
   {
-    // a ComputeForces still reading the previous pressure on the side stream
-    // must finish before it is overwritten
-    rc = join_forces(c);
-    if (rc) return rc;
     KernelTimer kt(c, T_FFT_ROWS_INV);
-    rc = rows_inv(c, P->Lr, P->spec, P->NYs, sol, NX, NX, NY, P->tw_r);
+    rc = rows_inv<float, OUT_SET>(c, P->Lr, s32, P->NYs, sol, nullptr, NX, NX, NY, twr32);
   }
   if (rc) return rc;
+  const int sweeps = P->fs ? 0 : c->refine_sweeps;
+  static_assert(sizeof(StepState::res) / sizeof(double) >= 4, "one residual per sweep");
+  float* res = tp<float>(c->rres);
+  for (int k = 0; k < sweeps; ++k) {
+    TimerScope ts(c, T_REFINE);
+    const RowSource plain{c->g, 0, nullptr, c->st, nullptr, nullptr, nullptr, P->cs, 0};
+    rc = rows_fwd<double>(c, P->Lr, sol, NX, NX, NY, P->spec, P->NYs, P->tw_r, SRC_PLAIN, plain);
+    if (rc) return rc;
+    int mode;
+    ColParams ca = col_params(c, true, &mode);
+    ca.dc_out = &c->st->pmean;
+    rc = mode == 5 ? cols<5, double>(c, P->Lc, P->spec, P->ncg, ca, P->tw_c)
+                   : cols<4, double>(c, P->Lc, P->spec, P->ncg, ca, P->tw_c);
+    if (rc) return rc;
+    rc = rows_inv<double, OUT_RES>(c, P->Lr, P->spec, P->NYs, res, b, NX, NX, NY, P->tw_r);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_res_reduce, dim3(1), dim3(256), 0, c->stream, c->respart,
+                       c->rows_inv_wgs, &c->st->res[k]);
+    rc = rows_fwd<float>(c, P->Lr, res, NX, NX, NY, s32, P->NYs, twr32, SRC_PLAIN, plain);
+    if (rc) return rc;
+    rc = cols_solve<float>(c, s32, twc32);
+    if (rc) return rc;
+    rc = rows_inv<float, OUT_ADD>(c, P->Lr, s32, P->NYs, sol, nullptr, NX, NX, NY, twr32,
+                                  &c->st->pmean);
+    if (rc) return rc;
+  }
   return launch_check(c, "poisson_fft");
 }
 

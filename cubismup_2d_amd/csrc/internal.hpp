@@ -48,6 +48,8 @@ struct StepState {
   double flux_corr;  // advDiff boundary-flux correction of the last step
   double dt_done;    // dt of the step whose forces are pending (set by the body update)
   double tnext[2];   // time + dt of the step with that parity, set by its list builder
+  double pmean;      // FP32 refinement: mean of the pressure before the sweep's update
+  double res[4];     // FP32 refinement: |b - A p| / |b| at the start of each sweep
 };
 
 // Body state captured by the body update for ComputeForces, which may run on
@@ -64,6 +66,19 @@ struct ForceSnap {
 // Pthrust Pdrag PoutBnd Pout defPowerBnd defPower EffPDefBnd EffPDef
 // n_surface_points.
 constexpr int FLOG_REC = CUP2D_FORCE_LOG_REC;
+
+// Imposed surface velocity of one shape (cup2d_set_surface_velocity): a
+// Fourier series in the angle phi about the shape's centre of mass,
+//   u_n(phi) = sum_m a_m cos(m phi) + b_m sin(m phi)   (along the outward normal)
+//   u_t(phi) = sum_m c_m cos(m phi) + d_m sin(m phi)   (along the tangent),
+// m = 1..M, added to the shape's deformation velocity where that enters the
+// step (integrateMomenta, penalize, ComputeForces' deformation power).
+constexpr int SURF_MAX_MODES = 32;
+struct SurfVel {
+  int shape;           // -1: none
+  int M;
+  const double* coef;  // device [4M]: a_1..a_M b_1..b_M c_1..c_M d_1..d_M
+};
 
 // Constant run parameters passed by value to kernels.
 struct Grid {
@@ -119,7 +134,7 @@ struct FFTPlan;  // fft.hip
 // the kernel is bracketed by two events on the context's stream.
 enum TimerId {
   T_MAXU = 0, T_DT, T_ADVDIFF, T_FLUXFIX, T_RHS, T_RHS_OBST, T_FADE, T_FFT_ROWS_FWD,
-  T_FFT_COLS, T_FFT_ROWS_INV, T_PROJECT, T_PUT_OBJECTS, T_BODY, T_PENALIZE, T_FORCES,
+  T_FFT_COLS, T_FFT_ROWS_INV, T_PROJECT, T_PUT_OBJECTS, T_BODY, T_PENALIZE, T_FORCES, T_REFINE,
   T_NTIMERS
 };
 struct Timer {
@@ -171,6 +186,33 @@ struct cup2d_ctx {
   int device = 0;
   std::string err;
 
+  // Precision (cup2d_cfg.precision): with CUP2D_PRECISION_FP32 the velocity
+  // planes (u, v, tu, tv) and the Poisson RHS plane (rhs) hold float, the
+  // stencils and the Poisson FFT run in float, and the pressure stays double:
+  // refine_sweeps FP64 defect-correction sweeps (residual of the FP64
+  // spectral operator, FP32 solve of it, FP64 update) follow the FP32 solve.
+  // The planes keep their double-sized allocations; at FP32 they are read
+  // through tp<float>.
+  bool f32 = false;
+  int refine_sweeps = 0;
+  // Captured whole steps (cup2d_simulate_graph): two steps per graph, from
+  // and back to the quiescent state (nothing pending on the side streams);
+  // one cached graph per step parity, valid while the velocity and list
+  // buffers it was captured with are the current ones.
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    const double* u0 = nullptr;
+    const int* ob0 = nullptr;
+  };
+  StepGraph graphs[2];
+  double* surf_coef = nullptr;  // device [4 SURF_MAX_MODES]
+  int surf_shape = -1, surf_modes = 0;
+  bool graph_capture = false;  // enqueueing into a capture: device-side step index
+  hipEvent_t ev_fork = nullptr, ev_qaux = nullptr, ev_qmid = nullptr;
+  double* rres = nullptr;  // FP32: the refinement residual plane (float)
+  double* respart = nullptr;  // FP32: residual-norm partials of the inverse row pass
+  int rows_inv_wgs = 0;       // workgroups of the last inverse row pass
+
   // fields (dense planes)
   double* u = nullptr;   // velocity x
   double* v = nullptr;   // velocity y
@@ -220,6 +262,30 @@ struct cup2d_ctx {
   // events of the next timed launch (KernelTimer), recorded by its dispatch
   hipEvent_t lt_start = nullptr, lt_stop = nullptr;
 };
+
+// A velocity / RHS plane as its element type (see cup2d_ctx::f32).
+namespace cup2d {
+template <typename T>
+inline T* tp(double* p) { return reinterpret_cast<T*>(p); }
+template <typename T>
+inline const T* tp(const double* p) { return reinterpret_cast<const T*>(p); }
+}  // namespace cup2d
+namespace cup2d {
+inline SurfVel surf_of(const cup2d_ctx* c) {
+  return SurfVel{c->surf_modes > 0 ? c->surf_shape : -1, c->surf_modes, c->surf_coef};
+}
+}  // namespace cup2d
+// Runs the statement with T = float at FP32 precision, T = double otherwise.
+#define CUP2D_REAL_DISPATCH(ctx, ...) \
+  do {                                \
+    if ((ctx)->f32) {                 \
+      using T = float;                \
+      __VA_ARGS__;                    \
+    } else {                          \
+      using T = double;               \
+      __VA_ARGS__;                    \
+    }                                 \
+  } while (0)
 
 #define CUP2D_HIP_CHECK(ctx, call)                                             \
   do {                                                                         \
@@ -281,8 +347,13 @@ int enqueue_set_dt(cup2d_ctx* c, double dt);
 int enqueue_ic(cup2d_ctx* c);
 int enqueue_apply_obj_vel(cup2d_ctx* c);
 int enqueue_step_end(cup2d_ctx* c);
-int enqueue_blocks_to_dense(cup2d_ctx* c, const double* stage, double* a, double* b, int nc);
-int enqueue_dense_to_blocks(cup2d_ctx* c, const double* a, const double* b, double* stage, int nc);
+// f32: the planes hold float (the velocity and RHS planes at FP32 precision)
+int enqueue_blocks_to_dense(cup2d_ctx* c, const double* stage, double* a, double* b, int nc,
+                            bool f32 = false);
+int enqueue_dense_to_blocks(cup2d_ctx* c, const double* a, const double* b, double* stage, int nc,
+                            bool f32 = false);
+int enqueue_stage_to_f32(cup2d_ctx* c, const double* stage, double* plane);
+int enqueue_f32_to_stage(cup2d_ctx* c, const double* plane, double* stage);
 
 // fft.hip
 int fft_create(cup2d_ctx* c);

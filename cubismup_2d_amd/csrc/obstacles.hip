@@ -53,7 +53,8 @@ __device__ inline SlotView slot(const Obstacles& ob, int s) {
 
 // value of a dense field at global cell (gx, gy) with BlockLabOpen ghosts
 // (zero-gradient at the four domain faces, Definitions.h:125-157)
-__device__ inline double lab_at(const double* f, const Grid& g, int gx, int gy) {
+template <typename RT>
+__device__ inline double lab_at(const RT* f, const Grid& g, int gx, int gy) {
   gx = gx < 0 ? 0 : (gx >= g.NX ? g.NX - 1 : gx);
   gy = gy < 0 ? 0 : (gy >= g.NY ? g.NY - 1 : gy);
   return f[(size_t)gy * g.NX + gx];
@@ -170,12 +171,16 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
     s_fix[t][1] = sp[t].bFixedy;
   }
   double sTime = 0, sDt = 0, sU0 = 0, sU1 = 0, sRamp = 0;
+  // the step's index (the CFL ramp, the end-time slot): the host's count, or
+  // with step_idx < 0 (launches replayed from a captured graph) the device's
+  int sIdx = step_idx;
   if (t == 0) {
     sTime = st->time;
     sDt = st->dt;
     sU0 = st->uinf[0];
     sU1 = st->uinf[1];
-    if (phase != 2 && compute_dt && step_idx < 100) sRamp = ramp[step_idx];
+    if (sIdx < 0) sIdx = st->step + end_prev;
+    if (phase != 2 && compute_dt && sIdx < 100) sRamp = ramp[sIdx];
     if (end_prev) {  // the previous whole step ends (Simulation.cpp:289-290)
       sTime += st->dt_done;
       st->time = sTime;
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
         const double dtBody = maxUb < 2.2e-16 ? 1 : h / maxUb;
         const double mn = dmin(dmin(dtCFL, dtFourier), dtBody);
         double dt = g.CFL * mn;
-        if (step_idx < 100) dt = sRamp * mn;
+        if (sIdx < 100) dt = sRamp * mn;
         st->dt = dt;
         st->maxU = maxU;
         if (!finite_max(maxU) && st->error == 0) st->error = CUP2D_ERR_NONFINITE;
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       st->uinf[1] = U1;
       s_step[1] = U0;
       s_step[2] = U1;
-      st->tnext[step_idx & 1] = sTime + s_step[0];  // this step's end time
+      st->tnext[sIdx & 1] = sTime + s_step[0];  // this step's end time
     }
     __syncthreads();
     if (t < ns) {  // each shape moves on its own lane
@@ -821,7 +826,8 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     TimerScope ts(c, T_PUT_OBJECTS);
     hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
                        c->ob, c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg,
-                       c->ramp, compute_dt, c->host_step, fish ? 1 : 0, end_prev);
+                       c->ramp, compute_dt, c->graph_capture ? -1 : c->host_step, fish ? 1 : 0,
+                       end_prev);
   }
   // overlap 2: advDiff needs only dt and U_inf from the first phase of the
   // list builder; it runs on the side stream beside the whole chain, which
@@ -863,7 +869,7 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     RC_OB(enqueue_fish_midline(c, s));
     hipLaunchKernelGGL(k_obst_begin, dim3(1), dim3(NT), 0, s, c->g, c->sp, c->ss, c->st, c->ob,
                        c->ob_prev, c->obmap, c->ubidx, c->fishd, c->maxpart, c->nwg, c->ramp, 0,
-                       c->host_step, 2, 0);
+                       c->graph_capture ? -1 : c->host_step, 2, 0);
     if (chain_main && mid_a_here) CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_lists, c->stream));
     if (!(s = stream_for(ST_FISH_PLACE))) return CUP2D_ERR_HIP;
     RC_OB(enqueue_fish_place(c, s));
@@ -951,11 +957,14 @@ int join_obstacles(cup2d_ctx* c) {
 // advDiff output (rhs_cell: PressureSingle.cpp:112-128 with the deferred
 // boundary-flux correction), the cells the whole-step row FFT takes from the
 // RHS plane.
+// RT: the type of the RHS and velocity planes (double, or float at FP32
+// precision); the corrections are computed in double and stored rounded.
+template <typename RT>
 __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __restrict__ st,
                                                   Obstacles ob, const double* __restrict__ chi,
-                                                  double* __restrict__ r,
-                                                  const double* __restrict__ u,
-                                                  const double* __restrict__ v,
+                                                  RT* __restrict__ r,
+                                                  const RT* __restrict__ u,
+                                                  const RT* __restrict__ v,
                                                   cup2d_shape_state* __restrict__ commit_ss) {
   __shared__ double lds[LDS_RED];
   if (commit_ss && blockIdx.x == gridDim.x - 1) {  // the placement commit (k_obst_commit)
@@ -966,11 +975,11 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
   const int b = ob.ub_block[blockIdx.x];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   if (u) {
-    const double corr = st->flux_corr;  // deferred advDiff flux correction
-    const double facDiv = 0.5 * g.h / st->dt;
+    const RT corr = (RT)st->flux_corr;  // deferred advDiff flux correction
+    const RT facDiv = (RT)(0.5 * g.h / st->dt);
     CELL_LOOP(i) {
       const int x = bx * BS + i % BS, y = by * BS + i / BS;
-      r[(size_t)y * g.NX + x] = rhs_cell(u, v, g.NX, g.NY, x, y, corr, facDiv);
+      r[(size_t)y * g.NX + x] = rhs_cell<RT>(u, v, g.NX, g.NY, x, y, corr, facDiv);
     }
   }
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
@@ -984,9 +993,10 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
       if (chi[d] > o.chi[i]) continue;
       const double gradXx = lab_at(chi, g, gx + 1, gy) - lab_at(chi, g, gx - 1, gy);
       const double gradXy = lab_at(chi, g, gx, gy + 1) - lab_at(chi, g, gx, gy - 1);
-      const double srcBulk = -o.chi[i] * r[d];
+      const double rd = r[d];
+      const double srcBulk = -o.chi[i] * rd;
       sum += srcBulk;
-      r[d] += srcBulk;
+      r[d] = (RT)(rd + srcBulk);
       const double isPerim = (gradXx * gradXx + gradXy * gradXy > 0) ? 1.0 : 0.0;
       const double T = r[d];
       pos += isPerim * T * (double)(T > 0);
@@ -1003,9 +1013,10 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
 // band_index != nullptr: the fade sums were formed from the advDiff output
 // before the obstacle RHS; the partial of an obstacle block in the fade band
 // is formed again here from its final RHS.
+template <typename RT>
 __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __restrict__ st,
                                                   Obstacles ob, const double* __restrict__ chi,
-                                                  double* __restrict__ r,
+                                                  RT* __restrict__ r,
                                                   const int* __restrict__ band_index,
                                                   double* __restrict__ fadepart) {
   static_assert(NT == 256, "fade_block_sums is a 256-thread reduction");
@@ -1030,15 +1041,15 @@ __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __res
       const double gradXy = lab_at(chi, g, gx, gy + 1) - lab_at(chi, g, gx, gy - 1);
       const bool isPerim = gradXx * gradXx + gradXy * gradXy > 0;
       const double T = r[d];
-      if (isPerim && T > 0 && corr > 0) r[d] = T - corr * T;
-      else if (isPerim && T < 0 && corr < 0) r[d] = T + corr * T;
+      if (isPerim && T > 0 && corr > 0) r[d] = (RT)(T - corr * T);
+      else if (isPerim && T < 0 && corr < 0) r[d] = (RT)(T + corr * T);
     }
   }
   const int j = band_index ? band_index[b] : -1;
   if (j < 0) return;
   __syncthreads();  // the block's RHS is final
   double a[3];
-  fade_block_sums(g, b, r, nullptr, nullptr, true, 0.0, 0.0, a, lds);
+  fade_block_sums<RT>(g, b, r, nullptr, nullptr, true, (RT)0, (RT)0, a, lds);
   if (threadIdx.x == 0) {
     fadepart[3 * j] = a[0];
     fadepart[3 * j + 1] = a[1];
@@ -1052,12 +1063,14 @@ int enqueue_rhs_obstacles(cup2d_ctx* c, int compute) {
   TimerScope ts(c, T_RHS_OBST);
   // a pending placement commit takes one more workgroup
   const int G1 = c->commit_pending ? G + 1 : G;
-  hipLaunchKernelGGL(k_rhs_obst1, dim3(G1), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
-                     c->rhs, compute ? c->u : nullptr, c->v,
-                     c->commit_pending ? c->ss : nullptr);
+  CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_rhs_obst1<T>, dim3(G1), dim3(NT), 0, c->stream,
+                                            c->g, c->st, c->ob, c->chi, tp<T>(c->rhs),
+                                            compute ? tp<T>(c->u) : nullptr, tp<T>(c->v),
+                                            c->commit_pending ? c->ss : nullptr));
   c->commit_pending = false;
-  hipLaunchKernelGGL(k_rhs_obst2, dim3(G), dim3(NT), 0, c->stream, c->g, c->st, c->ob, c->chi,
-                     c->rhs, c->fade_early ? c->band_index : nullptr, c->red);
+  CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_rhs_obst2<T>, dim3(G), dim3(NT), 0, c->stream,
+                                            c->g, c->st, c->ob, c->chi, tp<T>(c->rhs),
+                                            c->fade_early ? c->band_index : nullptr, c->red));
   return launch_check(c, "rhs_obstacles");
 }
 
@@ -1126,7 +1139,7 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
                               cup2d_shape_state* __restrict__ ss,
                               StepState* __restrict__ st, const Obstacles& ob,
                               const int* __restrict__ ubidx, ForceSnap* __restrict__ snap,
-                              int collide) {
+                              int collide, const SurfVel& surf) {
   __shared__ double lds[LDS_RED];
   __shared__ double coll[MAX_SHAPES][10];
   const double dt = st->dt, time = st->time;
@@ -1180,12 +1193,23 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
               const double iUr = -si.omega * (p1 - si.centerOfMass[1]), iVr = si.omega * (p0 - si.centerOfMass[0]);
               const double jUr = -sj.omega * (p1 - sj.centerOfMass[1]), jVr = sj.omega * (p0 - sj.centerOfMass[0]);
               const double X = oi.chi[c], Y = oj.chi[c];
+              double iUd = oi.udx[c], iVd = oi.udy[c], jUd = oj.udx[c], jVd = oj.udy[c];
+              if (i == surf.shape || j == surf.shape) {
+                double sx, sy;
+                if (i == surf.shape) {
+                  surf_vel(surf, p0 - si.centerOfMass[0], p1 - si.centerOfMass[1], &sx, &sy);
+                  iUd += sx; iVd += sy;
+                } else {
+                  surf_vel(surf, p0 - sj.centerOfMass[0], p1 - sj.centerOfMass[1], &sx, &sy);
+                  jUd += sx; jVd += sy;
+                }
+              }
               acc[0] += X; acc[1] += X * p0; acc[2] += X * p1;
-              acc[3] += X * (si.u + iUr + oi.udx[c]);
-              acc[4] += X * (si.v + iVr + oi.udy[c]);
+              acc[3] += X * (si.u + iUr + iUd);
+              acc[4] += X * (si.v + iVr + iVd);
               acc[5] += Y; acc[6] += Y * p0; acc[7] += Y * p1;
-              acc[8] += Y * (sj.u + jUr + oj.udx[c]);
-              acc[9] += Y * (sj.v + jVr + oj.udy[c]);
+              acc[8] += Y * (sj.u + jUr + jUd);
+              acc[9] += Y * (sj.v + jVr + jVd);
             }
           }
         }
@@ -1231,10 +1255,11 @@ __device__ void body_solve_wg(const Grid& g, const ShapeParams* __restrict__ sp,
 }
 
 // PressureSingle::integrateMomenta (PressureSingle.cpp:23-67), per slot.
+template <typename RT>
 __device__ void momenta_wg(const Grid& g, const cup2d_shape_state* __restrict__ ss,
                            const StepState* __restrict__ st, const Obstacles& ob,
-                           const double* __restrict__ u, const double* __restrict__ v,
-                           double* lds) {
+                           const RT* __restrict__ u, const RT* __restrict__ v, double* lds,
+                           const SurfVel& sv) {
   const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   SlotView o = slot(ob, s);
@@ -1246,7 +1271,13 @@ __device__ void momenta_wg(const Grid& g, const cup2d_shape_state* __restrict__ 
     if (o.chi[i] <= 0) continue;
     const int ix = i % BS, iy = i / BS;
     const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-    const double ud0 = u[d] - o.udx[i], ud1 = v[d] - o.udy[i];
+    double ud0 = (double)u[d] - o.udx[i], ud1 = (double)v[d] - o.udy[i];
+    if (k == sv.shape) {
+      double sx, sy;
+      surf_vel(sv, cell_pos(bx, ix, g.hb, g.h) - Cx, cell_pos(by, iy, g.hb, g.h) - Cy, &sx, &sy);
+      ud0 -= sx;
+      ud1 -= sy;
+    }
     double F;
     if (!g.implicit_penal) {
       F = hsq * o.rho[i] * o.chi[i];
@@ -1273,8 +1304,9 @@ __device__ void momenta_wg(const Grid& g, const cup2d_shape_state* __restrict__ 
 __device__ void body_update_tail(const Grid& g, const ShapeParams* __restrict__ sp,
                                  cup2d_shape_state* __restrict__ ss, StepState* __restrict__ st,
                                  const Obstacles& ob, const int* __restrict__ ubidx,
-                                 ForceSnap* __restrict__ snap, int collide) {
-  body_solve_wg(g, sp, ss, st, ob, ubidx, snap, collide);
+                                 ForceSnap* __restrict__ snap, int collide,
+                                 const SurfVel& surf) {
+  body_solve_wg(g, sp, ss, st, ob, ubidx, snap, collide, surf);
   __syncthreads();
   // state ComputeForces uses (it may run after the next step moved the bodies)
   if (threadIdx.x < g.nshapes) {
@@ -1292,28 +1324,31 @@ __global__ __launch_bounds__(NT) void k_body_solve(Grid g, const ShapeParams* __
                                                    cup2d_shape_state* __restrict__ ss,
                                                    StepState* __restrict__ st, Obstacles ob,
                                                    const int* __restrict__ ubidx,
-                                                   ForceSnap* __restrict__ snap, int collide) {
-  body_update_tail(g, sp, ss, st, ob, ubidx, snap, collide);
+                                                   ForceSnap* __restrict__ snap, int collide,
+                                                   SurfVel surf) {
+  body_update_tail(g, sp, ss, st, ob, ubidx, snap, collide, surf);
 }
 
 // PressureSingle::integrateMomenta (PressureSingle.cpp:23-67) per slot.
+template <typename RT>
 __global__ __launch_bounds__(NT) void k_momenta(Grid g, const ShapeParams* __restrict__ sp,
                                                 cup2d_shape_state* __restrict__ ss,
                                                 StepState* __restrict__ st, Obstacles ob,
-                                                const double* __restrict__ u,
-                                                const double* __restrict__ v) {
+                                                const RT* __restrict__ u,
+                                                const RT* __restrict__ v, SurfVel sv) {
   __shared__ double lds[LDS_RED];
-  if ((int)blockIdx.x < ob.cnt[0]) momenta_wg(g, ss, st, ob, u, v, lds);
+  if ((int)blockIdx.x < ob.cnt[0]) momenta_wg(g, ss, st, ob, u, v, lds, sv);
 }
 
 int enqueue_body_update(cup2d_ctx* c) {
   if (c->g.nshapes == 0) return CUP2D_OK;
   const int G = grid_cap(c);
   TimerScope ts(c, T_BODY);
-  hipLaunchKernelGGL(k_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st, c->ob,
-                     c->u, c->v);
+  CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_momenta<T>, dim3(G), dim3(NT), 0, c->stream, c->g,
+                                            c->sp, c->ss, c->st, c->ob, tp<T>(c->u),
+                                            tp<T>(c->v), surf_of(c)));
   hipLaunchKernelGGL(k_body_solve, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                     c->ob, c->ubidx, c->fsnap, 1);
+                     c->ob, c->ubidx, c->fsnap, 1, surf_of(c));
   return launch_check(c, "body_update");
 }
 
@@ -1328,14 +1363,14 @@ int enqueue_body_update(cup2d_ctx* c) {
 // fewer than two shapes, where no collision step follows the solve).
 // Workgroup 0 stores the shape state, the ComputeForces snapshot and
 // dt_done; where a shape is forced it rewrites the value every workgroup read.
-template <bool SOLVE>
+template <bool SOLVE, typename RT>
 __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __restrict__ sp,
                                                  cup2d_shape_state* __restrict__ ss,
                                                  StepState* __restrict__ st, Obstacles ob,
                                                  const double* __restrict__ chi,
-                                                 double* __restrict__ u, double* __restrict__ v,
+                                                 RT* __restrict__ u, RT* __restrict__ v,
                                                  double* __restrict__ maxpart,
-                                                 ForceSnap* __restrict__ snap) {
+                                                 ForceSnap* __restrict__ snap, SurfVel sv) {
   __shared__ double lds[LDS_RED];
   __shared__ double s_uvw[MAX_SHAPES][3];
   const int nub = ob.cnt[1];
@@ -1398,16 +1433,22 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
       if (X <= 0) continue;
       const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
       const double alpha = g.implicit_penal ? 1 / (1 + g.lambda * dt * X) : 1 - X;
-      const double US = u_s - omega_s * p1 + o.udx[i];
-      const double VS = v_s + omega_s * p0 + o.udy[i];
+      double US = u_s - omega_s * p1 + o.udx[i];
+      double VS = v_s + omega_s * p0 + o.udy[i];
+      if (k == sv.shape) {
+        double sx, sy;
+        surf_vel(sv, p0, p1, &sx, &sy);
+        US += sx;
+        VS += sy;
+      }
       uR[i_it] = alpha * uR[i_it] + (1 - alpha) * US;
       vR[i_it] = alpha * vR[i_it] + (1 - alpha) * VS;
     }
   }
   CELL_LOOP(i) {
     const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
-    u[d] = uR[i_it];
-    v[d] = vR[i_it];
+    u[d] = (RT)uR[i_it];
+    v[d] = (RT)vR[i_it];
   }
   if (!maxpart) return;
   // findMaxU (Helpers.cpp:160-175) over this block's final velocities; the
@@ -1416,7 +1457,8 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
   double m = 0;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const double a = uR[c], w = vR[c];
+    // the stored values
+    const double a = (double)(RT)uR[c], w = (double)(RT)vR[c];
     m = nmax(m, fabs(a + UINF));
     m = nmax(m, fabs(w + VINF));
     m = nmax(m, fabs(a));
@@ -1429,8 +1471,10 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
 int enqueue_penalize(cup2d_ctx* c, double* maxpart) {
   if (c->g.nshapes == 0) return CUP2D_OK;
   TimerScope ts(c, T_PENALIZE);
-  hipLaunchKernelGGL(k_penalize<false>, dim3(grid_cap(c)), dim3(NT), 0, c->stream, c->g, c->sp,
-                     c->ss, c->st, c->ob, c->chi, c->u, c->v, maxpart, c->fsnap);
+  CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL((k_penalize<false, T>), dim3(grid_cap(c)), dim3(NT), 0,
+                                            c->stream, c->g, c->sp, c->ss, c->st, c->ob, c->chi,
+                                            tp<T>(c->u), tp<T>(c->v), maxpart, c->fsnap,
+                                            surf_of(c)));
   return launch_check(c, "penalize");
 }
 
@@ -1445,12 +1489,15 @@ int enqueue_body_penalize(cup2d_ctx* c, double* maxpart) {
   const int G = grid_cap(c);
   {
     TimerScope ts(c, T_BODY);
-    hipLaunchKernelGGL(k_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                       c->ob, c->u, c->v);
+    CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_momenta<T>, dim3(G), dim3(NT), 0, c->stream,
+                                              c->g, c->sp, c->ss, c->st, c->ob, tp<T>(c->u),
+                                              tp<T>(c->v), surf_of(c)));
   }
   TimerScope ts(c, T_PENALIZE);
-  hipLaunchKernelGGL(k_penalize<true>, dim3(G), dim3(NT), 0, c->stream, c->g, c->sp, c->ss,
-                     c->st, c->ob, c->chi, c->u, c->v, maxpart, c->fsnap);
+  CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL((k_penalize<true, T>), dim3(G), dim3(NT), 0,
+                                            c->stream, c->g, c->sp, c->ss, c->st, c->ob, c->chi,
+                                            tp<T>(c->u), tp<T>(c->v), maxpart, c->fsnap,
+                                            surf_of(c)));
   return launch_check(c, "body update + penalize");
 }
 
@@ -1495,7 +1542,7 @@ int enqueue_iter_body(cup2d_ctx* c) {
   hipLaunchKernelGGL(k_iter_momenta, dim3(G), dim3(NT), 0, c->stream, c->g, c->ss, c->ob, c->vfu,
                      c->vfv);
   hipLaunchKernelGGL(k_body_solve, dim3(1), dim3(NT), 0, c->stream, c->g, c->sp, c->ss, c->st,
-                     c->ob, c->ubidx, c->fsnap, 0);
+                     c->ob, c->ubidx, c->fsnap, 0, surf_of(c));
   return launch_check(c, "iterator body update");
 }
 
@@ -1630,9 +1677,10 @@ __device__ void forces_fin_wg(const Grid& g, cup2d_shape_state* __restrict__ ss,
 }
 
 // ComputeForces.cpp:16-107, per slot; surface points are the cells with D>0.
+template <typename RT>
 __device__ void forces_wg(const Grid& g, const ForceSnap* __restrict__ snap, const Obstacles& ob,
-                          const double* __restrict__ u, const double* __restrict__ v,
-                          const double* __restrict__ p, double* lds) {
+                          const RT* __restrict__ u, const RT* __restrict__ v,
+                          const double* __restrict__ p, double* lds, const SurfVel& surf) {
   const int s = blockIdx.x, k = ob.slot_shape[s], b = ob.slot_block[s];
   const int bx = b % g.bpdx, by = b / g.bpdx;
   SlotView o = slot(ob, s);
@@ -1659,7 +1707,14 @@ __device__ void forces_wg(const Grid& g, const ForceSnap* __restrict__ snap, con
     const double fXV = D11 * normX + D12 * normY, fXP = -Pc * normX;
     const double fYV = D12 * normX + D22 * normY, fYP = -Pc * normY;
     const double fXT = fXV + fXP, fYT = fYV + fYP;
-    const double vx = u[d], vy = v[d], vxDef = o.udx[i], vyDef = o.udy[i];
+    const double vx = (double)u[d], vy = (double)v[d];
+    double vxDef = o.udx[i], vyDef = o.udy[i];
+    if (k == surf.shape) {
+      double sx, sy;
+      surf_vel(surf, p0 - Cx, p1 - Cy, &sx, &sy);
+      vxDef += sx;
+      vyDef += sy;
+    }
     a[0] += sqrt(normX * normX + normY * normY);
     a[1] += normX * vy - normY * vx;
     a[2] += fXT; a[3] += fYT;
@@ -1693,12 +1748,13 @@ __global__ void k_step_end_done(StepState* st) {
 }
 
 // ComputeForces per slot (ComputeForces.cpp:16-107)
+template <typename RT>
 __global__ __launch_bounds__(NT) void k_forces(Grid g, const ForceSnap* __restrict__ snap,
-                                               Obstacles ob, const double* __restrict__ u,
-                                               const double* __restrict__ v,
-                                               const double* __restrict__ p) {
+                                               Obstacles ob, const RT* __restrict__ u,
+                                               const RT* __restrict__ v,
+                                               const double* __restrict__ p, SurfVel sv) {
   __shared__ double lds[LDS_RED];
-  if ((int)blockIdx.x < ob.cnt[0]) forces_wg(g, snap, ob, u, v, p, lds);
+  if ((int)blockIdx.x < ob.cnt[0]) forces_wg(g, snap, ob, u, v, p, lds, sv);
 }
 
 // shape totals (Shape::computeForces, Shape.cpp:252-300) and, with end_step,
@@ -1713,8 +1769,9 @@ __global__ __launch_bounds__(NT) void k_forces_fin(Grid g, cup2d_shape_state* __
 static int launch_forces(cup2d_ctx* c, hipStream_t s, const Obstacles& lists, int end_step,
                          const double* u, const double* v, int deferred) {
   TimerScope ts(c, T_FORCES, s);
-  hipLaunchKernelGGL(k_forces, dim3(grid_cap(c)), dim3(NT), 0, s, c->g, c->fsnap, lists, u, v,
-                     c->pres);
+  CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_forces<T>, dim3(grid_cap(c)), dim3(NT), 0, s, c->g,
+                                            c->fsnap, lists, tp<T>(u), tp<T>(v), c->pres,
+                                            surf_of(c)));
   const ForceLog flog{c->flog, c->flog_cnt, c->flog_cap};
   hipLaunchKernelGGL(k_forces_fin, dim3(1), dim3(NT), 0, s, c->g, c->ss, lists, c->st, end_step,
                      c->fsnap, flog, deferred);

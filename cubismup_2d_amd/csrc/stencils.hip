@@ -47,8 +47,9 @@ int grid_workgroups(int NX, int NY) { return (NX / strip_width(NX)) * (NY / RYS_
 // Helpers.cpp:136-176: max over cells of |u+Uinf|, |v+Vinf|, |u|, |v|.
 // Writes one partial per workgroup into part[0..gridDim.x) and clears the
 // nclear obstacle-block partials that follow.
-__global__ __launch_bounds__(256) void k_find_max_u(const double* __restrict__ u,
-                                                    const double* __restrict__ v, size_t n,
+template <typename T>
+__global__ __launch_bounds__(256) void k_find_max_u(const T* __restrict__ u,
+                                                    const T* __restrict__ v, size_t n,
                                                     const StepState* __restrict__ st,
                                                     double* __restrict__ part, int nclear) {
   __shared__ double lds[8];
@@ -95,8 +96,10 @@ __global__ __launch_bounds__(256) void k_dt(const double* __restrict__ part, int
 
 int enqueue_find_max_u_partials(cup2d_ctx* c) {
   TimerScope ts(c, T_MAXU);
-  hipLaunchKernelGGL(k_find_max_u, dim3(c->nwg), dim3(256), 0, c->stream, c->u, c->v,
-                     (size_t)c->g.NX * c->g.NY, c->st, c->maxpart, c->maxpart_n - c->nwg);
+  CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_find_max_u<T>, dim3(c->nwg), dim3(256), 0,
+                                            c->stream, tp<T>(c->u), tp<T>(c->v),
+                                            (size_t)c->g.NX * c->g.NY, c->st, c->maxpart,
+                                            c->maxpart_n - c->nwg));
   return launch_check(c, "find_max_u");
 }
 
@@ -142,9 +145,11 @@ __device__ inline void stencil_tile(int* bx, int* by) {
 // advDiff.cpp:14-34 (stencil), :47-55 (factors), :68-79 (faded lab).  Each
 // workgroup also writes its share of the boundary flux
 // IF = sum_E u - sum_W u + sum_N v - sum_S v (advDiff.cpp:93-102).
-template <int RY>
-__device__ inline void advdiff_strip(const double* __restrict__ u, const double* __restrict__ v,
-                                     double* __restrict__ uo, double* __restrict__ vo,
+// T: the velocity planes' type; the stencil is evaluated in T (FP32 at
+// FP32 precision), the factors are formed in double and rounded once.
+template <int RY, typename T>
+__device__ inline void advdiff_strip(const T* __restrict__ u, const T* __restrict__ v,
+                                     T* __restrict__ uo, T* __restrict__ vo,
                                      const Grid& g, const StepState* __restrict__ st,
                                      double* __restrict__ fluxpart, int bx, int by, int gx, int gy,
                                      double* lds) {
@@ -152,13 +157,14 @@ __device__ inline void advdiff_strip(const double* __restrict__ u, const double*
   const int x = bx * blockDim.x + threadIdx.x;
   const int y0 = by * RY;
   const double dt = st->dt, h = g.h;
-  const double UINF0 = st->uinf[0], UINF1 = st->uinf[1];
-  const double dfac = (g.nu / h) * (dt / h), afac = -0.5 * dt / h;
-  const double norUinf = dmax(dmax(fabs(UINF0), fabs(UINF1)), EPSD);
-  const double tW = dmax(UINF0, 0.0) / norUinf, tS = dmax(UINF1, 0.0) / norUinf;
-  const double tE = dmin(UINF0, 0.0) / norUinf, tN = dmin(UINF1, 0.0) / norUinf;
-  const double fadeW = 1 - tW * tW / BC_KILL_FAC, fadeS = 1 - tS * tS / BC_KILL_FAC;
-  const double fadeE = 1 - tE * tE / BC_KILL_FAC, fadeN = 1 - tN * tN / BC_KILL_FAC;
+  const double dUINF0 = st->uinf[0], dUINF1 = st->uinf[1];
+  const T UINF0 = (T)dUINF0, UINF1 = (T)dUINF1;
+  const T dfac = (T)((g.nu / h) * (dt / h)), afac = (T)(-0.5 * dt / h);
+  const double norUinf = dmax(dmax(fabs(dUINF0), fabs(dUINF1)), EPSD);
+  const double tW = dmax(dUINF0, 0.0) / norUinf, tS = dmax(dUINF1, 0.0) / norUinf;
+  const double tE = dmin(dUINF0, 0.0) / norUinf, tN = dmin(dUINF1, 0.0) / norUinf;
+  const T fadeW = (T)(1 - tW * tW / BC_KILL_FAC), fadeS = (T)(1 - tS * tS / BC_KILL_FAC);
+  const T fadeE = (T)(1 - tE * tE / BC_KILL_FAC), fadeN = (T)(1 - tN * tN / BC_KILL_FAC);
 
   // All loads of the strip are issued first (clamped indices, so every load
   // is unconditional), then the rows are computed with selects of the
@@ -166,7 +172,7 @@ __device__ inline void advdiff_strip(const double* __restrict__ u, const double*
   const size_t i0 = (size_t)y0 * NX + x;
   const bool hasW = x > 0, hasE = x + 1 < NX;
   const size_t dW = hasW ? 1 : 0, dE = hasE ? 1 : 0;
-  double uC[RY + 2], vC[RY + 2], uW[RY], vW[RY], uE[RY], vE[RY];
+  T uC[RY + 2], vC[RY + 2], uW[RY], vW[RY], uE[RY], vE[RY];
   {
     const size_t is = y0 > 0 ? i0 - NX : i0;
     uC[0] = u[is];
@@ -186,23 +192,23 @@ __device__ inline void advdiff_strip(const double* __restrict__ u, const double*
       vE[iy] = v[i + dE];
     }
   }
-  double uc = uC[1], vc = vC[1];
-  double us = y0 == 0 ? uc * fadeS : uC[0];
-  double vs = y0 == 0 ? vc * fadeS : vC[0];
+  T uc = uC[1], vc = vC[1];
+  T us = y0 == 0 ? uc * fadeS : uC[0];
+  T vs = y0 == 0 ? vc * fadeS : vC[0];
   double flux = 0;
 #pragma unroll
   for (int iy = 0; iy < RY; ++iy) {
     const int y = y0 + iy;
     const bool hasN = y + 1 < NY;
-    const double un = hasN ? uC[iy + 2] : uc * fadeN, vn = hasN ? vC[iy + 2] : vc * fadeN;
-    const double uw = hasW ? uW[iy] : uc * fadeW, vw = hasW ? vW[iy] : vc * fadeW;
-    const double ue = hasE ? uE[iy] : uc * fadeE, ve = hasE ? vE[iy] : vc * fadeE;
-    const double dUadv = (uc + UINF0) * (ue - uw) + (vc + UINF1) * (un - us);
-    const double dUdif = ue + un + uw + us - 4 * uc;
-    const double dVadv = (uc + UINF0) * (ve - vw) + (vc + UINF1) * (vn - vs);
-    const double dVdif = ve + vn + vw + vs - 4 * vc;
-    const double u1 = uc + (afac * dUadv + dfac * dUdif);
-    const double v1 = vc + (afac * dVadv + dfac * dVdif);
+    const T un = hasN ? uC[iy + 2] : uc * fadeN, vn = hasN ? vC[iy + 2] : vc * fadeN;
+    const T uw = hasW ? uW[iy] : uc * fadeW, vw = hasW ? vW[iy] : vc * fadeW;
+    const T ue = hasE ? uE[iy] : uc * fadeE, ve = hasE ? vE[iy] : vc * fadeE;
+    const T dUadv = (uc + UINF0) * (ue - uw) + (vc + UINF1) * (un - us);
+    const T dUdif = ue + un + uw + us - 4 * uc;
+    const T dVadv = (uc + UINF0) * (ve - vw) + (vc + UINF1) * (vn - vs);
+    const T dVdif = ve + vn + vw + vs - 4 * vc;
+    const T u1 = uc + (afac * dUadv + dfac * dUdif);
+    const T v1 = vc + (afac * dVadv + dfac * dVdif);
     const size_t idx = i0 + (size_t)iy * NX;
     uo[idx] = u1;
     vo[idx] = v1;
@@ -224,17 +230,17 @@ __device__ inline void advdiff_strip(const double* __restrict__ u, const double*
 // grid smaller than that each workgroup takes the strips id, id + gridDim.x,
 // ... (the grid is a multiple of 8, so a workgroup keeps its XCD), which
 // leaves workgroup slots free for the obstacle kernels of the side stream.
-template <int RY>
-__global__ __launch_bounds__(256) void k_advdiff(const double* __restrict__ u,
-                                                 const double* __restrict__ v,
-                                                 double* __restrict__ uo, double* __restrict__ vo,
+template <int RY, typename T>
+__global__ __launch_bounds__(256) void k_advdiff(const T* __restrict__ u,
+                                                 const T* __restrict__ v,
+                                                 T* __restrict__ uo, T* __restrict__ vo,
                                                  Grid g, const StepState* __restrict__ st,
                                                  double* __restrict__ fluxpart, int gx, int gy) {
   __shared__ double lds[8];
   const int n = gx * gy;
   for (int id = blockIdx.x; id < n; id += gridDim.x) {
     const int t = (n % 8 == 0) ? (id % 8) * (n / 8) + id / 8 : id;
-    advdiff_strip<RY>(u, v, uo, vo, g, st, fluxpart, t % gx, t / gx, gx, gy, lds);
+    advdiff_strip<RY, T>(u, v, uo, vo, g, st, fluxpart, t % gx, t / gx, gx, gy, lds);
   }
 }
 
@@ -247,11 +253,11 @@ __global__ __launch_bounds__(256) void k_flux_reduce(const double* __restrict__ 
 }
 
 // advDiff.cpp:107-115 applied to the stored velocity (operator-level API).
-__global__ __launch_bounds__(256) void k_flux_fix(double* __restrict__ u,
-                                                  double* __restrict__ v, Grid g,
+template <typename T>
+__global__ __launch_bounds__(256) void k_flux_fix(T* __restrict__ u, T* __restrict__ v, Grid g,
                                                   const StepState* __restrict__ st) {
   const int NX = g.NX, NY = g.NY;
-  const double corr = st->flux_corr;
+  const T corr = (T)st->flux_corr;
   const int n = 2 * NY + 2 * NX;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     if (i < NY) u[(size_t)i * NX] += corr;
@@ -273,8 +279,10 @@ int enqueue_advdiff_on(cup2d_ctx* c, int defer_fix, hipStream_t s) {
   {
     const int nwg = gx * gy;
     KernelTimer kt(c, T_ADVDIFF);
-    CUP2D_TIMED_LAUNCH(c, k_advdiff<RYS>, dim3(nwg), dim3(tx), 0, s, c->u, c->v, c->tu,
-                       c->tv, c->g, (const StepState*)c->st, c->fluxpart, gx, gy);
+    CUP2D_REAL_DISPATCH(c, CUP2D_TIMED_LAUNCH(c, (k_advdiff<RYS, T>), dim3(nwg), dim3(tx), 0, s,
+                                              tp<T>(c->u), tp<T>(c->v), tp<T>(c->tu),
+                                              tp<T>(c->tv), c->g, (const StepState*)c->st,
+                                              c->fluxpart, gx, gy));
   }
   std::swap(c->u, c->tu);  // advDiff.cpp:81-87 copy tmpV -> vel
   std::swap(c->v, c->tv);
@@ -288,8 +296,8 @@ int enqueue_advdiff_on(cup2d_ctx* c, int defer_fix, hipStream_t s) {
     hipLaunchKernelGGL(k_flux_reduce, dim3(1), dim3(256), 0, s, c->fluxpart, gx, gy, c->g,
                        c->st);
     const int n = 2 * c->g.NX + 2 * c->g.NY;
-    hipLaunchKernelGGL(k_flux_fix, dim3((n + 255) / 256), dim3(256), 0, s, c->u, c->v,
-                       c->g, c->st);
+    CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_flux_fix<T>, dim3((n + 255) / 256), dim3(256), 0,
+                                              s, tp<T>(c->u), tp<T>(c->v), c->g, c->st));
   }
   return launch_check(c, "advdiff");
 }
@@ -302,10 +310,9 @@ int enqueue_advdiff_on(cup2d_ctx* c, int defer_fix, hipStream_t s) {
 // The correction is computed from the advDiff edge partials by the edge
 // workgroups, the only ones with boundary cells; workgroup (0,0) stores it
 // for the projection.
-template <int RY, bool FIX>
-__global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
-                                             const double* __restrict__ v,
-                                             double* __restrict__ r, Grid g,
+template <int RY, bool FIX, typename T>
+__global__ __launch_bounds__(256) void k_rhs(const T* __restrict__ u, const T* __restrict__ v,
+                                             T* __restrict__ r, Grid g,
                                              StepState* __restrict__ st,
                                              const double* __restrict__ fluxpart) {
   const int NX = g.NX, NY = g.NY;
@@ -313,32 +320,33 @@ __global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
   stencil_tile(&bx, &by);
   const int x = bx * blockDim.x + threadIdx.x;
   const int y0 = by * RY;
-  const double facDiv = 0.5 * g.h / st->dt;
-  double corr = 0.0;
+  const T facDiv = (T)(0.5 * g.h / st->dt);
+  T corr = 0;
   if (FIX && edge_index(bx, by, gridDim.x, gridDim.y) >= 0) {
     __shared__ double lds[8];
-    corr = flux_corr(fluxpart, edge_count(gridDim.x, gridDim.y), g, lds);
-    if (bx == 0 && by == 0 && threadIdx.x == 0) st->flux_corr = corr;
+    const double dcorr = flux_corr(fluxpart, edge_count(gridDim.x, gridDim.y), g, lds);
+    corr = (T)dcorr;
+    if (bx == 0 && by == 0 && threadIdx.x == 0) st->flux_corr = dcorr;
   }
   size_t idx = (size_t)y0 * NX + x;
-  double vc = v[idx];
+  T vc = v[idx];
   if (FIX && y0 == 0) vc = vc + corr;
-  const double vs0 = (y0 == 0) ? vc : v[idx - NX];  // y0-1 is never a boundary row
-  double vs = vs0;
+  const T vs0 = (y0 == 0) ? vc : v[idx - NX];  // y0-1 is never a boundary row
+  T vs = vs0;
   CUP2D_UNROLL_ROWS
   for (int iy = 0; iy < RY; ++iy, idx += NX) {
     const int y = y0 + iy;
-    double vn;
+    T vn;
     if (y + 1 < NY) {
       vn = v[idx + NX];
       if (FIX && y + 1 == NY - 1) vn = vn - corr;
     } else {
       vn = vc;
     }
-    double uc = u[idx];
+    T uc = u[idx];
     if (FIX && x == 0) uc = uc + corr;
     if (FIX && x == NX - 1) uc = uc - corr;
-    double uw, ue;
+    T uw, ue;
     if (x > 0) {
       uw = u[idx - 1];
       if (FIX && x - 1 == 0) uw = uw + corr;
@@ -351,8 +359,8 @@ __global__ __launch_bounds__(256) void k_rhs(const double* __restrict__ u,
     } else {
       ue = uc;
     }
-    const double divVx = ue - uw;
-    const double divVy = vn - vs;
+    const T divVx = ue - uw;
+    const T divVy = vn - vs;
     r[idx] = facDiv * (divVx + divVy);
     vs = vc;
     vc = vn;
@@ -370,11 +378,13 @@ int enqueue_pressure_rhs(cup2d_ctx* c) {
   TimerScope ts(c, T_RHS);
   const dim3 grid(c->g.NX / tx, c->g.NY / RYS);
   if (c->flux_deferred)
-    hipLaunchKernelGGL((k_rhs<RYS, true>), grid, dim3(tx), 0, c->stream, c->u, c->v, c->rhs, c->g,
-                       c->st, c->fluxpart);
+    CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL((k_rhs<RYS, true, T>), grid, dim3(tx), 0,
+                                              c->stream, tp<T>(c->u), tp<T>(c->v), tp<T>(c->rhs),
+                                              c->g, c->st, c->fluxpart));
   else
-    hipLaunchKernelGGL((k_rhs<RYS, false>), grid, dim3(tx), 0, c->stream, c->u, c->v, c->rhs,
-                       c->g, c->st, c->fluxpart);
+    CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL((k_rhs<RYS, false, T>), grid, dim3(tx), 0,
+                                              c->stream, tp<T>(c->u), tp<T>(c->v), tp<T>(c->rhs),
+                                              c->g, c->st, c->fluxpart));
   return launch_check(c, "pressure_rhs");
 }
 
@@ -385,11 +395,12 @@ int enqueue_pressure_rhs(cup2d_ctx* c) {
 // band block writes three partials.  The correction factor and the rewrite of
 // the band cells happen in the forward row FFT, whose workgroups each add
 // these partials in the same order.
-__global__ __launch_bounds__(256) void k_fade_sums(const double* __restrict__ r,
+template <typename T>
+__global__ __launch_bounds__(256) void k_fade_sums(const T* __restrict__ r,
                                                    const int* __restrict__ blocks, int nblocks,
                                                    Grid g, double* __restrict__ part,
-                                                   const double* __restrict__ u,
-                                                   const double* __restrict__ v,
+                                                   const T* __restrict__ u,
+                                                   const T* __restrict__ v,
                                                    const unsigned char* __restrict__ obmap,
                                                    const StepState* __restrict__ st) {
   __shared__ double lds[12];
@@ -400,15 +411,15 @@ __global__ __launch_bounds__(256) void k_fade_sums(const double* __restrict__ r,
   // obstacle blocks' from the RHS plane.  obmap == nullptr: every block from
   // the advDiff output (the obstacle blocks' partials are then replaced once
   // their RHS is final, by the obstacle RHS kernel).
-  double corr = 0, facDiv = 0;
+  T corr = 0, facDiv = 0;
   if (u) {
-    corr = st->flux_corr;
-    facDiv = 0.5 * g.h / st->dt;
+    corr = (T)st->flux_corr;
+    facDiv = (T)(0.5 * g.h / st->dt);
   }
   const int b = blocks[j];
   const bool from_plane = !u || (obmap && obmap[b]);
   double a[3];
-  fade_block_sums(g, b, r, u, v, from_plane, corr, facDiv, a, lds);
+  fade_block_sums<T>(g, b, r, u, v, from_plane, corr, facDiv, a, lds);
   if (threadIdx.x == 0) {
     part[3 * j] = a[0];
     part[3 * j + 1] = a[1];
@@ -434,8 +445,10 @@ __global__ __launch_bounds__(256) void k_fade_reduce(const double* __restrict__ 
 int enqueue_fade_sums_early(cup2d_ctx* c, hipStream_t s) {
   c->fade_early = false;
   if (c->band_nblocks == 0) return CUP2D_OK;
-  hipLaunchKernelGGL(k_fade_sums, dim3(fade_partials(c)), dim3(256), 0, s, c->rhs,
-                     c->band_blocks, c->band_nblocks, c->g, c->red, c->u, c->v, nullptr, c->st);
+  CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_fade_sums<T>, dim3(fade_partials(c)), dim3(256), 0,
+                                            s, tp<T>(c->rhs), c->band_blocks, c->band_nblocks,
+                                            c->g, c->red, tp<T>(c->u), tp<T>(c->v), nullptr,
+                                            c->st));
   c->fade_early = true;
   return launch_check(c, "cub2rhs fade (early)");
 }
@@ -444,9 +457,11 @@ int enqueue_fade_sums(cup2d_ctx* c, int compute) {
   if (c->band_nblocks == 0) return CUP2D_OK;
   TimerScope ts(c, T_FADE);
   if (!c->fade_early || !compute)
-    hipLaunchKernelGGL(k_fade_sums, dim3(fade_partials(c)), dim3(256), 0, c->stream, c->rhs,
-                       c->band_blocks, c->band_nblocks, c->g, c->red, compute ? c->u : nullptr,
-                       c->v, c->obmap, c->st);
+    CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_fade_sums<T>, dim3(fade_partials(c)), dim3(256),
+                                              0, c->stream, tp<T>(c->rhs), c->band_blocks,
+                                              c->band_nblocks, c->g, c->red,
+                                              compute ? tp<T>(c->u) : nullptr, tp<T>(c->v),
+                                              c->obmap, c->st));
   c->fade_early = false;
   hipLaunchKernelGGL(k_fade_reduce, dim3(1), dim3(256), 0, c->stream, c->red, fade_partials(c),
                      c->st);
@@ -458,8 +473,11 @@ int enqueue_fade_sums(cup2d_ctx* c, int compute) {
 // Applies a deferred advDiff flux correction (see k_rhs) and writes the
 // findMaxU partial (Helpers.cpp:160-175) of the cells outside obstacle
 // blocks; penalization reports the obstacle blocks after it changes them.
-template <int RY>
-__global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double* __restrict__ v,
+// T: the velocity planes' type; the pressure is double.  At FP32 precision
+// the gradient of the (FP64-refined) pressure is formed in double and the
+// update is rounded to float once.
+template <int RY, typename T>
+__global__ __launch_bounds__(256) void k_project(T* __restrict__ u, T* __restrict__ v,
                                                  const double* __restrict__ p, Grid g,
                                                  const StepState* __restrict__ st, int fix,
                                                  const unsigned char* __restrict__ obmap,
@@ -479,7 +497,8 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
   const size_t i0 = (size_t)y0 * NX + x;
   const bool hasW = x > 0, hasE = x + 1 < NX;
   const size_t dW = hasW ? 1 : 0, dE = hasE ? 1 : 0;
-  double pC[RY + 2], pW[RY], pE[RY], uu[RY], vv[RY];
+  double pC[RY + 2], pW[RY], pE[RY];
+  T uu[RY], vv[RY];
   pC[0] = p[y0 > 0 ? i0 - NX : i0];
 #pragma unroll
   for (int iy = 0; iy <= RY; ++iy)
@@ -504,18 +523,19 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
     const double pw = hasW ? pW[iy] : pc;
     const double pe = hasE ? pE[iy] : pc;
     // selects, not branches, so the row loop stays one basic block
-    double ux = uu[iy], vy = vv[iy];
-    ux = fix && x == 0 ? ux + corr : (fix && x == NX - 1 ? ux - corr : ux);
-    vy = fix && y == 0 ? vy + corr : (fix && y == NY - 1 ? vy - corr : vy);
-    const double u1 = ux + pFac * (pe - pw);
-    const double v1 = vy + pFac * (pn - ps);
+    T ux = uu[iy], vy = vv[iy];
+    const T tcorr = (T)corr;
+    ux = fix && x == 0 ? ux + tcorr : (fix && x == NX - 1 ? ux - tcorr : ux);
+    vy = fix && y == 0 ? vy + tcorr : (fix && y == NY - 1 ? vy - tcorr : vy);
+    const T u1 = (T)((double)ux + pFac * (pe - pw));
+    const T v1 = (T)((double)vy + pFac * (pn - ps));
     const size_t idx = i0 + (size_t)iy * NX;
     u[idx] = u1;
     v[idx] = v1;
-    double mc = nmax(m, fabs(u1 + UINF));
-    mc = nmax(mc, fabs(v1 + VINF));
-    mc = nmax(mc, fabs(u1));
-    mc = nmax(mc, fabs(v1));
+    double mc = nmax(m, fabs((double)u1 + UINF));
+    mc = nmax(mc, fabs((double)v1 + VINF));
+    mc = nmax(mc, fabs((double)u1));
+    mc = nmax(mc, fabs((double)v1));
     m = count ? mc : m;
     ps = pc;
     pc = pn;
@@ -529,10 +549,13 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ u, double*
 int enqueue_pressure_correction(cup2d_ctx* c, int want_max) {
   const int tx = strip_width(c->g.NX);
   KernelTimer kt(c, T_PROJECT);
-  CUP2D_TIMED_LAUNCH(c, k_project<RYS>, dim3(c->g.NX / tx, c->g.NY / RYS), dim3(tx), 0, c->stream,
-                     c->u, c->v, (const double*)c->pres, c->g, (const StepState*)c->st,
-                     c->flux_deferred ? 1 : 0, (const unsigned char*)c->obmap,
-                     want_max ? c->maxpart : nullptr);
+  CUP2D_REAL_DISPATCH(c, CUP2D_TIMED_LAUNCH(c, (k_project<RYS, T>), dim3(c->g.NX / tx,
+                                                                         c->g.NY / RYS),
+                                            dim3(tx), 0, c->stream, tp<T>(c->u), tp<T>(c->v),
+                                            (const double*)c->pres, c->g,
+                                            (const StepState*)c->st, c->flux_deferred ? 1 : 0,
+                                            (const unsigned char*)c->obmap,
+                                            want_max ? c->maxpart : nullptr));
   c->flux_deferred = false;
   return launch_check(c, "pressure_correction");
 }
@@ -654,57 +677,96 @@ int enqueue_ic(cup2d_ctx* c) {
 }
 
 // Helpers.cpp:290-307: u = u(1-chi) + udef*chi.
-__global__ void k_apply_obj_vel(double* u, double* v, const double* chi, const double* udx,
+template <typename T>
+__global__ void k_apply_obj_vel(T* u, T* v, const double* chi, const double* udx,
                                 const double* udy, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const double X = chi[i];
-    u[i] = u[i] * (1 - X) + udx[i] * X;
-    v[i] = v[i] * (1 - X) + udy[i] * X;
+    u[i] = (T)((double)u[i] * (1 - X) + udx[i] * X);
+    v[i] = (T)((double)v[i] * (1 - X) + udy[i] * X);
   }
 }
 int enqueue_apply_obj_vel(cup2d_ctx* c) {
-  hipLaunchKernelGGL(k_apply_obj_vel, dim3(2048), dim3(256), 0, c->stream, c->u, c->v, c->chi,
-                     c->udx, c->udy, (size_t)c->g.NX * c->g.NY);
+  CUP2D_REAL_DISPATCH(c, hipLaunchKernelGGL(k_apply_obj_vel<T>, dim3(2048), dim3(256), 0,
+                                            c->stream, tp<T>(c->u), tp<T>(c->v), c->chi, c->udx,
+                                            c->udy, (size_t)c->g.NX * c->g.NY));
   return launch_check(c, "apply_obj_vel");
 }
 
 // ------------------------------------------------------ layout converters
 // Block layout: [(by*bpdx+bx)][iy][ix][comp]; dense: plane[(by*32+iy)*NX + bx*32+ix].
-__global__ void k_blocks_to_dense(const double* __restrict__ s, double* a, double* b, int nc,
-                                  Grid g, size_t n) {
+// T: the planes' type (float for the velocity and RHS planes at FP32
+// precision); the staging buffer is double.
+template <typename T>
+__global__ void k_blocks_to_dense(const double* __restrict__ s, T* a, T* b, int nc, Grid g,
+                                  size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const size_t blk = i / (BS * BS);
     const int k = (int)(i % (BS * BS)), ix = k % BS, iy = k / BS;
     const int bx = (int)(blk % g.bpdx), by = (int)(blk / g.bpdx);
     const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-    a[d] = s[i * nc];
-    if (nc > 1) b[d] = s[i * nc + 1];
+    a[d] = (T)s[i * nc];
+    if (nc > 1) b[d] = (T)s[i * nc + 1];
   }
 }
-__global__ void k_dense_to_blocks(const double* __restrict__ a, const double* __restrict__ b,
-                                  double* s, int nc, Grid g, size_t n) {
+template <typename T>
+__global__ void k_dense_to_blocks(const T* __restrict__ a, const T* __restrict__ b, double* s,
+                                  int nc, Grid g, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const size_t blk = i / (BS * BS);
     const int k = (int)(i % (BS * BS)), ix = k % BS, iy = k / BS;
     const int bx = (int)(blk % g.bpdx), by = (int)(blk / g.bpdx);
     const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-    s[i * nc] = a[d];
-    if (nc > 1) s[i * nc + 1] = b[d];
+    s[i * nc] = (double)a[d];
+    if (nc > 1) s[i * nc + 1] = (double)b[d];
   }
 }
-int enqueue_blocks_to_dense(cup2d_ctx* c, const double* stage, double* a, double* b, int nc) {
-  hipLaunchKernelGGL(k_blocks_to_dense, dim3(2048), dim3(256), 0, c->stream, stage, a, b, nc,
-                     c->g, (size_t)c->g.NX * c->g.NY);
+int enqueue_blocks_to_dense(cup2d_ctx* c, const double* stage, double* a, double* b, int nc,
+                            bool f32) {
+  const size_t n = (size_t)c->g.NX * c->g.NY;
+  if (f32)
+    hipLaunchKernelGGL(k_blocks_to_dense<float>, dim3(2048), dim3(256), 0, c->stream, stage,
+                       tp<float>(a), b ? tp<float>(b) : nullptr, nc, c->g, n);
+  else
+    hipLaunchKernelGGL(k_blocks_to_dense<double>, dim3(2048), dim3(256), 0, c->stream, stage, a,
+                       b, nc, c->g, n);
   return launch_check(c, "blocks_to_dense");
 }
 int enqueue_dense_to_blocks(cup2d_ctx* c, const double* a, const double* b, double* stage,
-                            int nc) {
-  hipLaunchKernelGGL(k_dense_to_blocks, dim3(2048), dim3(256), 0, c->stream, a, b, stage, nc,
-                     c->g, (size_t)c->g.NX * c->g.NY);
+                            int nc, bool f32) {
+  const size_t n = (size_t)c->g.NX * c->g.NY;
+  if (f32)
+    hipLaunchKernelGGL(k_dense_to_blocks<float>, dim3(2048), dim3(256), 0, c->stream,
+                       tp<float>(a), b ? tp<float>(b) : nullptr, stage, nc, c->g, n);
+  else
+    hipLaunchKernelGGL(k_dense_to_blocks<double>, dim3(2048), dim3(256), 0, c->stream, a, b,
+                       stage, nc, c->g, n);
   return launch_check(c, "dense_to_blocks");
+}
+
+// dense double staging <-> a float plane (FP32 precision)
+__global__ void k_stage_to_f32(const double* __restrict__ s, float* __restrict__ a, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    a[i] = (float)s[i];
+}
+__global__ void k_f32_to_stage(const float* __restrict__ a, double* __restrict__ s, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    s[i] = (double)a[i];
+}
+int enqueue_stage_to_f32(cup2d_ctx* c, const double* stage, double* plane) {
+  hipLaunchKernelGGL(k_stage_to_f32, dim3(2048), dim3(256), 0, c->stream, stage, tp<float>(plane),
+                     (size_t)c->g.NX * c->g.NY);
+  return launch_check(c, "stage_to_f32");
+}
+int enqueue_f32_to_stage(cup2d_ctx* c, const double* plane, double* stage) {
+  hipLaunchKernelGGL(k_f32_to_stage, dim3(2048), dim3(256), 0, c->stream, tp<float>(plane), stage,
+                     (size_t)c->g.NX * c->g.NY);
+  return launch_check(c, "f32_to_stage");
 }
 
 }  // namespace cup2d

@@ -176,6 +176,8 @@ void Simulation::parseRuntime() {
   sim.fadeLen = parser.asDouble("fadeLen", 0.01);
   sim.verbose = parser.asInt("verbose", 1);
   sim.iterativePenalization = parser.asInt("iterativePenalization", 0);
+  sim.precision = parser.asString("precision", "fp64");
+  sim.refineSweeps = parser.asInt("refineSweeps", 1);
   sim.muteAll = parser.asInt("muteAll", 0);
   sim.path2file = parser.asString("file", "./");
   if (sim.muteAll) sim.verbose = 0;

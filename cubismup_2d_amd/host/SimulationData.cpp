@@ -50,6 +50,16 @@ void SimulationData::allocateGrid() {
   cfg.implicit_penal = 0;
   cfg.iterative_penal = iterativePenalization ? 1 : 0;
   cfg.stream = stream;
+  // -precision fp32 is the reference's float build (_FLOAT_PRECISION_,
+  // Definitions.h:18-22) with the Poisson solve refined in FP64
+  if (precision == "fp64" || precision == "")
+    cfg.precision = CUP2D_PRECISION_FP64;
+  else if (precision == "fp32")
+    cfg.precision = CUP2D_PRECISION_FP32;
+  else
+    throw Error(CUP2D_ERR_ARG, "precision '" + precision + "' (fp64, fp32)");
+  cfg.refine_sweeps = cfg.precision == CUP2D_PRECISION_FP32 && kind != CUP2D_POISSON_FREESPACE
+                          ? refineSweeps : 0;
   check(nullptr, cup2d_create(&cfg, &ctx), "cup2d_create");
 }
 

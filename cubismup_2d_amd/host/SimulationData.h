@@ -57,6 +57,8 @@ class SimulationData {
   double fadeLen = 0.01;                 // -fadeLen, fraction of the extent
   double fadeLenX = 0.01, fadeLenY = 0.01;  // fadeLen * extent (Simulation.cpp:97-98)
   int verbose = 1, iterativePenalization = 0, muteAll = 0;
+  std::string precision = "fp64";  // -precision fp64|fp32
+  int refineSweeps = 1;            // -refineSweeps: FP64 sweeps per FP32 Poisson solve
   std::string path2file = "./";  // -file: directory of the text logs
 
   // state (SimulationData.h:41-45)

@@ -63,7 +63,20 @@ typedef struct {
   int iterative_penal;   /* -iterativePenalization: 1 = PressureIterator_unif
                             replaces PressureSingle (Simulation.cpp:215-216) */
   void* stream;          /* hipStream_t to enqueue on; NULL = own stream     */
+  /* Real type of the hot path (the reference's build switch
+   * _FLOAT_PRECISION_, Definitions.h:18-22; fftwf plans FFTW_periodic.h:
+   * 120-126).  CUP2D_PRECISION_FP32: velocity and Poisson-RHS planes in
+   * float, advDiff / RHS / FFT Poisson / projection in float; the pressure,
+   * obstacle fields, body integrals and forces stay FP64, and refine_sweeps
+   * FP64 defect-correction sweeps (FP64 residual of the spectral operator,
+   * FP32 solve of it, FP64 update) follow each FP32 Poisson solve (periodic
+   * and cosine solvers; 0 for free-space).  Iterative penalization needs
+   * FP64.  Fields still cross the C-ABI as double. */
+  int precision;
+  int refine_sweeps;
 } cup2d_cfg;
+
+enum cup2d_precision { CUP2D_PRECISION_FP64 = 0, CUP2D_PRECISION_FP32 = 1 };
 
 /* Shape description: the keys parsed by Shape::Shape, Disk::Disk and
  * Fish / CarlingFish (Shape.cpp:331-345, ShapesSimple.h:19-21, Fish.h:27-29,
@@ -174,6 +187,36 @@ int cup2d_advance(cup2d_ctx* ctx, double dt);
 /* nsteps x (calcMaxTimestep + advance), dt computed on the device, no host
  * synchronization inside (Simulation::simulate, Simulation.cpp:243-252). */
 int cup2d_simulate(cup2d_ctx* ctx, int nsteps);
+/* Imposed surface velocity of a disk, the deformation velocity the reference
+ * keeps in ObstacleBlock::udef (zero for its Disk, ShapeLibrary.cpp:21-34):
+ * with phi the angle about the disk's centre of mass, the velocity along
+ * the outward normal is sum_m a_m cos(m phi) + b_m sin(m phi) and along the
+ * tangent sum_m c_m cos(m phi) + d_m sin(m phi), m = 1..nmodes (<= 32); coef
+ * holds a_1..a_M b_1..b_M c_1..c_M d_1..d_M.  It acts wherever the
+ * reference reads udef: integrateMomenta (PressureSingle.cpp:23-67), the
+ * penalization target (:69-108) and the deformation power of ComputeForces
+ * (ComputeForces.cpp:74-99); not the reset's ApplyObjVel.  nmodes = 0
+ * removes it. */
+int cup2d_set_surface_velocity(cup2d_ctx* ctx, int shape_id, int nmodes, const double* coef);
+
+/* FP32 precision: the relative residual |b - A p| / |b| of the FP64
+ * spectral operator at the start of each refinement sweep of the last
+ * Poisson solve (the residual of the FP32 solve, then of each corrected
+ * solution but the last); *nsweeps receives the sweep count (0 at FP64 or
+ * with the free-space solver).  res holds at least 4 doubles. */
+int cup2d_refine_residuals(cup2d_ctx* ctx, double* res, int* nsweeps);
+
+/* cup2d_simulate with the steps replayed from a captured HIP graph: two
+ * whole steps (PutObjectsOnGrid .. ComputeForces, dt on the device) per
+ * graph launch, captured on first use and cached per step parity.  Each
+ * graph starts and ends with nothing pending on the context's side streams
+ * (the last step's ComputeForces runs inside it), so it is launch-for-launch
+ * the same work as cup2d_simulate.  Needs PressureSingle (no iterative
+ * penalization: its convergence test waits on the host) and the kernel
+ * timers off.  The reference's loop is Simulation::simulate
+ * (Simulation.cpp:243-252). */
+int cup2d_simulate_graph(cup2d_ctx* ctx, int nsteps);
+
 /* Wait for the context's stream; reports device-side errors (body out of
  * domain) raised by enqueued steps. */
 int cup2d_synchronize(cup2d_ctx* ctx);
@@ -211,6 +254,29 @@ int cup2d_timer_read(cup2d_ctx* ctx, const char* kernel, double* total_ms, int* 
 /* Time only every n-th launch of the kernel ("*": all timers; n >= 1, default
  * 1), so a long device loop can be timed with few event pairs on its stream. */
 int cup2d_timer_sample(cup2d_ctx* ctx, const char* kernel, int every);
+
+/* --- ensembles ------------------------------------------------------------
+ * Independent simulations advanced together: the reference runs one
+ * Simulation per process under an RL driver (main_RL_smartCyl.cpp:102-205);
+ * here captured graphs hold two whole steps of every member.  Members are
+ * spread over `lanes` lanes (default 1; 0 = one lane per member): each lane
+ * is one captured graph, its members one after another, launched on its own
+ * stream, the lanes side by side, so two steps of the whole ensemble cost
+ * `lanes` graph launches.  Members are existing contexts on one device (any
+ * grid sizes, solvers and precisions, reset done, same step parity); each
+ * keeps its own API (fields, shapes, forces) between ensemble calls, and
+ * cup2d_ensemble_simulate orders itself after and before their streams.
+ * cup2d_ensemble_tune times every lane count 1, 2, 4, .. n over `launches`
+ * launches each on the live run (those steps are real steps) and keeps the
+ * fastest.  Errors are read with cup2d_ensemble_last_error. */
+typedef struct cup2d_ensemble cup2d_ensemble;
+int cup2d_ensemble_create(cup2d_ctx* const* ctxs, int n, cup2d_ensemble** out);
+int cup2d_ensemble_destroy(cup2d_ensemble* ens);
+const char* cup2d_ensemble_last_error(const cup2d_ensemble* ens);
+int cup2d_ensemble_set_lanes(cup2d_ensemble* ens, int lanes);
+int cup2d_ensemble_simulate(cup2d_ensemble* ens, int nsteps);
+int cup2d_ensemble_synchronize(cup2d_ensemble* ens);
+int cup2d_ensemble_tune(cup2d_ensemble* ens, int launches, int* lanes, double* ms_per_launch);
 
 /* findMaxU over caller-owned device planes, allocating nothing.  No
  * counterpart in the reference: it is for callers that cannot allocate device

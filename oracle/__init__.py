@@ -108,6 +108,8 @@ def lib():
         L.orc_pressure_iterator.restype = C.c_int
         L.orc_iter_stats.argtypes = [P, C.POINTER(C.c_int), dp, dp]
         L.orc_set_collisions.argtypes = [P, C.c_int]
+        L.orc_set_surface_velocity.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.orc_set_surface_velocity.restype = C.c_int
         L.orc_fft1d.argtypes = [dp, C.c_int, C.c_int]
         L.orc_get_timing.argtypes = [P, dp, dp]
         L.orc_num_threads.restype = C.c_int
@@ -310,6 +312,12 @@ class Oracle:
         """Test hook: with enabled=False the body update skips
         preventCollidingObstacles."""
         self.L.orc_set_collisions(self.h, int(enabled))
+
+    def set_surface_velocity(self, coef, shape=0):
+        """Fourier-mode surface velocity of a disk (the library's extension;
+        coef = 4M amplitudes: normal cos, normal sin, tangential cos, sin)."""
+        c = np.ascontiguousarray(np.asarray(coef, dtype=np.float64).ravel())
+        assert self.L.orc_set_surface_velocity(self.h, shape, c.size // 4, _dp(c)) == 0
 
     def advance(self, dt):
         return self.L.orc_advance(self.h, dt)

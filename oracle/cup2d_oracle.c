@@ -99,8 +99,13 @@ struct orc_sim {
   int iter_last;
   double iter_relDF;
   int no_collide; /* test hook: skip preventCollidingObstacles */
+  /* imposed surface velocity of one disk (the library's
+   * cup2d_set_surface_velocity; the reference's Disk keeps udef = 0) */
+  int surf_shape, surf_M;
+  double surf_coef[4 * 32];
 };
 
+static void surf_add(const orc_sim* s, int k, double p0, double p1, double* ux, double* uy);
 static fish_t* fish_new(const orc_shape_desc* d, double h);
 static void fish_free(fish_t* f);
 
@@ -1793,8 +1798,8 @@ void orc_body_update(orc_sim* s, double dt)
       for (int iy = 0; iy < BS; ++iy)
         for (int ix = 0; ix < BS; ++ix) {
           if (o->chi[iy][ix] <= 0) continue;
-          const double udiff[2] = {VEL[2 * (iy * BS + ix)] - o->udef[iy][ix][0],
-                                   VEL[2 * (iy * BS + ix) + 1] - o->udef[iy][ix][1]};
+          double udiff[2] = {VEL[2 * (iy * BS + ix)] - o->udef[iy][ix][0],
+                             VEL[2 * (iy * BS + ix) + 1] - o->udef[iy][ix][1]};
           double F;
           if (!impl) F = hsq * o->rho[iy][ix] * o->chi[iy][ix];
           else {
@@ -1804,6 +1809,12 @@ void orc_body_update(orc_sim* s, double dt)
           double p[2];
           bpos(s, b, ix, iy, p);
           p[0] -= Cx; p[1] -= Cy;
+          {
+            double su = 0, sv = 0;
+            surf_add(s, (int)(sh - s->shapes), p[0], p[1], &su, &sv);
+            udiff[0] -= su;
+            udiff[1] -= sv;
+          }
           PM += F;
           PJ += F * (p[0] * p[0] + p[1] * p[1]);
           PX += F * p[0]; PY += F * p[1];
@@ -1825,6 +1836,38 @@ void orc_body_update(orc_sim* s, double dt)
 }
 
 void orc_set_collisions(orc_sim* s, int enabled) { s->no_collide = !enabled; }
+
+/* Fourier-mode surface velocity of disk `shape`, added wherever udef is read
+ * (integrateMomenta, collisions, penalize, ComputeForces' deformation
+ * power): with z = e^{i phi} = p/|p| about the centre of mass,
+ * u_n = sum_m a_m Re z^m + b_m Im z^m along the normal z and
+ * u_t = sum_m c_m Re z^m + d_m Im z^m along the tangent i z. */
+int orc_set_surface_velocity(orc_sim* s, int shape, int M, const double* coef)
+{
+  if (M < 0 || M > 32 || (M > 0 && (shape < 0 || shape >= s->nshapes))) return -1;
+  s->surf_shape = M > 0 ? shape : -1;
+  s->surf_M = M;
+  for (int i = 0; i < 4 * M; ++i) s->surf_coef[i] = coef[i];
+  return 0;
+}
+static void surf_add(const orc_sim* s, int k, double p0, double p1, double* ux, double* uy)
+{
+  if (k != s->surf_shape || s->surf_M <= 0) return;
+  const double r = sqrt(p0 * p0 + p1 * p1);
+  const double zr = r > 0 ? p0 / r : 1.0, zi = r > 0 ? p1 / r : 0.0;
+  const int M = s->surf_M;
+  const double* c = s->surf_coef;
+  double wr = 1, wi = 0, un = 0, ut = 0;
+  for (int m = 0; m < M; ++m) {
+    const double tr = wr * zr - wi * zi;
+    wi = wr * zi + wi * zr;
+    wr = tr;
+    un += c[m] * wr + c[M + m] * wi;
+    ut += c[2 * M + m] * wr + c[3 * M + m] * wi;
+  }
+  *ux += un * zr - ut * zi;
+  *uy += un * zi + ut * zr;
+}
 
 /* Shape::updateVelocity (Shape.cpp:25-59) from the momenta in the state */
 static void update_velocity(orc_sim* s, shape_t* sh, double dt)
@@ -1873,12 +1916,16 @@ static void prevent_colliding_obstacles(orc_sim* s)
             const double iUr = -si->omega * (pos[1] - si->centerOfMass[1]), iVr = si->omega * (pos[0] - si->centerOfMass[0]);
             const double jUr = -sj->omega * (pos[1] - sj->centerOfMass[1]), jVr = sj->omega * (pos[0] - sj->centerOfMass[0]);
             const double X = oi->chi[iy][ix], Y = oj->chi[iy][ix];
+            double iUd = oi->udef[iy][ix][0], iVd = oi->udef[iy][ix][1];
+            double jUd = oj->udef[iy][ix][0], jVd = oj->udef[iy][ix][1];
+            surf_add(s, i, pos[0] - si->centerOfMass[0], pos[1] - si->centerOfMass[1], &iUd, &iVd);
+            surf_add(s, j, pos[0] - sj->centerOfMass[0], pos[1] - sj->centerOfMass[1], &jUd, &jVd);
             c[0] += X; c[1] += X * pos[0]; c[2] += X * pos[1];
-            c[3] += X * (si->u + iUr + oi->udef[iy][ix][0]);
-            c[4] += X * (si->v + iVr + oi->udef[iy][ix][1]);
+            c[3] += X * (si->u + iUr + iUd);
+            c[4] += X * (si->v + iVr + iVd);
             c[5] += Y; c[6] += Y * pos[0]; c[7] += Y * pos[1];
-            c[8] += Y * (sj->u + jUr + oj->udef[iy][ix][0]);
-            c[9] += Y * (sj->v + jVr + oj->udef[iy][ix][1]);
+            c[8] += Y * (sj->u + jUr + jUd);
+            c[9] += Y * (sj->v + jVr + jVd);
           }
       }
     }
@@ -1939,8 +1986,9 @@ void orc_penalize(orc_sim* s, double dt)
           p[0] -= Cx; p[1] -= Cy;
           const double alpha = s->cfg.implicit_penal ? 1 / (1 + s->cfg.lambda * dt * o->chi[iy][ix])
                                                      : 1 - o->chi[iy][ix];
-          const double US = u_s - omega_s * p[1] + o->udef[iy][ix][0];
-          const double VS = v_s + omega_s * p[0] + o->udef[iy][ix][1];
+          double US = u_s - omega_s * p[1] + o->udef[iy][ix][0];
+          double VS = v_s + omega_s * p[0] + o->udef[iy][ix][1];
+          surf_add(s, k, p[0], p[1], &US, &VS);
           V[2 * (iy * BS + ix)] = alpha * V[2 * (iy * BS + ix)] + (1 - alpha) * US;
           V[2 * (iy * BS + ix) + 1] = alpha * V[2 * (iy * BS + ix) + 1] + (1 - alpha) * VS;
         }
@@ -2183,7 +2231,8 @@ void orc_compute_forces(orc_sim* s)
           const double fYV = D12 * normX + D22 * normY, fYP = -Pc * normY;
           const double fXT = fXV + fXP, fYT = fYV + fYP;
           const double vx = LAB(V, 2, ix, iy, 0), vy = LAB(V, 2, ix, iy, 1);
-          const double vxDef = O->udef[iy][ix][0], vyDef = O->udef[iy][ix][1];
+          double vxDef = O->udef[iy][ix][0], vyDef = O->udef[iy][ix][1];
+          surf_add(s, (int)(sh - s->shapes), p[0] - Cx, p[1] - Cy, &vxDef, &vyDef);
           O->perimeter += sqrt(normX * normX + normY * normY);
           O->circulation += normX * vy - normY * vx;
           O->forcex += fXT; O->forcey += fYT;

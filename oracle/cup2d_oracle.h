@@ -102,6 +102,7 @@ void   orc_iter_stats(const orc_sim* s, int* iters, double* relDF, double* targe
 /* Test hook: enabled = 0 skips preventCollidingObstacles in the body update,
  * so a test can see the velocities the impulse starts from. */
 void   orc_set_collisions(orc_sim* s, int enabled);
+int    orc_set_surface_velocity(orc_sim* s, int shape, int M, const double* coef);
 void   orc_compute_forces(orc_sim* s);
 int    orc_advance(orc_sim* s, double dt);    /* Simulation::advance         */
 int    orc_simulate(orc_sim* s, int nsteps);  /* nsteps x (dt + advance)     */

@@ -281,3 +281,43 @@ def test_divergence_after_projection(cup, oracle_mod, poisson):
           f"away from body and faces x{drop:.2e}, rel L2 vs oracle {e:.2e}", flush=True)
     assert e < 1e-9
     assert drop < 1e-2  # the projection did its work
+
+
+@pytest.mark.parametrize("device_loop", [False, True])
+def test_surface_velocity_vs_oracle(cup, oracle_mod, device_loop):
+    """The Fourier-mode surface velocity (16 modes, normal and tangential,
+    coefficients N(0, 1e-2)) on a towed disk and on a free disk: it enters
+    the penalization target, the free disk's momentum integrals and the
+    deformation power, on the device as in the oracle."""
+    rng = np.random.default_rng(91)
+    shapes = [dict(radius=0.05, xpos=0.25, ypos=0.25, xvel=0.2, bFixed=True, bForced=True),
+              dict(radius=0.04, xpos=0.6, ypos=0.3, rhoS=1.3)]
+    kw = dict(extent=1.0, nu=1e-4, CFL=0.2, poisson="freespace")
+    g = cup.Simulation(16, 8, shapes=[cup.disk(**s) for s in shapes], **kw)
+    o = oracle_mod.Oracle(16, 8, fs_pad2=True, shapes=[oracle_mod.disk(**s) for s in shapes],
+                          **kw)
+    g.reset()
+    o.reset()
+    # one shape carries a surface velocity at a time: the free disk, so the
+    # momentum integrals see it
+    coef = rng.normal(0.0, 1e-2, 64)
+    g.set_surface_velocity(coef, shape=1)
+    o.set_surface_velocity(coef, shape=1)
+    n = 12
+    if device_loop:
+        g.simulate(n)
+        o.simulate(n)
+    else:
+        for _ in range(n):
+            dg, do = g.calc_max_timestep(), o.calc_max_timestep()
+            g.advance(dg)
+            o.advance(do)
+    ev, ep = rel(g.get_field(0), o.get_field(0)), rel(g.get_field(2), o.get_field(2))
+    a, b = g.shape(1), o.shape(1)
+    print(f"  rel L2 vel {ev:.3e} pres {ep:.3e}; free disk u {a['u']:.6e} vs {b['u']:.6e}",
+          flush=True)
+    assert ev < 1e-10 and ep < 1e-10
+    for key in ["u", "v", "omega"]:
+        assert abs(a[key] - b[key]) <= 1e-10 * max(abs(b[key]), 1e-3), key
+    for key in ["drag", "defPower"]:
+        assert abs(a[key] - b[key]) <= 1e-8 * max(abs(b[key]), 1e-6), key

@@ -357,3 +357,17 @@ def test_cosine_poisson_matches_scipy_dct(oracle_mod):
     Y[0, 0] = 0
     ref = F.dct(F.dct(Y, type=3, axis=1), type=3, axis=0)
     assert np.abs(p - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("pad2", [False, True])
+def test_freespace_poisson_direct_sum_non_pow2(oracle_mod, pad2):
+    """A grid of 3 x 1 blocks (96 x 32 cells): the oracle's (2N-1) = 191 x 63
+    cyclic solve (the reference's padding, FFTW_freespace.h:25-27; mixed radix
+    and Bluestein lengths) and its 2N one equal the brute-force linear
+    convolution with the free-space kernel."""
+    rng = np.random.default_rng(71)
+    o = oracle_mod.Oracle(3, 1, extent=1.0, poisson="freespace", fs_pad2=pad2)
+    r = rng.standard_normal((o.NY, o.NX))
+    p = o.poisson_dense(r)
+    ref = R.poisson_freespace_direct(r, o.scalars()["h"])
+    assert np.linalg.norm(p - ref) / np.linalg.norm(ref) < 1e-13
