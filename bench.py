@@ -87,6 +87,28 @@ def ensemble_members(n=64, seed=3):
 
 ENSEMBLE_KW = dict(extent=1.0, nu=1e-4, CFL=0.1, poisson="freespace")
 
+# The adjoint workload: the impulsively started towed cylinder at 1024x512
+# (free-space, r = 0.05 at (0.25, 0.25) towed at 0.2, nu = 2e-5, CFL 0.1),
+# 200 steps, gradient of the mean drag with respect to 64 surface-velocity
+# coefficients (16 modes, normal and tangential) drawn N(0, 1e-3) from the
+# seed, checkpoints every 20 steps.
+ADJOINT = dict(bpdx=32, bpdy=16, extent=1.0, nu=2e-5, CFL=0.1, poisson="freespace",
+               body=dict(radius=0.05, xpos=0.25, ypos=0.25, xvel=0.2, bFixed=True, bForced=True),
+               nsteps=200, every=20, nmodes=16, sigma=1e-3, seed=4)
+
+
+def adjoint_theta(cfg=ADJOINT):
+    return np.random.default_rng(cfg["seed"]).normal(0.0, cfg["sigma"], 4 * cfg["nmodes"])
+
+
+def adjoint_sim(mod, cfg=ADJOINT, bpdx=None, bpdy=None):
+    """A reset Simulation of the adjoint workload (library or oracle module)."""
+    kw = {k: cfg[k] for k in ("extent", "nu", "CFL", "poisson")}
+    sim = mod.Simulation(bpdx or cfg["bpdx"], bpdy or cfg["bpdy"],
+                         shapes=[mod.disk(**cfg["body"])], **kw)
+    sim.reset()
+    return sim
+
 
 def random_vorticity_velocity(NX, NY, h, sigma=1e-3, seed=12345):
     """Seeded Gaussian vorticity -> periodic streamfunction -> central-difference u."""
@@ -352,6 +374,32 @@ def ensemble_leg(cup, steps, precision):
             "launches_per_step": 0.5}
 
 
+def adjoint_leg(cup):
+    """adjoint/fwd on the adjoint workload: wall time of the backward sweep
+    (every segment recomputed from its checkpoint, then the adjoint steps)
+    over that of the forward run (the same steps, time steps fixed)."""
+    cfg = ADJOINT
+    sim = adjoint_sim(cup, cfg)
+    sim.set_surface_velocity(adjoint_theta(cfg))
+    _, dts = sim.adjoint_forward(cfg["nsteps"], cfg["every"])  # the CFL time steps
+    sim.close()
+    sim = adjoint_sim(cup, cfg)
+    sim.set_surface_velocity(adjoint_theta(cfg))
+    t0 = time.perf_counter()
+    J, _ = sim.adjoint_forward(cfg["nsteps"], cfg["every"], dts)
+    t1 = time.perf_counter()
+    grad = sim.adjoint_backward()
+    t2 = time.perf_counter()
+    sim.close()
+    cells = cfg["bpdx"] * cfg["bpdy"] * 1024
+    return {"adjoint_over_forward": round((t2 - t1) / (t1 - t0), 3),
+            "forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t2 - t1) * 1e3, 2),
+            "workload": f"towed cylinder {cells // 512}x512 free-space, {cfg['nsteps']} steps, "
+                        f"checkpoint every {cfg['every']}, {4 * cfg['nmodes']} surface-velocity "
+                        f"coefficients",
+            "J": J, "grad_norm": float(np.linalg.norm(grad))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,6 +410,7 @@ def main():
                     help="steps of each CPU-baseline mode (lean, faithful)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-precision legs")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/fwd leg")
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="steps of the 64-member ensemble legs (0 skips them)")
     ap.add_argument("--timer-every", type=int, default=8, metavar="K",
@@ -453,6 +502,8 @@ def main():
     sim.close()
     if cfg["poisson"] == "periodic" and not args.no_fp32:
         out["fp32"] = {f"sweeps{k}": fp32_leg(cup, cfg, vel, args, k) for k in (0, 1, 2, 3)}
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(cup)
     if args.ensemble_steps > 0:
         out["ensemble64"] = {p: ensemble_leg(cup, args.ensemble_steps, p)
                              for p in ("fp64", "fp32")}

@@ -87,7 +87,7 @@ EXPORTS = [
     "cup2d_grid_size", "cup2d_timer_enable", "cup2d_timer_read", "cup2d_timer_sample",
     "cup2d_force_log_enable", "cup2d_force_log_read",
     "cup2d_find_max_u_noalloc", "cup2d_simulate_graph", "cup2d_refine_residuals",
-    "cup2d_set_surface_velocity",
+    "cup2d_set_surface_velocity", "cup2d_adjoint_forward", "cup2d_adjoint_backward",
     "cup2d_ensemble_create", "cup2d_ensemble_destroy", "cup2d_ensemble_last_error",
     "cup2d_ensemble_set_lanes", "cup2d_ensemble_simulate", "cup2d_ensemble_synchronize",
     "cup2d_ensemble_tune",
@@ -148,6 +148,8 @@ def lib():
     L.cup2d_simulate_graph.argtypes = [P, I]
     L.cup2d_refine_residuals.argtypes = [P, dp, C.POINTER(I)]
     L.cup2d_set_surface_velocity.argtypes = [P, I, I, dp]
+    L.cup2d_adjoint_forward.argtypes = [P, I, I, dp, dp, dp]
+    L.cup2d_adjoint_backward.argtypes = [P, dp]
     L.cup2d_ensemble_create.argtypes = [C.POINTER(C.c_void_p), I, C.POINTER(C.c_void_p)]
     L.cup2d_ensemble_destroy.argtypes = [P]
     L.cup2d_ensemble_last_error.argtypes = [P]
@@ -217,6 +219,8 @@ class Simulation:
         cfg = Cfg(bpdx, bpdy, extent, nu, CFL, lam, fadeLen, POISSON[poisson],
                   int(implicit_penal), int(iterative_penal), stream, prec, int(refine_sweeps))
         self.precision = precision
+        self._surf_modes = 0
+        self._adj_modes = 0
         h = C.c_void_p()
         rc = self.L.cup2d_create(C.byref(cfg), C.byref(h))
         if rc != OK:
@@ -357,10 +361,33 @@ class Simulation:
         (or the flat 4M vector); None removes it."""
         if coef is None:
             self._chk(self.L.cup2d_set_surface_velocity(self.h, shape, 0, None))
+            self._surf_modes = 0
             return
         c = np.ascontiguousarray(np.asarray(coef, dtype=np.float64).ravel())
         assert c.size % 4 == 0
         self._chk(self.L.cup2d_set_surface_velocity(self.h, shape, c.size // 4, _dp(c)))
+        self._surf_modes = c.size // 4
+
+    def adjoint_forward(self, nsteps, every=20, dts=None):
+        """cup2d_adjoint_forward: J = mean drag over nsteps steps (dt from the
+        CFL rule, or the given sequence); returns (J, the dts used)."""
+        out = np.zeros(nsteps)
+        J = C.c_double()
+        din = None
+        if dts is not None:
+            din = np.ascontiguousarray(np.asarray(dts, dtype=np.float64))
+            assert din.size == nsteps
+        self._chk(self.L.cup2d_adjoint_forward(self.h, nsteps, every,
+                                               _dp(din) if din is not None else None,
+                                               _dp(out), C.byref(J)))
+        self._adj_modes = self._surf_modes
+        return J.value, out
+
+    def adjoint_backward(self):
+        """cup2d_adjoint_backward: dJ/dcoef of the last adjoint_forward."""
+        g = np.zeros(4 * self._adj_modes)
+        self._chk(self.L.cup2d_adjoint_backward(self.h, _dp(g)))
+        return g
 
     def refine_residuals(self):
         """FP32: relative residual at the start of each refinement sweep of

@@ -357,6 +357,105 @@ namespace cup2d {
 void invalidate_graphs(cup2d_ctx* c) { drop_graphs(c); }
 }  // namespace cup2d
 
+// ------------------------------------------------------------ the adjoint
+// Buffers of cup2d_adjoint_forward / _backward (see adjoint.hip for the
+// operator sequence).  Checkpoints hold the velocity and the step and shape
+// state every `every` steps; a segment's recompute stores, per step, the
+// input velocity, the flux-corrected advDiff output and the pressure.
+struct AdjointState {
+  int nsteps = 0, every = 0, nseg = 0, cap_steps = 0;
+  std::vector<double> dts;
+  double uinf0 = 0, uinf1 = 0, vu0 = 1, vu1 = 0;
+  size_t n = 0;                        // cells
+  std::vector<double*> ck;             // [nseg + 1] (u, v) pairs; the last: the final state
+  std::vector<StepState> ck_st;
+  std::vector<std::vector<cup2d_shape_state>> ck_ss;
+  double* seg = nullptr;               // (every + 1) (u, v), every (wu, wv), every p
+  double* work = nullptr;              // ub vb pb rb wbu wbv ub2 vb2
+  double* part = nullptr;              // 5 per block
+  double* theta_part = nullptr;
+  double* theta = nullptr;             // [4M]
+  double* scal = nullptr;              // J, flux sum
+  bool ready = false;                  // a forward run whose backward may follow
+};
+
+namespace {
+void adjoint_free(cup2d_ctx* c) {
+  AdjointState* a = c->adj;
+  if (!a) return;
+  for (double* p : a->ck) (void)hipFree(p);
+  for (double* p : {a->seg, a->work, a->part, a->theta_part, a->theta, a->scal})
+    if (p) (void)hipFree(p);
+  delete a;
+  c->adj = nullptr;
+}
+
+// the forced, fixed single disk with an imposed surface velocity, explicit
+// penalization, FP64: the configuration the adjoint is written for
+int adjoint_eligible(cup2d_ctx* c) {
+  if (!c->shapes_frozen) return fail(c, CUP2D_ERR_STATE, "call reset first");
+  if (c->f32) return fail(c, CUP2D_ERR_STATE, "the adjoint runs at FP64 precision");
+  if (c->cfg.iterative_penal || c->cfg.implicit_penal)
+    return fail(c, CUP2D_ERR_STATE, "the adjoint needs PressureSingle with explicit penalization");
+  if (c->g.nshapes != 1 || c->hsp[0].kind != 0)
+    return fail(c, CUP2D_ERR_STATE, "the adjoint is written for one disk");
+  const ShapeParams& p = c->hsp[0];
+  if (!(p.bForcedx && p.bForcedy && p.bBlockAng && p.bFixedx && p.bFixedy))
+    return fail(c, CUP2D_ERR_STATE, "the adjoint's disk must be forced and fixed (its "
+                                    "velocity and position are then independent of the flow)");
+  if (c->surf_shape != 0 || c->surf_modes <= 0)
+    return fail(c, CUP2D_ERR_STATE, "set the disk's surface velocity first");
+  return need_fft(c);
+}
+
+// one forward step at fixed dt, operator by operator on the main stream
+// (put objects, advDiff with its flux correction, PressureSingle, forces and
+// the step end); rec != nullptr stores the flux-corrected advDiff output and
+// the pressure there.
+int adjoint_fwd_step(cup2d_ctx* c, double dt, double* rec_w, double* rec_p) {
+  const size_t n = (size_t)c->g.NX * c->g.NY;
+  RC(enqueue_set_dt(c, dt));
+  RC(enqueue_put_objects(c, 0, 0));
+  RC(enqueue_advdiff(c, 0));
+  if (rec_w) {
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(rec_w, c->u, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(rec_w + n, c->v, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  }
+  RC(pressure_single(c, 0));
+  RC(enqueue_forces(c, 1, 0));
+  if (rec_p)
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(rec_p, c->pres, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  return CUP2D_OK;
+}
+
+int adjoint_save(cup2d_ctx* c, int k) {
+  AdjointState* a = c->adj;
+  const size_t n = a->n;
+  CUP2D_HIP_CHECK(c, hipMemcpyAsync(a->ck[k], c->u, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  CUP2D_HIP_CHECK(c, hipMemcpyAsync(a->ck[k] + n, c->v, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  CUP2D_HIP_CHECK(c, hipMemcpyAsync(&a->ck_st[k], c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
+  a->ck_ss[k].resize(c->g.nshapes);
+  CUP2D_HIP_CHECK(c, hipMemcpyAsync(a->ck_ss[k].data(), c->ss, sizeof(cup2d_shape_state) * c->g.nshapes,
+                                    hipMemcpyDeviceToHost, c->stream));
+  CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return CUP2D_OK;
+}
+
+int adjoint_restore(cup2d_ctx* c, int k) {
+  AdjointState* a = c->adj;
+  const size_t n = a->n;
+  RC(sync_all(c));
+  CUP2D_HIP_CHECK(c, hipMemcpy(c->u, a->ck[k], sizeof(double) * n, hipMemcpyDeviceToDevice));
+  CUP2D_HIP_CHECK(c, hipMemcpy(c->v, a->ck[k] + n, sizeof(double) * n, hipMemcpyDeviceToDevice));
+  CUP2D_HIP_CHECK(c, hipMemcpy(c->st, &a->ck_st[k], sizeof(StepState), hipMemcpyHostToDevice));
+  CUP2D_HIP_CHECK(c, hipMemcpy(c->ss, a->ck_ss[k].data(), sizeof(cup2d_shape_state) * c->g.nshapes,
+                               hipMemcpyHostToDevice));
+  c->host_step = a->ck_st[k].step;
+  c->midA_ready = false;
+  return CUP2D_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* cup2d_last_error(const cup2d_ctx* c) {
@@ -549,6 +648,7 @@ int cup2d_destroy(cup2d_ctx* c) {
   if (c->ev_qmid) (void)hipEventDestroy(c->ev_qmid);
   if (c->fsnap) (void)hipFree(c->fsnap);
   if (c->surf_coef) (void)hipFree(c->surf_coef);
+  adjoint_free(c);
   if (c->flog) (void)hipFree(c->flog);
   if (c->flog_cnt) (void)hipFree(c->flog_cnt);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -871,6 +971,150 @@ int cup2d_set_surface_velocity(cup2d_ctx* c, int shape_id, int nmodes, const dou
   c->surf_shape = nmodes > 0 ? shape_id : -1;
   c->surf_modes = nmodes;
   return CUP2D_OK;
+}
+
+int cup2d_adjoint_forward(cup2d_ctx* c, int nsteps, int every, const double* dts_in,
+                          double* dts_out, double* J) {
+  CHECK_CTX(c);
+  RC(adjoint_eligible(c));
+  if (nsteps < 1 || every < 1) return fail(c, CUP2D_ERR_ARG, "nsteps and every must be >= 1");
+  RC(sync_all(c));
+  // the obstacle blocks and the fade band must not overlap (the adjoint of
+  // the fade holds the RHS formed by the divergence alone)
+  {
+    std::vector<unsigned char> om((size_t)c->g.bpdx * c->g.bpdy);
+    CUP2D_HIP_CHECK(c, hipMemcpy(om.data(), c->obmap, om.size(), hipMemcpyDeviceToHost));
+    for (int by = 0; by < c->g.bpdy; ++by)
+      for (int bx = 0; bx < c->g.bpdx; ++bx)
+        if (om[bx + c->g.bpdx * by] && band_block(c->g, bx, by))
+          return fail(c, CUP2D_ERR_STATE, "the disk's blocks reach the boundary fade band");
+  }
+  const size_t n = (size_t)c->g.NX * c->g.NY;
+  const int nseg = (nsteps + every - 1) / every;
+  AdjointState* a = c->adj;
+  if (!a || a->n != n || a->every != every || a->nseg < nseg) {
+    adjoint_free(c);
+    a = c->adj = new AdjointState();
+    a->n = n;
+    a->every = every;
+    a->nseg = nseg;
+    a->ck.assign(nseg + 1, nullptr);
+    for (double*& p : a->ck) CUP2D_HIP_CHECK(c, hipMalloc(&p, sizeof(double) * 2 * n));
+    CUP2D_HIP_CHECK(c, hipMalloc(&a->seg, sizeof(double) * n * (2 * (every + 1) + 3 * every)));
+    CUP2D_HIP_CHECK(c, hipMalloc(&a->work, sizeof(double) * n * 8));
+    const int G = c->ob.cap > 0 ? c->ob.cap : 1;
+    const int nblk = std::max(G, c->band_nblocks);
+    CUP2D_HIP_CHECK(c, hipMalloc(&a->part, sizeof(double) * 5 * (size_t)(nblk + 1)));
+    CUP2D_HIP_CHECK(c, hipMalloc(&a->theta_part, sizeof(double) * 4 * SURF_MAX_MODES * (size_t)G));
+    CUP2D_HIP_CHECK(c, hipMalloc(&a->theta, sizeof(double) * 4 * SURF_MAX_MODES));
+    CUP2D_HIP_CHECK(c, hipMalloc(&a->scal, sizeof(double) * 4));
+  }
+  a->ck_st.assign(nseg + 1, StepState{});
+  a->ck_ss.assign(nseg + 1, {});
+  a->nsteps = nsteps;
+  a->dts.assign(nsteps, 0.0);
+  a->ready = false;
+  drop_graphs(c);
+  RC(enqueue_adjoint_fill(c, a->scal, 1, 0.0));
+  RC(adjoint_save(c, 0));
+  const double wJ = 1.0 / nsteps;
+  for (int k = 0; k < nsteps; ++k) {
+    double dt;
+    if (dts_in) {
+      dt = dts_in[k];
+    } else {  // Simulation::calcMaxTimestep
+      RC(enqueue_dt(c));
+      StepState s;
+      CUP2D_HIP_CHECK(c, hipMemcpyAsync(&s, c->st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+      CUP2D_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+      dt = s.dt;
+    }
+    a->dts[k] = dt;
+    RC(adjoint_fwd_step(c, dt, nullptr, nullptr));
+    c->host_step += 1;
+    RC(enqueue_adjoint_acc_drag(c, 0, wJ, a->scal));
+    if ((k + 1) % every == 0 && k + 1 < nsteps) RC(adjoint_save(c, (k + 1) / every));
+  }
+  RC(adjoint_save(c, nseg));  // the final state, restored after the backward sweep
+  {
+    const StepState& s = a->ck_st[nseg];
+    a->uinf0 = s.uinf[0];
+    a->uinf1 = s.uinf[1];
+    const cup2d_shape_state& sh = a->ck_ss[nseg][0];
+    const double vn = std::sqrt(sh.u * sh.u + sh.v * sh.v);
+    a->vu0 = vn > 0 ? sh.u / vn : 0;
+    a->vu1 = vn > 0 ? sh.v / vn : 0;
+  }
+  double Jh = 0;
+  CUP2D_HIP_CHECK(c, hipMemcpy(&Jh, a->scal, sizeof(double), hipMemcpyDeviceToHost));
+  if (J) *J = Jh;
+  if (dts_out) std::memcpy(dts_out, a->dts.data(), sizeof(double) * nsteps);
+  a->ready = true;
+  return sync_and_check(c);
+}
+
+int cup2d_adjoint_backward(cup2d_ctx* c, double* grad) {
+  CHECK_CTX(c);
+  AdjointState* a = c->adj;
+  if (!a || !a->ready) return fail(c, CUP2D_ERR_STATE, "run cup2d_adjoint_forward first");
+  if (!grad) return fail(c, CUP2D_ERR_ARG, "null gradient");
+  RC(adjoint_eligible(c));
+  const size_t n = a->n;
+  const int K = a->every, N = a->nsteps;
+  double* ub = a->work;
+  double* vb = ub + n;
+  double* pb = vb + n;
+  double* rb = pb + n;
+  double* wbu = rb + n;
+  double* wbv = wbu + n;
+  double* ub2 = wbv + n;
+  double* vb2 = ub2 + n;
+  double* segU = a->seg;                       // (K + 1) x (u, v)
+  double* segW = segU + 2 * n * (K + 1);       // K x (wu, wv)
+  double* segP = segW + 2 * n * K;             // K x p
+  const int M = c->surf_modes;
+  RC(enqueue_adjoint_fill(c, ub, 2 * n, 0.0));
+  RC(enqueue_adjoint_fill(c, a->theta, 4 * M, 0.0));
+  const int nseg = (N + K - 1) / K;
+  for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+    const int k0 = sgi * K, len = std::min(K, N - k0);
+    RC(adjoint_restore(c, sgi));
+    // recompute the segment, storing each step's input, advDiff output and pressure
+    for (int j = 0; j < len; ++j) {
+      CUP2D_HIP_CHECK(c, hipMemcpyAsync(segU + 2 * n * j, c->u, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+      CUP2D_HIP_CHECK(c, hipMemcpyAsync(segU + 2 * n * j + n, c->v, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+      RC(adjoint_fwd_step(c, a->dts[k0 + j], segW + 2 * n * j, segP + n * j));
+    }
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(segU + 2 * n * len, c->u, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    CUP2D_HIP_CHECK(c, hipMemcpyAsync(segU + 2 * n * len + n, c->v, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    for (int j = len - 1; j >= 0; --j) {
+      AdjStep s{};
+      s.shape = 0;
+      s.dt = a->dts[k0 + j];
+      s.uinf0 = a->uinf0;
+      s.uinf1 = a->uinf1;
+      s.vu0 = a->vu0;
+      s.vu1 = a->vu1;
+      s.wJ = 1.0 / N;
+      s.u0 = segU + 2 * n * j;
+      s.v0 = s.u0 + n;
+      s.wu = segW + 2 * n * j;
+      s.wv = s.wu + n;
+      s.u1 = segU + 2 * n * (j + 1);
+      s.v1 = s.u1 + n;
+      s.p1 = segP + n * j;
+      s.ub = ub; s.vb = vb; s.pb = pb; s.rb = rb; s.wbu = wbu; s.wbv = wbv;
+      s.part = a->part;
+      s.theta_part = a->theta_part;
+      s.theta = a->theta;
+      s.fluxsum = a->scal + 1;
+      (void)ub2; (void)vb2;
+      RC(enqueue_adjoint_step(c, s));
+    }
+  }
+  CUP2D_HIP_CHECK(c, hipMemcpyAsync(grad, a->theta, sizeof(double) * 4 * M, hipMemcpyDeviceToHost, c->stream));
+  RC(adjoint_restore(c, nseg));  // back to the state after the forward run
+  return sync_and_check(c);
 }
 
 int cup2d_refine_residuals(cup2d_ctx* c, double* res, int* nsweeps) {

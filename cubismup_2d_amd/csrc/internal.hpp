@@ -205,6 +205,7 @@ struct cup2d_ctx {
     const int* ob0 = nullptr;
   };
   StepGraph graphs[2];
+  struct AdjointState* adj = nullptr;  // cup2d_adjoint_* buffers (api.hip)
   double* surf_coef = nullptr;  // device [4 SURF_MAX_MODES]
   int surf_shape = -1, surf_modes = 0;
   bool graph_capture = false;  // enqueueing into a capture: device-side step index
@@ -383,6 +384,25 @@ int flush_forces(cup2d_ctx* c);
 int join_forces(cup2d_ctx* c);
 int enqueue_shapes_reset(cup2d_ctx* c);
 int enqueue_obst_reset(cup2d_ctx* c);
+
+// adjoint.hip: one step of the adjoint of the drag objective (see there)
+struct AdjStep {
+  int shape;
+  double dt, uinf0, uinf1, vu0, vu1, wJ;
+  const double *u0, *v0;   // the step's input velocity
+  const double *wu, *wv;   // its flux-corrected advDiff output
+  const double *u1, *v1;   // its output velocity
+  const double* p1;        // its pressure
+  double *ub, *vb;         // in: adjoint of the output velocity; out: of the input
+  double *pb, *rb, *wbu, *wbv;  // work planes
+  double* part;            // work partials (5 per block)
+  double* theta_part;      // [obstacle slots][4M]
+  double* theta;           // [4M] gradient, accumulated
+  double* fluxsum;         // one double
+};
+int enqueue_adjoint_step(cup2d_ctx* c, const AdjStep& a);
+int enqueue_adjoint_fill(cup2d_ctx* c, double* a, size_t n, double val);
+int enqueue_adjoint_acc_drag(cup2d_ctx* c, int shape, double w, double* J);
 
 // fish.hip
 // FishData::FishData + width table (FishLibrary.cpp:14-38); returns Nm or -1

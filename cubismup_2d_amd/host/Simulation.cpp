@@ -177,6 +177,9 @@ void Simulation::parseRuntime() {
   sim.verbose = parser.asInt("verbose", 1);
   sim.iterativePenalization = parser.asInt("iterativePenalization", 0);
   sim.precision = parser.asString("precision", "fp64");
+  sim.dumpFreq = parser.asInt("fdump", 0);  // Simulation.cpp:84-85
+  sim.dumpTime = parser.asDouble("tdump", 0);
+  sim.path4serialization = parser.asString("serialization", "./");
   sim.refineSweeps = parser.asInt("refineSweeps", 1);
   sim.muteAll = parser.asInt("muteAll", 0);
   sim.path2file = parser.asString("file", "./");
@@ -221,6 +224,7 @@ void Simulation::reset() {
   check(sim.ctx, cup2d_reset(sim.ctx), "Simulation::reset");
   sim.pullScalars();
   sim.refreshShapes();
+  if (sim.dumpFreq > 0 || sim.dumpTime > 0) sim.dumpAll("IC");  // Simulation.cpp:229
 }
 
 double Simulation::calcMaxTimestep() {
@@ -237,6 +241,7 @@ double Simulation::calcMaxTimestep() {
 
 bool Simulation::advance(const double dt) {
   if (!(dt > 2.2e-16)) throw Error(CUP2D_ERR_ARG, "advance: dt must be > 2.2e-16");
+  const bool bDump = sim.bDump();  // Simulation.cpp:282
   sim.dt = dt;
   for (auto& op : pipeline) (*op)(sim.dt);
   sim.pullScalars();  // U_inf was set on the device by PutObjectsOnGrid
@@ -244,6 +249,10 @@ bool Simulation::advance(const double dt) {
   sim.step++;
   sim.pushScalars();
   sim.refreshShapes();
+  if (bDump) {  // Simulation.cpp:292-297
+    sim.registerDump();
+    sim.dumpAll("avemaria_");
+  }
   return sim.bOver();
 }
 

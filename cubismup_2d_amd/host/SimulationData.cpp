@@ -1,6 +1,10 @@
 // SimulationData.cpp -- device context lifetime and host <-> device state.
 #include "SimulationData.h"
 
+#include <cstdio>
+
+#include "DumpHDF5.h"
+
 #include <algorithm>
 #include <cmath>
 
@@ -75,6 +79,37 @@ void SimulationData::resetAll() {
   step = 0;
   uinfx = 0;
   uinfy = 0;
+  nextDumpTime = 0;
+}
+
+bool SimulationData::bDump() const {
+  const bool timeDump = dumpTime > 0 && time >= nextDumpTime;
+  const bool stepDump = dumpFreq > 0 && (step % dumpFreq) == 0;
+  return stepDump || timeDump;
+}
+
+void SimulationData::registerDump() { nextDumpTime += dumpTime; }
+
+void SimulationData::dumpAll(const std::string& name) {
+  const size_t n = (size_t)NX() * NY();
+  std::vector<double> v(2 * n), c(n), p(n), r(n);
+  download(vel, v.data(), false);
+  download(chi, c.data(), false);
+  download(pres, p.data(), false);
+  download(invRho, r.data(), false);
+  char num[16];
+  std::snprintf(num, sizeof(num), "%07d", step);
+  const double h = getH();
+  const double* glue[3] = {v.data(), v.data() + n, c.data()};
+  const double* ps[1] = {p.data()};
+  const double* rs[1] = {r.data()};
+  if (write_h5_xmf(path4serialization, "velChi_" + name + num, time, step, NX(), NY(), h, 3, glue,
+                   "Vector") ||
+      write_h5_xmf(path4serialization, "pres_" + name + num, time, step, NX(), NY(), h, 1, ps,
+                   "Scalar") ||
+      write_h5_xmf(path4serialization, "invRho_" + name + num, time, step, NX(), NY(), h, 1, rs,
+                   "Scalar"))
+    throw Error(CUP2D_ERR_ARG, "dumpAll: cannot write into " + path4serialization);
 }
 
 bool SimulationData::bOver() const {

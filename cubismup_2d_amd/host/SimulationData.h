@@ -60,6 +60,12 @@ class SimulationData {
   std::string precision = "fp64";  // -precision fp64|fp32
   int refineSweeps = 1;            // -refineSweeps: FP64 sweeps per FP32 Poisson solve
   std::string path2file = "./";  // -file: directory of the text logs
+  // field dumps (SimulationData.h:62-67): every dumpFreq steps (-fdump) and
+  // every dumpTime of simulated time (-tdump), into path4serialization
+  // (-serialization)
+  int dumpFreq = 0;
+  double dumpTime = 0, nextDumpTime = 0;
+  std::string path4serialization = "./";
 
   // state (SimulationData.h:41-45)
   double time = 0, dt = 0, uinfx = 0, uinfy = 0;
@@ -86,6 +92,12 @@ class SimulationData {
   // SimulationData::resetAll (SimulationData.cpp:99-109).
   void resetAll();
   bool bOver() const;        // SimulationData.cpp:172-177
+  bool bDump() const;        // SimulationData.cpp:179-185
+  void registerDump();       // SimulationData.cpp:111-114
+  // SimulationData::dumpAll (SimulationData.cpp:207-239): "velChi_" (u, v,
+  // chi), "pres_" and "invRho_" + name + the step in seven digits, as HDF5 +
+  // XDMF (DumpHDF5.h)
+  void dumpAll(const std::string& name);
   double maxRelSpeed() const;  // SimulationData.cpp:142-148
   double getH() const;         // h = extent / max(NX, NY)
   int NX() const { return bpdx * CUP2D_BS; }

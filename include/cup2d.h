@@ -199,6 +199,23 @@ int cup2d_simulate(cup2d_ctx* ctx, int nsteps);
  * removes it. */
 int cup2d_set_surface_velocity(cup2d_ctx* ctx, int shape_id, int nmodes, const double* coef);
 
+/* --- adjoint --------------------------------------------------------------
+ * Gradient of the time-averaged drag J = (1/nsteps) sum_n drag_n of a forced,
+ * fixed disk (ComputeForces.cpp:91-93) with respect to the coefficients of
+ * its surface velocity (cup2d_set_surface_velocity), by a hand-written
+ * discrete adjoint of the whole step.  No counterpart in the reference.
+ * cup2d_adjoint_forward runs nsteps steps from the current state (dt from
+ * the CFL rule, or dts_in[k] if given; the dts used go to dts_out when not
+ * NULL), checkpoints the velocity every `every` steps and returns J;
+ * cup2d_adjoint_backward recomputes each segment from its checkpoint,
+ * sweeps the adjoint backwards and writes dJ/dcoef (4 nmodes doubles, in
+ * the coefficient layout), then restores the state after the forward run.
+ * The configuration: one disk, forced and fixed, explicit penalization,
+ * FP64, its blocks outside the boundary fade band. */
+int cup2d_adjoint_forward(cup2d_ctx* ctx, int nsteps, int every, const double* dts_in,
+                          double* dts_out, double* J);
+int cup2d_adjoint_backward(cup2d_ctx* ctx, double* grad);
+
 /* FP32 precision: the relative residual |b - A p| / |b| of the FP64
  * spectral operator at the start of each refinement sweep of the last
  * Poisson solve (the residual of the FP32 solve, then of each corrected

@@ -156,3 +156,35 @@ def test_driver_carlingfish_matches_oracle(oracle_mod, tmp_path):
     so = o.shape()
     for key in ["u", "v", "omega"]:
         assert abs(out["shapes"][0][key] - so[key]) <= 1e-10 * max(abs(so[key]), 1e-3), key
+
+
+@pytest.mark.gpu
+def test_driver_field_dumps(tmp_path):
+    """-fdump 2 -serialization <dir>: the IC dump at reset and a dump after
+    every step that starts at an even step (SimulationData::bDump,
+    Simulation.cpp:282-297), as HDF5 + XDMF; the last one holds the final
+    fields (rounded to float)."""
+    _need_driver()
+    from test_io import HDF5_LIB, _hdf5, _read
+    if not os.path.exists(HDF5_LIB):
+        pytest.skip("no HDF5 library in this image")
+    dump = str(tmp_path / "final.bin")
+    r = _run(C1_SMALL + ["-nsteps", "2", "-fdump", "2", "-serialization", str(tmp_path),
+                         "-dumpFinal", dump])
+    assert r.returncode == 0, r.stderr
+    names = sorted(p.name for p in tmp_path.iterdir())
+    for stem in ("velChi_IC0000000", "velChi_avemaria_0000001", "velChi_avemaria_0000003",
+                 "pres_avemaria_0000003", "invRho_avemaria_0000003"):
+        assert stem + ".h5" in names and stem + ".xmf" in names, (stem, names)
+    NX, NY = 512, 256
+    buf = np.fromfile(dump, dtype=np.float64)
+    vel = buf[:2 * NX * NY].reshape(2, NY, NX)
+    pres = buf[2 * NX * NY:].reshape(NY, NX)
+    h5 = _hdf5()
+    arr, attrs = _read(h5, str(tmp_path / "velChi_avemaria_0000003.h5"))
+    assert attrs["step"] == 3
+    assert np.array_equal(arr[:, :, 0], vel[0].astype(np.float32))
+    assert np.array_equal(arr[:, :, 1], vel[1].astype(np.float32))
+    assert arr[:, :, 2].max() == 1.0 and arr[:, :, 2].min() == 0.0  # chi
+    parr, _ = _read(h5, str(tmp_path / "pres_avemaria_0000003.h5"))
+    assert np.array_equal(parr[:, :, 0], pres.astype(np.float32))
