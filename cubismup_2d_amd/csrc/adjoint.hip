@@ -250,6 +250,8 @@ __global__ __launch_bounds__(ANT) void k_adj_fade2(Grid g, const int* __restrict
   const double Sr = tot[0], P = dmax(tot[1], EPSD), Nn = dmax(tot[2], EPSD);
   const double corr = Sr > 0 ? Sr / P : Sr / Nn;
   const double Cbar = corr > 0 ? tot[3] : tot[4];
+  // a denominator held at epsilon (fade_corr_factor's clamp) has no derivative
+  const double kP = tot[1] < EPSD ? 0.0 : 1.0, kN = tot[2] < EPSD ? 0.0 : 1.0;
   const int b = blocks[j], bx = b % g.bpdx, by = b / g.bpdx;
   for (int i = threadIdx.x; i < BS * BS; i += ANT) {
     const int ix = i % BS, iy = i / BS, x = bx * BS + ix, y = by * BS + iy;
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(ANT) void k_adj_fade2(Grid g, const int* __restrict
     const double f = rhs_raw(g, wu, wv, x, y, facDiv) * fa2;
     const size_t d = (size_t)y * g.NX + x;
     const double direct = (1 - fa2) + fa2 * (corr > 0 ? corr * (f > 0) : -corr * (f < 0));
-    const double dcorr = Sr > 0 ? (1 - corr * (f > 0)) / P : (1 + corr * (f < 0)) / Nn;
+    const double dcorr = Sr > 0 ? (1 - kP * corr * (f > 0)) / P : (1 + kN * corr * (f < 0)) / Nn;
     rb[d] = rb[d] * direct + Cbar * dcorr * fa2;
   }
 }
@@ -324,7 +326,9 @@ __global__ __launch_bounds__(ANT) void k_adj_obst2(Grid g, Obstacles ob, int sha
   __syncthreads();
   if ((int)blockIdx.x >= nub) return;
   const double S = tot[0];
-  const double D = dmax(S > 0 ? tot[1] : tot[2], EPSD);
+  const double Draw = S > 0 ? tot[1] : tot[2];
+  const double D = dmax(Draw, EPSD);
+  const double kD = Draw < EPSD ? 0.0 : 1.0;  // a clamped denominator has no derivative
   const double corr = S / D;
   const double Cbar = corr > 0 ? tot[3] : (corr < 0 ? tot[4] : 0.0);
   const int b = ob.ub_block[blockIdx.x], bx = b % g.bpdx, by = b / g.bpdx;
@@ -347,8 +351,8 @@ __global__ __launch_bounds__(ANT) void k_adj_obst2(Grid g, Obstacles ob, int sha
       if (perim && T > 0 && corr > 0) sigma = 1;
       else if (perim && T < 0 && corr < 0) sigma = -1;
       double dcorr = 0;  // d corr / d T through the denominator
-      if (S > 0 && perim && T > 0) dcorr = -corr / D;
-      if (!(S > 0) && perim && T < 0) dcorr = corr / D;
+      if (S > 0 && perim && T > 0) dcorr = -kD * corr / D;
+      if (!(S > 0) && perim && T < 0) dcorr = kD * corr / D;
       const double Tbar = rb[d] * (1 - sigma * corr) + Cbar * dcorr;
       rb[d] = Tbar * (1 - X) + (Cbar / D) * (-X);
     }

@@ -57,14 +57,24 @@ def _fd_check(cup, make, theta, nsteps, every, ndirs, seed, eps=1e-6):
 @pytest.mark.parametrize("nsteps", [1, 2, 5])
 def test_adjoint_short_runs(cup, nsteps):
     """The same cylinder on 256x128 cells (8x4 blocks), one to five steps,
-    checkpoints every two steps, 4 directions (coefficients 10x larger)."""
+    checkpoints every two steps, 4 directions (coefficients 10x larger),
+    central differences with eps = 1e-7, to 1e-8.
+
+    J is piecewise smooth: the drag takes max(0, -F.u_hat) per surface point,
+    the fade and the obstacle RHS correction rescale by sign, advection
+    upwinds by the sign of the velocity.  A difference whose +-eps interval
+    crosses one of those switches is not the derivative: at eps = 1e-6, 9 of
+    24 directions over 3 to 5 steps differed from the adjoint by 1e-7 to
+    2.7e-6, while at eps = 1e-7 all 24 agreed to 1.3e-9 or better and the
+    forward and backward one-sided differences agreed with each other
+    (measured on MI355X).  Smaller eps is dominated by round-off of J."""
     cfg = dict(bench.ADJOINT)
     theta = bench.adjoint_theta(cfg) * 10
     J, grad, worst = _fd_check(cup, lambda: bench.adjoint_sim(cup, cfg, 8, 4), theta, nsteps,
-                               2, 4, 100 + nsteps)
+                               2, 4, 100 + nsteps, eps=1e-7)
     print(f"  {nsteps} steps: J {J:.6e}, |grad| {np.linalg.norm(grad):.3e}, worst rel {worst:.2e}",
           flush=True)
-    assert worst < 1e-6
+    assert worst < 1e-8
 
 
 def test_adjoint_workload_1024x512(cup):
