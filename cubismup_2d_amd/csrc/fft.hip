@@ -163,9 +163,12 @@ __device__ __forceinline__ void dft(V* v) {
   }
 }
 
-// Elements per thread of the FFT core (the first pass is radix E): 16 gives
-// fewer LDS round trips than 8 (measured: 8 was slower on every length).
-constexpr int FE = 16;
+// Elements per thread of the FFT core (the first pass is radix E).  With 8
+// and the kernels held to 128 VGPRs (4 waves per SIMD, launch bounds below)
+// two 8-wave workgroups share a CU; measured at 4096x2048 against 16 (237
+// VGPRs, 2 waves per SIMD): FP64 passes equal or up to 6% faster, the FP32
+// passes (fewer registers) 8% faster.
+constexpr int FE = 8;
 
 // LDS index with one pad slot every E entries (stride-E stores hit distinct banks)
 template <int E>
@@ -324,7 +327,15 @@ constexpr int rows_np() { return (L / E) >= 256 ? 1 : 256 / (L / E); }
 //
 // store_all (SRC_FADE, SRC_DIV): every cell's final RHS is written to the
 // RHS plane, not only the faded band cells (the FP64 refinement reads it).
-enum { SRC_PLAIN = 0, SRC_FADE = 1, SRC_DIV = 2 };
+//
+// SRC_DIV with obmap == nullptr is the early pass: every block's RHS is the
+// divergence (obstacle blocks included), run before the obstacle lists
+// exist.  SRC_PATCH then completes it: with st->rows_redo clear (no obstacle
+// block in the fade band) only the pairs in block rows that hold an obstacle
+// block transform delta = (final RHS - divergence) of the obstacle cells,
+// zero elsewhere, and add it to the spectrum; with st->rows_redo set every
+// pair redoes the whole SRC_DIV pass.
+enum { SRC_PLAIN = 0, SRC_FADE = 1, SRC_DIV = 2, SRC_PATCH = 3 };
 struct RowSource {
   Grid g;
   int npart;            // fade-sum workgroups (0: no fade)
@@ -337,27 +348,34 @@ struct RowSource {
   int store_all;
 };
 
+// One group of NP row pairs (group wg: pairs wg*NP .. wg*NP + NP-1) of
+// k_rows_fwd; w holds this thread's twiddle bases.
 template <int L, int NP, int E, int SRC, typename S>
-__global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
-    const S* in, int in_pitch, int ncols, int nrows, cplx<S>* __restrict__ spec,
-    int spec_rows, const cplx<S>* __restrict__ tw, RowSource fd) {
+__device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch, int ncols,
+                                               int nrows, cplx<S>* __restrict__ spec,
+                                               int spec_rows, const cplx<S>* w,
+                                               const RowSource& fd, cplx<S>* lds) {
   using V = cplx<S>;
   constexpr int NT = L / E, PADL = lds_len<L, E>();
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* lds = reinterpret_cast<V*>(smem_raw);
   const int f = threadIdx.x / NT, t = threadIdx.x % NT;
   S* rhsp = static_cast<S*>(fd.rhs);
-  // SRC_DIV reads the v rows next to its pair too: consecutive pairs run on
-  // one XCD (workgroup b runs on XCD b % 8) so those rows are L2 hits
-  int wg = blockIdx.x;
-  if (SRC == SRC_DIV && gridDim.x % 8 == 0) wg = (wg % 8) * (gridDim.x / 8) + wg / 8;
+  constexpr bool DIV = SRC == SRC_DIV || SRC == SRC_PATCH;
   const int y0 = 2 * (wg * NP + f);
   const bool valid = y0 < nrows;
   const int base = f * PADL;
-  V w[TwCount<L, E, E>::n];
-  tw_load<L, E, E>(tw, t, w);
+  // SRC_PATCH without a redo: delta rows, added to the spectrum
+  bool patch = false;
+  if constexpr (SRC == SRC_PATCH) {
+    patch = fd.st->rows_redo == 0;
+    if (patch) {  // groups whose block rows hold no obstacle block have nothing to add
+      int any = 0;
+      if (valid)
+        for (int bx = t; bx < fd.g.bpdx; bx += NT) any |= fd.obmap[(y0 / BS) * fd.g.bpdx + bx];
+      if (!__syncthreads_or(any)) return;
+    }
+  }
   V x[E];
-  if constexpr (SRC == SRC_DIV) {
+  if constexpr (DIV) {
     const Grid& g = fd.g;
     const S* fu = static_cast<const S*>(fd.u);
     const S* fv = static_cast<const S*>(fd.v);
@@ -409,8 +427,17 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
         r0 = facDiv * ((ue0 - uw0) + (vc1 - vs0));
         r1 = facDiv * ((ue1 - uw1) + (vn1 - vc0));
         const int bx = i / BS;
-        if (fd.obmap[(y0 / BS) * g.bpdx + bx]) r0 = rhsp[(size_t)y0 * NX + i];
-        if (fd.obmap[((y0 + 1) / BS) * g.bpdx + bx]) r1 = rhsp[(size_t)(y0 + 1) * NX + i];
+        if (patch) {  // a pair lies in one block row
+          if (fd.obmap[(y0 / BS) * g.bpdx + bx]) {
+            r0 = rhsp[(size_t)y0 * NX + i] - r0;
+            r1 = rhsp[(size_t)(y0 + 1) * NX + i] - r1;
+          } else {
+            r0 = r1 = 0;
+          }
+        } else if (fd.obmap) {
+          if (fd.obmap[(y0 / BS) * g.bpdx + bx]) r0 = rhsp[(size_t)y0 * NX + i];
+          if (fd.obmap[((y0 + 1) / BS) * g.bpdx + bx]) r1 = rhsp[(size_t)(y0 + 1) * NX + i];
+        }
       }
       x[q] = mk<V>(r0, r1);
     }
@@ -428,7 +455,8 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
     }
   }
   if constexpr (SRC != SRC_PLAIN) {
-    if (fd.npart > 0) {
+    // (a patch has no band cells: an obstacle block in the band forces the redo)
+    if (fd.npart > 0 && !patch) {
       const double corr = fd.st->fade_corr;  // k_fade_reduce
       const Grid& g = fd.g;
       const int by0 = y0 / BS, by1 = (y0 + 1) / BS;
@@ -457,7 +485,7 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
         }
       }
     }
-    if (fd.store_all && valid) {
+    if (fd.store_all && valid && !patch) {
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int i = t + q * NT;
@@ -494,9 +522,35 @@ __global__ __launch_bounds__(NP * L / E) void k_rows_fwd(
     }
     const int cg = k / G, c = k % G;
     V* o = spec + ((size_t)cg * spec_rows + y0) * G + c;
-    o[0] = A;
-    o[G] = B;
+    if (patch) {
+      const V A0 = o[0], B0 = o[G];
+      o[0] = mk<V>(A0.x + A.x, A0.y + A.y);
+      o[G] = mk<V>(B0.x + B.x, B0.y + B.y);
+    } else {
+      o[0] = A;
+      o[G] = B;
+    }
   }
+}
+
+template <int L, int NP, int E, int SRC, typename S>
+__global__ __launch_bounds__(NP * L / E, 4) void k_rows_fwd(
+    const S* in, int in_pitch, int ncols, int nrows, cplx<S>* __restrict__ spec,
+    int spec_rows, const cplx<S>* __restrict__ tw, RowSource fd) {
+  using V = cplx<S>;
+  constexpr int NT = L / E;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* lds = reinterpret_cast<V*>(smem_raw);
+  const int t = threadIdx.x % NT;
+  V w[TwCount<L, E, E>::n];
+  tw_load<L, E, E>(tw, t, w);
+  // SRC_DIV reads the v rows next to its pair too: consecutive pairs run on
+  // one XCD (workgroup b runs on XCD b % 8) so those rows are L2 hits.  The
+  // patch keeps the plain order: its few busy groups (the obstacle rows, a
+  // contiguous range) are then spread over all XCDs.
+  int wg = blockIdx.x;
+  if (SRC == SRC_DIV && gridDim.x % 8 == 0) wg = (wg % 8) * (gridDim.x / 8) + wg / 8;
+  rows_fwd_group<L, NP, E, SRC, S>(wg, in, in_pitch, ncols, nrows, spec, spec_rows, w, fd, lds);
 }
 
 // ---------------------------------------------------------- rows inverse
@@ -513,7 +567,7 @@ enum { OUT_SET = 0, OUT_ADD = 1, OUT_RES = 2 };
 // sum (b - mean_wg b)^2 (the residual norm of the sweep without the mean,
 // which the operator does not see; k_res_reduce merges them).
 template <int L, int NP, int E, typename S, int OUT>
-__global__ __launch_bounds__(NP * L / E) void k_rows_inv(const cplx<S>* __restrict__ spec,
+__global__ __launch_bounds__(NP * L / E, 4) void k_rows_inv(const cplx<S>* __restrict__ spec,
                                                          int spec_rows, void* __restrict__ outv,
                                                          const float* __restrict__ bres,
                                                          int out_pitch, int ncols, int nrows,
@@ -763,7 +817,7 @@ __device__ __forceinline__ void col_group_of_block(int b, int nblocks, int* cg, 
 }
 
 template <int LC, int MODE, int E, typename S>
-__global__ __launch_bounds__(CW * LC / E) void k_cols(cplx<S>* __restrict__ spec, ColParams P,
+__global__ __launch_bounds__(CW * LC / E, 4) void k_cols(cplx<S>* __restrict__ spec, ColParams P,
                                                       const cplx<S>* __restrict__ tw) {
   using V = cplx<S>;
   constexpr int NT = LC / E;
@@ -864,18 +918,21 @@ __global__ void k_ghat_extract(const double2* __restrict__ spec, int Lc, int ncg
 template <int L, typename S>
 static void launch_rows_fwd(cup2d_ctx* c, const S* in, int pitch, int ncols, int nrows,
                             cplx<S>* spec, int spec_rows, const cplx<S>* tw, int src,
-                            const RowSource& fd) {
+                            const RowSource& fd, hipStream_t st) {
   constexpr int NP = rows_np<L, FE>();
   const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(cplx<S>);
   const dim3 grid((nrows / 2 + NP - 1) / NP), block(NP * L / FE);
   if (src == SRC_DIV)
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_DIV, S>), grid, block, shm, c->stream, in,
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_DIV, S>), grid, block, shm, st, in,
+                       pitch, ncols, nrows, spec, spec_rows, tw, fd);
+  else if (src == SRC_PATCH)
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_PATCH, S>), grid, block, shm, st, in,
                        pitch, ncols, nrows, spec, spec_rows, tw, fd);
   else if (src == SRC_FADE)
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_FADE, S>), grid, block, shm, c->stream, in,
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_FADE, S>), grid, block, shm, st, in,
                        pitch, ncols, nrows, spec, spec_rows, tw, fd);
   else
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_PLAIN, S>), grid, block, shm, c->stream,
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_PLAIN, S>), grid, block, shm, st,
                        in, pitch, ncols, nrows, spec, spec_rows, tw, fd);
 }
 template <int L, typename S, int OUT>
@@ -901,17 +958,18 @@ static void launch_cols(cup2d_ctx* c, cplx<S>* spec, int ncg, const ColParams& P
 template <typename S>
 static int rows_fwd(cup2d_ctx* c, int L, const S* in, int pitch, int ncols, int nrows,
                     cplx<S>* spec, int spec_rows, const cplx<S>* tw, int src,
-                    const RowSource& fd) {
+                    const RowSource& fd, hipStream_t st = nullptr) {
+  if (!st) st = c->stream;
   switch (L) {
-    case 32: launch_rows_fwd<32, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
-    case 64: launch_rows_fwd<64, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
-    case 128: launch_rows_fwd<128, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
-    case 256: launch_rows_fwd<256, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
-    case 512: launch_rows_fwd<512, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
-    case 1024: launch_rows_fwd<1024, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
-    case 2048: launch_rows_fwd<2048, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
-    case 4096: launch_rows_fwd<4096, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
-    case 8192: launch_rows_fwd<8192, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd); break;
+    case 32: launch_rows_fwd<32, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd, st); break;
+    case 64: launch_rows_fwd<64, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd, st); break;
+    case 128: launch_rows_fwd<128, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd, st); break;
+    case 256: launch_rows_fwd<256, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd, st); break;
+    case 512: launch_rows_fwd<512, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd, st); break;
+    case 1024: launch_rows_fwd<1024, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd, st); break;
+    case 2048: launch_rows_fwd<2048, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd, st); break;
+    case 4096: launch_rows_fwd<4096, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd, st); break;
+    case 8192: launch_rows_fwd<8192, S>(c, in, pitch, ncols, nrows, spec, spec_rows, tw, src, fd, st); break;
     default: return CUP2D_ERR_ARG;
   }
   return CUP2D_OK;
@@ -960,6 +1018,8 @@ static void set_rows_attr_LS() {
   (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_FADE, S>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, shm);
   (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_DIV, S>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_PATCH, S>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, shm);
   (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, FE, S, OUT_SET>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, shm);
@@ -1171,6 +1231,38 @@ static int cols_solve(cup2d_ctx* c, cplx<S>* spec, const cplx<S>* tw) {
   return cols<1, S>(c, P->Lc, spec, P->ncg, cp, tw);
 }
 
+// The early forward row pass (see SRC_PATCH): the final fade factor from
+// the early fade sums (an obstacle block in the band changes it, and then
+// the pass is redone), then the RHS of every block formed from the advDiff
+// output and transformed, on stream s.
+int enqueue_rows_early(cup2d_ctx* c, hipStream_t s) {
+  c->rows_early = false;
+  FFTPlan* P = c->fft;
+  if (!P || c->cfg.poisson == CUP2D_POISSON_COSINE || !c->flux_deferred) return CUP2D_OK;
+  const int NX = c->g.NX, NY = c->g.NY;
+  const int npart = c->band_nblocks > 0 ? fade_partials(c) : 0;
+  if (npart > 0) {
+    if (!c->fade_early) return CUP2D_OK;
+    const int rf = enqueue_fade_reduce_on(c, s);
+    if (rf) return rf;
+  }
+  const RowSource fd{c->g, npart, c->rhs, c->st, c->u, c->v, nullptr, P->cs, c->f32 ? 1 : 0};
+  int rc;
+  {
+    KernelTimer kt(c, T_FFT_ROWS_FWD);
+    if (!c->f32)
+      rc = rows_fwd<double>(c, P->Lr, c->rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r, SRC_DIV, fd,
+                            s);
+    else
+      rc = rows_fwd<float>(c, P->Lr, tp<float>(c->rhs), NX, NX, NY,
+                           reinterpret_cast<float2*>(P->spec32), P->NYs,
+                           reinterpret_cast<const float2*>(P->tw_r32), SRC_DIV, fd, s);
+  }
+  if (rc) return rc;
+  c->rows_early = true;
+  return launch_check(c, "early row pass");
+}
+
 // src: SRC_PLAIN (rhs as given), SRC_FADE (rhs is the grid RHS: apply
 // cub2rhs's boundary fade after enqueue_fade_sums and write the faded cells
 // back), SRC_DIV (form the RHS from the advDiff output first, see k_rows_fwd).
@@ -1184,9 +1276,14 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   FFTPlan* P = c->fft;
   const int NX = c->g.NX, NY = c->g.NY;
   int rc;
+  // the forward row pass ran early on the side stream: add the obstacle
+  // blocks' correction (timed with the obstacle RHS, whose work it completes)
+  const bool patch = src == SRC_DIV && c->rows_early;
+  c->rows_early = false;
+  if (patch) src = SRC_PATCH;
   if (!c->f32) {
     {
-      KernelTimer kt(c, T_FFT_ROWS_FWD);
+      KernelTimer kt(c, patch ? T_RHS_OBST : T_FFT_ROWS_FWD);
       RowSource fd{c->g, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
                    c->u, c->v, c->obmap, P->cs, 0};
       rc = rows_fwd<double>(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r, src, fd);
@@ -1215,7 +1312,7 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   const float2* twr32 = reinterpret_cast<const float2*>(P->tw_r32);
   const float2* twc32 = reinterpret_cast<const float2*>(P->tw_c32);
   {
-    KernelTimer kt(c, T_FFT_ROWS_FWD);
+    KernelTimer kt(c, patch ? T_RHS_OBST : T_FFT_ROWS_FWD);
     RowSource fd{c->g, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
                  c->u, c->v, c->obmap, P->cs, src != SRC_PLAIN ? 1 : 0};
     rc = rows_fwd<float>(c, P->Lr, b, NX, NX, NY, s32, P->NYs, twr32, src, fd);

@@ -50,6 +50,8 @@ struct StepState {
   double tnext[2];   // time + dt of the step with that parity, set by its list builder
   double pmean;      // FP32 refinement: mean of the pressure before the sweep's update
   double res[4];     // FP32 refinement: |b - A p| / |b| at the start of each sweep
+  int rows_redo;     // an obstacle block lies in the fade band: the early row pass is redone
+  int pad_;
 };
 
 // Body state captured by the body update for ComputeForces, which may run on
@@ -256,6 +258,7 @@ struct cup2d_ctx {
   int* band_blocks = nullptr;              // blocks touched by the cub2rhs fade
   int* band_index = nullptr;               // [blocks] position in band_blocks, -1 if none
   bool fade_early = false;                 // this step's fade sums already launched
+  bool rows_early = false;  // this step's forward row FFT already ran on the side stream
   cup2d::FFTPlan* fft = nullptr;
   std::string fft_err;                     // why no FFT plan exists (size limits)
   bool shapes_frozen = false;
@@ -342,6 +345,7 @@ int enqueue_pressure_rhs(cup2d_ctx* c);
 int stencil_edge_count(const cup2d_ctx* c);
 int enqueue_fade_sums(cup2d_ctx* c, int compute = 0);
 int enqueue_fade_sums_early(cup2d_ctx* c, hipStream_t s);
+int enqueue_fade_reduce_on(cup2d_ctx* c, hipStream_t s);  // the fade factor from the sums
 int fade_partials(const cup2d_ctx* c);
 int enqueue_pressure_correction(cup2d_ctx* c, int want_max);
 int enqueue_set_dt(cup2d_ctx* c, double dt);
@@ -361,8 +365,15 @@ int fft_create(cup2d_ctx* c);
 void fft_destroy(cup2d_ctx* c);
 // src: 0 = rhs as given, 1 = grid RHS with the cub2rhs fade, 2 = RHS formed
 // from the advDiff output (deferred flux correction, obstacle blocks from the
-// RHS plane) then faded.
+// RHS plane) then faded.  With src = 2 after enqueue_rows_early, the row
+// pass only adds the transform of the obstacle blocks' correction.
 int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src);
+// The forward row FFT of the RHS formed from the advDiff output with every
+// block taken as free of obstacles, on stream s (the side stream, beside the
+// placement chain), after the early fade sums; the pressure solve then adds
+// the row transform of the obstacle RHS correction (the row transform is
+// linear), or redoes the pass when an obstacle block lies in the fade band.
+int enqueue_rows_early(cup2d_ctx* c, hipStream_t s);
 
 // obstacles.hip
 int obstacles_alloc(cup2d_ctx* c);

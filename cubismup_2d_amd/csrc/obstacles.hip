@@ -75,6 +75,27 @@ __device__ inline bool touching(const Grid& g, int bx, int by, const double bbox
 
 constexpr int LCAP = OBST_LIST_CAP;  // candidate blocks / slots handled by k_obst_begin
 
+// swimmer segments per chunk of the list builder's prefilter, and the
+// (candidate, chunk) pairs it lists per swimmer and window
+constexpr int SEG_CHUNK = 8;
+constexpr int SEG_LIST = 4096;
+
+// Does the box of a block (start, end: centres of its first and last cells)
+// grown by a chunk's safe distance, as AreaSegment::isIntersectingWithAABB
+// grows it, meet the union c[0..3] of the chunk's lab-frame boxes?  (Its
+// first test, with the union in place of one segment's box: min and max are
+// exact and the rounding of the differences monotone, so a miss here is a
+// miss of that test for every segment of the chunk.)
+__device__ inline bool box_meets_chunk(const double* c, const double start[2],
+                                       const double end[2]) {
+  const double safe = c[4];
+  const double w0 = (end[0] - start[0]) / 2 + safe, w1 = (end[1] - start[1]) / 2 + safe;
+  const double m0 = (end[0] + start[0]) / 2, m1 = (end[1] + start[1]) / 2;
+  const double iL00 = dmax(c[0], m0 - w0), iL01 = dmin(c[1], m0 + w0);
+  const double iL10 = dmax(c[2], m1 - w1), iL11 = dmin(c[3], m1 + w1);
+  return !(iL01 - iL00 < 0 || iL11 - iL10 < 0);
+}
+
 // exclusive scan of a[0..n) in LDS (n <= LCAP), returns the total
 __device__ int block_exscan(int* a, int n, int* tmp) {
   constexpr int CH = LCAP / NT;
@@ -134,6 +155,9 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
   __shared__ double s_box[MAX_SHAPES][4];
   __shared__ double lds[LDS_RED];
   __shared__ double s_seg[(FISH_MAX_NM / 8) * FISH_SEGD];
+  __shared__ double s_chunk[((FISH_MAX_NM / 8 + SEG_CHUNK - 1) / SEG_CHUNK) * 5];
+  __shared__ int s_list[SEG_LIST];
+  __shared__ int s_nlist;
   const int t = threadIdx.x;
   const int ns = g.nshapes;
   const int nub_prev = pv.cnt[1];
@@ -323,8 +347,12 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       s_b[c] = bx + g.bpdx * by;
       s_c[c] = k;
     }
-    // swimmers: any of the segments (staged in LDS) meets the block; all
-    // (candidate, segment) pairs of the window in parallel
+    // swimmers: any of the segments (staged in LDS) meets the block.  The
+    // segments are taken in chunks of SEG_CHUNK consecutive ones: a
+    // (candidate, chunk) pair whose union of lab-frame boxes misses the
+    // block's box (grown by the chunk's largest safe distance) cannot pass
+    // the first test of any of its segments, so only the pairs that meet it
+    // are listed, and their (candidate, segment) tests run in parallel.
     for (int k = 0; k < ns; ++k) {
       if (sp[k].kind != 1) continue;
       const int c0 = s_cstart[k] > w0 ? s_cstart[k] - w0 : 0;
@@ -332,16 +360,57 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       if (c1 <= c0) continue;
       const FishDev& F = fd[sp[k].fish];
       const int nseg = F.Nseg;
+      const int nch = (nseg + SEG_CHUNK - 1) / SEG_CHUNK;
       __syncthreads();
       for (int i = t; i < nseg * FISH_SEGD; i += NT) s_seg[i] = F.seg[i];
+      if (t == 0) s_nlist = 0;
+      __syncthreads();
+      for (int ch = t; ch < nch; ch += NT) {
+        double u[5] = {1e300, -1e300, 1e300, -1e300, 0.0};  // x0 x1 y0 y1 safe
+        const int i1 = (ch + 1) * SEG_CHUNK < nseg ? (ch + 1) * SEG_CHUNK : nseg;
+        for (int i = ch * SEG_CHUNK; i < i1; ++i) {
+          const double* S = s_seg + i * FISH_SEGD;
+          u[0] = dmin(u[0], S[SG_LAB]);
+          u[1] = dmax(u[1], S[SG_LAB + 1]);
+          u[2] = dmin(u[2], S[SG_LAB + 2]);
+          u[3] = dmax(u[3], S[SG_LAB + 3]);
+          u[4] = dmax(u[4], S[SG_SAFE]);
+        }
+        for (int q = 0; q < 5; ++q) s_chunk[ch * 5 + q] = u[q];
+      }
       __syncthreads();
       const int nck = c1 - c0;
-      for (int pr = t; pr < nck * nseg; pr += NT) {
-        const int c = c0 + pr / nseg, i = pr % nseg;
+      for (int pr = t; pr < nck * nch; pr += NT) {
+        const int c = c0 + pr / nch, ch = pr % nch;
         const int b = s_b[c];
         double start[2], end[2];
         block_corners(g, b % g.bpdx, b / g.bpdx, start, end);
-        if (seg_intersects(s_seg + i * FISH_SEGD, start, end)) s_a[c] = 1;
+        if (box_meets_chunk(s_chunk + ch * 5, start, end)) {
+          const int j = atomicAdd(&s_nlist, 1);
+          if (j < SEG_LIST) s_list[j] = pr;
+        }
+      }
+      __syncthreads();
+      const int nl = s_nlist;
+      if (nl <= SEG_LIST) {
+        for (int q = t; q < nl * SEG_CHUNK; q += NT) {
+          const int pr = s_list[q / SEG_CHUNK];
+          const int i = (pr % nch) * SEG_CHUNK + q % SEG_CHUNK;
+          if (i >= nseg) continue;
+          const int c = c0 + pr / nch;
+          const int b = s_b[c];
+          double start[2], end[2];
+          block_corners(g, b % g.bpdx, b / g.bpdx, start, end);
+          if (seg_intersects(s_seg + i * FISH_SEGD, start, end)) s_a[c] = 1;
+        }
+      } else {  // more meeting pairs than the list holds: every pair
+        for (int pr = t; pr < nck * nseg; pr += NT) {
+          const int c = c0 + pr / nseg, i = pr % nseg;
+          const int b = s_b[c];
+          double start[2], end[2];
+          block_corners(g, b % g.bpdx, b / g.bpdx, start, end);
+          if (seg_intersects(s_seg + i * FISH_SEGD, start, end)) s_a[c] = 1;
+        }
       }
     }
     __syncthreads();  // the window's flags are complete
@@ -840,6 +909,9 @@ int enqueue_put_objects(cup2d_ctx* c, int compute_dt, int overlap) {
     CUP2D_HIP_CHECK(c, hipStreamWaitEvent(c->aux, c->ev_begin, 0));
     RC_OB(enqueue_advdiff_on(c, 1, c->aux));
     RC_OB(enqueue_fade_sums_early(c, c->aux));
+    // the forward row FFT of the RHS without the obstacle correction, which
+    // needs no obstacle data: the pressure solve adds the correction's part
+    if (c->g.nshapes > 0) RC_OB(enqueue_rows_early(c, c->aux));
     CUP2D_HIP_CHECK(c, hipEventRecord(c->ev_obst, c->aux));  // joined before the RHS
     c->obst_pending = true;
     return CUP2D_OK;
@@ -960,13 +1032,15 @@ int join_obstacles(cup2d_ctx* c) {
 // RT: the type of the RHS and velocity planes (double, or float at FP32
 // precision); the corrections are computed in double and stored rounded.
 template <typename RT>
-__global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __restrict__ st,
+__global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, StepState* __restrict__ st,
                                                   Obstacles ob, const double* __restrict__ chi,
                                                   RT* __restrict__ r,
                                                   const RT* __restrict__ u,
                                                   const RT* __restrict__ v,
                                                   cup2d_shape_state* __restrict__ commit_ss) {
   __shared__ double lds[LDS_RED];
+  // k_rhs_obst2 sets it again when an obstacle block lies in the fade band
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->rows_redo = 0;
   if (commit_ss && blockIdx.x == gridDim.x - 1) {  // the placement commit (k_obst_commit)
     obst_commit_wg(g, commit_ss, ob);
     return;
@@ -1014,7 +1088,7 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, const StepState* __res
 // before the obstacle RHS; the partial of an obstacle block in the fade band
 // is formed again here from its final RHS.
 template <typename RT>
-__global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __restrict__ st,
+__global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, StepState* __restrict__ st,
                                                   Obstacles ob, const double* __restrict__ chi,
                                                   RT* __restrict__ r,
                                                   const int* __restrict__ band_index,
@@ -1047,6 +1121,7 @@ __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, const StepState* __res
   }
   const int j = band_index ? band_index[b] : -1;
   if (j < 0) return;
+  if (threadIdx.x == 0) st->rows_redo = 1;  // an early row pass is redone
   __syncthreads();  // the block's RHS is final
   double a[3];
   fade_block_sums<RT>(g, b, r, nullptr, nullptr, true, (RT)0, (RT)0, a, lds);

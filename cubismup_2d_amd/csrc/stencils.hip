@@ -453,6 +453,12 @@ int enqueue_fade_sums_early(cup2d_ctx* c, hipStream_t s) {
   return launch_check(c, "cub2rhs fade (early)");
 }
 
+int enqueue_fade_reduce_on(cup2d_ctx* c, hipStream_t s) {
+  if (c->band_nblocks == 0) return CUP2D_OK;
+  hipLaunchKernelGGL(k_fade_reduce, dim3(1), dim3(256), 0, s, c->red, fade_partials(c), c->st);
+  return launch_check(c, "cub2rhs fade reduce");
+}
+
 int enqueue_fade_sums(cup2d_ctx* c, int compute) {
   if (c->band_nblocks == 0) return CUP2D_OK;
   TimerScope ts(c, T_FADE);

@@ -21,7 +21,9 @@ def _announce(request):
 
 
 def _hip():
-    h = ctypes.CDLL("libamdhip64.so")
+    # by soname: the runtime already loaded with the library (torch's copy, when
+    # torch was imported first); a second HIP runtime in the process sees no device
+    h = ctypes.CDLL("libamdhip64.so.7")
     for n in ("hipMalloc", "hipFree", "hipMemcpy", "hipStreamCreate", "hipStreamDestroy",
               "hipStreamSynchronize"):
         getattr(h, n).restype = ctypes.c_int
