@@ -259,6 +259,23 @@ __device__ inline double block_sum_array(const double* a, int n, int stride, dou
   return s;
 }
 
+// dst[0..n) = src[0..n) (n <= NT * U) by NT threads, every load issued
+// before the first store: one memory round trip instead of one per element.
+template <int NT, int U>
+__device__ inline void stage_copy(double* dst, const double* __restrict__ src, int n) {
+  double v[U];
+#pragma unroll
+  for (int r = 0; r < U; ++r) {
+    const int i = (int)threadIdx.x + r * NT;
+    v[r] = i < n ? src[i] : 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < U; ++r) {
+    const int i = (int)threadIdx.x + r * NT;
+    if (i < n) dst[i] = v[r];
+  }
+}
+
 // Max of n non-negative values with U loads in flight per thread (max is
 // independent of the order; a NaN is kept).
 template <int NT, int U = 8>

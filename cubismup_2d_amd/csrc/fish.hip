@@ -258,14 +258,35 @@ __global__ __launch_bounds__(NT) void k_fish_mid_b(const FishDev* __restrict__ f
   const Mid M{lds + 2 * Nm, lds + 3 * Nm, lds + 4 * Nm, lds + 5 * Nm,
               lds + 6 * Nm, lds + 7 * Nm, lds + 8 * Nm, lds + 9 * Nm};
   double* red = lds + 10 * Nm;
-  for (int i = t; i < Nm; i += NT) {
-    rS[i] = F.rS[i];
-    width[i] = F.width[i];
+  {
+    // every load of the six arrays is issued before the first LDS store (a
+    // loop of load-store pairs waits one memory round trip per element)
+    constexpr int PER = (FISH_MAX_NM + NT - 1) / NT;
     const double* mA = F.midA + (size_t)buf * 4 * Nm;
-    M.rX[i] = mA[i];
-    M.rY[i] = mA[Nm + i];
-    M.vX[i] = mA[2 * Nm + i];
-    M.vY[i] = mA[3 * Nm + i];
+    double r[6][PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = t + k * NT;
+      const bool in = i < Nm;
+      r[0][k] = in ? F.rS[i] : 0.0;
+      r[1][k] = in ? F.width[i] : 0.0;
+      r[2][k] = in ? mA[i] : 0.0;
+      r[3][k] = in ? mA[Nm + i] : 0.0;
+      r[4][k] = in ? mA[2 * Nm + i] : 0.0;
+      r[5][k] = in ? mA[3 * Nm + i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = t + k * NT;
+      if (i < Nm) {
+        rS[i] = r[0][k];
+        width[i] = r[1][k];
+        M.rX[i] = r[2][k];
+        M.rY[i] = r[3][k];
+        M.vX[i] = r[4][k];
+        M.vY[i] = r[5][k];
+      }
+    }
   }
   const double theta = F.st[6] - st->dt * F.st[7];
   const double* sA = F.stA + buf * 8;

@@ -362,7 +362,8 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       const int nseg = F.Nseg;
       const int nch = (nseg + SEG_CHUNK - 1) / SEG_CHUNK;
       __syncthreads();
-      for (int i = t; i < nseg * FISH_SEGD; i += NT) s_seg[i] = F.seg[i];
+      stage_copy<NT, ((FISH_MAX_NM / 8) * FISH_SEGD + NT - 1) / NT>(s_seg, F.seg,
+                                                                   nseg * FISH_SEGD);
       if (t == 0) s_nlist = 0;
       __syncthreads();
       for (int ch = t; ch < nch; ch += NT) {
