@@ -163,12 +163,18 @@ __device__ __forceinline__ void dft(V* v) {
   }
 }
 
-// Elements per thread of the FFT core (the first pass is radix E).  With 8
-// and the kernels held to 128 VGPRs (4 waves per SIMD, launch bounds below)
-// two 8-wave workgroups share a CU; measured at 4096x2048 against 16 (237
-// VGPRs, 2 waves per SIMD): FP64 passes equal or up to 6% faster, the FP32
-// passes (fewer registers) 8% faster.
-constexpr int FE = 8;
+// Elements per thread of the FFT core (the first pass is radix E), by
+// transform length.  With 8 the kernels are held to 128 VGPRs (4 waves per
+// SIMD, min_waves below) and two 8-wave workgroups share a CU; with 16 they
+// take up to 256 VGPRs.  Measured on MI355X (µs per pass, 8 vs 16): rows of
+// 4096 63.8 vs 67.5, columns of 2048 41.5 vs 42.6; rows of 8192 382 vs 350,
+// inverse rows of 8192 157 vs 144, columns of 4096 205 vs 193.
+template <int L>
+constexpr int row_e() { return L >= 8192 ? 16 : 8; }
+template <int LC>
+constexpr int col_e() { return LC >= 4096 ? 16 : 8; }
+template <int E>
+constexpr int min_waves() { return E == 8 ? 4 : 1; }
 
 // LDS index with one pad slot every E entries (stride-E stores hit distinct banks)
 template <int E>
@@ -334,7 +340,11 @@ constexpr int rows_np() { return (L / E) >= 256 ? 1 : 256 / (L / E); }
 // block in the fade band) only the pairs in block rows that hold an obstacle
 // block transform delta = (final RHS - divergence) of the obstacle cells,
 // zero elsewhere, and add it to the spectrum; with st->rows_redo set every
-// pair redoes the whole SRC_DIV pass.
+// pair redoes the whole SRC_DIV pass.  st->rows_early_on (set by the list
+// builder's first phase from the previous step's obstacle rows) switches
+// the early pass off when many block rows hold obstacles, the patch then
+// costing about a whole pass: the early launch exits at once and SRC_PATCH
+// does the whole pass.
 enum { SRC_PLAIN = 0, SRC_FADE = 1, SRC_DIV = 2, SRC_PATCH = 3 };
 struct RowSource {
   Grid g;
@@ -366,7 +376,7 @@ __device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch
   // SRC_PATCH without a redo: delta rows, added to the spectrum
   bool patch = false;
   if constexpr (SRC == SRC_PATCH) {
-    patch = fd.st->rows_redo == 0;
+    patch = fd.st->rows_redo == 0 && fd.st->rows_early_on != 0;
     if (patch) {  // groups whose block rows hold no obstacle block have nothing to add
       int any = 0;
       if (valid)
@@ -534,7 +544,7 @@ __device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch
 }
 
 template <int L, int NP, int E, int SRC, typename S>
-__global__ __launch_bounds__(NP * L / E, 4) void k_rows_fwd(
+__global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_fwd(
     const S* in, int in_pitch, int ncols, int nrows, cplx<S>* __restrict__ spec,
     int spec_rows, const cplx<S>* __restrict__ tw, RowSource fd) {
   using V = cplx<S>;
@@ -542,14 +552,19 @@ __global__ __launch_bounds__(NP * L / E, 4) void k_rows_fwd(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* lds = reinterpret_cast<V*>(smem_raw);
   const int t = threadIdx.x % NT;
-  V w[TwCount<L, E, E>::n];
-  tw_load<L, E, E>(tw, t, w);
+  // the early pass runs only when the step's lists predict a small patch
+  if constexpr (SRC == SRC_DIV)
+    if (!fd.obmap && fd.st->rows_early_on == 0) return;
   // SRC_DIV reads the v rows next to its pair too: consecutive pairs run on
   // one XCD (workgroup b runs on XCD b % 8) so those rows are L2 hits.  The
   // patch keeps the plain order: its few busy groups (the obstacle rows, a
   // contiguous range) are then spread over all XCDs.
+  bool xcd_order = SRC == SRC_DIV;
+  if constexpr (SRC == SRC_PATCH) xcd_order = fd.st->rows_redo != 0 || fd.st->rows_early_on == 0;
+  V w[TwCount<L, E, E>::n];
+  tw_load<L, E, E>(tw, t, w);
   int wg = blockIdx.x;
-  if (SRC == SRC_DIV && gridDim.x % 8 == 0) wg = (wg % 8) * (gridDim.x / 8) + wg / 8;
+  if (xcd_order && gridDim.x % 8 == 0) wg = (wg % 8) * (gridDim.x / 8) + wg / 8;
   rows_fwd_group<L, NP, E, SRC, S>(wg, in, in_pitch, ncols, nrows, spec, spec_rows, w, fd, lds);
 }
 
@@ -567,7 +582,7 @@ enum { OUT_SET = 0, OUT_ADD = 1, OUT_RES = 2 };
 // sum (b - mean_wg b)^2 (the residual norm of the sweep without the mean,
 // which the operator does not see; k_res_reduce merges them).
 template <int L, int NP, int E, typename S, int OUT>
-__global__ __launch_bounds__(NP * L / E, 4) void k_rows_inv(const cplx<S>* __restrict__ spec,
+__global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_inv(const cplx<S>* __restrict__ spec,
                                                          int spec_rows, void* __restrict__ outv,
                                                          const float* __restrict__ bres,
                                                          int out_pitch, int ncols, int nrows,
@@ -817,7 +832,7 @@ __device__ __forceinline__ void col_group_of_block(int b, int nblocks, int* cg, 
 }
 
 template <int LC, int MODE, int E, typename S>
-__global__ __launch_bounds__(CW * LC / E, 4) void k_cols(cplx<S>* __restrict__ spec, ColParams P,
+__global__ __launch_bounds__(CW * LC / E, min_waves<E>()) void k_cols(cplx<S>* __restrict__ spec, ColParams P,
                                                       const cplx<S>* __restrict__ tw) {
   using V = cplx<S>;
   constexpr int NT = LC / E;
@@ -919,39 +934,42 @@ template <int L, typename S>
 static void launch_rows_fwd(cup2d_ctx* c, const S* in, int pitch, int ncols, int nrows,
                             cplx<S>* spec, int spec_rows, const cplx<S>* tw, int src,
                             const RowSource& fd, hipStream_t st) {
-  constexpr int NP = rows_np<L, FE>();
-  const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(cplx<S>);
-  const dim3 grid((nrows / 2 + NP - 1) / NP), block(NP * L / FE);
+  constexpr int RE = row_e<L>();
+  constexpr int NP = rows_np<L, RE>();
+  const size_t shm = (size_t)NP * lds_len<L, RE>() * sizeof(cplx<S>);
+  const dim3 grid((nrows / 2 + NP - 1) / NP), block(NP * L / RE);
   if (src == SRC_DIV)
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_DIV, S>), grid, block, shm, st, in,
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, RE, SRC_DIV, S>), grid, block, shm, st, in,
                        pitch, ncols, nrows, spec, spec_rows, tw, fd);
   else if (src == SRC_PATCH)
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_PATCH, S>), grid, block, shm, st, in,
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, RE, SRC_PATCH, S>), grid, block, shm, st, in,
                        pitch, ncols, nrows, spec, spec_rows, tw, fd);
   else if (src == SRC_FADE)
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_FADE, S>), grid, block, shm, st, in,
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, RE, SRC_FADE, S>), grid, block, shm, st, in,
                        pitch, ncols, nrows, spec, spec_rows, tw, fd);
   else
-    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, FE, SRC_PLAIN, S>), grid, block, shm, st,
+    CUP2D_TIMED_LAUNCH(c, (k_rows_fwd<L, NP, RE, SRC_PLAIN, S>), grid, block, shm, st,
                        in, pitch, ncols, nrows, spec, spec_rows, tw, fd);
 }
 template <int L, typename S, int OUT>
 static void launch_rows_inv(cup2d_ctx* c, const cplx<S>* spec, int spec_rows, void* out,
                             const float* bres, int pitch, int ncols, int nrows,
                             const cplx<S>* tw, const double* mean) {
-  c->rows_inv_wgs = (nrows / 2 + rows_np<L, FE>() - 1) / rows_np<L, FE>();
-  constexpr int NP = rows_np<L, FE>();
-  const size_t shm = (size_t)NP * lds_len<L, FE>() * sizeof(cplx<S>);
+  constexpr int RE = row_e<L>();
+  c->rows_inv_wgs = (nrows / 2 + rows_np<L, RE>() - 1) / rows_np<L, RE>();
+  constexpr int NP = rows_np<L, RE>();
+  const size_t shm = (size_t)NP * lds_len<L, RE>() * sizeof(cplx<S>);
   const int pairs = nrows / 2;
-  CUP2D_TIMED_LAUNCH(c, (k_rows_inv<L, NP, FE, S, OUT>), dim3((pairs + NP - 1) / NP),
-                     dim3(NP * L / FE), shm, c->stream, spec, spec_rows, out, bres, pitch,
+  CUP2D_TIMED_LAUNCH(c, (k_rows_inv<L, NP, RE, S, OUT>), dim3((pairs + NP - 1) / NP),
+                     dim3(NP * L / RE), shm, c->stream, spec, spec_rows, out, bres, pitch,
                      ncols, nrows, tw, mean, c->respart);
 }
 template <int LC, int MODE, typename S>
 static void launch_cols(cup2d_ctx* c, cplx<S>* spec, int ncg, const ColParams& P,
                         const cplx<S>* tw) {
-  const size_t shm = (size_t)lds_len<LC, FE>() * CW * sizeof(cplx<S>);
-  CUP2D_TIMED_LAUNCH(c, (k_cols<LC, MODE, FE, S>), dim3(ncg * (G / CW)), dim3(CW * LC / FE), shm,
+  constexpr int CE = col_e<LC>();
+  const size_t shm = (size_t)lds_len<LC, CE>() * CW * sizeof(cplx<S>);
+  CUP2D_TIMED_LAUNCH(c, (k_cols<LC, MODE, CE, S>), dim3(ncg * (G / CW)), dim3(CW * LC / CE), shm,
                      c->stream, spec, P, tw);
 }
 
@@ -1011,23 +1029,24 @@ static int cols(cup2d_ctx* c, int LC, cplx<S>* spec, int ncg, const ColParams& P
 
 template <int L, typename S>
 static void set_rows_attr_LS() {
-  constexpr int NP = rows_np<L, FE>();
-  const int shm = NP * lds_len<L, FE>() * (int)sizeof(cplx<S>);
-  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_PLAIN, S>,
+  constexpr int RE = row_e<L>();
+  constexpr int NP = rows_np<L, RE>();
+  const int shm = NP * lds_len<L, RE>() * (int)sizeof(cplx<S>);
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, RE, SRC_PLAIN, S>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_FADE, S>,
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, RE, SRC_FADE, S>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_DIV, S>,
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, RE, SRC_DIV, S>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, FE, SRC_PATCH, S>,
+  (void)hipFuncSetAttribute((const void*)k_rows_fwd<L, NP, RE, SRC_PATCH, S>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, FE, S, OUT_SET>,
+  (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, RE, S, OUT_SET>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, shm);
   if constexpr (sizeof(S) == 4)
-    (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, FE, S, OUT_ADD>,
+    (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, RE, S, OUT_ADD>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, shm);
   else
-    (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, FE, S, OUT_RES>,
+    (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, RE, S, OUT_RES>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, shm);
 }
 template <int L>
@@ -1037,18 +1056,19 @@ static void set_rows_attr_L() {
 }
 template <int LC>
 static void set_cols_attr() {
-  const int shm = lds_len<LC, FE>() * CW * (int)sizeof(double2);
-  const int shm32 = lds_len<LC, FE>() * CW * (int)sizeof(float2);
+  constexpr int CE = col_e<LC>();
+  const int shm = lds_len<LC, CE>() * CW * (int)sizeof(double2);
+  const int shm32 = lds_len<LC, CE>() * CW * (int)sizeof(float2);
   const hipFuncAttribute A = hipFuncAttributeMaxDynamicSharedMemorySize;
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0, FE, double>, A, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1, FE, double>, A, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 2, FE, double>, A, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 3, FE, double>, A, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 4, FE, double>, A, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 5, FE, double>, A, shm);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0, FE, float>, A, shm32);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1, FE, float>, A, shm32);
-  (void)hipFuncSetAttribute((const void*)k_cols<LC, 3, FE, float>, A, shm32);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0, CE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1, CE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 2, CE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 3, CE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 4, CE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 5, CE, double>, A, shm);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 0, CE, float>, A, shm32);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 1, CE, float>, A, shm32);
+  (void)hipFuncSetAttribute((const void*)k_cols<LC, 3, CE, float>, A, shm32);
 }
 
 static std::vector<double2> twiddles(int L) {

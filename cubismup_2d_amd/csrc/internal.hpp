@@ -51,7 +51,7 @@ struct StepState {
   double pmean;      // FP32 refinement: mean of the pressure before the sweep's update
   double res[4];     // FP32 refinement: |b - A p| / |b| at the start of each sweep
   int rows_redo;     // an obstacle block lies in the fade band: the early row pass is redone
-  int pad_;
+  int rows_early_on; // the early row pass runs this step (few block rows hold obstacles)
 };
 
 // Body state captured by the body update for ComputeForces, which may run on

@@ -282,6 +282,20 @@ __global__ __launch_bounds__(NT) void k_obst_begin(Grid g, const ShapeParams* __
       if (c0 < 0 || c0 > g.extent0 || c1 < 0 || c1 > g.extent1) st->error = CUP2D_ERR_BODY_OUT;
     }
   }
+  if (phase != 2) {
+    // the early row pass of the pressure solve runs when the previous step's
+    // obstacle blocks lie in at most a quarter of the block rows (its patch
+    // transforms the rows that hold obstacle blocks); ub_block is ascending
+    __shared__ int s_rows;
+    if (t == 0) s_rows = 0;
+    __syncthreads();
+    int nr = 0;
+    for (int u = t; u < nub_prev; u += NT)
+      nr += u == 0 || pv.ub_block[u - 1] / g.bpdx != pv.ub_block[u] / g.bpdx ? 1 : 0;
+    if (nr) atomicAdd(&s_rows, nr);
+    __syncthreads();
+    if (t == 0) st->rows_early_on = 4 * s_rows <= g.bpdy ? 1 : 0;
+  }
   if (phase != 1)
     for (int u = t; u < nub_prev; u += NT) {
       obmap[pv.ub_block[u]] = 0;
