@@ -17,6 +17,7 @@ baseline: the FP64 OpenMP restatement timed on the host on a bounded sample).
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -297,6 +298,8 @@ def traffic_from_counters(fetch_csv, write_csv, copy_bytes, config):
     # set-up solve and the whole-step one): take the one dispatched most often
     best = {}
     for k in fetch:
+        if re.search(r"k_rows_fwd<\d+, \d+, \d+, 3,", k):
+            continue  # the obstacle-row patch pass, not the forward row pass
         for pat, name in KERNEL_OF.items():
             if pat in k and k in write and len(fetch_all[k]) > best.get(name, (0, ""))[0]:
                 best[name] = (len(fetch_all[k]), k)
