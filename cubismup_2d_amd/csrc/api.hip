@@ -183,7 +183,12 @@ int pressure_single(cup2d_ctx* c, int fused) {
   if (fused && c->flux_deferred && c->cfg.poisson != CUP2D_POISSON_COSINE) {
     RC(join_obstacles(c));
     RC(enqueue_rhs_obstacles(c, 1));
-    RC(enqueue_fade_sums(c, 1));
+    // after an early row pass the fade factor is already reduced (the patch
+    // pass reduces the final one itself when it has to redo the pass)
+    if (c->rows_early)
+      c->fade_early = false;
+    else
+      RC(enqueue_fade_sums(c, 1));
     RC(enqueue_poisson_fft(c, c->rhs, c->pres, 2));
   } else {
     // the side stream may still be writing the advDiff output that the RHS

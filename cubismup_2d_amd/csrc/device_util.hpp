@@ -330,30 +330,6 @@ __device__ inline double flux_corr(const double* __restrict__ fluxpart, int ne, 
   return IF / (2 * (BS * g.bpdy) + 2 * (BS * g.bpdx));
 }
 
-// corr as above, summed by the first 256 threads in the grouping of a
-// 256-thread workgroup whatever blockDim.x (>= 256) is, so every kernel that
-// needs it gets the same value.
-__device__ inline double flux_corr256(const double* __restrict__ fluxpart, int ne, const Grid& g,
-                                      double* lds) {
-  constexpr int nt = 256;
-  double s = 0;
-  if (threadIdx.x < nt) {
-    int e = threadIdx.x;
-    for (; e < ne; e += 4 * nt) {  // four loads in flight, added in order
-      const double a0 = fluxpart[e];
-      const double a1 = e + nt < ne ? fluxpart[e + nt] : 0.0;
-      const double a2 = e + 2 * nt < ne ? fluxpart[e + 2 * nt] : 0.0;
-      const double a3 = e + 3 * nt < ne ? fluxpart[e + 3 * nt] : 0.0;
-      s += a0;
-      if (e + nt < ne) s += a1;
-      if (e + 2 * nt < ne) s += a2;
-      if (e + 3 * nt < ne) s += a3;
-    }
-  }
-  const double IF = block_sum_dyn(s, lds);
-  return IF / (2 * (BS * g.bpdy) + 2 * (BS * g.bpdx));
-}
-
 // A velocity component as the advDiff flux correction leaves it on the
 // boundary (advDiff.cpp:107-115): + corr on the first cell of its direction,
 // - corr on the last (n cells).

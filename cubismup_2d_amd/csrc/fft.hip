@@ -362,6 +362,7 @@ struct RowSource {
   const unsigned char* obmap;
   int mirror;           // cosine: columns past ncols read the row reversed
   int store_all;
+  const double* fadepart;  // SRC_PATCH redo: the fade sums' partials (npart x 3)
 };
 
 // One group of NP row pairs (group wg: pairs wg*NP .. wg*NP + NP-1) of
@@ -388,6 +389,23 @@ __device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch
       if (valid)
         for (int bx = t; bx < fd.g.bpdx; bx += NT) any |= fd.obmap[(y0 / BS) * fd.g.bpdx + bx];
       if (!__syncthreads_or(any)) return;
+    }
+  }
+  // the fade factor: k_fade_reduce's, or with a redo of the early pass the
+  // final one (the obstacle RHS kernel replaced the partials of the obstacle
+  // blocks in the band), reduced here in one fixed order by every workgroup
+  double fcorr = 0;
+  if constexpr (SRC != SRC_PLAIN) fcorr = fd.st->fade_corr;
+  if constexpr (SRC == SRC_PATCH) {
+    if (fd.st->rows_redo != 0 && fd.npart > 0) {
+      __shared__ double s_corr;
+      if (threadIdx.x < 64) {
+        double sums[3];
+        wave_sum_table<3>(fd.fadepart, fd.npart, 3, sums);
+        if (threadIdx.x == 0) s_corr = fade_corr_factor(sums[0], sums[1], sums[2]);
+      }
+      __syncthreads();
+      fcorr = s_corr;
     }
   }
   V x[E];
@@ -473,7 +491,7 @@ __device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch
   if constexpr (SRC != SRC_PLAIN) {
     // (a patch has no band cells: an obstacle block in the band forces the redo)
     if (fd.npart > 0 && !patch) {
-      const double corr = fd.st->fade_corr;  // k_fade_reduce
+      const double corr = fcorr;
       const Grid& g = fd.g;
       const int by0 = y0 / BS, by1 = (y0 + 1) / BS;
       const bool row0 = band_row(g, by0), row1 = band_row(g, by1);
@@ -1312,7 +1330,7 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
     {
       KernelTimer kt(c, patch ? T_RHS_OBST : T_FFT_ROWS_FWD);
       RowSource fd{c->g, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
-                   c->u, c->v, c->obmap, P->cs, 0};
+                   c->u, c->v, c->obmap, P->cs, 0, c->red};
       rc = rows_fwd<double>(c, P->Lr, rhs, NX, NX, NY, P->spec, P->NYs, P->tw_r, src, fd);
     }
     if (rc) return rc;
@@ -1341,7 +1359,7 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
   {
     KernelTimer kt(c, patch ? T_RHS_OBST : T_FFT_ROWS_FWD);
     RowSource fd{c->g, c->band_nblocks > 0 ? fade_partials(c) : 0, rhs, c->st,
-                 c->u, c->v, c->obmap, P->cs, src != SRC_PLAIN ? 1 : 0};
+                 c->u, c->v, c->obmap, P->cs, src != SRC_PLAIN ? 1 : 0, c->red};
     rc = rows_fwd<float>(c, P->Lr, b, NX, NX, NY, s32, P->NYs, twr32, src, fd);
   }
   if (rc) return rc;

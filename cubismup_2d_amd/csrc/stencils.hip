@@ -392,9 +392,9 @@ int enqueue_pressure_rhs(cup2d_ctx* c) {
 // ------------------------------------------------------- cub2rhs fade
 // PoissonSolver.cpp:81-98: the sums of the faded RHS r*fa^2 over the band
 // blocks, of its positive parts and of its negative parts; one workgroup per
-// band block writes three partials.  The correction factor and the rewrite of
-// the band cells happen in the forward row FFT, whose workgroups each add
-// these partials in the same order.
+// band block writes three partials.  k_fade_reduce forms the correction
+// factor from them (a redone early row pass forms it itself), and the
+// forward row FFT rewrites the band cells.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fade_sums(const T* __restrict__ r,
                                                    const int* __restrict__ blocks, int nblocks,
