@@ -145,3 +145,28 @@ def test_fp32_scaleup_100_steps(cup):
           + " ".join(f"{e:.1e}" for e in ferr), flush=True)
     # measured on MI355X: vel 9.2e-8, pres 2.2e-6, forces <= 1.4e-5
     assert ev < 1e-6 and ep < 2e-5 and max(ferr) < 2e-4
+
+
+def test_fp32_swimmer_in_fade_band(cup):
+    """A swimmer whose obstacle blocks reach into the fade band, at FP32 with
+    one refinement sweep, against the FP64 device run (which matches the
+    oracle, see the parity tests): the early float row pass is then redone
+    with the final fade factor, and the RHS plane the refinement reads must
+    hold every cell's final value."""
+    from test_gpu_parity import FISH_VARIANTS
+    fish = dict(xpos=1.0, ypos=0.08, angle=0.0, **FISH_VARIANTS["carling"])
+    kw = dict(extent=2.0, nu=1e-4, CFL=0.2, poisson="periodic", fadeLen=0.05)
+    runs = {}
+    for prec in ("fp64", "fp32"):
+        g = cup.Simulation(32, 16, shapes=[cup.carlingfish(**fish)], precision=prec, **kw)
+        g.reset()
+        rng = np.random.default_rng(63)
+        g.set_field(0, 0.02 * rng.standard_normal((2, g.NY, g.NX)))
+        g.simulate(8)
+        runs[prec] = (g.get_field(0), g.get_field(2))
+        g.close()
+    ev = rel(runs["fp32"][0], runs["fp64"][0])
+    ep = rel(runs["fp32"][1], runs["fp64"][1])
+    print(f"  FP32+refine vs FP64, swimmer in the fade band, 8 steps: rel L2 vel {ev:.2e} "
+          f"pres {ep:.2e}", flush=True)
+    assert ev < 1e-6 and ep < 2e-5
