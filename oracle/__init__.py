@@ -118,6 +118,7 @@ def lib():
         L.orc_det_sincos.argtypes = [D, dp, dp]
         L.orc_lane_sum.argtypes = [dp, C.c_int]
         L.orc_lane_sum.restype = D
+        L.orc_set_reference_arithmetic.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -189,6 +190,13 @@ def det_sincos(x):
 def lane_sum(v):
     v = np.ascontiguousarray(v, dtype=np.float64)
     return lib().orc_lane_sum(_dp(v), len(v))
+
+
+def set_reference_arithmetic(on):
+    """Process-wide: True runs the swimmer path in the reference's own
+    arithmetic (libm sine/cosine, index-order sums, sequential hat P2M);
+    False (the default) in the device library's, bit for bit."""
+    lib().orc_set_reference_arithmetic(1 if on else 0)
 
 
 class Oracle:

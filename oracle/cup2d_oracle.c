@@ -123,8 +123,26 @@ static inline double dmin(double a, double b) { return (b < a) ? b : a; } /* std
  * r^17, cosine to r^18; truncation below 1e-17).  The device library runs the
  * same operation sequence, so both sides produce identical bits; accuracy is
  * about one ulp. */
+/* Reference arithmetic (orc_set_reference_arithmetic): on the body path the
+ * sines and cosines come from libm (the reference calls std::sin/std::cos,
+ * CarlingFish.cpp:21,26, Fish.cpp:118-123), the midline's running sums add
+ * in index order (CarlingFish.cpp:82), the midline integrals add in index
+ * order (the reference's OpenMP reductions, run on one thread) and
+ * PutFishOnBlocks's hat P2M adds every contribution onto chi and udef in
+ * loop order (FishLibrary.cpp:619-631).  Off (the default) the oracle runs
+ * the device library's orders and sine/cosine, bit for bit.  The mode
+ * measures how far the device's summation orders and sine/cosine move the
+ * results from the reference's own arithmetic. */
+static int g_ref_arith = 0;
+void orc_set_reference_arithmetic(int on) { g_ref_arith = on ? 1 : 0; }
+
 void orc_det_sincos(double x, double* sn, double* cs)
 {
+  if (g_ref_arith) {
+    *sn = sin(x);
+    *cs = cos(x);
+    return;
+  }
   const double P1 = 0x1.921fb544p+0, P2 = 0x1.0b4611a6p-34, P3 = 0x1.3198a2e037073p-69;
   const double k = floor(x * 0x1.45f306dc9c883p-1 + 0.5);
   const double r = ((x - k * P1) - k * P2) - k * P3;
@@ -153,6 +171,11 @@ static inline double det_cos(double x) { double a, b; orc_det_sincos(x, &a, &b);
  * bit. */
 double orc_lane_sum(const double* v, int n)
 {
+  if (g_ref_arith) {
+    double a = 0;
+    for (int i = 0; i < n; ++i) a += v[i];
+    return a;
+  }
   double p[256];
   for (int l = 0; l < 256; ++l) {
     double a = 0;
@@ -645,7 +668,9 @@ static void fish_midline(fish_t* f, double t)
    * (as on the device) in a fixed two-level order: 64 chunks of
    * c = ceil((Nm-1)/64) increments, each summed in index order, chunk offsets
    * summed in chunk order, element = chunk-local sum + offset */
-  {
+  if (g_ref_arith) {
+    for (int i = 1; i < Nm; ++i) { rX[i] += rX[i - 1]; vX[i] += vX[i - 1]; }
+  } else {
     const int c = (Nm - 1 + 63) / 64;
     double* arr[2] = {rX, vX};
     for (int q = 0; q < 2; ++q) {
@@ -1024,10 +1049,16 @@ static void put_fish_on_block(const orc_sim* s, const fish_t* f, const fframe_t*
           for (int idx = x0; idx < x1; ++idx) {
             const int sx = idx - iap[0], sy = idy - iap[1];
             const double wxwy = wghts[1][sy] * wghts[0][sx];
-            qu[idy][idx] += llrint(wxwy * udef[0] * f->Su);
-            qv[idy][idx] += llrint(wxwy * udef[1] * f->Su);
-            qw[idy][idx] += llrint(wxwy * 0x1p52);
-            touched[idy][idx] = 1;
+            if (g_ref_arith) {
+              o->udef[idy][idx][0] += wxwy * udef[0];
+              o->udef[idy][idx][1] += wxwy * udef[1];
+              o->chi[idy][idx] += wxwy;
+            } else {
+              qu[idy][idx] += llrint(wxwy * udef[0] * f->Su);
+              qv[idy][idx] += llrint(wxwy * udef[1] * f->Su);
+              qw[idy][idx] += llrint(wxwy * 0x1p52);
+              touched[idy][idx] = 1;
+            }
             o->rho[idy][idx] = 1;
             if (fabs(o->dist[idy][idx] + 1) < EPS) o->dist[idy][idx] = 1;
           }

@@ -125,6 +125,9 @@ int orc_fft1d(double* data, int n, int sign);
 int  orc_fish_info(const orc_sim* s, int k, double* out);
 /* sin and cos as used on the swimmer path (for tests) */
 void orc_det_sincos(double x, double* sn, double* cs);
+/* 1: the reference's own arithmetic on the swimmer path (libm sine/cosine,
+ * index-order sums, sequential hat P2M); 0 (default): the device library's. */
+void orc_set_reference_arithmetic(int on);
 /* sum of n values in the fixed order used on the swimmer path (for tests) */
 double orc_lane_sum(const double* v, int n);
 
