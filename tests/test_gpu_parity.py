@@ -439,6 +439,21 @@ def test_free_swimmer_steps(cup, oracle_mod):
     assert abs(a["theta_internal"] - b["theta_internal"]) <= 1e-12
 
 
+def test_swimmer_many_bodies_device_loop(cup, oracle_mod):
+    """A swimmer and four towed disks whose obstacle blocks lie in more than
+    a quarter of the block rows: the early row pass of the pressure solve is
+    switched off on the device and the patch launch does the whole pass."""
+    fish = dict(xpos=0.9, ypos=0.5, angle=0.0, **FISH_VARIANTS["quadratic_naca"])
+    disks = [dict(radius=0.05, xpos=x, ypos=y, xvel=0.1, bFixed=True, bForced=True)
+             for x, y in ((1.5, 0.15), (1.6, 0.35), (1.5, 0.65), (1.6, 0.85))]
+    kw = dict(extent=2.0, nu=1e-4, CFL=0.2, poisson="periodic")
+    g = cup.Simulation(32, 16, shapes=[cup.carlingfish(**fish)] + [cup.disk(**d) for d in disks],
+                       **kw)
+    o = oracle_mod.Oracle(32, 16, shapes=[oracle_mod.carlingfish(**fish)]
+                          + [oracle_mod.disk(**d) for d in disks], **kw)
+    _run_and_compare(g, o, 10, nshapes=5, device_loop=True)
+
+
 def test_free_swimmer_device_loop_periodic(cup, oracle_mod):
     """The C2-style swimmer (quadratic envelope, NACA profile) in a periodic
     box next to a towed disk, advanced by the device loop."""
