@@ -567,8 +567,14 @@ __device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch
   }
 }
 
+// The obstacle-row patch is latency-bound (a few busy workgroups), so it
+// takes twice the registers rather than spill; its rare full redo then runs
+// one workgroup per CU.
+template <int E, int SRC>
+constexpr int rows_fwd_waves() { return SRC == SRC_PATCH && E == 8 ? 2 : min_waves<E>(); }
+
 template <int L, int NP, int E, int SRC, typename S>
-__global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_fwd(
+__global__ __launch_bounds__(NP * L / E, (rows_fwd_waves<E, SRC>())) void k_rows_fwd(
     const S* in, int in_pitch, int ncols, int nrows, cplx<S>* __restrict__ spec,
     int spec_rows, const cplx<S>* __restrict__ tw, RowSource fd) {
   using V = cplx<S>;
