@@ -366,11 +366,11 @@ struct RowSource {
 };
 
 // One group of NP row pairs (group wg: pairs wg*NP .. wg*NP + NP-1) of
-// k_rows_fwd; w holds this thread's twiddle bases.
+// k_rows_fwd; tw is the length-L twiddle table.
 template <int L, int NP, int E, int SRC, typename S>
 __device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch, int ncols,
                                                int nrows, cplx<S>* __restrict__ spec,
-                                               int spec_rows, const cplx<S>* w,
+                                               int spec_rows, const cplx<S>* __restrict__ tw,
                                                const RowSource& fd, cplx<S>* lds) {
   using V = cplx<S>;
   constexpr int NT = L / E, PADL = lds_len<L, E>();
@@ -530,6 +530,11 @@ __device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch
       }
     }
   }
+  // the thread's twiddle bases, loaded only now: held through the loads of
+  // the source rows they would push the kernel past its register budget
+  asm volatile("" ::: "memory");
+  V w[TwCount<L, E, E>::n];
+  tw_load<L, E, E>(tw, t, w);
   fft_regs<L, E, true>(x, lds, base, 1, t, w);
   // split z = a + i b: A_k = (Z_k + conj Z_{L-k})/2, B_k = (Z_k - conj Z_{L-k})/(2i);
   // the partners Z_{L-k} (upper half) go through LDS
@@ -567,14 +572,8 @@ __device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch
   }
 }
 
-// The obstacle-row patch is latency-bound (a few busy workgroups), so it
-// takes twice the registers rather than spill; its rare full redo then runs
-// one workgroup per CU.
-template <int E, int SRC>
-constexpr int rows_fwd_waves() { return SRC == SRC_PATCH && E == 8 ? 2 : min_waves<E>(); }
-
 template <int L, int NP, int E, int SRC, typename S>
-__global__ __launch_bounds__(NP * L / E, (rows_fwd_waves<E, SRC>())) void k_rows_fwd(
+__global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_fwd(
     const S* in, int in_pitch, int ncols, int nrows, cplx<S>* __restrict__ spec,
     int spec_rows, const cplx<S>* __restrict__ tw, RowSource fd) {
   using V = cplx<S>;
@@ -591,11 +590,9 @@ __global__ __launch_bounds__(NP * L / E, (rows_fwd_waves<E, SRC>())) void k_rows
   // contiguous range) are then spread over all XCDs.
   bool xcd_order = SRC == SRC_DIV;
   if constexpr (SRC == SRC_PATCH) xcd_order = fd.st->rows_redo != 0 || fd.st->rows_early_on == 0;
-  V w[TwCount<L, E, E>::n];
-  tw_load<L, E, E>(tw, t, w);
   int wg = blockIdx.x;
   if (xcd_order && gridDim.x % 8 == 0) wg = (wg % 8) * (gridDim.x / 8) + wg / 8;
-  rows_fwd_group<L, NP, E, SRC, S>(wg, in, in_pitch, ncols, nrows, spec, spec_rows, w, fd, lds);
+  rows_fwd_group<L, NP, E, SRC, S>(wg, in, in_pitch, ncols, nrows, spec, spec_rows, tw, fd, lds);
 }
 
 // ---------------------------------------------------------- rows inverse
@@ -931,6 +928,11 @@ __global__ __launch_bounds__(CW * LC / E, min_waves<E>()) void k_cols(cplx<S>* _
         x[q] = mk<V>(x[q].x * sy, x[q].y * sy);
       }
     }
+    // the bases again from the table (L1/L2 hits) rather than held through
+    // the symbol step, where they would push the kernel past its register
+    // budget
+    asm volatile("" ::: "memory");
+    tw_load<LC, E, E>(tw, t, w);
     fft_regs<LC, E, false>(x, lds, cl, CW, t, w);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
