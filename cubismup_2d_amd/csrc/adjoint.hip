@@ -130,54 +130,89 @@ __global__ __launch_bounds__(ANT) void k_adj_forces(Grid g, Obstacles ob, int sh
 // the penalized cells scales by 1 - chi; the surface-velocity coefficients
 // receive chi (ub, vb) . d(udef)/d(coef), per workgroup into part[wg][4M].
 // One workgroup per distinct obstacle block.
+// Modes per accumulation pass of k_adj_penalize: its 4 x ADJ_MCH sums stay
+// in registers (4 x SURF_MAX_MODES indexed by a run-time mode would live in
+// scratch memory).
+constexpr int ADJ_MCH = 8;
+constexpr int ADJ_CPT = BS * BS / ANT;  // cells per thread of a block
+static_assert(BS * BS % ANT == 0, "whole cells per thread");
+
 __global__ __launch_bounds__(ANT) void k_adj_penalize(Grid g, Obstacles ob, int shape,
                                                       const double* __restrict__ chi,
                                                       const cup2d_shape_state* __restrict__ ss,
                                                       SurfVel sv, double* ub, double* vb,
                                                       double* __restrict__ part) {
-  __shared__ double lds[(ANT / 64) * 4 * SURF_MAX_MODES];
+  __shared__ double lds[(ANT / 64) * 4 * ADJ_MCH];
   const int M = sv.M;
-  double acc[4 * SURF_MAX_MODES];
-#pragma unroll
-  for (int k = 0; k < 4 * SURF_MAX_MODES; ++k) acc[k] = 0;
+  double* out = part + (size_t)blockIdx.x * 4 * M;
   const int nub = ob.cnt[1];
+  int s = -1;  // this block's slot of the shape (a slot is one block and shape)
   if ((int)blockIdx.x < nub) {
-    const int b = ob.ub_block[blockIdx.x], bx = b % g.bpdx, by = b / g.bpdx;
     const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
-    for (int q = 0; q < cnt; ++q) {
-      const int s = ob.ub_slots[s0 + q];
-      if (ob.slot_shape[s] != shape) continue;
-      const ASlot o = aslot(ob, s);
-      const double Cx = ss[shape].centerOfMass[0], Cy = ss[shape].centerOfMass[1];
-      for (int i = threadIdx.x; i < BS * BS; i += ANT) {
-        const int ix = i % BS, iy = i / BS;
-        const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-        const double X = o.chi[i];
-        if (chi[d] > X) continue;
-        if (X <= 0) continue;
-        const double a = ub[d], c = vb[d];
-        ub[d] = (1 - X) * a;
-        vb[d] = (1 - X) * c;
-        const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
-        const double r = sqrt(p0 * p0 + p1 * p1);
-        const double zr = r > 0 ? p0 / r : 1.0, zi = r > 0 ? p1 / r : 0.0;
-        const double gn = X * (a * zr + c * zi), gt = X * (-a * zi + c * zr);
-        double wr = 1, wi = 0;
-        for (int m = 0; m < M; ++m) {
-          const double tr = wr * zr - wi * zi;
-          wi = wr * zi + wi * zr;
-          wr = tr;
-          acc[m] += gn * wr;
-          acc[M + m] += gn * wi;
-          acc[2 * M + m] += gt * wr;
-          acc[3 * M + m] += gt * wi;
-        }
+    for (int q = 0; q < cnt; ++q)
+      if (ob.slot_shape[ob.ub_slots[s0 + q]] == shape) s = ob.ub_slots[s0 + q];
+  }
+  if (s < 0) {  // no cell of the shape: zero partials
+    for (int k = threadIdx.x; k < 4 * M; k += ANT) out[k] = 0;
+    return;
+  }
+  // per cell: the weights of the normal and tangential parts and the unit
+  // direction z from the centre of mass, whose powers z^(m+1) carry mode m;
+  // the adjoint velocity is scaled by (1 - chi) on the way
+  const int b = ob.ub_block[blockIdx.x], bx = b % g.bpdx, by = b / g.bpdx;
+  const ASlot o = aslot(ob, s);
+  const double Cx = ss[shape].centerOfMass[0], Cy = ss[shape].centerOfMass[1];
+  double gn[ADJ_CPT], gt[ADJ_CPT], zr[ADJ_CPT], zi[ADJ_CPT], wr[ADJ_CPT], wi[ADJ_CPT];
+#pragma unroll
+  for (int j = 0; j < ADJ_CPT; ++j) {
+    const int i = threadIdx.x + j * ANT;
+    const int ix = i % BS, iy = i / BS;
+    const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
+    const double X = o.chi[i];
+    gn[j] = gt[j] = 0;
+    zr[j] = 1;
+    zi[j] = 0;
+    wr[j] = 1;
+    wi[j] = 0;
+    if (chi[d] > X || X <= 0) continue;
+    const double a = ub[d], c = vb[d];
+    ub[d] = (1 - X) * a;
+    vb[d] = (1 - X) * c;
+    const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
+    const double r = sqrt(p0 * p0 + p1 * p1);
+    zr[j] = r > 0 ? p0 / r : 1.0;
+    zi[j] = r > 0 ? p1 / r : 0.0;
+    gn[j] = X * (a * zr[j] + c * zi[j]);
+    gt[j] = X * (-a * zi[j] + c * zr[j]);
+  }
+  for (int m0 = 0; m0 < M; m0 += ADJ_MCH) {
+    double acc[4 * ADJ_MCH];
+#pragma unroll
+    for (int k = 0; k < 4 * ADJ_MCH; ++k) acc[k] = 0;
+#pragma unroll
+    for (int j = 0; j < ADJ_CPT; ++j) {
+#pragma unroll
+      for (int mm = 0; mm < ADJ_MCH; ++mm) {
+        if (m0 + mm >= M) continue;  // uniform; keeps acc's indices static
+        const double tr = wr[j] * zr[j] - wi[j] * zi[j];
+        wi[j] = wr[j] * zi[j] + wi[j] * zr[j];
+        wr[j] = tr;
+        acc[mm] += gn[j] * wr[j];
+        acc[ADJ_MCH + mm] += gn[j] * wi[j];
+        acc[2 * ADJ_MCH + mm] += gt[j] * wr[j];
+        acc[3 * ADJ_MCH + mm] += gt[j] * wi[j];
+      }
+    }
+    block_sum_multi<ANT, 4 * ADJ_MCH>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int mm = 0; mm < ADJ_MCH; ++mm) {
+        if (m0 + mm >= M) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q * M + m0 + mm] = acc[q * ADJ_MCH + mm];
       }
     }
   }
-  block_sum_multi<ANT, 4 * SURF_MAX_MODES>(acc, lds);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 4 * M; ++k) part[(size_t)blockIdx.x * 4 * M + k] = acc[k];
 }
 
 // the parameter gradient += sum over workgroups (in order) of the partials
