@@ -664,24 +664,25 @@ __global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_inv(const c
   fft_regs<L, E, false>(x, lds, base, 1, t, w);
   if constexpr (OUT == OUT_RES) {
     constexpr int NTB = NP * L / E;
-    double rr[2 * E], bv[2 * E];
+    // r is held for the second moments; b is read again there (the same
+    // floats, L1/L2 hits) rather than held, which would exceed the registers
+    double rr[2 * E];
     double rs = 0, bs = 0, cnt = 0;
+    const float* b0 = bres + (size_t)(valid ? y0 : 0) * out_pitch;
+    const float* b1 = b0 + out_pitch;
     if (valid) {
       float* o0 = static_cast<float*>(outv) + (size_t)y0 * out_pitch;
       float* o1 = o0 + out_pitch;
-      const float* b0 = bres + (size_t)y0 * out_pitch;
-      const float* b1 = b0 + out_pitch;
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int i = t + q * NT;
-        rr[2 * q] = rr[2 * q + 1] = bv[2 * q] = bv[2 * q + 1] = 0;
+        rr[2 * q] = rr[2 * q + 1] = 0;
         if (i < ncols) {
           const double bb0 = b0[i], bb1 = b1[i];
           const double r0 = bb0 - (double)x[q].x, r1 = bb1 - (double)x[q].y;
           o0[i] = (float)r0;
           o1[i] = (float)r1;
           rr[2 * q] = r0; rr[2 * q + 1] = r1;
-          bv[2 * q] = bb0; bv[2 * q + 1] = bb1;
           rs += r0 + r1;
           bs += bb0 + bb1;
           cnt += 2;
@@ -689,7 +690,7 @@ __global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_inv(const c
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < 2 * E; ++q) rr[q] = bv[q] = 0;
+      for (int q = 0; q < 2 * E; ++q) rr[q] = 0;
     }
     // every thread reaches the reductions (no early return in this mode)
     double* red = reinterpret_cast<double*>(smem_raw);
@@ -701,9 +702,10 @@ __global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_inv(const c
     if (valid) {
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        if (t + q * NT < ncols) {
+        const int i = t + q * NT;
+        if (i < ncols) {
           const double d0 = rr[2 * q] - mr, d1 = rr[2 * q + 1] - mr;
-          const double e0 = bv[2 * q] - mb, e1 = bv[2 * q + 1] - mb;
+          const double e0 = (double)b0[i] - mb, e1 = (double)b1[i] - mb;
           r2 += d0 * d0 + d1 * d1;
           b2 += e0 * e0 + e1 * e1;
         }
@@ -880,8 +882,10 @@ __global__ __launch_bounds__(CW * LC / E, min_waves<E>()) void k_cols(cplx<S>* _
     const int i = t + q * NT;
     if constexpr (MIRROR)  // mirrored grid: rows past NY are rows NY-1, NY-2, ...
       x[q] = col[(size_t)(i < P.rows ? i : 2 * P.rows - 1 - i) * G + c];
+    else if constexpr (MODE == 1)  // zero padding: rows <= LC/2 (checked at launch)
+      x[q] = q < E / 2 && i < P.rows ? col[(size_t)i * G + c] : mk<V>(0, 0);
     else
-      x[q] = (MODE != 1 || i < P.rows) ? col[(size_t)i * G + c] : mk<V>(0, 0);
+      x[q] = col[(size_t)i * G + c];
   }
   fft_regs<LC, E, true>(x, lds, cl, CW, t, w);
   if constexpr (MODE == 2) {
@@ -937,6 +941,7 @@ __global__ __launch_bounds__(CW * LC / E, min_waves<E>()) void k_cols(cplx<S>* _
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int i = t + q * NT;
+      if (MODE == 1 && q >= E / 2) continue;
       if ((MODE != 1 && !MIRROR) || i < P.rows) col[(size_t)i * G + c] = x[q];
     }
   }
@@ -998,12 +1003,14 @@ static void launch_rows_inv(cup2d_ctx* c, const cplx<S>* spec, int spec_rows, vo
                      ncols, nrows, tw, mean, c->respart);
 }
 template <int LC, int MODE, typename S>
-static void launch_cols(cup2d_ctx* c, cplx<S>* spec, int ncg, const ColParams& P,
-                        const cplx<S>* tw) {
+static int launch_cols(cup2d_ctx* c, cplx<S>* spec, int ncg, const ColParams& P,
+                       const cplx<S>* tw) {
   constexpr int CE = col_e<LC>();
+  if (MODE == 1 && 2 * P.rows > LC) return CUP2D_ERR_ARG;  // k_cols reads rows < LC/2 only
   const size_t shm = (size_t)lds_len<LC, CE>() * CW * sizeof(cplx<S>);
   CUP2D_TIMED_LAUNCH(c, (k_cols<LC, MODE, CE, S>), dim3(ncg * (G / CW)), dim3(CW * LC / CE), shm,
                      c->stream, spec, P, tw);
+  return CUP2D_OK;
 }
 
 template <typename S>
@@ -1047,14 +1054,14 @@ template <int MODE, typename S>
 static int cols(cup2d_ctx* c, int LC, cplx<S>* spec, int ncg, const ColParams& P,
                 const cplx<S>* tw) {
   switch (LC) {
-    case 32: launch_cols<32, MODE, S>(c, spec, ncg, P, tw); break;
-    case 64: launch_cols<64, MODE, S>(c, spec, ncg, P, tw); break;
-    case 128: launch_cols<128, MODE, S>(c, spec, ncg, P, tw); break;
-    case 256: launch_cols<256, MODE, S>(c, spec, ncg, P, tw); break;
-    case 512: launch_cols<512, MODE, S>(c, spec, ncg, P, tw); break;
-    case 1024: launch_cols<1024, MODE, S>(c, spec, ncg, P, tw); break;
-    case 2048: launch_cols<2048, MODE, S>(c, spec, ncg, P, tw); break;
-    case 4096: launch_cols<4096, MODE, S>(c, spec, ncg, P, tw); break;
+    case 32: return launch_cols<32, MODE, S>(c, spec, ncg, P, tw);
+    case 64: return launch_cols<64, MODE, S>(c, spec, ncg, P, tw);
+    case 128: return launch_cols<128, MODE, S>(c, spec, ncg, P, tw);
+    case 256: return launch_cols<256, MODE, S>(c, spec, ncg, P, tw);
+    case 512: return launch_cols<512, MODE, S>(c, spec, ncg, P, tw);
+    case 1024: return launch_cols<1024, MODE, S>(c, spec, ncg, P, tw);
+    case 2048: return launch_cols<2048, MODE, S>(c, spec, ncg, P, tw);
+    case 4096: return launch_cols<4096, MODE, S>(c, spec, ncg, P, tw);
     default: return CUP2D_ERR_ARG;
   }
   return CUP2D_OK;
