@@ -39,6 +39,43 @@ static inline int strip_width(int NX) {
     if (NX % t == 0) return t;
   return 32;
 }
+// A value of the lane to the left (lane - 1) or right (lane + 1) in the
+// wave, by a whole-wave DPP shift (a VALU move, no memory); the lane without
+// such a neighbour in the wave gets its own value back, and the callers
+// replace it.  A DPP move reads a disabled lane as the destination's old
+// value, so the shift must run with every lane of the wave active: callers
+// shift unconditionally, outside any branch or lazily evaluated operand, and
+// the empty volatile asm keeps the compiler from moving the move into one.
+__device__ __forceinline__ int dpp_move(int x, int ctrl_shl) {
+  int r = ctrl_shl ? __builtin_amdgcn_update_dpp(x, x, 0x130 /* wave_shl:1 */, 0xf, 0xf, false)
+                   : __builtin_amdgcn_update_dpp(x, x, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+  asm volatile("" : "+v"(r));
+  return r;
+}
+__device__ __forceinline__ float lane_from_left(float x) {
+  return __int_as_float(dpp_move(__float_as_int(x), 0));
+}
+__device__ __forceinline__ float lane_from_right(float x) {
+  return __int_as_float(dpp_move(__float_as_int(x), 1));
+}
+__device__ __forceinline__ double lane_from_left(double x) {
+  return __hiloint2double(dpp_move(__double2hiint(x), 0), dpp_move(__double2loint(x), 0));
+}
+__device__ __forceinline__ double lane_from_right(double x) {
+  return __hiloint2double(dpp_move(__double2hiint(x), 1), dpp_move(__double2loint(x), 1));
+}
+// The lanes of a strip workgroup's wave whose left / right neighbour column
+// lies outside the wave (the other wave, or the next strip).
+struct WaveEdges {
+  bool w, e;
+  __device__ WaveEdges() {
+    const int lane = threadIdx.x & 63;
+    const int last = (blockDim.x < 64 ? (int)blockDim.x : 64) - 1;
+    w = lane == 0;
+    e = lane == last;
+  }
+};
+
 static constexpr int RYS_ = STENCIL_RY;
 // workgroups of the stencil kernels (advDiff, RHS, projection share a grid)
 int grid_workgroups(int NX, int NY) { return (NX / strip_width(NX)) * (NY / RYS_); }
@@ -169,10 +206,14 @@ __device__ inline void advdiff_strip(const T* __restrict__ u, const T* __restric
   // All loads of the strip are issued first (clamped indices, so every load
   // is unconditional), then the rows are computed with selects of the
   // boundary values: the wave keeps the whole strip's loads in flight.
+  // The x neighbours come from the adjacent lanes (lane_from_left/right);
+  // only the wave's edge lanes load theirs (uH, vH: the west column for
+  // lane 0, the east one for the last lane).
   const size_t i0 = (size_t)y0 * NX + x;
   const bool hasW = x > 0, hasE = x + 1 < NX;
   const size_t dW = hasW ? 1 : 0, dE = hasE ? 1 : 0;
-  T uC[RY + 2], vC[RY + 2], uW[RY], vW[RY], uE[RY], vE[RY];
+  const WaveEdges we;
+  T uC[RY + 2], vC[RY + 2], uH[RY], vH[RY];
   {
     const size_t is = y0 > 0 ? i0 - NX : i0;
     uC[0] = u[is];
@@ -183,13 +224,14 @@ __device__ inline void advdiff_strip(const T* __restrict__ u, const T* __restric
       uC[iy + 1] = u[i];
       vC[iy + 1] = v[i];
     }
+    if (we.w || we.e) {
+      const size_t dh = we.w ? (size_t)0 - dW : dE;
 #pragma unroll
-    for (int iy = 0; iy < RY; ++iy) {
-      const size_t i = i0 + (size_t)iy * NX;
-      uW[iy] = u[i - dW];
-      vW[iy] = v[i - dW];
-      uE[iy] = u[i + dE];
-      vE[iy] = v[i + dE];
+      for (int iy = 0; iy < RY; ++iy) {
+        const size_t i = i0 + (size_t)iy * NX + dh;
+        uH[iy] = u[i];
+        vH[iy] = v[i];
+      }
     }
   }
   T uc = uC[1], vc = vC[1];
@@ -201,8 +243,13 @@ __device__ inline void advdiff_strip(const T* __restrict__ u, const T* __restric
     const int y = y0 + iy;
     const bool hasN = y + 1 < NY;
     const T un = hasN ? uC[iy + 2] : uc * fadeN, vn = hasN ? vC[iy + 2] : vc * fadeN;
-    const T uw = hasW ? uW[iy] : uc * fadeW, vw = hasW ? vW[iy] : vc * fadeW;
-    const T ue = hasE ? uE[iy] : uc * fadeE, ve = hasE ? vE[iy] : vc * fadeE;
+    // (the shifts run before the selects: every lane takes part in them)
+    const T sul = lane_from_left(uc), svl = lane_from_left(vc);
+    const T sur = lane_from_right(uc), svr = lane_from_right(vc);
+    const T ul = we.w ? uH[iy] : sul, vl = we.w ? vH[iy] : svl;
+    const T ur = we.e ? uH[iy] : sur, vr = we.e ? vH[iy] : svr;
+    const T uw = hasW ? ul : uc * fadeW, vw = hasW ? vl : vc * fadeW;
+    const T ue = hasE ? ur : uc * fadeE, ve = hasE ? vr : vc * fadeE;
     const T dUadv = (uc + UINF0) * (ue - uw) + (vc + UINF1) * (un - us);
     const T dUdif = ue + un + uw + us - 4 * uc;
     const T dVadv = (uc + UINF0) * (ve - vw) + (vc + UINF1) * (vn - vs);
@@ -500,10 +547,13 @@ __global__ __launch_bounds__(256) void k_project(T* __restrict__ u, T* __restric
   // All loads of the strip first (clamped indices), then the rows with
   // selects of the boundary values; the RY rows lie in one block row.
   static_assert(BS % RY == 0, "a strip lies in one block row");
+  // The x neighbours of p come from the adjacent lanes; only the wave's
+  // edge lanes load theirs (pH: west for lane 0, east for the last lane).
   const size_t i0 = (size_t)y0 * NX + x;
   const bool hasW = x > 0, hasE = x + 1 < NX;
   const size_t dW = hasW ? 1 : 0, dE = hasE ? 1 : 0;
-  double pC[RY + 2], pW[RY], pE[RY];
+  const WaveEdges we;
+  double pC[RY + 2], pH[RY];
   T uu[RY], vv[RY];
   pC[0] = p[y0 > 0 ? i0 - NX : i0];
 #pragma unroll
@@ -512,10 +562,13 @@ __global__ __launch_bounds__(256) void k_project(T* __restrict__ u, T* __restric
 #pragma unroll
   for (int iy = 0; iy < RY; ++iy) {
     const size_t i = i0 + (size_t)iy * NX;
-    pW[iy] = p[i - dW];
-    pE[iy] = p[i + dE];
     uu[iy] = u[i];
     vv[iy] = v[i];
+  }
+  if (we.w || we.e) {
+    const size_t dh = we.w ? (size_t)0 - dW : dE;
+#pragma unroll
+    for (int iy = 0; iy < RY; ++iy) pH[iy] = p[i0 + (size_t)iy * NX + dh];
   }
   const unsigned char inob = obmap[(y0 / BS) * g.bpdx + x / BS];  // always allocated
   const bool count = (maxpart != nullptr) & (inob == 0);
@@ -526,8 +579,12 @@ __global__ __launch_bounds__(256) void k_project(T* __restrict__ u, T* __restric
   for (int iy = 0; iy < RY; ++iy) {
     const int y = y0 + iy;
     const double pn = y + 1 < NY ? pC[iy + 2] : pc;
-    const double pw = hasW ? pW[iy] : pc;
-    const double pe = hasE ? pE[iy] : pc;
+    // (the shifts run before the selects: every lane takes part in them)
+    const double spl = lane_from_left(pc), spr = lane_from_right(pc);
+    const double pl = we.w ? pH[iy] : spl;
+    const double pr = we.e ? pH[iy] : spr;
+    const double pw = hasW ? pl : pc;
+    const double pe = hasE ? pr : pc;
     // selects, not branches, so the row loop stays one basic block
     T ux = uu[iy], vy = vv[iy];
     const T tcorr = (T)corr;
