@@ -39,31 +39,6 @@ static inline int strip_width(int NX) {
     if (NX % t == 0) return t;
   return 32;
 }
-// A value of the lane to the left (lane - 1) or right (lane + 1) in the
-// wave, by a whole-wave DPP shift (a VALU move, no memory); the lane without
-// such a neighbour in the wave gets its own value back, and the callers
-// replace it.  A DPP move reads a disabled lane as the destination's old
-// value, so the shift must run with every lane of the wave active: callers
-// shift unconditionally, outside any branch or lazily evaluated operand, and
-// the empty volatile asm keeps the compiler from moving the move into one.
-__device__ __forceinline__ int dpp_move(int x, int ctrl_shl) {
-  int r = ctrl_shl ? __builtin_amdgcn_update_dpp(x, x, 0x130 /* wave_shl:1 */, 0xf, 0xf, false)
-                   : __builtin_amdgcn_update_dpp(x, x, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
-  asm volatile("" : "+v"(r));
-  return r;
-}
-__device__ __forceinline__ float lane_from_left(float x) {
-  return __int_as_float(dpp_move(__float_as_int(x), 0));
-}
-__device__ __forceinline__ float lane_from_right(float x) {
-  return __int_as_float(dpp_move(__float_as_int(x), 1));
-}
-__device__ __forceinline__ double lane_from_left(double x) {
-  return __hiloint2double(dpp_move(__double2hiint(x), 0), dpp_move(__double2loint(x), 0));
-}
-__device__ __forceinline__ double lane_from_right(double x) {
-  return __hiloint2double(dpp_move(__double2hiint(x), 1), dpp_move(__double2loint(x), 1));
-}
 // The lanes of a strip workgroup's wave whose left / right neighbour column
 // lies outside the wave (the other wave, or the next strip).
 struct WaveEdges {
