@@ -577,10 +577,8 @@ __global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_fwd(
     const S* in, int in_pitch, int ncols, int nrows, cplx<S>* __restrict__ spec,
     int spec_rows, const cplx<S>* __restrict__ tw, RowSource fd) {
   using V = cplx<S>;
-  constexpr int NT = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* lds = reinterpret_cast<V*>(smem_raw);
-  const int t = threadIdx.x % NT;
   // the early pass runs only when the step's lists predict a small patch
   if constexpr (SRC == SRC_DIV)
     if (!fd.obmap && fd.st->rows_early_on == 0) return;
