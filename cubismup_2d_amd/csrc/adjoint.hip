@@ -215,14 +215,14 @@ __global__ __launch_bounds__(ANT) void k_adj_penalize(Grid g, Obstacles ob, int 
   }
 }
 
-// the parameter gradient += sum over workgroups (in order) of the partials
+// the parameter gradient += sum over workgroups of the partials: one
+// workgroup per parameter, the sum in a fixed order (block_sum_array)
 __global__ __launch_bounds__(ANT) void k_adj_theta(const double* __restrict__ part, int nwg,
                                                    int n4, double* __restrict__ theta) {
-  for (int k = threadIdx.x; k < n4; k += ANT) {
-    double s = 0;
-    for (int w = 0; w < nwg; ++w) s += part[(size_t)w * n4 + k];
-    theta[k] += s;
-  }
+  __shared__ double lds[ANT / 64];
+  const int k = blockIdx.x;
+  const double s = block_sum_array<ANT>(part + k, nwg, n4, lds);
+  if (threadIdx.x == 0) theta[k] += s;
 }
 
 // projection z = wf + pFac grad p: pb += pFac (Gx^T ub + Gy^T vb)
@@ -273,15 +273,11 @@ __global__ __launch_bounds__(ANT) void k_adj_fade2(Grid g, const int* __restrict
                                                    const double* __restrict__ wv,
                                                    const double* __restrict__ part,
                                                    double* __restrict__ rb) {
-  __shared__ double tot[5];
-  if (threadIdx.x < 5) {
-    double s = 0;
-    for (int k = 0; k < nblocks; ++k) s += part[5 * k + threadIdx.x];
-    tot[threadIdx.x] = s;
-  }
-  __syncthreads();
+  __shared__ double lds[(ANT / 64) * 5];
   const int j = blockIdx.x;
   if (j >= nblocks) return;
+  double tot[5];
+  block_sum_table<ANT, 5>(part, nblocks, 5, tot, lds);  // the same order in every workgroup
   const double Sr = tot[0], P = dmax(tot[1], EPSD), Nn = dmax(tot[2], EPSD);
   const double corr = Sr > 0 ? Sr / P : Sr / Nn;
   const double Cbar = corr > 0 ? tot[3] : tot[4];
@@ -549,8 +545,9 @@ int enqueue_adjoint_step(cup2d_ctx* c, const AdjStep& a) {
                      a.vu0, a.vu1, a.wJ, a.ub, a.vb, a.pb);
   hipLaunchKernelGGL(k_adj_penalize, dim3(G), dim3(ANT), 0, s, g, c->ob, a.shape, c->chi, c->ss,
                      surf_of(c), a.ub, a.vb, a.theta_part);
-  hipLaunchKernelGGL(k_adj_theta, dim3(1), dim3(ANT), 0, s, a.theta_part, G, 4 * c->surf_modes,
-                     a.theta);
+  if (c->surf_modes > 0)
+    hipLaunchKernelGGL(k_adj_theta, dim3(4 * c->surf_modes), dim3(ANT), 0, s, a.theta_part, G,
+                       4 * c->surf_modes, a.theta);
   hipLaunchKernelGGL(k_adj_project, dim3(2048), dim3(ANT), 0, s, g, pFac, a.ub, a.vb, a.pb);
   int rc = launch_check(c, "adjoint: forces, penalization, projection");
   if (rc) return rc;
