@@ -347,15 +347,11 @@ __global__ __launch_bounds__(ANT) void k_adj_obst2(Grid g, Obstacles ob, int sha
                                                    const double* __restrict__ wv,
                                                    const double* __restrict__ part,
                                                    double* __restrict__ rb) {
-  __shared__ double tot[5];
+  __shared__ double lds[(ANT / 64) * 5];
   const int nub = ob.cnt[1];
-  if (threadIdx.x < 5) {
-    double s = 0;
-    for (int k = 0; k < nub; ++k) s += part[5 * k + threadIdx.x];
-    tot[threadIdx.x] = s;
-  }
-  __syncthreads();
   if ((int)blockIdx.x >= nub) return;
+  double tot[5];
+  block_sum_table<ANT, 5>(part, nub, 5, tot, lds);  // the same order in every workgroup
   const double S = tot[0];
   const double Draw = S > 0 ? tot[1] : tot[2];
   const double D = dmax(Draw, EPSD);
