@@ -40,7 +40,9 @@ def test_layout_roundtrip(cup):
     assert np.array_equal(g.get_field(cup.PRES)[0], d[0])
 
 
-@pytest.mark.parametrize("bp", [(3, 2), (8, 4)])
+# strip widths 32 (half a wave), 64, 128 and 256 threads: the x neighbours
+# come from adjacent lanes, the wave's edge lanes load theirs
+@pytest.mark.parametrize("bp", [(3, 2), (10, 3), (4, 2), (8, 4)])
 def test_advdiff_bitexact(cup, oracle_mod, bp):
     rng = np.random.default_rng(11)
     kw = dict(extent=1.0, nu=3e-3)
@@ -69,9 +71,10 @@ def test_advdiff_golden(cup):
     assert np.abs(out[1] - gd["v1"]).max() < 1e-15
 
 
-def test_rhs_projection_bitexact(cup):
+@pytest.mark.parametrize("bp", [(3, 2), (10, 3), (4, 2), (8, 4)])
+def test_rhs_projection_bitexact(cup, bp):
     rng = np.random.default_rng(12)
-    g = cup.Simulation(4, 2, extent=1.0)
+    g = cup.Simulation(*bp, extent=1.0)
     h = g.scalars()["h"]
     u, v = rng.standard_normal((2, g.NY, g.NX))
     p = rng.standard_normal((g.NY, g.NX))
