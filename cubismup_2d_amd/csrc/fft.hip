@@ -365,6 +365,54 @@ struct RowSource {
   const double* fadepart;  // SRC_PATCH redo: the fade sums' partials (npart x 3)
 };
 
+// Forward transform of a packed row pair held in x (x[q] = element t + q*NT
+// of row y0 + i row y0+1) and its split into the two rows' half spectra,
+// stored (or with patch added) at spec[cg][y0, y0+1][G].  Every thread of the
+// workgroup takes part (the barriers); invalid pairs store nothing.
+template <int L, int E, typename S>
+__device__ __forceinline__ void rows_fwd_transform_store(cplx<S>* x, cplx<S>* lds, int base, int t,
+                                                         const cplx<S>* w, bool valid,
+                                                         cplx<S>* __restrict__ spec,
+                                                         int spec_rows, int y0, bool patch) {
+  using V = cplx<S>;
+  constexpr int NT = L / E;
+  fft_regs<L, E, true>(x, lds, base, 1, t, w);
+  // split z = a + i b: A_k = (Z_k + conj Z_{L-k})/2, B_k = (Z_k - conj Z_{L-k})/(2i);
+  // the partners Z_{L-k} (upper half) go through LDS
+#pragma unroll
+  for (int q = E / 2; q < E; ++q) lds[base + padE<E>(t + q * NT)] = x[q];
+  __syncthreads();
+  if (!valid) return;
+  const S half = (S)0.5;
+#pragma unroll
+  for (int q = 0; q < E / 2; ++q) {
+    const int k = t + q * NT;  // 0 .. L/2-1
+    const V Z = x[q];
+    V A, B;
+    if (k == 0) {
+      const V ZN = x[E / 2];  // element L/2
+      A = mk<V>(Z.x, ZN.x);
+      B = mk<V>(Z.y, ZN.y);
+    } else {
+      const V Zm = lds[base + padE<E>(L - k)];
+      const V sm = mk<V>(Z.x + Zm.x, Z.y - Zm.y);  // Z + conj(Zm)
+      const V d = mk<V>(Z.x - Zm.x, Z.y + Zm.y);   // Z - conj(Zm)
+      A = mk<V>(half * sm.x, half * sm.y);
+      B = mk<V>(half * d.y, -half * d.x);
+    }
+    const int cg = k / G, c = k % G;
+    V* o = spec + ((size_t)cg * spec_rows + y0) * G + c;
+    if (patch) {
+      const V A0 = o[0], B0 = o[G];
+      o[0] = mk<V>(A0.x + A.x, A0.y + A.y);
+      o[G] = mk<V>(B0.x + B.x, B0.y + B.y);
+    } else {
+      o[0] = A;
+      o[G] = B;
+    }
+  }
+}
+
 // One group of NP row pairs (group wg: pairs wg*NP .. wg*NP + NP-1) of
 // k_rows_fwd; tw is the length-L twiddle table.
 template <int L, int NP, int E, int SRC, typename S>
@@ -535,41 +583,7 @@ __device__ __forceinline__ void rows_fwd_group(int wg, const S* in, int in_pitch
   asm volatile("" ::: "memory");
   V w[TwCount<L, E, E>::n];
   tw_load<L, E, E>(tw, t, w);
-  fft_regs<L, E, true>(x, lds, base, 1, t, w);
-  // split z = a + i b: A_k = (Z_k + conj Z_{L-k})/2, B_k = (Z_k - conj Z_{L-k})/(2i);
-  // the partners Z_{L-k} (upper half) go through LDS
-#pragma unroll
-  for (int q = E / 2; q < E; ++q) lds[base + padE<E>(t + q * NT)] = x[q];
-  __syncthreads();
-  if (!valid) return;
-  const S half = (S)0.5;
-#pragma unroll
-  for (int q = 0; q < E / 2; ++q) {
-    const int k = t + q * NT;  // 0 .. L/2-1
-    const V Z = x[q];
-    V A, B;
-    if (k == 0) {
-      const V ZN = x[E / 2];  // element L/2
-      A = mk<V>(Z.x, ZN.x);
-      B = mk<V>(Z.y, ZN.y);
-    } else {
-      const V Zm = lds[base + padE<E>(L - k)];
-      const V sm = mk<V>(Z.x + Zm.x, Z.y - Zm.y);  // Z + conj(Zm)
-      const V d = mk<V>(Z.x - Zm.x, Z.y + Zm.y);   // Z - conj(Zm)
-      A = mk<V>(half * sm.x, half * sm.y);
-      B = mk<V>(half * d.y, -half * d.x);
-    }
-    const int cg = k / G, c = k % G;
-    V* o = spec + ((size_t)cg * spec_rows + y0) * G + c;
-    if (patch) {
-      const V A0 = o[0], B0 = o[G];
-      o[0] = mk<V>(A0.x + A.x, A0.y + A.y);
-      o[G] = mk<V>(B0.x + B.x, B0.y + B.y);
-    } else {
-      o[0] = A;
-      o[G] = B;
-    }
-  }
+  rows_fwd_transform_store<L, E, S>(x, lds, base, t, w, valid, spec, spec_rows, y0, patch);
 }
 
 template <int L, int NP, int E, int SRC, typename S>
@@ -599,7 +613,12 @@ __global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_fwd(
 // refinement sweep); OUT_RES: out (float) = b - x, b the float RHS plane
 // (the residual of a refinement sweep, x = the FP64 operator applied to p).
 // With S = double and OUT_SET the pass is the plain FP64 one.
-enum { OUT_SET = 0, OUT_ADD = 1, OUT_RES = 2 };
+enum { OUT_SET = 0, OUT_ADD = 1, OUT_RES = 2, OUT_RES_FWD = 3 };
+// OUT_RES_FWD (FP64 transform, float residual): OUT_RES whose residual rows,
+// rounded to float as OUT_RES stores them, are transformed forward in FP32
+// at once into spec32 (k_rows_fwd<SRC_PLAIN, float>'s arithmetic and
+// layout) instead of being stored: the FP32 solve of the residual starts at
+// the column pass.  Rows without a mirror only (L == ncols).
 // OUT_ADD also subtracts *mean (the solution's mean before the update, which
 // the spectral operator cannot see: the solve's convention is a zero mean).
 // OUT_RES also writes the workgroup's statistics of r and b to
@@ -613,9 +632,12 @@ __global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_inv(const c
                                                          int out_pitch, int ncols, int nrows,
                                                          const cplx<S>* __restrict__ tw,
                                                          const double* __restrict__ mean,
-                                                         double* __restrict__ respart) {
+                                                         double* __restrict__ respart,
+                                                         float2* __restrict__ spec32,
+                                                         const float2* __restrict__ tw32) {
   using V = cplx<S>;
   constexpr int NT = L / E, PADL = lds_len<L, E>();
+  constexpr bool RES = OUT == OUT_RES || OUT == OUT_RES_FWD;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* lds = reinterpret_cast<V*>(smem_raw);
   const int f = threadIdx.x / NT, t = threadIdx.x % NT;
@@ -660,7 +682,7 @@ __global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_inv(const c
   }
   __syncthreads();
   fft_regs<L, E, false>(x, lds, base, 1, t, w);
-  if constexpr (OUT == OUT_RES) {
+  if constexpr (RES) {
     constexpr int NTB = NP * L / E;
     // r is held for the second moments; b is read again there (the same
     // floats, L1/L2 hits) rather than held, which would exceed the registers
@@ -678,8 +700,10 @@ __global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_inv(const c
         if (i < ncols) {
           const double bb0 = b0[i], bb1 = b1[i];
           const double r0 = bb0 - (double)x[q].x, r1 = bb1 - (double)x[q].y;
-          o0[i] = (float)r0;
-          o1[i] = (float)r1;
+          if constexpr (OUT == OUT_RES) {
+            o0[i] = (float)r0;
+            o1[i] = (float)r1;
+          }
           rr[2 * q] = r0; rr[2 * q + 1] = r1;
           rs += r0 + r1;
           bs += bb0 + bb1;
@@ -714,6 +738,20 @@ __global__ __launch_bounds__(NP * L / E, min_waves<E>()) void k_rows_inv(const c
     if (threadIdx.x == 0) {
       double* o = respart + 5 * blockIdx.x;
       o[0] = n_wg; o[1] = rs; o[2] = r2; o[3] = bs; o[4] = b2;
+    }
+    if constexpr (OUT == OUT_RES_FWD) {
+      __syncthreads();  // the reductions' LDS is reused by the transform
+      float2* ldsf = reinterpret_cast<float2*>(smem_raw);
+      float2 xf[E];
+#pragma unroll
+      for (int q = 0; q < E; ++q)
+        xf[q] = t + q * NT < ncols ? make_float2((float)rr[2 * q], (float)rr[2 * q + 1])
+                                   : make_float2(0.f, 0.f);
+      asm volatile("" ::: "memory");  // the float bases are loaded only now
+      float2 wf[TwCount<L, E, E>::n];
+      tw_load<L, E, E>(tw32, t, wf);
+      rows_fwd_transform_store<L, E, float>(xf, ldsf, base, t, wf, valid, spec32, spec_rows, y0,
+                                            false);
     }
   } else {
     if (!valid) return;
@@ -990,7 +1028,8 @@ static void launch_rows_fwd(cup2d_ctx* c, const S* in, int pitch, int ncols, int
 template <int L, typename S, int OUT>
 static void launch_rows_inv(cup2d_ctx* c, const cplx<S>* spec, int spec_rows, void* out,
                             const float* bres, int pitch, int ncols, int nrows,
-                            const cplx<S>* tw, const double* mean) {
+                            const cplx<S>* tw, const double* mean, float2* spec32,
+                            const float2* tw32) {
   constexpr int RE = row_e<L>();
   c->rows_inv_wgs = (nrows / 2 + rows_np<L, RE>() - 1) / rows_np<L, RE>();
   constexpr int NP = rows_np<L, RE>();
@@ -998,7 +1037,7 @@ static void launch_rows_inv(cup2d_ctx* c, const cplx<S>* spec, int spec_rows, vo
   const int pairs = nrows / 2;
   CUP2D_TIMED_LAUNCH(c, (k_rows_inv<L, NP, RE, S, OUT>), dim3((pairs + NP - 1) / NP),
                      dim3(NP * L / RE), shm, c->stream, spec, spec_rows, out, bres, pitch,
-                     ncols, nrows, tw, mean, c->respart);
+                     ncols, nrows, tw, mean, c->respart, spec32, tw32);
 }
 template <int LC, int MODE, typename S>
 static int launch_cols(cup2d_ctx* c, cplx<S>* spec, int ncg, const ColParams& P,
@@ -1033,17 +1072,18 @@ static int rows_fwd(cup2d_ctx* c, int L, const S* in, int pitch, int ncols, int 
 template <typename S, int OUT>
 static int rows_inv(cup2d_ctx* c, int L, const cplx<S>* spec, int spec_rows, void* out,
                     const float* bres, int pitch, int ncols, int nrows, const cplx<S>* tw,
-                    const double* mean = nullptr) {
+                    const double* mean = nullptr, float2* spec32 = nullptr,
+                    const float2* tw32 = nullptr) {
   switch (L) {
-    case 32: launch_rows_inv<32, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
-    case 64: launch_rows_inv<64, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
-    case 128: launch_rows_inv<128, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
-    case 256: launch_rows_inv<256, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
-    case 512: launch_rows_inv<512, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
-    case 1024: launch_rows_inv<1024, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
-    case 2048: launch_rows_inv<2048, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
-    case 4096: launch_rows_inv<4096, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
-    case 8192: launch_rows_inv<8192, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean); break;
+    case 32: launch_rows_inv<32, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean, spec32, tw32); break;
+    case 64: launch_rows_inv<64, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean, spec32, tw32); break;
+    case 128: launch_rows_inv<128, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean, spec32, tw32); break;
+    case 256: launch_rows_inv<256, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean, spec32, tw32); break;
+    case 512: launch_rows_inv<512, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean, spec32, tw32); break;
+    case 1024: launch_rows_inv<1024, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean, spec32, tw32); break;
+    case 2048: launch_rows_inv<2048, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean, spec32, tw32); break;
+    case 4096: launch_rows_inv<4096, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean, spec32, tw32); break;
+    case 8192: launch_rows_inv<8192, S, OUT>(c, spec, spec_rows, out, bres, pitch, ncols, nrows, tw, mean, spec32, tw32); break;
     default: return CUP2D_ERR_ARG;
   }
   return CUP2D_OK;
@@ -1083,9 +1123,12 @@ static void set_rows_attr_LS() {
   if constexpr (sizeof(S) == 4)
     (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, RE, S, OUT_ADD>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, shm);
-  else
+  else {
     (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, RE, S, OUT_RES>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+    (void)hipFuncSetAttribute((const void*)k_rows_inv<L, NP, RE, S, OUT_RES_FWD>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+  }
 }
 template <int L>
 static void set_rows_attr_L() {
@@ -1402,12 +1445,20 @@ int enqueue_poisson_fft(cup2d_ctx* c, double* rhs, double* sol, int src) {
     rc = mode == 5 ? cols<5, double>(c, P->Lc, P->spec, P->ncg, ca, P->tw_c)
                    : cols<4, double>(c, P->Lc, P->spec, P->ncg, ca, P->tw_c);
     if (rc) return rc;
-    rc = rows_inv<double, OUT_RES>(c, P->Lr, P->spec, P->NYs, res, b, NX, NX, NY, P->tw_r);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_res_reduce, dim3(1), dim3(256), 0, c->stream, c->respart,
-                       c->rows_inv_wgs, &c->st->res[k]);
-    rc = rows_fwd<float>(c, P->Lr, res, NX, NX, NY, s32, P->NYs, twr32, SRC_PLAIN, plain);
-    if (rc) return rc;
+    if (!P->cs && P->Lr == NX) {  // the residual's forward rows fused into its pass
+      rc = rows_inv<double, OUT_RES_FWD>(c, P->Lr, P->spec, P->NYs, nullptr, b, NX, NX, NY,
+                                         P->tw_r, nullptr, s32, twr32);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_res_reduce, dim3(1), dim3(256), 0, c->stream, c->respart,
+                         c->rows_inv_wgs, &c->st->res[k]);
+    } else {
+      rc = rows_inv<double, OUT_RES>(c, P->Lr, P->spec, P->NYs, res, b, NX, NX, NY, P->tw_r);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_res_reduce, dim3(1), dim3(256), 0, c->stream, c->respart,
+                         c->rows_inv_wgs, &c->st->res[k]);
+      rc = rows_fwd<float>(c, P->Lr, res, NX, NX, NY, s32, P->NYs, twr32, SRC_PLAIN, plain);
+      if (rc) return rc;
+    }
     rc = cols_solve<float>(c, s32, twc32);
     if (rc) return rc;
     rc = rows_inv<float, OUT_ADD>(c, P->Lr, s32, P->NYs, sol, nullptr, NX, NX, NY, twr32,
