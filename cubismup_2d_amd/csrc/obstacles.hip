@@ -1357,11 +1357,21 @@ __device__ void momenta_wg(const Grid& g, const cup2d_shape_state* __restrict__ 
   const double lambdt = g.lambda * st->dt;
   const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
   double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  // every load of the block first (one memory round trip), then the sums
+  // over the cells inside the body
+  constexpr int NC = BS * BS / NT;
+  double cX[NC], cU[NC], cV[NC], cR[NC];
   CELL_LOOP(i) {
-    if (o.chi[i] <= 0) continue;
+    const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+    cX[i_it] = o.chi[i];
+    cU[i_it] = (double)u[d] - o.udx[i];
+    cV[i_it] = (double)v[d] - o.udy[i];
+    cR[i_it] = o.rho[i];
+  }
+  CELL_LOOP(i) {
+    if (cX[i_it] <= 0) continue;
     const int ix = i % BS, iy = i / BS;
-    const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-    double ud0 = (double)u[d] - o.udx[i], ud1 = (double)v[d] - o.udy[i];
+    double ud0 = cU[i_it], ud1 = cV[i_it];
     if (k == sv.shape) {
       double sx, sy;
       surf_vel(sv, cell_pos(bx, ix, g.hb, g.h) - Cx, cell_pos(by, iy, g.hb, g.h) - Cy, &sx, &sy);
@@ -1370,10 +1380,10 @@ __device__ void momenta_wg(const Grid& g, const cup2d_shape_state* __restrict__ 
     }
     double F;
     if (!g.implicit_penal) {
-      F = hsq * o.rho[i] * o.chi[i];
+      F = hsq * cR[i_it] * cX[i_it];
     } else {
-      const double Xlamdt = o.chi[i] * lambdt;
-      F = hsq * o.rho[i] * Xlamdt / (1 + Xlamdt);
+      const double Xlamdt = cX[i_it] * lambdt;
+      F = hsq * cR[i_it] * Xlamdt / (1 + Xlamdt);
     }
     const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
     a[0] += F;
@@ -1466,6 +1476,21 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
   const int nub = ob.cnt[1];
   if ((int)blockIdx.x >= nub && !(SOLVE && blockIdx.x == 0)) return;
   const double dt = st->dt;
+  // the block's velocities and chi are loaded first, so that their memory
+  // latency overlaps the solve's reduction of the partials
+  const bool mine = (int)blockIdx.x < nub;
+  const int b = mine ? ob.ub_block[blockIdx.x] : 0;
+  const int bx = b % g.bpdx, by = b / g.bpdx;
+  constexpr int NC = BS * BS / NT;
+  double uR[NC], vR[NC], cR[NC];
+  if (mine) {
+    CELL_LOOP(i) {
+      const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+      uR[i_it] = u[d];
+      vR[i_it] = v[d];
+      cR[i_it] = chi[d];
+    }
+  }
   if (SOLVE) {
     const double time = st->time;
     for (int k = 0; k < g.nshapes; ++k) {
@@ -1498,17 +1523,7 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
       if (nub == 0) return;
     }
   }
-  const int b = ob.ub_block[blockIdx.x];
-  const int bx = b % g.bpdx, by = b / g.bpdx;
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
-  constexpr int NC = BS * BS / NT;
-  double uR[NC], vR[NC], cR[NC];
-  CELL_LOOP(i) {
-    const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
-    uR[i_it] = u[d];
-    vR[i_it] = v[d];
-    cR[i_it] = chi[d];
-  }
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
     SlotView o = slot(ob, s);
@@ -1516,15 +1531,22 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
     const double v_s = SOLVE ? s_uvw[k][1] : ss[k].v;
     const double omega_s = SOLVE ? s_uvw[k][2] : ss[k].omega;
     const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
+    // the slot's loads of all cells first (one memory round trip)
+    double sX[NC], sU[NC], sV[NC];
+    CELL_LOOP(i) {
+      sX[i_it] = o.chi[i];
+      sU[i_it] = o.udx[i];
+      sV[i_it] = o.udy[i];
+    }
     CELL_LOOP(i) {
       const int ix = i % BS, iy = i / BS;
-      const double X = o.chi[i];
+      const double X = sX[i_it];
       if (cR[i_it] > X) continue;
       if (X <= 0) continue;
       const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
       const double alpha = g.implicit_penal ? 1 / (1 + g.lambda * dt * X) : 1 - X;
-      double US = u_s - omega_s * p1 + o.udx[i];
-      double VS = v_s + omega_s * p0 + o.udy[i];
+      double US = u_s - omega_s * p1 + sU[i_it];
+      double VS = v_s + omega_s * p0 + sV[i_it];
       if (k == sv.shape) {
         double sx, sy;
         surf_vel(sv, p0, p1, &sx, &sy);
