@@ -779,22 +779,32 @@ __device__ void obst_moments_wg(const Grid& g, const cup2d_shape_state* __restri
   const int bx = b % g.bpdx, by = b / g.bpdx;
   SlotView o = slot(ob, s);
   const double hsq = g.h * g.h;
+  // the slot's cells are loaded before the centre of mass is reduced, so
+  // the two memory latencies overlap
+  constexpr int NC = BS * BS / NT;
+  double cX[NC], cR[NC], cU[NC], cV[NC];
+  CELL_LOOP(i) {
+    cX[i_it] = o.chi[i];
+    cR[i_it] = o.rho[i];
+    cU[i_it] = o.udx[i];
+    cV[i_it] = o.udy[i];
+  }
   double C[2], mass;
   bool shifted;
   shape_com(ob, ss, k, C, &mass, &shifted);
   double a[7] = {0, 0, 0, 0, 0, 0, 0};
   CELL_LOOP(i) {
-    if (o.chi[i] <= 0) continue;
+    if (cX[i_it] <= 0) continue;
     const int ix = i % BS, iy = i / BS;
-    const double rhochi = o.chi[i] * o.rho[i] * hsq;
+    const double rhochi = cX[i_it] * cR[i_it] * hsq;
     const double p0 = cell_pos(bx, ix, g.hb, g.h) - C[0], p1 = cell_pos(by, iy, g.hb, g.h) - C[1];
     a[0] += rhochi * p0;
     a[1] += rhochi * p1;
     a[2] += rhochi;
     a[3] += rhochi * (p0 * p0 + p1 * p1);
-    a[4] += rhochi * o.udx[i];
-    a[5] += rhochi * o.udy[i];
-    a[6] += rhochi * (p0 * o.udy[i] - p1 * o.udx[i]);
+    a[4] += rhochi * cU[i_it];
+    a[5] += rhochi * cV[i_it];
+    a[6] += rhochi * (p0 * cV[i_it] - p1 * cU[i_it]);
   }
   block_sum_multi<NT, 7>(a, lds);
   if (threadIdx.x == 0)
@@ -816,6 +826,20 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __r
     const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
     for (int q = 0; q < cnt; ++q) {
       const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
+      SlotView o = slot(ob, s);
+      // the block's cells are loaded before the shape's sums are reduced,
+      // so the two memory latencies overlap
+      constexpr int NC = BS * BS / NT;
+      double cU[NC], cV[NC], cO[NC], cX[NC], dU[NC], dV[NC];
+      CELL_LOOP(i) {
+        const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+        cU[i_it] = o.udx[i];
+        cV[i_it] = o.udy[i];
+        cO[i_it] = o.chi[i];
+        cX[i_it] = chi[d];
+        dU[i_it] = udx[d];
+        dV[i_it] = udy[d];
+      }
       static_assert(P_COM == 0 && P_MOM == 3, "COM and moment partials are adjacent");
       double t[10];
       shape_sums<10>(ob, k, P_COM, t);
@@ -824,16 +848,16 @@ __global__ __launch_bounds__(NT) void k_obst_udef(Grid g, cup2d_shape_state* __r
       double C[2], mass;
       bool shifted;
       com_from_sums(ss, k, t, C, &mass, &shifted);
-      SlotView o = slot(ob, s);
       CELL_LOOP(i) {
         const int ix = i % BS, iy = i / BS;
         const double p0 = cell_pos(bx, ix, g.hb, g.h) - C[0], p1 = cell_pos(by, iy, g.hb, g.h) - C[1];
-        o.udx[i] -= Iu - Ia * p1;
-        o.udy[i] -= Iv + Ia * p0;
+        const double nu = cU[i_it] - (Iu - Ia * p1), nv = cV[i_it] - (Iv + Ia * p0);
+        o.udx[i] = nu;
+        o.udy[i] = nv;
         const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-        if (o.chi[i] < chi[d] || o.chi[i] <= 0) continue;
-        udx[d] += o.udx[i];
-        udy[d] += o.udy[i];
+        if (cO[i_it] < cX[i_it] || cO[i_it] <= 0) continue;
+        udx[d] = dU[i_it] + nu;
+        udy[d] = dV[i_it] + nv;
       }
     }
   }
@@ -1037,6 +1061,35 @@ int join_obstacles(cup2d_ctx* c) {
   return CUP2D_OK;
 }
 
+// A block's cells for the RHS obstacle correction, every load issued at once
+// (one memory round trip instead of one per cell the branch takes): the
+// dense chi and its four neighbours (BlockLabOpen ghosts), the slot's chi and
+// the RHS.
+struct CellStage {
+  static constexpr int NC = BS * BS / NT;
+  double X[NC], oX[NC], xe[NC], xw[NC], xn[NC], xs[NC], r[NC];
+  template <typename RT>
+  __device__ void load(const Grid& g, int bx, int by, const double* __restrict__ chi,
+                       const double* __restrict__ ochi, const RT* r_) {
+    CELL_LOOP(i) {
+      const int gx = bx * BS + i % BS, gy = by * BS + i / BS;
+      const size_t d = (size_t)gy * g.NX + gx;
+      X[i_it] = chi[d];
+      oX[i_it] = ochi[i];
+      xe[i_it] = lab_at(chi, g, gx + 1, gy);
+      xw[i_it] = lab_at(chi, g, gx - 1, gy);
+      xn[i_it] = lab_at(chi, g, gx, gy + 1);
+      xs[i_it] = lab_at(chi, g, gx, gy - 1);
+      r[i_it] = (double)r_[d];
+    }
+  }
+  // the cell lies on the perimeter (a nonzero chi gradient)
+  __device__ bool perim(int c) const {
+    const double gradXx = xe[c] - xw[c], gradXy = xn[c] - xs[c];
+    return gradXx * gradXx + gradXy * gradXy > 0;
+  }
+};
+
 // ------------------------------------------------ RHS obstacle correction
 // PressureSingle.cpp:130-168 (source term + perimeter sums) and :173-212
 // (sign-selective rescale), per distinct block, shapes in order.
@@ -1076,18 +1129,18 @@ __global__ __launch_bounds__(NT) void k_rhs_obst1(Grid g, StepState* __restrict_
     const int s = ob.ub_slots[s0 + q];
     SlotView o = slot(ob, s);
     double sum = 0, pos = 0, neg = 0;
+    CellStage cs;
+    cs.load(g, bx, by, chi, o.chi, r);
     CELL_LOOP(i) {
-      const int ix = i % BS, iy = i / BS, gx = bx * BS + ix, gy = by * BS + iy;
-      const size_t d = (size_t)gy * g.NX + gx;
-      if (chi[d] > o.chi[i]) continue;
-      const double gradXx = lab_at(chi, g, gx + 1, gy) - lab_at(chi, g, gx - 1, gy);
-      const double gradXy = lab_at(chi, g, gx, gy + 1) - lab_at(chi, g, gx, gy - 1);
-      const double rd = r[d];
-      const double srcBulk = -o.chi[i] * rd;
+      const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+      if (cs.X[i_it] > cs.oX[i_it]) continue;
+      const double rd = cs.r[i_it];
+      const double srcBulk = -cs.oX[i_it] * rd;
       sum += srcBulk;
-      r[d] = (RT)(rd + srcBulk);
-      const double isPerim = (gradXx * gradXx + gradXy * gradXy > 0) ? 1.0 : 0.0;
-      const double T = r[d];
+      const RT rn = (RT)(rd + srcBulk);
+      r[d] = rn;
+      const double isPerim = cs.perim(i_it) ? 1.0 : 0.0;
+      const double T = (double)rn;
       pos += isPerim * T * (double)(T > 0);
       neg -= isPerim * T * (double)(T < 0);
     }
@@ -1116,20 +1169,21 @@ __global__ __launch_bounds__(NT) void k_rhs_obst2(Grid g, StepState* __restrict_
   const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
   for (int q = 0; q < cnt; ++q) {
     const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
+    SlotView o = slot(ob, s);
+    // the block's cells are loaded before the shape's sums are reduced, so
+    // the two memory latencies overlap
+    CellStage cs;
+    cs.load(g, bx, by, chi, o.chi, r);
     double t[3];
     shape_sums<3>(ob, k, P_RHS, t);
     const double sum = t[0], pos = t[1], neg = t[2];
     const double corrDenom = sum > 0 ? pos : neg;
     const double corr = sum / dmax(corrDenom, EPSD);
-    SlotView o = slot(ob, s);
     CELL_LOOP(i) {
-      const int ix = i % BS, iy = i / BS, gx = bx * BS + ix, gy = by * BS + iy;
-      const size_t d = (size_t)gy * g.NX + gx;
-      if (chi[d] > o.chi[i]) continue;
-      const double gradXx = lab_at(chi, g, gx + 1, gy) - lab_at(chi, g, gx - 1, gy);
-      const double gradXy = lab_at(chi, g, gx, gy + 1) - lab_at(chi, g, gx, gy - 1);
-      const bool isPerim = gradXx * gradXx + gradXy * gradXy > 0;
-      const double T = r[d];
+      const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+      if (cs.X[i_it] > cs.oX[i_it]) continue;
+      const bool isPerim = cs.perim(i_it);
+      const double T = cs.r[i_it];
       if (isPerim && T > 0 && corr > 0) r[d] = (RT)(T - corr * T);
       else if (isPerim && T < 0 && corr < 0) r[d] = (RT)(T + corr * T);
     }
