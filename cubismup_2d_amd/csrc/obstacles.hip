@@ -1537,12 +1537,20 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
   const int bx = b % g.bpdx, by = b / g.bpdx;
   constexpr int NC = BS * BS / NT;
   double uR[NC], vR[NC], cR[NC];
+  // and the block's first slot (the only one unless shapes overlap)
+  const int s0 = mine ? ob.ub_start[blockIdx.x] : 0, cnt = mine ? ob.ub_count[blockIdx.x] : 0;
+  const int sf = mine ? ob.ub_slots[s0] : 0;
+  double fX[NC], fU[NC], fV[NC];
   if (mine) {
+    const SlotView o = slot(ob, sf);
     CELL_LOOP(i) {
       const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
       uR[i_it] = u[d];
       vR[i_it] = v[d];
       cR[i_it] = chi[d];
+      fX[i_it] = o.chi[i];
+      fU[i_it] = o.udx[i];
+      fV[i_it] = o.udy[i];
     }
   }
   if (SOLVE) {
@@ -1577,20 +1585,20 @@ __global__ __launch_bounds__(NT) void k_penalize(Grid g, const ShapeParams* __re
       if (nub == 0) return;
     }
   }
-  const int s0 = ob.ub_start[blockIdx.x], cnt = ob.ub_count[blockIdx.x];
   for (int q = 0; q < cnt; ++q) {
-    const int s = ob.ub_slots[s0 + q], k = ob.slot_shape[s];
+    const int s = q == 0 ? sf : ob.ub_slots[s0 + q], k = ob.slot_shape[s];
     SlotView o = slot(ob, s);
     const double u_s = SOLVE ? s_uvw[k][0] : ss[k].u;
     const double v_s = SOLVE ? s_uvw[k][1] : ss[k].v;
     const double omega_s = SOLVE ? s_uvw[k][2] : ss[k].omega;
     const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
-    // the slot's loads of all cells first (one memory round trip)
+    // the slot's loads of all cells first (one memory round trip; the first
+    // slot's were issued before the solve)
     double sX[NC], sU[NC], sV[NC];
     CELL_LOOP(i) {
-      sX[i_it] = o.chi[i];
-      sU[i_it] = o.udx[i];
-      sV[i_it] = o.udy[i];
+      sX[i_it] = q == 0 ? fX[i_it] : o.chi[i];
+      sU[i_it] = q == 0 ? fU[i_it] : o.udx[i];
+      sV[i_it] = q == 0 ? fV[i_it] : o.udy[i];
     }
     CELL_LOOP(i) {
       const int ix = i % BS, iy = i / BS;
@@ -1682,12 +1690,20 @@ __global__ __launch_bounds__(NT) void k_iter_momenta(Grid g, const cup2d_shape_s
   const double hsq = g.h * g.h;
   const double Cx = ss[k].centerOfMass[0], Cy = ss[k].centerOfMass[1];
   double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  // every load of the block first (one memory round trip), then the sums
+  constexpr int NC = BS * BS / NT;
+  double cX[NC], cU[NC], cV[NC];
   CELL_LOOP(i) {
-    if (o.chi[i] <= 0) continue;
+    const size_t d = (size_t)(by * BS + i / BS) * g.NX + bx * BS + i % BS;
+    cX[i_it] = o.chi[i];
+    cU[i_it] = fu[d];
+    cV[i_it] = fv[d];
+  }
+  CELL_LOOP(i) {
+    if (cX[i_it] <= 0) continue;
     const int ix = i % BS, iy = i / BS;
-    const size_t d = (size_t)(by * BS + iy) * g.NX + bx * BS + ix;
-    const double F = hsq * o.chi[i];
-    const double ud0 = fu[d], ud1 = fv[d];
+    const double F = hsq * cX[i_it];
+    const double ud0 = cU[i_it], ud1 = cV[i_it];
     const double p0 = cell_pos(bx, ix, g.hb, g.h) - Cx, p1 = cell_pos(by, iy, g.hb, g.h) - Cy;
     a[0] += F;
     a[1] += F * (p0 * p0 + p1 * p1);
